@@ -1,0 +1,251 @@
+/*
+ * iskra.h -- C ABI of libiskra_hip.so, the MI355X (gfx950) electrostatic
+ * particle-in-cell step: gather + push + boundary + CIC deposit, counting sort
+ * by cell, finite-difference Poisson solve, E = -grad(phi), null-collision MCC.
+ *
+ * Every entry point returns an int status (ISKRA_OK == 0); the message of the
+ * last failure on the calling thread is iskra_last_error().  No C++ exception
+ * crosses this boundary.  Host arrays passed in are only read/written during the
+ * call.  Arrays use the reference's layouts: particle positions x are np x D
+ * column-major FP64 (ParticleInCell/src/pic/kinetic.jl:2), velocities np x 3
+ * column-major FP64 (kinetic.jl:3), node fields nx x ny column-major FP64
+ * (FiniteDifferenceMethod/src/generalized_poisson.jl:40), node indices 0-based.
+ *
+ * A context owns one HIP stream and the device state of one simulation; it is
+ * not thread-safe (one host thread per context).
+ *
+ * Reference interfaces each entry point replaces are cited per declaration.
+ */
+#ifndef ISKRA_H
+#define ISKRA_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum iskra_status {
+  ISKRA_OK = 0,
+  ISKRA_E_ARG = 1,         /* invalid argument */
+  ISKRA_E_HIP = 2,         /* HIP runtime failure */
+  ISKRA_E_CAPACITY = 3,    /* particle capacity exceeded (sample!, ionisation) */
+  ISKRA_E_PROB = 4,        /* MCC probability assertion (Chemistry/src/mcc.jl:244-246,273-279) */
+  ISKRA_E_STATE = 5,       /* call out of order (no grid / solver / species) */
+  ISKRA_E_CONVERGE = 6,    /* Poisson solve did not reach its tolerance */
+  ISKRA_E_UNSUPPORTED = 7  /* configuration this build does not implement */
+};
+
+typedef struct iskra_ctx iskra_ctx;
+
+/* Message of the last failing call on this thread ("" if none). */
+const char *iskra_last_error(void);
+/* Library ABI version (major*100 + minor). */
+int iskra_version(void);
+
+/* ---- context ------------------------------------------------------------ */
+/* No reference counterpart: the reference runs on the host only
+ * (ParticleInCell/src/ParticleInCell.jl:84-139 owns all arrays). */
+int iskra_ctx_create(int device, iskra_ctx **out);
+int iskra_ctx_destroy(iskra_ctx *ctx);
+/* Wait for all work queued on the context's stream. */
+int iskra_sync(iskra_ctx *ctx);
+
+/* ---- grid --------------------------------------------------------------- */
+/* RegularGrids.create_uniform_grid (RegularGrids/src/RegularGrids.jl:135-162).
+ * dim 1 or 2; nodes[d] node counts (cells = nodes-1); dh[d] spacing;
+ * origin[d]; vol_periodic[4] = {left,right,bottom,top} edges declared
+ * :periodic, which disables the edge-volume halving of cell_volume
+ * (RegularGrids.jl:106-118).  Resets species, solver and fields. */
+int iskra_grid_set(iskra_ctx *ctx, int dim, const int64_t nodes[2],
+                   const double dh[2], const double origin[2],
+                   const int32_t vol_periodic[4]);
+
+/* Work-decomposition parameters (the autotuner's knobs).  tile: cells per
+ * tile edge (power of two); chunk: particles per work-group; margin: window
+ * halo in cells; sort_every: steps between counting sorts inside
+ * iskra_step.  Zero keeps the current value. */
+typedef struct {
+  int32_t tile;
+  int32_t chunk;
+  int32_t margin;
+  int32_t sort_every;
+} iskra_tuning;
+int iskra_tuning_set(iskra_ctx *ctx, const iskra_tuning *t);
+int iskra_tuning_get(iskra_ctx *ctx, iskra_tuning *t);
+
+/* ---- species ------------------------------------------------------------ */
+/* KineticSpecies{D,3} + create_kinetic_species
+ * (ParticleInCell/src/pic/kinetic.jl:1-18, problem/configuration.jl:95-102).
+ * q, m in SI; w0 = weight of every macro-particle (the reference's wg, which
+ * only ever holds w0); capacity = maximum particle count; track_ids != 0 keeps
+ * a uint32 id per particle (needed by MCC, whose random draws are keyed by
+ * id). */
+int iskra_species_add(iskra_ctx *ctx, double q, double m, double w0,
+                      int64_t capacity, int track_ids, int *sid);
+/* Boundary applied after the push (the after_push hook:
+ * ParticleInCell/src/ParticleInCell.jl:41, wrap!/discard!
+ * ParticleInCell/src/pic/surfaces/wrap.jl:1-33), one mode per dimension. */
+enum { ISKRA_BND_NONE = 0, ISKRA_BND_WRAP = 1, ISKRA_BND_DISCARD = 2 };
+int iskra_species_set_boundary(iskra_ctx *ctx, int sid, const int32_t mode[2]);
+/* Replace the species' particles by np host particles (x: np x D col-major,
+ * v: np x 3 col-major, id: np or NULL -> ids 0..np-1).  Positions must lie in
+ * the grid.  Leaves the species cell-sorted. */
+int iskra_species_upload(iskra_ctx *ctx, int sid, int64_t np, const double *x,
+                         const double *v, const uint32_t *id);
+/* Copy the species out in the current device order (x: cap x D col-major with
+ * leading dimension cap, v: cap x 3, id may be NULL). */
+int iskra_species_download(iskra_ctx *ctx, int sid, int64_t cap, int64_t *np,
+                           double *x, double *v, uint32_t *id);
+int iskra_species_count(iskra_ctx *ctx, int sid, int64_t *np);
+/* Raw device state in the library's own format (checkpoint / resume; no
+ * reference counterpart: the reference has no checkpoint, TODO:3):
+ * cell = i | (j << 16) with (i, j) the 0-based lower-left node, -1 for a
+ * discarded particle; ox, oy in [0,1) the in-cell offsets (oy ignored in
+ * 1-D); vx, vy, vz in m/s; id may be NULL.  get copies the current device
+ * order (discarded particles included) and sets *np (all arrays NULL: size
+ * query only); set replaces the
+ * particles and leaves them cell-sorted (stable). */
+int iskra_species_get_state(iskra_ctx *ctx, int sid, int64_t cap, int64_t *np,
+                            int32_t *cell, float *ox, float *oy, float *vx,
+                            float *vy, float *vz, uint32_t *id);
+int iskra_species_set_state(iskra_ctx *ctx, int sid, int64_t np,
+                            const int32_t *cell, const float *ox,
+                            const float *oy, const float *vx, const float *vy,
+                            const float *vz, const uint32_t *id);
+/* sample!(::MaxwellianSource) (ParticleInCell/src/pic/sources.jl:24-34):
+ * append n = min(free capacity, n) particles with x = U*wx + dx and
+ * v = N(0,1)*wv + dv per component, drawn on the device from Philox4x32-10
+ * keyed by (seed, stream).  wx, dx: D values; wv, dv: 3 values. */
+int iskra_species_sample_maxwellian(iskra_ctx *ctx, int sid, int64_t n,
+                                    const double *wx, const double *dx,
+                                    const double *wv, const double *dv,
+                                    uint64_t seed, uint32_t stream);
+
+/* ---- step operators ----------------------------------------------------- */
+/* Counting sort by cell (no reference counterpart; closest is the DSMC cell
+ * binning Chemistry/src/dsmc.jl:25-30).  exact_cell = 0 orders by cell tile
+ * (stable), 1 by exact cell; discarded particles are compacted away. */
+int iskra_sort(iskra_ctx *ctx, int sid, int exact_cell);
+/* grid_to_particle (ParticleInCell/src/pic/cloud_in_cell.jl:20-36) of the
+ * current E at the current positions; E_out: cap x 3 col-major. */
+int iskra_gather(iskra_ctx *ctx, int sid, int64_t cap, double *E_out);
+/* One advance! of a kinetic species (ParticleInCell.jl:51-61): gather E,
+ * Boris push with B = 0 (pushers.jl:37-50), after_push boundary (wrap.jl),
+ * fused with this species' CIC deposit at the new positions
+ * (cloud_in_cell.jl:1-18).  dt in seconds. */
+int iskra_push(iskra_ctx *ctx, int sid, double dt);
+/* CIC deposit of the current positions without pushing (density(),
+ * kinetic.jl:53). */
+int iskra_deposit_species(iskra_ctx *ctx, int sid);
+/* Merge all species' deposits: n_s = deposit_s / cell_volume and
+ * rho = sum_s q_s n_s + background (ParticleInCell.jl:118-124; fluid
+ * species enter as q*n without volume division, pic/fluid.jl:11). */
+int iskra_deposit(iskra_ctx *ctx);
+
+/* ---- fields ------------------------------------------------------------- */
+enum {
+  ISKRA_F_RHO = 0,      /* charge density rho (C/m^D), FP64 */
+  ISKRA_F_PHI = 1,      /* potential phi, FP64 */
+  ISKRA_F_EX = 2,       /* E_x at nodes (stored FP32) */
+  ISKRA_F_EY = 3,       /* E_y at nodes (stored FP32) */
+  ISKRA_F_BG = 4,       /* fixed background charge density (fluid species) */
+  ISKRA_F_DENSITY = 5,  /* number density of species `index` (download only) */
+  ISKRA_F_TARGET = 6    /* MCC target gas density of interaction `index` */
+};
+int iskra_field_download(iskra_ctx *ctx, int which, int index, double *out);
+int iskra_field_upload(iskra_ctx *ctx, int which, int index, const double *in);
+
+/* ---- Poisson ------------------------------------------------------------ */
+/* create_poisson_solver + apply_periodic + apply_dirichlet
+ * (FiniteDifferenceMethod/src/generalized_poisson.jl:22-68,205-215,271-324).
+ * tags: one int8 per node, 0 = free, t > 0 = Dirichlet node whose value is
+ * tag t's value (each apply_dirichlet mask is one tag); periodic_x/y link the
+ * first and last node of that axis as the reference's ring.
+ * kind: DIRECT (1-D tridiagonal), PCG (Jacobi-preconditioned CG), MG
+ * (geometric multigrid V-cycles, 2-D with Dirichlet walls), MGPCG (CG
+ * preconditioned by one V-cycle).  eps0 in F/m; rtol on ||b-A phi||2/||b||2
+ * evaluated in FP64. */
+enum {
+  ISKRA_POISSON_DIRECT = 0,
+  ISKRA_POISSON_PCG = 1,
+  ISKRA_POISSON_MG = 2,
+  ISKRA_POISSON_MGPCG = 3
+};
+int iskra_poisson_set(iskra_ctx *ctx, int kind, double eps0, const int8_t *tags,
+                      int periodic_x, int periodic_y, double rtol,
+                      int max_iter);
+/* apply_dirichlet(ps, mask, phi0) value update of an existing tag. */
+int iskra_poisson_set_tag_value(iskra_ctx *ctx, int tag, double value);
+/* value(t) = offset + amplitude*sin(2 pi freq t + phase), set inside
+ * iskra_step after each step's solve with t = (it-1)*dt, exactly like the
+ * after_loop hook of problem/11_rf_discharge.jl:95 (one-step lag). */
+int iskra_poisson_set_tag_waveform(iskra_ctx *ctx, int tag, double offset,
+                                   double amplitude, double freq, double phase);
+/* calculate_electric_potential(ps, -rho) (generalized_poisson.jl:372-378).
+ * warm: ZERO = start from phi = 0, PREV = start from the current phi field
+ * (previous solve or an uploaded warm start). */
+enum { ISKRA_WARM_ZERO = 0, ISKRA_WARM_PREV = 1 };
+int iskra_poisson_solve(iskra_ctx *ctx, int warm, int *iters, double *relres);
+/* calculate_electric_field (generalized_poisson.jl:385-410). */
+int iskra_efield(iskra_ctx *ctx);
+
+/* ---- Monte Carlo collisions -------------------------------------------- */
+/* mcc(@reactions ...) -> MonteCarloCollisions
+ * (Chemistry/src/mcc.jl:18-51,291-320).  n_coll processes in reaction order;
+ * types[k] one of ISKRA_COLL_*, energy[k] threshold in eV (excitation /
+ * ionisation); cross-section tables (CrossSection, cross_section.jl:3-17):
+ * process k owns nodes [tab_off[k], tab_off[k+1]) of tab_eps (eV) and
+ * tab_sigma (m^2).  Products of ionisation: electron -> the source species,
+ * ion -> species ion_sid (round(w0_src/w0_ion) ions each, mcc.jl:205-211).
+ * Target: fluid species with mass, charge, temperature; its density per node
+ * is the ISKRA_F_TARGET field of this interaction (default 0). */
+enum {
+  ISKRA_COLL_ELASTIC_ISOTROPIC = 0,
+  ISKRA_COLL_ELASTIC_BACKWARD = 1,
+  ISKRA_COLL_EXCITATION = 2,
+  ISKRA_COLL_IONIZATION = 3
+};
+int iskra_mcc_add(iskra_ctx *ctx, int src_sid, int n_coll, const int32_t *types,
+                  const double *energy, const int32_t *tab_off,
+                  const double *tab_eps, const double *tab_sigma, int ion_sid,
+                  double target_m, double target_q, double target_T,
+                  int *mcc_id);
+/* PIC.perform!(::MonteCarloCollisions, E, dt, config) (mcc.jl:231-289) with a
+ * per-particle null-collision rule; events[k] (nullable) receives this
+ * call's accepted events per process. */
+int iskra_mcc_run(iskra_ctx *ctx, int mcc_id, double dt, int64_t *events);
+/* Per-node event counts nu[node][k] accumulated since the last reset
+ * (mcc.jl:283-288), out: nodes x n_coll col-major. */
+int iskra_mcc_counts(iskra_ctx *ctx, int mcc_id, int reset, int64_t *out);
+
+/* ---- random numbers ------------------------------------------------------ */
+/* The seed of all device draws (Random.seed!, src/iskra.jl:22-26). */
+int iskra_rng_set(iskra_ctx *ctx, uint64_t seed);
+/* Export the device's Philox4x32-10 draws: for each id, blocks
+ * [0, nblocks) of counter (id, step, stream, block) under key = seed, 4 uint32
+ * words per block -> out[n][nblocks][4].  The CPU oracle consumes these. */
+int iskra_rng_export(iskra_ctx *ctx, uint32_t stream, uint32_t step,
+                     const uint32_t *ids, int64_t n, int nblocks, uint32_t *out);
+
+/* ---- the time loop ------------------------------------------------------- */
+/* nsteps iterations of ParticleInCell.solve's loop body
+ * (ParticleInCell.jl:102-135): MCC interactions, advance! of every species,
+ * deposit, Poisson, E; tag waveforms are applied after each solve. */
+int iskra_step(iskra_ctx *ctx, double dt, int64_t nsteps);
+
+typedef struct {
+  int64_t steps;            /* steps taken by iskra_step */
+  int64_t particle_steps;   /* sum over steps and species of live particles pushed */
+  int64_t poisson_solves;
+  int64_t poisson_iters;    /* summed over solves */
+  int32_t last_iters;
+  int32_t pad;
+  double last_relres;
+} iskra_stats;
+int iskra_stats_get(iskra_ctx *ctx, iskra_stats *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ISKRA_H */

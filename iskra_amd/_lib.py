@@ -1,0 +1,348 @@
+"""ctypes binding of libiskra_hip.so (the C ABI declared in include/iskra.h).
+
+The library is built in-tree by `python -m iskra_amd.build`; importing this
+module never falls back to anything else: if the shared object is missing or
+no GPU is visible, calls raise IskraError.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "libiskra_hip.so")
+
+OK, E_ARG, E_HIP, E_CAPACITY, E_PROB, E_STATE, E_CONVERGE, E_UNSUPPORTED = range(8)
+BND_NONE, BND_WRAP, BND_DISCARD = 0, 1, 2
+F_RHO, F_PHI, F_EX, F_EY, F_BG, F_DENSITY, F_TARGET = range(7)
+POISSON_DIRECT, POISSON_PCG, POISSON_MG, POISSON_MGPCG = range(4)
+WARM_ZERO, WARM_PREV = 0, 1
+COLL_ELASTIC_ISOTROPIC, COLL_ELASTIC_BACKWARD, COLL_EXCITATION, COLL_IONIZATION = range(4)
+
+EXPORTS = [
+    "iskra_last_error", "iskra_version", "iskra_ctx_create", "iskra_ctx_destroy", "iskra_sync",
+    "iskra_grid_set", "iskra_tuning_set", "iskra_tuning_get", "iskra_species_add",
+    "iskra_species_set_boundary", "iskra_species_upload", "iskra_species_download",
+    "iskra_species_count", "iskra_species_get_state", "iskra_species_set_state",
+    "iskra_species_sample_maxwellian", "iskra_sort", "iskra_gather",
+    "iskra_push", "iskra_deposit_species", "iskra_deposit", "iskra_field_download",
+    "iskra_field_upload", "iskra_poisson_set", "iskra_poisson_set_tag_value",
+    "iskra_poisson_set_tag_waveform", "iskra_poisson_solve", "iskra_efield", "iskra_mcc_add",
+    "iskra_mcc_run", "iskra_mcc_counts", "iskra_rng_set", "iskra_rng_export", "iskra_step",
+    "iskra_stats_get",
+]
+
+
+class IskraError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"iskra error {code}: {msg}")
+        self.code = code
+
+
+class Tuning(C.Structure):
+    _fields_ = [("tile", C.c_int32), ("chunk", C.c_int32), ("margin", C.c_int32),
+                ("sort_every", C.c_int32)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("steps", C.c_int64), ("particle_steps", C.c_int64),
+                ("poisson_solves", C.c_int64), ("poisson_iters", C.c_int64),
+                ("last_iters", C.c_int32), ("pad", C.c_int32), ("last_relres", C.c_double)]
+
+
+_lib = None
+P = C.c_void_p
+
+
+def lib():
+    """Load libiskra_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBPATH):
+            raise IskraError(E_STATE, f"{LIBPATH} is missing: run `python -m iskra_amd.build`")
+        L = C.CDLL(LIBPATH)
+        I, I64, D, U32, U64 = C.c_int, C.c_int64, C.c_double, C.c_uint32, C.c_uint64
+        sig = {
+            "iskra_last_error": (C.c_char_p, []),
+            "iskra_version": (I, []),
+            "iskra_ctx_create": (I, [I, C.POINTER(P)]),
+            "iskra_ctx_destroy": (I, [P]),
+            "iskra_sync": (I, [P]),
+            "iskra_grid_set": (I, [P, I, P, P, P, P]),
+            "iskra_tuning_set": (I, [P, C.POINTER(Tuning)]),
+            "iskra_tuning_get": (I, [P, C.POINTER(Tuning)]),
+            "iskra_species_add": (I, [P, D, D, D, I64, I, C.POINTER(I)]),
+            "iskra_species_set_boundary": (I, [P, I, P]),
+            "iskra_species_upload": (I, [P, I, I64, P, P, P]),
+            "iskra_species_download": (I, [P, I, I64, C.POINTER(I64), P, P, P]),
+            "iskra_species_count": (I, [P, I, C.POINTER(I64)]),
+            "iskra_species_get_state": (I, [P, I, I64, C.POINTER(I64), P, P, P, P, P, P, P]),
+            "iskra_species_set_state": (I, [P, I, I64, P, P, P, P, P, P, P]),
+            "iskra_species_sample_maxwellian": (I, [P, I, I64, P, P, P, P, U64, U32]),
+            "iskra_sort": (I, [P, I, I]),
+            "iskra_gather": (I, [P, I, I64, P]),
+            "iskra_push": (I, [P, I, D]),
+            "iskra_deposit_species": (I, [P, I]),
+            "iskra_deposit": (I, [P]),
+            "iskra_field_download": (I, [P, I, I, P]),
+            "iskra_field_upload": (I, [P, I, I, P]),
+            "iskra_poisson_set": (I, [P, I, D, P, I, I, D, I]),
+            "iskra_poisson_set_tag_value": (I, [P, I, D]),
+            "iskra_poisson_set_tag_waveform": (I, [P, I, D, D, D, D]),
+            "iskra_poisson_solve": (I, [P, I, C.POINTER(I), C.POINTER(D)]),
+            "iskra_efield": (I, [P]),
+            "iskra_mcc_add": (I, [P, I, I, P, P, P, P, P, I, D, D, D, C.POINTER(I)]),
+            "iskra_mcc_run": (I, [P, I, D, P]),
+            "iskra_mcc_counts": (I, [P, I, I, P]),
+            "iskra_rng_set": (I, [P, U64]),
+            "iskra_rng_export": (I, [P, U32, U32, P, I64, I, P]),
+            "iskra_step": (I, [P, D, I64]),
+            "iskra_stats_get": (I, [P, C.POINTER(Stats)]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != OK:
+        msg = lib().iskra_last_error().decode(errors="replace")
+        raise IskraError(rc, msg)
+
+
+def _p(a):
+    if a is None:
+        return None
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(P)
+
+
+class Context:
+    """One device context: grid, species, fields, Poisson solver, MCC."""
+
+    def __init__(self, device=0):
+        h = P()
+        check(lib().iskra_ctx_create(device, C.byref(h)))
+        self.h = h
+        self.dim = None
+        self.nodes = None
+
+    def close(self):
+        if self.h:
+            lib().iskra_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # grid
+    def grid(self, dim, nodes, dh, origin=(0.0, 0.0), vol_periodic=(0, 0, 0, 0)):
+        n = np.asarray(list(nodes) + [1] * (2 - len(nodes)), dtype=np.int64)
+        d = np.asarray(list(dh) + [1.0] * (2 - len(dh)), dtype=np.float64)
+        o = np.asarray(list(origin) + [0.0] * (2 - len(origin)), dtype=np.float64)
+        vp = np.asarray(vol_periodic, dtype=np.int32)
+        check(lib().iskra_grid_set(self.h, dim, _p(n), _p(d), _p(o), _p(vp)))
+        self.dim = dim
+        self.nodes = (int(n[0]), int(n[1]) if dim == 2 else 1)
+
+    @property
+    def nn(self):
+        return self.nodes[0] * self.nodes[1]
+
+    def tuning(self, tile=0, chunk=0, margin=0, sort_every=-1):
+        t = Tuning(tile, chunk, margin, sort_every)
+        check(lib().iskra_tuning_set(self.h, C.byref(t)))
+
+    def get_tuning(self):
+        t = Tuning()
+        check(lib().iskra_tuning_get(self.h, C.byref(t)))
+        return dict(tile=t.tile, chunk=t.chunk, margin=t.margin, sort_every=t.sort_every)
+
+    # species
+    def species(self, q, m, w0, capacity, track_ids=True):
+        s = C.c_int()
+        check(lib().iskra_species_add(self.h, q, m, w0, int(capacity), int(track_ids),
+                                      C.byref(s)))
+        return s.value
+
+    def boundary(self, sid, modes):
+        md = np.asarray(list(modes) + [BND_WRAP] * (2 - len(modes)), dtype=np.int32)
+        check(lib().iskra_species_set_boundary(self.h, sid, _p(md)))
+
+    def upload(self, sid, x, v, ids=None):
+        """x: (np, D) FP64, v: (np, 3) FP64 (converted to column-major)."""
+        x = np.asarray(x, dtype=np.float64)
+        v = np.asarray(v, dtype=np.float64)
+        n = x.shape[0]
+        xc = np.ascontiguousarray(x.T).ravel()
+        vc = np.ascontiguousarray(v.T).ravel()
+        idc = None if ids is None else np.ascontiguousarray(ids, dtype=np.uint32)
+        check(lib().iskra_species_upload(self.h, sid, n, _p(xc), _p(vc), _p(idc)))
+
+    def download(self, sid, cap=None):
+        if cap is None:
+            cap = self.count_raw(sid)
+        cap = max(int(cap), 1)
+        x = np.zeros(cap * self.dim)
+        v = np.zeros(cap * 3)
+        ids = np.zeros(cap, dtype=np.uint32)
+        n = C.c_int64()
+        check(lib().iskra_species_download(self.h, sid, cap, C.byref(n), _p(x), _p(v), _p(ids)))
+        n = n.value
+        X = x.reshape(self.dim, cap)[:, :n].T.copy()
+        V = v.reshape(3, cap)[:, :n].T.copy()
+        return X, V, ids[:n].copy()
+
+    def get_state(self, sid):
+        """Raw device state (current order, discarded particles included)."""
+        n = C.c_int64()
+        check(lib().iskra_species_get_state(self.h, sid, 0, C.byref(n), None, None, None, None,
+                                            None, None, None))
+        n = n.value
+        cap = max(n, 1)
+        st = dict(cell=np.zeros(cap, np.int32), ox=np.zeros(cap, np.float32),
+                  oy=np.zeros(cap, np.float32), vx=np.zeros(cap, np.float32),
+                  vy=np.zeros(cap, np.float32), vz=np.zeros(cap, np.float32),
+                  id=np.zeros(cap, np.uint32))
+        m = C.c_int64()
+        check(lib().iskra_species_get_state(self.h, sid, cap, C.byref(m), _p(st["cell"]),
+                                            _p(st["ox"]), _p(st["oy"]), _p(st["vx"]),
+                                            _p(st["vy"]), _p(st["vz"]), _p(st["id"])))
+        return {k: v[:n].copy() for k, v in st.items()}
+
+    def set_state(self, sid, cell, ox, oy, vx, vy, vz, ids=None):
+        n = len(cell)
+        a = [np.ascontiguousarray(cell, np.int32), np.ascontiguousarray(ox, np.float32),
+             np.ascontiguousarray(oy if oy is not None else np.zeros(n), np.float32),
+             np.ascontiguousarray(vx, np.float32), np.ascontiguousarray(vy, np.float32),
+             np.ascontiguousarray(vz, np.float32)]
+        idc = None if ids is None else np.ascontiguousarray(ids, np.uint32)
+        check(lib().iskra_species_set_state(self.h, sid, n, *[_p(x) for x in a], _p(idc)))
+
+    def count(self, sid):
+        n = C.c_int64()
+        check(lib().iskra_species_count(self.h, sid, C.byref(n)))
+        return n.value
+
+    def count_raw(self, sid):
+        return self.count(sid)
+
+    def sample_maxwellian(self, sid, n, wx, wv, dx=None, dv=None, seed=0, stream=0):
+        wx = np.asarray(wx, dtype=np.float64)
+        wv = np.asarray(wv, dtype=np.float64)
+        dx = None if dx is None else np.asarray(dx, dtype=np.float64)
+        dv = None if dv is None else np.asarray(dv, dtype=np.float64)
+        check(lib().iskra_species_sample_maxwellian(self.h, sid, int(n), _p(wx), _p(dx), _p(wv),
+                                                    _p(dv), int(seed), int(stream)))
+
+    # operators
+    def sort(self, sid, exact_cell=False):
+        check(lib().iskra_sort(self.h, sid, int(exact_cell)))
+
+    def gather(self, sid):
+        n = self.count(sid)
+        cap = max(n, 1)
+        out = np.zeros(3 * cap)
+        check(lib().iskra_gather(self.h, sid, cap, _p(out)))
+        return out.reshape(3, cap)[:, :n].T.copy()
+
+    def push(self, sid, dt):
+        check(lib().iskra_push(self.h, sid, float(dt)))
+
+    def deposit_species(self, sid):
+        check(lib().iskra_deposit_species(self.h, sid))
+
+    def deposit(self):
+        check(lib().iskra_deposit(self.h))
+
+    # fields (node arrays, flat with i fastest = column-major nx x ny)
+    def field(self, which, index=0):
+        out = np.zeros(self.nn)
+        check(lib().iskra_field_download(self.h, which, index, _p(out)))
+        return out
+
+    def set_field(self, which, values, index=0):
+        v = np.ascontiguousarray(values, dtype=np.float64).ravel()
+        assert v.size == self.nn
+        check(lib().iskra_field_upload(self.h, which, index, _p(v)))
+
+    # Poisson
+    def poisson(self, kind, eps0, tags=None, periodic_x=False, periodic_y=False, rtol=1e-8,
+                max_iter=0):
+        t = None if tags is None else np.ascontiguousarray(tags, dtype=np.int8).ravel()
+        check(lib().iskra_poisson_set(self.h, kind, float(eps0), _p(t), int(periodic_x),
+                                      int(periodic_y), float(rtol), int(max_iter)))
+
+    def tag_value(self, tag, value):
+        check(lib().iskra_poisson_set_tag_value(self.h, tag, float(value)))
+
+    def tag_waveform(self, tag, offset, amplitude, freq, phase=0.0):
+        check(lib().iskra_poisson_set_tag_waveform(self.h, tag, offset, amplitude, freq, phase))
+
+    def solve(self, warm=WARM_ZERO):
+        it = C.c_int()
+        rr = C.c_double()
+        check(lib().iskra_poisson_solve(self.h, warm, C.byref(it), C.byref(rr)))
+        return it.value, rr.value
+
+    def efield(self):
+        check(lib().iskra_efield(self.h))
+
+    # MCC
+    def mcc(self, src, types, energy, tables, ion=-1, target_m=1.0, target_q=0.0,
+            target_T=300.0):
+        off = [0]
+        e_all, s_all = [], []
+        for e, s in tables:
+            e_all += list(e)
+            s_all += list(s)
+            off.append(len(e_all))
+        ty = np.asarray(types, dtype=np.int32)
+        en = np.asarray(energy, dtype=np.float64)
+        off = np.asarray(off, dtype=np.int32)
+        e_all = np.asarray(e_all, dtype=np.float64)
+        s_all = np.asarray(s_all, dtype=np.float64)
+        mid = C.c_int()
+        check(lib().iskra_mcc_add(self.h, src, len(types), _p(ty), _p(en), _p(off), _p(e_all),
+                                  _p(s_all), ion, target_m, target_q, target_T, C.byref(mid)))
+        return mid.value
+
+    def mcc_run(self, mid, dt):
+        ev = np.zeros(8, dtype=np.int64)
+        check(lib().iskra_mcc_run(self.h, mid, float(dt), _p(ev)))
+        return ev
+
+    def mcc_counts(self, mid, ncoll, reset=False):
+        out = np.zeros(self.nn * ncoll, dtype=np.int64)
+        check(lib().iskra_mcc_counts(self.h, mid, int(reset), _p(out)))
+        return out.reshape(ncoll, self.nn)
+
+    # RNG
+    def rng(self, seed):
+        check(lib().iskra_rng_set(self.h, int(seed)))
+
+    def rng_export(self, stream, step, ids, nblocks):
+        ids = np.ascontiguousarray(ids, dtype=np.uint32)
+        out = np.zeros(len(ids) * nblocks * 4, dtype=np.uint32)
+        check(lib().iskra_rng_export(self.h, stream, step, _p(ids), len(ids), nblocks, _p(out)))
+        return out.reshape(len(ids), nblocks, 4)
+
+    # loop
+    def step(self, dt, nsteps=1):
+        check(lib().iskra_step(self.h, float(dt), int(nsteps)))
+
+    def stats(self):
+        s = Stats()
+        check(lib().iskra_stats_get(self.h, C.byref(s)))
+        return dict(steps=s.steps, particle_steps=s.particle_steps,
+                    poisson_solves=s.poisson_solves, poisson_iters=s.poisson_iters,
+                    last_iters=s.last_iters, last_relres=s.last_relres)
+
+    def sync(self):
+        check(lib().iskra_sync(self.h))

@@ -1,0 +1,257 @@
+/*
+ * isk_internal.h -- device data structures and helpers shared by the HIP
+ * translation units of libiskra_hip.so (not part of the public ABI).
+ *
+ * Particle state of a species lives in HBM as structure-of-arrays:
+ *   cell  int32   packed node index of the lower-left node: i | (j << 16);
+ *                 -1 marks a discarded particle (compacted by the next sort)
+ *   ox,oy float   in-cell offset in [0,1) along x / y  (oy unused in 1-D)
+ *   vx,vy,vz float velocity in m/s
+ *   id    uint32  particle identifier (optional; keys the MCC random draws)
+ * Two copies of every array exist; the counting sort scatters from one into
+ * the other and flips `cur`.
+ *
+ * Cells are grouped into square tiles of T x T cells (T cells in 1-D).  After
+ * a sort the particles of a tile are contiguous; the chunk table splits every
+ * tile into chunks of at most CH particles, one work-group each.  A chunk's
+ * work-group stages E of its tile plus a margin of M cells in LDS (window of
+ * W = T + 2M + 1 nodes per axis) and accumulates its CIC deposit there.
+ */
+#ifndef ISK_INTERNAL_H
+#define ISK_INTERNAL_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/iskra.h"
+
+#define ISK_MAX_SPECIES 8
+#define ISK_MAX_MCC 4
+#define ISK_MAX_COLL 8
+#define ISK_MAX_TAGS 16
+/* largest number of sort bins (tiles, or cells in a tile); LDS-resident */
+#define ISK_MAX_BINS 8192
+
+/* fixed-point scale of CIC weights: one particle deposits 2^32 in total */
+#define ISK_FIX_ONE 4294967296.0
+
+/* header words kept in device memory per species */
+enum {
+  HDR_NP = 0,        /* live + not yet compacted particle count */
+  HDR_NP_SORTED = 1, /* particles covered by the chunk table */
+  HDR_NCHUNKS = 2,   /* chunks in the chunk table */
+  HDR_NEXT_ID = 3,   /* next particle id handed out */
+  HDR_PUSHED = 4,    /* accumulated live particles pushed (particle-steps) */
+  HDR_OVERFLOW = 5,  /* set when an append would exceed the capacity */
+  HDR_WORDS = 8
+};
+
+struct IskChunk {
+  int tile;
+  int start;
+  int count;
+  int pad;
+};
+
+struct IskGrid {
+  int dim;
+  int nx, ny;   /* nodes (ny = 1 in 1-D) */
+  int ncx, ncy; /* cells (ncy = 1 in 1-D: the single row j = 0) */
+  double dx, dy;
+  double x0, y0;
+  int volper[4];
+};
+
+struct IskTune {
+  int T, log2T, CH, M, W;
+  int sort_every;
+};
+
+struct IskSpecies {
+  double q, m, w0;
+  int64_t cap;
+  int track_ids;
+  int bnd[2];
+  int cur;
+  int32_t *cell[2];
+  float *ox[2], *oy[2], *vx[2], *vy[2], *vz[2];
+  uint32_t *id[2];
+  long long *hdr;            /* HDR_WORDS int64 */
+  IskChunk *chunks;          /* max_chunks */
+  int *tile_chunk_begin;     /* ntiles + 1 */
+  long long *partial;        /* max_chunks * W^dim */
+  unsigned long long *gfall; /* nodes */
+  long long *cnt;            /* nodes: merged deposit (fixed point) */
+  int64_t max_chunks;
+  int deposited;             /* cnt holds this step's deposit */
+};
+
+struct IskMcc {
+  int src, ion;
+  int ncoll;
+  int types[ISK_MAX_COLL];
+  double energy[ISK_MAX_COLL];
+  std::vector<int> tab_off;
+  std::vector<double> tab_eps, tab_sig;
+  double *d_eps, *d_sig;
+  int *d_off;
+  double target_m, target_q, target_T;
+  double max_sigma_g;
+  double m_eV;               /* source mass in eV units (mcc.jl:26) */
+  double *target_n;          /* nodes */
+  double max_n;
+  unsigned long long *nu;    /* nodes * ncoll event counts */
+  unsigned long long *events;/* ncoll */
+  int *errflag;
+  /* ionisation product staging */
+  int *prod_flag;            /* per source particle: 1 if it ionised */
+  float *stage;              /* per source particle: 2 offsets + 3 v of the new electron + 3 v of the ions */
+  int *blk_cnt;              /* per block product count, scanned in place */
+  int64_t stage_cap;
+  uint32_t stream;
+  uint32_t step;
+};
+
+struct IskPoisson {
+  int kind = -1;
+  double eps0 = 1.0;
+  double rtol = 1e-8;
+  int max_iter = 10000;
+  int per_x = 0, per_y = 0;
+  std::vector<int8_t> tags;
+  int8_t *d_tags = nullptr;
+  double tag_val[ISK_MAX_TAGS + 1];
+  double *d_tag_val = nullptr;
+  /* waveforms */
+  int wf_on[ISK_MAX_TAGS + 1];
+  double wf_off[ISK_MAX_TAGS + 1], wf_amp[ISK_MAX_TAGS + 1],
+      wf_f[ISK_MAX_TAGS + 1], wf_ph[ISK_MAX_TAGS + 1];
+  int singular = 0; /* no Dirichlet node at all -> mean-free solution */
+  /* 1-D direct: factored tridiagonal (host-built) */
+  double *d_tri_a = nullptr, *d_tri_cp = nullptr, *d_tri_inv = nullptr;
+  double *d_work = nullptr;
+  /* CG state */
+  double *x = nullptr, *r = nullptr, *Ap = nullptr;
+  float *z = nullptr, *p[2] = {nullptr, nullptr};
+  double *scal = nullptr;      /* scalars, see poisson.hip */
+  double *partials = nullptr;  /* per-block partial sums */
+  unsigned int *counter = nullptr;
+  int *iflag = nullptr;        /* [0] done, [1] iterations */
+  /* multigrid */
+  int nlev = 0;
+  int ln[16];                  /* nodes per axis on each level */
+  float *me[16], *mf[16];      /* correction and rhs per level (FP32) */
+  int coarse_lev = 0;          /* first level solved by the single-WG kernel */
+  int last_iters = 0;
+  double last_relres = 0.0;
+};
+
+struct iskra_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int have_grid = 0;
+  IskGrid g;
+  IskTune t;
+  int ntx = 1, nty = 1, ntiles = 1;
+  int nspecies = 0;
+  IskSpecies sp[ISK_MAX_SPECIES];
+  int nmcc = 0;
+  IskMcc mcc[ISK_MAX_MCC];
+  IskPoisson ps;
+  /* fields */
+  double *rho = nullptr, *phi = nullptr, *bg = nullptr, *vol = nullptr;
+  float *E = nullptr; /* 1-D: Ex[nx]; 2-D: (Ex,Ey) interleaved per node */
+  /* sort scratch */
+  int *sort_H = nullptr, *sort_S = nullptr, *sort_tot = nullptr,
+      *sort_start = nullptr;
+  int64_t sort_H_words = 0, sort_S_words = 0;
+  /* misc scratch */
+  float *gather_out = nullptr;
+  int64_t gather_cap = 0;
+  uint64_t seed = 0;
+  int64_t step_count = 0;
+  iskra_stats stats;
+  double t_now = 0.0;
+};
+
+/* ---- error plumbing (api.hip) ---- */
+void isk_set_error(const char *fmt, ...);
+int isk_hip_fail(hipError_t e, const char *what, const char *file, int line);
+#define ISK_HIP(call)                                                   \
+  do {                                                                  \
+    hipError_t _e = (call);                                             \
+    if (_e != hipSuccess) return isk_hip_fail(_e, #call, __FILE__, __LINE__); \
+  } while (0)
+#define ISK_CHECK_LAUNCH() ISK_HIP(hipGetLastError())
+
+/* ---- Philox4x32-10 (Salmon et al., SC'11) ---- */
+struct IskU4 {
+  uint32_t x, y, z, w;
+};
+__host__ __device__ inline void isk_mulhilo(uint32_t a, uint32_t b, uint32_t *hi,
+                                            uint32_t *lo) {
+  uint64_t p = (uint64_t)a * (uint64_t)b;
+  *hi = (uint32_t)(p >> 32);
+  *lo = (uint32_t)p;
+}
+__host__ __device__ inline IskU4 isk_philox(IskU4 c, uint32_t k0, uint32_t k1) {
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    uint32_t hi0, lo0, hi1, lo1;
+    isk_mulhilo(0xD2511F53u, c.x, &hi0, &lo0);
+    isk_mulhilo(0xCD9E8D57u, c.z, &hi1, &lo1);
+    IskU4 n;
+    n.x = hi1 ^ c.y ^ k0;
+    n.y = lo1;
+    n.z = hi0 ^ c.w ^ k1;
+    n.w = lo0;
+    c = n;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+  return c;
+}
+/* uniform double in (0,1) from two words: 53 bits, centred in its bin */
+__host__ __device__ inline double isk_u53(uint32_t a, uint32_t b) {
+  uint64_t k = ((uint64_t)(a >> 6) << 27) | (uint64_t)(b >> 5);
+  return ((double)k + 0.5) * (1.0 / 9007199254740992.0);
+}
+/* uniform float in [0,1) from 24 bits */
+__host__ __device__ inline float isk_u24(uint32_t a) {
+  return (float)(a >> 8) * (1.0f / 16777216.0f);
+}
+
+/* ---- small host helpers ---- */
+int isk_alloc(void **p, size_t bytes);
+int isk_free(void *p);
+int isk_nodes(const iskra_ctx *c);
+int isk_species_ok(iskra_ctx *c, int sid);
+
+/* ---- kernels exposed across translation units ---- */
+int isk_fill_i64(iskra_ctx *c, void *p, int64_t n, long long v);
+int isk_sort_species(iskra_ctx *c, int sid, int exact_cell);
+int isk_push_species(iskra_ctx *c, int sid, double dt, int do_push);
+int isk_merge_species(iskra_ctx *c, int sid);
+int isk_rho(iskra_ctx *c);
+int isk_gather_species(iskra_ctx *c, int sid);
+int isk_convert_upload(iskra_ctx *c, int sid, int64_t np, const double *x,
+                       const double *v, const uint32_t *id);
+int isk_sample_maxwellian(iskra_ctx *c, int sid, int64_t n, const double *wx,
+                          const double *dx, const double *wv, const double *dv,
+                          uint64_t seed, uint32_t stream);
+int isk_poisson_setup(iskra_ctx *c);
+int isk_poisson_free(iskra_ctx *c);
+int isk_poisson_solve(iskra_ctx *c, int warm, int *iters, double *relres);
+int isk_efield(iskra_ctx *c);
+int isk_mcc_setup(iskra_ctx *c, int id);
+int isk_mcc_run(iskra_ctx *c, int id, double dt, int64_t *events);
+int isk_rng_export(iskra_ctx *c, uint32_t stream, uint32_t step,
+                   const uint32_t *ids, int64_t n, int nblocks, uint32_t *out);
+int isk_species_alloc(iskra_ctx *c, IskSpecies *s);
+void isk_species_release(IskSpecies *s);
+int isk_species_reset_chunks(iskra_ctx *c, int sid);
+
+#endif /* ISK_INTERNAL_H */

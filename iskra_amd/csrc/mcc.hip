@@ -1,0 +1,621 @@
+/*
+ * mcc.hip -- null-collision Monte Carlo collisions on gfx950.
+ *
+ * Reference: PIC.perform!(::MonteCarloCollisions) (Chemistry/src/mcc.jl:231-289)
+ * and the collision kernels (mcc.jl:53-229).  The reference draws
+ * Nc = N*max_P*np candidates by array index with replacement; here every
+ * particle is a candidate with probability N*max_P (the same expected number
+ * of candidates per particle, and no order dependence), after which process
+ * selection and acceptance follow the reference exactly:
+ *     k = floor(N U) + 1,  P_k = (1 - exp(-n sigma_k(eps) g dt)) / (N max_P),
+ *     accept if U > k/N - P_k.
+ * All per-candidate arithmetic is FP64 with the exp/log/sincos of
+ * isk_fp64math.h, so a CPU run that consumes the same draws reproduces every
+ * decision and every post-collision velocity bit for bit.
+ *
+ * Draws: Philox4x32-10, key = seed, counter = (id, step, stream, block);
+ * block 0 gives the candidate and process uniforms, blocks 1..5 the collision
+ * draws in the order the reference consumes them.  Uniforms are 53-bit
+ * (isk_u53); normals are Box-Muller pairs.
+ */
+#include <math.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "isk_fp64math.h"
+#include "isk_internal.h"
+
+#define MCC_BLOCK 256
+#define KB_MCC 1.3806503e-23      /* mcc.jl:75 */
+#define QE_MCC 1.60217646e-19     /* mcc.jl:26 */
+#define TWO_PI 6.283185307179586
+
+struct MccArgs {
+  int32_t *cell;
+  float *ox, *oy, *vx, *vy, *vz;
+  const uint32_t *id;
+  const long long *hdr;
+  const float *E;
+  const double *tn;
+  int dim, nx;
+  int ncoll;
+  int types[ISK_MAX_COLL];
+  double energy[ISK_MAX_COLL];
+  const int *toff;
+  const double *teps, *tsig;
+  double candP;  /* N * max_P */
+  double dt;
+  double m_eV;   /* source mass / qe */
+  double ms, mt; /* source and target masses (kg) */
+  double vth_t;  /* target thermal speed sqrt(2 kB T / m) */
+  double tqm;    /* target q/m */
+  uint32_t k0, k1, step, stream;
+  unsigned long long *nu, *events;
+  int *errflag;
+  int *blk_cnt;
+  float *stage;
+  int32_t *stage_cell;
+};
+
+struct Draws {
+  uint32_t id, step, stream, k0, k1;
+  int block;
+  int half;
+  IskU4 cur;
+  __device__ double next() {
+    if (half == 0) {
+      IskU4 c = {id, step, stream, (uint32_t)block};
+      cur = isk_philox(c, k0, k1);
+      ++block;
+      half = 1;
+      return isk_u53(cur.x, cur.y);
+    }
+    half = 0;
+    return isk_u53(cur.z, cur.w);
+  }
+};
+
+__device__ double interp_sigma(const MccArgs &a, int k, double e) {
+  int lo = a.toff[k], hi = a.toff[k + 1] - 1;
+  if (e <= a.teps[lo]) return a.tsig[lo];
+  if (e >= a.teps[hi]) return a.tsig[hi];
+  while (hi - lo > 1) {
+    int mid = (lo + hi) >> 1;
+    if (a.teps[mid] <= e) lo = mid;
+    else hi = mid;
+  }
+  double t = (e - a.teps[lo]) / (a.teps[hi] - a.teps[lo]);
+  return a.tsig[lo] * (1.0 - t) + a.tsig[hi] * t;
+}
+
+struct Ang {
+  double st, ct, sp, cp;
+};
+
+__device__ Ang euler_angles(double vx, double vy, double vz) {
+  Ang g;
+  double nv = sqrt(vx * vx + vy * vy + vz * vz);
+  g.ct = vz / nv;
+  g.st = sqrt(1.0 - g.ct * g.ct);
+  if (g.st == 0.0) {
+    g.cp = 1.0;
+    g.sp = 0.0;
+  } else {
+    g.cp = vx / nv / g.st;
+    g.sp = vy / nv / g.st;
+  }
+  return g;
+}
+
+/* [sinX cosE, sinX sinE, cosX] * unrotated(theta, phi)  (mcc.jl:83-87) */
+__device__ void rotate(const Ang &g, double sX, double cX, double sE, double cE,
+                       double *r) {
+  double a0 = sX * cE, a1 = sX * sE, a2 = cX;
+  r[0] = a0 * (g.cp * g.ct) + a1 * (-g.sp) + a2 * (g.cp * g.st);
+  r[1] = a0 * (-g.sp * g.ct) + a1 * g.cp + a2 * (g.sp * g.st);
+  r[2] = a0 * (-g.st) + a1 * 0.0 + a2 * g.ct;
+}
+
+__device__ void isotropic_scattering(double vx, double vy, double vz, Draws &d,
+                                     double *r) {
+  Ang g = euler_angles(vx, vy, vz);
+  double sX, cX, sE, cE;
+  isk_sincos2pi(d.next(), &sX, &cX);
+  isk_sincos2pi(d.next(), &sE, &cE);
+  rotate(g, sX, cX, sE, cE, r);
+}
+
+__device__ void diffuse_reflection(double vx, double vy, double vz, Draws &d,
+                                   double *r) {
+  Ang g = euler_angles(vx, vy, vz);
+  double sX = sqrt(d.next());
+  double cX = -sqrt(1.0 - sX * sX);
+  double sE, cE;
+  isk_sincos2pi(d.next(), &sE, &cE);
+  rotate(g, sX, cX, sE, cE, r);
+}
+
+__device__ void randn3(Draws &d, double *n) {
+  double u1 = d.next(), u2 = d.next(), u3 = d.next(), u4 = d.next();
+  double r1 = sqrt(-2.0 * isk_log(u1)), r2 = sqrt(-2.0 * isk_log(u3));
+  double s1, c1, s2, c2;
+  isk_sincos2pi(u2, &s1, &c1);
+  isk_sincos2pi(u4, &s2, &c2);
+  n[0] = r1 * c1;
+  n[1] = r1 * s1;
+  n[2] = r2 * c2;
+}
+
+__global__ __launch_bounds__(MCC_BLOCK) void k_mcc(MccArgs a) {
+  __shared__ int wcount[MCC_BLOCK / 64];
+  const long long np = a.hdr[HDR_NP];
+  const long long p = (long long)blockIdx.x * MCC_BLOCK + threadIdx.x;
+  int prod = 0;
+  float pe[3] = {0.f, 0.f, 0.f}, pi[3] = {0.f, 0.f, 0.f};
+  int32_t pcell = 0;
+  float pox = 0.f, poy = 0.f;
+  if (p < np) {
+    int32_t c = a.cell[p];
+    if (c >= 0) {
+      Draws d;
+      d.id = a.id[p];
+      d.step = a.step;
+      d.stream = a.stream;
+      d.k0 = a.k0;
+      d.k1 = a.k1;
+      d.block = 0;
+      d.half = 0;
+      double Uc = d.next();
+      double U = d.next();
+      if (Uc < a.candP) {
+        int i = c & 0xFFFF, j = c >> 16;
+        int node = a.dim == 2 ? j * a.nx + i : i;
+        double n = a.tn[node];
+        if (n >= 0.0) {
+          const int N = a.ncoll;
+          int k = (int)floor((double)N * U);
+          if (k > N - 1) k = N - 1;
+          double vx = a.vx[p], vy = a.vy[p], vz = a.vz[p];
+          double tvx = 0.0, tvy = 0.0;
+          if (a.tqm != 0.0) {
+            if (a.dim == 2) {
+              tvx = a.tqm * (double)a.E[2 * node] * a.dt;
+              tvy = a.tqm * (double)a.E[2 * node + 1] * a.dt;
+            } else {
+              tvx = a.tqm * (double)a.E[node] * a.dt;
+            }
+          }
+          double gx = tvx - vx, gy = tvy - vy, gz = 0.0 - vz;
+          double g = sqrt(gx * gx + gy * gy + gz * gz);
+          double eps = (0.5 * a.m_eV) * (g * g);
+          double sg = interp_sigma(a, k, eps) * g;
+          double P = 1.0 - isk_exp(((-n) * sg) * a.dt);
+          P /= a.candP;
+          if (P > 1.0) atomicOr(a.errflag, 1);
+          if (U > (double)(k + 1) / (double)N - P) {
+            atomicAdd(&a.nu[(size_t)node * N + k], 1ull);
+            atomicAdd(&a.events[k], 1ull);
+            const int type = a.types[k];
+            double sv[3] = {vx, vy, vz};
+            double out[3];
+            if (type == ISKRA_COLL_ELASTIC_ISOTROPIC || type == ISKRA_COLL_ELASTIC_BACKWARD) {
+              double mr1 = a.ms / (a.ms + a.mt), mr2 = a.mt / (a.ms + a.mt);
+              double tv[3];
+              randn3(d, tv);
+              for (int q = 0; q < 3; ++q) tv[q] *= a.vth_t;
+              double vr[3], w[3];
+              for (int q = 0; q < 3; ++q) {
+                vr[q] = sv[q] - tv[q];
+                w[q] = mr1 * sv[q] + mr2 * tv[q];
+              }
+              double nvr = sqrt(vr[0] * vr[0] + vr[1] * vr[1] + vr[2] * vr[2]);
+              double dir[3];
+              if (type == ISKRA_COLL_ELASTIC_ISOTROPIC)
+                isotropic_scattering(vr[0], vr[1], vr[2], d, dir);
+              else
+                diffuse_reflection(vr[0], vr[1], vr[2], d, dir);
+              for (int q = 0; q < 3; ++q) out[q] = w[q] + mr2 * (nvr * dir[q]);
+              a.vx[p] = (float)out[0];
+              a.vy[p] = (float)out[1];
+              a.vz[p] = (float)out[2];
+            } else if (type == ISKRA_COLL_EXCITATION) {
+              double sE = (0.5 * a.m_eV) * (vx * vx + vy * vy + vz * vz) - a.energy[k];
+              if (sE >= 0.0) {
+                double al = sqrt(2.0 / a.m_eV);
+                double ev = al * sqrt(sE);
+                double dir[3];
+                isotropic_scattering(vx, vy, vz, d, dir);
+                a.vx[p] = (float)(ev * dir[0]);
+                a.vy[p] = (float)(ev * dir[1]);
+                a.vz[p] = (float)(ev * dir[2]);
+              }
+            } else if (type == ISKRA_COLL_IONIZATION) {
+              double sE = (0.5 * a.m_eV) * (vx * vx + vy * vy + vz * vz) - a.energy[k];
+              if (sE >= 0.0) {
+                double e1E = sE * d.next();
+                double e2E = sE - e1E;
+                double al = sqrt(2.0 / a.m_eV);
+                double e1v = al * sqrt(e1E), e2v = al * sqrt(e2E);
+                double dir[3];
+                diffuse_reflection(vx, vy, vz, d, dir);
+                double nv[3] = {e1v * dir[0], e1v * dir[1], e1v * dir[2]};
+                a.vx[p] = (float)nv[0];
+                a.vy[p] = (float)nv[1];
+                a.vz[p] = (float)nv[2];
+                double dir2[3];
+                diffuse_reflection(nv[0], nv[1], nv[2], d, dir2);
+                double tv[3];
+                randn3(d, tv);
+                prod = 1;
+                pcell = c;
+                pox = a.ox[p];
+                poy = a.dim == 2 ? a.oy[p] : 0.f;
+                for (int q = 0; q < 3; ++q) {
+                  pe[q] = (float)(e2v * dir2[q]);
+                  pi[q] = (float)(tv[q] * a.vth_t);
+                }
+              }
+            }
+          }
+        }
+      }
+    }
+  }
+  /* deterministic block compaction of the ionisation products */
+  unsigned long long m = __ballot(prod != 0);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int rank = __popcll(m & ((1ull << lane) - 1ull));
+  if (lane == 0) wcount[w] = __popcll(m);
+  __syncthreads();
+  int off = 0;
+  for (int k = 0; k < w; ++k) off += wcount[k];
+  if (prod) {
+    size_t s = (size_t)blockIdx.x * MCC_BLOCK + off + rank;
+    a.stage_cell[s] = pcell;
+    float *st = a.stage + s * 8;
+    st[0] = pox;
+    st[1] = poy;
+    st[2] = pe[0];
+    st[3] = pe[1];
+    st[4] = pe[2];
+    st[5] = pi[0];
+    st[6] = pi[1];
+    st[7] = pi[2];
+  }
+  if (threadIdx.x == 0) {
+    int tot = 0;
+    for (int k = 0; k < MCC_BLOCK / 64; ++k) tot += wcount[k];
+    a.blk_cnt[blockIdx.x] = tot;
+  }
+}
+
+/* exclusive scan of the block product counts (one work-group); cnt[nblk] = total */
+__global__ __launch_bounds__(1024) void k_mcc_scan(int *cnt, int nblk) {
+  __shared__ int sh[1024];
+  int per = (nblk + 1023) / 1024;
+  int lo = threadIdx.x * per, hi = min(lo + per, nblk);
+  int sum = 0;
+  for (int b = lo; b < hi; ++b) sum += cnt[b];
+  sh[threadIdx.x] = sum;
+  __syncthreads();
+  for (int o = 1; o < 1024; o <<= 1) {
+    int v = threadIdx.x >= o ? sh[threadIdx.x - o] : 0;
+    __syncthreads();
+    sh[threadIdx.x] += v;
+    __syncthreads();
+  }
+  int run = sh[threadIdx.x] - sum;
+  for (int b = lo; b < hi; ++b) {
+    int v = cnt[b];
+    cnt[b] = run;
+    run += v;
+  }
+  if (threadIdx.x == 1023) cnt[nblk] = sh[1023];
+}
+
+struct AppendArgs {
+  const int *blk_off; /* scanned, [nblk] = total */
+  const float *stage;
+  const int32_t *stage_cell;
+  int nblk;
+  /* electron (source) species */
+  int32_t *ec;
+  float *eox, *eoy, *evx, *evy, *evz;
+  uint32_t *eid;
+  long long *ehdr;
+  long long ecap;
+  /* ion species */
+  int32_t *ic;
+  float *iox, *ioy, *ivx, *ivy, *ivz;
+  uint32_t *iid;
+  long long *ihdr;
+  long long icap;
+  int R;
+  int dim;
+};
+
+__global__ void k_mcc_append(AppendArgs a) {
+  const int b = blockIdx.x;
+  const int k = threadIdx.x;
+  const int base = a.blk_off[b];
+  const int n = a.blk_off[b + 1] - base;
+  if (k >= n) return;
+  const int g = base + k;
+  const size_t s = (size_t)b * MCC_BLOCK + k;
+  const float *st = a.stage + s * 8;
+  const int32_t c = a.stage_cell[s];
+  long long enp = a.ehdr[HDR_NP];
+  long long pe = enp + g;
+  if (pe < a.ecap) {
+    a.ec[pe] = c;
+    a.eox[pe] = st[0];
+    if (a.dim == 2) a.eoy[pe] = st[1];
+    a.evx[pe] = st[2];
+    a.evy[pe] = st[3];
+    a.evz[pe] = st[4];
+    if (a.eid) a.eid[pe] = (uint32_t)(a.ehdr[HDR_NEXT_ID] + g);
+  }
+  if (a.ic && a.R > 0) {
+    long long inp = a.ihdr[HDR_NP];
+    for (int r = 0; r < a.R; ++r) {
+      long long pi = inp + (long long)g * a.R + r;
+      if (pi >= a.icap) break;
+      a.ic[pi] = c;
+      a.iox[pi] = st[0];
+      if (a.dim == 2) a.ioy[pi] = st[1];
+      a.ivx[pi] = st[5];
+      a.ivy[pi] = st[6];
+      a.ivz[pi] = st[7];
+      if (a.iid) a.iid[pi] = (uint32_t)(a.ihdr[HDR_NEXT_ID] + (long long)g * a.R + r);
+    }
+  }
+}
+
+__global__ void k_mcc_commit(const int *blk_off, int nblk, long long *ehdr,
+                             long long ecap, long long *ihdr, long long icap,
+                             int R) {
+  long long tot = blk_off[nblk];
+  if (tot == 0) return;
+  long long ne = ehdr[HDR_NP] + tot;
+  if (ne > ecap) {
+    ehdr[HDR_OVERFLOW] = 1;
+    ne = ecap;
+  }
+  ehdr[HDR_NP] = ne;
+  ehdr[HDR_NEXT_ID] += tot;
+  if (ihdr && R > 0) {
+    long long ni = ihdr[HDR_NP] + tot * R;
+    if (ni > icap) {
+      ihdr[HDR_OVERFLOW] = 1;
+      ni = icap;
+    }
+    ihdr[HDR_NP] = ni;
+    ihdr[HDR_NEXT_ID] += tot * R;
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+int isk_mcc_setup(iskra_ctx *c, int idx) {
+  IskMcc &m = c->mcc[idx];
+  const IskSpecies &src = c->sp[m.src];
+  int nn = isk_nodes(c);
+  m.m_eV = src.m / QE_MCC;
+  /* max sigma_t g over the union of table nodes and 0 (mcc.jl:27-51) */
+  std::vector<double> eps = {0.0};
+  for (int k = 0; k < m.ncoll; ++k)
+    for (int q = m.tab_off[k]; q < m.tab_off[k + 1]; ++q) eps.push_back(m.tab_eps[q]);
+  std::sort(eps.begin(), eps.end());
+  eps.erase(std::unique(eps.begin(), eps.end()), eps.end());
+  double alpha = sqrt(2.0 / m.m_eV);
+  double best = 0.0;
+  for (double e : eps) {
+    double v = alpha * sqrt(e);
+    double s = 0.0;
+    for (int k = 0; k < m.ncoll; ++k) {
+      int lo = m.tab_off[k], hi = m.tab_off[k + 1] - 1;
+      double sig;
+      if (e <= m.tab_eps[lo]) sig = m.tab_sig[lo];
+      else if (e >= m.tab_eps[hi]) sig = m.tab_sig[hi];
+      else {
+        int a = lo, b = hi;
+        while (b - a > 1) {
+          int mid = (a + b) / 2;
+          if (m.tab_eps[mid] <= e) a = mid;
+          else b = mid;
+        }
+        double t = (e - m.tab_eps[a]) / (m.tab_eps[b] - m.tab_eps[a]);
+        sig = m.tab_sig[a] * (1.0 - t) + m.tab_sig[b] * t;
+      }
+      s += sig * v;
+    }
+    if (s > best) best = s;
+  }
+  m.max_sigma_g = best;
+  size_t ntab = m.tab_eps.size();
+  if (isk_alloc((void **)&m.d_eps, ntab * 8) || isk_alloc((void **)&m.d_sig, ntab * 8) ||
+      isk_alloc((void **)&m.d_off, (m.ncoll + 1) * 4) ||
+      isk_alloc((void **)&m.target_n, (size_t)nn * 8) ||
+      isk_alloc((void **)&m.nu, (size_t)nn * m.ncoll * 8) ||
+      isk_alloc((void **)&m.events, ISK_MAX_COLL * 8) || isk_alloc((void **)&m.errflag, 64))
+    return ISKRA_E_HIP;
+  ISK_HIP(hipMemcpy(m.d_eps, m.tab_eps.data(), ntab * 8, hipMemcpyHostToDevice));
+  ISK_HIP(hipMemcpy(m.d_sig, m.tab_sig.data(), ntab * 8, hipMemcpyHostToDevice));
+  ISK_HIP(hipMemcpy(m.d_off, m.tab_off.data(), (m.ncoll + 1) * 4, hipMemcpyHostToDevice));
+  ISK_HIP(hipMemset(m.target_n, 0, (size_t)nn * 8));
+  ISK_HIP(hipMemset(m.nu, 0, (size_t)nn * m.ncoll * 8));
+  ISK_HIP(hipMemset(m.events, 0, ISK_MAX_COLL * 8));
+  ISK_HIP(hipMemset(m.errflag, 0, 64));
+  m.max_n = 0.0;
+  m.step = 0;
+  m.stream = 0x4D430000u + (uint32_t)idx;
+  m.stage_cap = 0;
+  m.prod_flag = nullptr;
+  m.stage = nullptr;
+  m.blk_cnt = nullptr;
+  return ISKRA_OK;
+}
+
+int isk_mcc_run(iskra_ctx *c, int idx, double dt, int64_t *events) {
+  IskMcc &m = c->mcc[idx];
+  IskSpecies &s = c->sp[m.src];
+  const int N = m.ncoll;
+  /* max_P = 1 - exp(-max_n max_sigma_g dt); assert max_P <= 1/N (mcc.jl:242-246) */
+  double maxP = 1.0 - isk_exp(-m.max_n * m.max_sigma_g * dt);
+  if (maxP > 1.0 / N) {
+    isk_set_error("mcc: maximum probability %.6g is greater than 1/%d (mcc.jl:244)", maxP, N);
+    return ISKRA_E_PROB;
+  }
+  int64_t cap = s.cap;
+  int nblk = (int)((cap + MCC_BLOCK - 1) / MCC_BLOCK);
+  if (m.stage_cap < cap) {
+    isk_free(m.stage);
+    isk_free(m.blk_cnt);
+    isk_free(m.prod_flag);
+    m.stage = nullptr;
+    m.blk_cnt = nullptr;
+    m.prod_flag = nullptr;
+    if (isk_alloc((void **)&m.stage, (size_t)nblk * MCC_BLOCK * 8 * 4) ||
+        isk_alloc((void **)&m.blk_cnt, (size_t)(nblk + 1) * 4) ||
+        isk_alloc((void **)&m.prod_flag, (size_t)nblk * MCC_BLOCK * 4))
+      return ISKRA_E_HIP;
+    m.stage_cap = cap;
+  }
+  if (events) ISK_HIP(hipMemsetAsync(m.events, 0, ISK_MAX_COLL * 8, c->stream));
+  MccArgs a;
+  int b = s.cur;
+  a.cell = s.cell[b];
+  a.ox = s.ox[b];
+  a.oy = s.oy[b];
+  a.vx = s.vx[b];
+  a.vy = s.vy[b];
+  a.vz = s.vz[b];
+  a.id = s.id[b];
+  a.hdr = s.hdr;
+  a.E = c->E;
+  a.tn = m.target_n;
+  a.dim = c->g.dim;
+  a.nx = c->g.nx;
+  a.ncoll = N;
+  for (int k = 0; k < ISK_MAX_COLL; ++k) {
+    a.types[k] = k < N ? m.types[k] : 0;
+    a.energy[k] = k < N ? m.energy[k] : 0.0;
+  }
+  a.toff = m.d_off;
+  a.teps = m.d_eps;
+  a.tsig = m.d_sig;
+  a.candP = (double)N * maxP;
+  a.dt = dt;
+  a.m_eV = m.m_eV;
+  a.ms = s.m;
+  a.mt = m.target_m;
+  a.vth_t = sqrt(2.0 * KB_MCC * m.target_T / m.target_m);
+  a.tqm = m.target_q / m.target_m;
+  a.k0 = (uint32_t)c->seed;
+  a.k1 = (uint32_t)(c->seed >> 32);
+  a.step = m.step;
+  a.stream = m.stream;
+  a.nu = m.nu;
+  a.events = m.events;
+  a.errflag = m.errflag;
+  a.blk_cnt = m.blk_cnt;
+  a.stage = m.stage;
+  a.stage_cell = m.prod_flag;
+  hipLaunchKernelGGL(k_mcc, dim3(nblk), dim3(MCC_BLOCK), 0, c->stream, a);
+  ISK_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_mcc_scan, dim3(1), dim3(1024), 0, c->stream, m.blk_cnt, nblk);
+  ISK_CHECK_LAUNCH();
+  AppendArgs p;
+  p.blk_off = m.blk_cnt;
+  p.stage = m.stage;
+  p.stage_cell = m.prod_flag;
+  p.nblk = nblk;
+  p.ec = s.cell[b];
+  p.eox = s.ox[b];
+  p.eoy = s.oy[b];
+  p.evx = s.vx[b];
+  p.evy = s.vy[b];
+  p.evz = s.vz[b];
+  p.eid = s.track_ids ? s.id[b] : nullptr;
+  p.ehdr = s.hdr;
+  p.ecap = s.cap;
+  int R = 0;
+  p.ic = nullptr;
+  p.ihdr = nullptr;
+  p.icap = 0;
+  bool has_ion = false;
+  for (int k = 0; k < N; ++k) has_ion |= m.types[k] == ISKRA_COLL_IONIZATION;
+  if (has_ion && m.ion >= 0 && m.ion != m.src) {
+    IskSpecies &is = c->sp[m.ion];
+    R = (int)rint(s.w0 / is.w0);
+    int ib = is.cur;
+    p.ic = is.cell[ib];
+    p.iox = is.ox[ib];
+    p.ioy = is.oy[ib];
+    p.ivx = is.vx[ib];
+    p.ivy = is.vy[ib];
+    p.ivz = is.vz[ib];
+    p.iid = is.track_ids ? is.id[ib] : nullptr;
+    p.ihdr = is.hdr;
+    p.icap = is.cap;
+    is.deposited = 0;
+  }
+  p.R = R;
+  p.dim = c->g.dim;
+  hipLaunchKernelGGL(k_mcc_append, dim3(nblk), dim3(MCC_BLOCK), 0, c->stream, p);
+  ISK_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_mcc_commit, dim3(1), dim3(1), 0, c->stream, m.blk_cnt, nblk,
+                     s.hdr, (long long)s.cap, p.ihdr, (long long)p.icap, R);
+  ISK_CHECK_LAUNCH();
+  s.deposited = 0;
+  m.step += 1;
+  if (events) {
+    unsigned long long ev[ISK_MAX_COLL];
+    int err = 0;
+    ISK_HIP(hipMemcpyAsync(ev, m.events, sizeof(ev), hipMemcpyDeviceToHost, c->stream));
+    ISK_HIP(hipMemcpyAsync(&err, m.errflag, 4, hipMemcpyDeviceToHost, c->stream));
+    ISK_HIP(hipStreamSynchronize(c->stream));
+    for (int k = 0; k < N; ++k) events[k] = (int64_t)ev[k];
+    if (err) {
+      isk_set_error("mcc: collision probability above 1 (mcc.jl:273-279)");
+      return ISKRA_E_PROB;
+    }
+  }
+  return ISKRA_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+__global__ void k_rng_export(const uint32_t *ids, long long n, int nblocks,
+                             uint32_t step, uint32_t stream, uint32_t k0,
+                             uint32_t k1, uint32_t *out) {
+  long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n * nblocks) return;
+  long long p = q / nblocks;
+  int b = (int)(q % nblocks);
+  IskU4 c = {ids[p], step, stream, (uint32_t)b};
+  IskU4 r = isk_philox(c, k0, k1);
+  uint32_t *o = out + q * 4;
+  o[0] = r.x;
+  o[1] = r.y;
+  o[2] = r.z;
+  o[3] = r.w;
+}
+
+int isk_rng_export(iskra_ctx *c, uint32_t stream, uint32_t step, const uint32_t *ids,
+                   int64_t n, int nblocks, uint32_t *out) {
+  if (n <= 0 || nblocks <= 0) return ISKRA_OK;
+  uint32_t *dids = nullptr, *dout = nullptr;
+  if (isk_alloc((void **)&dids, n * 4) || isk_alloc((void **)&dout, n * nblocks * 16))
+    return ISKRA_E_HIP;
+  ISK_HIP(hipMemcpy(dids, ids, n * 4, hipMemcpyHostToDevice));
+  long long tot = n * nblocks;
+  hipLaunchKernelGGL(k_rng_export, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0,
+                     c->stream, dids, (long long)n, nblocks, step, stream,
+                     (uint32_t)c->seed, (uint32_t)(c->seed >> 32), dout);
+  ISK_CHECK_LAUNCH();
+  ISK_HIP(hipStreamSynchronize(c->stream));
+  ISK_HIP(hipMemcpy(out, dout, n * nblocks * 16, hipMemcpyDeviceToHost));
+  isk_free(dids);
+  isk_free(dout);
+  return ISKRA_OK;
+}

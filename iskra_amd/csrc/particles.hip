@@ -1,0 +1,1024 @@
+/*
+ * particles.hip -- particle kernels of the electrostatic PIC step on gfx950.
+ *
+ *  k_push<DIM>      fused gather + push + boundary + CIC deposit, one
+ *                   work-group per chunk of a tile; E window and deposit
+ *                   window in LDS; deposit accumulated as 64-bit fixed point
+ *                   (integer adds are associative, so the result does not
+ *                   depend on particle order or scheduling).
+ *  k_merge<DIM>     per node, sums the chunk windows that cover it in a fixed
+ *                   order plus the global fallback accumulator.
+ *  k_sort_*         stable counting sort by tile (or by cell in tile):
+ *                   block histograms -> column scans -> stable block scatter.
+ *  k_rho            rho = sum_s q_s n_s + background,  n_s = deposit / volume.
+ *
+ * Reference semantics followed (paths relative to the reference tree):
+ *  particle_cell            ParticleInCell/src/ParticleInCell.jl:28-35
+ *  grid_to_particle (CIC)   ParticleInCell/src/pic/cloud_in_cell.jl:20-36
+ *  push_in_cartesian! (B=0) ParticleInCell/src/pic/pushers.jl:37-50
+ *  wrap! / discard!         ParticleInCell/src/pic/surfaces/wrap.jl:1-33
+ *  particle_to_grid (CIC)   ParticleInCell/src/pic/cloud_in_cell.jl:1-18
+ *  density, cell_volume     pic/kinetic.jl:53, RegularGrids/src/RegularGrids.jl:106-118
+ *  rho accumulation         ParticleInCell/src/ParticleInCell.jl:118-124
+ */
+#include <math.h>
+#include <stdio.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "isk_fp64math.h"
+#include "isk_internal.h"
+
+#define PUSH_BLOCK 256
+#define PUSH_UNROLL 2
+#define SORT_BLOCK 256
+#define SORT_PT 32
+#define SORT_CHUNK (SORT_BLOCK * SORT_PT)
+#define SORT_SEG 32
+
+/* ------------------------------------------------------------------------ */
+/* small utility kernels                                                     */
+/* ------------------------------------------------------------------------ */
+__global__ void k_fill_i64(long long *p, long long n, long long v) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  long long stride = (long long)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) p[i] = v;
+}
+
+int isk_fill_i64(iskra_ctx *c, void *p, int64_t n, long long v) {
+  if (n <= 0) return ISKRA_OK;
+  int blocks = (int)std::min<int64_t>((n + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_fill_i64, dim3(blocks), dim3(256), 0, c->stream,
+                     (long long *)p, (long long)n, v);
+  ISK_CHECK_LAUNCH();
+  return ISKRA_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* upload (host conversion) and Maxwellian loader                           */
+/* ------------------------------------------------------------------------ */
+/* particle_cell with the grid origin removed: s = (x - x0)/dh, i = floor(s),
+ * offset = s - i rounded to FP32 (an offset that rounds up to 1 moves to the
+ * next cell with offset 0). */
+static int cell_of(double x, double x0, double dh, int nc, int *i, float *o) {
+  double s = (x - x0) / dh;
+  double fl = floor(s);
+  if (!(fl >= 0.0) || fl >= (double)nc) return -1;
+  int ii = (int)fl;
+  float of = (float)(s - fl);
+  if (of >= 1.0f) {
+    of = 0.0f;
+    ii += 1;
+    if (ii >= nc) return -1;
+  }
+  *i = ii;
+  *o = of;
+  return 0;
+}
+
+int isk_convert_upload(iskra_ctx *c, int sid, int64_t np, const double *x,
+                       const double *v, const uint32_t *id) {
+  IskSpecies *s = &c->sp[sid];
+  const IskGrid &g = c->g;
+  if (np < 0 || np > s->cap) {
+    isk_set_error("upload: np=%lld outside [0, capacity=%lld]", (long long)np,
+                  (long long)s->cap);
+    return ISKRA_E_CAPACITY;
+  }
+  std::vector<int32_t> cell(np);
+  std::vector<float> ox(np), oy(g.dim == 2 ? np : 0), vx(np), vy(np), vz(np);
+  std::vector<uint32_t> ids(s->track_ids ? np : 0);
+  uint32_t maxid = 0;
+  for (int64_t p = 0; p < np; ++p) {
+    int i = 0, j = 0;
+    float fx = 0.f, fy = 0.f;
+    if (cell_of(x[p], g.x0, g.dx, g.ncx, &i, &fx) != 0) {
+      isk_set_error("upload: particle %lld has x=%g outside the grid",
+                    (long long)p, x[p]);
+      return ISKRA_E_ARG;
+    }
+    if (g.dim == 2) {
+      if (cell_of(x[np + p], g.y0, g.dy, g.ncy, &j, &fy) != 0) {
+        isk_set_error("upload: particle %lld has y=%g outside the grid",
+                      (long long)p, x[np + p]);
+        return ISKRA_E_ARG;
+      }
+      oy[p] = fy;
+    }
+    cell[p] = i | (j << 16);
+    ox[p] = fx;
+    vx[p] = (float)v[p];
+    vy[p] = (float)v[np + p];
+    vz[p] = (float)v[2 * np + p];
+    if (s->track_ids) {
+      ids[p] = id ? id[p] : (uint32_t)p;
+      if (ids[p] > maxid) maxid = ids[p];
+    }
+  }
+  int b = s->cur;
+  if (np > 0) {
+    ISK_HIP(hipMemcpyAsync(s->cell[b], cell.data(), np * 4, hipMemcpyHostToDevice, c->stream));
+    ISK_HIP(hipMemcpyAsync(s->ox[b], ox.data(), np * 4, hipMemcpyHostToDevice, c->stream));
+    if (g.dim == 2)
+      ISK_HIP(hipMemcpyAsync(s->oy[b], oy.data(), np * 4, hipMemcpyHostToDevice, c->stream));
+    ISK_HIP(hipMemcpyAsync(s->vx[b], vx.data(), np * 4, hipMemcpyHostToDevice, c->stream));
+    ISK_HIP(hipMemcpyAsync(s->vy[b], vy.data(), np * 4, hipMemcpyHostToDevice, c->stream));
+    ISK_HIP(hipMemcpyAsync(s->vz[b], vz.data(), np * 4, hipMemcpyHostToDevice, c->stream));
+    if (s->track_ids)
+      ISK_HIP(hipMemcpyAsync(s->id[b], ids.data(), np * 4, hipMemcpyHostToDevice, c->stream));
+  }
+  long long hdr[HDR_WORDS];
+  ISK_HIP(hipMemcpyAsync(hdr, s->hdr, sizeof(hdr), hipMemcpyDeviceToHost, c->stream));
+  ISK_HIP(hipStreamSynchronize(c->stream));
+  hdr[HDR_NP] = np;
+  hdr[HDR_NP_SORTED] = 0;
+  hdr[HDR_NCHUNKS] = 0;
+  hdr[HDR_NEXT_ID] = s->track_ids ? (long long)maxid + 1 : np;
+  hdr[HDR_OVERFLOW] = 0;
+  ISK_HIP(hipMemcpyAsync(s->hdr, hdr, sizeof(hdr), hipMemcpyHostToDevice, c->stream));
+  ISK_HIP(hipStreamSynchronize(c->stream));
+  return isk_sort_species(c, sid, 0);
+}
+
+/* sample!(::MaxwellianSource): x = U*wx + dx, v = N(0,1)*wv + dv.
+ * Draws: Philox4x32-10, key = seed, counter = (k_lo, k_hi, stream, block). */
+struct SampleArgs {
+  int32_t *cell;
+  float *ox, *oy, *vx, *vy, *vz;
+  uint32_t *id;
+  long long *hdr;
+  long long n;
+  double wx[2], dxo[2], wv[3], dv[3];
+  double x0, y0, dx, dy;
+  int ncx, ncy, dim;
+  uint32_t k0, k1, stream;
+};
+
+__global__ __launch_bounds__(256) void k_sample(SampleArgs a) {
+  long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= a.n) return;
+  long long np0 = a.hdr[HDR_NP];
+  long long p = np0 + k;
+  IskU4 c0 = {(uint32_t)k, (uint32_t)(k >> 32), a.stream, 0u};
+  IskU4 r0 = isk_philox(c0, a.k0, a.k1);
+  c0.w = 1u;
+  IskU4 r1 = isk_philox(c0, a.k0, a.k1);
+  c0.w = 2u;
+  IskU4 r2 = isk_philox(c0, a.k0, a.k1);
+  /* positions */
+  double ux = isk_u53(r0.x, r0.y);
+  double px = ux * a.wx[0] + a.dxo[0];
+  double s = (px - a.x0) / a.dx;
+  double fl = floor(s);
+  int i = (int)fl;
+  float of = (float)(s - fl);
+  if (of >= 1.0f) { of = 0.0f; i += 1; }
+  i = min(max(i, 0), a.ncx - 1);
+  int j = 0;
+  if (a.dim == 2) {
+    double uy = isk_u53(r0.z, r0.w);
+    double py = uy * a.wx[1] + a.dxo[1];
+    double sy = (py - a.y0) / a.dy;
+    double fly = floor(sy);
+    j = (int)fly;
+    float ofy = (float)(sy - fly);
+    if (ofy >= 1.0f) { ofy = 0.0f; j += 1; }
+    j = min(max(j, 0), a.ncy - 1);
+    a.oy[p] = ofy;
+  }
+  /* velocities: Box-Muller on (r1.xy, r1.zw) and (r2.xy, r2.zw) */
+  double u1 = isk_u53(r1.x, r1.y), u2 = isk_u53(r1.z, r1.w);
+  double u3 = isk_u53(r2.x, r2.y), u4 = isk_u53(r2.z, r2.w);
+  double rad1 = sqrt(-2.0 * isk_log(u1)), rad2 = sqrt(-2.0 * isk_log(u3));
+  double s1, c1, s2, c2;
+  isk_sincos2pi(u2, &s1, &c1);
+  isk_sincos2pi(u4, &s2, &c2);
+  a.cell[p] = i | (j << 16);
+  a.ox[p] = of;
+  a.vx[p] = (float)(rad1 * c1 * a.wv[0] + a.dv[0]);
+  a.vy[p] = (float)(rad1 * s1 * a.wv[1] + a.dv[1]);
+  a.vz[p] = (float)(rad2 * c2 * a.wv[2] + a.dv[2]);
+  if (a.id) a.id[p] = (uint32_t)(a.hdr[HDR_NEXT_ID] + k);
+}
+
+__global__ void k_sample_commit(long long *hdr, long long n) {
+  hdr[HDR_NP] += n;
+  hdr[HDR_NEXT_ID] += n;
+}
+
+int isk_sample_maxwellian(iskra_ctx *c, int sid, int64_t n, const double *wx,
+                          const double *dx, const double *wv, const double *dv,
+                          uint64_t seed, uint32_t stream) {
+  IskSpecies *s = &c->sp[sid];
+  long long hdr[HDR_WORDS];
+  ISK_HIP(hipMemcpyAsync(hdr, s->hdr, sizeof(hdr), hipMemcpyDeviceToHost, c->stream));
+  ISK_HIP(hipStreamSynchronize(c->stream));
+  int64_t room = s->cap - hdr[HDR_NP];
+  if (n > room) n = room; /* sample! truncates to the free capacity (sources.jl:30) */
+  if (n <= 0) return ISKRA_OK;
+  SampleArgs a;
+  int b = s->cur;
+  a.cell = s->cell[b];
+  a.ox = s->ox[b];
+  a.oy = s->oy[b];
+  a.vx = s->vx[b];
+  a.vy = s->vy[b];
+  a.vz = s->vz[b];
+  a.id = s->track_ids ? s->id[b] : nullptr;
+  a.hdr = s->hdr;
+  a.n = n;
+  for (int d = 0; d < 2; ++d) {
+    a.wx[d] = d < c->g.dim ? wx[d] : 0.0;
+    a.dxo[d] = d < c->g.dim ? (dx ? dx[d] : 0.0) : 0.0;
+  }
+  for (int d = 0; d < 3; ++d) {
+    a.wv[d] = wv[d];
+    a.dv[d] = dv ? dv[d] : 0.0;
+  }
+  a.x0 = c->g.x0;
+  a.y0 = c->g.y0;
+  a.dx = c->g.dx;
+  a.dy = c->g.dy;
+  a.ncx = c->g.ncx;
+  a.ncy = c->g.ncy;
+  a.dim = c->g.dim;
+  a.k0 = (uint32_t)seed;
+  a.k1 = (uint32_t)(seed >> 32);
+  a.stream = stream;
+  int blocks = (int)((n + 255) / 256);
+  hipLaunchKernelGGL(k_sample, dim3(blocks), dim3(256), 0, c->stream, a);
+  ISK_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_sample_commit, dim3(1), dim3(1), 0, c->stream, s->hdr,
+                     (long long)n);
+  ISK_CHECK_LAUNCH();
+  return isk_sort_species(c, sid, 0);
+}
+
+/* ------------------------------------------------------------------------ */
+/* counting sort                                                             */
+/* ------------------------------------------------------------------------ */
+struct KeyDesc {
+  int mode; /* 0: tile, 1: cell inside tile */
+  int dim, T, log2T, ntx, nbins;
+};
+
+__device__ __forceinline__ int sort_key(int c, const KeyDesc &k) {
+  if (c < 0) return k.nbins; /* discarded -> last bin */
+  int i = c & 0xFFFF;
+  int j = c >> 16;
+  if (k.mode == 0) {
+    return (i >> k.log2T) + k.ntx * (j >> k.log2T);
+  }
+  return (i & (k.T - 1)) + (k.dim == 2 ? ((j & (k.T - 1)) << k.log2T) : 0);
+}
+
+__global__ __launch_bounds__(SORT_BLOCK) void k_sort_hist(const int32_t *cell,
+                                                         const long long *hdr,
+                                                         int *H, KeyDesc kd) {
+  extern __shared__ int sh[];
+  const int nb1 = kd.nbins + 1;
+  for (int b = threadIdx.x; b < nb1; b += SORT_BLOCK) sh[b] = 0;
+  __syncthreads();
+  const long long np = hdr[HDR_NP];
+  const long long base = (long long)blockIdx.x * SORT_CHUNK;
+  const int lane = threadIdx.x & 63;
+  for (int r = 0; r < SORT_PT; ++r) {
+    long long p = base + (long long)r * SORT_BLOCK + threadIdx.x;
+    bool valid = p < np;
+    int key = valid ? sort_key(cell[p], kd) : -1;
+    unsigned long long todo = __ballot(valid);
+    while (todo) {
+      int leader = __ffsll((long long)todo) - 1;
+      int lk = __shfl(key, leader);
+      unsigned long long m = __ballot(valid && key == lk);
+      if (lane == leader) atomicAdd(&sh[lk], (int)__popcll(m));
+      todo &= ~m;
+    }
+  }
+  __syncthreads();
+  int *Hb = H + (size_t)blockIdx.x * nb1;
+  for (int b = threadIdx.x; b < nb1; b += SORT_BLOCK) Hb[b] = sh[b];
+}
+
+/* per (bin, segment of SORT_SEG blocks): exclusive prefix over the blocks of
+ * the segment, segment totals into S */
+__global__ void k_sort_colscan(int *H, int *S, int nb1, int nblk) {
+  int b = blockIdx.x * blockDim.x + threadIdx.x;
+  int seg = blockIdx.y;
+  if (b >= nb1) return;
+  int run = 0;
+  int end = min((seg + 1) * SORT_SEG, nblk);
+  for (int k = seg * SORT_SEG; k < end; ++k) {
+    size_t at = (size_t)k * nb1 + b;
+    int v = H[at];
+    H[at] = run;
+    run += v;
+  }
+  S[(size_t)seg * nb1 + b] = run;
+}
+
+/* per bin: exclusive prefix over segments, bin totals */
+__global__ void k_sort_segscan(int *S, int *tot, int nb1, int nseg) {
+  int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb1) return;
+  int run = 0;
+  for (int s = 0; s < nseg; ++s) {
+    size_t at = (size_t)s * nb1 + b;
+    int v = S[at];
+    S[at] = run;
+    run += v;
+  }
+  tot[b] = run;
+}
+
+/* exclusive scan of the bin totals (single work-group of 1024 threads) */
+__global__ __launch_bounds__(1024) void k_sort_binscan(const int *tot, int *start,
+                                                      int nb1) {
+  __shared__ int sh[1024];
+  int per = (nb1 + 1023) / 1024;
+  int lo = threadIdx.x * per;
+  int hi = min(lo + per, nb1);
+  int sum = 0;
+  for (int b = lo; b < hi; ++b) sum += tot[b];
+  sh[threadIdx.x] = sum;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    int v = threadIdx.x >= off ? sh[threadIdx.x - off] : 0;
+    __syncthreads();
+    sh[threadIdx.x] += v;
+    __syncthreads();
+  }
+  int run = sh[threadIdx.x] - sum;
+  for (int b = lo; b < hi; ++b) {
+    start[b] = run;
+    run += tot[b];
+  }
+  if (threadIdx.x == 1023) start[nb1] = sh[1023];
+}
+
+struct ScatterArgs {
+  const int32_t *cell;
+  const float *ox, *oy, *vx, *vy, *vz;
+  const uint32_t *id;
+  int32_t *cell_o;
+  float *ox_o, *oy_o, *vx_o, *vy_o, *vz_o;
+  uint32_t *id_o;
+  const long long *hdr;
+  const int *H, *S, *start;
+  int nblk;
+  KeyDesc kd;
+};
+
+/* stable scatter: particles of a block are ranked in index order; within a
+ * round of 256, lanes of a wave are matched by key with ballots and waves are
+ * ordered through LDS group lists. */
+__global__ __launch_bounds__(SORT_BLOCK) void k_sort_scatter(ScatterArgs a) {
+  extern __shared__ int base[];
+  __shared__ int gkey[4][64];
+  __shared__ int gcnt[4][64];
+  __shared__ int ng[4];
+  const int nb1 = a.kd.nbins + 1;
+  const int blk = blockIdx.x;
+  const int seg = blk / SORT_SEG;
+  for (int b = threadIdx.x; b < nb1; b += SORT_BLOCK)
+    base[b] = a.start[b] + a.S[(size_t)seg * nb1 + b] + a.H[(size_t)blk * nb1 + b];
+  __syncthreads();
+  const long long np = a.hdr[HDR_NP];
+  const long long b0 = (long long)blk * SORT_CHUNK;
+  const int lane = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6;
+  for (int r = 0; r < SORT_PT; ++r) {
+    long long p = b0 + (long long)r * SORT_BLOCK + threadIdx.x;
+    bool valid = p < np;
+    int c = valid ? a.cell[p] : -1;
+    int key = valid ? sort_key(c, a.kd) : -1;
+    unsigned long long todo = __ballot(valid);
+    int myrank = 0;
+    int ngr = 0;
+    while (todo) {
+      int leader = __ffsll((long long)todo) - 1;
+      int lk = __shfl(key, leader);
+      unsigned long long m = __ballot(valid && key == lk);
+      if (valid && key == lk) myrank = __popcll(m & ((1ull << lane) - 1ull));
+      if (lane == 0) {
+        gkey[w][ngr] = lk;
+        gcnt[w][ngr] = __popcll(m);
+      }
+      ++ngr;
+      todo &= ~m;
+    }
+    if (lane == 0) ng[w] = ngr;
+    __syncthreads();
+    if (valid && key < a.kd.nbins) {
+      int pre = 0;
+      for (int w2 = 0; w2 < w; ++w2) {
+        int n2 = ng[w2];
+        for (int g = 0; g < n2; ++g)
+          if (gkey[w2][g] == key) pre += gcnt[w2][g];
+      }
+      long long d = (long long)base[key] + pre + myrank;
+      a.cell_o[d] = c;
+      a.ox_o[d] = a.ox[p];
+      if (a.kd.dim == 2) a.oy_o[d] = a.oy[p];
+      a.vx_o[d] = a.vx[p];
+      a.vy_o[d] = a.vy[p];
+      a.vz_o[d] = a.vz[p];
+      if (a.id) a.id_o[d] = a.id[p];
+    }
+    __syncthreads();
+    if (lane < ng[w]) atomicAdd(&base[gkey[w][lane]], gcnt[w][lane]);
+    __syncthreads();
+  }
+}
+
+/* new particle count; with build != 0 also the chunk table of the tile sort */
+__global__ __launch_bounds__(1024) void k_sort_finish(
+    long long *hdr, const int *tot, const int *start, int nbins, int build,
+    int CH, IskChunk *chunks, int *tcb) {
+  __shared__ int sh[1024];
+  long long live = start[nbins];
+  if (!build) {
+    if (threadIdx.x == 0) {
+      hdr[HDR_NP] = live;
+      hdr[HDR_NP_SORTED] = 0;
+      hdr[HDR_NCHUNKS] = 0;
+    }
+    return;
+  }
+  int per = (nbins + 1023) / 1024;
+  int lo = threadIdx.x * per;
+  int hi = min(lo + per, nbins);
+  int sum = 0;
+  for (int t = lo; t < hi; ++t) sum += (tot[t] + CH - 1) / CH;
+  sh[threadIdx.x] = sum;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    int v = threadIdx.x >= off ? sh[threadIdx.x - off] : 0;
+    __syncthreads();
+    sh[threadIdx.x] += v;
+    __syncthreads();
+  }
+  int run = sh[threadIdx.x] - sum;
+  for (int t = lo; t < hi; ++t) {
+    tcb[t] = run;
+    int n = tot[t];
+    int st = start[t];
+    for (int k = 0; k * CH < n; ++k) {
+      IskChunk ch;
+      ch.tile = t;
+      ch.start = st + k * CH;
+      ch.count = min(CH, n - k * CH);
+      ch.pad = 0;
+      chunks[run++] = ch;
+    }
+  }
+  if (threadIdx.x == 1023) {
+    tcb[nbins] = sh[1023];
+    hdr[HDR_NP] = live;
+    hdr[HDR_NP_SORTED] = live;
+    hdr[HDR_NCHUNKS] = sh[1023];
+  }
+}
+
+static int sort_pass(iskra_ctx *c, int sid, int mode, int build) {
+  IskSpecies *s = &c->sp[sid];
+  KeyDesc kd;
+  kd.mode = mode;
+  kd.dim = c->g.dim;
+  kd.T = c->t.T;
+  kd.log2T = c->t.log2T;
+  kd.ntx = c->ntx;
+  kd.nbins = mode == 0 ? c->ntiles : (c->g.dim == 2 ? c->t.T * c->t.T : c->t.T);
+  int nb1 = kd.nbins + 1;
+  if (nb1 > ISK_MAX_BINS + 1) {
+    isk_set_error("sort: %d bins exceed %d", kd.nbins, ISK_MAX_BINS);
+    return ISKRA_E_UNSUPPORTED;
+  }
+  int nblk = (int)((s->cap + SORT_CHUNK - 1) / SORT_CHUNK);
+  if (nblk < 1) nblk = 1;
+  int nseg = (nblk + SORT_SEG - 1) / SORT_SEG;
+  int64_t need_H = (int64_t)nblk * nb1, need_S = (int64_t)nseg * nb1;
+  if (need_H > c->sort_H_words) {
+    isk_free(c->sort_H);
+    c->sort_H = nullptr;
+    if (isk_alloc((void **)&c->sort_H, need_H * 4)) return ISKRA_E_HIP;
+    c->sort_H_words = need_H;
+  }
+  if (need_S > c->sort_S_words) {
+    isk_free(c->sort_S);
+    isk_free(c->sort_tot);
+    isk_free(c->sort_start);
+    c->sort_S = c->sort_tot = c->sort_start = nullptr;
+    if (isk_alloc((void **)&c->sort_S, need_S * 4)) return ISKRA_E_HIP;
+    /* bin arrays sized for the largest bin count any pass can use */
+    if (isk_alloc((void **)&c->sort_tot, (int64_t)(ISK_MAX_BINS + 2) * 4)) return ISKRA_E_HIP;
+    if (isk_alloc((void **)&c->sort_start, (int64_t)(ISK_MAX_BINS + 2) * 4)) return ISKRA_E_HIP;
+    c->sort_S_words = need_S;
+  }
+  int b = s->cur, o = 1 - s->cur;
+  size_t shm = (size_t)nb1 * sizeof(int);
+  hipLaunchKernelGGL(k_sort_hist, dim3(nblk), dim3(SORT_BLOCK), shm, c->stream,
+                     s->cell[b], s->hdr, c->sort_H, kd);
+  ISK_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_sort_colscan, dim3((nb1 + 255) / 256, nseg), dim3(256), 0,
+                     c->stream, c->sort_H, c->sort_S, nb1, nblk);
+  ISK_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_sort_segscan, dim3((nb1 + 255) / 256), dim3(256), 0,
+                     c->stream, c->sort_S, c->sort_tot, nb1, nseg);
+  ISK_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_sort_binscan, dim3(1), dim3(1024), 0, c->stream,
+                     c->sort_tot, c->sort_start, nb1);
+  ISK_CHECK_LAUNCH();
+  ScatterArgs a;
+  a.cell = s->cell[b];
+  a.ox = s->ox[b];
+  a.oy = s->oy[b];
+  a.vx = s->vx[b];
+  a.vy = s->vy[b];
+  a.vz = s->vz[b];
+  a.id = s->track_ids ? s->id[b] : nullptr;
+  a.cell_o = s->cell[o];
+  a.ox_o = s->ox[o];
+  a.oy_o = s->oy[o];
+  a.vx_o = s->vx[o];
+  a.vy_o = s->vy[o];
+  a.vz_o = s->vz[o];
+  a.id_o = s->track_ids ? s->id[o] : nullptr;
+  a.hdr = s->hdr;
+  a.H = c->sort_H;
+  a.S = c->sort_S;
+  a.start = c->sort_start;
+  a.nblk = nblk;
+  a.kd = kd;
+  hipLaunchKernelGGL(k_sort_scatter, dim3(nblk), dim3(SORT_BLOCK), shm,
+                     c->stream, a);
+  ISK_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_sort_finish, dim3(1), dim3(1024), 0, c->stream, s->hdr,
+                     c->sort_tot, c->sort_start, kd.nbins, build, c->t.CH,
+                     s->chunks, s->tile_chunk_begin);
+  ISK_CHECK_LAUNCH();
+  s->cur = o;
+  return ISKRA_OK;
+}
+
+int isk_sort_species(iskra_ctx *c, int sid, int exact_cell) {
+  int rc;
+  if (exact_cell) {
+    if ((rc = sort_pass(c, sid, 1, 0))) return rc;
+  }
+  return sort_pass(c, sid, 0, 1);
+}
+
+/* ------------------------------------------------------------------------ */
+/* fused gather + push + boundary + deposit                                  */
+/* ------------------------------------------------------------------------ */
+struct PushArgs {
+  int32_t *cell;
+  float *ox, *oy, *vx, *vy;
+  long long *hdr;
+  const IskChunk *chunks;
+  long long *partial;
+  unsigned long long *gfall;
+  const float *E;
+  int nx, ny, ncx, ncy;
+  int T, log2T, M, W, CH, ntx;
+  float qmdt, dtdx, dtdy;
+  int bndx, bndy;
+  int do_push;
+};
+
+__device__ __forceinline__ void fix_weights2(float hx, float hy, long long *w00,
+                                             long long *w10, long long *w01,
+                                             long long *w11) {
+  double dx = (double)hx, dy = (double)hy;
+  long long Wx = __double2ll_rn(dx * ISK_FIX_ONE);
+  long long Wy = __double2ll_rn(dy * ISK_FIX_ONE);
+  long long Wxy = __double2ll_rn((dx * dy) * ISK_FIX_ONE);
+  *w11 = Wxy;
+  *w10 = Wx - Wxy;
+  *w01 = Wy - Wxy;
+  *w00 = (1LL << 32) - Wx - Wy + Wxy;
+}
+
+/* one step of a coordinate: o += v*dtd, split off the integer part */
+__device__ __forceinline__ int advance_coord(float v, float dtd, float *o) {
+  float h = fmaf(v, dtd, *o);
+  float f = floorf(h);
+  h = h - f;
+  if (h >= 1.0f) {
+    h = 0.0f;
+    f += 1.0f;
+  }
+  *o = h;
+  return (int)f;
+}
+
+/* boundary on one axis; returns false if the particle is discarded */
+__device__ __forceinline__ bool apply_bnd(int *i, int n, int mode) {
+  if (*i >= 0 && *i < n) return true;
+  if (mode == ISKRA_BND_WRAP) {
+    int r = *i % n;
+    if (r < 0) r += n;
+    *i = r;
+    return true;
+  }
+  return false; /* discard (NONE also removes particles that leave the grid) */
+}
+
+template <int DIM>
+__global__ __launch_bounds__(PUSH_BLOCK) void k_push(PushArgs a) {
+  extern __shared__ unsigned long long sD[]; /* WW deposit counters, then E */
+  const int W = a.W;
+  const int WW = DIM == 2 ? W * W : W;
+  float *sE = reinterpret_cast<float *>(sD + WW);
+  const long long nch = a.hdr[HDR_NCHUNKS];
+  const long long np = a.hdr[HDR_NP];
+  const long long nps = a.hdr[HDR_NP_SORTED];
+  long long start, count;
+  bool windowed;
+  int wx0 = 0, wy0 = 0;
+  if ((long long)blockIdx.x < nch) {
+    IskChunk ch = a.chunks[blockIdx.x];
+    start = ch.start;
+    count = ch.count;
+    windowed = true;
+    if (DIM == 2) {
+      int ti = ch.tile % a.ntx, tj = ch.tile / a.ntx;
+      wx0 = ti * a.T - a.M;
+      wy0 = tj * a.T - a.M;
+    } else {
+      wx0 = ch.tile * a.T - a.M;
+    }
+  } else {
+    start = nps + ((long long)blockIdx.x - nch) * a.CH;
+    if (start >= np) return;
+    count = min((long long)a.CH, np - start);
+    windowed = false;
+  }
+  if (windowed) {
+    for (int k = threadIdx.x; k < WW; k += PUSH_BLOCK) {
+      sD[k] = 0ull;
+      if (a.do_push) {
+        int li = DIM == 2 ? k % W : k;
+        int lj = DIM == 2 ? k / W : 0;
+        int gi = wx0 + li, gj = wy0 + lj;
+        bool in = gi >= 0 && gi < a.nx && (DIM == 1 || (gj >= 0 && gj < a.ny));
+        if (DIM == 2) {
+          float2 e = in ? reinterpret_cast<const float2 *>(a.E)[gj * a.nx + gi]
+                        : make_float2(0.f, 0.f);
+          sE[2 * k] = e.x;
+          sE[2 * k + 1] = e.y;
+        } else {
+          sE[k] = in ? a.E[gi] : 0.f;
+        }
+      }
+    }
+    __syncthreads();
+  }
+  int live = 0;
+  for (long long b0 = 0; b0 < count; b0 += PUSH_BLOCK * PUSH_UNROLL) {
+    int32_t c[PUSH_UNROLL];
+    float hx[PUSH_UNROLL], hy[PUSH_UNROLL], vx[PUSH_UNROLL], vy[PUSH_UNROLL];
+    long long pidx[PUSH_UNROLL];
+#pragma unroll
+    for (int u = 0; u < PUSH_UNROLL; ++u) {
+      long long k = b0 + u * PUSH_BLOCK + threadIdx.x;
+      pidx[u] = k < count ? start + k : -1;
+      if (pidx[u] >= 0) {
+        c[u] = a.cell[pidx[u]];
+        hx[u] = a.ox[pidx[u]];
+        vx[u] = a.vx[pidx[u]];
+        if (DIM == 2) {
+          hy[u] = a.oy[pidx[u]];
+          vy[u] = a.vy[pidx[u]];
+        } else {
+          hy[u] = 0.f;
+          vy[u] = 0.f;
+        }
+      } else {
+        c[u] = -1;
+        hx[u] = hy[u] = vx[u] = vy[u] = 0.f;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < PUSH_UNROLL; ++u) {
+      if (c[u] < 0) continue;
+      int i = c[u] & 0xFFFF;
+      int j = c[u] >> 16;
+      bool alive = true;
+      if (a.do_push) {
+        /* gather (cloud_in_cell.jl:20-36) */
+        float ax = 1.0f - hx[u];
+        float Ex, Ey = 0.f;
+        if (DIM == 2) {
+          float ay = 1.0f - hy[u];
+          float w00 = ax * ay, w10 = hx[u] * ay, w01 = ax * hy[u], w11 = hx[u] * hy[u];
+          float2 e00, e10, e01, e11;
+          int li = i - wx0, lj = j - wy0;
+          if (windowed && (unsigned)li < (unsigned)(W - 1) &&
+              (unsigned)lj < (unsigned)(W - 1)) {
+            int q = lj * W + li;
+            e00 = make_float2(sE[2 * q], sE[2 * q + 1]);
+            e10 = make_float2(sE[2 * q + 2], sE[2 * q + 3]);
+            e01 = make_float2(sE[2 * (q + W)], sE[2 * (q + W) + 1]);
+            e11 = make_float2(sE[2 * (q + W) + 2], sE[2 * (q + W) + 3]);
+          } else {
+            const float2 *E2 = reinterpret_cast<const float2 *>(a.E);
+            int q = j * a.nx + i;
+            e00 = E2[q];
+            e10 = E2[q + 1];
+            e01 = E2[q + a.nx];
+            e11 = E2[q + a.nx + 1];
+          }
+          Ex = fmaf(w11, e11.x, fmaf(w01, e01.x, fmaf(w10, e10.x, w00 * e00.x)));
+          Ey = fmaf(w11, e11.y, fmaf(w01, e01.y, fmaf(w10, e10.y, w00 * e00.y)));
+        } else {
+          float e0, e1;
+          int li = i - wx0;
+          if (windowed && (unsigned)li < (unsigned)(W - 1)) {
+            e0 = sE[li];
+            e1 = sE[li + 1];
+          } else {
+            e0 = a.E[i];
+            e1 = a.E[i + 1];
+          }
+          Ex = fmaf(hx[u], e1, ax * e0);
+        }
+        /* push with B = 0 (pushers.jl:37-50): v += (q/m) E dt, x += v dt */
+        vx[u] = fmaf(a.qmdt, Ex, vx[u]);
+        i += advance_coord(vx[u], a.dtdx, &hx[u]);
+        alive = apply_bnd(&i, a.ncx, a.bndx);
+        if (DIM == 2) {
+          vy[u] = fmaf(a.qmdt, Ey, vy[u]);
+          j += advance_coord(vy[u], a.dtdy, &hy[u]);
+          alive = apply_bnd(&j, a.ncy, a.bndy) && alive;
+        }
+      }
+      if (alive) {
+        ++live;
+        /* deposit (cloud_in_cell.jl:1-18) */
+        if (DIM == 2) {
+          long long w00, w10, w01, w11;
+          fix_weights2(hx[u], hy[u], &w00, &w10, &w01, &w11);
+          int li = i - wx0, lj = j - wy0;
+          if (windowed && (unsigned)li < (unsigned)(W - 1) &&
+              (unsigned)lj < (unsigned)(W - 1)) {
+            int q = lj * W + li;
+            atomicAdd(&sD[q], (unsigned long long)w00);
+            atomicAdd(&sD[q + 1], (unsigned long long)w10);
+            atomicAdd(&sD[q + W], (unsigned long long)w01);
+            atomicAdd(&sD[q + W + 1], (unsigned long long)w11);
+          } else {
+            int q = j * a.nx + i;
+            atomicAdd(&a.gfall[q], (unsigned long long)w00);
+            atomicAdd(&a.gfall[q + 1], (unsigned long long)w10);
+            atomicAdd(&a.gfall[q + a.nx], (unsigned long long)w01);
+            atomicAdd(&a.gfall[q + a.nx + 1], (unsigned long long)w11);
+          }
+        } else {
+          long long w1 = __double2ll_rn((double)hx[u] * ISK_FIX_ONE);
+          long long w0 = (1LL << 32) - w1;
+          int li = i - wx0;
+          if (windowed && (unsigned)li < (unsigned)(W - 1)) {
+            atomicAdd(&sD[li], (unsigned long long)w0);
+            atomicAdd(&sD[li + 1], (unsigned long long)w1);
+          } else {
+            atomicAdd(&a.gfall[i], (unsigned long long)w0);
+            atomicAdd(&a.gfall[i + 1], (unsigned long long)w1);
+          }
+        }
+      }
+      if (a.do_push) {
+        long long p = pidx[u];
+        a.cell[p] = alive ? (i | (j << 16)) : -1;
+        a.ox[p] = hx[u];
+        a.vx[p] = vx[u];
+        if (DIM == 2) {
+          a.oy[p] = hy[u];
+          a.vy[p] = vy[u];
+        }
+      }
+    }
+  }
+  /* live count (particle-steps) */
+  for (int off = 32; off > 0; off >>= 1) live += __shfl_xor(live, off);
+  if (a.do_push && (threadIdx.x & 63) == 0 && live) atomicAdd((unsigned long long *)&a.hdr[HDR_PUSHED], (unsigned long long)live);
+  if (windowed) {
+    __syncthreads();
+    long long *dst = a.partial + (size_t)blockIdx.x * WW;
+    for (int k = threadIdx.x; k < WW; k += PUSH_BLOCK) dst[k] = (long long)sD[k];
+  }
+}
+
+int isk_push_species(iskra_ctx *c, int sid, double dt, int do_push) {
+  IskSpecies *s = &c->sp[sid];
+  const IskGrid &g = c->g;
+  int rc = isk_fill_i64(c, s->gfall, isk_nodes(c), 0);
+  if (rc) return rc;
+  PushArgs a;
+  int b = s->cur;
+  a.cell = s->cell[b];
+  a.ox = s->ox[b];
+  a.oy = s->oy[b];
+  a.vx = s->vx[b];
+  a.vy = s->vy[b];
+  a.hdr = s->hdr;
+  a.chunks = s->chunks;
+  a.partial = s->partial;
+  a.gfall = s->gfall;
+  a.E = c->E;
+  a.nx = g.nx;
+  a.ny = g.ny;
+  a.ncx = g.ncx;
+  a.ncy = g.ncy;
+  a.T = c->t.T;
+  a.log2T = c->t.log2T;
+  a.M = c->t.M;
+  a.W = c->t.W;
+  a.CH = c->t.CH;
+  a.ntx = c->ntx;
+  a.qmdt = (float)(s->q / s->m * dt);
+  a.dtdx = (float)(dt / g.dx);
+  a.dtdy = (float)(dt / g.dy);
+  a.bndx = s->bnd[0];
+  a.bndy = s->bnd[1];
+  a.do_push = do_push;
+  int grid = (int)s->max_chunks;
+  int WW = g.dim == 2 ? a.W * a.W : a.W;
+  size_t shm = (size_t)WW * (8 + 4 * g.dim);
+  if (g.dim == 2)
+    hipLaunchKernelGGL(k_push<2>, dim3(grid), dim3(PUSH_BLOCK), shm, c->stream, a);
+  else
+    hipLaunchKernelGGL(k_push<1>, dim3(grid), dim3(PUSH_BLOCK), shm, c->stream, a);
+  ISK_CHECK_LAUNCH();
+  s->deposited = 1;
+  return isk_merge_species(c, sid);
+}
+
+/* ------------------------------------------------------------------------ */
+/* deterministic merge of the chunk windows                                  */
+/* ------------------------------------------------------------------------ */
+struct MergeArgs {
+  const long long *partial;
+  const unsigned long long *gfall;
+  const int *tcb;
+  const long long *hdr;
+  long long *cnt;
+  int nx, ny, T, M, W, ntx, nty;
+};
+
+template <int DIM>
+__global__ void k_merge(MergeArgs a) {
+  int node = blockIdx.x * blockDim.x + threadIdx.x;
+  int nn = a.nx * a.ny;
+  if (node >= nn) return;
+  int i = node % a.nx, j = node / a.nx;
+  long long s = (long long)a.gfall[node];
+  if (a.hdr[HDR_NCHUNKS] > 0) {
+    const int WW = DIM == 2 ? a.W * a.W : a.W;
+    const int back = 1 + (2 * a.M + a.T - 1) / a.T;
+    int tlo = (i + a.M) / a.T - back, thi = (i + a.M) / a.T;
+    int ulo = DIM == 2 ? (j + a.M) / a.T - back : 0, uhi = DIM == 2 ? (j + a.M) / a.T : 0;
+    for (int tj = max(ulo, 0); tj <= min(uhi, a.nty - 1); ++tj) {
+      int wy0 = tj * a.T - a.M;
+      int lj = j - wy0;
+      if (DIM == 2 && (lj < 0 || lj >= a.W)) continue;
+      for (int ti = max(tlo, 0); ti <= min(thi, a.ntx - 1); ++ti) {
+        int wx0 = ti * a.T - a.M;
+        int li = i - wx0;
+        if (li < 0 || li >= a.W) continue;
+        int t = ti + a.ntx * tj;
+        int q = DIM == 2 ? lj * a.W + li : li;
+        for (int k = a.tcb[t]; k < a.tcb[t + 1]; ++k) s += a.partial[(size_t)k * WW + q];
+      }
+    }
+  }
+  a.cnt[node] = s;
+}
+
+int isk_merge_species(iskra_ctx *c, int sid) {
+  IskSpecies *s = &c->sp[sid];
+  MergeArgs a;
+  a.partial = s->partial;
+  a.gfall = s->gfall;
+  a.tcb = s->tile_chunk_begin;
+  a.hdr = s->hdr;
+  a.cnt = s->cnt;
+  a.nx = c->g.nx;
+  a.ny = c->g.ny;
+  a.T = c->t.T;
+  a.M = c->t.M;
+  a.W = c->t.W;
+  a.ntx = c->ntx;
+  a.nty = c->nty;
+  int nn = isk_nodes(c);
+  if (c->g.dim == 2)
+    hipLaunchKernelGGL(k_merge<2>, dim3((nn + 255) / 256), dim3(256), 0, c->stream, a);
+  else
+    hipLaunchKernelGGL(k_merge<1>, dim3((nn + 255) / 256), dim3(256), 0, c->stream, a);
+  ISK_CHECK_LAUNCH();
+  return ISKRA_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* rho = sum_s q_s n_s + background                                          */
+/* ------------------------------------------------------------------------ */
+struct RhoArgs {
+  const long long *cnt[ISK_MAX_SPECIES];
+  double qw[ISK_MAX_SPECIES]; /* w0 */
+  double q[ISK_MAX_SPECIES];
+  int ns;
+  const double *vol, *bg;
+  double *rho;
+  int nn;
+};
+
+__global__ void k_rho(RhoArgs a) {
+  int node = blockIdx.x * blockDim.x + threadIdx.x;
+  if (node >= a.nn) return;
+  double V = a.vol[node];
+  double r = 0.0;
+  for (int s = 0; s < a.ns; ++s) {
+    double n = ((double)a.cnt[s][node] * (1.0 / ISK_FIX_ONE)) * a.qw[s] / V;
+    r += a.q[s] * n;
+  }
+  a.rho[node] = r + a.bg[node];
+}
+
+int isk_rho(iskra_ctx *c) {
+  RhoArgs a;
+  int ns = 0;
+  for (int s = 0; s < c->nspecies; ++s) {
+    if (!c->sp[s].deposited) {
+      int rc = isk_push_species(c, s, 0.0, 0);
+      if (rc) return rc;
+    }
+    a.cnt[ns] = c->sp[s].cnt;
+    a.qw[ns] = c->sp[s].w0;
+    a.q[ns] = c->sp[s].q;
+    ++ns;
+  }
+  a.ns = ns;
+  a.vol = c->vol;
+  a.bg = c->bg;
+  a.rho = c->rho;
+  a.nn = isk_nodes(c);
+  hipLaunchKernelGGL(k_rho, dim3((a.nn + 255) / 256), dim3(256), 0, c->stream, a);
+  ISK_CHECK_LAUNCH();
+  return ISKRA_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* gather only (grid_to_particle), for inspection                           */
+/* ------------------------------------------------------------------------ */
+template <int DIM>
+__global__ void k_gather(const int32_t *cell, const float *ox, const float *oy,
+                         const long long *hdr, const float *E, int nx,
+                         float *out, long long cap) {
+  long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= hdr[HDR_NP]) return;
+  int cc = cell[p];
+  float Ex = 0.f, Ey = 0.f;
+  if (cc >= 0) {
+    int i = cc & 0xFFFF, j = cc >> 16;
+    float hx = ox[p];
+    float ax = 1.0f - hx;
+    if (DIM == 2) {
+      float hy = oy[p];
+      float ay = 1.0f - hy;
+      float w00 = ax * ay, w10 = hx * ay, w01 = ax * hy, w11 = hx * hy;
+      const float2 *E2 = reinterpret_cast<const float2 *>(E);
+      int q = j * nx + i;
+      float2 e00 = E2[q], e10 = E2[q + 1], e01 = E2[q + nx], e11 = E2[q + nx + 1];
+      Ex = fmaf(w11, e11.x, fmaf(w01, e01.x, fmaf(w10, e10.x, w00 * e00.x)));
+      Ey = fmaf(w11, e11.y, fmaf(w01, e01.y, fmaf(w10, e10.y, w00 * e00.y)));
+    } else {
+      Ex = fmaf(hx, E[i + 1], ax * E[i]);
+    }
+  }
+  out[p] = Ex;
+  out[cap + p] = Ey;
+}
+
+int isk_gather_species(iskra_ctx *c, int sid) {
+  IskSpecies *s = &c->sp[sid];
+  if (c->gather_cap < s->cap) {
+    isk_free(c->gather_out);
+    c->gather_out = nullptr;
+    if (isk_alloc((void **)&c->gather_out, (size_t)s->cap * 2 * sizeof(float))) return ISKRA_E_HIP;
+    c->gather_cap = s->cap;
+  }
+  int b = s->cur;
+  int blocks = (int)((s->cap + 255) / 256);
+  if (c->g.dim == 2)
+    hipLaunchKernelGGL(k_gather<2>, dim3(blocks), dim3(256), 0, c->stream, s->cell[b],
+                       s->ox[b], s->oy[b], s->hdr, c->E, c->g.nx, c->gather_out,
+                       (long long)s->cap);
+  else
+    hipLaunchKernelGGL(k_gather<1>, dim3(blocks), dim3(256), 0, c->stream, s->cell[b],
+                       s->ox[b], s->oy[b], s->hdr, c->E, c->g.nx, c->gather_out,
+                       (long long)s->cap);
+  ISK_CHECK_LAUNCH();
+  return ISKRA_OK;
+}

@@ -1,0 +1,490 @@
+"""CPU oracle of the electrostatic PIC step -- TEST INFRASTRUCTURE ONLY.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may import
+this package, and only as the checker; the product (iskra_amd/) never does.
+
+The reference (bchaber/iskra) is Julia, and no Julia toolchain exists in this
+environment, so the reference cannot be built or run: this oracle is a C
+restatement (oracle/oracle.c) of the reference's algorithms, read from its
+source, plus Python orchestration.
+
+How each oracle function is pinned (checks live in tests/):
+
+  ref_cell_volume        closed form (edge halving, corner quarter) and the
+                         restated RegularGrids.jl rule; uniform plasma gives
+                         uniform density including edges/corners.
+  ref_particle_to_grid   single-particle closed forms; charge conservation
+                         sum(n V) = sum(w); brute force on a hand table.
+  ref_grid_to_particle   exact for linear fields (CIC interpolates linears).
+  ref_push               closed form for B = 0 (v += qm E dt, x += v dt) and a
+                         uniform-E trajectory; Boris rotation preserves |v|
+                         when E = 0.
+  ref_wrap / ref_discard closed forms of fld; swap-remove order on tiny arrays
+                         enumerated by hand.
+  ref_poisson_* + LU     FiniteDifferenceMethod/test/runtests.jl parallel-plate
+                         capacitor (Ex = -1, Ey = 0), manufactured solutions,
+                         LU residual ~ 1e-15, agreement with numpy.linalg.solve.
+  ref_efield             closed form on linear / quadratic phi.
+  ref_sigma              interpolation nodes, midpoints, flat extrapolation.
+  ref_max_sigma_g        brute force over a dense energy grid (upper bound at
+                         nodes, as the reference defines it).
+  orc_philox4x32         Random123 known-answer vectors; structure cross-checked
+                         through orc_philox4x64 against numpy.random.Philox.
+  orc_exp/log/sincos2pi  libm within a few ulp over dense sweeps.
+  mir_*                  against the ref_* functions within FP32 tolerance
+                         (tests/test_oracle.py), and bit-exactly against the GPU.
+
+parity unpinned: none of the above can be checked against the reference's own
+run (Julia absent).  The reference's MersenneTwister streams, LAPACK round-off
+and Interpolations.jl round-off are not reproducible here; the MCC
+candidate-selection rule is pinned statistically (expected event counts), not
+bitwise, against the reference rule restated in ref_mcc_literal().
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "liboracle.so")
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBPATH):
+            from . import build as _b  # noqa
+            _b.build()
+        _lib = C.CDLL(LIBPATH)
+        _declare(_lib)
+    return _lib
+
+
+P = C.c_void_p
+I32, I64, U32, U64, D, F = C.c_int32, C.c_int64, C.c_uint32, C.c_uint64, C.c_double, C.c_float
+
+
+class MirMcc(C.Structure):
+    _fields_ = [("ncoll", C.c_int), ("types", I32 * 8), ("energy", D * 8), ("off", P),
+                ("eps", P), ("sig", P), ("candP", D), ("dt", D), ("m_eV", D), ("ms", D),
+                ("mt", D), ("vth_t", D), ("tqm", D), ("k0", U32), ("k1", U32), ("step", U32),
+                ("stream", U32), ("dim", C.c_int), ("nx", C.c_int)]
+
+
+def _declare(L):
+    sig = {
+        "orc_philox4x32": (None, [P, P, P]),
+        "orc_philox4x64": (None, [P, P, P]),
+        "orc_u53": (D, [U32, U32]),
+        "orc_exp": (D, [D]),
+        "orc_log": (D, [D]),
+        "orc_sincos2pi": (None, [D, P, P]),
+        "ref_cell_volume": (None, [C.c_int, C.c_int, C.c_int, D, D, P, P]),
+        "ref_particle_to_grid": (None, [C.c_int, I64, P, P, C.c_int, C.c_int, D, D, P]),
+        "ref_grid_to_particle": (None, [C.c_int, I64, P, C.c_int, C.c_int, D, D, P, P]),
+        "ref_push": (None, [I64, C.c_int, P, P, P, P, D, D]),
+        "ref_wrap": (None, [I64, C.c_int, P, P, P, P, C.c_int, P]),
+        "ref_discard": (I64, [P, I64, C.c_int, P, P, P, P, D, P, P, P, C.c_int, P]),
+        "ref_poisson_assemble": (None, [C.c_int, C.c_int, D, P]),
+        "ref_apply_periodic_1d": (None, [C.c_int, D, P]),
+        "ref_apply_periodic_2d": (None, [C.c_int, C.c_int, D, D, C.c_int, P]),
+        "ref_apply_dirichlet": (None, [C.c_int, P, D, P, P, P]),
+        "ref_lu_solve": (C.c_int, [C.c_int, P, P, P]),
+        "ref_efield": (None, [C.c_int, C.c_int, C.c_int, D, D, P, P]),
+        "ref_sigma": (D, [C.c_int, P, P, D]),
+        "ref_max_sigma_g": (D, [C.c_int, P, P, P, D, C.c_int, P]),
+        "mir_gather": (None, [C.c_int, I64, P, P, P, P, C.c_int, P, P]),
+        "mir_push_deposit": (I64, [C.c_int, I64, P, P, P, P, P, P, C.c_int, C.c_int, C.c_int,
+                                   F, F, F, C.c_int, C.c_int, C.c_int, P]),
+        "mir_rho": (None, [C.c_int, C.c_int, P, P, P, P, P, P]),
+        "mir_tri_factor": (C.c_int, [C.c_int, P, C.c_int, P, P, P]),
+        "mir_tri_solve": (None, [C.c_int, P, P, P, P, P, P, D, C.c_int, P]),
+        "mir_efield": (None, [C.c_int, C.c_int, C.c_int, D, D, P, P]),
+        "mir_mcc": (I64, [C.POINTER(MirMcc), I64, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P,
+                          P]),
+        "mir_sample": (None, [C.c_int, I64, I64, D, D, D, D, C.c_int, C.c_int, P, P, P, P, U64,
+                              U32, P, P, P, P, P, P, P]),
+        "mir_sort_tile": (I64, [C.c_int, I64, C.c_int, C.c_int, P, P, P, P, P, P, P]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+
+
+def ptr(a):
+    """ctypes pointer of a C-contiguous numpy array (or None)."""
+    if a is None:
+        return None
+    assert a.flags["C_CONTIGUOUS"], "array must be contiguous"
+    return a.ctypes.data_as(C.c_void_p)
+
+
+# --------------------------------------------------------------------------
+# RNG helpers
+# --------------------------------------------------------------------------
+def philox4x32(ctr, key):
+    c = np.asarray(ctr, dtype=np.uint32)
+    k = np.asarray(key, dtype=np.uint32)
+    o = np.zeros(4, dtype=np.uint32)
+    lib().orc_philox4x32(ptr(c), ptr(k), ptr(o))
+    return o
+
+
+def philox4x64(ctr, key):
+    c = np.asarray(ctr, dtype=np.uint64)
+    k = np.asarray(key, dtype=np.uint64)
+    o = np.zeros(4, dtype=np.uint64)
+    lib().orc_philox4x64(ptr(c), ptr(k), ptr(o))
+    return o
+
+
+def u53(a, b):
+    return lib().orc_u53(int(a), int(b))
+
+
+# --------------------------------------------------------------------------
+# reference-mode helpers (FP64, reference layouts)
+# --------------------------------------------------------------------------
+def cell_volume(dim, nx, ny, dx, dy, volper=(0, 0, 0, 0)):
+    V = np.zeros(nx * ny)
+    vp = np.asarray(volper, dtype=np.int32)
+    lib().ref_cell_volume(dim, nx, ny, dx, dy, ptr(vp), ptr(V))
+    return V
+
+
+def particle_to_grid(dim, x, wg, nx, ny, dx, dy):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    np_ = len(wg)
+    u = np.zeros(nx * ny)
+    lib().ref_particle_to_grid(dim, np_, ptr(x), ptr(np.ascontiguousarray(wg, dtype=np.float64)),
+                               nx, ny, dx, dy, ptr(u))
+    return u
+
+
+def grid_to_particle(dim, x, nx, ny, dx, dy, E):
+    """x: flat column-major np x D; E: flat column-major nx x ny x 3."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    np_ = len(x) // dim
+    pE = np.zeros(3 * np_)
+    lib().ref_grid_to_particle(dim, np_, ptr(x), nx, ny, dx, dy,
+                               ptr(np.ascontiguousarray(E, dtype=np.float64)), ptr(pE))
+    return pE
+
+
+def poisson_matrix(dim, nx, ny, dx, dy, periodic=(), dirichlet=()):
+    """Assemble the reference's dense A and b (generalized_poisson.jl).
+    periodic: list of axes as apply_periodic takes them (1-D: [1]);
+    dirichlet: list of (mask nodes bool array, value).  Returns A, b, isdof."""
+    nn = nx * ny
+    A = np.zeros(nn * nn)
+    lib().ref_poisson_assemble(nx, ny, dx, ptr(A))
+    for ax in periodic:
+        if dim == 1:
+            lib().ref_apply_periodic_1d(nx, dx, ptr(A))
+        else:
+            lib().ref_apply_periodic_2d(nx, ny, dx, dy, ax, ptr(A))
+    b = np.zeros(nn)
+    isdof = np.ones(nn, dtype=np.uint8)
+    for mask, val in dirichlet:
+        m = np.ascontiguousarray(mask, dtype=np.uint8).ravel(order="F")
+        lib().ref_apply_dirichlet(nn, ptr(m), float(val), ptr(A), ptr(b), ptr(isdof))
+    return A, b, isdof
+
+
+def lu_solve(A, b):
+    n = len(b)
+    x = np.zeros(n)
+    sing = lib().ref_lu_solve(n, ptr(np.ascontiguousarray(A)), ptr(np.ascontiguousarray(b)), ptr(x))
+    return x, sing
+
+
+def potential(A, b, isdof, rho, eps0):
+    """calculate_electric_potential(ps, -rho): b[dofs] = -rho/eps0, x = A\\b."""
+    bb = b.copy()
+    bb[isdof == 1] = -rho[isdof == 1] / eps0
+    return lu_solve(A, bb)
+
+
+def efield(dim, nx, ny, dx, dy, phi):
+    E = np.zeros(nx * ny * 3)
+    lib().ref_efield(dim, nx, ny, dx, dy, ptr(np.ascontiguousarray(phi, dtype=np.float64)), ptr(E))
+    return E
+
+
+def sigma(eps, sig, e):
+    eps = np.ascontiguousarray(eps, dtype=np.float64)
+    sig = np.ascontiguousarray(sig, dtype=np.float64)
+    return lib().ref_sigma(len(eps), ptr(eps), ptr(sig), float(e))
+
+
+def max_sigma_g(tables, m_eV):
+    off = [0]
+    eps_all, sig_all = [], []
+    for e, s in tables:
+        eps_all += list(e)
+        sig_all += list(s)
+        off.append(len(eps_all))
+    off = np.asarray(off, dtype=np.int32)
+    eps_all = np.asarray(eps_all, dtype=np.float64)
+    sig_all = np.asarray(sig_all, dtype=np.float64)
+    work = np.zeros(len(eps_all) + 1)
+    return lib().ref_max_sigma_g(len(tables), ptr(off), ptr(eps_all), ptr(sig_all), m_eV,
+                                 len(work), ptr(work))
+
+
+def ref_push(x, v, E, B, qm, dt, D):
+    np_ = len(v) // 3
+    lib().ref_push(np_, D, ptr(x), ptr(v), ptr(E), ptr(B), qm, dt)
+
+
+def ref_mcc_literal(rng, np_, candP_per_particle_fn, N, max_P, remainder=0.0):
+    """The reference's candidate rule (mcc.jl:248-251): Nc = floor(N max_P np +
+    remainder) picks by index with replacement.  Returns (picks, remainder)."""
+    val = N * max_P * np_ + remainder
+    Nc = int(np.floor(val))
+    rem = val - Nc
+    picks = rng.integers(0, np_, size=Nc)
+    return picks, rem
+
+
+# --------------------------------------------------------------------------
+# mirror mode: a CPU twin of the GPU step (1-D and 2-D operators; full 1-D loop)
+# --------------------------------------------------------------------------
+QE_MCC = 1.60217646e-19
+KB_MCC = 1.3806503e-23
+
+
+class MirSpecies:
+    def __init__(self, dim, cap, q, m, w0, track_ids=True, bnd=(1, 1)):
+        self.dim, self.cap, self.q, self.m, self.w0 = dim, cap, q, m, w0
+        self.cell = np.zeros(cap, dtype=np.int32)
+        self.ox = np.zeros(cap, dtype=np.float32)
+        self.oy = np.zeros(cap, dtype=np.float32)
+        self.vx = np.zeros(cap, dtype=np.float32)
+        self.vy = np.zeros(cap, dtype=np.float32)
+        self.vz = np.zeros(cap, dtype=np.float32)
+        self.id = np.zeros(cap, dtype=np.uint32)
+        self.np = 0
+        self.next_id = 0
+        self.track_ids = track_ids
+        self.bnd = bnd
+        self.cnt = None
+
+    def set_state(self, cell, ox, oy, vx, vy, vz, ids):
+        n = len(cell)
+        self.cell[:n] = cell
+        self.ox[:n] = ox
+        if oy is not None:
+            self.oy[:n] = oy
+        self.vx[:n] = vx
+        self.vy[:n] = vy
+        self.vz[:n] = vz
+        if ids is not None:
+            self.id[:n] = ids
+            self.next_id = int(ids.max()) + 1 if n else 0
+        else:
+            self.id[:n] = np.arange(n, dtype=np.uint32)
+            self.next_id = n
+        self.np = n
+
+    def live(self):
+        n = self.np
+        m = self.cell[:n] >= 0
+        return m
+
+
+class MirSim:
+    """CPU twin of iskra_step for grids whose Poisson solve is the 1-D direct
+    solver: same number format, same operation order, same particle order
+    (stable tile sort every sort_every steps, products appended in parent
+    order), same Philox draws."""
+
+    def __init__(self, nx, dx, x0=0.0, volper=(0, 0, 0, 0), T=64, sort_every=4, seed=0):
+        self.dim, self.nx, self.ny = 1, nx, 1
+        self.ncx, self.ncy = nx - 1, 1
+        self.dx, self.dy, self.x0 = dx, 1.0, x0
+        self.vol = cell_volume(1, nx, 1, dx, 1.0, volper)
+        self.T, self.sort_every = T, sort_every
+        self.ntx = (self.ncx + T - 1) // T
+        self.species = []
+        self.bg = np.zeros(nx)
+        self.rho = np.zeros(nx)
+        self.phi = np.zeros(nx)
+        self.E = np.zeros(nx, dtype=np.float32)
+        self.seed = seed
+        self.step_count = 0
+        self.mccs = []
+        self.tags = np.zeros(nx, dtype=np.int8)
+        self.tagval = np.zeros(17)
+        self.wf = {}
+        self.per_x = 0
+        self.eps0 = 1.0
+        self.pushed = 0
+
+    def add_species(self, sp):
+        self.species.append(sp)
+        return len(self.species) - 1
+
+    def set_poisson(self, eps0, tags, per_x):
+        self.eps0 = eps0
+        self.tags = np.ascontiguousarray(tags, dtype=np.int8)
+        self.per_x = per_x
+        self.singular = int(per_x and not np.any(self.tags))
+        n = self.nx
+        self.a, self.cp, self.inv = np.zeros(n), np.zeros(n), np.zeros(n)
+        rc = lib().mir_tri_factor(n, ptr(self.tags), self.singular, ptr(self.a), ptr(self.cp),
+                                  ptr(self.inv))
+        assert rc == 0
+
+    def add_mcc(self, src, ion, types, energy, tables, target_m, target_q, target_T, tn):
+        off = [0]
+        eps_all, sig_all = [], []
+        for e, s in tables:
+            eps_all += list(e)
+            sig_all += list(s)
+            off.append(len(eps_all))
+        m = dict(src=src, ion=ion, types=list(types), energy=list(energy),
+                 off=np.asarray(off, dtype=np.int32), eps=np.asarray(eps_all, dtype=np.float64),
+                 sig=np.asarray(sig_all, dtype=np.float64), target_m=target_m,
+                 target_q=target_q, target_T=target_T,
+                 tn=np.ascontiguousarray(tn, dtype=np.float64), step=0,
+                 stream=0x4D430000 + len(self.mccs))
+        sp = self.species[src]
+        m["m_eV"] = sp.m / QE_MCC
+        m["max_sg"] = max_sigma_g(tables, m["m_eV"])
+        m["nu"] = np.zeros(self.nx * self.ny * len(types), dtype=np.int64)
+        m["events"] = np.zeros(8, dtype=np.int64)
+        self.mccs.append(m)
+        return len(self.mccs) - 1
+
+    def mcc_run(self, k, dt):
+        m = self.mccs[k]
+        sp = self.species[m["src"]]
+        N = len(m["types"])
+        maxP = 1.0 - lib().orc_exp(-float(m["tn"].max()) * m["max_sg"] * dt)
+        a = MirMcc()
+        a.ncoll = N
+        for q in range(N):
+            a.types[q] = m["types"][q]
+            a.energy[q] = m["energy"][q]
+        a.off, a.eps, a.sig = ptr(m["off"]), ptr(m["eps"]), ptr(m["sig"])
+        a.candP = N * maxP
+        a.dt = dt
+        a.m_eV = m["m_eV"]
+        a.ms = sp.m
+        a.mt = m["target_m"]
+        a.vth_t = np.sqrt(2.0 * KB_MCC * m["target_T"] / m["target_m"])
+        a.tqm = m["target_q"] / m["target_m"]
+        a.k0 = self.seed & 0xFFFFFFFF
+        a.k1 = (self.seed >> 32) & 0xFFFFFFFF
+        a.step = m["step"]
+        a.stream = m["stream"]
+        a.dim = self.dim
+        a.nx = self.nx
+        n = sp.np
+        pcell = np.zeros(max(n, 1), dtype=np.int32)
+        pox = np.zeros(max(n, 1), dtype=np.float32)
+        poy = np.zeros(max(n, 1), dtype=np.float32)
+        pe = np.zeros(3 * max(n, 1), dtype=np.float32)
+        pi = np.zeros(3 * max(n, 1), dtype=np.float32)
+        ev = np.zeros(8, dtype=np.int64)
+        nprod = lib().mir_mcc(C.byref(a), n, ptr(sp.cell), ptr(sp.ox), ptr(sp.oy), ptr(sp.vx),
+                              ptr(sp.vy), ptr(sp.vz), ptr(sp.id), ptr(self.E), ptr(m["tn"]),
+                              ptr(m["nu"]), ptr(ev), ptr(pcell), ptr(pox), ptr(poy), ptr(pe),
+                              ptr(pi))
+        if nprod < 0:
+            raise RuntimeError("mirror mcc: probability above 1")
+        m["events"] += ev
+        m["last_events"] = ev[:N].copy()
+        m["step"] += 1
+        if nprod:
+            R = 0
+            if m["ion"] is not None and m["ion"] >= 0:
+                R = int(np.rint(sp.w0 / self.species[m["ion"]].w0))
+            # electrons appended to the source species
+            s0 = sp.np
+            k = min(nprod, sp.cap - s0)
+            sp.cell[s0:s0 + k] = pcell[:k]
+            sp.ox[s0:s0 + k] = pox[:k]
+            sp.oy[s0:s0 + k] = poy[:k]
+            sp.vx[s0:s0 + k] = pe[0:3 * k:3]
+            sp.vy[s0:s0 + k] = pe[1:3 * k:3]
+            sp.vz[s0:s0 + k] = pe[2:3 * k:3]
+            sp.id[s0:s0 + k] = (sp.next_id + np.arange(k)).astype(np.uint32)
+            sp.np += k
+            sp.next_id += nprod
+            if R > 0:
+                ion = self.species[m["ion"]]
+                i0 = ion.np
+                g = np.repeat(np.arange(nprod), R)
+                kk = min(len(g), ion.cap - i0)
+                g = g[:kk]
+                ion.cell[i0:i0 + kk] = pcell[g]
+                ion.ox[i0:i0 + kk] = pox[g]
+                ion.oy[i0:i0 + kk] = poy[g]
+                ion.vx[i0:i0 + kk] = pi[3 * g]
+                ion.vy[i0:i0 + kk] = pi[3 * g + 1]
+                ion.vz[i0:i0 + kk] = pi[3 * g + 2]
+                ion.id[i0:i0 + kk] = (ion.next_id + np.arange(kk)).astype(np.uint32)
+                ion.np += kk
+                ion.next_id += nprod * R
+        return m["last_events"]
+
+    def push(self, s, dt, do_push=1):
+        sp = self.species[s]
+        cnt = np.zeros(self.nx * self.ny, dtype=np.int64)
+        qmdt = np.float32(sp.q / sp.m * dt)
+        live = lib().mir_push_deposit(self.dim, sp.np, ptr(sp.cell), ptr(sp.ox), ptr(sp.oy),
+                                      ptr(sp.vx), ptr(sp.vy), ptr(self.E), self.nx, self.ncx,
+                                      self.ncy, qmdt, np.float32(dt / self.dx),
+                                      np.float32(dt / self.dy), sp.bnd[0], sp.bnd[1], do_push,
+                                      ptr(cnt))
+        if do_push:
+            self.pushed += live
+        sp.cnt = cnt
+
+    def compute_rho(self):
+        ns = len(self.species)
+        cnt = np.ascontiguousarray(np.stack([sp.cnt for sp in self.species]))
+        w0 = np.array([sp.w0 for sp in self.species])
+        q = np.array([sp.q for sp in self.species])
+        lib().mir_rho(self.nx * self.ny, ns, ptr(cnt), ptr(w0), ptr(q), ptr(self.vol),
+                      ptr(self.bg), ptr(self.rho))
+
+    def solve(self):
+        lib().mir_tri_solve(self.nx, ptr(self.rho), ptr(self.tags), ptr(self.tagval), ptr(self.a),
+                            ptr(self.cp), ptr(self.inv), (self.dx * self.dx) / self.eps0,
+                            self.singular, ptr(self.phi))
+        lib().mir_efield(1, self.nx, 1, self.dx, 1.0, ptr(self.phi), ptr(self.E))
+
+    def sort(self, s):
+        sp = self.species[s]
+        sp.np = int(lib().mir_sort_tile(self.dim, sp.np, self.T, self.ntx, ptr(sp.cell),
+                                        ptr(sp.ox), ptr(sp.oy), ptr(sp.vx), ptr(sp.vy),
+                                        ptr(sp.vz), ptr(sp.id)))
+
+    def waveform(self, t):
+        for tag, (off, amp, f, ph_turns) in self.wf.items():
+            u = f * t + ph_turns
+            u = u - np.floor(u)
+            s = C.c_double()
+            c = C.c_double()
+            lib().orc_sincos2pi(u, C.byref(s), C.byref(c))
+            self.tagval[tag] = off + amp * s.value
+
+    def step(self, dt, nsteps=1):
+        for _ in range(nsteps):
+            for k in range(len(self.mccs)):
+                self.mcc_run(k, dt)
+            for s in range(len(self.species)):
+                self.push(s, dt)
+            self.compute_rho()
+            self.solve()
+            self.waveform(self.step_count * dt)
+            self.step_count += 1
+            if self.sort_every > 0 and self.step_count % self.sort_every == 0:
+                for s in range(len(self.species)):
+                    self.sort(s)

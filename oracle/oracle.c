@@ -1,0 +1,904 @@
+/*
+ * oracle.c -- TEST INFRASTRUCTURE ONLY.  CPU oracle of the electrostatic PIC
+ * step used by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg
+ * as the checker; nothing in the product (iskra_amd/) links, loads or calls it.
+ *
+ * Two families of functions:
+ *
+ *  ref_*   literal FP64 restatement of the reference Julia code (absolute
+ *          positions, column-major arrays, 1-based logic translated to
+ *          0-based indices).  Each function cites the reference lines it
+ *          follows.  This is the "reference" every GPU result is compared to
+ *          within a stated tolerance.
+ *
+ *  mir_*   the same algorithms in the GPU's number format (int cell index +
+ *          FP32 offsets, FP32 velocities, 64-bit fixed-point CIC weights,
+ *          FP64 Poisson / E-field, Philox4x32-10 draws), written
+ *          independently of the HIP sources but with the same operation
+ *          order, so that GPU results can be compared bit for bit.
+ *
+ * The FP64 transcendental functions used by the MCC mirror come from
+ * iskra_amd/csrc/isk_fp64math.h (shared on purpose: bit-identical exp/log/
+ * sin/cos on CPU and GPU); the test suite pins that header against libm.
+ *
+ * Parity status: see the header of oracle/__init__.py for which functions are
+ * pinned by which check.
+ *
+ * Build: oracle/build.py (gcc -O2 -ffp-contract=off).
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "../iskra_amd/csrc/isk_fp64math.h"
+
+#define EXPORT __attribute__((visibility("default")))
+#define FIX_ONE 4294967296.0
+
+/* ======================================================================== */
+/* Philox (Salmon, Moraes, Dror, Shaw: "Parallel random numbers: as easy as  */
+/* 1, 2, 3", SC'11)                                                         */
+/* ======================================================================== */
+EXPORT void orc_philox4x32(const uint32_t ctr_in[4], const uint32_t key_in[2], uint32_t out[4]) {
+  uint32_t c0 = ctr_in[0], c1 = ctr_in[1], c2 = ctr_in[2], c3 = ctr_in[3];
+  uint32_t k0 = key_in[0], k1 = key_in[1];
+  for (int r = 0; r < 10; ++r) {
+    uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+    uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+    uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+    uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+    uint32_t n0 = hi1 ^ c1 ^ k0, n1 = lo1, n2 = hi0 ^ c3 ^ k1, n3 = lo0;
+    c0 = n0; c1 = n1; c2 = n2; c3 = n3;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+  out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+/* Philox4x64-10, same construction with 64-bit words (cross-check against
+ * numpy.random.Philox, which implements this variant) */
+EXPORT void orc_philox4x64(const uint64_t ctr_in[4], const uint64_t key_in[2], uint64_t out[4]) {
+  uint64_t c0 = ctr_in[0], c1 = ctr_in[1], c2 = ctr_in[2], c3 = ctr_in[3];
+  uint64_t k0 = key_in[0], k1 = key_in[1];
+  for (int r = 0; r < 10; ++r) {
+    __uint128_t p0 = (__uint128_t)0xD2E7470EE14C6C93ULL * c0;
+    __uint128_t p1 = (__uint128_t)0xCA5A826395121157ULL * c2;
+    uint64_t hi0 = (uint64_t)(p0 >> 64), lo0 = (uint64_t)p0;
+    uint64_t hi1 = (uint64_t)(p1 >> 64), lo1 = (uint64_t)p1;
+    uint64_t n0 = hi1 ^ c1 ^ k0, n1 = lo1, n2 = hi0 ^ c3 ^ k1, n3 = lo0;
+    c0 = n0; c1 = n1; c2 = n2; c3 = n3;
+    k0 += 0x9E3779B97F4A7C15ULL;
+    k1 += 0xBB67AE8584CAA73BULL;
+  }
+  out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+/* 53-bit uniform in (0,1) from two words, centred in its bin */
+EXPORT double orc_u53(uint32_t a, uint32_t b) {
+  uint64_t k = ((uint64_t)(a >> 6) << 27) | (uint64_t)(b >> 5);
+  return ((double)k + 0.5) * (1.0 / 9007199254740992.0);
+}
+
+/* thin wrappers so tests can pin the shared FP64 math against libm */
+EXPORT double orc_exp(double x) { return isk_exp(x); }
+EXPORT double orc_log(double x) { return isk_log(x); }
+EXPORT void orc_sincos2pi(double u, double *s, double *c) { isk_sincos2pi(u, s, c); }
+
+/* ======================================================================== */
+/* reference mode (FP64)                                                    */
+/* ======================================================================== */
+
+/* cell_volume (RegularGrids/src/RegularGrids.jl:106-118); the 1-D version is
+ * inferred by analogy (no 1-D method exists in the reference). */
+EXPORT void ref_cell_volume(int dim, int nx, int ny, double dx, double dy,
+                            const int32_t volper[4], double *V) {
+  double V0 = dim == 2 ? dx * dy : dx;
+  for (int j = 0; j < ny; ++j)
+    for (int i = 0; i < nx; ++i) V[j * nx + i] = V0;
+  for (int j = 0; j < ny; ++j) {
+    if (!volper[0]) V[j * nx + 0] *= 0.5;
+    if (!volper[1]) V[j * nx + nx - 1] *= 0.5;
+  }
+  if (dim == 2) {
+    for (int i = 0; i < nx; ++i) {
+      if (!volper[2]) V[i] *= 0.5;
+      if (!volper[3]) V[(ny - 1) * nx + i] *= 0.5;
+    }
+  }
+}
+
+/* particle_cell (ParticleInCell/src/ParticleInCell.jl:28-35): f = 1 + x/dh,
+ * i = floor(f) (1-based), h = f - i.  Returns the 0-based i. */
+static inline void ref_particle_cell(double x, double dh, int *i, double *h) {
+  double f = 1.0 + x / dh;
+  double fl = floor(f);
+  *i = (int)fl - 1;
+  *h = f - fl;
+}
+
+/* particle_to_grid (ParticleInCell/src/pic/cloud_in_cell.jl:1-18) with
+ * pu(p) = wg[p]; u is nx x ny column-major (i fastest) and is overwritten.
+ * 1-D: weights (1-h), h by analogy (no 1-D method in the reference). */
+EXPORT void ref_particle_to_grid(int dim, int64_t np, const double *x, const double *wg,
+                                 int nx, int ny, double dx, double dy, double *u) {
+  memset(u, 0, sizeof(double) * (size_t)nx * ny);
+  for (int64_t p = 0; p < np; ++p) {
+    int i, j = 0;
+    double hx, hy = 0.0;
+    ref_particle_cell(x[p], dx, &i, &hx);
+    if (dim == 2) ref_particle_cell(x[np + p], dy, &j, &hy);
+    double w = wg[p];
+    if (dim == 2) {
+      u[j * nx + i] += (1.0 - hx) * (1.0 - hy) * w;
+      u[j * nx + i + 1] += (hx) * (1.0 - hy) * w;
+      u[(j + 1) * nx + i] += (1.0 - hx) * (hy)*w;
+      u[(j + 1) * nx + i + 1] += (hx) * (hy)*w;
+    } else {
+      u[i] += (1.0 - hx) * w;
+      u[i + 1] += hx * w;
+    }
+  }
+}
+
+/* grid_to_particle (cloud_in_cell.jl:20-36): E is nx x ny x 3 column-major
+ * (component slowest); pE is np x 3 column-major. */
+EXPORT void ref_grid_to_particle(int dim, int64_t np, const double *x, int nx, int ny,
+                                 double dx, double dy, const double *E, double *pE) {
+  size_t nn = (size_t)nx * ny;
+  for (int64_t p = 0; p < np; ++p) {
+    int i, j = 0;
+    double hx, hy = 0.0;
+    ref_particle_cell(x[p], dx, &i, &hx);
+    if (dim == 2) ref_particle_cell(x[np + p], dy, &j, &hy);
+    for (int c = 0; c < 3; ++c) {
+      const double *u = E + c * nn;
+      double v;
+      if (dim == 2)
+        v = (1.0 - hx) * (1.0 - hy) * u[j * nx + i] + (hx) * (1.0 - hy) * u[j * nx + i + 1] +
+            (1.0 - hx) * (hy)*u[(j + 1) * nx + i] + (hx) * (hy)*u[(j + 1) * nx + i + 1];
+      else
+        v = (1.0 - hx) * u[i] + hx * u[i + 1];
+      pE[c * np + p] = v;
+    }
+  }
+}
+
+/* push_in_cartesian! (ParticleInCell/src/pic/pushers.jl:37-50), Boris as the
+ * reference writes it (v- x B, not v- x t); x: np x D, v, E, B: np x 3. */
+EXPORT void ref_push(int64_t np, int D, double *x, double *v, const double *E, const double *B,
+                     double qm, double dt) {
+  for (int64_t p = 0; p < np; ++p) {
+    double vm[3], t[3], vp[3], s[3], vpl[3];
+    for (int c = 0; c < 3; ++c) {
+      vm[c] = 0.5 * dt * qm * E[c * np + p] + v[c * np + p];
+      t[c] = 0.5 * dt * qm * B[c * np + p];
+    }
+    double t2 = t[0] * t[0] + t[1] * t[1] + t[2] * t[2];
+    /* v' = v- + v- x B */
+    vp[0] = vm[0] + (vm[1] * B[2 * np + p] - vm[2] * B[1 * np + p]);
+    vp[1] = vm[1] + (vm[2] * B[0 * np + p] - vm[0] * B[2 * np + p]);
+    vp[2] = vm[2] + (vm[0] * B[1 * np + p] - vm[1] * B[0 * np + p]);
+    for (int c = 0; c < 3; ++c) s[c] = 2.0 / (1.0 + t2) * t[c];
+    vpl[0] = vm[0] + (vp[1] * s[2] - vp[2] * s[1]);
+    vpl[1] = vm[1] + (vp[2] * s[0] - vp[0] * s[2]);
+    vpl[2] = vm[2] + (vp[0] * s[1] - vp[1] * s[0]);
+    for (int c = 0; c < 3; ++c) v[c * np + p] = dt * E[c * np + p] * qm * 0.5 + vpl[c];
+    for (int c = 0; c < D; ++c) x[c * np + p] = dt * v[c * np + p] + x[c * np + p];
+  }
+}
+
+/* Julia fld(a, b) for b > 0 */
+static inline double jl_fld(double a, double b) { return floor(a / b); }
+
+/* wrap! (ParticleInCell/src/pic/surfaces/wrap.jl:20-33) over the listed dims */
+EXPORT void ref_wrap(int64_t np, int D, double *x, const int64_t *n, const double *dh,
+                     const double *origin, int ndims, const int32_t *dims) {
+  for (int k = 0; k < ndims; ++k) {
+    int d = dims[k];
+    double Lx = (double)(n[d] - 1) * dh[d];
+    for (int64_t p = 0; p < np; ++p) {
+      double a = jl_fld(x[d * np + p] - origin[d], Lx);
+      if (a != 0.0) x[d * np + p] -= a * Lx;
+    }
+  }
+}
+
+/* remove! (ParticleInCell/src/pic/kinetic.jl:20-27): slot i takes the last
+ * particle; ids are swapped; wg[np-1] = w0. Arrays have leading dimension cap. */
+static void ref_remove(int64_t *np, int64_t cap, int D, double *x, double *v, double *wg,
+                       uint32_t *id, double w0, int64_t i) {
+  int64_t last = *np - 1;
+  for (int c = 0; c < D; ++c) x[c * cap + i] = x[c * cap + last];
+  for (int c = 0; c < 3; ++c) v[c * cap + i] = v[c * cap + last];
+  wg[i] = wg[last];
+  wg[last] = w0;
+  uint32_t t = id[i];
+  id[i] = id[last];
+  id[last] = t;
+  *np = last;
+}
+
+/* discard! (wrap.jl:1-18): reverse scan per dim, swap-remove when
+ * fld(x - origin, L) != 0.  Returns the number removed. */
+EXPORT int64_t ref_discard(int64_t *np, int64_t cap, int D, double *x, double *v, double *wg,
+                           uint32_t *id, double w0, const int64_t *n, const double *dh,
+                           const double *origin, int ndims, const int32_t *dims) {
+  int64_t before = *np;
+  for (int k = 0; k < ndims; ++k) {
+    int d = dims[k];
+    double Lx = (double)(n[d] - 1) * dh[d];
+    for (int64_t p = *np - 1; p >= 0; --p) {
+      double a = jl_fld(x[d * cap + p] - origin[d], Lx);
+      if (a != 0.0) ref_remove(np, cap, D, x, v, wg, id, w0, p);
+    }
+  }
+  return before - *np;
+}
+
+/* create_generalized_poisson_solver (FiniteDifferenceMethod/src/
+ * generalized_poisson.jl:34-68): dense nn x nn A, column-major, node
+ * k = i + nx*j.  For dim 1, ny = 1. */
+EXPORT void ref_poisson_assemble(int nx, int ny, double dx, double *A) {
+  size_t nn = (size_t)nx * ny;
+  memset(A, 0, sizeof(double) * nn * nn);
+#define AE(r, c) A[(size_t)(c)*nn + (r)]
+  for (int j = 0; j < ny; ++j)
+    for (int i = 0; i < nx; ++i) {
+      int k = i + nx * j;
+      if (i < nx - 1) { AE(k, k) -= 1.0; AE(k, k + 1) += 1.0; }
+      if (i > 0) { AE(k, k) -= 1.0; AE(k, k - 1) += 1.0; }
+      if (j < ny - 1) { AE(k, k) -= 1.0; AE(k, k + nx) += 1.0; }
+      if (j > 0) { AE(k, k) -= 1.0; AE(k, k - nx) += 1.0; }
+    }
+  double d2 = dx * dx;
+  for (size_t q = 0; q < nn * nn; ++q) A[q] /= d2;
+}
+
+/* apply_periodic 1-D (generalized_poisson.jl:271-284) */
+EXPORT void ref_apply_periodic_1d(int nx, double dx, double *A) {
+  size_t nn = (size_t)nx;
+  double c = (0.5 * 1.0 + 0.5 * 1.0) / (dx * dx);
+  AE(0, 0) -= c;
+  AE(0, nx - 1) += c;
+  AE(nx - 1, nx - 1) -= c;
+  AE(nx - 1, 0) += c;
+}
+
+/* apply_periodic 2-D (generalized_poisson.jl:286-324); axis 1 links j=1<->ny
+ * with dx^2, axis 2 links i=1<->nx with dy^2 (as written in the reference) */
+EXPORT void ref_apply_periodic_2d(int nx, int ny, double dx, double dy, int axis, double *A) {
+  size_t nn = (size_t)nx * ny;
+  if (axis == 1) {
+    double c = (0.5 + 0.5) / (dx * dx);
+    for (int jj = 0; jj < 2; ++jj) {
+      int j = jj == 0 ? 0 : ny - 1;
+      for (int i = 0; i < nx; ++i) {
+        int k = i + nx * j;
+        if (j == ny - 1) { AE(k, k) -= c; AE(k, i) += c; }
+        if (j == 0) { AE(k, k) -= c; AE(k, i + nx * (ny - 1)) += c; }
+      }
+      if (ny == 1) break;
+    }
+  } else {
+    double c = (0.5 + 0.5) / (dy * dy);
+    for (int j = 0; j < ny; ++j)
+      for (int ii = 0; ii < 2; ++ii) {
+        int i = ii == 0 ? 0 : nx - 1;
+        int k = i + nx * j;
+        if (i == nx - 1) { AE(k, k) -= c; AE(k, nx * j) += c; }
+        if (i == 0) { AE(k, k) -= c; AE(k, nx - 1 + nx * j) += c; }
+        if (nx == 1) break;
+      }
+  }
+}
+
+/* apply_dirichlet (generalized_poisson.jl:205-215): row cleared, diagonal 1,
+ * b = value, node removed from the rho dofs (isdof[k] = 0) */
+EXPORT void ref_apply_dirichlet(int nn, const uint8_t *mask, double value, double *A, double *b,
+                                uint8_t *isdof) {
+  for (int k = 0; k < nn; ++k) {
+    if (!mask[k]) continue;
+    for (int c = 0; c < nn; ++c) AE(k, c) = 0.0;
+    AE(k, k) = 1.0;
+    b[k] = value;
+    isdof[k] = 0;
+  }
+}
+
+/* LU with partial pivoting (the algorithm of LAPACK dgetrf/dgetrs that
+ * A\b uses, generalized_poisson.jl:201-203), in place on a copy of A.
+ * Returns 0, or 1 if a pivot is exactly zero. */
+EXPORT int ref_lu_solve(int n, const double *Ain, const double *b, double *x) {
+  double *A = (double *)malloc(sizeof(double) * (size_t)n * n);
+  int *piv = (int *)malloc(sizeof(int) * n);
+  memcpy(A, Ain, sizeof(double) * (size_t)n * n);
+  size_t nn = (size_t)n;
+  int sing = 0;
+  for (int k = 0; k < n; ++k) {
+    int pr = k;
+    double mx = fabs(AE(k, k));
+    for (int r = k + 1; r < n; ++r)
+      if (fabs(AE(r, k)) > mx) { mx = fabs(AE(r, k)); pr = r; }
+    piv[k] = pr;
+    if (pr != k)
+      for (int c = 0; c < n; ++c) { double t = AE(k, c); AE(k, c) = AE(pr, c); AE(pr, c) = t; }
+    if (AE(k, k) == 0.0) { sing = 1; continue; }
+    double inv = 1.0 / AE(k, k);
+    for (int r = k + 1; r < n; ++r) AE(r, k) *= inv;
+    for (int c = k + 1; c < n; ++c) {
+      double akc = AE(k, c);
+      if (akc == 0.0) continue;
+      for (int r = k + 1; r < n; ++r) AE(r, c) -= AE(r, k) * akc;
+    }
+  }
+  for (int k = 0; k < n; ++k) x[k] = b[k];
+  for (int k = 0; k < n; ++k) if (piv[k] != k) { double t = x[k]; x[k] = x[piv[k]]; x[piv[k]] = t; }
+  for (int k = 0; k < n; ++k) for (int r = k + 1; r < n; ++r) x[r] -= AE(r, k) * x[k];
+  for (int k = n - 1; k >= 0; --k) {
+    x[k] /= AE(k, k);
+    for (int r = 0; r < k; ++r) x[r] -= AE(r, k) * x[k];
+  }
+  free(A);
+  free(piv);
+  return sing;
+}
+#undef AE
+
+/* calculate_electric_field! (generalized_poisson.jl:385-410): E is nx x ny x 3
+ * column-major; E[...,3] = 0; 1-D fills only component 1. */
+EXPORT void ref_efield(int dim, int nx, int ny, double dx, double dy, const double *phi,
+                       double *E) {
+  size_t nn = (size_t)nx * ny;
+  memset(E, 0, sizeof(double) * nn * 3);
+  for (int j = 0; j < ny; ++j)
+    for (int i = 0; i < nx; ++i) {
+      int k = i + nx * j;
+      double ex;
+      if (i == 0) ex = (phi[k] - phi[k + 1]) / dx;
+      else if (i == nx - 1) ex = (phi[k - 1] - phi[k]) / dx;
+      else ex = (phi[k - 1] - phi[k + 1]) / (2.0 * dx);
+      E[k] = ex;
+      if (dim == 2) {
+        double ey;
+        if (j == 0) ey = (phi[k] - phi[k + nx]) / dy;
+        else if (j == ny - 1) ey = (phi[k - nx] - phi[k]) / dy;
+        else ey = (phi[k - nx] - phi[k + nx]) / (2.0 * dy);
+        E[nn + k] = ey;
+      }
+    }
+}
+
+/* CrossSection call (Chemistry/src/cross_section.jl:3-17): linear
+ * interpolation between nodes, flat outside */
+EXPORT double ref_sigma(int n, const double *eps, const double *sig, double e) {
+  if (e <= eps[0]) return sig[0];
+  if (e >= eps[n - 1]) return sig[n - 1];
+  int lo = 0, hi = n - 1;
+  while (hi - lo > 1) {
+    int mid = (lo + hi) / 2;
+    if (eps[mid] <= e) lo = mid; else hi = mid;
+  }
+  double t = (e - eps[lo]) / (eps[hi] - eps[lo]);
+  return sig[lo] * (1.0 - t) + sig[hi] * t;
+}
+
+/* MonteCarloCollisions constructor (Chemistry/src/mcc.jl:27-51): max over the
+ * union of table energies (and 0) of sum_k sigma_k(eps) sqrt(2 eps / m) */
+EXPORT double ref_max_sigma_g(int ncoll, const int32_t *off, const double *eps,
+                              const double *sig, double m_eV, int nunion, double *work) {
+  (void)nunion;
+  int cnt = 0;
+  work[cnt++] = 0.0;
+  for (int k = 0; k < ncoll; ++k)
+    for (int q = off[k]; q < off[k + 1]; ++q) work[cnt++] = eps[q];
+  /* sort + unique (insertion sort: tables are small) */
+  for (int a = 1; a < cnt; ++a) {
+    double v = work[a];
+    int b = a - 1;
+    while (b >= 0 && work[b] > v) { work[b + 1] = work[b]; --b; }
+    work[b + 1] = v;
+  }
+  int u = 0;
+  for (int a = 0; a < cnt; ++a)
+    if (u == 0 || work[a] != work[u - 1]) work[u++] = work[a];
+  double alpha = sqrt(2.0 / m_eV), best = 0.0;
+  for (int a = 0; a < u; ++a) {
+    double v = alpha * sqrt(work[a]);
+    double s = 0.0;
+    for (int k = 0; k < ncoll; ++k)
+      s += ref_sigma(off[k + 1] - off[k], eps + off[k], sig + off[k], work[a]) * v;
+    if (s > best) best = s;
+  }
+  return best;
+}
+
+/* ======================================================================== */
+/* mirror mode: the GPU number format and operation order                   */
+/* ======================================================================== */
+
+static inline int mir_advance(float v, float dtd, float *o) {
+  float h = fmaf(v, dtd, *o);
+  float f = floorf(h);
+  h = h - f;
+  if (h >= 1.0f) { h = 0.0f; f += 1.0f; }
+  *o = h;
+  return (int)f;
+}
+
+static inline int mir_bnd(int *i, int n, int mode) {
+  if (*i >= 0 && *i < n) return 1;
+  if (mode == 1) {
+    int r = *i % n;
+    if (r < 0) r += n;
+    *i = r;
+    return 1;
+  }
+  return 0;
+}
+
+/* CIC gather of E (FP32) at a particle: the GPU's formula */
+static inline void mir_gather1(int dim, const float *E, int nx, int i, int j, float hx, float hy,
+                               float *Ex, float *Ey) {
+  float ax = 1.0f - hx;
+  if (dim == 2) {
+    float ay = 1.0f - hy;
+    float w00 = ax * ay, w10 = hx * ay, w01 = ax * hy, w11 = hx * hy;
+    int q = j * nx + i;
+    const float *e00 = E + 2 * q, *e10 = E + 2 * (q + 1), *e01 = E + 2 * (q + nx),
+                *e11 = E + 2 * (q + nx + 1);
+    *Ex = fmaf(w11, e11[0], fmaf(w01, e01[0], fmaf(w10, e10[0], w00 * e00[0])));
+    *Ey = fmaf(w11, e11[1], fmaf(w01, e01[1], fmaf(w10, e10[1], w00 * e00[1])));
+  } else {
+    *Ex = fmaf(hx, E[i + 1], ax * E[i]);
+    *Ey = 0.0f;
+  }
+}
+
+EXPORT void mir_gather(int dim, int64_t np, const int32_t *cell, const float *ox, const float *oy,
+                       const float *E, int nx, float *outx, float *outy) {
+  for (int64_t p = 0; p < np; ++p) {
+    float ex = 0.f, ey = 0.f;
+    if (cell[p] >= 0) {
+      int i = cell[p] & 0xFFFF, j = cell[p] >> 16;
+      mir_gather1(dim, E, nx, i, j, ox[p], dim == 2 ? oy[p] : 0.f, &ex, &ey);
+    }
+    outx[p] = ex;
+    if (outy) outy[p] = ey;
+  }
+}
+
+/* fixed-point CIC deposit of one particle into cnt (nodes) */
+static inline void mir_deposit1(int dim, int nx, int i, int j, float hx, float hy, int64_t *cnt) {
+  if (dim == 2) {
+    double dxh = (double)hx, dyh = (double)hy;
+    int64_t Wx = llrint(dxh * FIX_ONE), Wy = llrint(dyh * FIX_ONE);
+    int64_t Wxy = llrint((dxh * dyh) * FIX_ONE);
+    int q = j * nx + i;
+    cnt[q] += ((int64_t)1 << 32) - Wx - Wy + Wxy;
+    cnt[q + 1] += Wx - Wxy;
+    cnt[q + nx] += Wy - Wxy;
+    cnt[q + nx + 1] += Wxy;
+  } else {
+    int64_t W1 = llrint((double)hx * FIX_ONE);
+    cnt[i] += ((int64_t)1 << 32) - W1;
+    cnt[i + 1] += W1;
+  }
+}
+
+/* gather + push (B = 0) + boundary + deposit, the GPU's arithmetic.
+ * cnt is accumulated (caller zeroes it).  Returns the live count. */
+EXPORT int64_t mir_push_deposit(int dim, int64_t np, int32_t *cell, float *ox, float *oy,
+                                float *vx, float *vy, const float *E, int nx, int ncx, int ncy,
+                                float qmdt, float dtdx, float dtdy, int bndx, int bndy,
+                                int do_push, int64_t *cnt) {
+  int64_t live = 0;
+  for (int64_t p = 0; p < np; ++p) {
+    if (cell[p] < 0) continue;
+    int i = cell[p] & 0xFFFF, j = cell[p] >> 16;
+    float hx = ox[p], hy = dim == 2 ? oy[p] : 0.f;
+    int alive = 1;
+    if (do_push) {
+      float Ex, Ey;
+      mir_gather1(dim, E, nx, i, j, hx, hy, &Ex, &Ey);
+      vx[p] = fmaf(qmdt, Ex, vx[p]);
+      i += mir_advance(vx[p], dtdx, &hx);
+      alive = mir_bnd(&i, ncx, bndx);
+      if (dim == 2) {
+        vy[p] = fmaf(qmdt, Ey, vy[p]);
+        j += mir_advance(vy[p], dtdy, &hy);
+        alive = mir_bnd(&j, ncy, bndy) && alive;
+      }
+      cell[p] = alive ? (i | (j << 16)) : -1;
+      ox[p] = hx;
+      if (dim == 2) oy[p] = hy;
+    }
+    if (alive) {
+      ++live;
+      mir_deposit1(dim, nx, i, j, hx, hy, cnt);
+    }
+  }
+  return live;
+}
+
+/* rho = sum_s q_s n_s + bg, n_s = (cnt_s 2^-32) w0_s / V (GPU order) */
+EXPORT void mir_rho(int nn, int ns, const int64_t *cnt /* ns x nn */, const double *w0,
+                    const double *q, const double *vol, const double *bg, double *rho) {
+  for (int k = 0; k < nn; ++k) {
+    double r = 0.0;
+    for (int s = 0; s < ns; ++s) {
+      double n = ((double)cnt[(size_t)s * nn + k] * (1.0 / FIX_ONE)) * w0[s] / vol[k];
+      r += q[s] * n;
+    }
+    rho[k] = r + bg[k];
+  }
+}
+
+/* 1-D tridiagonal factor of the reference matrix scaled by dx^2
+ * (generalized_poisson.jl:22-25,42-68,205-215,271-284): a = sub-diagonal,
+ * inv = 1/pivot, cp = super-diagonal * inv.  singular = periodic ring with
+ * no Dirichlet node: rows 1..n-1 with node 0 pinned. */
+EXPORT int mir_tri_factor(int n, const int8_t *tags, int singular, double *a, double *cp,
+                          double *inv) {
+  for (int i = 0; i < n; ++i) { a[i] = 0.0; cp[i] = 0.0; inv[i] = 0.0; }
+  double cprev = 0.0;
+  int i0 = singular ? 1 : 0;
+  for (int i = i0; i < n; ++i) {
+    double ai, ci, di;
+    if (!singular && tags[i]) {
+      ai = 0.0; ci = 0.0; di = 1.0;
+    } else if (!singular) {
+      ai = i > 0 ? 1.0 : 0.0;
+      ci = i < n - 1 ? 1.0 : 0.0;
+      di = -(ai + ci);
+    } else {
+      ai = i > 1 ? 1.0 : 0.0;
+      ci = i < n - 1 ? 1.0 : 0.0;
+      di = -2.0;
+    }
+    double den = di - ai * cprev;
+    if (den == 0.0) return 1;
+    a[i] = ai;
+    inv[i] = 1.0 / den;
+    cp[i] = ci * inv[i];
+    cprev = cp[i];
+  }
+  return 0;
+}
+
+EXPORT void mir_tri_solve(int n, const double *rho, const int8_t *tags, const double *tagval,
+                          const double *a, const double *cp, const double *inv, double scale,
+                          int singular, double *phi) {
+  double *y = (double *)malloc(sizeof(double) * n);
+  if (!singular) {
+    double yprev = 0.0;
+    for (int i = 0; i < n; ++i) {
+      int tg = tags[i];
+      double r = tg ? tagval[tg] : -(rho[i] * scale);
+      double v = fma(-a[i], yprev, r) * inv[i];
+      y[i] = v;
+      yprev = v;
+    }
+    double xn = 0.0;
+    for (int i = n - 1; i >= 0; --i) {
+      double v = fma(-cp[i], xn, y[i]);
+      phi[i] = v;
+      xn = v;
+    }
+  } else {
+    double sum = 0.0;
+    for (int i = 0; i < n; ++i) sum += -(rho[i] * scale);
+    double mean = sum / (double)n;
+    double yprev = 0.0;
+    for (int i = 1; i < n; ++i) {
+      double r = -(rho[i] * scale) - mean;
+      double v = fma(-a[i], yprev, r) * inv[i];
+      y[i] = v;
+      yprev = v;
+    }
+    double xn = 0.0;
+    for (int i = n - 1; i >= 1; --i) {
+      double v = fma(-cp[i], xn, y[i]);
+      phi[i] = v;
+      xn = v;
+    }
+    phi[0] = 0.0;
+    double s2 = 0.0;
+    for (int i = 0; i < n; ++i) s2 += phi[i];
+    double m2 = s2 / (double)n;
+    for (int i = 0; i < n; ++i) phi[i] = phi[i] - m2;
+  }
+  free(y);
+}
+
+/* E = -grad phi in FP64, stored FP32 (1-D: nx values; 2-D: interleaved) */
+EXPORT void mir_efield(int dim, int nx, int ny, double dx, double dy, const double *phi, float *E) {
+  for (int j = 0; j < ny; ++j)
+    for (int i = 0; i < nx; ++i) {
+      int n = j * nx + i;
+      double ex;
+      if (nx < 2) ex = 0.0;
+      else if (i == 0) ex = (phi[n] - phi[n + 1]) / dx;
+      else if (i == nx - 1) ex = (phi[n - 1] - phi[n]) / dx;
+      else ex = (phi[n - 1] - phi[n + 1]) / (2.0 * dx);
+      if (dim == 1) { E[n] = (float)ex; continue; }
+      double ey;
+      if (ny < 2) ey = 0.0;
+      else if (j == 0) ey = (phi[n] - phi[n + nx]) / dy;
+      else if (j == ny - 1) ey = (phi[n - nx] - phi[n]) / dy;
+      else ey = (phi[n - nx] - phi[n + nx]) / (2.0 * dy);
+      E[2 * n] = (float)ex;
+      E[2 * n + 1] = (float)ey;
+    }
+}
+
+/* ---- MCC mirror ---- */
+typedef struct {
+  uint32_t id, step, stream, k0, k1;
+  int block, half;
+  uint32_t cur[4];
+} MirDraws;
+
+static double md_next(MirDraws *d) {
+  if (d->half == 0) {
+    uint32_t c[4] = {d->id, d->step, d->stream, (uint32_t)d->block};
+    uint32_t k[2] = {d->k0, d->k1};
+    orc_philox4x32(c, k, d->cur);
+    d->block++;
+    d->half = 1;
+    return orc_u53(d->cur[0], d->cur[1]);
+  }
+  d->half = 0;
+  return orc_u53(d->cur[2], d->cur[3]);
+}
+
+/* euler_angles (mcc.jl:89-106) */
+static void m_angles(double vx, double vy, double vz, double *st, double *ct, double *sp, double *cp) {
+  double nv = sqrt(vx * vx + vy * vy + vz * vz);
+  *ct = vz / nv;
+  *st = sqrt(1.0 - *ct * *ct);
+  if (*st == 0.0) { *cp = 1.0; *sp = 0.0; }
+  else { *cp = vx / nv / *st; *sp = vy / nv / *st; }
+}
+
+/* [sX cE, sX sE, cX] * unrotated (mcc.jl:83-87,108-113) */
+static void m_rotate(double st, double ct, double sp, double cp, double sX, double cX, double sE,
+                     double cE, double *r) {
+  double a0 = sX * cE, a1 = sX * sE, a2 = cX;
+  r[0] = a0 * (cp * ct) + a1 * (-sp) + a2 * (cp * st);
+  r[1] = a0 * (-sp * ct) + a1 * cp + a2 * (sp * st);
+  r[2] = a0 * (-st) + a1 * 0.0 + a2 * ct;
+}
+
+/* isotropic_scattering (mcc.jl:53-62,108-113): chi = 2 pi U, eta = 2 pi U */
+static void m_iso(double vx, double vy, double vz, MirDraws *d, double *r) {
+  double st, ct, sp, cp, sX, cX, sE, cE;
+  m_angles(vx, vy, vz, &st, &ct, &sp, &cp);
+  isk_sincos2pi(md_next(d), &sX, &cX);
+  isk_sincos2pi(md_next(d), &sE, &cE);
+  m_rotate(st, ct, sp, cp, sX, cX, sE, cE, r);
+}
+
+/* diffuse_reflection (mcc.jl:64-72,122-127): sin chi = sqrt(U), cos chi < 0 */
+static void m_diffuse(double vx, double vy, double vz, MirDraws *d, double *r) {
+  double st, ct, sp, cp, sE, cE;
+  m_angles(vx, vy, vz, &st, &ct, &sp, &cp);
+  double sX = sqrt(md_next(d));
+  double cX = -sqrt(1.0 - sX * sX);
+  isk_sincos2pi(md_next(d), &sE, &cE);
+  m_rotate(st, ct, sp, cp, sX, cX, sE, cE, r);
+}
+
+/* maxwellian_velocity / randn(3) (mcc.jl:79-81) via Box-Muller */
+static void m_randn3(MirDraws *d, double *n) {
+  double u1 = md_next(d), u2 = md_next(d), u3 = md_next(d), u4 = md_next(d);
+  double r1 = sqrt(-2.0 * isk_log(u1)), r2 = sqrt(-2.0 * isk_log(u3));
+  double s1, c1, s2, c2;
+  isk_sincos2pi(u2, &s1, &c1);
+  isk_sincos2pi(u4, &s2, &c2);
+  n[0] = r1 * c1;
+  n[1] = r1 * s1;
+  n[2] = r2 * c2;
+}
+
+typedef struct {
+  int ncoll;
+  int32_t types[8];
+  double energy[8];
+  const int32_t *off;
+  const double *eps, *sig;
+  double candP, dt, m_eV, ms, mt, vth_t, tqm;
+  uint32_t k0, k1, step, stream;
+  int dim, nx;
+} MirMcc;
+
+/* PIC.perform!(::MonteCarloCollisions) with per-particle candidates
+ * (mcc.jl:231-289; kinematics mcc.jl:129-229).  Products (ionisation) are
+ * returned in particle order: pcell/pox/poy/pe (3 per product)/pi (3 per
+ * product).  nu: nodes x ncoll counts (accumulated), events: per process.
+ * Returns the number of products, or -1 if some P_k > 1. */
+EXPORT int64_t mir_mcc(const MirMcc *a, int64_t np, const int32_t *cell, const float *ox,
+                       const float *oy, float *vx, float *vy, float *vz, const uint32_t *id,
+                       const float *E, const double *tn, int64_t *nu, int64_t *events,
+                       int32_t *pcell, float *pox, float *poy, float *pe, float *pi) {
+  int64_t nprod = 0;
+  int err = 0;
+  const int N = a->ncoll;
+  for (int64_t p = 0; p < np; ++p) {
+    int32_t c = cell[p];
+    if (c < 0) continue;
+    MirDraws d = {id[p], a->step, a->stream, a->k0, a->k1, 0, 0, {0, 0, 0, 0}};
+    double Uc = md_next(&d);
+    double U = md_next(&d);
+    if (!(Uc < a->candP)) continue;
+    int i = c & 0xFFFF, j = c >> 16;
+    int node = a->dim == 2 ? j * a->nx + i : i;
+    double n = tn[node];
+    if (n < 0.0) continue;
+    int k = (int)floor((double)N * U);
+    if (k > N - 1) k = N - 1;
+    double v0 = vx[p], v1 = vy[p], v2 = vz[p];
+    double tvx = 0.0, tvy = 0.0;
+    if (a->tqm != 0.0) {
+      if (a->dim == 2) {
+        tvx = a->tqm * (double)E[2 * node] * a->dt;
+        tvy = a->tqm * (double)E[2 * node + 1] * a->dt;
+      } else {
+        tvx = a->tqm * (double)E[node] * a->dt;
+      }
+    }
+    double gx = tvx - v0, gy = tvy - v1, gz = 0.0 - v2;
+    double g = sqrt(gx * gx + gy * gy + gz * gz);
+    double eps = (0.5 * a->m_eV) * (g * g);
+    double sg = ref_sigma(a->off[k + 1] - a->off[k], a->eps + a->off[k], a->sig + a->off[k], eps) * g;
+    double P = 1.0 - isk_exp(((-n) * sg) * a->dt);
+    P /= a->candP;
+    if (P > 1.0) err = 1;
+    if (!(U > (double)(k + 1) / (double)N - P)) continue;
+    nu[(size_t)node * N + k] += 1;
+    events[k] += 1;
+    int type = a->types[k];
+    if (type == 0 || type == 1) {
+      double mr1 = a->ms / (a->ms + a->mt), mr2 = a->mt / (a->ms + a->mt);
+      double tv[3], sv[3] = {v0, v1, v2}, vr[3], w[3], dir[3];
+      m_randn3(&d, tv);
+      for (int q = 0; q < 3; ++q) tv[q] *= a->vth_t;
+      for (int q = 0; q < 3; ++q) { vr[q] = sv[q] - tv[q]; w[q] = mr1 * sv[q] + mr2 * tv[q]; }
+      double nvr = sqrt(vr[0] * vr[0] + vr[1] * vr[1] + vr[2] * vr[2]);
+      if (type == 0) m_iso(vr[0], vr[1], vr[2], &d, dir);
+      else m_diffuse(vr[0], vr[1], vr[2], &d, dir);
+      vx[p] = (float)(w[0] + mr2 * (nvr * dir[0]));
+      vy[p] = (float)(w[1] + mr2 * (nvr * dir[1]));
+      vz[p] = (float)(w[2] + mr2 * (nvr * dir[2]));
+    } else if (type == 2) {
+      double sE = (0.5 * a->m_eV) * (v0 * v0 + v1 * v1 + v2 * v2) - a->energy[k];
+      if (sE >= 0.0) {
+        double ev = sqrt(2.0 / a->m_eV) * sqrt(sE);
+        double dir[3];
+        m_iso(v0, v1, v2, &d, dir);
+        vx[p] = (float)(ev * dir[0]);
+        vy[p] = (float)(ev * dir[1]);
+        vz[p] = (float)(ev * dir[2]);
+      }
+    } else if (type == 3) {
+      double sE = (0.5 * a->m_eV) * (v0 * v0 + v1 * v1 + v2 * v2) - a->energy[k];
+      if (sE >= 0.0) {
+        double e1E = sE * md_next(&d);
+        double e2E = sE - e1E;
+        double al = sqrt(2.0 / a->m_eV);
+        double e1v = al * sqrt(e1E), e2v = al * sqrt(e2E);
+        double dir[3], dir2[3], tv[3];
+        m_diffuse(v0, v1, v2, &d, dir);
+        double nv[3] = {e1v * dir[0], e1v * dir[1], e1v * dir[2]};
+        vx[p] = (float)nv[0];
+        vy[p] = (float)nv[1];
+        vz[p] = (float)nv[2];
+        m_diffuse(nv[0], nv[1], nv[2], &d, dir2);
+        m_randn3(&d, tv);
+        pcell[nprod] = c;
+        pox[nprod] = ox[p];
+        poy[nprod] = a->dim == 2 ? oy[p] : 0.f;
+        for (int q = 0; q < 3; ++q) {
+          pe[3 * nprod + q] = (float)(e2v * dir2[q]);
+          pi[3 * nprod + q] = (float)(tv[q] * a->vth_t);
+        }
+        ++nprod;
+      }
+    }
+  }
+  return err ? -1 : nprod;
+}
+
+/* Maxwellian loader: the GPU's draws and arithmetic (sources.jl:24-34) */
+EXPORT void mir_sample(int dim, int64_t n, int64_t first_id, double x0, double y0, double dx,
+                       double dy, int ncx, int ncy, const double *wx, const double *dxo,
+                       const double *wv, const double *dv, uint64_t seed, uint32_t stream,
+                       int32_t *cell, float *ox, float *oy, float *vx, float *vy, float *vz,
+                       uint32_t *id) {
+  uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+  for (int64_t k = 0; k < n; ++k) {
+    uint32_t c0[4] = {(uint32_t)k, (uint32_t)(k >> 32), stream, 0u}, r0[4], r1[4], r2[4];
+    orc_philox4x32(c0, key, r0);
+    c0[3] = 1u;
+    orc_philox4x32(c0, key, r1);
+    c0[3] = 2u;
+    orc_philox4x32(c0, key, r2);
+    double ux = orc_u53(r0[0], r0[1]);
+    double px = ux * wx[0] + dxo[0];
+    double s = (px - x0) / dx;
+    double fl = floor(s);
+    int i = (int)fl;
+    float of = (float)(s - fl);
+    if (of >= 1.0f) { of = 0.0f; i += 1; }
+    if (i < 0) i = 0;
+    if (i > ncx - 1) i = ncx - 1;
+    int j = 0;
+    if (dim == 2) {
+      double uy = orc_u53(r0[2], r0[3]);
+      double py = uy * wx[1] + dxo[1];
+      double sy = (py - y0) / dy;
+      double fly = floor(sy);
+      j = (int)fly;
+      float ofy = (float)(sy - fly);
+      if (ofy >= 1.0f) { ofy = 0.0f; j += 1; }
+      if (j < 0) j = 0;
+      if (j > ncy - 1) j = ncy - 1;
+      oy[k] = ofy;
+    }
+    double u1 = orc_u53(r1[0], r1[1]), u2 = orc_u53(r1[2], r1[3]);
+    double u3 = orc_u53(r2[0], r2[1]), u4 = orc_u53(r2[2], r2[3]);
+    double rad1 = sqrt(-2.0 * isk_log(u1)), rad2 = sqrt(-2.0 * isk_log(u3));
+    double s1, c1, s2, c2;
+    isk_sincos2pi(u2, &s1, &c1);
+    isk_sincos2pi(u4, &s2, &c2);
+    cell[k] = i | (j << 16);
+    ox[k] = of;
+    vx[k] = (float)(rad1 * c1 * wv[0] + dv[0]);
+    vy[k] = (float)(rad1 * s1 * wv[1] + dv[1]);
+    vz[k] = (float)(rad2 * c2 * wv[2] + dv[2]);
+    if (id) id[k] = (uint32_t)(first_id + k);
+  }
+}
+
+/* stable counting sort by key = tile of the cell (or the dead bin) with
+ * compaction of discarded particles: the GPU's ordering rule.  Arrays are
+ * permuted in place; returns the live count. */
+EXPORT int64_t mir_sort_tile(int dim, int64_t np, int T, int ntx, int32_t *cell, float *ox,
+                             float *oy, float *vx, float *vy, float *vz, uint32_t *id) {
+  int log2T = 0;
+  while ((1 << log2T) < T) ++log2T;
+  int64_t *perm = (int64_t *)malloc(sizeof(int64_t) * (np > 0 ? np : 1));
+  int maxkey = 0;
+  int *key = (int *)malloc(sizeof(int) * (np > 0 ? np : 1));
+  for (int64_t p = 0; p < np; ++p) {
+    int c = cell[p];
+    if (c < 0) { key[p] = -1; continue; }
+    int i = c & 0xFFFF, j = c >> 16;
+    key[p] = (i >> log2T) + ntx * (j >> log2T);
+    if (key[p] > maxkey) maxkey = key[p];
+  }
+  int64_t *cnt = (int64_t *)calloc((size_t)maxkey + 2, sizeof(int64_t));
+  for (int64_t p = 0; p < np; ++p) if (key[p] >= 0) cnt[key[p] + 1]++;
+  for (int k = 0; k <= maxkey; ++k) cnt[k + 1] += cnt[k];
+  int64_t live = cnt[maxkey + 1];
+  for (int64_t p = 0; p < np; ++p) if (key[p] >= 0) perm[cnt[key[p]]++] = p;
+#define PERMUTE(T_, arr)                                                    \
+  if (arr) {                                                                \
+    T_ *tmp = (T_ *)malloc(sizeof(T_) * (live > 0 ? live : 1));              \
+    for (int64_t q = 0; q < live; ++q) tmp[q] = arr[perm[q]];               \
+    memcpy(arr, tmp, sizeof(T_) * live);                                    \
+    free(tmp);                                                              \
+  }
+  PERMUTE(int32_t, cell)
+  PERMUTE(float, ox)
+  if (dim == 2) { PERMUTE(float, oy) }
+  PERMUTE(float, vx)
+  PERMUTE(float, vy)
+  PERMUTE(float, vz)
+  PERMUTE(uint32_t, id)
+#undef PERMUTE
+  free(perm);
+  free(key);
+  free(cnt);
+  return live;
+}

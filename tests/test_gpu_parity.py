@@ -1,0 +1,292 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU oracle.
+
+Bit-exact against the oracle's mirror mode (same number format and operation
+order) for integer/index work and the fixed-point deposit; within stated
+tolerances against the FP64 reference mode.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+FIX = 2.0 ** -32
+
+
+def _rand_state(rng, n, ncx, ncy, vscale_x, vscale_y, dim=2):
+    i = rng.integers(0, ncx, n)
+    j = rng.integers(0, ncy, n) if dim == 2 else np.zeros(n, dtype=np.int64)
+    cell = (i | (j << 16)).astype(np.int32)
+    ox = rng.random(n, dtype=np.float32)
+    oy = rng.random(n, dtype=np.float32) if dim == 2 else np.zeros(n, np.float32)
+    vx = (rng.standard_normal(n) * vscale_x).astype(np.float32)
+    vy = (rng.standard_normal(n) * vscale_y).astype(np.float32)
+    vz = rng.standard_normal(n).astype(np.float32)
+    ids = rng.permutation(n).astype(np.uint32)
+    return cell, ox, oy, vx, vy, vz, ids
+
+
+def _mirror_push(orc, dim, st, E32, nx, ncx, ncy, q, m, dt, dx, dy, bnd):
+    s = {k: v.copy() for k, v in st.items()}
+    cnt = np.zeros(len(E32) // dim, dtype=np.int64)
+    live = orc.lib().mir_push_deposit(
+        dim, len(s["cell"]), orc.ptr(s["cell"]), orc.ptr(s["ox"]), orc.ptr(s["oy"]),
+        orc.ptr(s["vx"]), orc.ptr(s["vy"]), orc.ptr(E32), nx, ncx, ncy,
+        np.float32(q / m * dt), np.float32(dt / dx), np.float32(dt / dy), bnd[0], bnd[1], 1,
+        orc.ptr(cnt))
+    return s, cnt, live
+
+
+@pytest.mark.parametrize("tile,chunk,margin,vcells", [(8, 1024, 2, 0.4), (4, 256, 1, 2.5),
+                                                       (16, 4096, 3, 1.0)])
+def test_push_deposit_2d_bitexact(gpu_ready, orc, tile, chunk, margin, vcells):
+    from iskra_amd import Context, _lib as L
+    rng = np.random.default_rng(tile * 100 + chunk)
+    nx, ny = 65, 49
+    dx, dy = 1e-3, 1.25e-3
+    q, m, w0, dt = -1.6e-19, 9.1e-31, 1e6, 1e-11
+    ctx = Context()
+    ctx.grid(2, (nx, ny), (dx, dy))
+    ctx.tuning(tile=tile, chunk=chunk, margin=margin)
+    n = 50000
+    sid = ctx.species(q, m, w0, n + 16, track_ids=True)
+    ctx.boundary(sid, [L.BND_DISCARD, L.BND_WRAP])
+    st = _rand_state(rng, n, nx - 1, ny - 1, vcells * dx / dt, vcells * dy / dt)
+    ctx.set_state(sid, *st)
+    Ex = rng.standard_normal(nx * ny) * 3e4
+    Ey = rng.standard_normal(nx * ny) * 3e4
+    ctx.set_field(L.F_EX, Ex)
+    ctx.set_field(L.F_EY, Ey)
+    E32 = np.empty(2 * nx * ny, dtype=np.float32)
+    E32[0::2] = Ex.astype(np.float32)
+    E32[1::2] = Ey.astype(np.float32)
+    V = orc.cell_volume(2, nx, ny, dx, dy)
+    for step in range(3):  # no sort between pushes: windows go stale, fallbacks engage
+        st0 = ctx.get_state(sid)
+        ctx.push(sid, dt)
+        st1 = ctx.get_state(sid)
+        dens = ctx.field(L.F_DENSITY, sid)
+        ms, cnt, live = _mirror_push(orc, 2, st0, E32, nx, nx - 1, ny - 1, q, m, dt, dx, dy,
+                                     (L.BND_DISCARD, L.BND_WRAP))
+        for k in ("cell", "ox", "oy", "vx", "vy"):
+            assert np.array_equal(st1[k].view(np.uint32), ms[k].view(np.uint32)), (step, k)
+        dm = (cnt.astype(np.float64) * FIX) * w0 / V
+        assert np.array_equal(dens, dm), step
+        assert ctx.count(sid) == live
+    # sorting keeps the state (as a set) and compacts discarded particles
+    ctx.sort(sid)
+    st2 = ctx.get_state(sid)
+    assert len(st2["cell"]) == live and np.all(st2["cell"] >= 0)
+    key1 = np.sort(st1["id"][st1["cell"] >= 0])
+    assert np.array_equal(np.sort(st2["id"]), key1)
+
+
+def test_sort_matches_mirror_order(gpu_ready, orc):
+    from iskra_amd import Context, _lib as L
+    rng = np.random.default_rng(3)
+    nx, ny = 129, 67
+    ctx = Context()
+    ctx.grid(2, (nx, ny), (1.0, 1.0))
+    ctx.tuning(tile=8)
+    n = 200000
+    sid = ctx.species(1.0, 1.0, 1.0, n, track_ids=True)
+    cell, ox, oy, vx, vy, vz, ids = _rand_state(rng, n, nx - 1, ny - 1, 1.0, 1.0)
+    cell[rng.random(n) < 0.05] = -1
+    ctx.set_state(sid, cell, ox, oy, vx, vy, vz, ids)
+    st = ctx.get_state(sid)
+    m = dict(cell=cell.copy(), ox=ox.copy(), oy=oy.copy(), vx=vx.copy(), vy=vy.copy(),
+             vz=vz.copy(), id=ids.copy())
+    ntx = (nx - 1 + 7) // 8
+    live = orc.lib().mir_sort_tile(2, n, 8, ntx, orc.ptr(m["cell"]), orc.ptr(m["ox"]),
+                                   orc.ptr(m["oy"]), orc.ptr(m["vx"]), orc.ptr(m["vy"]),
+                                   orc.ptr(m["vz"]), orc.ptr(m["id"]))
+    assert len(st["cell"]) == live
+    for k in st:
+        assert np.array_equal(st[k], m[k][:live]), k
+    # exact cell order: non-decreasing (tile, cell-in-tile)
+    ctx.sort(sid, exact_cell=True)
+    st = ctx.get_state(sid)
+    i, j = st["cell"] & 0xFFFF, st["cell"] >> 16
+    key = ((i >> 3) + ntx * (j >> 3)) * 64 + (i & 7) + 8 * (j & 7)
+    assert np.all(np.diff(key) >= 0)
+
+
+def test_gather_bitexact(gpu_ready, orc):
+    from iskra_amd import Context, _lib as L
+    rng = np.random.default_rng(11)
+    nx, ny = 40, 33
+    ctx = Context()
+    ctx.grid(2, (nx, ny), (0.5, 0.25))
+    sid = ctx.species(1.0, 1.0, 1.0, 10000)
+    st = _rand_state(rng, 10000, nx - 1, ny - 1, 1.0, 1.0)
+    ctx.set_state(sid, *st)
+    Ex, Ey = rng.standard_normal(nx * ny), rng.standard_normal(nx * ny)
+    ctx.set_field(L.F_EX, Ex)
+    ctx.set_field(L.F_EY, Ey)
+    g = ctx.gather(sid)
+    s = ctx.get_state(sid)
+    E32 = np.empty(2 * nx * ny, np.float32)
+    E32[0::2], E32[1::2] = Ex.astype(np.float32), Ey.astype(np.float32)
+    ox_ = np.zeros(len(s["cell"]), np.float32)
+    oy_ = np.zeros(len(s["cell"]), np.float32)
+    orc.lib().mir_gather(2, len(s["cell"]), orc.ptr(s["cell"]), orc.ptr(s["ox"]),
+                         orc.ptr(s["oy"]), orc.ptr(E32), nx, orc.ptr(ox_), orc.ptr(oy_))
+    assert np.array_equal(g[:, 0], ox_.astype(np.float64))
+    assert np.array_equal(g[:, 1], oy_.astype(np.float64))
+    assert np.all(g[:, 2] == 0.0)
+    # against the FP64 reference gather (grid_to_particle): FP32 tolerance
+    X, _, _ = ctx.download(sid)
+    E3 = np.concatenate([Ex, Ey, np.zeros(nx * ny)])
+    pE = orc.grid_to_particle(2, np.ascontiguousarray(X.T).ravel(), nx, ny, 0.5, 0.25, E3)
+    pE = pE.reshape(3, -1).T
+    assert np.allclose(g[:, :2], pE[:, :2], rtol=0, atol=2e-6 * np.abs(E3).max())
+
+
+def test_rng_export_matches_philox(gpu_ready, orc):
+    from iskra_amd import Context
+    ctx = Context()
+    ctx.grid(1, (9,), (1.0,))
+    ctx.rng(0x123456789ABCDEF)
+    ids = np.array([0, 1, 7, 123456, 2 ** 32 - 1], dtype=np.uint32)
+    out = ctx.rng_export(0x4D430000, 17, ids, 3)
+    for a, i in enumerate(ids):
+        for b in range(3):
+            ref = orc.philox4x32([int(i), 17, 0x4D430000, b], [0x89ABCDEF, 0x01234567])
+            assert np.array_equal(out[a, b], ref)
+
+
+@pytest.mark.parametrize("dim", [1, 2])
+def test_maxwellian_loader_bitexact(gpu_ready, orc, dim):
+    from iskra_amd import Context
+    nx, ny = 33, 17
+    ctx = Context()
+    ctx.grid(dim, (nx, ny)[:dim], (0.01, 0.02)[:dim])
+    n = 30000
+    sid = ctx.species(-1.0, 1.0, 1.0, n, track_ids=True)
+    ctx.tuning(sort_every=0)
+    wx = [0.32, 0.32][:dim]
+    wv = [1.5, 2.0, 2.5]
+    dv = [0.1, -0.2, 0.0]
+    ctx.sample_maxwellian(sid, n, wx, wv, dv=dv, seed=42, stream=5)
+    st = ctx.get_state(sid)
+    cell = np.zeros(n, np.int32)
+    ox, oy, vx, vy, vz = (np.zeros(n, np.float32) for _ in range(5))
+    ids = np.zeros(n, np.uint32)
+    orc.lib().mir_sample(dim, n, 0, 0.0, 0.0, 0.01, 0.02 if dim == 2 else 1.0, nx - 1,
+                         ny - 1 if dim == 2 else 1, orc.ptr(np.array(wx + [0.0] * (2 - dim))),
+                         orc.ptr(np.zeros(2)), orc.ptr(np.array(wv)), orc.ptr(np.array(dv)), 42,
+                         5, orc.ptr(cell), orc.ptr(ox), orc.ptr(oy), orc.ptr(vx), orc.ptr(vy),
+                         orc.ptr(vz), orc.ptr(ids))
+    T = 8 if dim == 2 else 64
+    ntx = (nx - 1 + T - 1) // T
+    live = orc.lib().mir_sort_tile(dim, n, T, ntx, orc.ptr(cell), orc.ptr(ox), orc.ptr(oy),
+                                   orc.ptr(vx), orc.ptr(vy), orc.ptr(vz), orc.ptr(ids))
+    assert live == n == len(st["cell"])
+    for k, ref in (("cell", cell), ("ox", ox), ("vx", vx), ("vy", vy), ("vz", vz), ("id", ids)):
+        assert np.array_equal(st[k], ref), k
+    if dim == 2:
+        assert np.array_equal(st["oy"], oy)
+    # moments of the Maxwellian (N(0,1)*wv + dv)
+    assert abs(st["vx"].mean() - 0.1) < 5 * 1.5 / np.sqrt(n)
+    assert abs(st["vy"].std() - 2.0) < 0.05
+
+
+@pytest.mark.parametrize("periodic", [False, True])
+def test_poisson_1d_direct_bitexact(gpu_ready, orc, periodic):
+    from iskra_amd import Context, _lib as L
+    rng = np.random.default_rng(5)
+    n, dx, eps0 = 65, 0.1, 1.0
+    ctx = Context()
+    ctx.grid(1, (n,), (dx,))
+    tags = np.zeros(n, np.int8)
+    if not periodic:
+        tags[0], tags[-1] = 1, 2
+    ctx.poisson(L.POISSON_DIRECT, eps0, tags, periodic_x=periodic)
+    if not periodic:
+        ctx.tag_value(1, 0.0)
+        ctx.tag_value(2, 3.5)
+    rho = rng.standard_normal(n)
+    if periodic:
+        rho -= rho.mean()
+    ctx.set_field(L.F_RHO, rho)
+    ctx.solve()
+    ctx.efield()
+    phi = ctx.field(L.F_PHI)
+    Ex = ctx.field(L.F_EX)
+    a, cp, inv = np.zeros(n), np.zeros(n), np.zeros(n)
+    sing = int(periodic)
+    orc.lib().mir_tri_factor(n, orc.ptr(tags), sing, orc.ptr(a), orc.ptr(cp), orc.ptr(inv))
+    tv = np.zeros(17)
+    tv[2] = 3.5
+    pm = np.zeros(n)
+    orc.lib().mir_tri_solve(n, orc.ptr(rho), orc.ptr(tags), orc.ptr(tv), orc.ptr(a),
+                            orc.ptr(cp), orc.ptr(inv), dx * dx / eps0, sing, orc.ptr(pm))
+    assert np.array_equal(phi, pm)
+    Em = np.zeros(n, np.float32)
+    orc.lib().mir_efield(1, n, 1, dx, 1.0, orc.ptr(pm), orc.ptr(Em))
+    assert np.array_equal(Ex, Em.astype(np.float64))
+    # FP64 reference: dense A\b on the reference matrix
+    A, b, isdof = orc.poisson_matrix(1, n, 1, dx, 1.0, periodic=[1] if periodic else [],
+                                     dirichlet=[] if periodic else
+                                     [(tags == 1, 0.0), (tags == 2, 3.5)])
+    if periodic:  # singular: compare the mean-free solution
+        x = np.linalg.lstsq(A.reshape(n, n, order="F"), -rho / eps0, rcond=None)[0]
+        x -= x.mean()
+        assert np.allclose(phi, x, atol=1e-9 * np.abs(x).max())
+    else:
+        x, sing = orc.potential(A, b, isdof, rho, eps0)
+        assert sing == 0
+        assert np.allclose(phi, x, rtol=1e-12, atol=1e-12 * np.abs(x).max())
+
+
+@pytest.mark.parametrize("kind", [1, 2, 3])
+def test_poisson_2d_iterative_vs_direct(gpu_ready, orc, kind):
+    from iskra_amd import Context, _lib as L
+    rng = np.random.default_rng(kind)
+    nx = ny = 33
+    dx = dy = 0.01
+    eps0 = 8.854e-12
+    ctx = Context()
+    ctx.grid(2, (nx, ny), (dx, dy))
+    tags = np.zeros((ny, nx), np.int8)
+    tags[0, :] = tags[-1, :] = tags[:, 0] = tags[:, -1] = 1
+    tags[:, -1] = 2
+    ctx.poisson(kind, eps0, tags.ravel(), rtol=1e-10)
+    ctx.tag_value(2, 10.0)
+    rho = rng.standard_normal(nx * ny) * 1e-9
+    ctx.set_field(L.F_RHO, rho)
+    it, rr = ctx.solve()
+    assert rr <= 1e-10
+    phi = ctx.field(L.F_PHI)
+    mask1 = (tags == 1).ravel()
+    mask2 = (tags == 2).ravel()
+    # node k = i + nx j  ==  tags.ravel() (row j, column i)
+    A, b, isdof = orc.poisson_matrix(2, nx, ny, dx, dy,
+                                     dirichlet=[(mask1, 0.0), (mask2, 10.0)])
+    x, sing = orc.potential(A, b, isdof, rho, eps0)
+    assert sing == 0
+    assert np.max(np.abs(phi - x)) <= 1e-7 * np.max(np.abs(x))
+    ctx.efield()
+    Ex, Ey = ctx.field(L.F_EX), ctx.field(L.F_EY)
+    E = orc.efield(2, nx, ny, dx, dy, x).reshape(3, -1)
+    scale = np.abs(E).max()
+    assert np.max(np.abs(Ex - E[0])) <= 2e-6 * scale
+    assert np.max(np.abs(Ey - E[1])) <= 2e-6 * scale
+
+
+def test_capacitor_kat(gpu_ready):
+    """FiniteDifferenceMethod/test/runtests.jl:8-22: grid 0:0.5:1 x 0:0.5:2,
+    phi = 0 on the first column of nodes, 1 on the last -> Ex = -1, Ey = 0."""
+    from iskra_amd import Context, _lib as L
+    ctx = Context()
+    ctx.grid(2, (3, 5), (0.5, 0.5))
+    tags = np.zeros((5, 3), np.int8)
+    tags[:, 0] = 1
+    tags[:, 2] = 2
+    ctx.poisson(L.POISSON_PCG, 1.0, tags.ravel(), rtol=1e-14)
+    ctx.tag_value(1, 0.0)
+    ctx.tag_value(2, 1.0)
+    ctx.set_field(L.F_RHO, np.zeros(15))
+    ctx.solve()
+    ctx.efield()
+    assert np.allclose(ctx.field(L.F_EX), -1.0, atol=1e-7)
+    assert np.allclose(ctx.field(L.F_EY), 0.0, atol=1e-7)
