@@ -1,0 +1,220 @@
+#!/usr/bin/env python3
+"""Benchmark of the electrostatic PIC step on one MI355X.
+
+Workload (the 2-D configuration the headline metric is quoted on): a 2-D plasma
+slab, 256 x 256 cells (257^2 nodes), Dirichlet phi = 0 walls, 2^24 electrons +
+2^24 ions (m_i = 1836 m_e), Te = 2 eV / Ti = 0.03 eV Maxwellians, absorbing
+walls, Debye length = cell size, omega_pe dt = 0.1.  One step = gather + push +
+boundary + CIC deposit of both species, deposit merge, multigrid Poisson solve
+to relative residual 1e-8 (FP64 residual), E-field, and the counting sort every
+`sort_every` steps.  Inputs are generated on the device (Philox loader), so
+they are resident in HBM before the timed region.
+
+Prints one JSON line (see the keys below).  --gpus above 1 exits: this build
+runs on one GPU.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+METRIC = ("particle-steps/s + % peak HBM; Poisson solve ms & iters at 1e-8 residual, "
+          "learned vs plain")
+PEAK_HBM_GBS = 8000.0
+BYTES_PER_PSTEP_2D = 40  # read+write cell, ox, oy, vx, vy (4 B each)
+
+
+def cpu_baseline(cells, budget_s=12.0, sample=1 << 18):
+    """The oracle's FP64 restatement of the reference's per-step particle work
+    (grid_to_particle, push_in_cartesian! with B = 0, discard!, particle_to_grid)
+    on one host thread, for a bounded sample of electrons on the same grid."""
+    import oracle as O
+    from iskra_amd import problems as P
+    p = P.slab_2d(cells=cells)
+    nx = cells + 1
+    dx = p["dx"]
+    rng = np.random.default_rng(0)
+    L = cells * dx
+    x = np.ascontiguousarray(np.concatenate([rng.random(sample) * L, rng.random(sample) * L]))
+    v = np.ascontiguousarray((rng.standard_normal(3 * sample) * p["ve"]))
+    E = np.ascontiguousarray(rng.standard_normal(3 * nx * nx) * 10.0)
+    E[2 * nx * nx:] = 0.0
+    B = np.zeros(3 * sample)
+    wg = np.full(sample, p["w0"])
+    ids = np.arange(sample, dtype=np.uint32)
+    n = np.array([nx, nx], dtype=np.int64)
+    dh = np.array([dx, dx])
+    org = np.zeros(2)
+    dims = np.array([0, 1], dtype=np.int32)
+    lib = O.lib()
+    pE = np.zeros(3 * sample)
+    u = np.zeros(nx * nx)
+    qm = -P.QE / P.ME
+    steps = 0
+    psteps = 0
+    npart = np.array([sample], dtype=np.int64)
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < budget_s and steps < 200:
+        m = int(npart[0])
+        lib.ref_grid_to_particle(2, m, O.ptr(x), nx, nx, dx, dx, O.ptr(E), O.ptr(pE))
+        lib.ref_push(m, 2, O.ptr(x), O.ptr(v), O.ptr(pE), O.ptr(B), qm, p["dt"])
+        lib.ref_discard(O.ptr(npart), m, 2, O.ptr(x), O.ptr(v), O.ptr(wg), O.ptr(ids), p["w0"],
+                        O.ptr(n), O.ptr(dh), O.ptr(org), 2, O.ptr(dims))
+        # remaining particles keep leading dimension m; repack for the next step
+        k = int(npart[0])
+        if k != m:
+            x = np.ascontiguousarray(np.concatenate([x[:k], x[m:m + k]]))
+            v = np.ascontiguousarray(np.concatenate([v[:k], v[m:m + k], v[2 * m:2 * m + k]]))
+            wg = wg[:k].copy()
+            ids = ids[:k].copy()
+            pE = np.zeros(3 * k)
+            B = np.zeros(3 * k)
+        lib.ref_particle_to_grid(2, k, O.ptr(x), O.ptr(wg), nx, nx, dx, dx, O.ptr(u))
+        steps += 1
+        psteps += m
+    dt = time.perf_counter() - t0
+    return dict(value=psteps / dt, unit="particle-steps/s", cores=1, kind="port",
+                sample=(f"{sample} electrons on the {cells}x{cells}-cell grid, {steps} steps of "
+                        f"the FP64 reference algorithm (gather, push B=0, discard, CIC deposit), "
+                        f"1 host thread, {dt:.1f} s; Poisson excluded (the reference's dense "
+                        f"A\\b at 257^2 nodes is infeasible)"))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--cells", type=int, default=256)
+    ap.add_argument("--ppc", type=int, default=256)
+    ap.add_argument("--solver", default="mg", choices=["mg", "mgpcg", "pcg"])
+    ap.add_argument("--tile", type=int, default=0)
+    ap.add_argument("--chunk", type=int, default=0)
+    ap.add_argument("--margin", type=int, default=0)
+    ap.add_argument("--sort-every", type=int, default=-1)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-profile", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              f"(the multi-GPU decomposition is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    from iskra_amd import Context, _lib as L, problems as P
+
+    p = P.slab_2d(cells=args.cells, ppc=args.ppc)
+    nodes = p["nodes"]
+    n_species = p["n_part"]
+    t_setup = time.perf_counter()
+    ctx = Context(0)
+    ctx.grid(2, (nodes, nodes), (p["dx"], p["dx"]))
+    ctx.tuning(tile=args.tile, chunk=args.chunk, margin=args.margin,
+               sort_every=args.sort_every)
+    tags = np.zeros((nodes, nodes), np.int8)
+    tags[0, :] = tags[-1, :] = tags[:, 0] = tags[:, -1] = 1
+    kind = dict(mg=L.POISSON_MG, mgpcg=L.POISSON_MGPCG, pcg=L.POISSON_PCG)[args.solver]
+    ctx.poisson(kind, p["eps0"], tags.ravel(), rtol=1e-8)
+    ctx.tag_value(1, 0.0)
+    cap = n_species + 1024
+    e = ctx.species(-p["q"], p["me"], p["w0"], cap, track_ids=False)
+    i = ctx.species(p["q"], p["mi"], p["w0"], cap, track_ids=False)
+    ctx.boundary(e, [L.BND_DISCARD, L.BND_DISCARD])
+    ctx.boundary(i, [L.BND_DISCARD, L.BND_DISCARD])
+    Lx = p["L"]
+    ctx.sample_maxwellian(e, n_species, [Lx, Lx], [p["ve"]] * 3, seed=1, stream=0)
+    ctx.sample_maxwellian(i, n_species, [Lx, Lx], [p["vi"]] * 3, seed=1, stream=1)
+    ctx.sync()
+    t_setup = time.perf_counter() - t_setup
+
+    dt = p["dt"]
+    if args.warmup:
+        ctx.step(dt, args.warmup)
+    # timed region: exactly K steps, synchronised on both sides
+    ctx.sync()
+    s0 = ctx.stats()
+    t0 = time.perf_counter()
+    ctx.step(dt, args.steps)
+    ctx.sync()
+    t1 = time.perf_counter()
+    s1 = ctx.stats()
+    elapsed = t1 - t0
+    psteps = s1["particle_steps"] - s0["particle_steps"]
+    value = psteps / elapsed
+    solves = s1["poisson_solves"] - s0["poisson_solves"]
+    iters = s1["poisson_iters"] - s0["poisson_iters"]
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * elapsed / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (device Philox Maxwellian loader)",
+        "config": {
+            "workload": f"2-D ES-PIC slab {args.cells}x{args.cells} cells, "
+                        f"{n_species} e- + {n_species} ions, Dirichlet walls, "
+                        f"{args.solver.upper()} Poisson to 1e-8",
+            "cells": args.cells,
+            "particles_per_species": n_species,
+            "poisson": args.solver,
+            "tuning": ctx.get_tuning(),
+        },
+        "poisson": {"solver": args.solver, "iters_per_solve": iters / max(solves, 1),
+                    "rtol": 1e-8},
+        "setup_s": t_setup,
+    }
+
+    if not args.no_profile:
+        # profiled pass of the same K steps: per-kernel HIP-event durations
+        ctx.profile(True)
+        ctx.step(dt, args.steps)
+        sp = ctx.stats()
+        ctx.profile(False)
+        push_ms = sp["kernel_ms"]["push"]
+        push_calls = sp["kernel_calls"]["push"]
+        pp = sp["push_particles"]
+        bytes_total = BYTES_PER_PSTEP_2D * pp
+        achieved = bytes_total / (push_ms * 1e-3) / 1e9
+        traffic = None
+        tf = os.path.join(ROOT, "profiles", "traffic_push2d.json")
+        if os.path.exists(tf):
+            with open(tf) as f:
+                t = json.load(f)
+            if t.get("cells") == args.cells and t.get("particles_per_species") == n_species:
+                traffic = t.get("hbm_bytes_per_launch")
+        out["roofline"] = {
+            "bound": "hbm", "kernel": "k_push<2> (fused gather+push+boundary+deposit)",
+            "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+            "frac": achieved / PEAK_HBM_GBS, "traffic": traffic,
+            "bytes_per_particle_step": BYTES_PER_PSTEP_2D,
+            "avg_launch_ms": push_ms / max(push_calls, 1),
+            "algorithmic_bytes_per_launch": bytes_total / max(push_calls, 1),
+        }
+        out["kernel_ms_per_step"] = {k: v / args.steps for k, v in sp["kernel_ms"].items()}
+        out["poisson"]["ms_per_solve"] = sp["kernel_ms"]["poisson"] / max(
+            sp["kernel_calls"]["poisson"], 1)
+        nbytes = 1 << 30
+        cms = ctx.bench_copy(nbytes, 20)
+        out["hbm_copy_gbps"] = 2 * nbytes / (cms * 1e-3) / 1e9
+        out["roofline"]["frac_of_copy"] = achieved / out["hbm_copy_gbps"]
+
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(args.cells)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -234,6 +234,17 @@ int iskra_rng_export(iskra_ctx *ctx, uint32_t stream, uint32_t step,
  * deposit, Poisson, E; tag waveforms are applied after each solve. */
 int iskra_step(iskra_ctx *ctx, double dt, int64_t nsteps);
 
+/* Kernel classes timed when profiling is on (HIP events recorded on the
+ * context's stream around each launch). */
+enum {
+  ISKRA_K_PUSH = 0,    /* fused gather + push + boundary + deposit */
+  ISKRA_K_SORT = 1,    /* counting sort (all passes) */
+  ISKRA_K_POISSON = 2, /* one Poisson solve (all kernels) */
+  ISKRA_K_MCC = 3,     /* collisions + product append */
+  ISKRA_K_MERGE = 4,   /* window merge + rho */
+  ISKRA_K_EFIELD = 5,
+  ISKRA_K_COUNT = 8
+};
 typedef struct {
   int64_t steps;            /* steps taken by iskra_step */
   int64_t particle_steps;   /* sum over steps and species of live particles pushed */
@@ -242,8 +253,21 @@ typedef struct {
   int32_t last_iters;
   int32_t pad;
   double last_relres;
+  double kernel_ms[ISKRA_K_COUNT];     /* summed durations while profiling */
+  int64_t kernel_calls[ISKRA_K_COUNT]; /* launches timed */
+  int64_t push_particles;   /* particles handled by the timed push launches */
 } iskra_stats;
 int iskra_stats_get(iskra_ctx *ctx, iskra_stats *out);
+/* Turn per-kernel timing on (1) or off (0); turning it on clears the totals.
+ * No reference counterpart (the reference only @time's the whole solve,
+ * src/iskra.jl:51). */
+int iskra_profile(iskra_ctx *ctx, int enable);
+
+/* ---- diagnostics ---------------------------------------------------------- */
+/* Device-to-device copy of `bytes` bytes, `iters` times, with the library's
+ * own streaming copy kernel; *ms = mean duration of one copy (HIP events).
+ * Measures the achievable HBM bandwidth the roofline is reported against. */
+int iskra_bench_copy(iskra_ctx *ctx, int64_t bytes, int iters, double *ms);
 
 #ifdef __cplusplus
 }

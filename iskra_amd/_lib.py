@@ -29,7 +29,7 @@ EXPORTS = [
     "iskra_field_upload", "iskra_poisson_set", "iskra_poisson_set_tag_value",
     "iskra_poisson_set_tag_waveform", "iskra_poisson_solve", "iskra_efield", "iskra_mcc_add",
     "iskra_mcc_run", "iskra_mcc_counts", "iskra_rng_set", "iskra_rng_export", "iskra_step",
-    "iskra_stats_get",
+    "iskra_stats_get", "iskra_profile", "iskra_bench_copy",
 ]
 
 
@@ -44,10 +44,16 @@ class Tuning(C.Structure):
                 ("sort_every", C.c_int32)]
 
 
+K_PUSH, K_SORT, K_POISSON, K_MCC, K_MERGE, K_EFIELD = range(6)
+K_NAMES = ["push", "sort", "poisson", "mcc", "merge", "efield", "k6", "k7"]
+
+
 class Stats(C.Structure):
     _fields_ = [("steps", C.c_int64), ("particle_steps", C.c_int64),
                 ("poisson_solves", C.c_int64), ("poisson_iters", C.c_int64),
-                ("last_iters", C.c_int32), ("pad", C.c_int32), ("last_relres", C.c_double)]
+                ("last_iters", C.c_int32), ("pad", C.c_int32), ("last_relres", C.c_double),
+                ("kernel_ms", C.c_double * 8), ("kernel_calls", C.c_int64 * 8),
+                ("push_particles", C.c_int64)]
 
 
 _lib = None
@@ -98,6 +104,8 @@ def lib():
             "iskra_rng_export": (I, [P, U32, U32, P, I64, I, P]),
             "iskra_step": (I, [P, D, I64]),
             "iskra_stats_get": (I, [P, C.POINTER(Stats)]),
+            "iskra_profile": (I, [P, I]),
+            "iskra_bench_copy": (I, [P, I64, I, C.POINTER(D)]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -342,7 +350,18 @@ class Context:
         check(lib().iskra_stats_get(self.h, C.byref(s)))
         return dict(steps=s.steps, particle_steps=s.particle_steps,
                     poisson_solves=s.poisson_solves, poisson_iters=s.poisson_iters,
-                    last_iters=s.last_iters, last_relres=s.last_relres)
+                    last_iters=s.last_iters, last_relres=s.last_relres,
+                    kernel_ms={K_NAMES[k]: s.kernel_ms[k] for k in range(6)},
+                    kernel_calls={K_NAMES[k]: s.kernel_calls[k] for k in range(6)},
+                    push_particles=s.push_particles)
+
+    def profile(self, enable=True):
+        check(lib().iskra_profile(self.h, int(enable)))
+
+    def bench_copy(self, nbytes, iters=20):
+        ms = C.c_double()
+        check(lib().iskra_bench_copy(self.h, int(nbytes), int(iters), C.byref(ms)))
+        return ms.value
 
     def sync(self):
         check(lib().iskra_sync(self.h))

@@ -100,12 +100,14 @@ void isk_species_release(IskSpecies *s) {
   isk_free(s->chunks);
   isk_free(s->tile_chunk_begin);
   isk_free(s->partial);
+  isk_free(s->live_cnt);
   isk_free(s->gfall);
   isk_free(s->cnt);
   s->hdr = nullptr;
   s->chunks = nullptr;
   s->tile_chunk_begin = nullptr;
   s->partial = nullptr;
+  s->live_cnt = nullptr;
   s->gfall = nullptr;
   s->cnt = nullptr;
 }
@@ -114,14 +116,17 @@ static int alloc_chunk_tables(iskra_ctx *c, IskSpecies *s) {
   isk_free(s->chunks);
   isk_free(s->tile_chunk_begin);
   isk_free(s->partial);
+  isk_free(s->live_cnt);
   s->chunks = nullptr;
   s->tile_chunk_begin = nullptr;
   s->partial = nullptr;
+  s->live_cnt = nullptr;
   int WW = c->g.dim == 2 ? c->t.W * c->t.W : c->t.W;
   s->max_chunks = (s->cap + c->t.CH - 1) / c->t.CH + c->ntiles + 1;
   if (isk_alloc((void **)&s->chunks, (size_t)s->max_chunks * sizeof(IskChunk)) ||
       isk_alloc((void **)&s->tile_chunk_begin, (size_t)(c->ntiles + 1) * 4) ||
-      isk_alloc((void **)&s->partial, (size_t)s->max_chunks * WW * 8))
+      isk_alloc((void **)&s->partial, (size_t)s->max_chunks * WW * 8) ||
+      isk_alloc((void **)&s->live_cnt, (size_t)s->max_chunks * 4))
     return ISKRA_E_HIP;
   return ISKRA_OK;
 }
@@ -150,6 +155,86 @@ int isk_species_reset_chunks(iskra_ctx *c, int sid) {
   IskSpecies *s = &c->sp[sid];
   RC(alloc_chunk_tables(c, s));
   return isk_sort_species(c, sid, 0);
+}
+
+/* ---- profiling ---- */
+static int64_t g_prof_ps0 = 0;
+
+int isk_prof_begin(iskra_ctx *c, hipEvent_t *a) {
+  *a = nullptr;
+  if (!c->prof) return ISKRA_OK;
+  if (c->prof_pool.empty()) {
+    hipEvent_t e;
+    ISK_HIP(hipEventCreate(&e));
+    c->prof_pool.push_back(e);
+  }
+  *a = c->prof_pool.back();
+  c->prof_pool.pop_back();
+  ISK_HIP(hipEventRecord(*a, c->stream));
+  return ISKRA_OK;
+}
+
+int isk_prof_end(iskra_ctx *c, hipEvent_t a, int kind, int64_t particles) {
+  if (!c->prof || !a) return ISKRA_OK;
+  if (c->prof_pool.empty()) {
+    hipEvent_t e;
+    ISK_HIP(hipEventCreate(&e));
+    c->prof_pool.push_back(e);
+  }
+  hipEvent_t b = c->prof_pool.back();
+  c->prof_pool.pop_back();
+  ISK_HIP(hipEventRecord(b, c->stream));
+  iskra_ctx::Pending pd;
+  pd.a = a;
+  pd.b = b;
+  pd.kind = kind;
+  pd.particles = particles;
+  c->prof_pending.push_back(pd);
+  return ISKRA_OK;
+}
+
+static int prof_resolve(iskra_ctx *c) {
+  for (auto &pd : c->prof_pending) {
+    ISK_HIP(hipEventSynchronize(pd.b));
+    float ms = 0.f;
+    ISK_HIP(hipEventElapsedTime(&ms, pd.a, pd.b));
+    if (pd.kind >= 0 && pd.kind < ISKRA_K_COUNT) {
+      c->stats.kernel_ms[pd.kind] += ms;
+      c->stats.kernel_calls[pd.kind] += 1;
+    }
+    c->prof_pool.push_back(pd.a);
+    c->prof_pool.push_back(pd.b);
+  }
+  c->prof_pending.clear();
+  return ISKRA_OK;
+}
+
+static void prof_release(iskra_ctx *c) {
+  for (auto &pd : c->prof_pending) {
+    (void)hipEventDestroy(pd.a);
+    (void)hipEventDestroy(pd.b);
+  }
+  c->prof_pending.clear();
+  for (auto e : c->prof_pool) (void)hipEventDestroy(e);
+  c->prof_pool.clear();
+}
+
+typedef float isk_f4 __attribute__((ext_vector_type(4)));
+/* streaming copy: each work-group moves 4 x 256 x 16 B (16 KiB), loads first */
+__global__ __launch_bounds__(256) void k_copy(const isk_f4 *__restrict__ src,
+                                              isk_f4 *__restrict__ dst, long long n) {
+  long long base = (long long)blockIdx.x * 1024 + threadIdx.x;
+  isk_f4 v[4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    long long i = base + u * 256;
+    if (i < n) v[u] = __builtin_nontemporal_load(&src[i]);
+  }
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    long long i = base + u * 256;
+    if (i < n) __builtin_nontemporal_store(v[u], &dst[i]);
+  }
 }
 
 extern "C" {
@@ -227,6 +312,7 @@ int iskra_ctx_destroy(iskra_ctx *c) {
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
   release_fields(c);
+  prof_release(c);
   (void)hipStreamDestroy(c->stream);
   delete c;
   return ISKRA_OK;
@@ -995,13 +1081,24 @@ int iskra_step(iskra_ctx *c, double dt, int64_t nsteps) {
     return ISKRA_E_ARG;
   }
   for (int64_t it = 0; it < nsteps; ++it) {
-    for (int m = 0; m < c->nmcc; ++m) RC(isk_mcc_run(c, m, dt, nullptr));
+    hipEvent_t ev = nullptr;
+    if (c->nmcc) {
+      RC(isk_prof_begin(c, &ev));
+      for (int m = 0; m < c->nmcc; ++m) RC(isk_mcc_run(c, m, dt, nullptr));
+      RC(isk_prof_end(c, ev, ISKRA_K_MCC, 0));
+    }
     for (int s = 0; s < c->nspecies; ++s) RC(isk_push_species(c, s, dt, 1));
+    RC(isk_prof_begin(c, &ev));
     RC(isk_rho(c));
+    RC(isk_prof_end(c, ev, ISKRA_K_MERGE, 0));
     int iters = 0;
     double rr = 0.0;
+    RC(isk_prof_begin(c, &ev));
     RC(isk_poisson_solve(c, ISKRA_WARM_PREV, &iters, &rr));
+    RC(isk_prof_end(c, ev, ISKRA_K_POISSON, 0));
+    RC(isk_prof_begin(c, &ev));
     RC(isk_efield(c));
+    RC(isk_prof_end(c, ev, ISKRA_K_EFIELD, 0));
     /* after_loop: electrode values for the next step at t = (it-1) dt */
     double t = (double)c->step_count * dt;
     for (int k = 1; k <= ISK_MAX_TAGS; ++k)
@@ -1022,6 +1119,7 @@ int iskra_stats_get(iskra_ctx *c, iskra_stats *out) {
   CHECK_CTX(c);
   if (!out) return ISKRA_E_ARG;
   ISK_HIP(hipStreamSynchronize(c->stream));
+  RC(prof_resolve(c));
   int64_t ps = 0;
   for (int s = 0; s < c->nspecies; ++s) {
     long long v = 0;
@@ -1029,7 +1127,57 @@ int iskra_stats_get(iskra_ctx *c, iskra_stats *out) {
     ps += v;
   }
   c->stats.particle_steps = ps;
+  c->stats.push_particles = c->prof ? ps - g_prof_ps0 : 0;
   *out = c->stats;
+  return ISKRA_OK;
+}
+
+int iskra_profile(iskra_ctx *c, int enable) {
+  CHECK_CTX(c);
+  ISK_HIP(hipStreamSynchronize(c->stream));
+  RC(prof_resolve(c));
+  c->prof = enable ? 1 : 0;
+  for (int k = 0; k < ISKRA_K_COUNT; ++k) {
+    c->stats.kernel_ms[k] = 0.0;
+    c->stats.kernel_calls[k] = 0;
+  }
+  int64_t ps = 0;
+  for (int s = 0; s < c->nspecies; ++s) {
+    long long v = 0;
+    ISK_HIP(hipMemcpy(&v, c->sp[s].hdr + HDR_PUSHED, 8, hipMemcpyDeviceToHost));
+    ps += v;
+  }
+  g_prof_ps0 = ps;
+  return ISKRA_OK;
+}
+
+int iskra_bench_copy(iskra_ctx *c, int64_t bytes, int iters, double *ms) {
+  CHECK_CTX(c);
+  if (bytes < 16 || iters < 1 || !ms) return ISKRA_E_ARG;
+  long long n = bytes / 16;
+  void *a = nullptr, *b = nullptr;
+  RC(isk_alloc(&a, (size_t)n * 16));
+  RC(isk_alloc(&b, (size_t)n * 16));
+  ISK_HIP(hipMemsetAsync(a, 1, (size_t)n * 16, c->stream));
+  long long blocks = (n + 1023) / 1024;
+  hipLaunchKernelGGL(k_copy, dim3((unsigned)blocks), dim3(256), 0, c->stream, (const isk_f4 *)a, (isk_f4 *)b, n);
+  ISK_CHECK_LAUNCH();
+  hipEvent_t e0, e1;
+  ISK_HIP(hipEventCreate(&e0));
+  ISK_HIP(hipEventCreate(&e1));
+  ISK_HIP(hipEventRecord(e0, c->stream));
+  for (int k = 0; k < iters; ++k)
+    hipLaunchKernelGGL(k_copy, dim3((unsigned)blocks), dim3(256), 0, c->stream, (const isk_f4 *)a, (isk_f4 *)b, n);
+  ISK_CHECK_LAUNCH();
+  ISK_HIP(hipEventRecord(e1, c->stream));
+  ISK_HIP(hipEventSynchronize(e1));
+  float t = 0.f;
+  ISK_HIP(hipEventElapsedTime(&t, e0, e1));
+  *ms = (double)t / iters;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  isk_free(a);
+  isk_free(b);
   return ISKRA_OK;
 }
 

@@ -85,6 +85,7 @@ struct IskSpecies {
   long long *partial;        /* max_chunks * W^dim */
   unsigned long long *gfall; /* nodes */
   long long *cnt;            /* nodes: merged deposit (fixed point) */
+  int *live_cnt;             /* max_chunks: live particles per push work-group */
   int64_t max_chunks;
   int deposited;             /* cnt holds this step's deposit */
 };
@@ -137,7 +138,8 @@ struct IskPoisson {
   double *x = nullptr, *r = nullptr, *Ap = nullptr;
   float *z = nullptr, *p[2] = {nullptr, nullptr};
   double *scal = nullptr;      /* scalars, see poisson.hip */
-  double *partials = nullptr;  /* per-block partial sums */
+  double *partials = nullptr;  /* per-block partial sums (2 per block) */
+  int partials_blocks = 0;     /* blocks the partials array can hold */
   unsigned int *counter = nullptr;
   int *iflag = nullptr;        /* [0] done, [1] iterations */
   /* multigrid */
@@ -175,7 +177,20 @@ struct iskra_ctx {
   int64_t step_count = 0;
   iskra_stats stats;
   double t_now = 0.0;
+  /* profiling: event pairs recorded around launches, resolved at stats time */
+  int prof = 0;
+  struct Pending {
+    hipEvent_t a, b;
+    int kind;
+    int64_t particles;
+  };
+  std::vector<Pending> prof_pending;
+  std::vector<hipEvent_t> prof_pool;
 };
+
+/* bracket a launch with events when profiling (api.hip) */
+int isk_prof_begin(iskra_ctx *c, hipEvent_t *a);
+int isk_prof_end(iskra_ctx *c, hipEvent_t a, int kind, int64_t particles);
 
 /* ---- error plumbing (api.hip) ---- */
 void isk_set_error(const char *fmt, ...);
@@ -234,7 +249,7 @@ int isk_species_ok(iskra_ctx *c, int sid);
 int isk_fill_i64(iskra_ctx *c, void *p, int64_t n, long long v);
 int isk_sort_species(iskra_ctx *c, int sid, int exact_cell);
 int isk_push_species(iskra_ctx *c, int sid, double dt, int do_push);
-int isk_merge_species(iskra_ctx *c, int sid);
+int isk_merge_species(iskra_ctx *c, int sid, int count_live);
 int isk_rho(iskra_ctx *c);
 int isk_gather_species(iskra_ctx *c, int sid);
 int isk_convert_upload(iskra_ctx *c, int sid, int64_t np, const double *x,

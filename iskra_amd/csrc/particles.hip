@@ -31,7 +31,7 @@
 #include "isk_internal.h"
 
 #define PUSH_BLOCK 256
-#define PUSH_UNROLL 2
+#define PUSH_UNROLL 4
 #define SORT_BLOCK 256
 #define SORT_PT 32
 #define SORT_CHUNK (SORT_BLOCK * SORT_PT)
@@ -564,10 +564,13 @@ static int sort_pass(iskra_ctx *c, int sid, int mode, int build) {
 
 int isk_sort_species(iskra_ctx *c, int sid, int exact_cell) {
   int rc;
+  hipEvent_t ev = nullptr;
+  if ((rc = isk_prof_begin(c, &ev))) return rc;
   if (exact_cell) {
     if ((rc = sort_pass(c, sid, 1, 0))) return rc;
   }
-  return sort_pass(c, sid, 0, 1);
+  if ((rc = sort_pass(c, sid, 0, 1))) return rc;
+  return isk_prof_end(c, ev, ISKRA_K_SORT, 0);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -586,6 +589,7 @@ struct PushArgs {
   float qmdt, dtdx, dtdy;
   int bndx, bndy;
   int do_push;
+  int *live_cnt; /* per work-group live particles pushed */
 };
 
 __device__ __forceinline__ void fix_weights2(float hx, float hy, long long *w00,
@@ -627,11 +631,44 @@ __device__ __forceinline__ bool apply_bnd(int *i, int n, int mode) {
 }
 
 template <int DIM>
+struct PushRegs {
+  int32_t c[PUSH_UNROLL];
+  float hx[PUSH_UNROLL], hy[PUSH_UNROLL], vx[PUSH_UNROLL], vy[PUSH_UNROLL];
+};
+
+template <int DIM>
+__device__ __forceinline__ void push_load(PushRegs<DIM> &r, const int32_t *__restrict__ cellp,
+                                          const float *__restrict__ oxp,
+                                          const float *__restrict__ oyp,
+                                          const float *__restrict__ vxp,
+                                          const float *__restrict__ vyp, long long start,
+                                          long long count, long long b0) {
+  /* branch-free: slots past the end load particle `start` and are marked -2 */
+#pragma unroll
+  for (int u = 0; u < PUSH_UNROLL; ++u) {
+    long long k = b0 + u * PUSH_BLOCK + threadIdx.x;
+    bool in = k < count;
+    long long p = start + (in ? k : 0);
+    int32_t c = cellp[p];
+    r.c[u] = in ? c : -2;
+    r.hx[u] = oxp[p];
+    r.vx[u] = vxp[p];
+    r.hy[u] = DIM == 2 ? oyp[p] : 0.f;
+    r.vy[u] = DIM == 2 ? vyp[p] : 0.f;
+  }
+}
+
+template <int DIM>
 __global__ __launch_bounds__(PUSH_BLOCK) void k_push(PushArgs a) {
   extern __shared__ unsigned long long sD[]; /* WW deposit counters, then E */
   const int W = a.W;
   const int WW = DIM == 2 ? W * W : W;
   float *sE = reinterpret_cast<float *>(sD + WW);
+  int32_t *__restrict__ cellp = a.cell;
+  float *__restrict__ oxp = a.ox;
+  float *__restrict__ oyp = a.oy;
+  float *__restrict__ vxp = a.vx;
+  float *__restrict__ vyp = a.vy;
   const long long nch = a.hdr[HDR_NCHUNKS];
   const long long np = a.hdr[HDR_NP];
   const long long nps = a.hdr[HDR_NP_SORTED];
@@ -652,7 +689,10 @@ __global__ __launch_bounds__(PUSH_BLOCK) void k_push(PushArgs a) {
     }
   } else {
     start = nps + ((long long)blockIdx.x - nch) * a.CH;
-    if (start >= np) return;
+    if (start >= np) {
+      if (threadIdx.x == 0) a.live_cnt[blockIdx.x] = 0;
+      return;
+    }
     count = min((long long)a.CH, np - start);
     windowed = false;
   }
@@ -677,47 +717,29 @@ __global__ __launch_bounds__(PUSH_BLOCK) void k_push(PushArgs a) {
     __syncthreads();
   }
   int live = 0;
-  for (long long b0 = 0; b0 < count; b0 += PUSH_BLOCK * PUSH_UNROLL) {
-    int32_t c[PUSH_UNROLL];
-    float hx[PUSH_UNROLL], hy[PUSH_UNROLL], vx[PUSH_UNROLL], vy[PUSH_UNROLL];
-    long long pidx[PUSH_UNROLL];
+  const long long per = (long long)PUSH_BLOCK * PUSH_UNROLL;
+  for (long long b0 = 0; b0 < count; b0 += per) {
+    /* a batch of PUSH_UNROLL particles per thread: all loads, then compute + stores */
+    PushRegs<DIM> cur;
+    push_load<DIM>(cur, cellp, oxp, oyp, vxp, vyp, start, count, b0);
 #pragma unroll
     for (int u = 0; u < PUSH_UNROLL; ++u) {
-      long long k = b0 + u * PUSH_BLOCK + threadIdx.x;
-      pidx[u] = k < count ? start + k : -1;
-      if (pidx[u] >= 0) {
-        c[u] = a.cell[pidx[u]];
-        hx[u] = a.ox[pidx[u]];
-        vx[u] = a.vx[pidx[u]];
-        if (DIM == 2) {
-          hy[u] = a.oy[pidx[u]];
-          vy[u] = a.vy[pidx[u]];
-        } else {
-          hy[u] = 0.f;
-          vy[u] = 0.f;
-        }
-      } else {
-        c[u] = -1;
-        hx[u] = hy[u] = vx[u] = vy[u] = 0.f;
-      }
-    }
-#pragma unroll
-    for (int u = 0; u < PUSH_UNROLL; ++u) {
-      if (c[u] < 0) continue;
-      int i = c[u] & 0xFFFF;
-      int j = c[u] >> 16;
+      int32_t c = cur.c[u];
+      if (c < 0) continue;
+      int i = c & 0xFFFF;
+      int j = c >> 16;
+      float hx = cur.hx[u], hy = cur.hy[u], vx = cur.vx[u], vy = cur.vy[u];
       bool alive = true;
       if (a.do_push) {
         /* gather (cloud_in_cell.jl:20-36) */
-        float ax = 1.0f - hx[u];
+        float ax = 1.0f - hx;
         float Ex, Ey = 0.f;
         if (DIM == 2) {
-          float ay = 1.0f - hy[u];
-          float w00 = ax * ay, w10 = hx[u] * ay, w01 = ax * hy[u], w11 = hx[u] * hy[u];
+          float ay = 1.0f - hy;
+          float w00 = ax * ay, w10 = hx * ay, w01 = ax * hy, w11 = hx * hy;
           float2 e00, e10, e01, e11;
           int li = i - wx0, lj = j - wy0;
-          if (windowed && (unsigned)li < (unsigned)(W - 1) &&
-              (unsigned)lj < (unsigned)(W - 1)) {
+          if (windowed && (unsigned)li < (unsigned)(W - 1) && (unsigned)lj < (unsigned)(W - 1)) {
             int q = lj * W + li;
             e00 = make_float2(sE[2 * q], sE[2 * q + 1]);
             e10 = make_float2(sE[2 * q + 2], sE[2 * q + 3]);
@@ -743,15 +765,15 @@ __global__ __launch_bounds__(PUSH_BLOCK) void k_push(PushArgs a) {
             e0 = a.E[i];
             e1 = a.E[i + 1];
           }
-          Ex = fmaf(hx[u], e1, ax * e0);
+          Ex = fmaf(hx, e1, ax * e0);
         }
         /* push with B = 0 (pushers.jl:37-50): v += (q/m) E dt, x += v dt */
-        vx[u] = fmaf(a.qmdt, Ex, vx[u]);
-        i += advance_coord(vx[u], a.dtdx, &hx[u]);
+        vx = fmaf(a.qmdt, Ex, vx);
+        i += advance_coord(vx, a.dtdx, &hx);
         alive = apply_bnd(&i, a.ncx, a.bndx);
         if (DIM == 2) {
-          vy[u] = fmaf(a.qmdt, Ey, vy[u]);
-          j += advance_coord(vy[u], a.dtdy, &hy[u]);
+          vy = fmaf(a.qmdt, Ey, vy);
+          j += advance_coord(vy, a.dtdy, &hy);
           alive = apply_bnd(&j, a.ncy, a.bndy) && alive;
         }
       }
@@ -760,10 +782,9 @@ __global__ __launch_bounds__(PUSH_BLOCK) void k_push(PushArgs a) {
         /* deposit (cloud_in_cell.jl:1-18) */
         if (DIM == 2) {
           long long w00, w10, w01, w11;
-          fix_weights2(hx[u], hy[u], &w00, &w10, &w01, &w11);
+          fix_weights2(hx, hy, &w00, &w10, &w01, &w11);
           int li = i - wx0, lj = j - wy0;
-          if (windowed && (unsigned)li < (unsigned)(W - 1) &&
-              (unsigned)lj < (unsigned)(W - 1)) {
+          if (windowed && (unsigned)li < (unsigned)(W - 1) && (unsigned)lj < (unsigned)(W - 1)) {
             int q = lj * W + li;
             atomicAdd(&sD[q], (unsigned long long)w00);
             atomicAdd(&sD[q + 1], (unsigned long long)w10);
@@ -777,7 +798,7 @@ __global__ __launch_bounds__(PUSH_BLOCK) void k_push(PushArgs a) {
             atomicAdd(&a.gfall[q + a.nx + 1], (unsigned long long)w11);
           }
         } else {
-          long long w1 = __double2ll_rn((double)hx[u] * ISK_FIX_ONE);
+          long long w1 = __double2ll_rn((double)hx * ISK_FIX_ONE);
           long long w0 = (1LL << 32) - w1;
           int li = i - wx0;
           if (windowed && (unsigned)li < (unsigned)(W - 1)) {
@@ -790,22 +811,28 @@ __global__ __launch_bounds__(PUSH_BLOCK) void k_push(PushArgs a) {
         }
       }
       if (a.do_push) {
-        long long p = pidx[u];
-        a.cell[p] = alive ? (i | (j << 16)) : -1;
-        a.ox[p] = hx[u];
-        a.vx[p] = vx[u];
+        long long p = start + b0 + u * PUSH_BLOCK + threadIdx.x;
+        cellp[p] = alive ? (i | (j << 16)) : -1;
+        oxp[p] = hx;
+        vxp[p] = vx;
         if (DIM == 2) {
-          a.oy[p] = hy[u];
-          a.vy[p] = vy[u];
+          oyp[p] = hy;
+          vyp[p] = vy;
         }
       }
     }
   }
-  /* live count (particle-steps) */
+  /* per-work-group live count (summed by the merge kernel, no atomics) */
+  __shared__ int wl[PUSH_BLOCK / 64];
   for (int off = 32; off > 0; off >>= 1) live += __shfl_xor(live, off);
-  if (a.do_push && (threadIdx.x & 63) == 0 && live) atomicAdd((unsigned long long *)&a.hdr[HDR_PUSHED], (unsigned long long)live);
+  if ((threadIdx.x & 63) == 0) wl[threadIdx.x >> 6] = live;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int t = 0;
+    for (int k = 0; k < PUSH_BLOCK / 64; ++k) t += wl[k];
+    a.live_cnt[blockIdx.x] = t;
+  }
   if (windowed) {
-    __syncthreads();
     long long *dst = a.partial + (size_t)blockIdx.x * WW;
     for (int k = threadIdx.x; k < WW; k += PUSH_BLOCK) dst[k] = (long long)sD[k];
   }
@@ -844,22 +871,38 @@ int isk_push_species(iskra_ctx *c, int sid, double dt, int do_push) {
   a.bndx = s->bnd[0];
   a.bndy = s->bnd[1];
   a.do_push = do_push;
+  a.live_cnt = s->live_cnt;
   int grid = (int)s->max_chunks;
   int WW = g.dim == 2 ? a.W * a.W : a.W;
   size_t shm = (size_t)WW * (8 + 4 * g.dim);
+  hipEvent_t ev = nullptr;
+  if (do_push) {
+    int rc2 = isk_prof_begin(c, &ev);
+    if (rc2) return rc2;
+  }
   if (g.dim == 2)
     hipLaunchKernelGGL(k_push<2>, dim3(grid), dim3(PUSH_BLOCK), shm, c->stream, a);
   else
     hipLaunchKernelGGL(k_push<1>, dim3(grid), dim3(PUSH_BLOCK), shm, c->stream, a);
   ISK_CHECK_LAUNCH();
+  if (do_push) {
+    int rc2 = isk_prof_end(c, ev, ISKRA_K_PUSH, 0);
+    if (rc2) return rc2;
+  }
   s->deposited = 1;
-  return isk_merge_species(c, sid);
+  hipEvent_t em = nullptr;
+  if ((rc = isk_prof_begin(c, &em))) return rc;
+  if ((rc = isk_merge_species(c, sid, do_push))) return rc;
+  return isk_prof_end(c, em, ISKRA_K_MERGE, 0);
 }
 
 /* ------------------------------------------------------------------------ */
 /* deterministic merge of the chunk windows                                  */
 /* ------------------------------------------------------------------------ */
 struct MergeArgs {
+  const int *live_cnt;
+  int nlive; /* entries of live_cnt to add to HDR_PUSHED (0: deposit only) */
+  long long *hdr_rw;
   const long long *partial;
   const unsigned long long *gfall;
   const int *tcb;
@@ -869,9 +912,21 @@ struct MergeArgs {
 };
 
 template <int DIM>
-__global__ void k_merge(MergeArgs a) {
+__global__ __launch_bounds__(256) void k_merge(MergeArgs a) {
   int node = blockIdx.x * blockDim.x + threadIdx.x;
   int nn = a.nx * a.ny;
+  if (a.nlive > 0) {
+    __shared__ unsigned long long ls[4];
+    unsigned long long sum = 0;
+    for (int k = node; k < a.nlive; k += gridDim.x * blockDim.x) sum += (unsigned long long)a.live_cnt[k];
+    for (int off = 32; off > 0; off >>= 1) sum += __shfl_xor(sum, off);
+    if ((threadIdx.x & 63) == 0) ls[threadIdx.x >> 6] = sum;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      unsigned long long t = ls[0] + ls[1] + ls[2] + ls[3];
+      if (t) atomicAdd((unsigned long long *)&a.hdr_rw[HDR_PUSHED], t);
+    }
+  }
   if (node >= nn) return;
   int i = node % a.nx, j = node / a.nx;
   long long s = (long long)a.gfall[node];
@@ -897,9 +952,12 @@ __global__ void k_merge(MergeArgs a) {
   a.cnt[node] = s;
 }
 
-int isk_merge_species(iskra_ctx *c, int sid) {
+int isk_merge_species(iskra_ctx *c, int sid, int count_live) {
   IskSpecies *s = &c->sp[sid];
   MergeArgs a;
+  a.live_cnt = s->live_cnt;
+  a.nlive = count_live ? (int)s->max_chunks : 0;
+  a.hdr_rw = s->hdr;
   a.partial = s->partial;
   a.gfall = s->gfall;
   a.tcb = s->tile_chunk_begin;

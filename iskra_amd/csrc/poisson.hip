@@ -114,18 +114,21 @@ __device__ bool block_reduce2(double a, double b, double *partials,
     }
     __syncthreads();
   }
+  /* linear block id over a 1-D or 2-D grid */
+  const int bid = blockIdx.x + gridDim.x * blockIdx.y;
+  const int nb = gridDim.x * gridDim.y;
   if (threadIdx.x == 0) {
-    partials[2 * blockIdx.x] = ra[0];
-    partials[2 * blockIdx.x + 1] = rb[0];
+    partials[2 * bid] = ra[0];
+    partials[2 * bid + 1] = rb[0];
     __threadfence();
     unsigned int t = atomicAdd(counter, 1u);
-    last = (t == gridDim.x - 1);
+    last = (t == (unsigned int)nb - 1u);
   }
   __syncthreads();
   if (!last) return false;
   __threadfence();
   double sa = 0.0, sb = 0.0;
-  for (int k = threadIdx.x; k < (int)gridDim.x; k += RED_BLOCK) {
+  for (int k = threadIdx.x; k < nb; k += RED_BLOCK) {
     sa += __hip_atomic_load(&partials[2 * k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     sb += __hip_atomic_load(&partials[2 * k + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
@@ -490,20 +493,16 @@ struct MgCoarseArgs {
   const int *flag;
 };
 
-__device__ void lds_rbgs(float *e, const float *f, int nx, int ny, int color) {
-  int half = (nx * ny + 1) / 2;
-  for (int n = threadIdx.x; n < half; n += blockDim.x) {
-    int k = 2 * n;
-    int j = k / nx, i = k % nx;
-    if (((i + j) & 1) != color) {
-      ++i;
-      if (i >= nx) { i -= nx; ++j; }
+/* threads as a 32 x 32 grid; interior nodes of one colour, no divisions */
+__device__ __forceinline__ void lds_rbgs(float *e, const float *f, int nx, int ny, int color) {
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  for (int j = 1 + ty; j < ny - 1; j += 32) {
+    int i0 = 2 - ((color + j) & 1);
+    for (int i = i0 + 2 * tx; i < nx - 1; i += 64) {
+      int m = j * nx + i;
+      float s = (e[m - 1] + e[m + 1]) + (e[m - nx] + e[m + nx]);
+      e[m] = (f[m] + s) * 0.25f;
     }
-    if (((i + j) & 1) != color) continue;
-    if (i <= 0 || j <= 0 || i >= nx - 1 || j >= ny - 1) continue;
-    int m = j * nx + i;
-    float s = (e[m - 1] + e[m + 1]) + (e[m - nx] + e[m + nx]);
-    e[m] = (f[m] + s) * 0.25f;
   }
   __syncthreads();
 }
@@ -513,6 +512,7 @@ __global__ __launch_bounds__(1024) void k_mg_coarse(MgCoarseArgs a) {
   extern __shared__ float sm[];
   float *E = sm;
   float *Fv = sm + a.total;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
   int n0 = a.nx[0] * a.ny[0];
   for (int k = threadIdx.x; k < a.total; k += blockDim.x) {
     E[k] = 0.0f;
@@ -531,32 +531,36 @@ __global__ __launch_bounds__(1024) void k_mg_coarse(MgCoarseArgs a) {
     float *ec = E + a.off[l + 1];
     float *fc = Fv + a.off[l + 1];
     int cx = a.nx[l + 1], cy = a.ny[l + 1];
-    for (int n = threadIdx.x; n < cx * cy; n += blockDim.x) {
-      int I = n % cx, J = n / cx;
-      ec[n] = 0.0f;
-      if (I <= 0 || J <= 0 || I >= cx - 1 || J >= cy - 1) { fc[n] = 0.0f; continue; }
-      int i = 2 * I, j = 2 * J;
-      float acc[3][3];
-      for (int dj = -1; dj <= 1; ++dj)
-        for (int di = -1; di <= 1; ++di) {
-          int m = (j + dj) * nx + (i + di);
-          float s = (e[m - 1] + e[m + 1]) + (e[m - nx] + e[m + nx]);
-          acc[dj + 1][di + 1] = f[m] - (4.0f * e[m] - s);
-        }
-      float edge = (acc[0][1] + acc[2][1]) + (acc[1][0] + acc[1][2]);
-      float corner = (acc[0][0] + acc[0][2]) + (acc[2][0] + acc[2][2]);
-      fc[n] = (4.0f * acc[1][1] + 2.0f * edge + corner) * 0.25f;
-    }
+    for (int J = 1 + ty; J < cy - 1; J += 32)
+      for (int I = 1 + tx; I < cx - 1; I += 32) {
+        int i = 2 * I, j = 2 * J;
+        float acc[3][3];
+        for (int dj = -1; dj <= 1; ++dj)
+          for (int di = -1; di <= 1; ++di) {
+            int m = (j + dj) * nx + (i + di);
+            float s = (e[m - 1] + e[m + 1]) + (e[m - nx] + e[m + nx]);
+            acc[dj + 1][di + 1] = f[m] - (4.0f * e[m] - s);
+          }
+        float edge = (acc[0][1] + acc[2][1]) + (acc[1][0] + acc[1][2]);
+        float corner = (acc[0][0] + acc[0][2]) + (acc[2][0] + acc[2][2]);
+        fc[J * cx + I] = (4.0f * acc[1][1] + 2.0f * edge + corner) * 0.25f;
+      }
     __syncthreads();
   }
-  /* coarsest: Gauss-Seidel to convergence (tiny) */
+  /* coarsest: exact for one interior node, otherwise Gauss-Seidel sweeps */
   {
     int l = a.nlev_c - 1;
     float *e = E + a.off[l];
     float *f = Fv + a.off[l];
-    for (int it = 0; it < 32; ++it) {
-      lds_rbgs(e, f, a.nx[l], a.ny[l], 0);
-      lds_rbgs(e, f, a.nx[l], a.ny[l], 1);
+    int nx = a.nx[l], ny = a.ny[l];
+    if (nx == 3 && ny == 3) {
+      if (threadIdx.x == 0) e[4] = 0.25f * f[4];
+      __syncthreads();
+    } else {
+      for (int it = 0; it < 16; ++it) {
+        lds_rbgs(e, f, nx, ny, 0);
+        lds_rbgs(e, f, nx, ny, 1);
+      }
     }
   }
   /* up */
@@ -565,18 +569,17 @@ __global__ __launch_bounds__(1024) void k_mg_coarse(MgCoarseArgs a) {
     float *f = Fv + a.off[l];
     float *ec = E + a.off[l + 1];
     int nx = a.nx[l], ny = a.ny[l], cx = a.nx[l + 1];
-    for (int n = threadIdx.x; n < nx * ny; n += blockDim.x) {
-      int i = n % nx, j = n / nx;
-      if (i <= 0 || j <= 0 || i >= nx - 1 || j >= ny - 1) continue;
-      int I = i >> 1, J = j >> 1;
-      int m = J * cx + I;
-      float v;
-      if (!(i & 1) && !(j & 1)) v = ec[m];
-      else if ((i & 1) && !(j & 1)) v = 0.5f * (ec[m] + ec[m + 1]);
-      else if (!(i & 1) && (j & 1)) v = 0.5f * (ec[m] + ec[m + cx]);
-      else v = 0.25f * ((ec[m] + ec[m + 1]) + (ec[m + cx] + ec[m + cx + 1]));
-      e[n] += v;
-    }
+    for (int j = 1 + ty; j < ny - 1; j += 32)
+      for (int i = 1 + tx; i < nx - 1; i += 32) {
+        int I = i >> 1, J = j >> 1;
+        int m = J * cx + I;
+        float v;
+        if (!(i & 1) && !(j & 1)) v = ec[m];
+        else if ((i & 1) && !(j & 1)) v = 0.5f * (ec[m] + ec[m + 1]);
+        else if (!(i & 1) && (j & 1)) v = 0.5f * (ec[m] + ec[m + cx]);
+        else v = 0.25f * ((ec[m] + ec[m + 1]) + (ec[m + cx] + ec[m + cx + 1]));
+        e[j * nx + i] += v;
+      }
     __syncthreads();
     lds_rbgs(e, f, nx, ny, 1);
     lds_rbgs(e, f, nx, ny, 0);
@@ -584,6 +587,183 @@ __global__ __launch_bounds__(1024) void k_mg_coarse(MgCoarseArgs a) {
     lds_rbgs(e, f, nx, ny, 0);
   }
   for (int k = threadIdx.x; k < n0; k += blockDim.x) a.e_out[k] = E[k];
+}
+
+/* ------------------------------------------------------------------------ */
+/* fused fine-level multigrid kernels (temporal blocking in LDS)            */
+/* ------------------------------------------------------------------------ */
+/* A work-group owns CT x CT coarse nodes and the 2CT x 2CT fine nodes starting
+ * at twice their index; it loads a fine tile with a halo of H nodes, runs the
+ * four red-black half-sweeps of one smoothing stage there (each half-sweep
+ * leaves one fewer halo ring exact), and writes only owned nodes, so the
+ * result equals the global red-black Gauss-Seidel sweep. */
+#define MGT_CT 8
+#define MGT_H 6
+#define MGT_S (2 * MGT_CT + 2 * MGT_H)
+
+struct MgFused {
+  MgLevel F, C;
+  const int *flag;
+  /* finest level of the FP64 defect correction */
+  int mode_in;  /* down: 0 = f from F.f, 1 = f = dx^2 rho/eps0 - L x (FP64) */
+  int mode_out; /* up: 0 = write F.e, 1 = x += e (FP64) */
+  Sten s;
+  double *x;
+  const double *rho;
+  double scale;
+  double *scal, *partials;
+  unsigned int *counter;
+  int *iflag;
+  int first, max_iter;
+};
+
+__device__ __forceinline__ void tile_sweep(float *se, const float *sf, int ox0, int oy0, int nx,
+                                           int ny, int color) {
+  for (int k = threadIdx.x; k < MGT_S * MGT_S; k += blockDim.x) {
+    int lj = k / MGT_S, li = k - lj * MGT_S;
+    int gi = ox0 + li, gj = oy0 + lj;
+    if (li < 1 || lj < 1 || li > MGT_S - 2 || lj > MGT_S - 2) continue;
+    if (gi < 1 || gj < 1 || gi > nx - 2 || gj > ny - 2) continue;
+    if (((gi + gj) & 1) != color) continue;
+    float sum = (se[k - 1] + se[k + 1]) + (se[k - MGT_S] + se[k + MGT_S]);
+    se[k] = (sf[k] + sum) * 0.25f;
+  }
+  __syncthreads();
+}
+
+/* pre-smooth from e = 0, restrict the residual, write owned e (and f) */
+__global__ __launch_bounds__(256) void k_mgf_down(MgFused a) {
+  if (!a.first && a.flag[0]) return;
+  __shared__ float se[MGT_S * MGT_S], sf[MGT_S * MGT_S];
+  const int nx = a.F.nx, ny = a.F.ny;
+  const int I0 = blockIdx.x * MGT_CT, J0 = blockIdx.y * MGT_CT;
+  const int ox0 = 2 * I0 - MGT_H, oy0 = 2 * J0 - MGT_H;
+  double rr = 0.0, ff = 0.0;
+  for (int k = threadIdx.x; k < MGT_S * MGT_S; k += blockDim.x) {
+    int lj = k / MGT_S, li = k - lj * MGT_S;
+    int gi = ox0 + li, gj = oy0 + lj;
+    bool interior = gi >= 1 && gj >= 1 && gi <= nx - 2 && gj <= ny - 2;
+    float f = 0.0f;
+    if (interior) {
+      int m = gj * nx + gi;
+      if (a.mode_in == 1) {
+        double fs = a.rho[m] * a.scale;
+        double r = fs - apply_L(a.s, a.x, gi, gj, nullptr);
+        f = (float)r;
+        bool owned = li >= MGT_H && lj >= MGT_H && li < MGT_H + 2 * MGT_CT &&
+                     lj < MGT_H + 2 * MGT_CT;
+        if (owned) {
+          double fe = fs + dirichlet_rhs(a.s, gi, gj);
+          rr += r * r;
+          ff += fe * fe;
+        }
+      } else {
+        f = a.F.f[m];
+      }
+    }
+    sf[k] = f;
+    se[k] = 0.0f;
+  }
+  __syncthreads();
+  tile_sweep(se, sf, ox0, oy0, nx, ny, 0);
+  tile_sweep(se, sf, ox0, oy0, nx, ny, 1);
+  tile_sweep(se, sf, ox0, oy0, nx, ny, 0);
+  tile_sweep(se, sf, ox0, oy0, nx, ny, 1);
+  /* residual at fine nodes -> full weighting -> coarse rhs (x4) */
+  const int cx = a.C.nx, cy = a.C.ny;
+  for (int q = threadIdx.x; q < MGT_CT * MGT_CT; q += blockDim.x) {
+    int I = I0 + (q % MGT_CT), J = J0 + (q / MGT_CT);
+    if (I >= cx || J >= cy) continue;
+    int n = J * cx + I;
+    a.C.e[n] = 0.0f;
+    if (I <= 0 || J <= 0 || I >= cx - 1 || J >= cy - 1) {
+      a.C.f[n] = 0.0f;
+      continue;
+    }
+    int lc = (2 * J - oy0) * MGT_S + (2 * I - ox0);
+    float acc[3][3];
+#pragma unroll
+    for (int dj = -1; dj <= 1; ++dj)
+#pragma unroll
+      for (int di = -1; di <= 1; ++di) {
+        int k = lc + dj * MGT_S + di;
+        float sum = (se[k - 1] + se[k + 1]) + (se[k - MGT_S] + se[k + MGT_S]);
+        acc[dj + 1][di + 1] = sf[k] - (4.0f * se[k] - sum);
+      }
+    float edge = (acc[0][1] + acc[2][1]) + (acc[1][0] + acc[1][2]);
+    float corner = (acc[0][0] + acc[0][2]) + (acc[2][0] + acc[2][2]);
+    a.C.f[n] = (4.0f * acc[1][1] + 2.0f * edge + corner) * 0.25f;
+  }
+  /* owned fine nodes: smoothed e (and, at the finest level, f) */
+  for (int q = threadIdx.x; q < 4 * MGT_CT * MGT_CT; q += blockDim.x) {
+    int li = MGT_H + (q % (2 * MGT_CT)), lj = MGT_H + (q / (2 * MGT_CT));
+    int gi = ox0 + li, gj = oy0 + lj;
+    if (gi >= nx || gj >= ny) continue;
+    int m = gj * nx + gi;
+    int k = lj * MGT_S + li;
+    a.F.e[m] = se[k];
+    if (a.mode_in == 1) a.F.f[m] = sf[k];
+  }
+  if (a.mode_in == 1) {
+    double srr, sff;
+    if (block_reduce2(rr, ff, a.partials, a.counter, &srr, &sff) && threadIdx.x == 0) {
+      if (a.first) {
+        a.scal[S_FF] = sff;
+        a.scal[S_TOL2] = a.scal[S_TOL2] * sff;
+      }
+      a.scal[S_RR] = srr;
+      a.scal[S_RELRES] = a.scal[S_FF] > 0.0 ? sqrt(srr / a.scal[S_FF]) : 0.0;
+      if (srr <= a.scal[S_TOL2] || a.iflag[1] >= a.max_iter) a.iflag[0] = 1;
+      else a.iflag[1] += 1;
+    }
+  }
+}
+
+/* e += P e_coarse, post-smooth, write owned e (or x += e at the finest level) */
+__global__ __launch_bounds__(256) void k_mgf_up(MgFused a) {
+  if (a.flag[0]) return;
+  __shared__ float se[MGT_S * MGT_S], sf[MGT_S * MGT_S];
+  const int nx = a.F.nx, ny = a.F.ny, cx = a.C.nx;
+  const int I0 = blockIdx.x * MGT_CT, J0 = blockIdx.y * MGT_CT;
+  const int ox0 = 2 * I0 - MGT_H, oy0 = 2 * J0 - MGT_H;
+  for (int k = threadIdx.x; k < MGT_S * MGT_S; k += blockDim.x) {
+    int lj = k / MGT_S, li = k - lj * MGT_S;
+    int i = ox0 + li, j = oy0 + lj;
+    bool interior = i >= 1 && j >= 1 && i <= nx - 2 && j <= ny - 2;
+    float e = 0.0f, f = 0.0f;
+    if (interior) {
+      int m = j * nx + i;
+      int I = i >> 1, J = j >> 1;
+      const float *ec = a.C.e;
+      int mc = J * cx + I;
+      float v;
+      if (!(i & 1) && !(j & 1)) v = ec[mc];
+      else if ((i & 1) && !(j & 1)) v = 0.5f * (ec[mc] + ec[mc + 1]);
+      else if (!(i & 1) && (j & 1)) v = 0.5f * (ec[mc] + ec[mc + cx]);
+      else v = 0.25f * ((ec[mc] + ec[mc + 1]) + (ec[mc + cx] + ec[mc + cx + 1]));
+      e = a.F.e[m] + v;
+      f = a.F.f[m];
+    }
+    se[k] = e;
+    sf[k] = f;
+  }
+  __syncthreads();
+  tile_sweep(se, sf, ox0, oy0, nx, ny, 1);
+  tile_sweep(se, sf, ox0, oy0, nx, ny, 0);
+  tile_sweep(se, sf, ox0, oy0, nx, ny, 1);
+  tile_sweep(se, sf, ox0, oy0, nx, ny, 0);
+  for (int q = threadIdx.x; q < 4 * MGT_CT * MGT_CT; q += blockDim.x) {
+    int li = MGT_H + (q % (2 * MGT_CT)), lj = MGT_H + (q / (2 * MGT_CT));
+    int gi = ox0 + li, gj = oy0 + lj;
+    if (gi >= nx || gj >= ny) continue;
+    int m = gj * nx + gi;
+    int k = lj * MGT_S + li;
+    if (a.mode_out == 1) {
+      if (gi >= 1 && gj >= 1 && gi <= nx - 2 && gj <= ny - 2) a.x[m] += (double)se[k];
+    } else {
+      a.F.e[m] = se[k];
+    }
+  }
 }
 
 /* FP64 defect-correction residual: f0 = r = f_elim - L x, e0 = 0 */
@@ -783,11 +963,17 @@ int isk_poisson_setup(iskra_ctx *c) {
   }
   /* iterative solvers */
   int nblk = (nn + RED_BLOCK - 1) / RED_BLOCK;
+  {
+    /* the finest fused multigrid kernel reduces over one block per tile */
+    int cx = (c->g.nx - 1) / 2 + 1, cy = c->g.dim == 2 ? (c->g.ny - 1) / 2 + 1 : 1;
+    int tiles = ((cx + MGT_CT - 1) / MGT_CT) * ((cy + MGT_CT - 1) / MGT_CT);
+    p.partials_blocks = std::max(nblk, tiles);
+  }
   if (isk_alloc((void **)&p.x, nn * 8) || isk_alloc((void **)&p.r, nn * 8) ||
       isk_alloc((void **)&p.Ap, nn * 8) || isk_alloc((void **)&p.z, nn * 4) ||
       isk_alloc((void **)&p.p[0], nn * 4) || isk_alloc((void **)&p.p[1], nn * 4) ||
       isk_alloc((void **)&p.scal, S_WORDS * 8) ||
-      isk_alloc((void **)&p.partials, (size_t)nblk * 2 * 8) ||
+      isk_alloc((void **)&p.partials, (size_t)p.partials_blocks * 2 * 8) ||
       isk_alloc((void **)&p.counter, 64) || isk_alloc((void **)&p.iflag, 64))
     return ISKRA_E_HIP;
   ISK_HIP(hipMemset(p.counter, 0, 64));
@@ -865,7 +1051,10 @@ static PcgArgs make_pcg(iskra_ctx *c, int mgpre) {
   return a;
 }
 
-static int mg_vcycle(iskra_ctx *c) {
+/* one V-cycle; mode_in0 = 1 makes the finest down-kernel compute the FP64
+ * residual of x (defect correction, first = 1 on the solve's first cycle);
+ * mode_out0 = 1 adds the correction to x. */
+static int mg_vcycle(iskra_ctx *c, int mode_in0, int mode_out0, int first) {
   IskPoisson &p = c->ps;
   const int *flag = p.iflag;
   auto lev = [&](int l) {
@@ -876,44 +1065,63 @@ static int mg_vcycle(iskra_ctx *c) {
     L.ny = p.ln[l] >> 16;
     return L;
   };
+  MgFused a;
+  a.flag = flag;
+  a.s = make_sten(c);
+  a.x = p.x;
+  a.rho = c->rho;
+  a.scale = (c->g.dx * c->g.dx) / p.eps0;
+  a.scal = p.scal;
+  a.partials = p.partials;
+  a.counter = p.counter;
+  a.iflag = p.iflag;
+  a.max_iter = p.max_iter;
+  const int nn0 = (p.ln[0] & 0xFFFF) * (p.ln[0] >> 16);
+  const int nblk0 = (nn0 + RED_BLOCK - 1) / RED_BLOCK;
+  PcgArgs pa = make_pcg(c, 0);
+  if (p.coarse_lev == 0 && mode_in0 == 1) {
+    /* the whole grid is solved by the single work-group kernel: FP64 residual here */
+    hipLaunchKernelGGL(k_mg_resid0, dim3(nblk0), dim3(RED_BLOCK), 0, c->stream, pa, p.me[0], first);
+  }
   for (int l = 0; l < p.coarse_lev; ++l) {
-    MgLevel L = lev(l);
-    int half = (L.nx * L.ny + 1) / 2;
-    for (int s = 0; s < 2; ++s) {
-      hipLaunchKernelGGL(k_mg_rbgs, dim3((half + 255) / 256), dim3(256), 0, c->stream, L, 0, flag);
-      hipLaunchKernelGGL(k_mg_rbgs, dim3((half + 255) / 256), dim3(256), 0, c->stream, L, 1, flag);
+    a.F = lev(l);
+    a.C = lev(l + 1);
+    a.mode_in = l == 0 ? mode_in0 : 0;
+    a.first = l == 0 ? first : 0;
+    dim3 g((a.C.nx + MGT_CT - 1) / MGT_CT, (a.C.ny + MGT_CT - 1) / MGT_CT);
+    if (a.mode_in == 1 && (int)(g.x * g.y) > p.partials_blocks) {
+      isk_set_error("poisson: reduction scratch holds %d blocks, kernel needs %d",
+                    p.partials_blocks, (int)(g.x * g.y));
+      return ISKRA_E_STATE;
     }
-    MgLevel C = lev(l + 1);
-    hipLaunchKernelGGL(k_mg_restrict, dim3((C.nx * C.ny + 255) / 256), dim3(256), 0,
-                       c->stream, L, C, flag);
+    hipLaunchKernelGGL(k_mgf_down, g, dim3(256), 0, c->stream, a);
   }
   {
-    MgCoarseArgs a;
+    MgCoarseArgs ca;
     int tot = 0;
-    a.nlev_c = p.nlev - p.coarse_lev;
-    for (int k = 0; k < a.nlev_c; ++k) {
+    ca.nlev_c = p.nlev - p.coarse_lev;
+    for (int k = 0; k < ca.nlev_c; ++k) {
       int l = p.coarse_lev + k;
-      a.nx[k] = p.ln[l] & 0xFFFF;
-      a.ny[k] = p.ln[l] >> 16;
-      a.off[k] = tot;
-      tot += a.nx[k] * a.ny[k];
+      ca.nx[k] = p.ln[l] & 0xFFFF;
+      ca.ny[k] = p.ln[l] >> 16;
+      ca.off[k] = tot;
+      tot += ca.nx[k] * ca.ny[k];
     }
-    a.total = tot;
-    a.e_out = p.me[p.coarse_lev];
-    a.f_in = p.mf[p.coarse_lev];
-    a.flag = flag;
-    hipLaunchKernelGGL(k_mg_coarse, dim3(1), dim3(1024), (size_t)tot * 2 * 4, c->stream, a);
+    ca.total = tot;
+    ca.e_out = p.me[p.coarse_lev];
+    ca.f_in = p.mf[p.coarse_lev];
+    ca.flag = flag;
+    hipLaunchKernelGGL(k_mg_coarse, dim3(1), dim3(1024), (size_t)tot * 2 * 4, c->stream, ca);
   }
   for (int l = p.coarse_lev - 1; l >= 0; --l) {
-    MgLevel L = lev(l), C = lev(l + 1);
-    hipLaunchKernelGGL(k_mg_prolong, dim3((L.nx * L.ny + 255) / 256), dim3(256), 0,
-                       c->stream, C, L, flag);
-    int half = (L.nx * L.ny + 1) / 2;
-    for (int s = 0; s < 2; ++s) {
-      hipLaunchKernelGGL(k_mg_rbgs, dim3((half + 255) / 256), dim3(256), 0, c->stream, L, 1, flag);
-      hipLaunchKernelGGL(k_mg_rbgs, dim3((half + 255) / 256), dim3(256), 0, c->stream, L, 0, flag);
-    }
+    a.F = lev(l);
+    a.C = lev(l + 1);
+    a.mode_out = l == 0 ? mode_out0 : 0;
+    dim3 g((a.C.nx + MGT_CT - 1) / MGT_CT, (a.C.ny + MGT_CT - 1) / MGT_CT);
+    hipLaunchKernelGGL(k_mgf_up, g, dim3(256), 0, c->stream, a);
   }
+  if (p.coarse_lev == 0 && mode_out0 == 1)
+    hipLaunchKernelGGL(k_mg_update0, dim3(nblk0), dim3(RED_BLOCK), 0, c->stream, pa, p.me[0]);
   ISK_CHECK_LAUNCH();
   return ISKRA_OK;
 }
@@ -960,36 +1168,46 @@ int isk_poisson_solve(iskra_ctx *c, int warm, int *iters, double *relres) {
   }
   int hflag[4] = {0, 0, 0, 0};
   if (p.kind == ISKRA_POISSON_MG) {
-    hipLaunchKernelGGL(k_mg_resid0, dim3(nblk), dim3(RED_BLOCK), 0, c->stream, a, p.me[0], 1);
-    ISK_CHECK_LAUNCH();
+    /* cycles per host check: one more than the last solve needed, then 2 */
+    int batch = p.last_iters > 0 ? p.last_iters + 1 : MG_BATCH;
+    int first = 1;
+    int launched = batch;
     while (true) {
-      for (int b = 0; b < MG_BATCH; ++b) {
-        int rc = mg_vcycle(c);
+      for (int b = 0; b < batch; ++b) {
+        int rc = mg_vcycle(c, 1, 1, first);
         if (rc) return rc;
-        hipLaunchKernelGGL(k_mg_update0, dim3(nblk), dim3(RED_BLOCK), 0, c->stream, a, p.me[0]);
-        hipLaunchKernelGGL(k_mg_resid0, dim3(nblk), dim3(RED_BLOCK), 0, c->stream, a, p.me[0], 0);
-        ISK_CHECK_LAUNCH();
+        first = 0;
       }
+      /* the next cycle's finest down-kernel evaluates the residual of x */
+      int rc = mg_vcycle(c, 1, 1, 0);
+      if (rc) return rc;
       ISK_HIP(hipMemcpyAsync(hflag, p.iflag, sizeof(hflag), hipMemcpyDeviceToHost, c->stream));
       ISK_HIP(hipStreamSynchronize(c->stream));
       if (hflag[0]) break;
+      batch = 1;
+      launched += 2;
+      if (launched > p.max_iter + 4) {
+        isk_set_error("poisson: multigrid stopped after %d host cycles without a device verdict", launched);
+        return ISKRA_E_CONVERGE;
+      }
     }
   } else {
     hipLaunchKernelGGL(k_pcg_resid, dim3(nblk), dim3(RED_BLOCK), 0, c->stream, a);
     ISK_CHECK_LAUNCH();
     if (mgpre) {
-      int rc = mg_vcycle(c);
+      int rc = mg_vcycle(c, 0, 0, 0);
       if (rc) return rc;
       hipLaunchKernelGGL(k_mg_toz, dim3(nblk), dim3(RED_BLOCK), 0, c->stream, p.me[0], p.z, nn, p.iflag);
     }
     hipLaunchKernelGGL(k_pcg_rz, dim3(nblk), dim3(RED_BLOCK), 0, c->stream, a);
     ISK_CHECK_LAUNCH();
+    int launched = 0;
     while (true) {
       for (int b = 0; b < (mgpre ? MG_BATCH : PCG_BATCH); ++b) {
         hipLaunchKernelGGL(k_pcg_a, dim3(nblk), dim3(RED_BLOCK), 0, c->stream, a);
         hipLaunchKernelGGL(k_pcg_b, dim3(nblk), dim3(RED_BLOCK), 0, c->stream, a);
         if (mgpre) {
-          int rc = mg_vcycle(c);
+          int rc = mg_vcycle(c, 0, 0, 0);
           if (rc) return rc;
           hipLaunchKernelGGL(k_mg_toz, dim3(nblk), dim3(RED_BLOCK), 0, c->stream, p.me[0], p.z, nn, p.iflag);
           hipLaunchKernelGGL(k_pcg_rz, dim3(nblk), dim3(RED_BLOCK), 0, c->stream, a);
@@ -1000,6 +1218,11 @@ int isk_poisson_solve(iskra_ctx *c, int warm, int *iters, double *relres) {
       ISK_HIP(hipMemcpyAsync(hflag, p.iflag, sizeof(hflag), hipMemcpyDeviceToHost, c->stream));
       ISK_HIP(hipStreamSynchronize(c->stream));
       if (hflag[0]) break;
+      launched += mgpre ? MG_BATCH : PCG_BATCH;
+      if (launched > p.max_iter + 2 * PCG_BATCH) {
+        isk_set_error("poisson: CG stopped after %d host iterations without a device verdict", launched);
+        return ISKRA_E_CONVERGE;
+      }
     }
   }
   double scal[S_WORDS];

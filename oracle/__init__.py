@@ -93,6 +93,7 @@ def _declare(L):
         "ref_apply_dirichlet": (None, [C.c_int, P, D, P, P, P]),
         "ref_lu_solve": (C.c_int, [C.c_int, P, P, P]),
         "ref_efield": (None, [C.c_int, C.c_int, C.c_int, D, D, P, P]),
+        "ref_poisson_band_solve": (C.c_int, [C.c_int, C.c_int, D, P, P, P, D, P]),
         "ref_sigma": (D, [C.c_int, P, P, D]),
         "ref_max_sigma_g": (D, [C.c_int, P, P, P, D, C.c_int, P]),
         "mir_gather": (None, [C.c_int, I64, P, P, P, P, C.c_int, P, P]),
@@ -206,6 +207,20 @@ def potential(A, b, isdof, rho, eps0):
     bb = b.copy()
     bb[isdof == 1] = -rho[isdof == 1] / eps0
     return lu_solve(A, bb)
+
+
+def potential_banded(nx, ny, dx, tags, tagval, rho, eps0):
+    """Direct solve of the reference system for a non-periodic grid of any size
+    (band storage; the reference's dense A\\b is infeasible beyond ~4k nodes)."""
+    t = np.ascontiguousarray(tags, dtype=np.int8).ravel()
+    tv = np.zeros(17)
+    tv[:len(tagval)] = tagval
+    x = np.zeros(nx * ny)
+    rc = lib().ref_poisson_band_solve(nx, ny, dx, ptr(t), ptr(tv),
+                                      ptr(np.ascontiguousarray(rho, dtype=np.float64)), eps0,
+                                      ptr(x))
+    assert rc == 0, "banded solve failed"
+    return x
 
 
 def efield(dim, nx, ny, dx, dy, phi):

@@ -345,6 +345,64 @@ EXPORT int ref_lu_solve(int n, const double *Ain, const double *b, double *x) {
 }
 #undef AE
 
+/* calculate_electric_potential for non-periodic grids of any size: the same
+ * rows as ref_poisson_assemble + ref_apply_dirichlet (generalized_poisson.jl:
+ * 42-68,205-215), stored as a band of half-width nx and eliminated without
+ * pivoting (every row is diagonally dominant: Laplacian rows or identity
+ * rows), then back-substituted.  mask[k] = tag (0 free), tagval[tag] = value;
+ * b[k] = -rho[k]/eps0 at free nodes.  Returns 0 or 1 on a zero pivot. */
+EXPORT int ref_poisson_band_solve(int nx, int ny, double dx, const int8_t *mask,
+                                  const double *tagval, const double *rho, double eps0,
+                                  double *x) {
+  const int n = nx * ny, bw = nx, w = 2 * bw + 1;
+  double *B = (double *)calloc((size_t)n * w, sizeof(double));
+  double *b = (double *)malloc(sizeof(double) * n);
+  if (!B || !b) { free(B); free(b); return 2; }
+#define BE(r, c) B[(size_t)(r) * w + ((c) - (r) + bw)]
+  double d2 = dx * dx;
+  for (int j = 0; j < ny; ++j)
+    for (int i = 0; i < nx; ++i) {
+      int k = i + nx * j;
+      if (mask[k]) {
+        BE(k, k) = 1.0;
+        b[k] = tagval[mask[k]];
+        continue;
+      }
+      double dg = 0.0;
+      if (i < nx - 1) { dg -= 1.0; BE(k, k + 1) += 1.0 / d2; }
+      if (i > 0) { dg -= 1.0; BE(k, k - 1) += 1.0 / d2; }
+      if (j < ny - 1) { dg -= 1.0; BE(k, k + nx) += 1.0 / d2; }
+      if (j > 0) { dg -= 1.0; BE(k, k - nx) += 1.0 / d2; }
+      BE(k, k) = dg / d2;
+      b[k] = -rho[k] / eps0;
+    }
+  int sing = 0;
+  for (int k = 0; k < n; ++k) {
+    double piv = BE(k, k);
+    if (piv == 0.0) { sing = 1; break; }
+    int rmax = k + bw < n - 1 ? k + bw : n - 1;
+    for (int r = k + 1; r <= rmax; ++r) {
+      double a = BE(r, k);
+      if (a == 0.0) continue;
+      double l = a / piv;
+      BE(r, k) = 0.0;
+      for (int c = k + 1; c <= rmax; ++c) BE(r, c) -= l * BE(k, c);
+      b[r] -= l * b[k];
+    }
+  }
+  if (!sing)
+    for (int k = n - 1; k >= 0; --k) {
+      double s = b[k];
+      int cmax = k + bw < n - 1 ? k + bw : n - 1;
+      for (int c = k + 1; c <= cmax; ++c) s -= BE(k, c) * x[c];
+      x[k] = s / BE(k, k);
+    }
+#undef BE
+  free(B);
+  free(b);
+  return sing;
+}
+
 /* calculate_electric_field! (generalized_poisson.jl:385-410): E is nx x ny x 3
  * column-major; E[...,3] = 0; 1-D fills only component 1. */
 EXPORT void ref_efield(int dim, int nx, int ny, double dx, double dy, const double *phi,

@@ -290,3 +290,34 @@ def test_capacitor_kat(gpu_ready):
     ctx.efield()
     assert np.allclose(ctx.field(L.F_EX), -1.0, atol=1e-7)
     assert np.allclose(ctx.field(L.F_EY), 0.0, atol=1e-7)
+
+
+@pytest.mark.parametrize("nodes,kind", [(129, 2), (129, 3), (257, 2), (257, 3), (129, 1)])
+def test_poisson_2d_large_vs_banded(gpu_ready, orc, nodes, kind):
+    """Multigrid levels above 65^2 run in the fused tile kernels: check the
+    solution of the reference system (generalized_poisson.jl) to rtol 1e-8,
+    from a zero start and from a perturbed warm start."""
+    from iskra_amd import Context, _lib as L
+    rng = np.random.default_rng(nodes + kind)
+    dx, eps0 = 1e-4, 8.854e-12
+    ctx = Context()
+    ctx.grid(2, (nodes, nodes), (dx, dx))
+    tags = np.zeros((nodes, nodes), np.int8)
+    tags[0, :] = tags[-1, :] = tags[:, 0] = 1
+    tags[:, -1] = 2
+    ctx.poisson(kind, eps0, tags.ravel(), rtol=1e-8)
+    ctx.tag_value(2, -3.0)
+    rho = rng.standard_normal(nodes * nodes) * 1e-6
+    ref = orc.potential_banded(nodes, nodes, dx, tags, [0.0, 0.0, -3.0], rho, eps0)
+    ctx.set_field(L.F_RHO, rho)
+    it, rr = ctx.solve(L.WARM_ZERO)
+    assert rr <= 1e-8, (it, rr)
+    phi = ctx.field(L.F_PHI)
+    scale = np.max(np.abs(ref))
+    # error bound: cond(A) * 1e-8 is loose; observed errors are ~1e-8 relative
+    assert np.max(np.abs(phi - ref)) <= 1e-6 * scale, np.max(np.abs(phi - ref)) / scale
+    # warm start from a perturbed solution converges in fewer iterations
+    ctx.set_field(L.F_PHI, ref + 1e-4 * scale * rng.standard_normal(nodes * nodes))
+    it2, rr2 = ctx.solve(L.WARM_PREV)
+    assert rr2 <= 1e-8 and it2 <= it, (it, it2)
+    assert np.max(np.abs(ctx.field(L.F_PHI) - ref)) <= 1e-6 * scale
