@@ -229,10 +229,11 @@ __host__ __device__ inline IskU4 isk_philox(IskU4 c, uint32_t k0, uint32_t k1) {
   }
   return c;
 }
-/* uniform double in (0,1) from two words: 53 bits, centred in its bin */
-__host__ __device__ inline double isk_u53(uint32_t a, uint32_t b) {
-  uint64_t k = ((uint64_t)(a >> 6) << 27) | (uint64_t)(b >> 5);
-  return ((double)k + 0.5) * (1.0 / 9007199254740992.0);
+/* uniform double strictly inside (0,1) from two words: 52 bits, centred in
+ * its bin (the largest value, 1 - 2^-53, is representable) */
+__host__ __device__ inline double isk_u01(uint32_t a, uint32_t b) {
+  uint64_t k = ((uint64_t)(a >> 6) << 26) | (uint64_t)(b >> 6);
+  return ((double)k + 0.5) * (1.0 / 4503599627370496.0);
 }
 /* uniform float in [0,1) from 24 bits */
 __host__ __device__ inline float isk_u24(uint32_t a) {

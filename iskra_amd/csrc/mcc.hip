@@ -16,7 +16,7 @@
  * Draws: Philox4x32-10, key = seed, counter = (id, step, stream, block);
  * block 0 gives the candidate and process uniforms, blocks 1..5 the collision
  * draws in the order the reference consumes them.  Uniforms are 53-bit
- * (isk_u53); normals are Box-Muller pairs.
+ * (isk_u01); normals are Box-Muller pairs.
  */
 #include <math.h>
 
@@ -69,10 +69,10 @@ struct Draws {
       cur = isk_philox(c, k0, k1);
       ++block;
       half = 1;
-      return isk_u53(cur.x, cur.y);
+      return isk_u01(cur.x, cur.y);
     }
     half = 0;
-    return isk_u53(cur.z, cur.w);
+    return isk_u01(cur.z, cur.w);
   }
 };
 

@@ -167,7 +167,7 @@ __global__ __launch_bounds__(256) void k_sample(SampleArgs a) {
   c0.w = 2u;
   IskU4 r2 = isk_philox(c0, a.k0, a.k1);
   /* positions */
-  double ux = isk_u53(r0.x, r0.y);
+  double ux = isk_u01(r0.x, r0.y);
   double px = ux * a.wx[0] + a.dxo[0];
   double s = (px - a.x0) / a.dx;
   double fl = floor(s);
@@ -177,7 +177,7 @@ __global__ __launch_bounds__(256) void k_sample(SampleArgs a) {
   i = min(max(i, 0), a.ncx - 1);
   int j = 0;
   if (a.dim == 2) {
-    double uy = isk_u53(r0.z, r0.w);
+    double uy = isk_u01(r0.z, r0.w);
     double py = uy * a.wx[1] + a.dxo[1];
     double sy = (py - a.y0) / a.dy;
     double fly = floor(sy);
@@ -188,8 +188,8 @@ __global__ __launch_bounds__(256) void k_sample(SampleArgs a) {
     a.oy[p] = ofy;
   }
   /* velocities: Box-Muller on (r1.xy, r1.zw) and (r2.xy, r2.zw) */
-  double u1 = isk_u53(r1.x, r1.y), u2 = isk_u53(r1.z, r1.w);
-  double u3 = isk_u53(r2.x, r2.y), u4 = isk_u53(r2.z, r2.w);
+  double u1 = isk_u01(r1.x, r1.y), u2 = isk_u01(r1.z, r1.w);
+  double u3 = isk_u01(r2.x, r2.y), u4 = isk_u01(r2.z, r2.w);
   double rad1 = sqrt(-2.0 * isk_log(u1)), rad2 = sqrt(-2.0 * isk_log(u3));
   double s1, c1, s2, c2;
   isk_sincos2pi(u2, &s1, &c1);

@@ -77,7 +77,7 @@ def _declare(L):
     sig = {
         "orc_philox4x32": (None, [P, P, P]),
         "orc_philox4x64": (None, [P, P, P]),
-        "orc_u53": (D, [U32, U32]),
+        "orc_u01": (D, [U32, U32]),
         "orc_exp": (D, [D]),
         "orc_log": (D, [D]),
         "orc_sincos2pi": (None, [D, P, P]),
@@ -142,8 +142,8 @@ def philox4x64(ctr, key):
     return o
 
 
-def u53(a, b):
-    return lib().orc_u53(int(a), int(b))
+def u01(a, b):
+    return lib().orc_u01(int(a), int(b))
 
 
 # --------------------------------------------------------------------------

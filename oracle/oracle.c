@@ -74,10 +74,10 @@ EXPORT void orc_philox4x64(const uint64_t ctr_in[4], const uint64_t key_in[2], u
   out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
 }
 
-/* 53-bit uniform in (0,1) from two words, centred in its bin */
-EXPORT double orc_u53(uint32_t a, uint32_t b) {
-  uint64_t k = ((uint64_t)(a >> 6) << 27) | (uint64_t)(b >> 5);
-  return ((double)k + 0.5) * (1.0 / 9007199254740992.0);
+/* uniform in (0,1) from two words: 52 bits, centred in its bin */
+EXPORT double orc_u01(uint32_t a, uint32_t b) {
+  uint64_t k = ((uint64_t)(a >> 6) << 26) | (uint64_t)(b >> 6);
+  return ((double)k + 0.5) * (1.0 / 4503599627370496.0);
 }
 
 /* thin wrappers so tests can pin the shared FP64 math against libm */
@@ -704,10 +704,10 @@ static double md_next(MirDraws *d) {
     orc_philox4x32(c, k, d->cur);
     d->block++;
     d->half = 1;
-    return orc_u53(d->cur[0], d->cur[1]);
+    return orc_u01(d->cur[0], d->cur[1]);
   }
   d->half = 0;
-  return orc_u53(d->cur[2], d->cur[3]);
+  return orc_u01(d->cur[2], d->cur[3]);
 }
 
 /* euler_angles (mcc.jl:89-106) */
@@ -881,7 +881,7 @@ EXPORT void mir_sample(int dim, int64_t n, int64_t first_id, double x0, double y
     orc_philox4x32(c0, key, r1);
     c0[3] = 2u;
     orc_philox4x32(c0, key, r2);
-    double ux = orc_u53(r0[0], r0[1]);
+    double ux = orc_u01(r0[0], r0[1]);
     double px = ux * wx[0] + dxo[0];
     double s = (px - x0) / dx;
     double fl = floor(s);
@@ -892,7 +892,7 @@ EXPORT void mir_sample(int dim, int64_t n, int64_t first_id, double x0, double y
     if (i > ncx - 1) i = ncx - 1;
     int j = 0;
     if (dim == 2) {
-      double uy = orc_u53(r0[2], r0[3]);
+      double uy = orc_u01(r0[2], r0[3]);
       double py = uy * wx[1] + dxo[1];
       double sy = (py - y0) / dy;
       double fly = floor(sy);
@@ -903,8 +903,8 @@ EXPORT void mir_sample(int dim, int64_t n, int64_t first_id, double x0, double y
       if (j > ncy - 1) j = ncy - 1;
       oy[k] = ofy;
     }
-    double u1 = orc_u53(r1[0], r1[1]), u2 = orc_u53(r1[2], r1[3]);
-    double u3 = orc_u53(r2[0], r2[1]), u4 = orc_u53(r2[2], r2[3]);
+    double u1 = orc_u01(r1[0], r1[1]), u2 = orc_u01(r1[2], r1[3]);
+    double u3 = orc_u01(r2[0], r2[1]), u4 = orc_u01(r2[2], r2[3]);
     double rad1 = sqrt(-2.0 * isk_log(u1)), rad2 = sqrt(-2.0 * isk_log(u3));
     double s1, c1, s2, c2;
     isk_sincos2pi(u2, &s1, &c1);
