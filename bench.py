@@ -31,7 +31,7 @@ PEAK_HBM_GBS = 8000.0
 BYTES_PER_PSTEP_2D = 40  # read+write cell, ox, oy, vx, vy (4 B each)
 
 
-def cpu_baseline(cells, budget_s=12.0, sample=1 << 18):
+def cpu_baseline(cells, budget_s=20.0, sample=1 << 22, max_steps=100):
     """The oracle's FP64 restatement of the reference's per-step particle work
     (grid_to_particle, push_in_cartesian! with B = 0, discard!, particle_to_grid)
     on one host thread, for a bounded sample of electrons on the same grid."""
@@ -61,7 +61,7 @@ def cpu_baseline(cells, budget_s=12.0, sample=1 << 18):
     psteps = 0
     npart = np.array([sample], dtype=np.int64)
     t0 = time.perf_counter()
-    while time.perf_counter() - t0 < budget_s and steps < 200:
+    while time.perf_counter() - t0 < budget_s and steps < max_steps:
         m = int(npart[0])
         lib.ref_grid_to_particle(2, m, O.ptr(x), nx, nx, dx, dx, O.ptr(E), O.ptr(pE))
         lib.ref_push(m, 2, O.ptr(x), O.ptr(v), O.ptr(pE), O.ptr(B), qm, p["dt"])
@@ -207,8 +207,10 @@ def main():
         out["poisson"]["ms_per_solve"] = sp["kernel_ms"]["poisson"] / max(
             sp["kernel_calls"]["poisson"], 1)
         nbytes = 1 << 30
-        cms = ctx.bench_copy(nbytes, 20)
+        cms = ctx.bench_copy(nbytes, 20, nontemporal=True)
+        pms = ctx.bench_copy(nbytes, 5, nontemporal=False)
         out["hbm_copy_gbps"] = 2 * nbytes / (cms * 1e-3) / 1e9
+        out["hbm_copy_plain_gbps"] = 2 * nbytes / (pms * 1e-3) / 1e9
         out["roofline"]["frac_of_copy"] = achieved / out["hbm_copy_gbps"]
 
     if not args.no_cpu:

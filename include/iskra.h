@@ -265,9 +265,12 @@ int iskra_profile(iskra_ctx *ctx, int enable);
 
 /* ---- diagnostics ---------------------------------------------------------- */
 /* Device-to-device copy of `bytes` bytes, `iters` times, with the library's
- * own streaming copy kernel; *ms = mean duration of one copy (HIP events).
- * Measures the achievable HBM bandwidth the roofline is reported against. */
-int iskra_bench_copy(iskra_ctx *ctx, int64_t bytes, int iters, double *ms);
+ * own streaming copy kernel (nontemporal != 0: non-temporal loads/stores,
+ * else plain ones, as the particle kernels use); *ms = mean duration of one
+ * copy (HIP events).  Measures the achievable HBM bandwidth and calibrates
+ * the HBM traffic counters. */
+int iskra_bench_copy(iskra_ctx *ctx, int64_t bytes, int iters, int nontemporal,
+                     double *ms);
 
 #ifdef __cplusplus
 }

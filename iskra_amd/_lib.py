@@ -105,7 +105,7 @@ def lib():
             "iskra_step": (I, [P, D, I64]),
             "iskra_stats_get": (I, [P, C.POINTER(Stats)]),
             "iskra_profile": (I, [P, I]),
-            "iskra_bench_copy": (I, [P, I64, I, C.POINTER(D)]),
+            "iskra_bench_copy": (I, [P, I64, I, I, C.POINTER(D)]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -358,9 +358,10 @@ class Context:
     def profile(self, enable=True):
         check(lib().iskra_profile(self.h, int(enable)))
 
-    def bench_copy(self, nbytes, iters=20):
+    def bench_copy(self, nbytes, iters=20, nontemporal=True):
         ms = C.c_double()
-        check(lib().iskra_bench_copy(self.h, int(nbytes), int(iters), C.byref(ms)))
+        check(lib().iskra_bench_copy(self.h, int(nbytes), int(iters), int(nontemporal),
+                                     C.byref(ms)))
         return ms.value
 
     def sync(self):

@@ -221,6 +221,7 @@ static void prof_release(iskra_ctx *c) {
 
 typedef float isk_f4 __attribute__((ext_vector_type(4)));
 /* streaming copy: each work-group moves 4 x 256 x 16 B (16 KiB), loads first */
+template <int NT>
 __global__ __launch_bounds__(256) void k_copy(const isk_f4 *__restrict__ src,
                                               isk_f4 *__restrict__ dst, long long n) {
   long long base = (long long)blockIdx.x * 1024 + threadIdx.x;
@@ -228,12 +229,15 @@ __global__ __launch_bounds__(256) void k_copy(const isk_f4 *__restrict__ src,
 #pragma unroll
   for (int u = 0; u < 4; ++u) {
     long long i = base + u * 256;
-    if (i < n) v[u] = __builtin_nontemporal_load(&src[i]);
+    if (i < n) v[u] = NT ? __builtin_nontemporal_load(&src[i]) : src[i];
   }
 #pragma unroll
   for (int u = 0; u < 4; ++u) {
     long long i = base + u * 256;
-    if (i < n) __builtin_nontemporal_store(v[u], &dst[i]);
+    if (i < n) {
+      if (NT) __builtin_nontemporal_store(v[u], &dst[i]);
+      else dst[i] = v[u];
+    }
   }
 }
 
@@ -269,10 +273,10 @@ int iskra_ctx_create(int device, iskra_ctx **out) {
   memset(&c->stats, 0, sizeof(c->stats));
   c->t.T = 8;
   c->t.log2T = 3;
-  c->t.CH = 4096;
+  c->t.CH = 2048;
   c->t.M = 2;
   c->t.W = c->t.T + 2 * c->t.M + 1;
-  c->t.sort_every = 4;
+  c->t.sort_every = 8;
   for (int k = 0; k <= ISK_MAX_TAGS; ++k) {
     c->ps.tag_val[k] = 0.0;
     c->ps.wf_on[k] = 0;
@@ -1151,7 +1155,7 @@ int iskra_profile(iskra_ctx *c, int enable) {
   return ISKRA_OK;
 }
 
-int iskra_bench_copy(iskra_ctx *c, int64_t bytes, int iters, double *ms) {
+int iskra_bench_copy(iskra_ctx *c, int64_t bytes, int iters, int nontemporal, double *ms) {
   CHECK_CTX(c);
   if (bytes < 16 || iters < 1 || !ms) return ISKRA_E_ARG;
   long long n = bytes / 16;
@@ -1160,14 +1164,16 @@ int iskra_bench_copy(iskra_ctx *c, int64_t bytes, int iters, double *ms) {
   RC(isk_alloc(&b, (size_t)n * 16));
   ISK_HIP(hipMemsetAsync(a, 1, (size_t)n * 16, c->stream));
   long long blocks = (n + 1023) / 1024;
-  hipLaunchKernelGGL(k_copy, dim3((unsigned)blocks), dim3(256), 0, c->stream, (const isk_f4 *)a, (isk_f4 *)b, n);
+  auto kern = nontemporal ? k_copy<1> : k_copy<0>;
+  hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(256), 0, c->stream, (const isk_f4 *)a, (isk_f4 *)b, n);
   ISK_CHECK_LAUNCH();
   hipEvent_t e0, e1;
   ISK_HIP(hipEventCreate(&e0));
   ISK_HIP(hipEventCreate(&e1));
   ISK_HIP(hipEventRecord(e0, c->stream));
   for (int k = 0; k < iters; ++k)
-    hipLaunchKernelGGL(k_copy, dim3((unsigned)blocks), dim3(256), 0, c->stream, (const isk_f4 *)a, (isk_f4 *)b, n);
+    hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(256), 0, c->stream, (const isk_f4 *)a,
+                       (isk_f4 *)b, n);
   ISK_CHECK_LAUNCH();
   ISK_HIP(hipEventRecord(e1, c->stream));
   ISK_HIP(hipEventSynchronize(e1));
