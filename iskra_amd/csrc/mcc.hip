@@ -15,7 +15,7 @@
  *
  * Draws: Philox4x32-10, key = seed, counter = (id, step, stream, block);
  * block 0 gives the candidate and process uniforms, blocks 1..5 the collision
- * draws in the order the reference consumes them.  Uniforms are 53-bit
+ * draws in the order the reference consumes them.  Uniforms are 52-bit, in (0,1)
  * (isk_u01); normals are Box-Muller pairs.
  */
 #include <math.h>
