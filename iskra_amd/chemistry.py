@@ -1,0 +1,171 @@
+"""Chemistry: cross sections, reactions and null-collision Monte Carlo
+collisions (Chemistry/src/{cross_section,reactions,mcc}.jl).
+
+`reactions([...])` plays the part of the `@reactions` macro: each entry is
+(rate, reactants, products[, type]) with species objects instead of symbols,
+in the order the macro would list them.  `mcc(reactions)` builds the
+MonteCarloCollisions interaction; it runs on the GPU inside the loop
+(PIC.perform!, mcc.jl:231-289).
+"""
+import numpy as np
+
+from . import _lib as L
+from .particle_in_cell import FluidSpecies, KineticSpecies
+
+
+class CrossSection:
+    """CrossSection (cross_section.jl:3-17): piecewise-linear sigma(eps) over
+    table nodes (eps in eV, sigma in m^2), flat outside the table."""
+
+    def __init__(self, xs, ys=None):
+        if ys is None:
+            nodes = np.asarray(xs, dtype=np.float64)
+            xs, ys = nodes[:, 0], nodes[:, 1]
+        self.nodes = np.column_stack([np.asarray(xs, dtype=np.float64),
+                                      np.asarray(ys, dtype=np.float64)])
+
+    def maximum(self):
+        return float(self.nodes[:, 1].max())
+
+    def argmax(self):
+        return float(self.nodes[np.argmax(self.nodes[:, 1]), 0])
+
+
+# collision types (mcc.jl:2-16)
+class ElasticIsotropic:
+    code = L.COLL_ELASTIC_ISOTROPIC
+    energy = 0.0
+
+
+class ElasticBackward:
+    code = L.COLL_ELASTIC_BACKWARD
+    energy = 0.0
+
+
+class Excitation:
+    code = L.COLL_EXCITATION
+
+    def __init__(self, energy):
+        self.energy = float(energy)
+
+
+class Ionization:
+    code = L.COLL_IONIZATION
+
+    def __init__(self, energy):
+        self.energy = float(energy)
+
+
+class ChemicalReaction:
+    """ChemicalReaction (reactions.jl:7-12): type, rate, reactants and the
+    net stoichiometry (species -> net count)."""
+
+    def __init__(self, type_, rate, reactants, stoichiometry):
+        self.type = type_
+        self.rate = rate
+        self.reactants = reactants
+        self.stoichiometry = stoichiometry
+
+
+def reactions(entries):
+    """The @reactions macro (reactions.jl:3-55): entries of
+    (rate, reactants, products[, type]); reactants/products are lists of
+    species (repeats count)."""
+    out = []
+    for e in entries:
+        rate, reac, prod = e[0], list(e[1]), list(e[2])
+        type_ = e[3] if len(e) > 3 else None
+        order = []
+        for s in reac + prod:
+            if not any(s is o for o in order):
+                order.append(s)
+        stoich = []
+        reactants = []
+        for s in order:
+            r = sum(1 for t in reac if t is s)
+            p = sum(1 for t in prod if t is s)
+            if r:
+                reactants.append((s, r))
+            if p - r:
+                stoich.append((s, p - r))
+        out.append(ChemicalReaction(type_, rate, reactants, stoich))
+    return out
+
+
+class Collision:
+    """MCC.Collision (mcc.jl:10-16)."""
+
+    def __init__(self, type_, rate, source, target, products):
+        self.type = type_
+        self.rate = rate
+        self.source = source
+        self.target = target
+        self.products = products
+
+
+class MonteCarloCollisions:
+    """MonteCarloCollisions (mcc.jl:18-51): the collisions of one source
+    species with one fluid target, in reaction order."""
+
+    def __init__(self, collisions):
+        if not collisions:
+            raise ValueError("mcc: no collisions")
+        src = collisions[0].source
+        tgt = collisions[0].target
+        for c in collisions:
+            if c.source is not src or c.target is not tgt:
+                raise ValueError("mcc: all collisions must share one source and one target")
+        self.collisions = collisions
+        self.source = src
+        self.target = tgt
+        self._ctx = None
+        self._id = -1
+        self.events = np.zeros(len(collisions), dtype=np.int64)
+
+    def _bind(self, ctx, grid):
+        types, energy, tables = [], [], []
+        ion_sid = -1
+        for c in self.collisions:
+            types.append(c.type.code)
+            energy.append(c.type.energy)
+            tables.append((c.rate.nodes[:, 0], c.rate.nodes[:, 1]))
+            if c.type.code == L.COLL_IONIZATION:
+                ions = [p for p in c.products if p is not self.source]
+                if len(ions) != 1:
+                    raise ValueError("mcc: ionisation needs exactly one ion product species")
+                ion_sid = ions[0]._sid
+        tgt = self.target
+        self._id = ctx.mcc(self.source._sid, types, energy, tables, ion=ion_sid,
+                           target_m=tgt.m, target_q=tgt.q, target_T=tgt.T)
+        n = np.broadcast_to(tgt.n, grid.n)
+        ctx.set_field(L.F_TARGET, np.ascontiguousarray(n.T).ravel(), index=self._id)
+        self._ctx = ctx
+
+    def counts(self, reset=False):
+        """nu per node and process (mcc.jl:283-288), accumulated on the device."""
+        return self._ctx.mcc_counts(self._id, len(self.collisions), reset)
+
+
+def _accept(r):
+    """accept(::ChemicalReaction) (mcc.jl:291-312)."""
+    source = target = None
+    if len(r.reactants) != 2:
+        raise ValueError("Monte Carlo Collisions support only two reacting species: one fluid "
+                         "and one kinetic")
+    for s, _ in r.reactants:
+        if isinstance(s, FluidSpecies):
+            target = s
+        elif isinstance(s, KineticSpecies):
+            source = s
+    if source is None:
+        raise ValueError("Reaction without particle species")
+    if target is None:
+        raise ValueError("Reaction without fluid species")
+    products = [s for s, c in r.stoichiometry if c > 0]
+    type_ = r.type if r.type is not None else ElasticIsotropic()
+    return Collision(type_, r.rate, source, target, products)
+
+
+def mcc(reaction_list):
+    """mcc(reactions) (mcc.jl:314-320)."""
+    return MonteCarloCollisions([_accept(r) for r in reaction_list])

@@ -1,0 +1,454 @@
+"""ParticleInCell: species, sources, the particle operators and the time loop
+(ParticleInCell/src/ParticleInCell.jl, pic/*.jl) plus the set-up helpers of
+problem/configuration.jl.
+
+Species are host objects with the reference's fields (x: N x D, v: N x 3,
+np, wg, id, q, m, w0, name).  Once a simulation binds them (`solve`, or
+`bind`), their particles live on the GPU; the host arrays become a mirror
+that is downloaded when read and uploaded again before the next device
+operation if it was touched, so hooks may edit `part.x`, `part.v` and
+`part.np` as reference scripts do.  Every particle operation runs on the GPU;
+there is no host implementation of any of them.
+
+Hooks are module attributes that a script replaces, as a Julia script
+redefines ParticleInCell.after_push & co. (ParticleInCell.jl:38-41):
+
+    import iskra_amd.particle_in_cell as pic
+    pic.after_push = lambda part, grid: pic.wrap(part, grid)
+
+`after_push` is evaluated once per species when the simulation is bound: the
+wrap/discard calls it makes select the boundary that the fused push kernel
+applies after every push (wrap.jl:1-33).
+"""
+import math
+
+import numpy as np
+
+from . import _lib as L
+from .finite_difference_method import PoissonSolver
+from .regular_grids import UniformGrid
+
+KB = 1.380649e-23  # problem/units_and_constants.jl (CODATA 2018, Unitful)
+
+
+# ---------------------------------------------------------------------------
+# species and sources (pic/kinetic.jl, pic/fluid.jl, pic/sources.jl)
+
+class KineticSpecies:
+    """KineticSpecies{D,V} (pic/kinetic.jl:1-18) with a device copy."""
+
+    def __init__(self, name, N, D=2, V=3):
+        if V != 3:
+            raise ValueError("KineticSpecies: only V = 3 velocity components")
+        self.name = name
+        self.N = int(N)
+        self.D = int(D)
+        self.V = 3
+        self.m = 0.0
+        self.q = 0.0
+        self.w0 = 1.0
+        self.wg = np.ones(self.N)
+        self._x = np.zeros((self.N, self.D))
+        self._v = np.zeros((self.N, 3))
+        self._id = np.arange(1, self.N + 1, dtype=np.uint32)  # particle_uuids (kinetic.jl:15)
+        self._np = 0
+        self.n = None            # last density (set by the loop)
+        self._ctx = None
+        self._sid = -1
+        self._host_stale = False  # device holds newer particles than the host arrays
+        self._dev_stale = False   # host arrays were handed out / changed since upload
+
+    def __repr__(self):
+        return self.name
+
+    # host mirror -------------------------------------------------------------
+    def _pull(self):
+        if self._ctx is not None and self._host_stale:
+            X, Vv, ids = self._ctx.download(self._sid, self.N)
+            n = X.shape[0]
+            self._x[:n] = X
+            self._v[:n] = Vv
+            self._id[:n] = ids
+            self._np = n
+            self.wg[:] = self.w0
+            self._host_stale = False
+
+    def _push(self):
+        """Upload the host arrays if they may have changed."""
+        if self._ctx is not None and self._dev_stale:
+            n = self._np
+            self._ctx.upload(self._sid, self._x[:n], self._v[:n], self._id[:n])
+            self._dev_stale = False
+
+    def _touched(self):
+        self._pull()
+        if self._ctx is not None:
+            self._dev_stale = True
+
+    @property
+    def x(self):
+        self._touched()
+        return self._x
+
+    @x.setter
+    def x(self, val):
+        self._touched()
+        self._x[...] = val
+
+    @property
+    def v(self):
+        self._touched()
+        return self._v
+
+    @v.setter
+    def v(self, val):
+        self._touched()
+        self._v[...] = val
+
+    @property
+    def id(self):
+        self._touched()
+        return self._id
+
+    @property
+    def np(self):
+        if self._ctx is not None and self._host_stale:
+            return self._ctx.count(self._sid)
+        return self._np
+
+    @np.setter
+    def np(self, val):
+        self._touched()
+        if not 0 <= int(val) <= self.N:
+            raise ValueError(f"{self.name}: np = {val} outside [0, {self.N}]")
+        self._np = int(val)
+
+
+class FluidSpecies:
+    """FluidSpecies{D} (pic/fluid.jl:1-11): a background with density n (per
+    node), mobility mu, charge q, mass m, temperature T.  It enters rho as
+    q*n (no volume division, fluid.jl:11) and is the MCC target gas.  Its
+    density is held constant (the reference's advection-diffusion gives
+    dn = 0 for uniform n, advection_diffusion.jl:4-19)."""
+
+    def __init__(self, name, mu, q, m, n, T):
+        self.name = name
+        self.mu = float(mu)
+        self.q = float(q)
+        self.m = float(m)
+        self.n = np.asarray(n, dtype=np.float64)
+        self.T = float(T)
+
+    def __repr__(self):
+        return self.name
+
+
+class MaxwellianSource:
+    """MaxwellianSource{D,V} (pic/sources.jl:8-22): rate per second, position
+    box wx (+ offset dx), velocity spread wv (+ drift dv)."""
+
+    def __init__(self, rate, wx, wv, dx=None, dv=None):
+        self.rate = float(rate)
+        self.wx = np.atleast_1d(np.asarray(wx, dtype=np.float64)).ravel()
+        self.wv = np.atleast_1d(np.asarray(wv, dtype=np.float64)).ravel()
+        self.dx = np.zeros_like(self.wx) if dx is None else np.atleast_1d(
+            np.asarray(dx, dtype=np.float64)).ravel()
+        self.dv = np.zeros(3) if dv is None else np.atleast_1d(
+            np.asarray(dv, dtype=np.float64)).ravel()
+        self.species = None
+        self.seed_stream = None
+
+
+def thermal_speed(T, m):
+    """thermal_speed (problem/configuration.jl:79-81): sqrt(2 kB T / m)."""
+    return math.sqrt(2.0 * KB * T / m)
+
+
+def create_kinetic_species(name, N, q, m, weight, D=2, V=3):
+    """create_kinetic_species (problem/configuration.jl:95-102)."""
+    s = KineticSpecies(name, N, D, V)
+    s.q = float(q)
+    s.m = float(m)
+    s.wg *= weight
+    s.w0 = float(weight)
+    return s
+
+
+def create_thermalized_beam(species, x, vb, dx=None, T=300.0, rate=1.0):
+    """create_thermalized_beam (problem/configuration.jl:89-93)."""
+    vth = thermal_speed(T, species.m) * np.ones(3)
+    x = np.atleast_1d(np.asarray(x, dtype=np.float64)).ravel()
+    src = MaxwellianSource(rate, x, vth, dx=np.zeros_like(x) if dx is None else dx,
+                           dv=np.asarray(vb, dtype=np.float64).ravel())
+    src.species = species
+    return src
+
+
+class BorisPusher:
+    """BorisPusher{:xy} (pic/pushers.jl:4-6); E only (B = 0 on this path)."""
+
+
+def create_boris_pusher():
+    return BorisPusher()
+
+
+# ---------------------------------------------------------------------------
+# configuration (problem/configuration.jl:5-15)
+
+class Config:
+    def __init__(self, grid=None, solver=None, pusher=None, species=None, interactions=None,
+                 sources=None, tracker=None, circuit=None, cells=None):
+        self.grid = grid
+        self.solver = solver
+        self.pusher = pusher
+        self.species = list(species or [])
+        self.interactions = list(interactions or [])
+        self.sources = list(sources or [])
+        self.tracker = tracker
+        self.circuit = circuit
+        self.cells = cells
+        self.seed = 0
+        self.device = 0
+        self.ctx = None
+
+
+# ---------------------------------------------------------------------------
+# hooks (ParticleInCell.jl:38-41) -- replace these module attributes
+
+def enter_loop():
+    pass
+
+
+def after_loop(it, t, dt):
+    pass
+
+
+def exit_loop():
+    pass
+
+
+def _default_after_push(part, grid):
+    wrap(part, grid)
+
+
+after_push = _default_after_push
+
+_recording = {}
+
+
+def wrap(part, grid, dims=None):
+    """wrap! (pic/surfaces/wrap.jl:20-33): periodic re-entry along dims
+    (1-based like the reference; default all).  Inside `after_push` it
+    selects the boundary the push kernel applies."""
+    _boundary(part, grid, dims, L.BND_WRAP)
+
+
+def discard(part, grid, dims=None):
+    """discard! (pic/surfaces/wrap.jl:1-18): particles leaving along dims are
+    removed (compacted at the next sort instead of swap-removed)."""
+    _boundary(part, grid, dims, L.BND_DISCARD)
+
+
+def _boundary(part, grid, dims, mode):
+    modes = _recording.get(id(part))
+    if modes is None:
+        raise RuntimeError("wrap!/discard! run inside the fused push kernel; call them from "
+                           "the after_push hook, which selects the boundary per dimension")
+    for d in (range(1, grid.dim + 1) if dims is None else np.atleast_1d(dims)):
+        modes[int(d) - 1] = mode
+
+
+# ---------------------------------------------------------------------------
+# binding a configuration to a device context
+
+def _fluid_species(config):
+    return [s for s in config.species if isinstance(s, FluidSpecies)]
+
+
+def _kinetic_species(config):
+    return [s for s in config.species if isinstance(s, KineticSpecies)]
+
+
+def bind(config):
+    """Create the device context for `config` and move grid, solver,
+    species, fluids and interactions onto it.  Called by `solve`; callable
+    directly to use the operators below outside the loop."""
+    from ._lib import Context
+    from . import chemistry
+
+    g = config.grid
+    if not isinstance(g, UniformGrid):
+        raise TypeError("config.grid must come from create_uniform_grid")
+    ctx = Context(config.device)
+    ctx.grid(g.dim, g.n, g.dh, g.origin if g.dim == 2 else (g.origin[0], 0.0), g.vol_periodic())
+    ctx.rng(config.seed)
+    ctx._seed = config.seed
+    config.ctx = ctx
+    for s in _kinetic_species(config):
+        if s.D != g.dim:
+            raise ValueError(f"{s.name}: D = {s.D} on a {g.dim}-D grid")
+        s._ctx = None
+        s._pull()
+        s._sid = ctx.species(s.q, s.m, s.w0, s.N, track_ids=True)
+        modes = [L.BND_NONE, L.BND_NONE]
+        _recording[id(s)] = modes
+        try:
+            after_push(s, g)
+        finally:
+            del _recording[id(s)]
+        ctx.boundary(s._sid, modes[:g.dim])
+        s._ctx = ctx
+        s._dev_stale = True
+        s._push()
+    bg = np.zeros(g.n)
+    for f in _fluid_species(config):
+        bg += f.q * np.broadcast_to(f.n, g.n)
+    ctx.set_field(L.F_BG, bg.T.ravel())
+    if isinstance(config.solver, PoissonSolver):
+        config.solver._ctx = ctx
+        config.solver._configure(ctx)
+    for inter in config.interactions:
+        if isinstance(inter, chemistry.MonteCarloCollisions):
+            inter._bind(ctx, g)
+    return ctx
+
+
+def _sync_to_device(config):
+    for s in _kinetic_species(config):
+        s._push()
+
+
+def _mark_device_newer(config):
+    for s in _kinetic_species(config):
+        s._host_stale = True
+        s._dev_stale = False
+
+
+# ---------------------------------------------------------------------------
+# operators (each one a device call)
+
+def init(src, species, dt):
+    """init(src, species, dt) = sample!(src, species, dt) (ParticleInCell.jl:47,
+    pic/sources.jl:24-34): append min(free capacity, floor(rate*dt))
+    particles drawn on the device (Philox, keyed by the species' source
+    count)."""
+    if species._ctx is None:
+        raise RuntimeError("init: the species is not bound to a simulation yet (call it from "
+                           "enter_loop, or bind(config) first)")
+    species._push()
+    n = int(math.floor(src.rate * dt))
+    ctx = species._ctx
+    stream = getattr(species, "_nsample", 0)
+    species._nsample = stream + 1
+    wx = src.wx[:species.D]
+    dx = src.dx[:species.D]
+    ctx.sample_maxwellian(species._sid, n, wx, src.wv, dx=dx, dv=src.dv,
+                          seed=ctx_seed(ctx), stream=(species._sid << 16) | stream)
+    species._host_stale = True
+
+
+def ctx_seed(ctx):
+    return getattr(ctx, "_seed", 0)
+
+
+def advance(part, E, B, dt, config):
+    """advance!(::KineticSpecies, E, B, dt, config) (ParticleInCell.jl:51-61):
+    gather E, Boris push (B = 0), after_push boundary, and this species'
+    CIC deposit, in one kernel, with the device's current E."""
+    part._push()
+    config.ctx.push(part._sid, dt)
+    part._host_stale = True
+
+
+def push_particles(pusher, part, E, B, dt):
+    """push_particles! (pic/pushers.jl:8-11) is fused with the gather, the
+    boundary and the deposit; use advance."""
+    raise RuntimeError("push_particles!: the push is fused with gather, boundary and deposit; "
+                       "call advance(part, E, B, dt, config)")
+
+
+def particle_to_grid(part, grid):
+    """particle_to_grid(part, grid, p -> wg[p]) (pic/cloud_in_cell.jl:1-18):
+    CIC sum of the particle weights per node."""
+    from .regular_grids import cell_volume
+    return density(part, grid) * cell_volume(grid)
+
+
+def density(part, grid):
+    """density(::KineticSpecies, grid) (pic/kinetic.jl:53): CIC weights per
+    node divided by cell_volume; FluidSpecies return n (fluid.jl:11)."""
+    if isinstance(part, FluidSpecies):
+        return part.n
+    part._push()
+    ctx = part._ctx
+    ctx.deposit_species(part._sid)
+    ctx.deposit()
+    return ctx.field(L.F_DENSITY, part._sid).reshape(grid.n[::-1]).T.copy()
+
+
+def grid_to_particle(grid, part, E=None):
+    """grid_to_particle(grid, part, (i,j) -> E[i,j,:]) (pic/cloud_in_cell.jl:20-36):
+    bilinear interpolation of E to the particles (np x 3).  E (nx[,ny],3)
+    replaces the device field first when given."""
+    part._push()
+    ctx = part._ctx
+    if E is not None:
+        E = np.asarray(E, dtype=np.float64)
+        ctx.set_field(L.F_EX, E[..., 0].T.ravel())
+        if grid.dim == 2:
+            ctx.set_field(L.F_EY, E[..., 1].T.ravel())
+    return ctx.gather(part._sid)
+
+
+# ---------------------------------------------------------------------------
+# the loop (ParticleInCell.jl:84-139)
+
+_noop_after_loop = after_loop
+
+
+def solve(config, dt=1e-5, timesteps=200, seed=None):
+    """solve(config, dt, timesteps) (ParticleInCell.jl:84-139).
+
+    Per iteration: sources, MCC interactions (with the previous E), advance!
+    of every kinetic species (gather, push, after_push boundary, deposit),
+    rho = sum q n (+ q n of fluid species), phi from the Poisson solve, E,
+    then after_loop(it, it*dt - dt, dt).  When after_loop is the default and
+    there are no per-step sources, the iterations run as one device call."""
+    if seed is not None:
+        config.seed = int(seed)
+    ctx = bind(config)
+    enter_loop()
+    _sync_to_device(config)
+    dt = float(dt)
+    fused = after_loop is _noop_after_loop and not config.sources
+    if fused:
+        ctx.step(dt, int(timesteps))
+        _mark_device_newer(config)
+    else:
+        for it in range(1, int(timesteps) + 1):
+            for src in config.sources:
+                init(src, src.species, dt)
+            _sync_to_device(config)
+            ctx.step(dt, 1)
+            _mark_device_newer(config)
+            after_loop(it, it * dt - dt, dt)
+            _sync_to_device(config)
+    exit_loop()
+    return ctx
+
+
+def field(config, name):
+    """Node field of the bound simulation: "rho", "phi", "E" (nx[,ny],3)."""
+    ctx = config.ctx
+    n = config.grid.n
+    if name == "rho":
+        return ctx.field(L.F_RHO).reshape(n[::-1]).T.copy()
+    if name == "phi":
+        return ctx.field(L.F_PHI).reshape(n[::-1]).T.copy()
+    if name == "E":
+        E = np.zeros(n + (3,))
+        E[..., 0] = ctx.field(L.F_EX).reshape(n[::-1]).T
+        if len(n) == 2:
+            E[..., 1] = ctx.field(L.F_EY).reshape(n[::-1]).T
+        return E
+    raise KeyError(name)
