@@ -243,6 +243,7 @@ enum {
   ISKRA_K_MCC = 3,     /* collisions + product append */
   ISKRA_K_MERGE = 4,   /* window merge + rho */
   ISKRA_K_EFIELD = 5,
+  ISKRA_K_WARMSTART = 6, /* learned warm-start inference */
   ISKRA_K_COUNT = 8
 };
 typedef struct {
@@ -262,6 +263,52 @@ int iskra_stats_get(iskra_ctx *ctx, iskra_stats *out);
  * No reference counterpart (the reference only @time's the whole solve,
  * src/iskra.jl:51). */
 int iskra_profile(iskra_ctx *ctx, int enable);
+
+/* ---- learned warm start for the 2-D Poisson solve ----------------------- */
+/* No reference counterpart: the reference solves A\b directly every step
+ * (FiniteDifferenceMethod/src/generalized_poisson.jl:201-203,372-378), so it
+ * has no iterative solve to seed.  A convolutional encoder-decoder maps the
+ * right-hand side f = dx^2 rho / eps0 of a square 2-D grid of (side+1)^2
+ * nodes with phi = 0 on all four walls to phi at the interior nodes; the
+ * iterative solver then starts from that phi (ISKRA_WARM_PREV) and iterates
+ * to its usual tolerance.  levels encoder/decoder levels with base*2^l
+ * channels, 3x3 convolutions, BF16 activations and weights on MFMA with FP32
+ * accumulation, FP32 master weights trained with Adam on the device.
+ * Parameters are float32, layer by layer in the order enc0a, enc0b, ...,
+ * enc(L-1)b, dec(L-2)a, dec(L-2)b, ..., dec0b, out; each layer [cout][3x3][cin]
+ * (cin of enc0a is 8: channel 0 carries the input; cout of out is 16:
+ * channel 0 is the output).  The network is bound to ctx (its stream and,
+ * for capture/predict, its grid and fields). */
+typedef struct iskra_ws iskra_ws;
+int iskra_ws_create(iskra_ctx *ctx, int levels, int base_channels, int side,
+                    int max_batch, iskra_ws **net);
+int iskra_ws_destroy(iskra_ws *net);
+int iskra_ws_param_count(iskra_ws *net, int64_t *n);
+int iskra_ws_set_params(iskra_ws *net, const float *params);
+int iskra_ws_get_params(iskra_ws *net, float *params);
+/* phi = output * gain * rms(f) (the network sees f / rms(f)) */
+int iskra_ws_set_gain(iskra_ws *net, double gain);
+/* Host batches (side x side planes per sample, already normalised input):
+ * forward -> out; backward: forward, mean squared error over the interior
+ * (side-1)^2 pixels against target, gradients of all parameters -> grads. */
+int iskra_ws_forward(iskra_ws *net, int batch, const float *in, float *out);
+int iskra_ws_backward(iskra_ws *net, int batch, const float *in,
+                      const float *target, float *grads, double *loss);
+/* Device-resident training set: reserve n samples; capture appends the
+ * context's current (f / rms(f), phi / rms(f)) (call after a solve). */
+int iskra_ws_dataset_reserve(iskra_ws *net, int64_t n);
+int iskra_ws_capture(iskra_ws *net, int64_t *index);
+int iskra_ws_dataset_size(iskra_ws *net, int64_t *n);
+int iskra_ws_dataset_rms(iskra_ws *net, int64_t first, int64_t count,
+                         double *rms);
+/* steps Adam steps on random batches of samples [first, first+count)
+ * (targets divided by gain), batch draws from Philox4x32-10 keyed by seed;
+ * losses (nullable) receives each step's batch loss.  eval: mean loss. */
+int iskra_ws_train(iskra_ws *net, int64_t first, int64_t count, int batch,
+                   int steps, float lr, uint64_t seed, double *losses);
+int iskra_ws_eval(iskra_ws *net, int64_t first, int64_t count, double *mse);
+/* phi at the interior nodes of the context <- prediction from its rho */
+int iskra_ws_predict(iskra_ws *net);
 
 /* ---- diagnostics ---------------------------------------------------------- */
 /* Device-to-device copy of `bytes` bytes, `iters` times, with the library's

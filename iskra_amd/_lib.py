@@ -30,6 +30,10 @@ EXPORTS = [
     "iskra_poisson_set_tag_waveform", "iskra_poisson_solve", "iskra_efield", "iskra_mcc_add",
     "iskra_mcc_run", "iskra_mcc_counts", "iskra_rng_set", "iskra_rng_export", "iskra_step",
     "iskra_stats_get", "iskra_profile", "iskra_bench_copy",
+    "iskra_ws_create", "iskra_ws_destroy", "iskra_ws_param_count", "iskra_ws_set_params",
+    "iskra_ws_get_params", "iskra_ws_set_gain", "iskra_ws_forward", "iskra_ws_backward",
+    "iskra_ws_dataset_reserve", "iskra_ws_capture", "iskra_ws_dataset_size",
+    "iskra_ws_dataset_rms", "iskra_ws_train", "iskra_ws_eval", "iskra_ws_predict",
 ]
 
 
@@ -44,8 +48,8 @@ class Tuning(C.Structure):
                 ("sort_every", C.c_int32)]
 
 
-K_PUSH, K_SORT, K_POISSON, K_MCC, K_MERGE, K_EFIELD = range(6)
-K_NAMES = ["push", "sort", "poisson", "mcc", "merge", "efield", "k6", "k7"]
+K_PUSH, K_SORT, K_POISSON, K_MCC, K_MERGE, K_EFIELD, K_WARMSTART = range(7)
+K_NAMES = ["push", "sort", "poisson", "mcc", "merge", "efield", "warmstart", "k7"]
 
 
 class Stats(C.Structure):
@@ -106,6 +110,21 @@ def lib():
             "iskra_stats_get": (I, [P, C.POINTER(Stats)]),
             "iskra_profile": (I, [P, I]),
             "iskra_bench_copy": (I, [P, I64, I, I, C.POINTER(D)]),
+            "iskra_ws_create": (I, [P, I, I, I, I, C.POINTER(P)]),
+            "iskra_ws_destroy": (I, [P]),
+            "iskra_ws_param_count": (I, [P, C.POINTER(I64)]),
+            "iskra_ws_set_params": (I, [P, P]),
+            "iskra_ws_get_params": (I, [P, P]),
+            "iskra_ws_set_gain": (I, [P, D]),
+            "iskra_ws_forward": (I, [P, I, P, P]),
+            "iskra_ws_backward": (I, [P, I, P, P, P, C.POINTER(D)]),
+            "iskra_ws_dataset_reserve": (I, [P, I64]),
+            "iskra_ws_capture": (I, [P, C.POINTER(I64)]),
+            "iskra_ws_dataset_size": (I, [P, C.POINTER(I64)]),
+            "iskra_ws_dataset_rms": (I, [P, I64, I64, C.POINTER(D)]),
+            "iskra_ws_train": (I, [P, I64, I64, I, I, C.c_float, U64, P]),
+            "iskra_ws_eval": (I, [P, I64, I64, C.POINTER(D)]),
+            "iskra_ws_predict": (I, [P]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -351,8 +370,8 @@ class Context:
         return dict(steps=s.steps, particle_steps=s.particle_steps,
                     poisson_solves=s.poisson_solves, poisson_iters=s.poisson_iters,
                     last_iters=s.last_iters, last_relres=s.last_relres,
-                    kernel_ms={K_NAMES[k]: s.kernel_ms[k] for k in range(6)},
-                    kernel_calls={K_NAMES[k]: s.kernel_calls[k] for k in range(6)},
+                    kernel_ms={K_NAMES[k]: s.kernel_ms[k] for k in range(7)},
+                    kernel_calls={K_NAMES[k]: s.kernel_calls[k] for k in range(7)},
                     push_particles=s.push_particles)
 
     def profile(self, enable=True):
@@ -366,3 +385,89 @@ class Context:
 
     def sync(self):
         check(lib().iskra_sync(self.h))
+
+
+class WarmStart:
+    """Learned warm start for the 2-D Poisson solve of `ctx` (iskra_ws_*):
+    encoder-decoder with `levels` levels and base*2^l channels on a
+    side x side pixel grid (the context grid has side+1 nodes per axis)."""
+
+    def __init__(self, ctx, levels=4, base=32, side=512, max_batch=8):
+        h = P()
+        check(lib().iskra_ws_create(ctx.h, levels, base, side, max_batch, C.byref(h)))
+        self.h = h
+        self.ctx = ctx
+        self.levels, self.base, self.side, self.max_batch = levels, base, side, max_batch
+        n = C.c_int64()
+        check(lib().iskra_ws_param_count(self.h, C.byref(n)))
+        self.nparam = n.value
+
+    def close(self):
+        if self.h:
+            lib().iskra_ws_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        assert p.size == self.nparam
+        check(lib().iskra_ws_set_params(self.h, _p(p)))
+
+    def get_params(self):
+        p = np.zeros(self.nparam, dtype=np.float32)
+        check(lib().iskra_ws_get_params(self.h, _p(p)))
+        return p
+
+    def set_gain(self, g):
+        check(lib().iskra_ws_set_gain(self.h, float(g)))
+
+    def forward(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        out = np.zeros_like(x)
+        check(lib().iskra_ws_forward(self.h, x.shape[0], _p(x), _p(out)))
+        return out
+
+    def backward(self, x, t):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        g = np.zeros(self.nparam, dtype=np.float32)
+        loss = C.c_double()
+        check(lib().iskra_ws_backward(self.h, x.shape[0], _p(x), _p(t), _p(g), C.byref(loss)))
+        return loss.value, g
+
+    def reserve(self, n):
+        check(lib().iskra_ws_dataset_reserve(self.h, int(n)))
+
+    def capture(self):
+        i = C.c_int64()
+        check(lib().iskra_ws_capture(self.h, C.byref(i)))
+        return i.value
+
+    def size(self):
+        n = C.c_int64()
+        check(lib().iskra_ws_dataset_size(self.h, C.byref(n)))
+        return n.value
+
+    def dataset_rms(self, first, count):
+        r = C.c_double()
+        check(lib().iskra_ws_dataset_rms(self.h, int(first), int(count), C.byref(r)))
+        return r.value
+
+    def train(self, first, count, batch, steps, lr, seed=0):
+        losses = np.zeros(max(int(steps), 1))
+        check(lib().iskra_ws_train(self.h, int(first), int(count), int(batch), int(steps),
+                                   float(lr), int(seed), _p(losses)))
+        return losses[:steps]
+
+    def eval(self, first, count):
+        m = C.c_double()
+        check(lib().iskra_ws_eval(self.h, int(first), int(count), C.byref(m)))
+        return m.value
+
+    def predict(self):
+        check(lib().iskra_ws_predict(self.h))

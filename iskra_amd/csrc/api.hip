@@ -23,6 +23,8 @@ void isk_set_error(const char *fmt, ...) {
   va_end(ap);
 }
 
+void isk_clear_error(void) { g_err[0] = 0; }
+
 int isk_hip_fail(hipError_t e, const char *what, const char *file, int line) {
   isk_set_error("HIP error %d (%s) in %s at %s:%d", (int)e, hipGetErrorString(e), what,
                 file, line);

@@ -194,6 +194,7 @@ int isk_prof_end(iskra_ctx *c, hipEvent_t a, int kind, int64_t particles);
 
 /* ---- error plumbing (api.hip) ---- */
 void isk_set_error(const char *fmt, ...);
+void isk_clear_error(void);
 int isk_hip_fail(hipError_t e, const char *what, const char *file, int line);
 #define ISK_HIP(call)                                                   \
   do {                                                                  \

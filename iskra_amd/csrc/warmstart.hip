@@ -1,0 +1,1175 @@
+/*
+ * warmstart.hip -- learned warm start for the 2-D Poisson solve on gfx950.
+ *
+ * A convolutional encoder-decoder maps the right-hand side f = dx^2 rho/eps0
+ * of a square grid with phi = 0 on all four walls to phi at the interior
+ * nodes; its output seeds the iterative solver (PCG or multigrid), which then
+ * iterates to the same FP64 residual as from any other start, so the network
+ * can only change the iteration count, never the answer.  There is no
+ * counterpart in the reference (its Poisson solve is a dense direct solve,
+ * FiniteDifferenceMethod/src/generalized_poisson.jl:201-203,372-378).
+ *
+ * Network (L levels, channels c_l = base * 2^l, 3x3 convolutions, zero
+ * padding, no biases, so the map is positively homogeneous and inputs are
+ * normalised by their RMS):
+ *   encoder  l = 0..L-1:  Ea_l = relu(conv(P_l)),  E_l = relu(conv(Ea_l)),
+ *                         P_0 = input (8 channels, channel 0 = f / s),
+ *                         P_l = avgpool2(E_{l-1})
+ *   decoder  l = L-2..0:  U_l = upsample2(D_{l+1}) (D_{L-1} = E_{L-1}),
+ *                         Da_l = relu(conv(U_l) + E_l),  D_l = relu(conv(Da_l))
+ *   output   o = conv(D_0) channel 0;  phi = o * gain * s.
+ * Tensors are NHWC bfloat16 (interior nodes padded to side S = nodes - 1);
+ * convolutions are implicit GEMMs on v_mfma_f32_16x16x32_bf16 with FP32
+ * accumulation: forward and input-gradient kernels read 8 consecutive
+ * channels per lane (16-byte loads) for both operands, the weight-gradient
+ * kernel stages pixel tiles in LDS and reduces over pixels in fixed order
+ * (split into chunks, summed chunk by chunk), so training is deterministic.
+ * Gradients of activations are FP32; Adam keeps FP32 master weights.
+ */
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "isk_internal.h"
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef uint16_t u16x8 __attribute__((ext_vector_type(8)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+#define WS_MAX_LEVELS 6
+#define WS_IN_CH 8       /* input channels (channel 0 carries f) */
+#define WS_OUT_CH 16     /* output channels computed (channel 0 is used) */
+#define WS_PART_BYTES (64ll << 20)
+#define EW_BLOCK 256
+
+__host__ __device__ inline uint16_t ws_f2bf(float f) {
+  uint32_t u = __builtin_bit_cast(uint32_t, f);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__host__ __device__ inline float ws_bf2f(uint16_t h) {
+  return __builtin_bit_cast(float, (uint32_t)h << 16);
+}
+
+#define WRC(x)             \
+  do {                     \
+    int _rc = (x);         \
+    if (_rc) return _rc;   \
+  } while (0)
+
+/* ------------------------------------------------------------------------ */
+/* convolution: forward (mode 0, bf16 out, optional skip + relu), input       */
+/* gradient (mode 1, fp32 accumulate), network output (mode 2, channel 0)     */
+
+struct ConvArgs {
+  const uint16_t *X;     /* NHWC, Cin channels */
+  const uint16_t *Wt;    /* [Cout][Kpad], k = tap * Cin + ci, zero padded */
+  const uint16_t *skip;  /* NHWC, Cout channels, or null */
+  uint16_t *Y;           /* mode 0 */
+  float *Yf;             /* mode 1: NHWC fp32 (+=); mode 2: plane [npix] */
+  int H, W, Cin, Cout, Kpad, relu, mode;
+};
+
+template <int TM>
+__global__ __launch_bounds__(256) void k_ws_conv(ConvArgs a) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int col = lane & 15, grp = lane >> 4;
+  const long long p = (long long)blockIdx.x * 64 + wv * 16 + col;
+  const int HW = a.H * a.W;
+  const int n = (int)(p / HW);
+  const int rem = (int)(p - (long long)n * HW);
+  const int h = rem / a.W, w = rem - h * a.W;
+  const int co0 = blockIdx.y * TM;
+  f32x4 acc[TM / 16];
+#pragma unroll
+  for (int j = 0; j < TM / 16; ++j) acc[j] = (f32x4){0.f, 0.f, 0.f, 0.f};
+  const int nkb = a.Kpad >> 5;
+  for (int kb = 0; kb < nkb; ++kb) {
+    const int k0 = kb * 32 + grp * 8;
+    const int tap = k0 / a.Cin;
+    const int ci0 = k0 - tap * a.Cin;
+    u32x4 bv = (u32x4){0u, 0u, 0u, 0u};
+    if (tap < 9) {
+      const int hh = h + tap / 3 - 1, ww = w + tap % 3 - 1;
+      if (hh >= 0 && hh < a.H && ww >= 0 && ww < a.W)
+        bv = *(const u32x4 *)(a.X + ((long long)(n * a.H + hh) * a.W + ww) * a.Cin + ci0);
+    }
+    const bf16x8 B = __builtin_bit_cast(bf16x8, bv);
+#pragma unroll
+    for (int j = 0; j < TM / 16; ++j) {
+      const int co = co0 + j * 16 + col;
+      const u32x4 av = *(const u32x4 *)(a.Wt + (long long)co * a.Kpad + k0);
+      acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, av), B, acc[j],
+                                                       0, 0, 0);
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < TM / 16; ++j) {
+    const int cb = co0 + j * 16 + grp * 4;
+    float v[4] = {acc[j][0], acc[j][1], acc[j][2], acc[j][3]};
+    if (a.mode == 0) {
+      if (a.skip) {
+        const uint2 s = *(const uint2 *)(a.skip + p * a.Cout + cb);
+        v[0] += ws_bf2f((uint16_t)(s.x & 0xffffu));
+        v[1] += ws_bf2f((uint16_t)(s.x >> 16));
+        v[2] += ws_bf2f((uint16_t)(s.y & 0xffffu));
+        v[3] += ws_bf2f((uint16_t)(s.y >> 16));
+      }
+      if (a.relu) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) v[r] = fmaxf(v[r], 0.f);
+      }
+      uint2 o;
+      o.x = (uint32_t)ws_f2bf(v[0]) | ((uint32_t)ws_f2bf(v[1]) << 16);
+      o.y = (uint32_t)ws_f2bf(v[2]) | ((uint32_t)ws_f2bf(v[3]) << 16);
+      *(uint2 *)(a.Y + p * a.Cout + cb) = o;
+    } else if (a.mode == 1) {
+      float4 *dst = (float4 *)(a.Yf + p * a.Cout + cb);
+      float4 d = *dst;
+      d.x += v[0];
+      d.y += v[1];
+      d.z += v[2];
+      d.w += v[3];
+      *dst = d;
+    } else if (cb == 0) {
+      a.Yf[p] = v[0];
+    }
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* weight gradient: part[chunk][co][tap*Cin+ci] = sum over the chunk's pixels */
+/* of G[p][co] * X[p + tap][ci]                                               */
+
+struct WgradArgs {
+  const uint16_t *X;  /* NHWC Cin */
+  const uint16_t *G;  /* NHWC Cout (masked output gradient) */
+  float *part;
+  int H, W, Cin, Cout;
+  long long nseg;     /* pixel segments of 64 */
+  int seg_per_chunk;
+};
+
+__global__ __launch_bounds__(256) void k_ws_wgrad(WgradArgs a) {
+  __shared__ __attribute__((aligned(16))) uint16_t sG[64 * 32];
+  __shared__ __attribute__((aligned(16))) uint16_t sX[9 * 64 * 16];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int col = lane & 15, grp = lane >> 4;
+  const int co0 = blockIdx.y * 32, ci0 = blockIdx.z * 16;
+  const int cb = wv & 1;
+  const int t0 = (wv >> 1) * 5;
+  const int t1 = t0 + 5 < 9 ? t0 + 5 : 9;
+  f32x4 acc[5];
+#pragma unroll
+  for (int t = 0; t < 5; ++t) acc[t] = (f32x4){0.f, 0.f, 0.f, 0.f};
+  const int HW = a.H * a.W;
+  const long long s0 = (long long)blockIdx.x * a.seg_per_chunk;
+  const long long s1 = s0 + a.seg_per_chunk < a.nseg ? s0 + a.seg_per_chunk : a.nseg;
+  for (long long s = s0; s < s1; ++s) {
+    const long long pbase = s * 64;
+    __syncthreads();
+    {
+      const int px = threadIdx.x >> 2, c8 = (threadIdx.x & 3) * 8;
+      u32x4 v = (u32x4){0u, 0u, 0u, 0u};
+      if (co0 + c8 < a.Cout) v = *(const u32x4 *)(a.G + (pbase + px) * a.Cout + co0 + c8);
+      *(u32x4 *)(sG + px * 32 + c8) = v;
+    }
+    for (int idx = threadIdx.x; idx < 9 * 128; idx += 256) {
+      const int tap = idx >> 7, r = idx & 127, px = r >> 1, c8 = (r & 1) * 8;
+      const long long p = pbase + px;
+      const int n = (int)(p / HW);
+      const int rem = (int)(p - (long long)n * HW);
+      const int h = rem / a.W, w = rem - h * a.W;
+      const int hh = h + tap / 3 - 1, ww = w + tap % 3 - 1;
+      u32x4 v = (u32x4){0u, 0u, 0u, 0u};
+      if (ci0 + c8 < a.Cin && hh >= 0 && hh < a.H && ww >= 0 && ww < a.W)
+        v = *(const u32x4 *)(a.X + ((long long)(n * a.H + hh) * a.W + ww) * a.Cin + ci0 + c8);
+      *(u32x4 *)(sX + (tap * 64 + px) * 16 + c8) = v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kc = 0; kc < 2; ++kc) {
+      u16x8 av;
+#pragma unroll
+      for (int q = 0; q < 8; ++q) av[q] = sG[(kc * 32 + grp * 8 + q) * 32 + cb * 16 + col];
+      const bf16x8 A = __builtin_bit_cast(bf16x8, av);
+#pragma unroll
+      for (int t = 0; t < 5; ++t) {
+        if (t0 + t < t1) {
+          u16x8 bv;
+#pragma unroll
+          for (int q = 0; q < 8; ++q)
+            bv[q] = sX[((t0 + t) * 64 + kc * 32 + grp * 8 + q) * 16 + col];
+          acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, __builtin_bit_cast(bf16x8, bv),
+                                                           acc[t], 0, 0, 0);
+        }
+      }
+    }
+  }
+  const long long K = 9ll * a.Cin;
+  float *out = a.part + (long long)blockIdx.x * a.Cout * K;
+  const int ci = ci0 + col;
+#pragma unroll
+  for (int t = 0; t < 5; ++t) {
+    if (t0 + t >= t1) continue;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int co = co0 + cb * 16 + grp * 4 + r;
+      if (co < a.Cout && ci < a.Cin) out[co * K + (t0 + t) * a.Cin + ci] = acc[t][r];
+    }
+  }
+}
+
+/* grads[i] = sum_c part[c][i], chunks in order */
+__global__ void k_ws_reduce(const float *__restrict__ part, float *__restrict__ out, long long n,
+                            int nchunk) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    float s = 0.f;
+    for (int c = 0; c < nchunk; ++c) s += part[(long long)c * n + i];
+    out[i] = s;
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* elementwise kernels (grid-stride, one element per thread)                  */
+
+__global__ void k_ws_fill(float *p, long long n, float v) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+/* P[n,h,w,c] = bf16(0.25 * sum of the 2x2 block of X) ; X is H x W, P H/2 x W/2 */
+__global__ void k_ws_pool(const uint16_t *X, uint16_t *P, int H, int W, int C, long long n_out) {
+  const int H2 = H >> 1, W2 = W >> 1;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n_out;
+       i += (long long)gridDim.x * blockDim.x) {
+    const int c = (int)(i % C);
+    const long long pix = i / C;
+    const int w2 = (int)(pix % W2);
+    const long long r = pix / W2;
+    const int h2 = (int)(r % H2);
+    const int n = (int)(r / H2);
+    const long long b = ((long long)(n * H + 2 * h2) * W + 2 * w2) * C + c;
+    const float s = (ws_bf2f(X[b]) + ws_bf2f(X[b + C])) +
+                    (ws_bf2f(X[b + (long long)W * C]) + ws_bf2f(X[b + (long long)W * C + C]));
+    P[i] = ws_f2bf(0.25f * s);
+  }
+}
+
+/* GX[n,h,w,c] += 0.25 * GP[n,h/2,w/2,c] */
+__global__ void k_ws_pool_bwd(const float *GP, float *GX, int H, int W, int C, long long n_in) {
+  const int H2 = H >> 1, W2 = W >> 1;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n_in;
+       i += (long long)gridDim.x * blockDim.x) {
+    const int c = (int)(i % C);
+    const long long pix = i / C;
+    const int w = (int)(pix % W);
+    const long long r = pix / W;
+    const int h = (int)(r % H);
+    const int n = (int)(r / H);
+    GX[i] += 0.25f * GP[((long long)(n * H2 + (h >> 1)) * W2 + (w >> 1)) * C + c];
+  }
+}
+
+/* U[n,h,w,c] = X[n,h/2,w/2,c] ; U is H x W */
+__global__ void k_ws_up(const uint16_t *X, uint16_t *U, int H, int W, int C, long long n_out) {
+  const int H2 = H >> 1, W2 = W >> 1;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n_out;
+       i += (long long)gridDim.x * blockDim.x) {
+    const int c = (int)(i % C);
+    const long long pix = i / C;
+    const int w = (int)(pix % W);
+    const long long r = pix / W;
+    const int h = (int)(r % H);
+    const int n = (int)(r / H);
+    U[i] = X[((long long)(n * H2 + (h >> 1)) * W2 + (w >> 1)) * C + c];
+  }
+}
+
+/* GX[n,h2,w2,c] += sum of GU over the 2x2 block ; GU is H x W */
+__global__ void k_ws_up_bwd(const float *GU, float *GX, int H, int W, int C, long long n_in) {
+  const int H2 = H >> 1, W2 = W >> 1;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n_in;
+       i += (long long)gridDim.x * blockDim.x) {
+    const int c = (int)(i % C);
+    const long long pix = i / C;
+    const int w2 = (int)(pix % W2);
+    const long long r = pix / W2;
+    const int h2 = (int)(r % H2);
+    const int n = (int)(r / H2);
+    const long long b = ((long long)(n * H + 2 * h2) * W + 2 * w2) * C + c;
+    GX[i] += (GU[b] + GU[b + C]) + (GU[b + (long long)W * C] + GU[b + (long long)W * C + C]);
+  }
+}
+
+/* Gm = bf16(G * [Y > 0]); skip (optional) += G * [Y > 0] */
+__global__ void k_ws_mask(const float *G, const uint16_t *Y, uint16_t *Gm, float *skip,
+                          long long n) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const float g = ws_bf2f(Y[i]) > 0.f ? G[i] : 0.f;
+    Gm[i] = ws_f2bf(g);
+    if (skip) skip[i] += g;
+  }
+}
+
+/* loss over interior pixels (h, w < S-1): sum (o - t)^2 per block -> part;
+ * Gout[p][0] = bf16(2 (o - t) * inv_count), other output channels 0 */
+__global__ __launch_bounds__(EW_BLOCK) void k_ws_loss(const float *o, const float *t,
+                                                      uint16_t *Gout, double *part, int S,
+                                                      long long npix, float inv_count) {
+  __shared__ double red[EW_BLOCK];
+  double acc = 0.0;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < npix;
+       i += (long long)gridDim.x * blockDim.x) {
+    const int pix = (int)(i % ((long long)S * S));
+    const int h = pix / S, w = pix - (pix / S) * S;
+    float d = 0.f;
+    if (h < S - 1 && w < S - 1) d = o[i] - t[i];
+    acc += (double)d * (double)d;
+    if (Gout) {
+      uint16_t *g = Gout + i * WS_OUT_CH;
+      g[0] = ws_f2bf(2.f * d * inv_count);
+      for (int c = 1; c < WS_OUT_CH; ++c) g[c] = 0;
+    }
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = EW_BLOCK / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+__global__ void k_ws_adam(float *p, const float *g, float *m, float *v, long long n, float lr,
+                          float b1, float b2, float eps, float c1, float c2) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.f - b1) * gi;
+    const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    p[i] -= lr * (mi * c1) / (sqrtf(vi * c2) + eps);
+  }
+}
+
+/* bf16 weights of one layer: Wt[co][k] (forward) and Wtt[ci][k'] (input
+ * gradient: taps mirrored, in/out channels swapped) */
+__global__ void k_ws_wprep(const float *p, uint16_t *Wt, uint16_t *Wtt, int Cin, int Cout,
+                           int Kpad, int KpadT) {
+  const long long n1 = (long long)Cout * Kpad, n2 = Wtt ? (long long)Cin * KpadT : 0;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n1 + n2;
+       i += (long long)gridDim.x * blockDim.x) {
+    if (i < n1) {
+      const int co = (int)(i / Kpad), k = (int)(i - (long long)co * Kpad);
+      float v = 0.f;
+      if (k < 9 * Cin) v = p[(long long)co * 9 * Cin + k];
+      Wt[i] = ws_f2bf(v);
+    } else {
+      const long long j = i - n1;
+      const int ci = (int)(j / KpadT), k = (int)(j - (long long)ci * KpadT);
+      float v = 0.f;
+      if (k < 9 * Cout) {
+        const int tp = k / Cout, co = k - tp * Cout;
+        v = p[(long long)co * 9 * Cin + (8 - tp) * Cin + ci];
+      }
+      Wtt[j] = ws_f2bf(v);
+    }
+  }
+}
+
+/* training batch: A0[b][pix][0..7] = (ds_in[idx[b]][pix], 0, ...);
+ * tgt[b][pix] = ds_out[idx[b]][pix] * inv_gain */
+__global__ void k_ws_batch(const uint16_t *ds_in, const float *ds_out, const long long *idx,
+                           uint16_t *A0, float *tgt, int S2, int nb, float inv_gain) {
+  const long long n = (long long)nb * S2;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const int b = (int)(i / S2);
+    const int pix = (int)(i - (long long)b * S2);
+    const long long s = idx[b];
+    u16x8 v = (u16x8){0, 0, 0, 0, 0, 0, 0, 0};
+    v[0] = ds_in[s * S2 + pix];
+    *(u16x8 *)(A0 + i * WS_IN_CH) = v;
+    if (tgt) tgt[i] = ds_out ? ds_out[s * S2 + pix] * inv_gain : 0.f;
+  }
+}
+
+/* host-supplied (already normalised) inputs: A0[i][0] = bf16(x[i]) */
+__global__ void k_ws_input(const float *x, uint16_t *A0, long long n) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    u16x8 v = (u16x8){0, 0, 0, 0, 0, 0, 0, 0};
+    v[0] = ws_f2bf(x[i]);
+    *(u16x8 *)(A0 + i * WS_IN_CH) = v;
+  }
+}
+
+/* right-hand side of the context's rho: sum of f^2 over interior nodes per
+ * block, f = rho * fscale */
+__global__ __launch_bounds__(EW_BLOCK) void k_ws_rhs_sumsq(const double *rho, int nx,
+                                                           double fscale, double *part) {
+  __shared__ double red[EW_BLOCK];
+  const int S = nx - 1;
+  double acc = 0.0;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+       i < (long long)(S - 1) * (S - 1); i += (long long)gridDim.x * blockDim.x) {
+    const int h = (int)(i / (S - 1)), w = (int)(i - (long long)h * (S - 1));
+    const double f = rho[(long long)(h + 1) * nx + (w + 1)] * fscale;
+    acc += f * f;
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = EW_BLOCK / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+/* scal[0] = s = sqrt(mean f^2) (1 if f == 0), scal[1] = 1/s */
+__global__ void k_ws_rhs_scale(const double *part, int nb, long long count, double *scal) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    double t = 0.0;
+    for (int b = 0; b < nb; ++b) t += part[b];
+    double s = sqrt(t / (double)count);
+    if (!(s > 0.0)) s = 1.0;
+    scal[0] = s;
+    scal[1] = 1.0 / s;
+  }
+}
+
+/* normalised input (and, for capture, the normalised target phi / s) */
+__global__ void k_ws_rhs_store(const double *rho, const double *phi, int nx, double fscale,
+                               const double *scal, uint16_t *in_plane, uint16_t *A0,
+                               float *out_plane) {
+  const int S = nx - 1;
+  const double inv = scal[1];
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < (long long)S * S;
+       i += (long long)gridDim.x * blockDim.x) {
+    const int h = (int)(i / S), w = (int)(i - (long long)h * S);
+    const bool in = h < S - 1 && w < S - 1;
+    const long long node = (long long)(h + 1) * nx + (w + 1);
+    const float x = in ? (float)(rho[node] * fscale * inv) : 0.f;
+    if (in_plane) in_plane[i] = ws_f2bf(x);
+    if (A0) {
+      u16x8 v = (u16x8){0, 0, 0, 0, 0, 0, 0, 0};
+      v[0] = ws_f2bf(x);
+      *(u16x8 *)(A0 + i * WS_IN_CH) = v;
+    }
+    if (out_plane) out_plane[i] = in ? (float)(phi[node] * inv) : 0.f;
+  }
+}
+
+/* phi (interior nodes) = o * gain * s */
+__global__ void k_ws_phi_out(const float *o, double *phi, int nx, double gain,
+                             const double *scal) {
+  const int S = nx - 1;
+  const double sc = gain * scal[0];
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+       i < (long long)(S - 1) * (S - 1); i += (long long)gridDim.x * blockDim.x) {
+    const int h = (int)(i / (S - 1)), w = (int)(i - (long long)h * (S - 1));
+    phi[(long long)(h + 1) * nx + (w + 1)] = (double)o[(long long)h * S + w] * sc;
+  }
+}
+
+/* sum of squares of a float range, per block */
+__global__ __launch_bounds__(EW_BLOCK) void k_ws_sumsq(const float *x, long long n,
+                                                       double *part) {
+  __shared__ double red[EW_BLOCK];
+  double acc = 0.0;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    acc += (double)x[i] * (double)x[i];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = EW_BLOCK / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+/* ------------------------------------------------------------------------ */
+/* host side                                                                  */
+
+struct WsConv {
+  int cin, cout, kpad, kpadt, level, relu;
+  long long woff;  /* offset in the parameter vector ([co][tap][ci]) */
+  uint16_t *wt = nullptr, *wtt = nullptr;
+};
+
+struct WsTensor {
+  int level = 0, C = 0;
+  uint16_t *a = nullptr; /* bf16 activations */
+  float *g = nullptr;    /* fp32 gradient */
+};
+
+struct iskra_ws {
+  iskra_ctx *ctx = nullptr;
+  int L = 0, base = 0, S = 0, maxb = 0;
+  std::vector<WsConv> conv;
+  long long nparam = 0;
+  float *params = nullptr, *grads = nullptr, *m = nullptr, *v = nullptr;
+  long long adam_t = 0;
+  double gain = 1.0;
+  WsTensor A0, Ea[WS_MAX_LEVELS], E[WS_MAX_LEVELS], P[WS_MAX_LEVELS], U[WS_MAX_LEVELS],
+      Da[WS_MAX_LEVELS], D[WS_MAX_LEVELS];
+  float *out = nullptr, *tgt = nullptr;
+  uint16_t *gout = nullptr, *gm = nullptr;
+  long long gm_elems = 0;
+  float *part = nullptr;
+  double *dpart = nullptr, *scal = nullptr;
+  int dpart_n = 0;
+  long long *idx = nullptr;
+  float *xin = nullptr;  /* host-input staging */
+  /* dataset */
+  uint16_t *ds_in = nullptr;
+  float *ds_out = nullptr;
+  long long ds_cap = 0, ds_n = 0;
+};
+
+static int ws_side(const iskra_ws *w, int l) { return w->S >> l; }
+static long long ws_npix(const iskra_ws *w, int l, int b) {
+  return (long long)b * ws_side(w, l) * ws_side(w, l);
+}
+static unsigned ws_grid(long long n) {
+  long long b = (n + EW_BLOCK - 1) / EW_BLOCK;
+  return (unsigned)std::max(1ll, std::min(b, 8192ll));
+}
+
+static int ws_tensor(iskra_ws *w, WsTensor *t, int level, int C, bool grad) {
+  t->level = level;
+  t->C = C;
+  const long long n = ws_npix(w, level, w->maxb) * C;
+  WRC(isk_alloc((void **)&t->a, n * 2));
+  if (grad) WRC(isk_alloc((void **)&t->g, n * 4));
+  w->gm_elems = std::max(w->gm_elems, n);
+  return ISKRA_OK;
+}
+
+static void ws_free_tensor(WsTensor *t) {
+  isk_free(t->a);
+  isk_free(t->g);
+  t->a = nullptr;
+  t->g = nullptr;
+}
+
+static int ws_chan(const iskra_ws *w, int l) { return w->base << l; }
+
+/* layer order: enc0A, enc0B, ..., enc(L-1)B, dec(L-2)A, dec(L-2)B, ..., dec0B, out */
+static int ws_enc(int l, int b) { return 2 * l + b; }
+static int ws_dec(const iskra_ws *w, int l, int b) { return 2 * w->L + 2 * (w->L - 2 - l) + b; }
+static int ws_out(const iskra_ws *w) { return 2 * w->L + 2 * (w->L - 1); }
+
+static void ws_add_conv(iskra_ws *w, int cin, int cout, int level, int relu) {
+  WsConv c;
+  c.cin = cin;
+  c.cout = cout;
+  c.kpad = (9 * cin + 31) / 32 * 32;
+  c.kpadt = (9 * cout + 31) / 32 * 32;
+  c.level = level;
+  c.relu = relu;
+  c.woff = w->nparam;
+  w->nparam += (long long)cout * 9 * cin;
+  w->conv.push_back(c);
+}
+
+static int ws_launch_conv(iskra_ws *w, const WsConv &c, int b, const uint16_t *X,
+                          const uint16_t *Wt, int cin, int cout, int kpad, const uint16_t *skip,
+                          uint16_t *Y, float *Yf, int relu, int mode) {
+  ConvArgs a;
+  a.X = X;
+  a.Wt = Wt;
+  a.skip = skip;
+  a.Y = Y;
+  a.Yf = Yf;
+  a.H = a.W = ws_side(w, c.level);
+  a.Cin = cin;
+  a.Cout = cout;
+  a.Kpad = kpad;
+  a.relu = relu;
+  a.mode = mode;
+  const long long npix = ws_npix(w, c.level, b);
+  if (npix % 64 || cin % 8 || cout % 16 || kpad % 32) {
+    isk_set_error("warm start: conv shape %d->%d on %lld pixels not supported", cin, cout, npix);
+    return ISKRA_E_ARG;
+  }
+  const int tm = cout % 64 == 0 ? 64 : (cout % 32 == 0 ? 32 : 16);
+  dim3 grid((unsigned)(npix / 64), (unsigned)(cout / tm));
+  hipStream_t st = w->ctx->stream;
+  if (tm == 64) hipLaunchKernelGGL(k_ws_conv<64>, grid, dim3(256), 0, st, a);
+  else if (tm == 32) hipLaunchKernelGGL(k_ws_conv<32>, grid, dim3(256), 0, st, a);
+  else hipLaunchKernelGGL(k_ws_conv<16>, grid, dim3(256), 0, st, a);
+  ISK_CHECK_LAUNCH();
+  return ISKRA_OK;
+}
+
+static int ws_fwd_conv(iskra_ws *w, int layer, int b, const WsTensor &in, const uint16_t *skip,
+                       uint16_t *Y, float *Yf, int mode) {
+  const WsConv &c = w->conv[layer];
+  return ws_launch_conv(w, c, b, in.a, c.wt, c.cin, c.cout, c.kpad, skip, Y, Yf, c.relu, mode);
+}
+
+/* input gradient: GX += conv(Gm, W mirrored) */
+static int ws_dgrad(iskra_ws *w, int layer, int b, float *GX) {
+  const WsConv &c = w->conv[layer];
+  return ws_launch_conv(w, c, b, w->gm, c.wtt, c.cout, c.cin, c.kpadt, nullptr, nullptr, GX, 0,
+                        1);
+}
+
+static int ws_wgrad(iskra_ws *w, int layer, int b, const uint16_t *X) {
+  const WsConv &c = w->conv[layer];
+  const long long npix = ws_npix(w, c.level, b);
+  const long long nseg = npix / 64;
+  const long long K = 9ll * c.cin;
+  const long long per_chunk = (long long)c.cout * K;
+  const int gy = (c.cout + 31) / 32, gz = (c.cin + 15) / 16;
+  long long max_chunks = std::max(1ll, (WS_PART_BYTES / 4) / per_chunk);
+  long long want = std::max(1ll, (long long)(1024 / (gy * gz)));
+  long long nchunk = std::min(std::min(max_chunks, want), nseg);
+  const int spc = (int)((nseg + nchunk - 1) / nchunk);
+  nchunk = (nseg + spc - 1) / spc;
+  WgradArgs a;
+  a.X = X;
+  a.G = w->gm;
+  a.part = w->part;
+  a.H = a.W = ws_side(w, c.level);
+  a.Cin = c.cin;
+  a.Cout = c.cout;
+  a.nseg = nseg;
+  a.seg_per_chunk = spc;
+  hipLaunchKernelGGL(k_ws_wgrad, dim3((unsigned)nchunk, gy, gz), dim3(256), 0, w->ctx->stream,
+                     a);
+  ISK_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_ws_reduce, dim3(ws_grid(per_chunk)), dim3(EW_BLOCK), 0, w->ctx->stream,
+                     (const float *)w->part, w->grads + c.woff, per_chunk, (int)nchunk);
+  ISK_CHECK_LAUNCH();
+  return ISKRA_OK;
+}
+
+static int ws_mask(iskra_ws *w, const WsTensor &t, int b, float *skip) {
+  const long long n = ws_npix(w, t.level, b) * t.C;
+  hipLaunchKernelGGL(k_ws_mask, dim3(ws_grid(n)), dim3(EW_BLOCK), 0, w->ctx->stream,
+                     (const float *)t.g, (const uint16_t *)t.a, w->gm, skip, n);
+  ISK_CHECK_LAUNCH();
+  return ISKRA_OK;
+}
+
+static int ws_zero(iskra_ws *w, float *p, long long n) {
+  hipLaunchKernelGGL(k_ws_fill, dim3(ws_grid(n)), dim3(EW_BLOCK), 0, w->ctx->stream, p, n, 0.f);
+  ISK_CHECK_LAUNCH();
+  return ISKRA_OK;
+}
+
+static const WsTensor &ws_dec_out(const iskra_ws *w, int l) {
+  return l == w->L - 1 ? w->E[l] : w->D[l];
+}
+
+/* forward of a batch already in A0; output plane in w->out */
+static int ws_forward(iskra_ws *w, int b) {
+  hipStream_t st = w->ctx->stream;
+  const int L = w->L;
+  for (int l = 0; l < L; ++l) {
+    if (l > 0) {
+      const WsTensor &src = w->E[l - 1];
+      const long long n = ws_npix(w, l, b) * src.C;
+      const int Hs = ws_side(w, l - 1);
+      hipLaunchKernelGGL(k_ws_pool, dim3(ws_grid(n)), dim3(EW_BLOCK), 0, st,
+                         (const uint16_t *)src.a, w->P[l].a, Hs, Hs, src.C, n);
+      ISK_CHECK_LAUNCH();
+    }
+    WRC(ws_fwd_conv(w, ws_enc(l, 0), b, l == 0 ? w->A0 : w->P[l], nullptr, w->Ea[l].a, nullptr,
+                    0));
+    WRC(ws_fwd_conv(w, ws_enc(l, 1), b, w->Ea[l], nullptr, w->E[l].a, nullptr, 0));
+  }
+  for (int l = L - 2; l >= 0; --l) {
+    const WsTensor &src = ws_dec_out(w, l + 1);
+    const long long n = ws_npix(w, l, b) * src.C;
+    const int H = ws_side(w, l);
+    hipLaunchKernelGGL(k_ws_up, dim3(ws_grid(n)), dim3(EW_BLOCK), 0, st, (const uint16_t *)src.a,
+                       w->U[l].a, H, H, src.C, n);
+    ISK_CHECK_LAUNCH();
+    WRC(ws_fwd_conv(w, ws_dec(w, l, 0), b, w->U[l], w->E[l].a, w->Da[l].a, nullptr, 0));
+    WRC(ws_fwd_conv(w, ws_dec(w, l, 1), b, w->Da[l], nullptr, w->D[l].a, nullptr, 0));
+  }
+  const WsTensor &last = L > 1 ? w->D[0] : w->E[0];
+  return ws_fwd_conv(w, ws_out(w), b, last, nullptr, nullptr, w->out, 2);
+}
+
+/* loss of w->out against w->tgt; with_grad: also Gout; returns the mean */
+static int ws_loss(iskra_ws *w, int b, bool with_grad, double *loss) {
+  const long long npix = ws_npix(w, 0, b);
+  const long long count = (long long)b * (w->S - 1) * (w->S - 1);
+  const unsigned nb = std::min(ws_grid(npix), (unsigned)w->dpart_n);
+  hipLaunchKernelGGL(k_ws_loss, dim3(nb), dim3(EW_BLOCK), 0, w->ctx->stream,
+                     (const float *)w->out, (const float *)w->tgt, with_grad ? w->gout : nullptr,
+                     w->dpart, w->S, npix, (float)(1.0 / (double)count));
+  ISK_CHECK_LAUNCH();
+  std::vector<double> h(nb);
+  ISK_HIP(hipMemcpyAsync(h.data(), w->dpart, nb * 8, hipMemcpyDeviceToHost, w->ctx->stream));
+  ISK_HIP(hipStreamSynchronize(w->ctx->stream));
+  double s = 0.0;
+  for (unsigned i = 0; i < nb; ++i) s += h[i];
+  if (loss) *loss = s / (double)count;
+  return ISKRA_OK;
+}
+
+/* backward after ws_forward + ws_loss(with_grad): parameter gradients in w->grads */
+static int ws_backward(iskra_ws *w, int b) {
+  hipStream_t st = w->ctx->stream;
+  const int L = w->L;
+  WsTensor *all[] = {w->Ea, w->E, w->P, w->U, w->Da, w->D};
+  for (WsTensor *arr : all)
+    for (int l = 0; l < L; ++l)
+      if (arr[l].g) WRC(ws_zero(w, arr[l].g, ws_npix(w, arr[l].level, b) * arr[l].C));
+  /* output layer (linear): its masked gradient is Gout itself */
+  const WsTensor &last = L > 1 ? w->D[0] : w->E[0];
+  {
+    const WsConv &c = w->conv[ws_out(w)];
+    const long long npix = ws_npix(w, 0, b);
+    /* Gm <- Gout (16 channels) */
+    ISK_HIP(hipMemcpyAsync(w->gm, w->gout, npix * WS_OUT_CH * 2, hipMemcpyDeviceToDevice, st));
+    WRC(ws_wgrad(w, ws_out(w), b, last.a));
+    WRC(ws_launch_conv(w, c, b, w->gm, c.wtt, c.cout, c.cin, c.kpadt, nullptr, nullptr, last.g,
+                       0, 1));
+  }
+  for (int l = 0; l <= L - 2; ++l) {
+    WRC(ws_mask(w, w->D[l], b, nullptr));
+    WRC(ws_wgrad(w, ws_dec(w, l, 1), b, w->Da[l].a));
+    WRC(ws_dgrad(w, ws_dec(w, l, 1), b, w->Da[l].g));
+    WRC(ws_mask(w, w->Da[l], b, w->E[l].g));
+    WRC(ws_wgrad(w, ws_dec(w, l, 0), b, w->U[l].a));
+    WRC(ws_dgrad(w, ws_dec(w, l, 0), b, w->U[l].g));
+    const WsTensor &dst = ws_dec_out(w, l + 1);
+    const long long n = ws_npix(w, l + 1, b) * dst.C;
+    const int H = ws_side(w, l);
+    hipLaunchKernelGGL(k_ws_up_bwd, dim3(ws_grid(n)), dim3(EW_BLOCK), 0, st,
+                       (const float *)w->U[l].g, dst.g, H, H, dst.C, n);
+    ISK_CHECK_LAUNCH();
+  }
+  for (int l = L - 1; l >= 0; --l) {
+    WRC(ws_mask(w, w->E[l], b, nullptr));
+    WRC(ws_wgrad(w, ws_enc(l, 1), b, w->Ea[l].a));
+    WRC(ws_dgrad(w, ws_enc(l, 1), b, w->Ea[l].g));
+    WRC(ws_mask(w, w->Ea[l], b, nullptr));
+    WRC(ws_wgrad(w, ws_enc(l, 0), b, l == 0 ? w->A0.a : w->P[l].a));
+    if (l > 0) {
+      WRC(ws_dgrad(w, ws_enc(l, 0), b, w->P[l].g));
+      const WsTensor &dst = w->E[l - 1];
+      const long long n = ws_npix(w, l - 1, b) * dst.C;
+      const int H = ws_side(w, l - 1);
+      hipLaunchKernelGGL(k_ws_pool_bwd, dim3(ws_grid(n)), dim3(EW_BLOCK), 0, st,
+                         (const float *)w->P[l].g, dst.g, H, H, dst.C, n);
+      ISK_CHECK_LAUNCH();
+    }
+  }
+  return ISKRA_OK;
+}
+
+static int ws_prep_weights(iskra_ws *w) {
+  for (size_t i = 0; i < w->conv.size(); ++i) {
+    WsConv &c = w->conv[i];
+    const long long n = (long long)c.cout * c.kpad + (c.wtt ? (long long)c.cin * c.kpadt : 0);
+    hipLaunchKernelGGL(k_ws_wprep, dim3(ws_grid(n)), dim3(EW_BLOCK), 0, w->ctx->stream,
+                       (const float *)(w->params + c.woff), c.wt, c.wtt, c.cin, c.cout, c.kpad,
+                       c.kpadt);
+    ISK_CHECK_LAUNCH();
+  }
+  return ISKRA_OK;
+}
+
+static void ws_release(iskra_ws *w) {
+  for (WsConv &c : w->conv) {
+    isk_free(c.wt);
+    isk_free(c.wtt);
+  }
+  ws_free_tensor(&w->A0);
+  for (int l = 0; l < WS_MAX_LEVELS; ++l) {
+    ws_free_tensor(&w->Ea[l]);
+    ws_free_tensor(&w->E[l]);
+    ws_free_tensor(&w->P[l]);
+    ws_free_tensor(&w->U[l]);
+    ws_free_tensor(&w->Da[l]);
+    ws_free_tensor(&w->D[l]);
+  }
+  void *ptrs[] = {w->params, w->grads, w->m, w->v, w->out, w->tgt, w->gout, w->gm, w->part,
+                  w->dpart, w->scal, w->idx, w->xin, w->ds_in, w->ds_out};
+  for (void *p : ptrs) isk_free(p);
+}
+
+static int ws_check(iskra_ws *w) {
+  if (!w || !w->ctx) {
+    isk_set_error("warm start: null network");
+    return ISKRA_E_ARG;
+  }
+  isk_clear_error();
+  ISK_HIP(hipSetDevice(w->ctx->device));
+  return ISKRA_OK;
+}
+
+/* the context's grid must be the network's: square, 2-D, (S+1)^2 nodes */
+static int ws_check_grid(iskra_ws *w) {
+  const iskra_ctx *c = w->ctx;
+  if (!c->have_grid || c->g.dim != 2 || c->g.nx != w->S + 1 || c->g.ny != w->S + 1) {
+    isk_set_error("warm start: the context grid must be 2-D with %d x %d nodes", w->S + 1,
+                  w->S + 1);
+    return ISKRA_E_STATE;
+  }
+  return ISKRA_OK;
+}
+
+/* f = rho * dx^2 / eps0 of the context -> scale in w->scal */
+static int ws_rhs_scale(iskra_ws *w, double *fscale) {
+  iskra_ctx *c = w->ctx;
+  *fscale = c->g.dx * c->g.dx / c->ps.eps0;
+  const long long cnt = (long long)(w->S - 1) * (w->S - 1);
+  const unsigned nb = std::min(ws_grid(cnt), (unsigned)w->dpart_n);
+  hipLaunchKernelGGL(k_ws_rhs_sumsq, dim3(nb), dim3(EW_BLOCK), 0, c->stream,
+                     (const double *)c->rho, c->g.nx, *fscale, w->dpart);
+  ISK_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_ws_rhs_scale, dim3(1), dim3(64), 0, c->stream, (const double *)w->dpart,
+                     (int)nb, cnt, w->scal);
+  ISK_CHECK_LAUNCH();
+  return ISKRA_OK;
+}
+
+extern "C" {
+
+int iskra_ws_create(iskra_ctx *ctx, int levels, int base, int side, int max_batch,
+                    iskra_ws **out) {
+  if (!ctx || !out) {
+    isk_set_error("ws_create: null argument");
+    return ISKRA_E_ARG;
+  }
+  ISK_HIP(hipSetDevice(ctx->device));
+  isk_clear_error();
+  *out = nullptr;
+  if (levels < 1 || levels > WS_MAX_LEVELS || base < 16 || base % 16 || side < 16 ||
+      (side & (side - 1)) || (side >> (levels - 1)) < 8 || max_batch < 1 || max_batch > 1024) {
+    isk_set_error("ws_create: levels in [1,%d], base a multiple of 16, side a power of two with "
+                  "side/2^(levels-1) >= 8, max_batch in [1,1024]", WS_MAX_LEVELS);
+    return ISKRA_E_ARG;
+  }
+  iskra_ws *w = new iskra_ws();
+  w->ctx = ctx;
+  w->L = levels;
+  w->base = base;
+  w->S = side;
+  w->maxb = max_batch;
+  for (int l = 0; l < levels; ++l) {
+    ws_add_conv(w, l == 0 ? WS_IN_CH : ws_chan(w, l - 1), ws_chan(w, l), l, 1);
+    ws_add_conv(w, ws_chan(w, l), ws_chan(w, l), l, 1);
+  }
+  for (int l = levels - 2; l >= 0; --l) {
+    ws_add_conv(w, ws_chan(w, l + 1), ws_chan(w, l), l, 1);
+    ws_add_conv(w, ws_chan(w, l), ws_chan(w, l), l, 1);
+  }
+  ws_add_conv(w, ws_chan(w, 0), WS_OUT_CH, 0, 0);
+  int rc = ISKRA_OK;
+#define WTRY(x)                  \
+  do {                           \
+    rc = (x);                    \
+    if (rc) goto fail;           \
+  } while (0)
+  for (size_t i = 0; i < w->conv.size(); ++i) {
+    WsConv &c = w->conv[i];
+    WTRY(isk_alloc((void **)&c.wt, (size_t)c.cout * c.kpad * 2));
+    if (i != 0) WTRY(isk_alloc((void **)&c.wtt, (size_t)c.cin * c.kpadt * 2));
+  }
+  WTRY(isk_alloc((void **)&w->params, w->nparam * 4));
+  WTRY(isk_alloc((void **)&w->grads, w->nparam * 4));
+  WTRY(isk_alloc((void **)&w->m, w->nparam * 4));
+  WTRY(isk_alloc((void **)&w->v, w->nparam * 4));
+  WTRY(ws_tensor(w, &w->A0, 0, WS_IN_CH, false));
+  for (int l = 0; l < levels; ++l) {
+    WTRY(ws_tensor(w, &w->Ea[l], l, ws_chan(w, l), true));
+    WTRY(ws_tensor(w, &w->E[l], l, ws_chan(w, l), true));
+    if (l > 0) WTRY(ws_tensor(w, &w->P[l], l, ws_chan(w, l - 1), true));
+    if (l <= levels - 2) {
+      WTRY(ws_tensor(w, &w->U[l], l, ws_chan(w, l + 1), true));
+      WTRY(ws_tensor(w, &w->Da[l], l, ws_chan(w, l), true));
+      WTRY(ws_tensor(w, &w->D[l], l, ws_chan(w, l), true));
+    }
+  }
+  {
+    const long long npix = ws_npix(w, 0, max_batch);
+    w->gm_elems = std::max(w->gm_elems, npix * WS_OUT_CH);
+    WTRY(isk_alloc((void **)&w->out, npix * 4));
+    WTRY(isk_alloc((void **)&w->tgt, npix * 4));
+    WTRY(isk_alloc((void **)&w->xin, npix * 4));
+    WTRY(isk_alloc((void **)&w->gout, npix * WS_OUT_CH * 2));
+    WTRY(isk_alloc((void **)&w->gm, w->gm_elems * 2));
+    WTRY(isk_alloc((void **)&w->part, WS_PART_BYTES));
+    w->dpart_n = 8192;
+    WTRY(isk_alloc((void **)&w->dpart, w->dpart_n * 8));
+    WTRY(isk_alloc((void **)&w->scal, 64));
+    WTRY(isk_alloc((void **)&w->idx, (size_t)max_batch * 8));
+  }
+  WTRY(ws_zero(w, w->params, w->nparam));
+  WTRY(ws_zero(w, w->m, w->nparam));
+  WTRY(ws_zero(w, w->v, w->nparam));
+  WTRY(ws_zero(w, w->grads, w->nparam));
+  WTRY(ws_prep_weights(w));
+  {
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) {
+      rc = isk_hip_fail(e, "hipStreamSynchronize", __FILE__, __LINE__);
+      goto fail;
+    }
+  }
+#undef WTRY
+  *out = w;
+  return ISKRA_OK;
+fail:
+  ws_release(w);
+  delete w;
+  return rc;
+}
+
+int iskra_ws_destroy(iskra_ws *w) {
+  if (!w) return ISKRA_OK;
+  if (w->ctx) (void)hipSetDevice(w->ctx->device);
+  if (w->ctx) (void)hipStreamSynchronize(w->ctx->stream);
+  ws_release(w);
+  delete w;
+  return ISKRA_OK;
+}
+
+int iskra_ws_param_count(iskra_ws *w, int64_t *n) {
+  WRC(ws_check(w));
+  if (!n) return ISKRA_E_ARG;
+  *n = w->nparam;
+  return ISKRA_OK;
+}
+
+int iskra_ws_set_params(iskra_ws *w, const float *p) {
+  WRC(ws_check(w));
+  if (!p) return ISKRA_E_ARG;
+  ISK_HIP(hipMemcpyAsync(w->params, p, w->nparam * 4, hipMemcpyHostToDevice, w->ctx->stream));
+  WRC(ws_zero(w, w->m, w->nparam));
+  WRC(ws_zero(w, w->v, w->nparam));
+  w->adam_t = 0;
+  WRC(ws_prep_weights(w));
+  ISK_HIP(hipStreamSynchronize(w->ctx->stream));
+  return ISKRA_OK;
+}
+
+int iskra_ws_get_params(iskra_ws *w, float *p) {
+  WRC(ws_check(w));
+  if (!p) return ISKRA_E_ARG;
+  ISK_HIP(hipMemcpyAsync(p, w->params, w->nparam * 4, hipMemcpyDeviceToHost, w->ctx->stream));
+  ISK_HIP(hipStreamSynchronize(w->ctx->stream));
+  return ISKRA_OK;
+}
+
+int iskra_ws_set_gain(iskra_ws *w, double gain) {
+  WRC(ws_check(w));
+  if (!(gain > 0.0)) {
+    isk_set_error("ws_set_gain: gain must be positive");
+    return ISKRA_E_ARG;
+  }
+  w->gain = gain;
+  return ISKRA_OK;
+}
+
+static int ws_host_input(iskra_ws *w, int batch, const float *in) {
+  const long long npix = ws_npix(w, 0, batch);
+  ISK_HIP(hipMemcpyAsync(w->xin, in, npix * 4, hipMemcpyHostToDevice, w->ctx->stream));
+  hipLaunchKernelGGL(k_ws_input, dim3(ws_grid(npix)), dim3(EW_BLOCK), 0, w->ctx->stream,
+                     (const float *)w->xin, w->A0.a, npix);
+  ISK_CHECK_LAUNCH();
+  return ISKRA_OK;
+}
+
+int iskra_ws_forward(iskra_ws *w, int batch, const float *in, float *out) {
+  WRC(ws_check(w));
+  if (batch < 1 || batch > w->maxb || !in || !out) return ISKRA_E_ARG;
+  WRC(ws_host_input(w, batch, in));
+  WRC(ws_forward(w, batch));
+  ISK_HIP(hipMemcpyAsync(out, w->out, ws_npix(w, 0, batch) * 4, hipMemcpyDeviceToHost,
+                         w->ctx->stream));
+  ISK_HIP(hipStreamSynchronize(w->ctx->stream));
+  return ISKRA_OK;
+}
+
+int iskra_ws_backward(iskra_ws *w, int batch, const float *in, const float *target,
+                      float *grads, double *loss) {
+  WRC(ws_check(w));
+  if (batch < 1 || batch > w->maxb || !in || !target) return ISKRA_E_ARG;
+  WRC(ws_host_input(w, batch, in));
+  ISK_HIP(hipMemcpyAsync(w->tgt, target, ws_npix(w, 0, batch) * 4, hipMemcpyHostToDevice,
+                         w->ctx->stream));
+  WRC(ws_forward(w, batch));
+  WRC(ws_loss(w, batch, true, loss));
+  WRC(ws_backward(w, batch));
+  if (grads)
+    ISK_HIP(hipMemcpyAsync(grads, w->grads, w->nparam * 4, hipMemcpyDeviceToHost,
+                           w->ctx->stream));
+  ISK_HIP(hipStreamSynchronize(w->ctx->stream));
+  return ISKRA_OK;
+}
+
+int iskra_ws_dataset_reserve(iskra_ws *w, int64_t n) {
+  WRC(ws_check(w));
+  if (n < 1) return ISKRA_E_ARG;
+  ISK_HIP(hipStreamSynchronize(w->ctx->stream));
+  isk_free(w->ds_in);
+  isk_free(w->ds_out);
+  w->ds_in = nullptr;
+  w->ds_out = nullptr;
+  w->ds_cap = w->ds_n = 0;
+  const long long S2 = (long long)w->S * w->S;
+  WRC(isk_alloc((void **)&w->ds_in, (size_t)n * S2 * 2));
+  WRC(isk_alloc((void **)&w->ds_out, (size_t)n * S2 * 4));
+  w->ds_cap = n;
+  return ISKRA_OK;
+}
+
+int iskra_ws_capture(iskra_ws *w, int64_t *index) {
+  WRC(ws_check(w));
+  WRC(ws_check_grid(w));
+  if (w->ds_n >= w->ds_cap) {
+    isk_set_error("ws_capture: dataset full (%lld samples)", (long long)w->ds_cap);
+    return ISKRA_E_CAPACITY;
+  }
+  double fscale;
+  WRC(ws_rhs_scale(w, &fscale));
+  const long long S2 = (long long)w->S * w->S;
+  iskra_ctx *c = w->ctx;
+  hipLaunchKernelGGL(k_ws_rhs_store, dim3(ws_grid(S2)), dim3(EW_BLOCK), 0, c->stream,
+                     (const double *)c->rho, (const double *)c->phi, c->g.nx, fscale,
+                     (const double *)w->scal, w->ds_in + w->ds_n * S2, (uint16_t *)nullptr,
+                     w->ds_out + w->ds_n * S2);
+  ISK_CHECK_LAUNCH();
+  if (index) *index = w->ds_n;
+  ++w->ds_n;
+  return ISKRA_OK;
+}
+
+int iskra_ws_dataset_size(iskra_ws *w, int64_t *n) {
+  WRC(ws_check(w));
+  if (!n) return ISKRA_E_ARG;
+  *n = w->ds_n;
+  return ISKRA_OK;
+}
+
+/* RMS of the stored (phi / s) targets of samples [first, first+count) */
+int iskra_ws_dataset_rms(iskra_ws *w, int64_t first, int64_t count, double *rms) {
+  WRC(ws_check(w));
+  if (first < 0 || count < 1 || first + count > w->ds_n || !rms) return ISKRA_E_ARG;
+  const long long S2 = (long long)w->S * w->S;
+  const long long n = count * S2;
+  const unsigned nb = std::min(ws_grid(n), (unsigned)w->dpart_n);
+  hipLaunchKernelGGL(k_ws_sumsq, dim3(nb), dim3(EW_BLOCK), 0, w->ctx->stream,
+                     (const float *)(w->ds_out + first * S2), n, w->dpart);
+  ISK_CHECK_LAUNCH();
+  std::vector<double> h(nb);
+  ISK_HIP(hipMemcpyAsync(h.data(), w->dpart, nb * 8, hipMemcpyDeviceToHost, w->ctx->stream));
+  ISK_HIP(hipStreamSynchronize(w->ctx->stream));
+  double s = 0.0;
+  for (unsigned i = 0; i < nb; ++i) s += h[i];
+  *rms = sqrt(s / (double)(count * (long long)(w->S - 1) * (w->S - 1)));
+  return ISKRA_OK;
+}
+
+static int ws_load_batch(iskra_ws *w, const long long *idx, int b) {
+  ISK_HIP(hipMemcpyAsync(w->idx, idx, (size_t)b * 8, hipMemcpyHostToDevice, w->ctx->stream));
+  const long long S2 = (long long)w->S * w->S;
+  hipLaunchKernelGGL(k_ws_batch, dim3(ws_grid(b * S2)), dim3(EW_BLOCK), 0, w->ctx->stream,
+                     (const uint16_t *)w->ds_in, (const float *)w->ds_out,
+                     (const long long *)w->idx, w->A0.a, w->tgt, (int)S2, b,
+                     (float)(1.0 / w->gain));
+  ISK_CHECK_LAUNCH();
+  /* the index upload reads host memory of this call: finish before returning */
+  ISK_HIP(hipStreamSynchronize(w->ctx->stream));
+  return ISKRA_OK;
+}
+
+/* steps of Adam on random batches of samples [first, first+count), drawn with
+ * Philox4x32-10 keyed by seed; losses[s] (nullable) = batch loss of step s */
+int iskra_ws_train(iskra_ws *w, int64_t first, int64_t count, int batch, int steps, float lr,
+                   uint64_t seed, double *losses) {
+  WRC(ws_check(w));
+  if (first < 0 || count < 1 || first + count > w->ds_n || batch < 1 || batch > w->maxb ||
+      steps < 0 || !(lr > 0.f)) {
+    isk_set_error("ws_train: bad range, batch or learning rate");
+    return ISKRA_E_ARG;
+  }
+  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+  std::vector<long long> idx(batch);
+  for (int s = 0; s < steps; ++s) {
+    for (int k = 0; k < batch; ++k) {
+      IskU4 r = isk_philox((IskU4){(uint32_t)(w->adam_t + s), (uint32_t)k, 0x57534200u, 0u},
+                           (uint32_t)seed, (uint32_t)(seed >> 32));
+      idx[k] = first + (long long)(((uint64_t)r.x << 32 | r.y) % (uint64_t)count);
+    }
+    WRC(ws_load_batch(w, idx.data(), batch));
+    WRC(ws_forward(w, batch));
+    double loss = 0.0;
+    WRC(ws_loss(w, batch, true, &loss));
+    if (losses) losses[s] = loss;
+    WRC(ws_backward(w, batch));
+    const long long t = ++w->adam_t;
+    const float c1 = (float)(1.0 / (1.0 - pow(b1, (double)t)));
+    const float c2 = (float)(1.0 / (1.0 - pow(b2, (double)t)));
+    hipLaunchKernelGGL(k_ws_adam, dim3(ws_grid(w->nparam)), dim3(EW_BLOCK), 0, w->ctx->stream,
+                       w->params, (const float *)w->grads, w->m, w->v, w->nparam, lr, b1, b2, eps,
+                       c1, c2);
+    ISK_CHECK_LAUNCH();
+    WRC(ws_prep_weights(w));
+  }
+  ISK_HIP(hipStreamSynchronize(w->ctx->stream));
+  return ISKRA_OK;
+}
+
+/* mean loss over samples [first, first+count), batches of the maximum size */
+int iskra_ws_eval(iskra_ws *w, int64_t first, int64_t count, double *mse) {
+  WRC(ws_check(w));
+  if (first < 0 || count < 1 || first + count > w->ds_n || !mse) return ISKRA_E_ARG;
+  double tot = 0.0;
+  std::vector<long long> idx(w->maxb);
+  for (long long s = 0; s < count; s += w->maxb) {
+    const int b = (int)std::min((long long)w->maxb, count - s);
+    for (int k = 0; k < b; ++k) idx[k] = first + s + k;
+    WRC(ws_load_batch(w, idx.data(), b));
+    WRC(ws_forward(w, b));
+    double l = 0.0;
+    WRC(ws_loss(w, b, false, &l));
+    tot += l * b;
+  }
+  *mse = tot / (double)count;
+  return ISKRA_OK;
+}
+
+/* phi at the context's interior nodes <- network prediction from its rho */
+int iskra_ws_predict(iskra_ws *w) {
+  WRC(ws_check(w));
+  WRC(ws_check_grid(w));
+  iskra_ctx *c = w->ctx;
+  hipEvent_t ev = nullptr;
+  WRC(isk_prof_begin(c, &ev));
+  double fscale;
+  WRC(ws_rhs_scale(w, &fscale));
+  const long long S2 = (long long)w->S * w->S;
+  hipLaunchKernelGGL(k_ws_rhs_store, dim3(ws_grid(S2)), dim3(EW_BLOCK), 0, c->stream,
+                     (const double *)c->rho, (const double *)nullptr, c->g.nx, fscale,
+                     (const double *)w->scal, (uint16_t *)nullptr, w->A0.a, (float *)nullptr);
+  ISK_CHECK_LAUNCH();
+  WRC(ws_forward(w, 1));
+  hipLaunchKernelGGL(k_ws_phi_out, dim3(ws_grid(S2)), dim3(EW_BLOCK), 0, c->stream,
+                     (const float *)w->out, c->phi, c->g.nx, w->gain, (const double *)w->scal);
+  ISK_CHECK_LAUNCH();
+  WRC(isk_prof_end(c, ev, ISKRA_K_WARMSTART, 0));
+  return ISKRA_OK;
+}
+
+} /* extern "C" */

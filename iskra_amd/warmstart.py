@@ -1,0 +1,177 @@
+"""Learned warm start for the 2-D Poisson solve: host driver.
+
+The network, its training and its inference run in libiskra_hip.so
+(iskra_ws_*, hand-written BF16 MFMA kernels); this module initialises
+parameters, generates (rho, phi) snapshots with the device simulator, trains,
+and measures what the trained network gains: Poisson iterations and time to
+relative residual rtol from a zero start, from the previous step's phi and
+from the network's prediction, with the same solver and the same stopping
+rule (the solver always iterates to rtol, so the warm start cannot change the
+answer beyond that tolerance).
+
+Snapshot source: a 2-D two-stream plasma (the 1-D two-stream instability of
+the 64-cell configuration, set on a side x side grid with phi = 0 walls):
+electrons in two counter-streaming beams along a random direction with small
+random multi-mode density perturbations, an immobile neutralising
+background, normalised units (eps0 = 1, omega_p = 1, dx = 1).
+"""
+import time
+
+import numpy as np
+
+from . import _lib as L
+from ._lib import Context, WarmStart
+
+WS_IN, WS_OUT = 8, 16
+
+
+def layer_shapes(levels, base):
+    """(cin, cout) of each convolution, in parameter order."""
+    out = []
+    for l in range(levels):
+        out.append((WS_IN if l == 0 else base << (l - 1), base << l))
+        out.append((base << l, base << l))
+    for l in range(levels - 2, -1, -1):
+        out.append((base << (l + 1), base << l))
+        out.append((base << l, base << l))
+    out.append((base, WS_OUT))
+    return out
+
+
+def layer_slices(levels, base):
+    """(offset, count, cin, cout) of each layer in the parameter vector."""
+    res, off = [], 0
+    for cin, cout in layer_shapes(levels, base):
+        n = 9 * cin * cout
+        res.append((off, n, cin, cout))
+        off += n
+    return res
+
+
+def he_init(levels, base, seed=0):
+    """He-normal initialisation ([cout][3x3][cin] per layer).  Padded
+    channels (inputs 1..7 of the first layer, outputs 1..15 of the last) are
+    zero; the output layer starts small."""
+    rng = np.random.default_rng(seed)
+    parts = []
+    shapes = layer_shapes(levels, base)
+    for k, (cin, cout) in enumerate(shapes):
+        w = np.zeros((cout, 9, cin), dtype=np.float32)
+        if k == 0:
+            w[:, :, 0] = rng.standard_normal((cout, 9)) * np.sqrt(2.0 / 9.0)
+        elif k == len(shapes) - 1:
+            w[0] = rng.standard_normal((9, cin)) * np.sqrt(1.0 / (9.0 * cin))
+        else:
+            w[:] = rng.standard_normal((cout, 9, cin)) * np.sqrt(2.0 / (9.0 * cin))
+        parts.append(w.ravel())
+    return np.concatenate(parts).astype(np.float32)
+
+
+# ---------------------------------------------------------------------------
+# snapshots
+
+def two_stream_setup(ctx, side, ppc=16, seed=0, v0=None, amp=0.02, modes=4, solver=None,
+                     rtol=1e-10):
+    """Grid, solver and one electron species on `ctx` for a 2-D two-stream
+    run; returns the species id and the time step."""
+    rng = np.random.default_rng(seed)
+    n = side + 1
+    ctx.grid(2, (n, n), (1.0, 1.0))
+    tags = np.zeros((n, n), np.int8)
+    tags[0, :] = tags[-1, :] = tags[:, 0] = tags[:, -1] = 1
+    ctx.poisson(L.POISSON_MG if solver is None else solver, 1.0, tags.ravel(), rtol=rtol)
+    ctx.tag_value(1, 0.0)
+    npart = ppc * side * side
+    w0 = float(side * side) / npart          # n0 = 1
+    sid = ctx.species(-1.0, 1.0, w0, npart + 16, track_ids=False)
+    ctx.boundary(sid, [L.BND_DISCARD, L.BND_DISCARD])
+    if v0 is None:
+        v0 = 0.9 * (side / 8.0) / (2 * np.pi)  # most unstable wavelength ~ side / 8
+    x = rng.random((npart, 2)) * side
+    # multi-mode density perturbation by displacement along random directions
+    for _ in range(modes):
+        k = 2 * np.pi * rng.integers(1, 9, size=2) / side
+        ph = rng.random() * 2 * np.pi
+        a = amp * rng.random() / np.linalg.norm(k)
+        arg = x @ k + ph
+        x += (a * np.sin(arg))[:, None] * (k / np.linalg.norm(k))[None, :]
+    x = np.clip(x, 1e-6, side - 1e-6)
+    ang = rng.random() * 2 * np.pi
+    d = np.array([np.cos(ang), np.sin(ang), 0.0])
+    v = np.zeros((npart, 3))
+    sgn = np.where(np.arange(npart) % 2 == 0, 1.0, -1.0)
+    v += (sgn * v0)[:, None] * d[None, :]
+    v += rng.standard_normal((npart, 3)) * (0.05 * v0)
+    ctx.upload(sid, x, v)
+    ctx.set_field(L.F_BG, np.full(n * n, 1.0))
+    return sid, 0.1
+
+
+def capture_two_stream(ws, runs, steps, skip=20, ppc=16, seed=0, record=None):
+    """Run `runs` two-stream simulations on the network's context, `steps`
+    steps each, capturing (f, phi) after every step beyond `skip`.  `record`
+    (list, optional) receives (rho, phi_prev, phi) of every captured step."""
+    ctx = ws.ctx
+    for r in range(runs):
+        sid, dt = two_stream_setup(ctx, ws.side, ppc=ppc, seed=seed * 1000 + r)
+        ctx.step(dt, skip)
+        prev = ctx.field(L.F_PHI) if record is not None else None
+        for s in range(steps):
+            ctx.step(dt, 1)
+            ws.capture()
+            if record is not None:
+                phi = ctx.field(L.F_PHI)
+                record.append((ctx.field(L.F_RHO), prev, phi))
+                prev = phi
+
+
+def train(ws, first, count, steps, batch=8, lr=1e-3, lr_end=1e-5, chunk=50, seed=1, log=None):
+    """Adam with a cosine learning-rate decay, `chunk` steps per device call.
+    The output gain is set from the dataset so that targets have unit RMS."""
+    gain = ws.dataset_rms(first, count)
+    ws.set_gain(gain)
+    losses = []
+    done = 0
+    t0 = time.perf_counter()
+    while done < steps:
+        k = min(chunk, steps - done)
+        frac = (done + 0.5 * k) / max(steps, 1)
+        rate = lr_end + 0.5 * (lr - lr_end) * (1 + np.cos(np.pi * frac))
+        losses.extend(ws.train(first, count, batch, k, rate, seed=seed + done))
+        done += k
+        if log is not None:
+            log(f"train {done}/{steps} loss {np.mean(losses[-k:]):.4e} "
+                f"({time.perf_counter() - t0:.1f} s)")
+    return np.asarray(losses), gain
+
+
+def solve_counts(ctx, ws, rho, phi_prev, kinds=(L.POISSON_PCG, L.POISSON_MG), rtol=1e-8):
+    """Iterations and ms of one solve of rho from zero, previous-step and
+    learned starts, for each solver kind.  The context's solver is
+    reconfigured for each kind (its grid must be the network's)."""
+    n = ws.side + 1
+    tags = np.zeros((n, n), np.int8)
+    tags[0, :] = tags[-1, :] = tags[:, 0] = tags[:, -1] = 1
+    res = {}
+    for kind in kinds:
+        ctx.poisson(kind, 1.0, tags.ravel(), rtol=rtol, max_iter=100000)
+        ctx.tag_value(1, 0.0)
+        ctx.set_field(L.F_RHO, rho)
+        row = {}
+        for start in ("zero", "prev", "learned"):
+            ctx.sync()
+            t0 = time.perf_counter()
+            if start == "zero":
+                it, rr = ctx.solve(L.WARM_ZERO)
+            else:
+                if start == "prev":
+                    ctx.set_field(L.F_PHI, phi_prev)
+                    ctx.sync()
+                    t0 = time.perf_counter()
+                else:
+                    ws.predict()
+                it, rr = ctx.solve(L.WARM_PREV)
+            ctx.sync()
+            row[start] = dict(iters=it, relres=rr, ms=1e3 * (time.perf_counter() - t0))
+        res[kind] = row
+    return res

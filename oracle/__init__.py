@@ -33,6 +33,12 @@ How each oracle function is pinned (checks live in tests/):
   orc_exp/log/sincos2pi  libm within a few ulp over dense sweeps.
   mir_*                  against the ref_* functions within FP32 tolerance
                          (tests/test_oracle.py), and bit-exactly against the GPU.
+  orc_ws_run             (learned warm start, oracle/ws_oracle.c; no reference
+                         counterpart) FP64 gradients against central finite
+                         differences; convolution, pooling and upsampling
+                         against torch.nn.functional on the CPU (a library
+                         routine, not the reference); the network is linear
+                         in the output layer and positively homogeneous.
 
 parity unpinned: none of the above can be checked against the reference's own
 run (Julia absent).  The reference's MersenneTwister streams, LAPACK round-off
@@ -108,6 +114,8 @@ def _declare(L):
         "mir_sample": (None, [C.c_int, I64, I64, D, D, D, D, C.c_int, C.c_int, P, P, P, P, U64,
                               U32, P, P, P, P, P, P, P]),
         "mir_sort_tile": (I64, [C.c_int, I64, C.c_int, C.c_int, P, P, P, P, P, P, P]),
+        "orc_ws_param_count": (I64, [C.c_int, C.c_int]),
+        "orc_ws_run": (D, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, P, P, P, P, P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -503,3 +511,24 @@ class MirSim:
             if self.sort_every > 0 and self.step_count % self.sort_every == 0:
                 for s in range(len(self.species)):
                     self.sort(s)
+
+
+# --------------------------------------------------------------------------
+# learned warm start network (oracle/ws_oracle.c)
+# --------------------------------------------------------------------------
+def ws_param_count(L, base):
+    return int(lib().orc_ws_param_count(L, base))
+
+
+def ws_run(L, base, params, inputs, target=None, round_bf16=True, grads=True):
+    """inputs: (B, S, S); returns (out (B,S,S), loss, grads or None)."""
+    x = np.ascontiguousarray(inputs, dtype=np.float64)
+    B, S = x.shape[0], x.shape[1]
+    p = np.ascontiguousarray(params, dtype=np.float64)
+    assert p.size == ws_param_count(L, base)
+    out = np.zeros_like(x)
+    t = None if target is None else np.ascontiguousarray(target, dtype=np.float64)
+    g = np.zeros_like(p) if (grads and t is not None) else None
+    loss = lib().orc_ws_run(L, base, S, B, int(round_bf16), ptr(p), ptr(x), ptr(t), ptr(out),
+                            ptr(g))
+    return out, loss, g

@@ -1,0 +1,106 @@
+"""GPU parity of the learned warm-start network (BF16 MFMA forward, input-
+and weight-gradient kernels) against the CPU oracle (oracle/ws_oracle.c in
+its bfloat16-rounding mode: same rounding points, FP64 sums), and the effect
+of a trained network on the Poisson solver."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+L_, BASE_, S_ = 3, 16, 64
+
+
+def _he_params(L, base, seed):
+    from iskra_amd import warmstart as W
+    return W.he_init(L, base, seed)
+
+
+def _net(ctx, L=L_, base=BASE_, S=S_, batch=2):
+    from iskra_amd import _lib as Lb
+    ctx.grid(2, (S + 1, S + 1), (1.0, 1.0))
+    return Lb.WarmStart(ctx, levels=L, base=base, side=S, max_batch=batch)
+
+
+def _relerr(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def test_ws_forward_matches_oracle(gpu_ready):
+    import oracle as O
+    from iskra_amd import Context
+    ctx = Context(0)
+    ws = _net(ctx)
+    p = _he_params(L_, BASE_, 1)
+    assert p.size == ws.nparam == O.ws_param_count(L_, BASE_)
+    ws.set_params(p)
+    rng = np.random.default_rng(11)
+    x = rng.standard_normal((2, S_, S_)).astype(np.float32)
+    out = ws.forward(x)
+    ref, _, _ = O.ws_run(L_, BASE_, p.astype(np.float64), x.astype(np.float64), round_bf16=True)
+    # same bf16 rounding points; FP32 (GPU) vs FP64 (oracle) sums may flip a
+    # rounding now and then: relative L2 error below 5e-3
+    e = _relerr(out, ref)
+    print("forward relerr", e)
+    assert e < 5e-3
+    ws.close()
+    ctx.close()
+
+
+def test_ws_gradients_match_oracle(gpu_ready):
+    import oracle as O
+    from iskra_amd import Context
+    ctx = Context(0)
+    ws = _net(ctx)
+    p = _he_params(L_, BASE_, 2)
+    ws.set_params(p)
+    rng = np.random.default_rng(12)
+    x = rng.standard_normal((2, S_, S_)).astype(np.float32)
+    t = rng.standard_normal((2, S_, S_)).astype(np.float32)
+    loss, g = ws.backward(x, t)
+    _, lref, gref = O.ws_run(L_, BASE_, p.astype(np.float64), x.astype(np.float64),
+                             t.astype(np.float64), round_bf16=True)
+    assert loss == pytest.approx(lref, rel=1e-3)
+    from iskra_amd import warmstart as W
+    for k, (off, n, cin, cout) in enumerate(W.layer_slices(L_, BASE_)):
+        e = _relerr(g[off:off + n], gref[off:off + n])
+        print("layer", k, cin, cout, "grad relerr", e)
+        assert e < 3e-2, (k, e)
+    ws.close()
+    ctx.close()
+
+
+def test_ws_training_cuts_pcg_iterations(gpu_ready):
+    """Train on snapshots of a 64 x 64 two-stream run, then solve held-out
+    snapshots of another run: the learned start must need fewer PCG
+    iterations than a zero start, and every solve still reaches 1e-8."""
+    from iskra_amd import Context, _lib as Lb
+    from iskra_amd import warmstart as W
+    ctx = Context(0)
+    ctx.grid(2, (S_ + 1, S_ + 1), (1.0, 1.0))
+    ws = Lb.WarmStart(ctx, levels=L_, base=BASE_, side=S_, max_batch=8)
+    ws.set_params(W.he_init(L_, BASE_, 3))
+    ws.reserve(400)
+    W.capture_two_stream(ws, runs=2, steps=150, seed=1)
+    assert ws.size() == 300
+    held = []
+    W.capture_two_stream(ws, runs=1, steps=40, seed=2, record=held)
+    before = None
+    ws.set_gain(ws.dataset_rms(0, 300))
+    before = ws.eval(300, 40)
+    losses, gain = W.train(ws, 0, 300, steps=400, batch=8, lr=2e-3)
+    after = ws.eval(300, 40)
+    print("held-out loss", before, "->", after, "gain", gain)
+    assert np.all(np.isfinite(losses))
+    assert after < 0.2 * before
+    zero, learned = [], []
+    for rho, prev, phi in held[::8]:
+        r = W.solve_counts(ctx, ws, rho, prev, kinds=(Lb.POISSON_PCG,))
+        row = r[Lb.POISSON_PCG]
+        print({k: v["iters"] for k, v in row.items()})
+        for v in row.values():
+            assert v["relres"] <= 1e-8
+        zero.append(row["zero"]["iters"])
+        learned.append(row["learned"]["iters"])
+    assert np.mean(learned) < np.mean(zero)
+    ws.close()
+    ctx.close()
