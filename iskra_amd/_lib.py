@@ -4,7 +4,9 @@ The library is built in-tree by `python -m iskra_amd.build`; importing this
 module never falls back to anything else: if the shared object is missing or
 no GPU is visible, calls raise IskraError.
 """
+import atexit
 import ctypes as C
+import weakref
 import os
 
 import numpy as np
@@ -147,6 +149,29 @@ def _p(a):
     return a.ctypes.data_as(P)
 
 
+_live_ws = weakref.WeakSet()
+_live_ctx = weakref.WeakSet()
+
+
+def close_all():
+    """Destroy every live warm-start network, then every live context, in that
+    order (networks use their context's stream).  Registered with atexit so
+    that teardown happens while the interpreter and the library are intact."""
+    for w in list(_live_ws):
+        try:
+            w.close()
+        except Exception:
+            pass
+    for c in list(_live_ctx):
+        try:
+            c.close()
+        except Exception:
+            pass
+
+
+atexit.register(close_all)
+
+
 class Context:
     """One device context: grid, species, fields, Poisson solver, MCC."""
 
@@ -156,9 +181,13 @@ class Context:
         self.h = h
         self.dim = None
         self.nodes = None
+        _live_ctx.add(self)
 
     def close(self):
         if self.h:
+            for w in list(_live_ws):
+                if w.ctx is self:
+                    w.close()
             lib().iskra_ctx_destroy(self.h)
             self.h = None
 
@@ -397,6 +426,7 @@ class WarmStart:
         check(lib().iskra_ws_create(ctx.h, levels, base, side, max_batch, C.byref(h)))
         self.h = h
         self.ctx = ctx
+        _live_ws.add(self)
         self.levels, self.base, self.side, self.max_batch = levels, base, side, max_batch
         n = C.c_int64()
         check(lib().iskra_ws_param_count(self.h, C.byref(n)))

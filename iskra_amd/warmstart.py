@@ -15,6 +15,7 @@ electrons in two counter-streaming beams along a random direction with small
 random multi-mode density perturbations, an immobile neutralising
 background, normalised units (eps0 = 1, omega_p = 1, dx = 1).
 """
+import json
 import time
 
 import numpy as np
@@ -175,3 +176,113 @@ def solve_counts(ctx, ws, rho, phi_prev, kinds=(L.POISSON_PCG, L.POISSON_MG), rt
             row[start] = dict(iters=it, relres=rr, ms=1e3 * (time.perf_counter() - t0))
         res[kind] = row
     return res
+
+
+def experiment(side=512, levels=4, base=32, runs=8, steps_per_run=500, train_steps=1000,
+               batch=8, lr=1e-3, held_steps=41, every=10, seed=0, log=print):
+    """Capture, train, evaluate; returns a dict of the measured figures.
+    Everything runs on device 0 in one context, closed before returning."""
+    t0 = time.perf_counter()
+
+    def say(m):
+        if log is not None:
+            log(f"[{time.perf_counter() - t0:7.1f} s] {m}")
+
+    ctx = Context(0)
+    ws = None
+    try:
+        ctx.grid(2, (side + 1, side + 1), (1.0, 1.0))
+        ws = WarmStart(ctx, levels=levels, base=base, side=side, max_batch=batch)
+        ws.set_params(he_init(levels, base, seed))
+        n = runs * steps_per_run
+        ws.reserve(n + held_steps)
+        t = time.perf_counter()
+        for r in range(runs):
+            capture_two_stream(ws, runs=1, steps=steps_per_run, seed=seed * 100 + 10 + r)
+        ctx.sync()
+        cap_s = time.perf_counter() - t
+        say(f"captured {ws.size()} snapshots in {cap_s:.1f} s")
+        held = []
+        capture_two_stream(ws, runs=1, steps=held_steps, seed=seed * 100 + 99, record=held)
+        ws.set_gain(ws.dataset_rms(0, n))
+        e0 = ws.eval(n, held_steps)
+        t = time.perf_counter()
+        losses, gain = train(ws, 0, n, steps=train_steps, batch=batch, lr=lr,
+                             chunk=max(1, train_steps // 10), log=say)
+        ctx.sync()
+        tr_s = time.perf_counter() - t
+        e1 = ws.eval(n, held_steps)
+        say(f"held-out relative MSE {e0:.4e} -> {e1:.4e}")
+        ctx.set_field(L.F_RHO, held[0][0])
+        ctx.profile(True)
+        for _ in range(10):
+            ws.predict()
+        st = ctx.stats()
+        ctx.profile(False)
+        inf_ms = st["kernel_ms"]["warmstart"] / max(st["kernel_calls"]["warmstart"], 1)
+        names = {L.POISSON_PCG: "pcg", L.POISSON_MG: "mg"}
+        acc = {nm: {s: {"iters": [], "ms": []} for s in ("zero", "prev", "learned")}
+               for nm in names.values()}
+        # one untimed solve per kind first (first-use set-up)
+        solve_counts(ctx, ws, held[0][0], held[0][1])
+        for rho, prev, _phi in held[1::every]:
+            r = solve_counts(ctx, ws, rho, prev)
+            for kind, row in r.items():
+                for s, v in row.items():
+                    acc[names[kind]][s]["iters"].append(v["iters"])
+                    acc[names[kind]][s]["ms"].append(v["ms"])
+        summary = {k: {s: {"iters": float(np.mean(v["iters"])), "ms": float(np.mean(v["ms"]))}
+                       for s, v in row.items()} for k, row in acc.items()}
+        flops = forward_flops(levels, base, side)
+        out = dict(side=side, levels=levels, base=base, params=ws.nparam, snapshots=n,
+                   capture_s=cap_s, train_steps=train_steps, batch=batch, train_s=tr_s,
+                   train_ms_per_step=1e3 * tr_s / max(train_steps, 1),
+                   train_tflops=3 * flops * batch / (tr_s / max(train_steps, 1)) / 1e12,
+                   heldout_mse_before=e0, heldout_mse_after=e1, gain=gain,
+                   inference_ms=inf_ms, inference_tflops=flops / (inf_ms * 1e-3) / 1e12,
+                   solves=summary, evaluated=len(held[1::every]))
+        say(json.dumps(summary))
+        return out
+    finally:
+        if ws is not None:
+            ws.close()
+        ctx.close()
+
+
+def forward_flops(levels, base, side):
+    """Multiply-add flops (x2) of one forward pass on one sample, counting the
+    padded channels the kernels compute."""
+    f = 0
+    shapes = layer_shapes(levels, base)
+    lev = [l for l in range(levels) for _ in (0, 1)] + \
+          [l for l in range(levels - 2, -1, -1) for _ in (0, 1)] + [0]
+    for (cin, cout), l in zip(shapes, lev):
+        f += 2 * (side >> l) ** 2 * 9 * cin * cout
+    return f
+
+
+def main(argv=None):
+    import argparse
+    ap = argparse.ArgumentParser(description="learned warm start: capture, train, evaluate")
+    ap.add_argument("--side", type=int, default=512)
+    ap.add_argument("--levels", type=int, default=4)
+    ap.add_argument("--base", type=int, default=32)
+    ap.add_argument("--runs", type=int, default=40)
+    ap.add_argument("--steps-per-run", type=int, default=500)
+    ap.add_argument("--train-steps", type=int, default=4000)
+    ap.add_argument("--batch", type=int, default=8)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--out", default=None)
+    a = ap.parse_args(argv)
+    res = experiment(a.side, a.levels, a.base, a.runs, a.steps_per_run, a.train_steps, a.batch,
+                     a.lr, log=lambda m: print(m, flush=True))
+    s = json.dumps(res)
+    print(s, flush=True)
+    if a.out:
+        with open(a.out, "w") as f:
+            f.write(s + "\n")
+    print("closed", flush=True)
+
+
+if __name__ == "__main__":
+    main()
