@@ -264,6 +264,29 @@ int iskra_stats_get(iskra_ctx *ctx, iskra_stats *out);
  * src/iskra.jl:51). */
 int iskra_profile(iskra_ctx *ctx, int enable);
 
+/* ---- batched ensembles of 1-D discharges --------------------------------- */
+/* No reference counterpart: the reference runs one simulation per process
+ * (ParticleInCell/src/ParticleInCell.jl:84-139).  A batch groups n contexts
+ * set up through the calls above (each a 1-D grid with the DIRECT Poisson
+ * solve; all with the same number of species and interactions, the same
+ * species roles in each interaction, the same sort interval and step count).
+ * iskra_batch_step advances all of them by nsteps iterations of iskra_step:
+ * each kernel type of the step is one launch covering every instance's
+ * work-groups, each instance computing exactly what iskra_step computes for
+ * it alone (particles, fields and collision counts are bit for bit those of
+ * the same context stepped alone with the same tuning).  Cycles of
+ * 2 x sort_every steps are captured in a hipGraph and replayed
+ * (iskra_batch_set_graph(b, 0) launches step by step instead).  The contexts
+ * stay usable through every other call between batch steps; a context must
+ * not be destroyed while its batch exists. */
+typedef struct iskra_batch iskra_batch;
+int iskra_batch_create(iskra_ctx **ctxs, int n, iskra_batch **batch);
+int iskra_batch_destroy(iskra_batch *batch);
+int iskra_batch_set_graph(iskra_batch *batch, int enable);
+int iskra_batch_step(iskra_batch *batch, double dt, int64_t nsteps);
+/* graph replays issued so far (each one 2 x sort_every steps) */
+int iskra_batch_graphs_launched(iskra_batch *batch, int64_t *count);
+
 /* ---- learned warm start for the 2-D Poisson solve ----------------------- */
 /* No reference counterpart: the reference solves A\b directly every step
  * (FiniteDifferenceMethod/src/generalized_poisson.jl:201-203,372-378), so it

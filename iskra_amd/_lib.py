@@ -36,6 +36,8 @@ EXPORTS = [
     "iskra_ws_get_params", "iskra_ws_set_gain", "iskra_ws_forward", "iskra_ws_backward",
     "iskra_ws_dataset_reserve", "iskra_ws_capture", "iskra_ws_dataset_size",
     "iskra_ws_dataset_rms", "iskra_ws_train", "iskra_ws_eval", "iskra_ws_predict",
+    "iskra_batch_create", "iskra_batch_destroy", "iskra_batch_set_graph", "iskra_batch_step",
+    "iskra_batch_graphs_launched",
 ]
 
 
@@ -127,6 +129,11 @@ def lib():
             "iskra_ws_train": (I, [P, I64, I64, I, I, C.c_float, U64, P]),
             "iskra_ws_eval": (I, [P, I64, I64, C.POINTER(D)]),
             "iskra_ws_predict": (I, [P]),
+            "iskra_batch_create": (I, [P, I, C.POINTER(P)]),
+            "iskra_batch_destroy": (I, [P]),
+            "iskra_batch_set_graph": (I, [P, I]),
+            "iskra_batch_step": (I, [P, D, I64]),
+            "iskra_batch_graphs_launched": (I, [P, C.POINTER(I64)]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -150,6 +157,7 @@ def _p(a):
 
 
 _live_ws = weakref.WeakSet()
+_live_batch = weakref.WeakSet()
 _live_ctx = weakref.WeakSet()
 
 
@@ -157,7 +165,7 @@ def close_all():
     """Destroy every live warm-start network, then every live context, in that
     order (networks use their context's stream).  Registered with atexit so
     that teardown happens while the interpreter and the library are intact."""
-    for w in list(_live_ws):
+    for w in list(_live_ws) + list(_live_batch):
         try:
             w.close()
         except Exception:
@@ -188,6 +196,9 @@ class Context:
             for w in list(_live_ws):
                 if w.ctx is self:
                     w.close()
+            for bt in list(_live_batch):
+                if any(c is self for c in bt.ctxs):
+                    bt.close()
             lib().iskra_ctx_destroy(self.h)
             self.h = None
 
@@ -501,3 +512,38 @@ class WarmStart:
 
     def predict(self):
         check(lib().iskra_ws_predict(self.h))
+
+
+class Batch:
+    """Contexts advanced together (iskra_batch_*): each member computes
+    exactly what it computes when stepped alone."""
+
+    def __init__(self, ctxs):
+        self.ctxs = list(ctxs)
+        arr = (P * len(self.ctxs))(*[c.h for c in self.ctxs])
+        h = P()
+        check(lib().iskra_batch_create(arr, len(self.ctxs), C.byref(h)))
+        self.h = h
+        _live_batch.add(self)
+
+    def close(self):
+        if self.h:
+            lib().iskra_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def graph(self, enable=True):
+        check(lib().iskra_batch_set_graph(self.h, int(enable)))
+
+    def step(self, dt, nsteps=1):
+        check(lib().iskra_batch_step(self.h, float(dt), int(nsteps)))
+
+    def graphs_launched(self):
+        n = C.c_int64()
+        check(lib().iskra_batch_graphs_launched(self.h, C.byref(n)))
+        return n.value

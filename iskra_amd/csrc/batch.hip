@@ -1,0 +1,407 @@
+/*
+ * batch.hip -- many independent 1-D discharges advanced together.
+ *
+ * A batch groups contexts that were set up one by one through the ordinary
+ * API (grid, species, particles, MCC, direct Poisson solve, electrode
+ * waveforms, tuning).  iskra_batch_step replays iskra_step's sequence for all
+ * of them at once: every kernel type of the step is ONE launch whose
+ * work-groups are the union of the instances' own work-groups, each running
+ * the same device function the single-context kernel runs (see
+ * isk_internal.h).  Each instance therefore computes bit for bit what it
+ * computes when stepped alone, whatever the other instances are.
+ *
+ * Whole sort cycles (2 x sort_every steps, after which every species is back
+ * in the same ping-pong buffer) are captured once in a hipGraph and replayed;
+ * the step counter that the MCC draws and the electrode waveforms depend on
+ * lives in device memory and is advanced by a kernel inside the graph.
+ * There is no counterpart in the reference, which runs one simulation per
+ * process (ParticleInCell/src/ParticleInCell.jl:84-139).
+ */
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "isk_fp64math.h"
+#include "isk_internal.h"
+
+#define BRC(x)             \
+  do {                     \
+    int _rc = (x);         \
+    if (_rc) return _rc;   \
+  } while (0)
+
+int isk_blaunch_upload(IskBLaunch *L, int kind, const void *args, size_t size,
+                       const std::vector<int> &blocks, int block, size_t shm) {
+  isk_blaunch_free(L);
+  const int n = (int)blocks.size();
+  std::vector<int> bs(n + 1, 0);
+  for (int i = 0; i < n; ++i) bs[i + 1] = bs[i] + blocks[i];
+  BRC(isk_alloc(&L->args, std::max<size_t>(size * n, 64)));
+  BRC(isk_alloc((void **)&L->bstart, (size_t)(n + 1) * 4));
+  ISK_HIP(hipMemcpy(L->args, args, size * n, hipMemcpyHostToDevice));
+  ISK_HIP(hipMemcpy(L->bstart, bs.data(), (size_t)(n + 1) * 4, hipMemcpyHostToDevice));
+  L->kind = kind;
+  L->n = n;
+  L->total = bs[n];
+  L->block = block;
+  L->shm = shm;
+  return ISKRA_OK;
+}
+
+void isk_blaunch_free(IskBLaunch *L) {
+  isk_free(L->args);
+  isk_free(L->bstart);
+  L->args = nullptr;
+  L->bstart = nullptr;
+  L->kind = -1;
+  L->n = L->total = 0;
+}
+
+/* ---- electrode waveforms and the step counter ---- */
+struct WaveArgs {
+  double *tagval;
+  int tag;
+  double off, amp, f, ph, dt;
+  long long step0;
+  const long long *step_dev;
+};
+
+/* value(t) = off + amp sin(2 pi (f t + ph)), t = (step0 + steps taken) dt:
+ * the arithmetic of the single-context step (k_tag_waveform) */
+__global__ void kb_waveform(const WaveArgs *A, const int *bstart, int n) {
+  const int i = isk_bfind(bstart, n, blockIdx.x);
+  if (threadIdx.x != 0) return;
+  const WaveArgs &a = A[i];
+  const double t = (double)(a.step0 + *a.step_dev) * a.dt;
+  double u = a.f * t + a.ph;
+  u = u - floor(u);
+  double s, co;
+  isk_sincos2pi(u, &s, &co);
+  a.tagval[a.tag] = a.off + a.amp * s;
+}
+
+__global__ void kb_stepinc(long long *step_dev, long long v) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *step_dev = v < 0 ? *step_dev + 1 : v;
+}
+
+int isk_bbuild_waveform(iskra_ctx **cs, int n, double dt, long long *step_dev, IskBLaunch *L) {
+  std::vector<WaveArgs> A;
+  std::vector<int> blocks;
+  for (int i = 0; i < n; ++i) {
+    iskra_ctx *c = cs[i];
+    for (int k = 1; k <= ISK_MAX_TAGS; ++k)
+      if (c->ps.wf_on[k] && c->ps.d_tag_val) {
+        WaveArgs a;
+        a.tagval = c->ps.d_tag_val;
+        a.tag = k;
+        a.off = c->ps.wf_off[k];
+        a.amp = c->ps.wf_amp[k];
+        a.f = c->ps.wf_f[k];
+        a.ph = c->ps.wf_ph[k];
+        a.dt = dt;
+        a.step0 = c->step_count;
+        a.step_dev = step_dev;
+        A.push_back(a);
+        blocks.push_back(1);
+      }
+  }
+  if (A.empty()) {
+    isk_blaunch_free(L);
+    L->kind = ISK_B_WAVEFORM;
+    return ISKRA_OK;
+  }
+  return isk_blaunch_upload(L, ISK_B_WAVEFORM, A.data(), sizeof(WaveArgs), blocks, 64, 0);
+}
+
+/* ---- the batch ---- */
+struct iskra_batch {
+  std::vector<iskra_ctx *> cs;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  long long *step_dev = nullptr;
+  int nspecies = 0, nmcc = 0, sort_every = 0;
+  int use_graph = 1;
+  /* per phase (species buffers flipped or not) */
+  std::vector<IskBLaunch> mcc[2], fill, push[2], merge, sort[2];
+  IskBLaunch rho, tri, efield, wave;
+  double built_dt = 0.0;
+  int64_t graphs_launched = 0;
+};
+
+static void batch_free_launches(iskra_batch *b) {
+  for (int ph = 0; ph < 2; ++ph) {
+    for (auto &L : b->mcc[ph]) isk_blaunch_free(&L);
+    for (auto &L : b->push[ph]) isk_blaunch_free(&L);
+    for (auto &L : b->sort[ph]) isk_blaunch_free(&L);
+    b->mcc[ph].clear();
+    b->push[ph].clear();
+    b->sort[ph].clear();
+  }
+  for (auto &L : b->fill) isk_blaunch_free(&L);
+  for (auto &L : b->merge) isk_blaunch_free(&L);
+  b->fill.clear();
+  b->merge.clear();
+  isk_blaunch_free(&b->rho);
+  isk_blaunch_free(&b->tri);
+  isk_blaunch_free(&b->efield);
+  isk_blaunch_free(&b->wave);
+}
+
+static int batch_launch(const IskBLaunch &L, hipStream_t st) {
+  switch (L.kind) {
+    case ISK_B_MCC:
+    case ISK_B_MCC_SCAN:
+    case ISK_B_MCC_APPEND:
+    case ISK_B_MCC_COMMIT:
+      return isk_blaunch_mcc(L, st);
+    case ISK_B_TRI:
+    case ISK_B_EFIELD:
+      return isk_blaunch_poisson(L, st);
+    case ISK_B_WAVEFORM:
+      if (L.total <= 0) return ISKRA_OK;
+      hipLaunchKernelGGL(kb_waveform, dim3((unsigned)L.total), dim3(64), 0, st,
+                         (const WaveArgs *)L.args, L.bstart, L.n);
+      ISK_CHECK_LAUNCH();
+      return ISKRA_OK;
+    default:
+      return isk_blaunch_particles(L, st);
+  }
+}
+
+/* all launch descriptors for time step dt (both buffer phases) */
+static int batch_build(iskra_batch *b, double dt) {
+  batch_free_launches(b);
+  const int n = (int)b->cs.size();
+  iskra_ctx **cs = b->cs.data();
+  std::vector<int> cur(n), cur_ion(n);
+  for (int ph = 0; ph < 2; ++ph) {
+    b->mcc[ph].resize(4 * b->nmcc);
+    for (int m = 0; m < b->nmcc; ++m) {
+      for (int i = 0; i < n; ++i) {
+        const IskMcc &mm = cs[i]->mcc[m];
+        cur[i] = cs[i]->sp[mm.src].cur ^ ph;
+        cur_ion[i] = (mm.ion >= 0 && mm.ion < cs[i]->nspecies) ? (cs[i]->sp[mm.ion].cur ^ ph) : 0;
+      }
+      BRC(isk_bbuild_mcc(cs, n, m, dt, cur.data(), cur_ion.data(), b->step_dev,
+                         &b->mcc[ph][4 * m]));
+    }
+    b->push[ph].resize(b->nspecies);
+    b->sort[ph].resize(6 * b->nspecies);
+    for (int s = 0; s < b->nspecies; ++s) {
+      for (int i = 0; i < n; ++i) cur[i] = cs[i]->sp[s].cur ^ ph;
+      BRC(isk_bbuild_push(cs, n, s, dt, cur.data(), &b->push[ph][s]));
+      BRC(isk_bbuild_sort(cs, n, s, cur.data(), &b->sort[ph][6 * s]));
+    }
+  }
+  b->fill.resize(b->nspecies);
+  b->merge.resize(b->nspecies);
+  for (int s = 0; s < b->nspecies; ++s) {
+    BRC(isk_bbuild_fill(cs, n, s, &b->fill[s]));
+    BRC(isk_bbuild_merge(cs, n, s, &b->merge[s]));
+  }
+  BRC(isk_bbuild_rho(cs, n, &b->rho));
+  BRC(isk_bbuild_tri(cs, n, &b->tri));
+  BRC(isk_bbuild_efield(cs, n, &b->efield));
+  BRC(isk_bbuild_waveform(cs, n, dt, b->step_dev, &b->wave));
+  b->built_dt = dt;
+  return ISKRA_OK;
+}
+
+/* one time step for all instances in buffer phase ph; sorts if asked */
+static int batch_enqueue_step(iskra_batch *b, int ph, bool sort) {
+  hipStream_t st = b->stream;
+  for (auto &L : b->mcc[ph]) BRC(batch_launch(L, st));
+  for (int s = 0; s < b->nspecies; ++s) {
+    BRC(batch_launch(b->fill[s], st));
+    BRC(batch_launch(b->push[ph][s], st));
+    BRC(batch_launch(b->merge[s], st));
+  }
+  BRC(batch_launch(b->rho, st));
+  BRC(batch_launch(b->tri, st));
+  BRC(batch_launch(b->efield, st));
+  BRC(batch_launch(b->wave, st));
+  hipLaunchKernelGGL(kb_stepinc, dim3(1), dim3(64), 0, st, b->step_dev, -1ll);
+  ISK_CHECK_LAUNCH();
+  if (sort)
+    for (int s = 0; s < b->nspecies; ++s)
+      for (int k = 0; k < 6; ++k) BRC(batch_launch(b->sort[ph][6 * s + k], st));
+  return ISKRA_OK;
+}
+
+extern "C" {
+
+int iskra_batch_create(iskra_ctx **ctxs, int n, iskra_batch **out) {
+  if (!ctxs || n < 1 || !out) {
+    isk_set_error("batch_create: need at least one context");
+    return ISKRA_E_ARG;
+  }
+  isk_clear_error();
+  *out = nullptr;
+  iskra_ctx *c0 = ctxs[0];
+  if (!c0) return ISKRA_E_ARG;
+  for (int i = 0; i < n; ++i) {
+    iskra_ctx *c = ctxs[i];
+    if (!c || !c->have_grid) {
+      isk_set_error("batch_create: instance %d has no grid", i);
+      return ISKRA_E_STATE;
+    }
+    if (c->g.dim != 1 || c->ps.kind != ISKRA_POISSON_DIRECT) {
+      isk_set_error("batch_create: instance %d is not a 1-D grid with the direct Poisson solve",
+                    i);
+      return ISKRA_E_UNSUPPORTED;
+    }
+    if (c->device != c0->device || c->nspecies != c0->nspecies || c->nmcc != c0->nmcc ||
+        c->t.sort_every != c0->t.sort_every || c->step_count != c0->step_count) {
+      isk_set_error("batch_create: instance %d differs from instance 0 in device, species or "
+                    "interaction count, sort interval or step count", i);
+      return ISKRA_E_ARG;
+    }
+    for (int m = 0; m < c->nmcc; ++m)
+      if (c->mcc[m].src != c0->mcc[m].src || c->mcc[m].ion != c0->mcc[m].ion) {
+        isk_set_error("batch_create: instance %d: interaction %d has other species", i, m);
+        return ISKRA_E_ARG;
+      }
+    for (int s = 0; s < c->nspecies; ++s)
+      if (c->sp[s].cap <= 0) {
+        isk_set_error("batch_create: instance %d species %d has no capacity", i, s);
+        return ISKRA_E_ARG;
+      }
+  }
+  ISK_HIP(hipSetDevice(c0->device));
+  iskra_batch *b = new iskra_batch();
+  b->cs.assign(ctxs, ctxs + n);
+  b->device = c0->device;
+  b->nspecies = c0->nspecies;
+  b->nmcc = c0->nmcc;
+  b->sort_every = c0->t.sort_every;
+  hipError_t e = hipStreamCreate(&b->stream);
+  if (e != hipSuccess) {
+    delete b;
+    return isk_hip_fail(e, "hipStreamCreate", __FILE__, __LINE__);
+  }
+  if (isk_alloc((void **)&b->step_dev, 64)) {
+    (void)hipStreamDestroy(b->stream);
+    delete b;
+    return ISKRA_E_HIP;
+  }
+  *out = b;
+  return ISKRA_OK;
+}
+
+int iskra_batch_destroy(iskra_batch *b) {
+  if (!b) return ISKRA_OK;
+  (void)hipSetDevice(b->device);
+  (void)hipStreamSynchronize(b->stream);
+  batch_free_launches(b);
+  isk_free(b->step_dev);
+  (void)hipStreamDestroy(b->stream);
+  delete b;
+  return ISKRA_OK;
+}
+
+int iskra_batch_set_graph(iskra_batch *b, int enable) {
+  if (!b) return ISKRA_E_ARG;
+  b->use_graph = enable ? 1 : 0;
+  return ISKRA_OK;
+}
+
+int iskra_batch_step(iskra_batch *b, double dt, int64_t nsteps) {
+  if (!b) return ISKRA_E_ARG;
+  isk_clear_error();
+  if (!(dt > 0) || nsteps < 0) {
+    isk_set_error("batch_step: dt must be positive");
+    return ISKRA_E_ARG;
+  }
+  ISK_HIP(hipSetDevice(b->device));
+  const int n = (int)b->cs.size();
+  for (int i = 0; i < n; ++i) ISK_HIP(hipStreamSynchronize(b->cs[i]->stream));
+  if (nsteps == 0) return ISKRA_OK;
+  /* descriptors are rebuilt per call: buffers, tunings and scratch of the
+   * contexts may have changed through the single-context API in between.
+   * The first build sizes every scratch area, the second one is kept. */
+  BRC(batch_build(b, dt));
+  BRC(batch_build(b, dt));
+  hipLaunchKernelGGL(kb_stepinc, dim3(1), dim3(64), 0, b->stream, b->step_dev, 0ll);
+  ISK_CHECK_LAUNCH();
+  const int64_t sc0 = b->cs[0]->step_count;
+  const int se = b->sort_every;
+  const int64_t G = se > 0 ? 2 * (int64_t)se : 16;
+  int ph = 0;
+  int64_t k = 0;
+  hipGraphExec_t exec = nullptr;
+  hipGraph_t graph = nullptr;
+  int rc = ISKRA_OK;
+  while (k < nsteps && rc == ISKRA_OK) {
+    if (b->use_graph && nsteps - k >= G) {
+      if (!exec) {
+        /* phase at the start of every full cycle is the same (two flips) */
+        ISK_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        int cph = ph;
+        for (int64_t q = 0; q < G && rc == ISKRA_OK; ++q) {
+          const bool sort = se > 0 && (sc0 + k + q + 1) % se == 0;
+          rc = batch_enqueue_step(b, cph, sort);
+          if (sort) cph ^= 1;
+        }
+        hipError_t e = hipStreamEndCapture(b->stream, &graph);
+        if (rc) break;
+        if (e != hipSuccess) {
+          rc = isk_hip_fail(e, "hipStreamEndCapture", __FILE__, __LINE__);
+          break;
+        }
+        e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        if (e != hipSuccess) {
+          rc = isk_hip_fail(e, "hipGraphInstantiate", __FILE__, __LINE__);
+          break;
+        }
+      }
+      hipError_t e = hipGraphLaunch(exec, b->stream);
+      if (e != hipSuccess) {
+        rc = isk_hip_fail(e, "hipGraphLaunch", __FILE__, __LINE__);
+        break;
+      }
+      ++b->graphs_launched;
+      k += G; /* ph unchanged after a full cycle */
+    } else {
+      const bool sort = se > 0 && (sc0 + k + 1) % se == 0;
+      rc = batch_enqueue_step(b, ph, sort);
+      if (sort) ph ^= 1;
+      ++k;
+    }
+  }
+  hipError_t es = hipStreamSynchronize(b->stream);
+  if (exec) (void)hipGraphExecDestroy(exec);
+  if (graph) (void)hipGraphDestroy(graph);
+  if (rc) return rc;
+  if (es != hipSuccess) return isk_hip_fail(es, "hipStreamSynchronize", __FILE__, __LINE__);
+  /* host-side state of every instance, as iskra_step leaves it */
+  for (int i = 0; i < n; ++i) {
+    iskra_ctx *c = b->cs[i];
+    c->step_count += nsteps;
+    c->stats.steps += nsteps;
+    c->stats.poisson_solves += nsteps;
+    c->stats.poisson_iters += nsteps;
+    c->stats.last_iters = 1;
+    c->stats.last_relres = 0.0;
+    c->ps.last_iters = 1;
+    c->ps.last_relres = 0.0;
+    for (int m = 0; m < c->nmcc; ++m) c->mcc[m].step += (uint32_t)nsteps;
+    for (int s = 0; s < c->nspecies; ++s) {
+      c->sp[s].cur ^= ph;
+      c->sp[s].deposited = 1;
+    }
+  }
+  for (int i = 0; i < n; ++i) {
+    int r = iskra_sync(b->cs[i]);
+    if (r) return r;
+  }
+  return ISKRA_OK;
+}
+
+int iskra_batch_graphs_launched(iskra_batch *b, int64_t *count) {
+  if (!b || !count) return ISKRA_E_ARG;
+  *count = b->graphs_launched;
+  return ISKRA_OK;
+}
+
+} /* extern "C" */

@@ -271,4 +271,55 @@ int isk_species_alloc(iskra_ctx *c, IskSpecies *s);
 void isk_species_release(IskSpecies *s);
 int isk_species_reset_chunks(iskra_ctx *c, int sid);
 
+/* ---- batched launches over instances (iskra_batch_*) ----
+ * A batched launch runs one kernel type for n contexts at once: work-group
+ * g belongs to instance i with bstart[i] <= g < bstart[i+1] and acts as
+ * work-group g - bstart[i] of that instance's own launch, through the same
+ * device function as the single-context kernel, so every instance computes
+ * exactly what it computes alone. */
+enum IskBKind {
+  ISK_B_FILL = 0, ISK_B_PUSH1, ISK_B_MERGE1, ISK_B_RHO,
+  ISK_B_SORT_HIST, ISK_B_SORT_COLSCAN, ISK_B_SORT_SEGSCAN, ISK_B_SORT_BINSCAN,
+  ISK_B_SORT_SCATTER, ISK_B_SORT_FINISH,
+  ISK_B_MCC, ISK_B_MCC_SCAN, ISK_B_MCC_APPEND, ISK_B_MCC_COMMIT,
+  ISK_B_TRI, ISK_B_EFIELD, ISK_B_WAVEFORM, ISK_B_STEPINC
+};
+struct IskBLaunch {
+  int kind = -1;
+  void *args = nullptr;   /* device array of n per-instance argument structs */
+  int *bstart = nullptr;  /* device, n + 1 work-group offsets */
+  int n = 0, total = 0;   /* instances, work-groups of the launch */
+  int block = 256;        /* work-group size */
+  size_t shm = 0;         /* dynamic LDS bytes (maximum over instances) */
+  long long *step_dev = nullptr; /* device step counter of the batch */
+};
+__device__ __forceinline__ int isk_bfind(const int *bstart, int n, int g) {
+  int lo = 0, hi = n; /* largest i with bstart[i] <= g */
+  while (hi - lo > 1) {
+    int mid = (lo + hi) >> 1;
+    if (bstart[mid] <= g) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+/* uploads args (n structs of `size` bytes) and the block prefix of `blocks` */
+int isk_blaunch_upload(IskBLaunch *L, int kind, const void *args, size_t size,
+                       const std::vector<int> &blocks, int block, size_t shm);
+void isk_blaunch_free(IskBLaunch *L);
+/* builders (cur: the species buffer each instance reads, per context) */
+int isk_bbuild_fill(iskra_ctx **cs, int n, int sid, IskBLaunch *L);
+int isk_bbuild_push(iskra_ctx **cs, int n, int sid, double dt, const int *cur, IskBLaunch *L);
+int isk_bbuild_merge(iskra_ctx **cs, int n, int sid, IskBLaunch *L);
+int isk_bbuild_rho(iskra_ctx **cs, int n, IskBLaunch *L);
+int isk_bbuild_sort(iskra_ctx **cs, int n, int sid, const int *cur, IskBLaunch L[6]);
+int isk_bbuild_mcc(iskra_ctx **cs, int n, int m, double dt, const int *cur_src,
+                   const int *cur_ion, long long *step_dev, IskBLaunch L[4]);
+int isk_bbuild_tri(iskra_ctx **cs, int n, IskBLaunch *L);
+int isk_bbuild_efield(iskra_ctx **cs, int n, IskBLaunch *L);
+int isk_bbuild_waveform(iskra_ctx **cs, int n, double dt, long long *step_dev, IskBLaunch *L);
+/* launchers, one per translation unit (by kind) */
+int isk_blaunch_particles(const IskBLaunch &L, hipStream_t st);
+int isk_blaunch_mcc(const IskBLaunch &L, hipStream_t st);
+int isk_blaunch_poisson(const IskBLaunch &L, hipStream_t st);
+
 #endif /* ISK_INTERNAL_H */

@@ -51,6 +51,7 @@ struct MccArgs {
   double vth_t;  /* target thermal speed sqrt(2 kB T / m) */
   double tqm;    /* target q/m */
   uint32_t k0, k1, step, stream;
+  const long long *step_dev; /* batched runs: draws use step + *step_dev */
   unsigned long long *nu, *events;
   int *errflag;
   int *blk_cnt;
@@ -147,10 +148,10 @@ __device__ void randn3(Draws &d, double *n) {
   n[2] = r2 * c2;
 }
 
-__global__ __launch_bounds__(MCC_BLOCK) void k_mcc(MccArgs a) {
+__device__ __forceinline__ void mcc_body(const MccArgs &a, const int blk) {
   __shared__ int wcount[MCC_BLOCK / 64];
   const long long np = a.hdr[HDR_NP];
-  const long long p = (long long)blockIdx.x * MCC_BLOCK + threadIdx.x;
+  const long long p = (long long)blk * MCC_BLOCK + threadIdx.x;
   int prod = 0;
   float pe[3] = {0.f, 0.f, 0.f}, pi[3] = {0.f, 0.f, 0.f};
   int32_t pcell = 0;
@@ -160,7 +161,7 @@ __global__ __launch_bounds__(MCC_BLOCK) void k_mcc(MccArgs a) {
     if (c >= 0) {
       Draws d;
       d.id = a.id[p];
-      d.step = a.step;
+      d.step = a.step_dev ? (uint32_t)(a.step + *a.step_dev) : a.step;
       d.stream = a.stream;
       d.k0 = a.k0;
       d.k1 = a.k1;
@@ -271,7 +272,7 @@ __global__ __launch_bounds__(MCC_BLOCK) void k_mcc(MccArgs a) {
   int off = 0;
   for (int k = 0; k < w; ++k) off += wcount[k];
   if (prod) {
-    size_t s = (size_t)blockIdx.x * MCC_BLOCK + off + rank;
+    size_t s = (size_t)blk * MCC_BLOCK + off + rank;
     a.stage_cell[s] = pcell;
     float *st = a.stage + s * 8;
     st[0] = pox;
@@ -286,12 +287,14 @@ __global__ __launch_bounds__(MCC_BLOCK) void k_mcc(MccArgs a) {
   if (threadIdx.x == 0) {
     int tot = 0;
     for (int k = 0; k < MCC_BLOCK / 64; ++k) tot += wcount[k];
-    a.blk_cnt[blockIdx.x] = tot;
+    a.blk_cnt[blk] = tot;
   }
 }
 
+__global__ __launch_bounds__(MCC_BLOCK) void k_mcc(MccArgs a) { mcc_body(a, blockIdx.x); }
+
 /* exclusive scan of the block product counts (one work-group); cnt[nblk] = total */
-__global__ __launch_bounds__(1024) void k_mcc_scan(int *cnt, int nblk) {
+__device__ __forceinline__ void mcc_scan_body(int *cnt, int nblk) {
   __shared__ int sh[1024];
   int per = (nblk + 1023) / 1024;
   int lo = threadIdx.x * per, hi = min(lo + per, nblk);
@@ -314,6 +317,8 @@ __global__ __launch_bounds__(1024) void k_mcc_scan(int *cnt, int nblk) {
   if (threadIdx.x == 1023) cnt[nblk] = sh[1023];
 }
 
+__global__ __launch_bounds__(1024) void k_mcc_scan(int *cnt, int nblk) { mcc_scan_body(cnt, nblk); }
+
 struct AppendArgs {
   const int *blk_off; /* scanned, [nblk] = total */
   const float *stage;
@@ -335,8 +340,7 @@ struct AppendArgs {
   int dim;
 };
 
-__global__ void k_mcc_append(AppendArgs a) {
-  const int b = blockIdx.x;
+__device__ __forceinline__ void mcc_append_body(const AppendArgs &a, const int b) {
   const int k = threadIdx.x;
   const int base = a.blk_off[b];
   const int n = a.blk_off[b + 1] - base;
@@ -372,9 +376,11 @@ __global__ void k_mcc_append(AppendArgs a) {
   }
 }
 
-__global__ void k_mcc_commit(const int *blk_off, int nblk, long long *ehdr,
-                             long long ecap, long long *ihdr, long long icap,
-                             int R) {
+__global__ void k_mcc_append(AppendArgs a) { mcc_append_body(a, blockIdx.x); }
+
+__device__ __forceinline__ void mcc_commit_body(const int *blk_off, int nblk, long long *ehdr,
+                                                long long ecap, long long *ihdr, long long icap,
+                                                int R) {
   long long tot = blk_off[nblk];
   if (tot == 0) return;
   long long ne = ehdr[HDR_NP] + tot;
@@ -393,6 +399,11 @@ __global__ void k_mcc_commit(const int *blk_off, int nblk, long long *ehdr,
     ihdr[HDR_NP] = ni;
     ihdr[HDR_NEXT_ID] += tot * R;
   }
+}
+
+__global__ void k_mcc_commit(const int *blk_off, int nblk, long long *ehdr, long long ecap,
+                             long long *ihdr, long long icap, int R) {
+  mcc_commit_body(blk_off, nblk, ehdr, ecap, ihdr, icap, R);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -456,7 +467,20 @@ int isk_mcc_setup(iskra_ctx *c, int idx) {
   return ISKRA_OK;
 }
 
-int isk_mcc_run(iskra_ctx *c, int idx, double dt, int64_t *events) {
+struct CommitArgs {
+  const int *blk_off;
+  int nblk;
+  long long *ehdr;
+  long long ecap;
+  long long *ihdr;
+  long long icap;
+  int R;
+};
+
+/* arguments of the four launches of one MCC run, the source species reading
+ * buffer b_src and the ion species buffer b_ion; allocates the staging area */
+static int mcc_prepare(iskra_ctx *c, int idx, double dt, int b_src, int b_ion, MccArgs *pa,
+                       AppendArgs *pp, CommitArgs *pc, int *nblk_out) {
   IskMcc &m = c->mcc[idx];
   IskSpecies &s = c->sp[m.src];
   const int N = m.ncoll;
@@ -481,9 +505,8 @@ int isk_mcc_run(iskra_ctx *c, int idx, double dt, int64_t *events) {
       return ISKRA_E_HIP;
     m.stage_cap = cap;
   }
-  if (events) ISK_HIP(hipMemsetAsync(m.events, 0, ISK_MAX_COLL * 8, c->stream));
-  MccArgs a;
-  int b = s.cur;
+  MccArgs &a = *pa;
+  int b = b_src;
   a.cell = s.cell[b];
   a.ox = s.ox[b];
   a.oy = s.oy[b];
@@ -514,6 +537,7 @@ int isk_mcc_run(iskra_ctx *c, int idx, double dt, int64_t *events) {
   a.k0 = (uint32_t)c->seed;
   a.k1 = (uint32_t)(c->seed >> 32);
   a.step = m.step;
+  a.step_dev = nullptr;
   a.stream = m.stream;
   a.nu = m.nu;
   a.events = m.events;
@@ -521,11 +545,7 @@ int isk_mcc_run(iskra_ctx *c, int idx, double dt, int64_t *events) {
   a.blk_cnt = m.blk_cnt;
   a.stage = m.stage;
   a.stage_cell = m.prod_flag;
-  hipLaunchKernelGGL(k_mcc, dim3(nblk), dim3(MCC_BLOCK), 0, c->stream, a);
-  ISK_CHECK_LAUNCH();
-  hipLaunchKernelGGL(k_mcc_scan, dim3(1), dim3(1024), 0, c->stream, m.blk_cnt, nblk);
-  ISK_CHECK_LAUNCH();
-  AppendArgs p;
+  AppendArgs &p = *pp;
   p.blk_off = m.blk_cnt;
   p.stage = m.stage;
   p.stage_cell = m.prod_flag;
@@ -548,7 +568,7 @@ int isk_mcc_run(iskra_ctx *c, int idx, double dt, int64_t *events) {
   if (has_ion && m.ion >= 0 && m.ion != m.src) {
     IskSpecies &is = c->sp[m.ion];
     R = (int)rint(s.w0 / is.w0);
-    int ib = is.cur;
+    int ib = b_ion;
     p.ic = is.cell[ib];
     p.iox = is.ox[ib];
     p.ioy = is.oy[ib];
@@ -558,15 +578,42 @@ int isk_mcc_run(iskra_ctx *c, int idx, double dt, int64_t *events) {
     p.iid = is.track_ids ? is.id[ib] : nullptr;
     p.ihdr = is.hdr;
     p.icap = is.cap;
-    is.deposited = 0;
   }
   p.R = R;
   p.dim = c->g.dim;
+  pc->blk_off = m.blk_cnt;
+  pc->nblk = nblk;
+  pc->ehdr = s.hdr;
+  pc->ecap = (long long)s.cap;
+  pc->ihdr = p.ihdr;
+  pc->icap = (long long)p.icap;
+  pc->R = R;
+  *nblk_out = nblk;
+  return ISKRA_OK;
+}
+
+int isk_mcc_run(iskra_ctx *c, int idx, double dt, int64_t *events) {
+  IskMcc &m = c->mcc[idx];
+  IskSpecies &s = c->sp[m.src];
+  const int N = m.ncoll;
+  MccArgs a;
+  AppendArgs p;
+  CommitArgs cm;
+  int nblk;
+  int b_ion = (m.ion >= 0 && m.ion < c->nspecies) ? c->sp[m.ion].cur : 0;
+  int rc = mcc_prepare(c, idx, dt, s.cur, b_ion, &a, &p, &cm, &nblk);
+  if (rc) return rc;
+  if (events) ISK_HIP(hipMemsetAsync(m.events, 0, ISK_MAX_COLL * 8, c->stream));
+  hipLaunchKernelGGL(k_mcc, dim3(nblk), dim3(MCC_BLOCK), 0, c->stream, a);
+  ISK_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_mcc_scan, dim3(1), dim3(1024), 0, c->stream, m.blk_cnt, nblk);
+  ISK_CHECK_LAUNCH();
   hipLaunchKernelGGL(k_mcc_append, dim3(nblk), dim3(MCC_BLOCK), 0, c->stream, p);
   ISK_CHECK_LAUNCH();
-  hipLaunchKernelGGL(k_mcc_commit, dim3(1), dim3(1), 0, c->stream, m.blk_cnt, nblk,
-                     s.hdr, (long long)s.cap, p.ihdr, (long long)p.icap, R);
+  hipLaunchKernelGGL(k_mcc_commit, dim3(1), dim3(1), 0, c->stream, cm.blk_off, cm.nblk,
+                     cm.ehdr, cm.ecap, cm.ihdr, cm.icap, cm.R);
   ISK_CHECK_LAUNCH();
+  if (p.ihdr) c->sp[m.ion].deposited = 0;
   s.deposited = 0;
   m.step += 1;
   if (events) {
@@ -581,6 +628,79 @@ int isk_mcc_run(iskra_ctx *c, int idx, double dt, int64_t *events) {
       return ISKRA_E_PROB;
     }
   }
+  return ISKRA_OK;
+}
+
+/* ---- batched launches ---- */
+__global__ __launch_bounds__(MCC_BLOCK) void kb_mcc(const MccArgs *A, const int *bstart, int n) {
+  const int i = isk_bfind(bstart, n, blockIdx.x);
+  mcc_body(A[i], blockIdx.x - bstart[i]);
+}
+
+__global__ __launch_bounds__(1024) void kb_mcc_scan(const CommitArgs *A, const int *bstart,
+                                                     int n) {
+  const int i = isk_bfind(bstart, n, blockIdx.x);
+  mcc_scan_body(const_cast<int *>(A[i].blk_off), A[i].nblk);
+}
+
+__global__ __launch_bounds__(MCC_BLOCK) void kb_mcc_append(const AppendArgs *A, const int *bstart,
+                                                            int n) {
+  const int i = isk_bfind(bstart, n, blockIdx.x);
+  mcc_append_body(A[i], blockIdx.x - bstart[i]);
+}
+
+__global__ void kb_mcc_commit(const CommitArgs *A, const int *bstart, int n) {
+  const int i = isk_bfind(bstart, n, blockIdx.x);
+  if (threadIdx.x == 0)
+    mcc_commit_body(A[i].blk_off, A[i].nblk, A[i].ehdr, A[i].ecap, A[i].ihdr, A[i].icap,
+                    A[i].R);
+}
+
+int isk_bbuild_mcc(iskra_ctx **cs, int n, int m, double dt, const int *cur_src,
+                   const int *cur_ion, long long *step_dev, IskBLaunch L[4]) {
+  std::vector<MccArgs> A(n);
+  std::vector<AppendArgs> P(n);
+  std::vector<CommitArgs> CM(n);
+  std::vector<int> blocks(n), one(n, 1);
+  for (int i = 0; i < n; ++i) {
+    int rc = mcc_prepare(cs[i], m, dt, cur_src[i], cur_ion[i], &A[i], &P[i], &CM[i], &blocks[i]);
+    if (rc) return rc;
+    A[i].step_dev = step_dev;
+  }
+  int rc;
+  if ((rc = isk_blaunch_upload(&L[0], ISK_B_MCC, A.data(), sizeof(MccArgs), blocks, MCC_BLOCK,
+                               0)))
+    return rc;
+  if ((rc = isk_blaunch_upload(&L[1], ISK_B_MCC_SCAN, CM.data(), sizeof(CommitArgs), one, 1024,
+                               0)))
+    return rc;
+  if ((rc = isk_blaunch_upload(&L[2], ISK_B_MCC_APPEND, P.data(), sizeof(AppendArgs), blocks,
+                               MCC_BLOCK, 0)))
+    return rc;
+  return isk_blaunch_upload(&L[3], ISK_B_MCC_COMMIT, CM.data(), sizeof(CommitArgs), one, 64, 0);
+}
+
+int isk_blaunch_mcc(const IskBLaunch &L, hipStream_t st) {
+  if (L.total <= 0) return ISKRA_OK;
+  dim3 g((unsigned)L.total), b((unsigned)L.block);
+  switch (L.kind) {
+    case ISK_B_MCC:
+      hipLaunchKernelGGL(kb_mcc, g, b, 0, st, (const MccArgs *)L.args, L.bstart, L.n);
+      break;
+    case ISK_B_MCC_SCAN:
+      hipLaunchKernelGGL(kb_mcc_scan, g, b, 0, st, (const CommitArgs *)L.args, L.bstart, L.n);
+      break;
+    case ISK_B_MCC_APPEND:
+      hipLaunchKernelGGL(kb_mcc_append, g, b, 0, st, (const AppendArgs *)L.args, L.bstart, L.n);
+      break;
+    case ISK_B_MCC_COMMIT:
+      hipLaunchKernelGGL(kb_mcc_commit, g, b, 0, st, (const CommitArgs *)L.args, L.bstart, L.n);
+      break;
+    default:
+      isk_set_error("batched launch: kind %d is not an MCC kernel", L.kind);
+      return ISKRA_E_ARG;
+  }
+  ISK_CHECK_LAUNCH();
   return ISKRA_OK;
 }
 

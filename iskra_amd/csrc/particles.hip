@@ -273,15 +273,14 @@ __device__ __forceinline__ int sort_key(int c, const KeyDesc &k) {
   return (i & (k.T - 1)) + (k.dim == 2 ? ((j & (k.T - 1)) << k.log2T) : 0);
 }
 
-__global__ __launch_bounds__(SORT_BLOCK) void k_sort_hist(const int32_t *cell,
-                                                         const long long *hdr,
-                                                         int *H, KeyDesc kd) {
+__device__ void sort_hist_body(const int32_t *cell, const long long *hdr, int *H,
+                               const KeyDesc &kd, const int blk) {
   extern __shared__ int sh[];
   const int nb1 = kd.nbins + 1;
   for (int b = threadIdx.x; b < nb1; b += SORT_BLOCK) sh[b] = 0;
   __syncthreads();
   const long long np = hdr[HDR_NP];
-  const long long base = (long long)blockIdx.x * SORT_CHUNK;
+  const long long base = (long long)blk * SORT_CHUNK;
   const int lane = threadIdx.x & 63;
   for (int r = 0; r < SORT_PT; ++r) {
     long long p = base + (long long)r * SORT_BLOCK + threadIdx.x;
@@ -297,15 +296,20 @@ __global__ __launch_bounds__(SORT_BLOCK) void k_sort_hist(const int32_t *cell,
     }
   }
   __syncthreads();
-  int *Hb = H + (size_t)blockIdx.x * nb1;
+  int *Hb = H + (size_t)blk * nb1;
   for (int b = threadIdx.x; b < nb1; b += SORT_BLOCK) Hb[b] = sh[b];
+}
+
+__global__ __launch_bounds__(SORT_BLOCK) void k_sort_hist(const int32_t *cell,
+                                                         const long long *hdr,
+                                                         int *H, KeyDesc kd) {
+  sort_hist_body(cell, hdr, H, kd, blockIdx.x);
 }
 
 /* per (bin, segment of SORT_SEG blocks): exclusive prefix over the blocks of
  * the segment, segment totals into S */
-__global__ void k_sort_colscan(int *H, int *S, int nb1, int nblk) {
-  int b = blockIdx.x * blockDim.x + threadIdx.x;
-  int seg = blockIdx.y;
+__device__ void sort_colscan_body(int *H, int *S, int nb1, int nblk, int bx, int seg) {
+  int b = bx * blockDim.x + threadIdx.x;
   if (b >= nb1) return;
   int run = 0;
   int end = min((seg + 1) * SORT_SEG, nblk);
@@ -318,9 +322,13 @@ __global__ void k_sort_colscan(int *H, int *S, int nb1, int nblk) {
   S[(size_t)seg * nb1 + b] = run;
 }
 
+__global__ void k_sort_colscan(int *H, int *S, int nb1, int nblk) {
+  sort_colscan_body(H, S, nb1, nblk, blockIdx.x, blockIdx.y);
+}
+
 /* per bin: exclusive prefix over segments, bin totals */
-__global__ void k_sort_segscan(int *S, int *tot, int nb1, int nseg) {
-  int b = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ void sort_segscan_body(int *S, int *tot, int nb1, int nseg, int bx) {
+  int b = bx * blockDim.x + threadIdx.x;
   if (b >= nb1) return;
   int run = 0;
   for (int s = 0; s < nseg; ++s) {
@@ -332,9 +340,12 @@ __global__ void k_sort_segscan(int *S, int *tot, int nb1, int nseg) {
   tot[b] = run;
 }
 
+__global__ void k_sort_segscan(int *S, int *tot, int nb1, int nseg) {
+  sort_segscan_body(S, tot, nb1, nseg, blockIdx.x);
+}
+
 /* exclusive scan of the bin totals (single work-group of 1024 threads) */
-__global__ __launch_bounds__(1024) void k_sort_binscan(const int *tot, int *start,
-                                                      int nb1) {
+__device__ void sort_binscan_body(const int *tot, int *start, int nb1) {
   __shared__ int sh[1024];
   int per = (nb1 + 1023) / 1024;
   int lo = threadIdx.x * per;
@@ -357,6 +368,11 @@ __global__ __launch_bounds__(1024) void k_sort_binscan(const int *tot, int *star
   if (threadIdx.x == 1023) start[nb1] = sh[1023];
 }
 
+__global__ __launch_bounds__(1024) void k_sort_binscan(const int *tot, int *start,
+                                                      int nb1) {
+  sort_binscan_body(tot, start, nb1);
+}
+
 struct ScatterArgs {
   const int32_t *cell;
   const float *ox, *oy, *vx, *vy, *vz;
@@ -373,13 +389,12 @@ struct ScatterArgs {
 /* stable scatter: particles of a block are ranked in index order; within a
  * round of 256, lanes of a wave are matched by key with ballots and waves are
  * ordered through LDS group lists. */
-__global__ __launch_bounds__(SORT_BLOCK) void k_sort_scatter(ScatterArgs a) {
+__device__ void sort_scatter_body(const ScatterArgs &a, const int blk) {
   extern __shared__ int base[];
   __shared__ int gkey[4][64];
   __shared__ int gcnt[4][64];
   __shared__ int ng[4];
   const int nb1 = a.kd.nbins + 1;
-  const int blk = blockIdx.x;
   const int seg = blk / SORT_SEG;
   for (int b = threadIdx.x; b < nb1; b += SORT_BLOCK)
     base[b] = a.start[b] + a.S[(size_t)seg * nb1 + b] + a.H[(size_t)blk * nb1 + b];
@@ -432,10 +447,13 @@ __global__ __launch_bounds__(SORT_BLOCK) void k_sort_scatter(ScatterArgs a) {
   }
 }
 
+__global__ __launch_bounds__(SORT_BLOCK) void k_sort_scatter(ScatterArgs a) {
+  sort_scatter_body(a, blockIdx.x);
+}
+
 /* new particle count; with build != 0 also the chunk table of the tile sort */
-__global__ __launch_bounds__(1024) void k_sort_finish(
-    long long *hdr, const int *tot, const int *start, int nbins, int build,
-    int CH, IskChunk *chunks, int *tcb) {
+__device__ void sort_finish_body(long long *hdr, const int *tot, const int *start, int nbins,
+                                 int build, int CH, IskChunk *chunks, int *tcb) {
   __shared__ int sh[1024];
   long long live = start[nbins];
   if (!build) {
@@ -481,7 +499,28 @@ __global__ __launch_bounds__(1024) void k_sort_finish(
   }
 }
 
-static int sort_pass(iskra_ctx *c, int sid, int mode, int build) {
+__global__ __launch_bounds__(1024) void k_sort_finish(
+    long long *hdr, const int *tot, const int *start, int nbins, int build,
+    int CH, IskChunk *chunks, int *tcb) {
+  sort_finish_body(hdr, tot, start, nbins, build, CH, chunks, tcb);
+}
+
+/* everything one sort pass of one species needs (single or batched launch) */
+struct SortArgs {
+  KeyDesc kd;
+  const int32_t *cell;
+  const long long *hdr;
+  int *H, *S, *tot, *start;
+  int nb1, nblk, nseg;
+  ScatterArgs sc;
+  long long *hdr_w;
+  int build, CH;
+  IskChunk *chunks;
+  int *tcb;
+};
+
+/* scratch sizing and arguments of a pass reading buffer `b` */
+static int sort_prepare(iskra_ctx *c, int sid, int mode, int build, int b, SortArgs *A) {
   IskSpecies *s = &c->sp[sid];
   KeyDesc kd;
   kd.mode = mode;
@@ -516,21 +555,18 @@ static int sort_pass(iskra_ctx *c, int sid, int mode, int build) {
     if (isk_alloc((void **)&c->sort_start, (int64_t)(ISK_MAX_BINS + 2) * 4)) return ISKRA_E_HIP;
     c->sort_S_words = need_S;
   }
-  int b = s->cur, o = 1 - s->cur;
-  size_t shm = (size_t)nb1 * sizeof(int);
-  hipLaunchKernelGGL(k_sort_hist, dim3(nblk), dim3(SORT_BLOCK), shm, c->stream,
-                     s->cell[b], s->hdr, c->sort_H, kd);
-  ISK_CHECK_LAUNCH();
-  hipLaunchKernelGGL(k_sort_colscan, dim3((nb1 + 255) / 256, nseg), dim3(256), 0,
-                     c->stream, c->sort_H, c->sort_S, nb1, nblk);
-  ISK_CHECK_LAUNCH();
-  hipLaunchKernelGGL(k_sort_segscan, dim3((nb1 + 255) / 256), dim3(256), 0,
-                     c->stream, c->sort_S, c->sort_tot, nb1, nseg);
-  ISK_CHECK_LAUNCH();
-  hipLaunchKernelGGL(k_sort_binscan, dim3(1), dim3(1024), 0, c->stream,
-                     c->sort_tot, c->sort_start, nb1);
-  ISK_CHECK_LAUNCH();
-  ScatterArgs a;
+  int o = 1 - b;
+  A->kd = kd;
+  A->cell = s->cell[b];
+  A->hdr = s->hdr;
+  A->H = c->sort_H;
+  A->S = c->sort_S;
+  A->tot = c->sort_tot;
+  A->start = c->sort_start;
+  A->nb1 = nb1;
+  A->nblk = nblk;
+  A->nseg = nseg;
+  ScatterArgs &a = A->sc;
   a.cell = s->cell[b];
   a.ox = s->ox[b];
   a.oy = s->oy[b];
@@ -551,14 +587,41 @@ static int sort_pass(iskra_ctx *c, int sid, int mode, int build) {
   a.start = c->sort_start;
   a.nblk = nblk;
   a.kd = kd;
+  A->hdr_w = s->hdr;
+  A->build = build;
+  A->CH = c->t.CH;
+  A->chunks = s->chunks;
+  A->tcb = s->tile_chunk_begin;
+  return ISKRA_OK;
+}
+
+static int sort_pass(iskra_ctx *c, int sid, int mode, int build) {
+  IskSpecies *s = &c->sp[sid];
+  SortArgs A;
+  int rc = sort_prepare(c, sid, mode, build, s->cur, &A);
+  if (rc) return rc;
+  const int nb1 = A.nb1, nblk = A.nblk, nseg = A.nseg;
+  size_t shm = (size_t)nb1 * sizeof(int);
+  hipLaunchKernelGGL(k_sort_hist, dim3(nblk), dim3(SORT_BLOCK), shm, c->stream,
+                     A.cell, A.hdr, A.H, A.kd);
+  ISK_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_sort_colscan, dim3((nb1 + 255) / 256, nseg), dim3(256), 0,
+                     c->stream, A.H, A.S, nb1, nblk);
+  ISK_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_sort_segscan, dim3((nb1 + 255) / 256), dim3(256), 0,
+                     c->stream, A.S, A.tot, nb1, nseg);
+  ISK_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_sort_binscan, dim3(1), dim3(1024), 0, c->stream,
+                     (const int *)A.tot, A.start, nb1);
+  ISK_CHECK_LAUNCH();
   hipLaunchKernelGGL(k_sort_scatter, dim3(nblk), dim3(SORT_BLOCK), shm,
-                     c->stream, a);
+                     c->stream, A.sc);
   ISK_CHECK_LAUNCH();
-  hipLaunchKernelGGL(k_sort_finish, dim3(1), dim3(1024), 0, c->stream, s->hdr,
-                     c->sort_tot, c->sort_start, kd.nbins, build, c->t.CH,
-                     s->chunks, s->tile_chunk_begin);
+  hipLaunchKernelGGL(k_sort_finish, dim3(1), dim3(1024), 0, c->stream, A.hdr_w,
+                     (const int *)A.tot, (const int *)A.start, A.kd.nbins, build, A.CH,
+                     A.chunks, A.tcb);
   ISK_CHECK_LAUNCH();
-  s->cur = o;
+  s->cur = 1 - s->cur;
   return ISKRA_OK;
 }
 
@@ -659,7 +722,7 @@ __device__ __forceinline__ void push_load(PushRegs<DIM> &r, const int32_t *__res
 }
 
 template <int DIM>
-__global__ __launch_bounds__(PUSH_BLOCK) void k_push(PushArgs a) {
+__device__ __forceinline__ void push_body(const PushArgs &a, const int blk) {
   extern __shared__ unsigned long long sD[]; /* WW deposit counters, then E */
   const int W = a.W;
   const int WW = DIM == 2 ? W * W : W;
@@ -675,8 +738,8 @@ __global__ __launch_bounds__(PUSH_BLOCK) void k_push(PushArgs a) {
   long long start, count;
   bool windowed;
   int wx0 = 0, wy0 = 0;
-  if ((long long)blockIdx.x < nch) {
-    IskChunk ch = a.chunks[blockIdx.x];
+  if ((long long)blk < nch) {
+    IskChunk ch = a.chunks[blk];
     start = ch.start;
     count = ch.count;
     windowed = true;
@@ -688,9 +751,9 @@ __global__ __launch_bounds__(PUSH_BLOCK) void k_push(PushArgs a) {
       wx0 = ch.tile * a.T - a.M;
     }
   } else {
-    start = nps + ((long long)blockIdx.x - nch) * a.CH;
+    start = nps + ((long long)blk - nch) * a.CH;
     if (start >= np) {
-      if (threadIdx.x == 0) a.live_cnt[blockIdx.x] = 0;
+      if (threadIdx.x == 0) a.live_cnt[blk] = 0;
       return;
     }
     count = min((long long)a.CH, np - start);
@@ -830,21 +893,25 @@ __global__ __launch_bounds__(PUSH_BLOCK) void k_push(PushArgs a) {
   if (threadIdx.x == 0) {
     int t = 0;
     for (int k = 0; k < PUSH_BLOCK / 64; ++k) t += wl[k];
-    a.live_cnt[blockIdx.x] = t;
+    a.live_cnt[blk] = t;
   }
   if (windowed) {
-    long long *dst = a.partial + (size_t)blockIdx.x * WW;
+    long long *dst = a.partial + (size_t)blk * WW;
     for (int k = threadIdx.x; k < WW; k += PUSH_BLOCK) dst[k] = (long long)sD[k];
   }
 }
 
-int isk_push_species(iskra_ctx *c, int sid, double dt, int do_push) {
+template <int DIM>
+__global__ __launch_bounds__(PUSH_BLOCK) void k_push(PushArgs a) {
+  push_body<DIM>(a, blockIdx.x);
+}
+
+/* arguments of one push launch reading buffer `b`; grid and LDS size */
+static void push_prepare(iskra_ctx *c, int sid, double dt, int do_push, int b, PushArgs *pa,
+                         int *grid, size_t *shm) {
   IskSpecies *s = &c->sp[sid];
   const IskGrid &g = c->g;
-  int rc = isk_fill_i64(c, s->gfall, isk_nodes(c), 0);
-  if (rc) return rc;
-  PushArgs a;
-  int b = s->cur;
+  PushArgs &a = *pa;
   a.cell = s->cell[b];
   a.ox = s->ox[b];
   a.oy = s->oy[b];
@@ -872,15 +939,25 @@ int isk_push_species(iskra_ctx *c, int sid, double dt, int do_push) {
   a.bndy = s->bnd[1];
   a.do_push = do_push;
   a.live_cnt = s->live_cnt;
-  int grid = (int)s->max_chunks;
+  *grid = (int)s->max_chunks;
   int WW = g.dim == 2 ? a.W * a.W : a.W;
-  size_t shm = (size_t)WW * (8 + 4 * g.dim);
+  *shm = (size_t)WW * (8 + 4 * g.dim);
+}
+
+int isk_push_species(iskra_ctx *c, int sid, double dt, int do_push) {
+  IskSpecies *s = &c->sp[sid];
+  int rc = isk_fill_i64(c, s->gfall, isk_nodes(c), 0);
+  if (rc) return rc;
+  PushArgs a;
+  int grid;
+  size_t shm;
+  push_prepare(c, sid, dt, do_push, s->cur, &a, &grid, &shm);
   hipEvent_t ev = nullptr;
   if (do_push) {
     int rc2 = isk_prof_begin(c, &ev);
     if (rc2) return rc2;
   }
-  if (g.dim == 2)
+  if (c->g.dim == 2)
     hipLaunchKernelGGL(k_push<2>, dim3(grid), dim3(PUSH_BLOCK), shm, c->stream, a);
   else
     hipLaunchKernelGGL(k_push<1>, dim3(grid), dim3(PUSH_BLOCK), shm, c->stream, a);
@@ -912,13 +989,13 @@ struct MergeArgs {
 };
 
 template <int DIM>
-__global__ __launch_bounds__(256) void k_merge(MergeArgs a) {
-  int node = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void merge_body(const MergeArgs &a, const int blk, const int nblk) {
+  int node = blk * blockDim.x + threadIdx.x;
   int nn = a.nx * a.ny;
   if (a.nlive > 0) {
     __shared__ unsigned long long ls[4];
     unsigned long long sum = 0;
-    for (int k = node; k < a.nlive; k += gridDim.x * blockDim.x) sum += (unsigned long long)a.live_cnt[k];
+    for (int k = node; k < a.nlive; k += nblk * blockDim.x) sum += (unsigned long long)a.live_cnt[k];
     for (int off = 32; off > 0; off >>= 1) sum += __shfl_xor(sum, off);
     if ((threadIdx.x & 63) == 0) ls[threadIdx.x >> 6] = sum;
     __syncthreads();
@@ -952,9 +1029,14 @@ __global__ __launch_bounds__(256) void k_merge(MergeArgs a) {
   a.cnt[node] = s;
 }
 
-int isk_merge_species(iskra_ctx *c, int sid, int count_live) {
+template <int DIM>
+__global__ __launch_bounds__(256) void k_merge(MergeArgs a) {
+  merge_body<DIM>(a, blockIdx.x, gridDim.x);
+}
+
+static void merge_prepare(iskra_ctx *c, int sid, int count_live, MergeArgs *pa, int *grid) {
   IskSpecies *s = &c->sp[sid];
-  MergeArgs a;
+  MergeArgs &a = *pa;
   a.live_cnt = s->live_cnt;
   a.nlive = count_live ? (int)s->max_chunks : 0;
   a.hdr_rw = s->hdr;
@@ -970,11 +1052,17 @@ int isk_merge_species(iskra_ctx *c, int sid, int count_live) {
   a.W = c->t.W;
   a.ntx = c->ntx;
   a.nty = c->nty;
-  int nn = isk_nodes(c);
+  *grid = (isk_nodes(c) + 255) / 256;
+}
+
+int isk_merge_species(iskra_ctx *c, int sid, int count_live) {
+  MergeArgs a;
+  int grid;
+  merge_prepare(c, sid, count_live, &a, &grid);
   if (c->g.dim == 2)
-    hipLaunchKernelGGL(k_merge<2>, dim3((nn + 255) / 256), dim3(256), 0, c->stream, a);
+    hipLaunchKernelGGL(k_merge<2>, dim3(grid), dim3(256), 0, c->stream, a);
   else
-    hipLaunchKernelGGL(k_merge<1>, dim3((nn + 255) / 256), dim3(256), 0, c->stream, a);
+    hipLaunchKernelGGL(k_merge<1>, dim3(grid), dim3(256), 0, c->stream, a);
   ISK_CHECK_LAUNCH();
   return ISKRA_OK;
 }
@@ -992,8 +1080,8 @@ struct RhoArgs {
   int nn;
 };
 
-__global__ void k_rho(RhoArgs a) {
-  int node = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void rho_body(const RhoArgs &a, const int blk) {
+  int node = blk * blockDim.x + threadIdx.x;
   if (node >= a.nn) return;
   double V = a.vol[node];
   double r = 0.0;
@@ -1003,6 +1091,8 @@ __global__ void k_rho(RhoArgs a) {
   }
   a.rho[node] = r + a.bg[node];
 }
+
+__global__ void k_rho(RhoArgs a) { rho_body(a, blockIdx.x); }
 
 int isk_rho(iskra_ctx *c) {
   RhoArgs a;
@@ -1077,6 +1167,202 @@ int isk_gather_species(iskra_ctx *c, int sid) {
     hipLaunchKernelGGL(k_gather<1>, dim3(blocks), dim3(256), 0, c->stream, s->cell[b],
                        s->ox[b], s->oy[b], s->hdr, c->E, c->g.nx, c->gather_out,
                        (long long)s->cap);
+  ISK_CHECK_LAUNCH();
+  return ISKRA_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* batched launches over instances (one work-group = one work-group of one  */
+/* instance's own launch, same device function)                             */
+/* ------------------------------------------------------------------------ */
+struct FillArgs {
+  long long *p;
+  long long n;
+};
+
+__global__ void kb_fill(const FillArgs *A, const int *bstart, int n) {
+  const int i = isk_bfind(bstart, n, blockIdx.x);
+  const long long k = (long long)(blockIdx.x - bstart[i]) * blockDim.x + threadIdx.x;
+  if (k < A[i].n) A[i].p[k] = 0;
+}
+
+__global__ __launch_bounds__(PUSH_BLOCK) void kb_push1(const PushArgs *A, const int *bstart,
+                                                        int n) {
+  const int i = isk_bfind(bstart, n, blockIdx.x);
+  push_body<1>(A[i], blockIdx.x - bstart[i]);
+}
+
+__global__ __launch_bounds__(256) void kb_merge1(const MergeArgs *A, const int *bstart, int n) {
+  const int i = isk_bfind(bstart, n, blockIdx.x);
+  merge_body<1>(A[i], blockIdx.x - bstart[i], bstart[i + 1] - bstart[i]);
+}
+
+__global__ void kb_rho(const RhoArgs *A, const int *bstart, int n) {
+  const int i = isk_bfind(bstart, n, blockIdx.x);
+  rho_body(A[i], blockIdx.x - bstart[i]);
+}
+
+__global__ __launch_bounds__(SORT_BLOCK) void kb_sort_hist(const SortArgs *A, const int *bstart,
+                                                            int n) {
+  const int i = isk_bfind(bstart, n, blockIdx.x);
+  sort_hist_body(A[i].cell, A[i].hdr, A[i].H, A[i].kd, blockIdx.x - bstart[i]);
+}
+
+__global__ void kb_sort_colscan(const SortArgs *A, const int *bstart, int n) {
+  const int i = isk_bfind(bstart, n, blockIdx.x);
+  const int gx = (A[i].nb1 + 255) / 256;
+  const int l = blockIdx.x - bstart[i];
+  sort_colscan_body(A[i].H, A[i].S, A[i].nb1, A[i].nblk, l % gx, l / gx);
+}
+
+__global__ void kb_sort_segscan(const SortArgs *A, const int *bstart, int n) {
+  const int i = isk_bfind(bstart, n, blockIdx.x);
+  sort_segscan_body(A[i].S, A[i].tot, A[i].nb1, A[i].nseg, blockIdx.x - bstart[i]);
+}
+
+__global__ __launch_bounds__(1024) void kb_sort_binscan(const SortArgs *A, const int *bstart,
+                                                         int n) {
+  const int i = isk_bfind(bstart, n, blockIdx.x);
+  sort_binscan_body(A[i].tot, A[i].start, A[i].nb1);
+}
+
+__global__ __launch_bounds__(SORT_BLOCK) void kb_sort_scatter(const SortArgs *A,
+                                                               const int *bstart, int n) {
+  const int i = isk_bfind(bstart, n, blockIdx.x);
+  sort_scatter_body(A[i].sc, blockIdx.x - bstart[i]);
+}
+
+__global__ __launch_bounds__(1024) void kb_sort_finish(const SortArgs *A, const int *bstart,
+                                                        int n) {
+  const int i = isk_bfind(bstart, n, blockIdx.x);
+  sort_finish_body(A[i].hdr_w, A[i].tot, A[i].start, A[i].kd.nbins, A[i].build, A[i].CH,
+                   A[i].chunks, A[i].tcb);
+}
+
+int isk_bbuild_fill(iskra_ctx **cs, int n, int sid, IskBLaunch *L) {
+  std::vector<FillArgs> A(n);
+  std::vector<int> blocks(n);
+  for (int i = 0; i < n; ++i) {
+    A[i].p = (long long *)cs[i]->sp[sid].gfall;
+    A[i].n = isk_nodes(cs[i]);
+    blocks[i] = (int)((A[i].n + 255) / 256);
+  }
+  return isk_blaunch_upload(L, ISK_B_FILL, A.data(), sizeof(FillArgs), blocks, 256, 0);
+}
+
+int isk_bbuild_push(iskra_ctx **cs, int n, int sid, double dt, const int *cur, IskBLaunch *L) {
+  std::vector<PushArgs> A(n);
+  std::vector<int> blocks(n);
+  size_t shm = 0;
+  for (int i = 0; i < n; ++i) {
+    int grid;
+    size_t sh;
+    push_prepare(cs[i], sid, dt, 1, cur[i], &A[i], &grid, &sh);
+    blocks[i] = grid;
+    shm = std::max(shm, sh);
+  }
+  return isk_blaunch_upload(L, ISK_B_PUSH1, A.data(), sizeof(PushArgs), blocks, PUSH_BLOCK, shm);
+}
+
+int isk_bbuild_merge(iskra_ctx **cs, int n, int sid, IskBLaunch *L) {
+  std::vector<MergeArgs> A(n);
+  std::vector<int> blocks(n);
+  for (int i = 0; i < n; ++i) merge_prepare(cs[i], sid, 1, &A[i], &blocks[i]);
+  return isk_blaunch_upload(L, ISK_B_MERGE1, A.data(), sizeof(MergeArgs), blocks, 256, 0);
+}
+
+int isk_bbuild_rho(iskra_ctx **cs, int n, IskBLaunch *L) {
+  std::vector<RhoArgs> A(n);
+  std::vector<int> blocks(n);
+  for (int i = 0; i < n; ++i) {
+    iskra_ctx *c = cs[i];
+    RhoArgs &a = A[i];
+    a.ns = c->nspecies;
+    for (int s = 0; s < c->nspecies; ++s) {
+      a.cnt[s] = c->sp[s].cnt;
+      a.qw[s] = c->sp[s].w0;
+      a.q[s] = c->sp[s].q;
+    }
+    a.vol = c->vol;
+    a.bg = c->bg;
+    a.rho = c->rho;
+    a.nn = isk_nodes(c);
+    blocks[i] = (a.nn + 255) / 256;
+  }
+  return isk_blaunch_upload(L, ISK_B_RHO, A.data(), sizeof(RhoArgs), blocks, 256, 0);
+}
+
+/* the six launches of one tile-sort pass (with chunk table) */
+int isk_bbuild_sort(iskra_ctx **cs, int n, int sid, const int *cur, IskBLaunch L[6]) {
+  std::vector<SortArgs> A(n);
+  std::vector<int> bh(n), bc(n), bs(n), one(n, 1);
+  size_t shm = 0;
+  for (int i = 0; i < n; ++i) {
+    int rc = sort_prepare(cs[i], sid, 0, 1, cur[i], &A[i]);
+    if (rc) return rc;
+    bh[i] = A[i].nblk;
+    bc[i] = ((A[i].nb1 + 255) / 256) * A[i].nseg;
+    bs[i] = (A[i].nb1 + 255) / 256;
+    shm = std::max(shm, (size_t)A[i].nb1 * sizeof(int));
+  }
+  int rc;
+  if ((rc = isk_blaunch_upload(&L[0], ISK_B_SORT_HIST, A.data(), sizeof(SortArgs), bh,
+                               SORT_BLOCK, shm)))
+    return rc;
+  if ((rc = isk_blaunch_upload(&L[1], ISK_B_SORT_COLSCAN, A.data(), sizeof(SortArgs), bc, 256,
+                               0)))
+    return rc;
+  if ((rc = isk_blaunch_upload(&L[2], ISK_B_SORT_SEGSCAN, A.data(), sizeof(SortArgs), bs, 256,
+                               0)))
+    return rc;
+  if ((rc = isk_blaunch_upload(&L[3], ISK_B_SORT_BINSCAN, A.data(), sizeof(SortArgs), one, 1024,
+                               0)))
+    return rc;
+  if ((rc = isk_blaunch_upload(&L[4], ISK_B_SORT_SCATTER, A.data(), sizeof(SortArgs), bh,
+                               SORT_BLOCK, shm)))
+    return rc;
+  return isk_blaunch_upload(&L[5], ISK_B_SORT_FINISH, A.data(), sizeof(SortArgs), one, 1024, 0);
+}
+
+int isk_blaunch_particles(const IskBLaunch &L, hipStream_t st) {
+  if (L.total <= 0) return ISKRA_OK;
+  dim3 g((unsigned)L.total), b((unsigned)L.block);
+  switch (L.kind) {
+    case ISK_B_FILL:
+      hipLaunchKernelGGL(kb_fill, g, b, 0, st, (const FillArgs *)L.args, L.bstart, L.n);
+      break;
+    case ISK_B_PUSH1:
+      hipLaunchKernelGGL(kb_push1, g, b, L.shm, st, (const PushArgs *)L.args, L.bstart, L.n);
+      break;
+    case ISK_B_MERGE1:
+      hipLaunchKernelGGL(kb_merge1, g, b, 0, st, (const MergeArgs *)L.args, L.bstart, L.n);
+      break;
+    case ISK_B_RHO:
+      hipLaunchKernelGGL(kb_rho, g, b, 0, st, (const RhoArgs *)L.args, L.bstart, L.n);
+      break;
+    case ISK_B_SORT_HIST:
+      hipLaunchKernelGGL(kb_sort_hist, g, b, L.shm, st, (const SortArgs *)L.args, L.bstart, L.n);
+      break;
+    case ISK_B_SORT_COLSCAN:
+      hipLaunchKernelGGL(kb_sort_colscan, g, b, 0, st, (const SortArgs *)L.args, L.bstart, L.n);
+      break;
+    case ISK_B_SORT_SEGSCAN:
+      hipLaunchKernelGGL(kb_sort_segscan, g, b, 0, st, (const SortArgs *)L.args, L.bstart, L.n);
+      break;
+    case ISK_B_SORT_BINSCAN:
+      hipLaunchKernelGGL(kb_sort_binscan, g, b, 0, st, (const SortArgs *)L.args, L.bstart, L.n);
+      break;
+    case ISK_B_SORT_SCATTER:
+      hipLaunchKernelGGL(kb_sort_scatter, g, b, L.shm, st, (const SortArgs *)L.args, L.bstart,
+                         L.n);
+      break;
+    case ISK_B_SORT_FINISH:
+      hipLaunchKernelGGL(kb_sort_finish, g, b, 0, st, (const SortArgs *)L.args, L.bstart, L.n);
+      break;
+    default:
+      isk_set_error("batched launch: kind %d is not a particle kernel", L.kind);
+      return ISKRA_E_ARG;
+  }
   ISK_CHECK_LAUNCH();
   return ISKRA_OK;
 }

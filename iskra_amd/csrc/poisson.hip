@@ -163,8 +163,7 @@ struct TriArgs {
 };
 
 /* one thread: rhs, forward elimination, back substitution (FP64, explicit fma) */
-__global__ void k_tri_solve(TriArgs t) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+__device__ __forceinline__ void tri_solve_body(const TriArgs &t) {
   const int n = t.n;
   if (!t.singular) {
     double yprev = 0.0;
@@ -205,6 +204,11 @@ __global__ void k_tri_solve(TriArgs t) {
   for (int i = 0; i < n; ++i) s2 += t.phi[i];
   double m2 = s2 / (double)n;
   for (int i = 0; i < n; ++i) t.phi[i] = t.phi[i] - m2;
+}
+
+__global__ void k_tri_solve(TriArgs t) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  tri_solve_body(t);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -815,9 +819,16 @@ __global__ void k_mg_toz(const float *e0, float *z, int nn, const int *flag) {
 /* ------------------------------------------------------------------------ */
 /* E = -grad(phi)                                                            */
 /* ------------------------------------------------------------------------ */
-__global__ void k_efield(const double *phi, float *E, int dim, int nx, int ny,
-                         double dx, double dy) {
-  int n = blockIdx.x * blockDim.x + threadIdx.x;
+struct EfieldArgs {
+  const double *phi;
+  float *E;
+  int dim, nx, ny;
+  double dx, dy;
+};
+
+__device__ __forceinline__ void efield_body(const double *phi, float *E, int dim, int nx, int ny,
+                                            double dx, double dy, const int blk) {
+  int n = blk * blockDim.x + threadIdx.x;
   if (n >= nx * ny) return;
   int i = n % nx, j = n / nx;
   double ex;
@@ -836,6 +847,11 @@ __global__ void k_efield(const double *phi, float *E, int dim, int nx, int ny,
   else ey = (phi[n - nx] - phi[n + nx]) / (2.0 * dy);
   E[2 * n] = (float)ex;
   E[2 * n + 1] = (float)ey;
+}
+
+__global__ void k_efield(const double *phi, float *E, int dim, int nx, int ny, double dx,
+                         double dy) {
+  efield_body(phi, E, dim, nx, ny, dx, dy, blockIdx.x);
 }
 
 int isk_efield(iskra_ctx *c) {
@@ -1126,22 +1142,28 @@ static int mg_vcycle(iskra_ctx *c, int mode_in0, int mode_out0, int first) {
   return ISKRA_OK;
 }
 
+static void tri_prepare(iskra_ctx *c, TriArgs *pt) {
+  IskPoisson &p = c->ps;
+  TriArgs &t = *pt;
+  t.rho = c->rho;
+  t.tags = p.d_tags;
+  t.tagval = p.d_tag_val;
+  t.a = p.d_tri_a;
+  t.cp = p.d_tri_cp;
+  t.inv = p.d_tri_inv;
+  t.y = p.d_work;
+  t.phi = c->phi;
+  t.n = c->g.nx;
+  t.scale = (c->g.dx * c->g.dx) / p.eps0;
+  t.singular = p.singular;
+}
+
 int isk_poisson_solve(iskra_ctx *c, int warm, int *iters, double *relres) {
   IskPoisson &p = c->ps;
   const int nn = isk_nodes(c);
   if (p.kind == ISKRA_POISSON_DIRECT) {
     TriArgs t;
-    t.rho = c->rho;
-    t.tags = p.d_tags;
-    t.tagval = p.d_tag_val;
-    t.a = p.d_tri_a;
-    t.cp = p.d_tri_cp;
-    t.inv = p.d_tri_inv;
-    t.y = p.d_work;
-    t.phi = c->phi;
-    t.n = c->g.nx;
-    t.scale = (c->g.dx * c->g.dx) / p.eps0;
-    t.singular = p.singular;
+    tri_prepare(c, &t);
     hipLaunchKernelGGL(k_tri_solve, dim3(1), dim3(64), 0, c->stream, t);
     ISK_CHECK_LAUNCH();
     p.last_iters = 1;
@@ -1251,5 +1273,65 @@ int isk_poisson_solve(iskra_ctx *c, int warm, int *iters, double *relres) {
                   scal[S_RELRES], hflag[1], p.rtol);
     return ISKRA_E_CONVERGE;
   }
+  return ISKRA_OK;
+}
+
+/* ---- batched launches ---- */
+__global__ void kb_tri(const TriArgs *A, const int *bstart, int n) {
+  const int i = isk_bfind(bstart, n, blockIdx.x);
+  if (threadIdx.x == 0) tri_solve_body(A[i]);
+}
+
+__global__ void kb_efield(const EfieldArgs *A, const int *bstart, int n) {
+  const int i = isk_bfind(bstart, n, blockIdx.x);
+  const EfieldArgs &a = A[i];
+  efield_body(a.phi, a.E, a.dim, a.nx, a.ny, a.dx, a.dy, blockIdx.x - bstart[i]);
+}
+
+int isk_bbuild_tri(iskra_ctx **cs, int n, IskBLaunch *L) {
+  std::vector<TriArgs> A(n);
+  std::vector<int> one(n, 1);
+  for (int i = 0; i < n; ++i) {
+    if (cs[i]->ps.kind != ISKRA_POISSON_DIRECT) {
+      isk_set_error("batch: instance %d does not use the direct 1-D Poisson solve", i);
+      return ISKRA_E_UNSUPPORTED;
+    }
+    tri_prepare(cs[i], &A[i]);
+  }
+  return isk_blaunch_upload(L, ISK_B_TRI, A.data(), sizeof(TriArgs), one, 64, 0);
+}
+
+int isk_bbuild_efield(iskra_ctx **cs, int n, IskBLaunch *L) {
+  std::vector<EfieldArgs> A(n);
+  std::vector<int> blocks(n);
+  for (int i = 0; i < n; ++i) {
+    iskra_ctx *c = cs[i];
+    A[i].phi = c->phi;
+    A[i].E = c->E;
+    A[i].dim = c->g.dim;
+    A[i].nx = c->g.nx;
+    A[i].ny = c->g.ny;
+    A[i].dx = c->g.dx;
+    A[i].dy = c->g.dy;
+    blocks[i] = (isk_nodes(c) + 255) / 256;
+  }
+  return isk_blaunch_upload(L, ISK_B_EFIELD, A.data(), sizeof(EfieldArgs), blocks, 256, 0);
+}
+
+int isk_blaunch_poisson(const IskBLaunch &L, hipStream_t st) {
+  if (L.total <= 0) return ISKRA_OK;
+  dim3 g((unsigned)L.total), b((unsigned)L.block);
+  switch (L.kind) {
+    case ISK_B_TRI:
+      hipLaunchKernelGGL(kb_tri, g, b, 0, st, (const TriArgs *)L.args, L.bstart, L.n);
+      break;
+    case ISK_B_EFIELD:
+      hipLaunchKernelGGL(kb_efield, g, b, 0, st, (const EfieldArgs *)L.args, L.bstart, L.n);
+      break;
+    default:
+      isk_set_error("batched launch: kind %d is not a Poisson kernel", L.kind);
+      return ISKRA_E_ARG;
+  }
+  ISK_CHECK_LAUNCH();
   return ISKRA_OK;
 }
