@@ -5,10 +5,16 @@ Workload (the 2-D configuration the headline metric is quoted on): a 2-D plasma
 slab, 256 x 256 cells (257^2 nodes), Dirichlet phi = 0 walls, 2^24 electrons +
 2^24 ions (m_i = 1836 m_e), Te = 2 eV / Ti = 0.03 eV Maxwellians, absorbing
 walls, Debye length = cell size, omega_pe dt = 0.1.  One step = gather + push +
-boundary + CIC deposit of both species, deposit merge, multigrid Poisson solve
-to relative residual 1e-8 (FP64 residual), E-field, and the counting sort every
-`sort_every` steps.  Inputs are generated on the device (Philox loader), so
-they are resident in HBM before the timed region.
+boundary + CIC deposit of both species, deposit merge, Poisson solve to
+relative residual 1e-8 (FP64 residual; by default the direct sine-transform
+solve, multigrid and PCG timed beside it on the same charge density), E-field,
+and the counting sort every `sort_every` steps.  Inputs are generated on the
+device (Philox loader), so they are resident in HBM before the timed region.
+
+After the timed step, a reduced learned-warm-start leg (513^2 nodes: capture
+two-stream snapshots, train the encoder-decoder on the device, compare PCG
+and multigrid iterations from zero, previous and learned starts) reports the
+"learned vs plain" part of the metric; --no-warmstart skips it.
 
 Prints one JSON line (see the keys below).  --gpus above 1 exits: this build
 runs on one GPU.
@@ -94,13 +100,16 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--cells", type=int, default=256)
     ap.add_argument("--ppc", type=int, default=256)
-    ap.add_argument("--solver", default="mg", choices=["mg", "mgpcg", "pcg"])
+    ap.add_argument("--solver", default="spectral", choices=["spectral", "mg", "mgpcg", "pcg"])
     ap.add_argument("--tile", type=int, default=0)
     ap.add_argument("--chunk", type=int, default=0)
     ap.add_argument("--margin", type=int, default=0)
     ap.add_argument("--sort-every", type=int, default=-1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
+    ap.add_argument("--no-warmstart", action="store_true")
+    ap.add_argument("--ws-runs", type=int, default=4)
+    ap.add_argument("--ws-train-steps", type=int, default=800)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -119,7 +128,9 @@ def main():
                sort_every=args.sort_every)
     tags = np.zeros((nodes, nodes), np.int8)
     tags[0, :] = tags[-1, :] = tags[:, 0] = tags[:, -1] = 1
-    kind = dict(mg=L.POISSON_MG, mgpcg=L.POISSON_MGPCG, pcg=L.POISSON_PCG)[args.solver]
+    kinds = dict(spectral=L.POISSON_SPECTRAL, mg=L.POISSON_MG, mgpcg=L.POISSON_MGPCG,
+                 pcg=L.POISSON_PCG)
+    kind = kinds[args.solver]
     ctx.poisson(kind, p["eps0"], tags.ravel(), rtol=1e-8)
     ctx.tag_value(1, 0.0)
     cap = n_species + 1024
@@ -166,7 +177,7 @@ def main():
         "config": {
             "workload": f"2-D ES-PIC slab {args.cells}x{args.cells} cells, "
                         f"{n_species} e- + {n_species} ions, Dirichlet walls, "
-                        f"{args.solver.upper()} Poisson to 1e-8",
+                        f"{args.solver} Poisson to 1e-8",
             "cells": args.cells,
             "particles_per_species": n_species,
             "poisson": args.solver,
@@ -212,7 +223,44 @@ def main():
         out["hbm_copy_gbps"] = 2 * nbytes / (cms * 1e-3) / 1e9
         out["hbm_copy_plain_gbps"] = 2 * nbytes / (pms * 1e-3) / 1e9
         out["roofline"]["frac_of_copy"] = achieved / out["hbm_copy_gbps"]
+        # the other solvers on this step's charge density (zero and previous-phi starts)
+        phi_prev = ctx.field(L.F_PHI)
+        ctx.step(dt, 1)
+        rho = ctx.field(L.F_RHO)
+        alt = {}
+        for name in ("spectral", "mg", "pcg"):
+            ctx.poisson(kinds[name], p["eps0"], tags.ravel(), rtol=1e-8, max_iter=100000)
+            ctx.tag_value(1, 0.0)
+            ctx.set_field(L.F_RHO, rho)
+            ctx.solve(L.WARM_ZERO)  # first use
+            row = {}
+            for start in ("zero", "prev"):
+                if start == "prev":
+                    ctx.set_field(L.F_PHI, phi_prev)
+                ctx.sync()
+                t0 = time.perf_counter()
+                it, rr = ctx.solve(L.WARM_ZERO if start == "zero" else L.WARM_PREV)
+                ctx.sync()
+                row[start] = {"iters": it, "ms": 1e3 * (time.perf_counter() - t0), "relres": rr}
+            alt[name] = row
+        out["poisson"]["solvers_same_rho"] = alt
 
+    ctx.close()
+    if not args.no_warmstart:
+        from iskra_amd import warmstart as W
+        ws = W.experiment(side=512, levels=4, base=32, runs=args.ws_runs, steps_per_run=250,
+                          train_steps=args.ws_train_steps, held_steps=41, every=10, log=None)
+        out["learned_warm_start"] = {
+            "grid": "513x513 nodes, phi = 0 walls, 2-D two-stream snapshots",
+            "network": f"{ws['levels']}-level conv encoder-decoder, {ws['base']}-"
+                       f"{ws['base'] << (ws['levels'] - 1)} channels, {ws['params']} parameters",
+            "snapshots": ws["snapshots"], "train_steps": ws["train_steps"],
+            "train_ms_per_step": ws["train_ms_per_step"], "train_tflops": ws["train_tflops"],
+            "inference_ms": ws["inference_ms"], "heldout_rel_mse": ws["heldout_mse_after"],
+            "solves": ws["solves"],
+            "pcg_iters_plain_over_learned": ws["solves"]["pcg"]["zero"]["iters"] /
+            ws["solves"]["pcg"]["learned"]["iters"],
+        }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cells)
     print(json.dumps(out), flush=True)

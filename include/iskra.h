@@ -164,13 +164,17 @@ int iskra_field_upload(iskra_ctx *ctx, int which, int index, const double *in);
  * first and last node of that axis as the reference's ring.
  * kind: DIRECT (1-D tridiagonal), PCG (Jacobi-preconditioned CG), MG
  * (geometric multigrid V-cycles, 2-D with Dirichlet walls), MGPCG (CG
- * preconditioned by one V-cycle).  eps0 in F/m; rtol on ||b-A phi||2/||b||2
+ * preconditioned by one V-cycle), SPECTRAL (2-D direct solve by the discrete
+ * sine transform, which diagonalises the operator when all four walls are
+ * Dirichlet and no interior node is: FP64 dense transforms, then the FP64
+ * residual is checked and corrected until it meets rtol).  eps0 in F/m; rtol on ||b-A phi||2/||b||2
  * evaluated in FP64. */
 enum {
   ISKRA_POISSON_DIRECT = 0,
   ISKRA_POISSON_PCG = 1,
   ISKRA_POISSON_MG = 2,
-  ISKRA_POISSON_MGPCG = 3
+  ISKRA_POISSON_MGPCG = 3,
+  ISKRA_POISSON_SPECTRAL = 4
 };
 int iskra_poisson_set(iskra_ctx *ctx, int kind, double eps0, const int8_t *tags,
                       int periodic_x, int periodic_y, double rtol,

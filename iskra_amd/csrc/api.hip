@@ -102,6 +102,7 @@ void isk_species_release(IskSpecies *s) {
   isk_free(s->chunks);
   isk_free(s->tile_chunk_begin);
   isk_free(s->partial);
+  isk_free(s->tilewin);
   isk_free(s->live_cnt);
   isk_free(s->gfall);
   isk_free(s->cnt);
@@ -109,6 +110,7 @@ void isk_species_release(IskSpecies *s) {
   s->chunks = nullptr;
   s->tile_chunk_begin = nullptr;
   s->partial = nullptr;
+  s->tilewin = nullptr;
   s->live_cnt = nullptr;
   s->gfall = nullptr;
   s->cnt = nullptr;
@@ -118,16 +120,19 @@ static int alloc_chunk_tables(iskra_ctx *c, IskSpecies *s) {
   isk_free(s->chunks);
   isk_free(s->tile_chunk_begin);
   isk_free(s->partial);
+  isk_free(s->tilewin);
   isk_free(s->live_cnt);
   s->chunks = nullptr;
   s->tile_chunk_begin = nullptr;
   s->partial = nullptr;
+  s->tilewin = nullptr;
   s->live_cnt = nullptr;
   int WW = c->g.dim == 2 ? c->t.W * c->t.W : c->t.W;
   s->max_chunks = (s->cap + c->t.CH - 1) / c->t.CH + c->ntiles + 1;
   if (isk_alloc((void **)&s->chunks, (size_t)s->max_chunks * sizeof(IskChunk)) ||
       isk_alloc((void **)&s->tile_chunk_begin, (size_t)(c->ntiles + 1) * 4) ||
       isk_alloc((void **)&s->partial, (size_t)s->max_chunks * WW * 8) ||
+      isk_alloc((void **)&s->tilewin, (size_t)c->ntiles * WW * 8) ||
       isk_alloc((void **)&s->live_cnt, (size_t)s->max_chunks * 4))
     return ISKRA_E_HIP;
   return ISKRA_OK;
@@ -884,7 +889,7 @@ int iskra_poisson_set(iskra_ctx *c, int kind, double eps0, const int8_t *tags, i
                       int periodic_y, double rtol, int max_iter) {
   CHECK_CTX(c);
   CHECK_GRID(c);
-  if (kind < ISKRA_POISSON_DIRECT || kind > ISKRA_POISSON_MGPCG || !(eps0 > 0)) {
+  if (kind < ISKRA_POISSON_DIRECT || kind > ISKRA_POISSON_SPECTRAL || !(eps0 > 0)) {
     isk_set_error("poisson_set: bad kind %d or eps0 %g", kind, eps0);
     return ISKRA_E_ARG;
   }

@@ -83,6 +83,7 @@ struct IskSpecies {
   IskChunk *chunks;          /* max_chunks */
   int *tile_chunk_begin;     /* ntiles + 1 */
   long long *partial;        /* max_chunks * W^dim */
+  long long *tilewin;        /* ntiles * W^dim: chunk windows summed per tile (2-D) */
   unsigned long long *gfall; /* nodes */
   long long *cnt;            /* nodes: merged deposit (fixed point) */
   int *live_cnt;             /* max_chunks: live particles per push work-group */
@@ -147,6 +148,11 @@ struct IskPoisson {
   int ln[16];                  /* nodes per axis on each level */
   float *me[16], *mf[16];      /* correction and rhs per level (FP32) */
   int coarse_lev = 0;          /* first level solved by the single-WG kernel */
+  /* spectral (DST-I) direct solve: interior sizes, sine bases, 1/eigenvalue,
+   * right-hand side and two work matrices (row-major, rows = j) */
+  int sp_nx = 0, sp_ny = 0;
+  double *sp_Sx = nullptr, *sp_Sy = nullptr, *sp_Li = nullptr, *sp_F = nullptr,
+         *sp_T1 = nullptr, *sp_T2 = nullptr;
   int last_iters = 0;
   double last_relres = 0.0;
 };

@@ -977,6 +977,7 @@ int isk_push_species(iskra_ctx *c, int sid, double dt, int do_push) {
 /* deterministic merge of the chunk windows                                  */
 /* ------------------------------------------------------------------------ */
 struct MergeArgs {
+  const long long *tilewin; /* 2-D: per-tile sums of the chunk windows (null: per chunk) */
   const int *live_cnt;
   int nlive; /* entries of live_cnt to add to HDR_PUSHED (0: deposit only) */
   long long *hdr_rw;
@@ -1022,7 +1023,9 @@ __device__ __forceinline__ void merge_body(const MergeArgs &a, const int blk, co
         if (li < 0 || li >= a.W) continue;
         int t = ti + a.ntx * tj;
         int q = DIM == 2 ? lj * a.W + li : li;
-        for (int k = a.tcb[t]; k < a.tcb[t + 1]; ++k) s += a.partial[(size_t)k * WW + q];
+        if (a.tilewin) s += a.tilewin[(size_t)t * WW + q];
+        else
+          for (int k = a.tcb[t]; k < a.tcb[t + 1]; ++k) s += a.partial[(size_t)k * WW + q];
       }
     }
   }
@@ -1032,6 +1035,21 @@ __device__ __forceinline__ void merge_body(const MergeArgs &a, const int blk, co
 template <int DIM>
 __global__ __launch_bounds__(256) void k_merge(MergeArgs a) {
   merge_body<DIM>(a, blockIdx.x, gridDim.x);
+}
+
+/* stage 1 of the 2-D merge: one work-group per tile sums its chunks' windows
+ * (contiguous in memory, chunks of a tile are consecutive) */
+__global__ __launch_bounds__(256) void k_merge_tiles(const long long *__restrict__ partial,
+                                                     const int *__restrict__ tcb,
+                                                     const long long *__restrict__ hdr,
+                                                     long long *__restrict__ tilewin, int WW) {
+  const int t = blockIdx.x;
+  const int k0 = hdr[HDR_NCHUNKS] > 0 ? tcb[t] : 0, k1 = hdr[HDR_NCHUNKS] > 0 ? tcb[t + 1] : 0;
+  for (int q = threadIdx.x; q < WW; q += blockDim.x) {
+    long long s = 0;
+    for (int k = k0; k < k1; ++k) s += partial[(size_t)k * WW + q];
+    tilewin[(size_t)t * WW + q] = s;
+  }
 }
 
 static void merge_prepare(iskra_ctx *c, int sid, int count_live, MergeArgs *pa, int *grid) {
@@ -1052,6 +1070,7 @@ static void merge_prepare(iskra_ctx *c, int sid, int count_live, MergeArgs *pa, 
   a.W = c->t.W;
   a.ntx = c->ntx;
   a.nty = c->nty;
+  a.tilewin = nullptr;
   *grid = (isk_nodes(c) + 255) / 256;
 }
 
@@ -1059,8 +1078,15 @@ int isk_merge_species(iskra_ctx *c, int sid, int count_live) {
   MergeArgs a;
   int grid;
   merge_prepare(c, sid, count_live, &a, &grid);
-  if (c->g.dim == 2)
+  if (c->g.dim == 2) {
+    IskSpecies *s = &c->sp[sid];
+    hipLaunchKernelGGL(k_merge_tiles, dim3(c->ntiles), dim3(256), 0, c->stream,
+                       (const long long *)s->partial, (const int *)s->tile_chunk_begin,
+                       (const long long *)s->hdr, s->tilewin, c->t.W * c->t.W);
+    ISK_CHECK_LAUNCH();
+    a.tilewin = s->tilewin;
     hipLaunchKernelGGL(k_merge<2>, dim3(grid), dim3(256), 0, c->stream, a);
+  }
   else
     hipLaunchKernelGGL(k_merge<1>, dim3(grid), dim3(256), 0, c->stream, a);
   ISK_CHECK_LAUNCH();

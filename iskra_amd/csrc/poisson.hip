@@ -418,6 +418,87 @@ __global__ void k_pcg_out(PcgArgs a, double *phi, double mean_sub) {
 }
 
 /* ------------------------------------------------------------------------ */
+/* spectral direct solve: phi = Sy ((Sy F Sx) / lambda) Sx                   */
+/* ------------------------------------------------------------------------ */
+/* F (interior, row-major [j-1][i-1]) = f_elim, or the residual r (correction) */
+__global__ void k_sp_pack(PcgArgs a, double *F, int mx, int my, int from_r) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= mx * my) return;
+  int i = e % mx + 1, j = e / mx + 1;
+  int n = j * a.s.nx + i;
+  F[e] = from_r ? a.r[n] : a.rho[n] * a.scale + dirichlet_rhs(a.s, i, j);
+}
+
+struct SpGemm {
+  const double *A, *B, *L; /* L: elementwise factor (mode 1) */
+  double *C;               /* modes 0, 1 */
+  double *x;               /* modes 2, 3: node array, interior (i, j) = (col+1, row+1) */
+  int M, N, K, nx, mode;
+};
+
+/* C = A B (row-major FP64), 32 x 32 outputs per work-group, 2 x 2 per thread,
+ * K in slabs of 32 through LDS, fixed fma order (deterministic) */
+__global__ __launch_bounds__(256) void k_sp_gemm(SpGemm g) {
+  __shared__ double sA[32][33], sB[32][33];
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  const int row0 = blockIdx.y * 32, col0 = blockIdx.x * 32;
+  double c00 = 0.0, c01 = 0.0, c10 = 0.0, c11 = 0.0;
+  for (int k0 = 0; k0 < g.K; k0 += 32) {
+    for (int e = threadIdx.x; e < 1024; e += 256) {
+      const int r = e >> 5, q = e & 31;
+      const int ar = row0 + r, ak = k0 + q;
+      sA[r][q] = (ar < g.M && ak < g.K) ? g.A[(size_t)ar * g.K + ak] : 0.0;
+      const int bk = k0 + r, bc = col0 + q;
+      sB[r][q] = (bk < g.K && bc < g.N) ? g.B[(size_t)bk * g.N + bc] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll 8
+    for (int kk = 0; kk < 32; ++kk) {
+      const double a0 = sA[2 * ty][kk], a1 = sA[2 * ty + 1][kk];
+      const double b0 = sB[kk][2 * tx], b1 = sB[kk][2 * tx + 1];
+      c00 = __builtin_fma(a0, b0, c00);
+      c01 = __builtin_fma(a0, b1, c01);
+      c10 = __builtin_fma(a1, b0, c10);
+      c11 = __builtin_fma(a1, b1, c11);
+    }
+    __syncthreads();
+  }
+  const double cv[2][2] = {{c00, c01}, {c10, c11}};
+  for (int r = 0; r < 2; ++r)
+    for (int q = 0; q < 2; ++q) {
+      const int row = row0 + 2 * ty + r, col = col0 + 2 * tx + q;
+      if (row >= g.M || col >= g.N) continue;
+      const double v = cv[r][q];
+      const size_t e = (size_t)row * g.N + col;
+      if (g.mode == 0) g.C[e] = v;
+      else if (g.mode == 1) g.C[e] = v * g.L[e];
+      else {
+        const size_t n = (size_t)(row + 1) * g.nx + (col + 1);
+        if (g.mode == 2) g.x[n] = v;
+        else g.x[n] += v;
+      }
+    }
+}
+
+static int sp_gemm(iskra_ctx *c, const double *A, const double *B, const double *L, double *C,
+                   double *x, int M, int N, int K, int mode) {
+  SpGemm g;
+  g.A = A;
+  g.B = B;
+  g.L = L;
+  g.C = C;
+  g.x = x;
+  g.M = M;
+  g.N = N;
+  g.K = K;
+  g.nx = c->g.nx;
+  g.mode = mode;
+  hipLaunchKernelGGL(k_sp_gemm, dim3((N + 31) / 32, (M + 31) / 32), dim3(256), 0, c->stream, g);
+  ISK_CHECK_LAUNCH();
+  return ISKRA_OK;
+}
+
+/* ------------------------------------------------------------------------ */
 /* multigrid                                                                 */
 /* ------------------------------------------------------------------------ */
 struct MgLevel {
@@ -869,8 +950,11 @@ int isk_poisson_free(iskra_ctx *c) {
   IskPoisson &p = c->ps;
   void *ptrs[] = {p.d_tags, p.d_tag_val, p.d_tri_a, p.d_tri_cp, p.d_tri_inv,
                   p.d_work, p.x, p.r, p.Ap, p.z, p.p[0], p.p[1], p.scal,
-                  p.partials, p.counter, p.iflag};
+                  p.partials, p.counter, p.iflag, p.sp_Sx, p.sp_Sy, p.sp_Li,
+                  p.sp_F, p.sp_T1, p.sp_T2};
   for (void *q : ptrs) isk_free(q);
+  p.sp_Sx = p.sp_Sy = p.sp_Li = p.sp_F = p.sp_T1 = p.sp_T2 = nullptr;
+  p.sp_nx = p.sp_ny = 0;
   for (int l = 0; l < p.nlev; ++l) {
     isk_free(p.me[l]);
     isk_free(p.mf[l]);
@@ -995,6 +1079,58 @@ int isk_poisson_setup(iskra_ctx *c) {
   ISK_HIP(hipMemset(p.counter, 0, 64));
   ISK_HIP(hipMemset(p.iflag, 0, 64));
   p.nlev = 0;
+  if (p.kind == ISKRA_POISSON_SPECTRAL) {
+    if (c->g.dim != 2 || p.per_x || p.per_y || interior_tag) {
+      isk_set_error("poisson: the spectral solve needs a 2-D grid with Dirichlet walls only");
+      return ISKRA_E_UNSUPPORTED;
+    }
+    for (int k = 0; k < nn; ++k) {
+      int i = k % c->g.nx, j = k / c->g.nx;
+      bool edge = i == 0 || i == c->g.nx - 1 || j == 0 || j == c->g.ny - 1;
+      if (edge && !p.tags[k]) {
+        isk_set_error("poisson: the spectral solve needs Dirichlet values on all four walls");
+        return ISKRA_E_UNSUPPORTED;
+      }
+    }
+    const int mx = c->g.nx - 2, my = c->g.ny - 2;
+    if (mx < 1 || my < 1) {
+      isk_set_error("poisson: the spectral solve needs interior nodes");
+      return ISKRA_E_ARG;
+    }
+    /* S[a][b] = sqrt(2/(m+1)) sin(pi (a+1)(b+1)/(m+1)): symmetric, S S = I;
+     * eigenvalues of the unit 5-point operator: (2-2cos pi p/(mx+1)) + (2-2cos pi q/(my+1)) */
+    auto basis = [](int m, std::vector<double> &S) {
+      S.assign((size_t)m * m, 0.0);
+      const double c0 = sqrt(2.0 / (m + 1)), w = M_PI / (m + 1);
+      for (int a = 0; a < m; ++a)
+        for (int b = 0; b < m; ++b) {
+          /* reduce (a+1)(b+1) modulo 2(m+1) so the sine argument stays small */
+          long long k = (long long)(a + 1) * (b + 1) % (2ll * (m + 1));
+          S[(size_t)a * m + b] = c0 * sin(w * (double)k);
+        }
+    };
+    std::vector<double> Sx, Sy, Li((size_t)mx * my);
+    basis(mx, Sx);
+    basis(my, Sy);
+    for (int q = 0; q < my; ++q)
+      for (int pp = 0; pp < mx; ++pp) {
+        double lx = 2.0 - 2.0 * cos(M_PI * (pp + 1) / (mx + 1));
+        double ly = 2.0 - 2.0 * cos(M_PI * (q + 1) / (my + 1));
+        Li[(size_t)q * mx + pp] = 1.0 / (lx + ly);
+      }
+    size_t m2 = (size_t)mx * my;
+    if (isk_alloc((void **)&p.sp_Sx, (size_t)mx * mx * 8) ||
+        isk_alloc((void **)&p.sp_Sy, (size_t)my * my * 8) ||
+        isk_alloc((void **)&p.sp_Li, m2 * 8) || isk_alloc((void **)&p.sp_F, m2 * 8) ||
+        isk_alloc((void **)&p.sp_T1, m2 * 8) || isk_alloc((void **)&p.sp_T2, m2 * 8))
+      return ISKRA_E_HIP;
+    ISK_HIP(hipMemcpy(p.sp_Sx, Sx.data(), (size_t)mx * mx * 8, hipMemcpyHostToDevice));
+    ISK_HIP(hipMemcpy(p.sp_Sy, Sy.data(), (size_t)my * my * 8, hipMemcpyHostToDevice));
+    ISK_HIP(hipMemcpy(p.sp_Li, Li.data(), m2 * 8, hipMemcpyHostToDevice));
+    p.sp_nx = mx;
+    p.sp_ny = my;
+    return ISKRA_OK;
+  }
   if (p.kind == ISKRA_POISSON_MG || p.kind == ISKRA_POISSON_MGPCG) {
     if (c->g.dim != 2 || p.per_x || p.per_y || interior_tag) {
       isk_set_error("poisson: multigrid needs a 2-D grid with Dirichlet walls only");
@@ -1177,6 +1313,57 @@ int isk_poisson_solve(iskra_ctx *c, int warm, int *iters, double *relres) {
     return ISKRA_OK;
   }
   const int nblk = (nn + RED_BLOCK - 1) / RED_BLOCK;
+  if (p.kind == ISKRA_POISSON_SPECTRAL) {
+    /* transform solve, then FP64 residual; repeat on the residual (defect
+     * correction) until it meets rtol */
+    PcgArgs a = make_pcg(c, 0);
+    a.warm = 0;
+    const int mx = p.sp_nx, my = p.sp_ny;
+    hipLaunchKernelGGL(k_pcg_setx, dim3(nblk), dim3(RED_BLOCK), 0, c->stream, a);
+    ISK_CHECK_LAUNCH();
+    int rounds = 0;
+    double rel = 0.0;
+    bool done = false;
+    while (!done) {
+      if (rounds >= 4) {
+        isk_set_error("poisson: spectral solve reached relative residual %.3g > %.3g", rel,
+                      p.rtol);
+        return ISKRA_E_CONVERGE;
+      }
+      hipLaunchKernelGGL(k_sp_pack, dim3((mx * my + 255) / 256), dim3(256), 0, c->stream, a,
+                         p.sp_F, mx, my, rounds > 0 ? 1 : 0);
+      ISK_CHECK_LAUNCH();
+      int rc;
+      if ((rc = sp_gemm(c, p.sp_Sy, p.sp_F, nullptr, p.sp_T1, nullptr, my, mx, my, 0))) return rc;
+      if ((rc = sp_gemm(c, p.sp_T1, p.sp_Sx, p.sp_Li, p.sp_T2, nullptr, my, mx, mx, 1))) return rc;
+      if ((rc = sp_gemm(c, p.sp_Sy, p.sp_T2, nullptr, p.sp_T1, nullptr, my, mx, my, 0))) return rc;
+      if ((rc = sp_gemm(c, p.sp_T1, p.sp_Sx, nullptr, nullptr, p.x, my, mx, mx,
+                        rounds > 0 ? 3 : 2)))
+        return rc;
+      hipLaunchKernelGGL(k_pcg_init, dim3(1), dim3(64), 0, c->stream, p.scal, p.iflag,
+                         p.rtol * p.rtol, 0);
+      ISK_CHECK_LAUNCH();
+      hipLaunchKernelGGL(k_pcg_resid, dim3(nblk), dim3(RED_BLOCK), 0, c->stream, a);
+      ISK_CHECK_LAUNCH();
+      ++rounds;
+      double sc[S_WORDS];
+      ISK_HIP(hipMemcpyAsync(sc, p.scal, sizeof(sc), hipMemcpyDeviceToHost, c->stream));
+      ISK_HIP(hipStreamSynchronize(c->stream));
+      rel = sc[S_RELRES];
+      done = rel <= p.rtol;
+    }
+    hipLaunchKernelGGL(k_pcg_out, dim3(nblk), dim3(RED_BLOCK), 0, c->stream, a, c->phi, 0.0);
+    ISK_CHECK_LAUNCH();
+    p.last_iters = rounds;
+    p.last_relres = rel;
+    if (iters) *iters = rounds;
+    if (relres) *relres = rel;
+    c->stats.poisson_solves++;
+    c->stats.poisson_iters += rounds;
+    c->stats.last_iters = rounds;
+    c->stats.last_relres = rel;
+    return ISKRA_OK;
+  }
   const int mgpre = p.kind == ISKRA_POISSON_MGPCG;
   PcgArgs a = make_pcg(c, mgpre);
   a.warm = warm;

@@ -136,3 +136,105 @@ def autotune(instances, candidates=((32, 1024), (64, 2048), (64, 4096), (128, 40
                 log(f"class {cl} tile {cand[0]} chunk {cand[1]}: {1e3 * t / steps:.3f} ms/step")
         best[cl] = min(timings, key=timings.get)
     return [best[size_class(i["cells"])] for i in instances], table
+
+
+def experiment(count=256, steps=500, seed=0, tune_subset=48, single_steps=48, check=4,
+               log=print):
+    """Autotune, then advance `count` instances `steps` steps as one batch;
+    time the same instances stepped one by one (iskra_step per context) for
+    `single_steps` steps; check `check` members bit for bit against single
+    runs.  Returns the measured figures."""
+    import json as _json
+    t0 = time.perf_counter()
+
+    def say(m):
+        if log is not None:
+            log(f"[{time.perf_counter() - t0:7.1f} s] {m}")
+
+    insts = draw_instances(count, seed=seed)
+    sub = insts[:tune_subset]
+    tun_sub, table = autotune(sub, candidates=((32, 1024), (64, 2048), (128, 4096)), steps=16,
+                              log=say)
+    best = {size_class(i["cells"]): t for i, t in zip(sub, tun_sub)}
+    tunings = [best.get(size_class(i["cells"]), (64, 2048)) for i in insts]
+    say(f"autotuned shapes per size class: {best}")
+    t = time.perf_counter()
+    ctxs, dt = build(insts, tunings)
+    setup_s = time.perf_counter() - t
+    np0 = sum(c.count(0) + c.count(1) for c in ctxs)
+    say(f"built {count} instances ({np0} particles) in {setup_s:.1f} s")
+    b = Batch(ctxs)
+    res = {}
+    try:
+        b.step(dt, 16)
+        ps0 = sum(c.stats()["particle_steps"] for c in ctxs)
+        t = time.perf_counter()
+        b.step(dt, steps)
+        tb = time.perf_counter() - t
+        ps1 = sum(c.stats()["particle_steps"] for c in ctxs)
+        res.update(batch_s=tb, batch_steps=steps, batch_psteps=ps1 - ps0,
+                   batch_rate=(ps1 - ps0) / tb, batch_ms_per_step=1e3 * tb / steps,
+                   graphs=b.graphs_launched())
+        say(f"batch: {steps} steps in {tb:.2f} s, {(ps1 - ps0) / tb:.3e} particle-steps/s")
+        # the same contexts stepped one by one
+        ps0 = sum(c.stats()["particle_steps"] for c in ctxs)
+        t = time.perf_counter()
+        for c in ctxs:
+            c.step(dt, single_steps)
+        ts = time.perf_counter() - t
+        ps1 = sum(c.stats()["particle_steps"] for c in ctxs)
+        res.update(single_s=ts, single_steps=single_steps, single_psteps=ps1 - ps0,
+                   single_rate=(ps1 - ps0) / ts)
+        say(f"one by one: {single_steps} steps each in {ts:.2f} s, "
+            f"{(ps1 - ps0) / ts:.3e} particle-steps/s")
+    finally:
+        b.close()
+    # bitwise spot check: a fresh batch of the first 8 instances against
+    # fresh single runs of its first `check` members
+    ok = []
+    m = min(8, count)
+    fresh, _ = build(insts[:m], tunings[:m])
+    bb = Batch(fresh)
+    bb.step(dt, 48)
+    bb.close()
+    for k in range(min(check, m)):
+        cs, _ = ccp_context(insts[k], tile=tunings[k][0], chunk=tunings[k][1])
+        cs.step(dt, 48)
+        cb = fresh[k]
+        same = all(np.array_equal(cb.get_state(s)[f].view(np.uint32),
+                                  cs.get_state(s)[f].view(np.uint32))
+                   for s in (0, 1) for f in ("cell", "ox", "vx", "vy", "vz", "id"))
+        same = same and np.array_equal(cb.field(L.F_PHI), cs.field(L.F_PHI))
+        ok.append(bool(same))
+        cs.close()
+    close_all(fresh)
+    say(f"bitwise spot checks: {ok}")
+    res.update(instances=count, particles_initial=int(np0), setup_s=setup_s,
+               tunings_by_class={str(k): list(v) for k, v in best.items()}, autotune=table,
+               bitwise_equal=ok,
+               cells_range=[min(i["cells"] for i in insts), max(i["cells"] for i in insts)],
+               particles_range=[min(i["particles"] for i in insts),
+                                max(i["particles"] for i in insts)])
+    close_all(ctxs)
+    say(_json.dumps({k: v for k, v in res.items() if k != "autotune"}))
+    return res
+
+
+def main(argv=None):
+    import argparse
+    import json as _json
+    ap = argparse.ArgumentParser(description="batched ensemble of 1-D RF discharges")
+    ap.add_argument("--count", type=int, default=256)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--out", default=None)
+    a = ap.parse_args(argv)
+    res = experiment(a.count, a.steps, a.seed, log=lambda m: print(m, flush=True))
+    if a.out:
+        with open(a.out, "w") as f:
+            f.write(_json.dumps(res) + "\n")
+    print("closed", flush=True)
+
+
+if __name__ == "__main__":
+    main()

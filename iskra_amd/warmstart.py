@@ -146,10 +146,12 @@ def train(ws, first, count, steps, batch=8, lr=1e-3, lr_end=1e-5, chunk=50, seed
     return np.asarray(losses), gain
 
 
-def solve_counts(ctx, ws, rho, phi_prev, kinds=(L.POISSON_PCG, L.POISSON_MG), rtol=1e-8):
-    """Iterations and ms of one solve of rho from zero, previous-step and
-    learned starts, for each solver kind.  The context's solver is
-    reconfigured for each kind (its grid must be the network's)."""
+def solve_table(ctx, ws, snapshots, kinds=(L.POISSON_PCG, L.POISSON_MG), rtol=1e-8):
+    """For each solver kind (configured once, then one untimed solve), the
+    iterations and wall ms of solving every snapshot (rho, phi_prev) from a
+    zero start, from phi_prev and from the network's prediction (inference
+    time included).  Every solve must reach rtol.  Returns
+    {kind: {start: {"iters": [...], "ms": [...], "relres": [...]}}}."""
     n = ws.side + 1
     tags = np.zeros((n, n), np.int8)
     tags[0, :] = tags[-1, :] = tags[:, 0] = tags[:, -1] = 1
@@ -157,23 +159,26 @@ def solve_counts(ctx, ws, rho, phi_prev, kinds=(L.POISSON_PCG, L.POISSON_MG), rt
     for kind in kinds:
         ctx.poisson(kind, 1.0, tags.ravel(), rtol=rtol, max_iter=100000)
         ctx.tag_value(1, 0.0)
-        ctx.set_field(L.F_RHO, rho)
-        row = {}
-        for start in ("zero", "prev", "learned"):
-            ctx.sync()
-            t0 = time.perf_counter()
-            if start == "zero":
-                it, rr = ctx.solve(L.WARM_ZERO)
-            else:
+        ctx.set_field(L.F_RHO, snapshots[0][0])
+        ctx.solve(L.WARM_ZERO)
+        row = {s: {"iters": [], "ms": [], "relres": []} for s in ("zero", "prev", "learned")}
+        for rho, prev in snapshots:
+            ctx.set_field(L.F_RHO, rho)
+            for start in ("zero", "prev", "learned"):
                 if start == "prev":
-                    ctx.set_field(L.F_PHI, phi_prev)
-                    ctx.sync()
-                    t0 = time.perf_counter()
+                    ctx.set_field(L.F_PHI, prev)
+                ctx.sync()
+                t0 = time.perf_counter()
+                if start == "zero":
+                    it, rr = ctx.solve(L.WARM_ZERO)
                 else:
-                    ws.predict()
-                it, rr = ctx.solve(L.WARM_PREV)
-            ctx.sync()
-            row[start] = dict(iters=it, relres=rr, ms=1e3 * (time.perf_counter() - t0))
+                    if start == "learned":
+                        ws.predict()
+                    it, rr = ctx.solve(L.WARM_PREV)
+                ctx.sync()
+                row[start]["iters"].append(it)
+                row[start]["ms"].append(1e3 * (time.perf_counter() - t0))
+                row[start]["relres"].append(rr)
         res[kind] = row
     return res
 
@@ -221,18 +226,12 @@ def experiment(side=512, levels=4, base=32, runs=8, steps_per_run=500, train_ste
         ctx.profile(False)
         inf_ms = st["kernel_ms"]["warmstart"] / max(st["kernel_calls"]["warmstart"], 1)
         names = {L.POISSON_PCG: "pcg", L.POISSON_MG: "mg"}
-        acc = {nm: {s: {"iters": [], "ms": []} for s in ("zero", "prev", "learned")}
-               for nm in names.values()}
-        # one untimed solve per kind first (first-use set-up)
-        solve_counts(ctx, ws, held[0][0], held[0][1])
-        for rho, prev, _phi in held[1::every]:
-            r = solve_counts(ctx, ws, rho, prev)
-            for kind, row in r.items():
-                for s, v in row.items():
-                    acc[names[kind]][s]["iters"].append(v["iters"])
-                    acc[names[kind]][s]["ms"].append(v["ms"])
-        summary = {k: {s: {"iters": float(np.mean(v["iters"])), "ms": float(np.mean(v["ms"]))}
-                       for s, v in row.items()} for k, row in acc.items()}
+        snaps = [(rho, prev) for rho, prev, _phi in held[1::every]]
+        table = solve_table(ctx, ws, snaps)
+        summary = {names[k]: {s: {"iters": float(np.mean(v["iters"])),
+                                  "ms": float(np.median(v["ms"])),
+                                  "max_relres": float(np.max(v["relres"]))}
+                              for s, v in row.items()} for k, row in table.items()}
         flops = forward_flops(levels, base, side)
         out = dict(side=side, levels=levels, base=base, params=ws.nparam, snapshots=n,
                    capture_s=cap_s, train_steps=train_steps, batch=batch, train_s=tr_s,
@@ -240,7 +239,7 @@ def experiment(side=512, levels=4, base=32, runs=8, steps_per_run=500, train_ste
                    train_tflops=3 * flops * batch / (tr_s / max(train_steps, 1)) / 1e12,
                    heldout_mse_before=e0, heldout_mse_after=e1, gain=gain,
                    inference_ms=inf_ms, inference_tflops=flops / (inf_ms * 1e-3) / 1e12,
-                   solves=summary, evaluated=len(held[1::every]))
+                   solves=summary, evaluated=len(snaps))
         say(json.dumps(summary))
         return out
     finally:

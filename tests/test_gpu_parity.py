@@ -238,7 +238,7 @@ def test_poisson_1d_direct_bitexact(gpu_ready, orc, periodic):
         assert np.allclose(phi, x, rtol=1e-12, atol=1e-12 * np.abs(x).max())
 
 
-@pytest.mark.parametrize("kind", [1, 2, 3])
+@pytest.mark.parametrize("kind", [1, 2, 3, 4])
 def test_poisson_2d_iterative_vs_direct(gpu_ready, orc, kind):
     from iskra_amd import Context, _lib as L
     rng = np.random.default_rng(kind)
@@ -292,7 +292,8 @@ def test_capacitor_kat(gpu_ready):
     assert np.allclose(ctx.field(L.F_EY), 0.0, atol=1e-7)
 
 
-@pytest.mark.parametrize("nodes,kind", [(129, 2), (129, 3), (257, 2), (257, 3), (129, 1)])
+@pytest.mark.parametrize("nodes,kind", [(129, 2), (129, 3), (257, 2), (257, 3), (129, 1),
+                                        (129, 4), (257, 4)])
 def test_poisson_2d_large_vs_banded(gpu_ready, orc, nodes, kind):
     """Multigrid levels above 65^2 run in the fused tile kernels: check the
     solution of the reference system (generalized_poisson.jl) to rtol 1e-8,
@@ -321,3 +322,29 @@ def test_poisson_2d_large_vs_banded(gpu_ready, orc, nodes, kind):
     it2, rr2 = ctx.solve(L.WARM_PREV)
     assert rr2 <= 1e-8 and it2 <= it, (it, it2)
     assert np.max(np.abs(ctx.field(L.F_PHI) - ref)) <= 1e-6 * scale
+
+
+def test_poisson_spectral_rectangular_vs_lu(gpu_ready, orc):
+    """The spectral solve on a rectangular grid (different sine bases along x
+    and y) with four different wall values, against the dense LU of the
+    reference system."""
+    from iskra_amd import Context, _lib as L
+    rng = np.random.default_rng(5)
+    nx, ny = 23, 41
+    dx, eps0 = 0.02, 1.0
+    ctx = Context()
+    ctx.grid(2, (nx, ny), (dx, dx))
+    tags = np.zeros((ny, nx), np.int8)
+    tags[0, :], tags[-1, :], tags[:, 0], tags[:, -1] = 1, 2, 3, 4
+    ctx.poisson(L.POISSON_SPECTRAL, eps0, tags.ravel(), rtol=1e-12)
+    vals = {1: 0.5, 2: -1.0, 3: 2.0, 4: 0.25}
+    for t, v in vals.items():
+        ctx.tag_value(t, v)
+    rho = rng.standard_normal(nx * ny)
+    ctx.set_field(L.F_RHO, rho)
+    it, rr = ctx.solve()
+    assert rr <= 1e-12 and it <= 2, (it, rr)
+    A, b, isdof = orc.poisson_matrix(2, nx, ny, dx, dx,
+                                     dirichlet=[((tags == t).ravel(), v) for t, v in vals.items()])
+    x, sing = orc.potential(A, b, isdof, rho, eps0)
+    assert np.max(np.abs(ctx.field(L.F_PHI) - x)) <= 1e-10 * np.max(np.abs(x))
