@@ -436,48 +436,48 @@ struct SpGemm {
   int M, N, K, nx, mode;
 };
 
-/* C = A B (row-major FP64), 32 x 32 outputs per work-group, 2 x 2 per thread,
- * K in slabs of 32 through LDS, fixed fma order (deterministic) */
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+/* C = A B (row-major FP64) on the matrix cores (v_mfma_f64_16x16x4f64): one
+ * 16 x 16 block of C per work-group, its K range split over the 4 waves, the
+ * 4 partial blocks summed in LDS in wave order (deterministic).  Operand
+ * lanes: A[row0 + l%16][k + l/16], B[k + l/16][col0 + l%16]; result lane l
+ * holds C[row0 + l/16 + 4i][col0 + l%16], i = 0..3 (rows interleaved). */
 __global__ __launch_bounds__(256) void k_sp_gemm(SpGemm g) {
-  __shared__ double sA[32][33], sB[32][33];
-  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
-  const int row0 = blockIdx.y * 32, col0 = blockIdx.x * 32;
-  double c00 = 0.0, c01 = 0.0, c10 = 0.0, c11 = 0.0;
-  for (int k0 = 0; k0 < g.K; k0 += 32) {
-    for (int e = threadIdx.x; e < 1024; e += 256) {
-      const int r = e >> 5, q = e & 31;
-      const int ar = row0 + r, ak = k0 + q;
-      sA[r][q] = (ar < g.M && ak < g.K) ? g.A[(size_t)ar * g.K + ak] : 0.0;
-      const int bk = k0 + r, bc = col0 + q;
-      sB[r][q] = (bk < g.K && bc < g.N) ? g.B[(size_t)bk * g.N + bc] : 0.0;
-    }
-    __syncthreads();
-#pragma unroll 8
-    for (int kk = 0; kk < 32; ++kk) {
-      const double a0 = sA[2 * ty][kk], a1 = sA[2 * ty + 1][kk];
-      const double b0 = sB[kk][2 * tx], b1 = sB[kk][2 * tx + 1];
-      c00 = __builtin_fma(a0, b0, c00);
-      c01 = __builtin_fma(a0, b1, c01);
-      c10 = __builtin_fma(a1, b0, c10);
-      c11 = __builtin_fma(a1, b1, c11);
-    }
-    __syncthreads();
+  __shared__ double red[4][256];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int lr = lane & 15, kq = lane >> 4;
+  const int row0 = blockIdx.y * 16, col0 = blockIdx.x * 16;
+  const int arow = row0 + lr, bcol = col0 + lr;
+  const int kper = ((g.K + 15) / 16) * 4;
+  const int kbeg = w * kper;
+  const int kend = min(g.K, kbeg + kper);
+  f64x4 acc = {0.0, 0.0, 0.0, 0.0};
+  for (int k = kbeg; k < kend; k += 4) {
+    const int kk = k + kq;
+    const double a = (arow < g.M && kk < kend) ? g.A[(size_t)arow * g.K + kk] : 0.0;
+    const double b = (kk < kend && bcol < g.N) ? g.B[(size_t)kk * g.N + bcol] : 0.0;
+    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
   }
-  const double cv[2][2] = {{c00, c01}, {c10, c11}};
-  for (int r = 0; r < 2; ++r)
-    for (int q = 0; q < 2; ++q) {
-      const int row = row0 + 2 * ty + r, col = col0 + 2 * tx + q;
-      if (row >= g.M || col >= g.N) continue;
-      const double v = cv[r][q];
-      const size_t e = (size_t)row * g.N + col;
-      if (g.mode == 0) g.C[e] = v;
-      else if (g.mode == 1) g.C[e] = v * g.L[e];
-      else {
-        const size_t n = (size_t)(row + 1) * g.nx + (col + 1);
-        if (g.mode == 2) g.x[n] = v;
-        else g.x[n] += v;
-      }
+#pragma unroll
+  for (int i = 0; i < 4; ++i) red[w][(kq + 4 * i) * 16 + lr] = acc[i];
+  __syncthreads();
+  if (w != 0) return;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int e = (kq + 4 * i) * 16 + lr;
+    const double v = ((red[0][e] + red[1][e]) + red[2][e]) + red[3][e];
+    const int row = row0 + kq + 4 * i, col = col0 + lr;
+    if (row >= g.M || col >= g.N) continue;
+    const size_t o = (size_t)row * g.N + col;
+    if (g.mode == 0) g.C[o] = v;
+    else if (g.mode == 1) g.C[o] = v * g.L[o];
+    else {
+      const size_t n = (size_t)(row + 1) * g.nx + (col + 1);
+      if (g.mode == 2) g.x[n] = v;
+      else g.x[n] += v;
     }
+  }
 }
 
 static int sp_gemm(iskra_ctx *c, const double *A, const double *B, const double *L, double *C,
@@ -493,7 +493,7 @@ static int sp_gemm(iskra_ctx *c, const double *A, const double *B, const double 
   g.K = K;
   g.nx = c->g.nx;
   g.mode = mode;
-  hipLaunchKernelGGL(k_sp_gemm, dim3((N + 31) / 32, (M + 31) / 32), dim3(256), 0, c->stream, g);
+  hipLaunchKernelGGL(k_sp_gemm, dim3((N + 15) / 16, (M + 15) / 16), dim3(256), 0, c->stream, g);
   ISK_CHECK_LAUNCH();
   return ISKRA_OK;
 }

@@ -256,6 +256,8 @@ def test_poisson_2d_iterative_vs_direct(gpu_ready, orc, kind):
     ctx.set_field(L.F_RHO, rho)
     it, rr = ctx.solve()
     assert rr <= 1e-10
+    if kind == L.POISSON_SPECTRAL:
+        assert it == 1, it
     phi = ctx.field(L.F_PHI)
     mask1 = (tags == 1).ravel()
     mask2 = (tags == 2).ravel()
@@ -313,6 +315,8 @@ def test_poisson_2d_large_vs_banded(gpu_ready, orc, nodes, kind):
     ctx.set_field(L.F_RHO, rho)
     it, rr = ctx.solve(L.WARM_ZERO)
     assert rr <= 1e-8, (it, rr)
+    if kind == L.POISSON_SPECTRAL:
+        assert it == 1, it  # one transform pass is exact to FP64 round-off
     phi = ctx.field(L.F_PHI)
     scale = np.max(np.abs(ref))
     # error bound: cond(A) * 1e-8 is loose; observed errors are ~1e-8 relative
@@ -343,7 +347,7 @@ def test_poisson_spectral_rectangular_vs_lu(gpu_ready, orc):
     rho = rng.standard_normal(nx * ny)
     ctx.set_field(L.F_RHO, rho)
     it, rr = ctx.solve()
-    assert rr <= 1e-12 and it <= 2, (it, rr)
+    assert rr <= 1e-12 and it == 1, (it, rr)
     A, b, isdof = orc.poisson_matrix(2, nx, ny, dx, dx,
                                      dirichlet=[((tags == t).ravel(), v) for t, v in vals.items()])
     x, sing = orc.potential(A, b, isdof, rho, eps0)
