@@ -169,4 +169,43 @@ ISK_HD void isk_sincos2pi(double u, double *s, double *c) {
   }
 }
 
+/* ---- fixed-point CIC weights ----
+ * round-to-nearest-even of h * 2^32 and of a * b * 2^32 for floats in [0, 1),
+ * computed from the float bit patterns with integer operations.  The scaled
+ * values are exact binary numbers (24 or 48 significant bits), so these equal
+ * llrint((double)h * 2^32) and llrint((double)a * (double)b * 2^32)
+ * exactly; they replace the FP64 conversions on the particle path. */
+ISK_HD long long isk_rne_shift(unsigned long long m, int s) {
+  /* round-half-even of m / 2^s, m < 2^48, s >= 1 */
+  if (s > 49) return 0;
+  unsigned long long q = m >> s;
+  unsigned long long rem = m & ((1ull << s) - 1ull);
+  unsigned long long half = 1ull << (s - 1);
+  if (rem > half || (rem == half && (q & 1ull))) ++q;
+  return (long long)q;
+}
+
+ISK_HD long long isk_fix32(float h) {
+  unsigned int b;
+  __builtin_memcpy(&b, &h, 4);
+  const int E = (int)((b >> 23) & 0xFFu);
+  if (E == 0) return 0; /* zero or subnormal: below 2^-126, rounds to 0 */
+  const unsigned long long m = (unsigned long long)((b & 0x7FFFFFu) | 0x800000u);
+  const int sh = E - 118; /* h * 2^32 = m * 2^(E - 118) */
+  if (sh >= 0) return (long long)(m << sh);
+  return isk_rne_shift(m, -sh);
+}
+
+ISK_HD long long isk_fixprod32(float a, float b) {
+  unsigned int ba, bb;
+  __builtin_memcpy(&ba, &a, 4);
+  __builtin_memcpy(&bb, &b, 4);
+  const int Ea = (int)((ba >> 23) & 0xFFu), Eb = (int)((bb >> 23) & 0xFFu);
+  if (Ea == 0 || Eb == 0) return 0;
+  const unsigned long long ma = (unsigned long long)((ba & 0x7FFFFFu) | 0x800000u);
+  const unsigned long long mb = (unsigned long long)((bb & 0x7FFFFFu) | 0x800000u);
+  /* a b 2^32 = ma mb 2^(Ea + Eb - 268); Ea, Eb <= 126 so the shift is >= 16 */
+  return isk_rne_shift(ma * mb, 268 - Ea - Eb);
+}
+
 #endif /* ISK_FP64MATH_H */

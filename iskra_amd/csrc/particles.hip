@@ -655,13 +655,14 @@ struct PushArgs {
   int *live_cnt; /* per work-group live particles pushed */
 };
 
+/* Wx = rn(hx 2^32), Wy = rn(hy 2^32), Wxy = rn(hx hy 2^32) (exact products,
+ * integer arithmetic on the float bits: isk_fix32 / isk_fixprod32) */
 __device__ __forceinline__ void fix_weights2(float hx, float hy, long long *w00,
                                              long long *w10, long long *w01,
                                              long long *w11) {
-  double dx = (double)hx, dy = (double)hy;
-  long long Wx = __double2ll_rn(dx * ISK_FIX_ONE);
-  long long Wy = __double2ll_rn(dy * ISK_FIX_ONE);
-  long long Wxy = __double2ll_rn((dx * dy) * ISK_FIX_ONE);
+  long long Wx = isk_fix32(hx);
+  long long Wy = isk_fix32(hy);
+  long long Wxy = isk_fixprod32(hx, hy);
   *w11 = Wxy;
   *w10 = Wx - Wxy;
   *w01 = Wy - Wxy;
@@ -861,7 +862,7 @@ __device__ __forceinline__ void push_body(const PushArgs &a, const int blk) {
             atomicAdd(&a.gfall[q + a.nx + 1], (unsigned long long)w11);
           }
         } else {
-          long long w1 = __double2ll_rn((double)hx * ISK_FIX_ONE);
+          long long w1 = isk_fix32(hx);
           long long w0 = (1LL << 32) - w1;
           int li = i - wx0;
           if (windowed && (unsigned)li < (unsigned)(W - 1)) {
@@ -902,7 +903,7 @@ __device__ __forceinline__ void push_body(const PushArgs &a, const int blk) {
 }
 
 template <int DIM>
-__global__ __launch_bounds__(PUSH_BLOCK) void k_push(PushArgs a) {
+__global__ __launch_bounds__(PUSH_BLOCK) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_push(PushArgs a) {
   push_body<DIM>(a, blockIdx.x);
 }
 

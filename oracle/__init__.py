@@ -31,6 +31,9 @@ How each oracle function is pinned (checks live in tests/):
   orc_philox4x32         Random123 known-answer vectors; structure cross-checked
                          through orc_philox4x64 against numpy.random.Philox.
   orc_exp/log/sincos2pi  libm within a few ulp over dense sweeps.
+  orc_fix32/fixprod32    (the integer fixed-point CIC weights of the particle
+                         kernels) against rint of the exact FP64 products on
+                         edge cases and random offsets.
   mir_*                  against the ref_* functions within FP32 tolerance
                          (tests/test_oracle.py), and bit-exactly against the GPU.
   orc_ws_run             (learned warm start, oracle/ws_oracle.c; no reference
@@ -115,6 +118,8 @@ def _declare(L):
                               U32, P, P, P, P, P, P, P]),
         "mir_sort_tile": (I64, [C.c_int, I64, C.c_int, C.c_int, P, P, P, P, P, P, P]),
         "orc_ws_param_count": (I64, [C.c_int, C.c_int]),
+        "orc_fix32": (I64, [F]),
+        "orc_fixprod32": (I64, [F, F]),
         "orc_ws_run": (D, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, P, P, P, P, P]),
     }
     for name, (res, args) in sig.items():

@@ -81,7 +81,11 @@ EXPORT double orc_u01(uint32_t a, uint32_t b) {
 }
 
 /* thin wrappers so tests can pin the shared FP64 math against libm */
-EXPORT double orc_exp(double x) { return isk_exp(x); }
+EXPORT /* fixed-point CIC weights of the particle kernels (shared header) */
+long long orc_fix32(float h) { return isk_fix32(h); }
+long long orc_fixprod32(float a, float b) { return isk_fixprod32(a, b); }
+
+double orc_exp(double x) { return isk_exp(x); }
 EXPORT double orc_log(double x) { return isk_log(x); }
 EXPORT void orc_sincos2pi(double u, double *s, double *c) { isk_sincos2pi(u, s, c); }
 

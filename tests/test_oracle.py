@@ -421,3 +421,22 @@ def test_cpu_twin_two_stream_energy(orc):
     sim.step(p["dt"], 20)
     assert sp.np == 4000
     assert abs(sim.phi.mean()) < 1e-12 * max(1.0, np.abs(sim.phi).max())
+
+
+def test_fixed_point_weights_equal_fp64_rounding(orc):
+    """isk_fix32 / isk_fixprod32 (integer arithmetic on float bits, used by the
+    particle kernels) equal round-half-even of the exact FP64 products."""
+    lib = orc.lib()
+    rng = np.random.default_rng(4)
+    vals = [0.0, 1e-45, 1e-40, 1.1754944e-38, 1e-30, 2.3283064e-10, 1.1641532e-10,
+            3.4924597e-10, 5.9604645e-08, 0.001953125, 0.00390625, 0.25, 0.5, 0.75,
+            np.nextafter(np.float32(1), np.float32(0))]
+    vals += list(rng.random(400).astype(np.float32))
+    vals += list((rng.random(200) * 2.0 ** -rng.integers(0, 40, 200)).astype(np.float32))
+    vals = np.array(vals, dtype=np.float32)
+    for a in vals:
+        assert lib.orc_fix32(float(a)) == int(np.rint(np.float64(a) * 2.0 ** 32))
+    for a in vals[::3]:
+        for b in vals[1::7]:
+            exact = np.float64(a) * np.float64(b)  # 48-bit product: exact in FP64
+            assert lib.orc_fixprod32(float(a), float(b)) == int(np.rint(exact * 2.0 ** 32))
