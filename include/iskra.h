@@ -48,7 +48,10 @@ int iskra_version(void);
  * (ParticleInCell/src/ParticleInCell.jl:84-139 owns all arrays). */
 int iskra_ctx_create(int device, iskra_ctx **out);
 int iskra_ctx_destroy(iskra_ctx *ctx);
-/* Wait for all work queued on the context's stream. */
+/* Wait for all work queued on the context's stream.  Reports errors the
+ * device recorded meanwhile: particle-capacity overflow (ISKRA_E_CAPACITY),
+ * a collision probability above 1 (ISKRA_E_PROB), and a spectral Poisson solve inside iskra_step that missed rtol after its
+ * correction pass (ISKRA_E_CONVERGE; the flag is cleared once reported). */
 int iskra_sync(iskra_ctx *ctx);
 
 /* ---- grid --------------------------------------------------------------- */
@@ -235,7 +238,9 @@ int iskra_rng_export(iskra_ctx *ctx, uint32_t stream, uint32_t step,
 /* ---- the time loop ------------------------------------------------------- */
 /* nsteps iterations of ParticleInCell.solve's loop body
  * (ParticleInCell.jl:102-135): MCC interactions, advance! of every species,
- * deposit, Poisson, E; tag waveforms are applied after each solve. */
+ * deposit, Poisson, E; tag waveforms are applied after each solve.  With
+ * the spectral solver no step reads anything back; the call waits for its
+ * stream once, at its end (iskra_sync), which reports device-side failures. */
 int iskra_step(iskra_ctx *ctx, double dt, int64_t nsteps);
 
 /* Kernel classes timed when profiling is on (HIP events recorded on the

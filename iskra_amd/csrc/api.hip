@@ -155,6 +155,7 @@ int isk_species_alloc(iskra_ctx *c, IskSpecies *s) {
     return ISKRA_E_HIP;
   ISK_HIP(hipMemset(s->hdr, 0, HDR_WORDS * 8));
   ISK_HIP(hipMemset(s->cnt, 0, (size_t)nn * 8));
+  ISK_HIP(hipMemset(s->gfall, 0, (size_t)nn * 8));
   return alloc_chunk_tables(c, s);
 }
 
@@ -332,6 +333,7 @@ int iskra_ctx_destroy(iskra_ctx *c) {
 int iskra_sync(iskra_ctx *c) {
   CHECK_CTX(c);
   ISK_HIP(hipStreamSynchronize(c->stream));
+  RC(isk_poisson_check(c, nullptr));
   for (int s = 0; s < c->nspecies; ++s) {
     long long of = 0;
     ISK_HIP(hipMemcpy(&of, c->sp[s].hdr + HDR_OVERFLOW, 8, hipMemcpyDeviceToHost));
@@ -1098,14 +1100,12 @@ int iskra_step(iskra_ctx *c, double dt, int64_t nsteps) {
       for (int m = 0; m < c->nmcc; ++m) RC(isk_mcc_run(c, m, dt, nullptr));
       RC(isk_prof_end(c, ev, ISKRA_K_MCC, 0));
     }
-    for (int s = 0; s < c->nspecies; ++s) RC(isk_push_species(c, s, dt, 1));
-    RC(isk_prof_begin(c, &ev));
-    RC(isk_rho(c));
-    RC(isk_prof_end(c, ev, ISKRA_K_MERGE, 0));
+    RC(isk_push_deposit_all(c, dt));
     int iters = 0;
     double rr = 0.0;
+    const bool async = c->ps.kind == ISKRA_POISSON_SPECTRAL;
     RC(isk_prof_begin(c, &ev));
-    RC(isk_poisson_solve(c, ISKRA_WARM_PREV, &iters, &rr));
+    RC(isk_poisson_solve(c, ISKRA_WARM_PREV, async ? nullptr : &iters, async ? nullptr : &rr));
     RC(isk_prof_end(c, ev, ISKRA_K_POISSON, 0));
     RC(isk_prof_begin(c, &ev));
     RC(isk_efield(c));

@@ -259,6 +259,7 @@ int isk_sort_species(iskra_ctx *c, int sid, int exact_cell);
 int isk_push_species(iskra_ctx *c, int sid, double dt, int do_push);
 int isk_merge_species(iskra_ctx *c, int sid, int count_live);
 int isk_rho(iskra_ctx *c);
+int isk_push_deposit_all(iskra_ctx *c, double dt);
 int isk_gather_species(iskra_ctx *c, int sid);
 int isk_convert_upload(iskra_ctx *c, int sid, int64_t np, const double *x,
                        const double *v, const uint32_t *id);
@@ -268,6 +269,7 @@ int isk_sample_maxwellian(iskra_ctx *c, int sid, int64_t n, const double *wx,
 int isk_poisson_setup(iskra_ctx *c);
 int isk_poisson_free(iskra_ctx *c);
 int isk_poisson_solve(iskra_ctx *c, int warm, int *iters, double *relres);
+int isk_poisson_check(iskra_ctx *c, double *relres);
 int isk_efield(iskra_ctx *c);
 int isk_mcc_setup(iskra_ctx *c, int id);
 int isk_mcc_run(iskra_ctx *c, int id, double dt, int64_t *events);

@@ -31,7 +31,7 @@
 #include "isk_internal.h"
 
 #define PUSH_BLOCK 256
-#define PUSH_UNROLL 4
+#define PUSH_UNROLL 8
 #define SORT_BLOCK 256
 #define SORT_PT 32
 #define SORT_CHUNK (SORT_BLOCK * SORT_PT)
@@ -406,7 +406,20 @@ __device__ void sort_scatter_body(const ScatterArgs &a, const int blk) {
   for (int r = 0; r < SORT_PT; ++r) {
     long long p = b0 + (long long)r * SORT_BLOCK + threadIdx.x;
     bool valid = p < np;
-    int c = valid ? a.cell[p] : -1;
+    /* the payload is loaded together with the key, so its latency overlaps
+     * the ranking below instead of following it */
+    int c = -1;
+    float fox = 0.f, foy = 0.f, fvx = 0.f, fvy = 0.f, fvz = 0.f;
+    uint32_t fid = 0;
+    if (valid) {
+      c = a.cell[p];
+      fox = a.ox[p];
+      if (a.kd.dim == 2) foy = a.oy[p];
+      fvx = a.vx[p];
+      fvy = a.vy[p];
+      fvz = a.vz[p];
+      if (a.id) fid = a.id[p];
+    }
     int key = valid ? sort_key(c, a.kd) : -1;
     unsigned long long todo = __ballot(valid);
     int myrank = 0;
@@ -434,12 +447,12 @@ __device__ void sort_scatter_body(const ScatterArgs &a, const int blk) {
       }
       long long d = (long long)base[key] + pre + myrank;
       a.cell_o[d] = c;
-      a.ox_o[d] = a.ox[p];
-      if (a.kd.dim == 2) a.oy_o[d] = a.oy[p];
-      a.vx_o[d] = a.vx[p];
-      a.vy_o[d] = a.vy[p];
-      a.vz_o[d] = a.vz[p];
-      if (a.id) a.id_o[d] = a.id[p];
+      a.ox_o[d] = fox;
+      if (a.kd.dim == 2) a.oy_o[d] = foy;
+      a.vx_o[d] = fvx;
+      a.vy_o[d] = fvy;
+      a.vz_o[d] = fvz;
+      if (a.id) a.id_o[d] = fid;
     }
     __syncthreads();
     if (lane < ng[w]) atomicAdd(&base[gkey[w][lane]], gcnt[w][lane]);
@@ -760,6 +773,10 @@ __device__ __forceinline__ void push_body(const PushArgs &a, const int blk) {
     count = min((long long)a.CH, np - start);
     windowed = false;
   }
+  /* the first batch of particles is requested before the E window, so the
+   * two memory latencies overlap instead of following each other */
+  PushRegs<DIM> cur;
+  push_load<DIM>(cur, cellp, oxp, oyp, vxp, vyp, start, count, 0);
   if (windowed) {
     for (int k = threadIdx.x; k < WW; k += PUSH_BLOCK) {
       sD[k] = 0ull;
@@ -784,8 +801,7 @@ __device__ __forceinline__ void push_body(const PushArgs &a, const int blk) {
   const long long per = (long long)PUSH_BLOCK * PUSH_UNROLL;
   for (long long b0 = 0; b0 < count; b0 += per) {
     /* a batch of PUSH_UNROLL particles per thread: all loads, then compute + stores */
-    PushRegs<DIM> cur;
-    push_load<DIM>(cur, cellp, oxp, oyp, vxp, vyp, start, count, b0);
+    if (b0 > 0) push_load<DIM>(cur, cellp, oxp, oyp, vxp, vyp, start, count, b0);
 #pragma unroll
     for (int u = 0; u < PUSH_UNROLL; ++u) {
       int32_t c = cur.c[u];
@@ -903,7 +919,7 @@ __device__ __forceinline__ void push_body(const PushArgs &a, const int blk) {
 }
 
 template <int DIM>
-__global__ __launch_bounds__(PUSH_BLOCK) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_push(PushArgs a) {
+__global__ __launch_bounds__(PUSH_BLOCK) void k_push(PushArgs a) {
   push_body<DIM>(a, blockIdx.x);
 }
 
@@ -945,10 +961,13 @@ static void push_prepare(iskra_ctx *c, int sid, double dt, int do_push, int b, P
   *shm = (size_t)WW * (8 + 4 * g.dim);
 }
 
-int isk_push_species(iskra_ctx *c, int sid, double dt, int do_push) {
+/* the push (or deposit-only) launch of one species; fill != 0 clears the
+ * fallback grid first (every merge clears it after reading, so a push that
+ * follows a merge finds it zero) */
+static int push_launch(iskra_ctx *c, int sid, double dt, int do_push, int fill) {
   IskSpecies *s = &c->sp[sid];
-  int rc = isk_fill_i64(c, s->gfall, isk_nodes(c), 0);
-  if (rc) return rc;
+  int rc;
+  if (fill && (rc = isk_fill_i64(c, s->gfall, isk_nodes(c), 0))) return rc;
   PushArgs a;
   int grid;
   size_t shm;
@@ -967,7 +986,13 @@ int isk_push_species(iskra_ctx *c, int sid, double dt, int do_push) {
     int rc2 = isk_prof_end(c, ev, ISKRA_K_PUSH, 0);
     if (rc2) return rc2;
   }
-  s->deposited = 1;
+  return ISKRA_OK;
+}
+
+int isk_push_species(iskra_ctx *c, int sid, double dt, int do_push) {
+  int rc = push_launch(c, sid, dt, do_push, 1);
+  if (rc) return rc;
+  c->sp[sid].deposited = 1;
   hipEvent_t em = nullptr;
   if ((rc = isk_prof_begin(c, &em))) return rc;
   if ((rc = isk_merge_species(c, sid, do_push))) return rc;
@@ -983,54 +1008,84 @@ struct MergeArgs {
   int nlive; /* entries of live_cnt to add to HDR_PUSHED (0: deposit only) */
   long long *hdr_rw;
   const long long *partial;
-  const unsigned long long *gfall;
+  unsigned long long *gfall; /* read, then cleared for the next push */
   const int *tcb;
   const long long *hdr;
   long long *cnt;
   int nx, ny, T, M, W, ntx, nty;
 };
 
+/* pushed-particle count of the step: sum of the per-chunk live counts */
+__device__ __forceinline__ void merge_live(const MergeArgs &a, const int blk, const int nblk) {
+  __shared__ unsigned long long ls[4];
+  unsigned long long sum = 0;
+  for (int k = blk * blockDim.x + threadIdx.x; k < a.nlive; k += nblk * blockDim.x)
+    sum += (unsigned long long)a.live_cnt[k];
+  for (int off = 32; off > 0; off >>= 1) sum += __shfl_xor(sum, off);
+  if ((threadIdx.x & 63) == 0) ls[threadIdx.x >> 6] = sum;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t = ls[0] + ls[1] + ls[2] + ls[3];
+    if (t) atomicAdd((unsigned long long *)&a.hdr_rw[HDR_PUSHED], t);
+  }
+  __syncthreads();
+}
+
+/* deposit count of one node: fallback grid + the windows that cover it.
+ * Window of tile t spans nodes [tT - M, tT - M + W - 1] per axis, so the
+ * covering tiles are t in [ceil((i + M - W + 1) / T), floor((i + M) / T)]:
+ * at most 4 per axis because the margin M is at most T.  All terms are
+ * int64, so the sum does not depend on the order of the additions. */
+__device__ __forceinline__ int merge_tlo(int i, const MergeArgs &a) {
+  const int u = i + a.M - a.W + 1;
+  return u <= 0 ? 0 : (u + a.T - 1) / a.T;
+}
+
+template <int DIM>
+__device__ __forceinline__ long long merge_node(const MergeArgs &a, const int node) {
+  const int i = node % a.nx, j = node / a.nx;
+  const unsigned long long g = a.gfall[node];
+  if (g) a.gfall[node] = 0ull;
+  long long s = (long long)g;
+  if (a.hdr[HDR_NCHUNKS] <= 0) return s;
+  const int WW = DIM == 2 ? a.W * a.W : a.W;
+  const int tlo = merge_tlo(i, a), thi = min((i + a.M) / a.T, a.ntx - 1);
+  const int ulo = DIM == 2 ? merge_tlo(j, a) : 0;
+  const int uhi = DIM == 2 ? min((j + a.M) / a.T, a.nty - 1) : 0;
+  if (DIM == 2 && a.tilewin) {
+    long long v[16];
+#pragma unroll
+    for (int dy = 0; dy < 4; ++dy)
+#pragma unroll
+      for (int dx = 0; dx < 4; ++dx) {
+        const int ti = tlo + dx, tj = ulo + dy;
+        const int li = i - (ti * a.T - a.M), lj = j - (tj * a.T - a.M);
+        v[4 * dy + dx] = (ti <= thi && tj <= uhi)
+                             ? a.tilewin[(size_t)(ti + a.ntx * tj) * WW + lj * a.W + li]
+                             : 0ll;
+      }
+#pragma unroll
+    for (int k = 0; k < 16; ++k) s += v[k];
+    return s;
+  }
+  for (int tj = ulo; tj <= uhi; ++tj) {
+    const int lj = j - (tj * a.T - a.M);
+    for (int ti = tlo; ti <= thi; ++ti) {
+      const int li = i - (ti * a.T - a.M);
+      const int t = ti + a.ntx * tj;
+      const int q = DIM == 2 ? lj * a.W + li : li;
+      for (int k = a.tcb[t]; k < a.tcb[t + 1]; ++k) s += a.partial[(size_t)k * WW + q];
+    }
+  }
+  return s;
+}
+
 template <int DIM>
 __device__ __forceinline__ void merge_body(const MergeArgs &a, const int blk, const int nblk) {
-  int node = blk * blockDim.x + threadIdx.x;
-  int nn = a.nx * a.ny;
-  if (a.nlive > 0) {
-    __shared__ unsigned long long ls[4];
-    unsigned long long sum = 0;
-    for (int k = node; k < a.nlive; k += nblk * blockDim.x) sum += (unsigned long long)a.live_cnt[k];
-    for (int off = 32; off > 0; off >>= 1) sum += __shfl_xor(sum, off);
-    if ((threadIdx.x & 63) == 0) ls[threadIdx.x >> 6] = sum;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      unsigned long long t = ls[0] + ls[1] + ls[2] + ls[3];
-      if (t) atomicAdd((unsigned long long *)&a.hdr_rw[HDR_PUSHED], t);
-    }
-  }
-  if (node >= nn) return;
-  int i = node % a.nx, j = node / a.nx;
-  long long s = (long long)a.gfall[node];
-  if (a.hdr[HDR_NCHUNKS] > 0) {
-    const int WW = DIM == 2 ? a.W * a.W : a.W;
-    const int back = 1 + (2 * a.M + a.T - 1) / a.T;
-    int tlo = (i + a.M) / a.T - back, thi = (i + a.M) / a.T;
-    int ulo = DIM == 2 ? (j + a.M) / a.T - back : 0, uhi = DIM == 2 ? (j + a.M) / a.T : 0;
-    for (int tj = max(ulo, 0); tj <= min(uhi, a.nty - 1); ++tj) {
-      int wy0 = tj * a.T - a.M;
-      int lj = j - wy0;
-      if (DIM == 2 && (lj < 0 || lj >= a.W)) continue;
-      for (int ti = max(tlo, 0); ti <= min(thi, a.ntx - 1); ++ti) {
-        int wx0 = ti * a.T - a.M;
-        int li = i - wx0;
-        if (li < 0 || li >= a.W) continue;
-        int t = ti + a.ntx * tj;
-        int q = DIM == 2 ? lj * a.W + li : li;
-        if (a.tilewin) s += a.tilewin[(size_t)t * WW + q];
-        else
-          for (int k = a.tcb[t]; k < a.tcb[t + 1]; ++k) s += a.partial[(size_t)k * WW + q];
-      }
-    }
-  }
-  a.cnt[node] = s;
+  if (a.nlive > 0) merge_live(a, blk, nblk);
+  const int node = blk * blockDim.x + threadIdx.x;
+  if (node >= a.nx * a.ny) return;
+  a.cnt[node] = merge_node<DIM>(a, node);
 }
 
 template <int DIM>
@@ -1040,17 +1095,32 @@ __global__ __launch_bounds__(256) void k_merge(MergeArgs a) {
 
 /* stage 1 of the 2-D merge: one work-group per tile sums its chunks' windows
  * (contiguous in memory, chunks of a tile are consecutive) */
+__device__ __forceinline__ void merge_tile_body(const long long *__restrict__ partial,
+                                                const int *__restrict__ tcb,
+                                                const long long *__restrict__ hdr,
+                                                long long *__restrict__ tilewin, int WW, int t) {
+  const int k0 = hdr[HDR_NCHUNKS] > 0 ? tcb[t] : 0, k1 = hdr[HDR_NCHUNKS] > 0 ? tcb[t + 1] : 0;
+  for (int q = threadIdx.x; q < WW; q += blockDim.x) {
+    /* four independent partial sums keep four loads in flight (integer
+     * sums: the grouping does not change the result) */
+    long long s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+    int k = k0;
+    for (; k + 4 <= k1; k += 4) {
+      s0 += partial[(size_t)k * WW + q];
+      s1 += partial[(size_t)(k + 1) * WW + q];
+      s2 += partial[(size_t)(k + 2) * WW + q];
+      s3 += partial[(size_t)(k + 3) * WW + q];
+    }
+    for (; k < k1; ++k) s0 += partial[(size_t)k * WW + q];
+    tilewin[(size_t)t * WW + q] = (s0 + s1) + (s2 + s3);
+  }
+}
+
 __global__ __launch_bounds__(256) void k_merge_tiles(const long long *__restrict__ partial,
                                                      const int *__restrict__ tcb,
                                                      const long long *__restrict__ hdr,
                                                      long long *__restrict__ tilewin, int WW) {
-  const int t = blockIdx.x;
-  const int k0 = hdr[HDR_NCHUNKS] > 0 ? tcb[t] : 0, k1 = hdr[HDR_NCHUNKS] > 0 ? tcb[t + 1] : 0;
-  for (int q = threadIdx.x; q < WW; q += blockDim.x) {
-    long long s = 0;
-    for (int k = k0; k < k1; ++k) s += partial[(size_t)k * WW + q];
-    tilewin[(size_t)t * WW + q] = s;
-  }
+  merge_tile_body(partial, tcb, hdr, tilewin, WW, blockIdx.x);
 }
 
 static void merge_prepare(iskra_ctx *c, int sid, int count_live, MergeArgs *pa, int *grid) {
@@ -1107,19 +1177,59 @@ struct RhoArgs {
   int nn;
 };
 
+/* r + q_s n_s with n_s = count * w0 / V (count in 2^-32 units) */
+__device__ __forceinline__ double rho_term(double r, long long cnt, double qw, double q, double V) {
+  const double n = ((double)cnt * (1.0 / ISK_FIX_ONE)) * qw / V;
+  return r + q * n;
+}
+
 __device__ __forceinline__ void rho_body(const RhoArgs &a, const int blk) {
   int node = blk * blockDim.x + threadIdx.x;
   if (node >= a.nn) return;
   double V = a.vol[node];
   double r = 0.0;
-  for (int s = 0; s < a.ns; ++s) {
-    double n = ((double)a.cnt[s][node] * (1.0 / ISK_FIX_ONE)) * a.qw[s] / V;
-    r += a.q[s] * n;
-  }
+  for (int s = 0; s < a.ns; ++s) r = rho_term(r, a.cnt[s][node], a.qw[s], a.q[s], V);
   a.rho[node] = r + a.bg[node];
 }
 
 __global__ void k_rho(RhoArgs a) { rho_body(a, blockIdx.x); }
+
+/* the time loop's deposit tail for all species in two launches: the tile
+ * windows of every species (grid tiles x species), then per node the counts
+ * of every species and rho (same arithmetic as k_merge + k_rho) */
+struct MergeTilesN {
+  const long long *partial[ISK_MAX_SPECIES];
+  const int *tcb[ISK_MAX_SPECIES];
+  const long long *hdr[ISK_MAX_SPECIES];
+  long long *tilewin[ISK_MAX_SPECIES];
+  int WW;
+};
+
+__global__ __launch_bounds__(256) void k_merge_tiles_n(MergeTilesN a) {
+  const int s = blockIdx.y;
+  merge_tile_body(a.partial[s], a.tcb[s], a.hdr[s], a.tilewin[s], a.WW, blockIdx.x);
+}
+
+struct MergeRhoArgs {
+  MergeArgs m[ISK_MAX_SPECIES];
+  RhoArgs r;
+};
+
+template <int DIM>
+__global__ __launch_bounds__(256) void k_merge_rho(MergeRhoArgs a) {
+  for (int s = 0; s < a.r.ns; ++s)
+    if (a.m[s].nlive > 0) merge_live(a.m[s], blockIdx.x, gridDim.x);
+  const int node = blockIdx.x * blockDim.x + threadIdx.x;
+  if (node >= a.r.nn) return;
+  const double V = a.r.vol[node];
+  double r = 0.0;
+  for (int s = 0; s < a.r.ns; ++s) {
+    const long long cnt = merge_node<DIM>(a.m[s], node);
+    a.m[s].cnt[node] = cnt;
+    r = rho_term(r, cnt, a.r.qw[s], a.r.q[s], V);
+  }
+  a.r.rho[node] = r + a.r.bg[node];
+}
 
 int isk_rho(iskra_ctx *c) {
   RhoArgs a;
@@ -1142,6 +1252,51 @@ int isk_rho(iskra_ctx *c) {
   hipLaunchKernelGGL(k_rho, dim3((a.nn + 255) / 256), dim3(256), 0, c->stream, a);
   ISK_CHECK_LAUNCH();
   return ISKRA_OK;
+}
+
+/* time loop: push every species, then the tile windows of all species in
+ * one launch and counts + rho in another */
+int isk_push_deposit_all(iskra_ctx *c, double dt) {
+  int rc;
+  for (int s = 0; s < c->nspecies; ++s)
+    if ((rc = push_launch(c, s, dt, 1, 0))) return rc;
+  hipEvent_t em = nullptr;
+  if ((rc = isk_prof_begin(c, &em))) return rc;
+  MergeRhoArgs a;
+  int grid = 0;
+  for (int s = 0; s < c->nspecies; ++s) {
+    merge_prepare(c, s, 1, &a.m[s], &grid);
+    if (c->g.dim == 2) a.m[s].tilewin = c->sp[s].tilewin;
+    a.r.cnt[s] = c->sp[s].cnt;
+    a.r.qw[s] = c->sp[s].w0;
+    a.r.q[s] = c->sp[s].q;
+  }
+  a.r.ns = c->nspecies;
+  a.r.vol = c->vol;
+  a.r.bg = c->bg;
+  a.r.rho = c->rho;
+  a.r.nn = isk_nodes(c);
+  if (c->g.dim == 2 && c->nspecies > 0) {
+    MergeTilesN t;
+    for (int s = 0; s < c->nspecies; ++s) {
+      t.partial[s] = c->sp[s].partial;
+      t.tcb[s] = c->sp[s].tile_chunk_begin;
+      t.hdr[s] = c->sp[s].hdr;
+      t.tilewin[s] = c->sp[s].tilewin;
+    }
+    t.WW = c->t.W * c->t.W;
+    hipLaunchKernelGGL(k_merge_tiles_n, dim3(c->ntiles, c->nspecies), dim3(256), 0, c->stream,
+                       t);
+    ISK_CHECK_LAUNCH();
+  }
+  grid = (a.r.nn + 255) / 256;
+  if (c->g.dim == 2)
+    hipLaunchKernelGGL(k_merge_rho<2>, dim3(grid), dim3(256), 0, c->stream, a);
+  else
+    hipLaunchKernelGGL(k_merge_rho<1>, dim3(grid), dim3(256), 0, c->stream, a);
+  ISK_CHECK_LAUNCH();
+  for (int s = 0; s < c->nspecies; ++s) c->sp[s].deposited = 1;
+  return isk_prof_end(c, em, ISKRA_K_MERGE, 0);
 }
 
 /* ------------------------------------------------------------------------ */

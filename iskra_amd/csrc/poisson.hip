@@ -40,6 +40,9 @@
 #define MG_COARSE_MAX 65
 
 enum { S_RZ = 0, S_PAP, S_ALPHA, S_BETA, S_RR, S_FF, S_TOL2, S_MEAN, S_RELRES, S_RZNEW, S_WORDS = 16 };
+/* iflag words: [0] done, [1] iterations, [2] singular (PCG/MG); spectral solve:
+ * [4] this solve's pass met rtol, [5] sticky: an in-loop solve missed rtol */
+enum { IF_SP_DONE = 4, IF_SP_FAIL = 5 };
 
 struct Sten {
   int dim, nx, ny, per_x, per_y;
@@ -421,7 +424,8 @@ __global__ void k_pcg_out(PcgArgs a, double *phi, double mean_sub) {
 /* spectral direct solve: phi = Sy ((Sy F Sx) / lambda) Sx                   */
 /* ------------------------------------------------------------------------ */
 /* F (interior, row-major [j-1][i-1]) = f_elim, or the residual r (correction) */
-__global__ void k_sp_pack(PcgArgs a, double *F, int mx, int my, int from_r) {
+__global__ void k_sp_pack(PcgArgs a, double *F, int mx, int my, int from_r, const int *skip) {
+  if (skip && *skip) return;
   int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= mx * my) return;
   int i = e % mx + 1, j = e / mx + 1;
@@ -430,6 +434,7 @@ __global__ void k_sp_pack(PcgArgs a, double *F, int mx, int my, int from_r) {
 }
 
 struct SpGemm {
+  const int *skip;         /* non-null and set: nothing to do (solve already converged) */
   const double *A, *B, *L; /* L: elementwise factor (mode 1) */
   double *C;               /* modes 0, 1 */
   double *x;               /* modes 2, 3: node array, interior (i, j) = (col+1, row+1) */
@@ -444,6 +449,7 @@ typedef double f64x4 __attribute__((ext_vector_type(4)));
  * lanes: A[row0 + l%16][k + l/16], B[k + l/16][col0 + l%16]; result lane l
  * holds C[row0 + l/16 + 4i][col0 + l%16], i = 0..3 (rows interleaved). */
 __global__ __launch_bounds__(256) void k_sp_gemm(SpGemm g) {
+  if (g.skip && *g.skip) return;
   __shared__ double red[4][256];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int lr = lane & 15, kq = lane >> 4;
@@ -453,11 +459,22 @@ __global__ __launch_bounds__(256) void k_sp_gemm(SpGemm g) {
   const int kbeg = w * kper;
   const int kend = min(g.K, kbeg + kper);
   f64x4 acc = {0.0, 0.0, 0.0, 0.0};
-  for (int k = kbeg; k < kend; k += 4) {
-    const int kk = k + kq;
-    const double a = (arow < g.M && kk < kend) ? g.A[(size_t)arow * g.K + kk] : 0.0;
-    const double b = (kk < kend && bcol < g.N) ? g.B[(size_t)kk * g.N + bcol] : 0.0;
-    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+  /* all loads of a 64-wide k block are issued before its MFMAs, so the wave
+   * waits for one memory latency per block instead of one per k step; the
+   * accumulation order (k ascending) is the same either way */
+  for (int k0 = kbeg; k0 < kend; k0 += 64) {
+    double av[16], bv[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+      const int kk = k0 + 4 * u + kq;
+      av[u] = (arow < g.M && kk < kend) ? g.A[(size_t)arow * g.K + kk] : 0.0;
+      bv[u] = (kk < kend && bcol < g.N) ? g.B[(size_t)kk * g.N + bcol] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+      if (k0 + 4 * u >= kend) break;
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u], bv[u], acc, 0, 0, 0);
+    }
   }
 #pragma unroll
   for (int i = 0; i < 4; ++i) red[w][(kq + 4 * i) * 16 + lr] = acc[i];
@@ -481,8 +498,9 @@ __global__ __launch_bounds__(256) void k_sp_gemm(SpGemm g) {
 }
 
 static int sp_gemm(iskra_ctx *c, const double *A, const double *B, const double *L, double *C,
-                   double *x, int M, int N, int K, int mode) {
+                   double *x, int M, int N, int K, int mode, const int *skip) {
   SpGemm g;
+  g.skip = skip;
   g.A = A;
   g.B = B;
   g.L = L;
@@ -496,6 +514,55 @@ static int sp_gemm(iskra_ctx *c, const double *A, const double *B, const double 
   hipLaunchKernelGGL(k_sp_gemm, dim3((N + 15) / 16, (M + 15) / 16), dim3(256), 0, c->stream, g);
   ISK_CHECK_LAUNCH();
   return ISKRA_OK;
+}
+
+/* r = f_elim - L x at free nodes, FP64 relative residual -> scal[S_RELRES],
+ * iflag[IF_SP_DONE] = (relres <= rtol).  A correction pass (second != 0) does
+ * nothing when the first pass already met rtol, and marks iflag[IF_SP_FAIL]
+ * if it still misses it. */
+__global__ void k_sp_resid(PcgArgs a, double rtol, int second) {
+  if (second && a.iflag[IF_SP_DONE]) return;
+  int n = blockIdx.x * blockDim.x + threadIdx.x;
+  int nn = a.s.nx * a.s.ny;
+  double rr = 0.0, ff = 0.0;
+  if (n < nn) {
+    int i = n % a.s.nx, j = n / a.s.nx;
+    if (a.s.tags[n] == 0) {
+      double f = a.rho[n] * a.scale;
+      double Lx = apply_L(a.s, a.x, i, j, nullptr);
+      double fe = f + dirichlet_rhs(a.s, i, j);
+      double r = f - Lx;
+      a.r[n] = r;
+      rr = r * r;
+      ff = fe * fe;
+    } else {
+      a.r[n] = 0.0;
+    }
+  }
+  double srr, sff;
+  if (block_reduce2(rr, ff, a.partials, a.counter, &srr, &sff) && threadIdx.x == 0) {
+    double rel = sff > 0.0 ? sqrt(srr / sff) : 0.0;
+    a.scal[S_RELRES] = rel;
+    int done = rel <= rtol;
+    a.iflag[IF_SP_DONE] = done;
+    if (second && !done) a.iflag[IF_SP_FAIL] = 1;
+  }
+}
+
+/* one transform pass: x = L^-1 f_elim (first) or x += L^-1 r (correction) */
+static int sp_pass(iskra_ctx *c, const PcgArgs &a, int correction, const int *skip) {
+  IskPoisson &p = c->ps;
+  const int mx = p.sp_nx, my = p.sp_ny;
+  hipLaunchKernelGGL(k_sp_pack, dim3((mx * my + 255) / 256), dim3(256), 0, c->stream, a, p.sp_F,
+                     mx, my, correction, skip);
+  ISK_CHECK_LAUNCH();
+  int rc;
+  if ((rc = sp_gemm(c, p.sp_Sy, p.sp_F, nullptr, p.sp_T1, nullptr, my, mx, my, 0, skip))) return rc;
+  if ((rc = sp_gemm(c, p.sp_T1, p.sp_Sx, p.sp_Li, p.sp_T2, nullptr, my, mx, mx, 1, skip)))
+    return rc;
+  if ((rc = sp_gemm(c, p.sp_Sy, p.sp_T2, nullptr, p.sp_T1, nullptr, my, mx, my, 0, skip))) return rc;
+  return sp_gemm(c, p.sp_T1, p.sp_Sx, nullptr, nullptr, p.x, my, mx, mx, correction ? 3 : 2,
+                 skip);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -1314,43 +1381,45 @@ int isk_poisson_solve(iskra_ctx *c, int warm, int *iters, double *relres) {
   }
   const int nblk = (nn + RED_BLOCK - 1) / RED_BLOCK;
   if (p.kind == ISKRA_POISSON_SPECTRAL) {
-    /* transform solve, then FP64 residual; repeat on the residual (defect
-     * correction) until it meets rtol */
+    /* transform pass, FP64 residual; if it misses rtol, solve the residual
+     * and add it (defect correction).  Without iters/relres (the time loop)
+     * the correction pass is enqueued unconditionally and skips itself on
+     * the device when the first pass converged; a miss after it is reported
+     * by iskra_sync.  With them, the residual is read back after each pass. */
     PcgArgs a = make_pcg(c, 0);
     a.warm = 0;
-    const int mx = p.sp_nx, my = p.sp_ny;
     hipLaunchKernelGGL(k_pcg_setx, dim3(nblk), dim3(RED_BLOCK), 0, c->stream, a);
     ISK_CHECK_LAUNCH();
-    int rounds = 0;
+    const int *done = p.iflag + IF_SP_DONE;
+    int rc;
+    int rounds = 1;
     double rel = 0.0;
-    bool done = false;
-    while (!done) {
-      if (rounds >= 4) {
-        isk_set_error("poisson: spectral solve reached relative residual %.3g > %.3g", rel,
-                      p.rtol);
-        return ISKRA_E_CONVERGE;
+    if ((rc = sp_pass(c, a, 0, nullptr))) return rc;
+    hipLaunchKernelGGL(k_sp_resid, dim3(nblk), dim3(RED_BLOCK), 0, c->stream, a, p.rtol, 0);
+    ISK_CHECK_LAUNCH();
+    if (!iters && !relres) {
+      if ((rc = sp_pass(c, a, 1, done))) return rc;
+      hipLaunchKernelGGL(k_sp_resid, dim3(nblk), dim3(RED_BLOCK), 0, c->stream, a, p.rtol, 1);
+      ISK_CHECK_LAUNCH();
+    } else {
+      while (true) {
+        double sc[S_WORDS];
+        int fl[8];
+        ISK_HIP(hipMemcpyAsync(sc, p.scal, sizeof(sc), hipMemcpyDeviceToHost, c->stream));
+        ISK_HIP(hipMemcpyAsync(fl, p.iflag, sizeof(fl), hipMemcpyDeviceToHost, c->stream));
+        ISK_HIP(hipStreamSynchronize(c->stream));
+        rel = sc[S_RELRES];
+        if (fl[IF_SP_DONE]) break;
+        if (rounds >= 4) {
+          isk_set_error("poisson: spectral solve reached relative residual %.3g > %.3g", rel,
+                        p.rtol);
+          return ISKRA_E_CONVERGE;
+        }
+        if ((rc = sp_pass(c, a, 1, nullptr))) return rc;
+        hipLaunchKernelGGL(k_sp_resid, dim3(nblk), dim3(RED_BLOCK), 0, c->stream, a, p.rtol, 0);
+        ISK_CHECK_LAUNCH();
+        ++rounds;
       }
-      hipLaunchKernelGGL(k_sp_pack, dim3((mx * my + 255) / 256), dim3(256), 0, c->stream, a,
-                         p.sp_F, mx, my, rounds > 0 ? 1 : 0);
-      ISK_CHECK_LAUNCH();
-      int rc;
-      if ((rc = sp_gemm(c, p.sp_Sy, p.sp_F, nullptr, p.sp_T1, nullptr, my, mx, my, 0))) return rc;
-      if ((rc = sp_gemm(c, p.sp_T1, p.sp_Sx, p.sp_Li, p.sp_T2, nullptr, my, mx, mx, 1))) return rc;
-      if ((rc = sp_gemm(c, p.sp_Sy, p.sp_T2, nullptr, p.sp_T1, nullptr, my, mx, my, 0))) return rc;
-      if ((rc = sp_gemm(c, p.sp_T1, p.sp_Sx, nullptr, nullptr, p.x, my, mx, mx,
-                        rounds > 0 ? 3 : 2)))
-        return rc;
-      hipLaunchKernelGGL(k_pcg_init, dim3(1), dim3(64), 0, c->stream, p.scal, p.iflag,
-                         p.rtol * p.rtol, 0);
-      ISK_CHECK_LAUNCH();
-      hipLaunchKernelGGL(k_pcg_resid, dim3(nblk), dim3(RED_BLOCK), 0, c->stream, a);
-      ISK_CHECK_LAUNCH();
-      ++rounds;
-      double sc[S_WORDS];
-      ISK_HIP(hipMemcpyAsync(sc, p.scal, sizeof(sc), hipMemcpyDeviceToHost, c->stream));
-      ISK_HIP(hipStreamSynchronize(c->stream));
-      rel = sc[S_RELRES];
-      done = rel <= p.rtol;
     }
     hipLaunchKernelGGL(k_pcg_out, dim3(nblk), dim3(RED_BLOCK), 0, c->stream, a, c->phi, 0.0);
     ISK_CHECK_LAUNCH();
@@ -1520,5 +1589,26 @@ int isk_blaunch_poisson(const IskBLaunch &L, hipStream_t st) {
       return ISKRA_E_ARG;
   }
   ISK_CHECK_LAUNCH();
+  return ISKRA_OK;
+}
+
+/* in-loop spectral solves that missed rtol since the last check (clears it) */
+int isk_poisson_check(iskra_ctx *c, double *relres) {
+  IskPoisson &p = c->ps;
+  if (p.kind != ISKRA_POISSON_SPECTRAL || !p.iflag) return ISKRA_OK;
+  int fl[8];
+  double sc[S_WORDS];
+  ISK_HIP(hipMemcpy(fl, p.iflag, sizeof(fl), hipMemcpyDeviceToHost));
+  ISK_HIP(hipMemcpy(sc, p.scal, sizeof(sc), hipMemcpyDeviceToHost));
+  if (relres) *relres = sc[S_RELRES];
+  p.last_relres = sc[S_RELRES];
+  c->stats.last_relres = sc[S_RELRES];
+  if (fl[IF_SP_FAIL]) {
+    int zero = 0;
+    ISK_HIP(hipMemcpy(p.iflag + IF_SP_FAIL, &zero, 4, hipMemcpyHostToDevice));
+    isk_set_error("poisson: a spectral solve in the time loop missed rtol %.3g after "
+                  "correction (relative residual %.3g)", p.rtol, sc[S_RELRES]);
+    return ISKRA_E_CONVERGE;
+  }
   return ISKRA_OK;
 }

@@ -352,3 +352,43 @@ def test_poisson_spectral_rectangular_vs_lu(gpu_ready, orc):
                                      dirichlet=[((tags == t).ravel(), v) for t, v in vals.items()])
     x, sing = orc.potential(A, b, isdof, rho, eps0)
     assert np.max(np.abs(ctx.field(L.F_PHI) - x)) <= 1e-10 * np.max(np.abs(x))
+
+
+def test_spectral_in_loop_equals_direct_call(gpu_ready):
+    """In the time loop the spectral solve is enqueued without reading its
+    residual back (the correction pass skips itself on the device when the
+    first pass met rtol): the potential it leaves equals a synchronous
+    iskra_poisson_solve of the same charge density bit for bit, and a solve
+    that misses rtol is reported once, when iskra_step waits for the stream
+    at its end."""
+    from iskra_amd import Context, IskraError, _lib as L
+    rng = np.random.default_rng(21)
+    nx = ny = 65
+    dx = 1e-3
+    eps0 = 8.854e-12
+    ctx = Context()
+    ctx.grid(2, (nx, ny), (dx, dx))
+    sid = ctx.species(-1.6e-19, 9.1e-31, 1e4, 300000)
+    ctx.boundary(sid, [L.BND_DISCARD, L.BND_DISCARD])
+    st = _rand_state(rng, 200000, nx - 1, ny - 1, 1e5, 1e5)
+    ctx.set_state(sid, *st)
+    tags = np.zeros((ny, nx), np.int8)
+    tags[0, :] = tags[-1, :] = tags[:, 0] = 1
+    tags[:, -1] = 2
+    ctx.poisson(L.POISSON_SPECTRAL, eps0, tags.ravel(), rtol=1e-10)
+    ctx.tag_value(2, 5.0)
+    ctx.step(1e-12, 3)
+    ctx.sync()
+    s = ctx.stats()
+    assert s["poisson_solves"] == 3 and s["last_relres"] <= 1e-10, s
+    phi_loop = ctx.field(L.F_PHI)
+    it, rr = ctx.solve(L.WARM_ZERO)
+    assert it == 1 and rr <= 1e-10
+    assert np.array_equal(ctx.field(L.F_PHI).view(np.uint64), phi_loop.view(np.uint64))
+    # an unreachable tolerance: the call reports the miss, the next sync is clean
+    ctx.poisson(L.POISSON_SPECTRAL, eps0, tags.ravel(), rtol=1e-30)
+    with pytest.raises(IskraError) as ei:
+        ctx.step(1e-12, 2)
+    assert ei.value.code == L.E_CONVERGE
+    ctx.sync()
+    ctx.close()

@@ -1,0 +1,83 @@
+"""Physics checks of the whole GPU step against closed forms (no oracle):
+the 1-D two-stream instability of the 64-cell configuration grows at the
+cold-beam linear-theory rate, the step conserves energy through it, and a
+capacitively coupled discharge with MCC is bitwise reproducible run to run."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _two_stream_run(steps, particles=20000):
+    from iskra_amd import Context, _lib as L, problems as P
+    p = P.two_stream_1d(particles=particles)
+    ctx = Context(0)
+    ctx.grid(1, (p["nodes"],), (p["dx"],))
+    n = len(p["x"])
+    e = ctx.species(p["q"], p["m"], p["w0"], n, track_ids=True)
+    ctx.boundary(e, [L.BND_WRAP])
+    ctx.upload(e, p["x"], p["v"])
+    ctx.set_field(L.F_BG, np.full(p["nodes"], p["bg"]))
+    ctx.poisson(L.POISSON_DIRECT, p["eps0"], None, periodic_x=True)
+    amp, wf, wk = [], [], []
+    for _ in range(steps):
+        ctx.step(p["dt"], 1)
+        E = ctx.field(L.F_EX)
+        Ei = E[:-1]  # the periodic cells (node n-1 repeats node 0 in the wrap period)
+        amp.append(np.abs(np.fft.rfft(Ei)[1]) / len(Ei))
+        wf.append(0.5 * p["eps0"] * np.sum(Ei ** 2) * p["dx"])
+        X, V, _ = ctx.download(e)
+        wk.append(0.5 * p["m"] * p["w0"] * np.sum(V[:, 0] ** 2))
+    ctx.close()
+    return p, np.array(amp), np.array(wf), np.array(wk)
+
+
+def test_two_stream_growth_rate_matches_linear_theory(gpu_ready):
+    p, amp, wf, wk = _two_stream_run(500)
+    # cold beams of density n0/2 at +-v0: 1 = (wp^2/2)/(w - k v0)^2 + (wp^2/2)/(w + k v0)^2
+    x = p["k"] * 1.0 / p["wp"]  # k v0 / wp (v0 = 1)
+    W = ((2 * x * x + 1) - np.sqrt(8 * x * x + 1)) / 2
+    gamma = np.sqrt(-W) * p["wp"]
+    t = (np.arange(len(amp)) + 1) * p["dt"]
+    la = np.log(amp)
+    sat = int(np.argmax(la))
+    # linear phase: from half an e-fold above the start to 1.5 e-folds below the peak
+    lo = int(np.argmax(la > la[:10].mean() + 0.5))
+    hi = int(np.argmax(la > la[sat] - 1.5))
+    assert hi - lo > 20, (lo, hi, sat)
+    g = np.polyfit(t[lo:hi], la[lo:hi], 1)[0]
+    print("growth rate", g, "theory", gamma, "window", lo, hi, "peak", sat)
+    assert abs(g / gamma - 1.0) < 0.15
+
+
+def test_two_stream_energy_drift(gpu_ready):
+    p, amp, wf, wk = _two_stream_run(400)
+    tot = wf + wk
+    drift = np.max(np.abs(tot - tot[0])) / tot[0]
+    # field energy grows from ~0 to a sizeable fraction of the beam energy
+    assert wf.max() > 0.01 * wk[0]
+    print("max relative energy drift", drift, "peak field/kinetic", wf.max() / wk[0])
+    # explicit PIC with cold beams: a few per cent of exchange error; bound 10 %
+    assert drift < 0.10
+
+
+def test_ccp_runs_are_bitwise_reproducible(gpu_ready):
+    from iskra_amd import ensemble as E, _lib as L
+    inst = dict(cells=512, particles=100000, volts=200.0, torr=1.0, seed=11)
+    runs = []
+    for _ in range(2):
+        c, dt = E.ccp_context(inst)
+        c.step(dt, 400)
+        st = [c.get_state(s) for s in (0, 1)]
+        runs.append((st, c.field(L.F_PHI), c.mcc_counts(0, 3), c.stats()))
+        c.close()
+    (sa, pa, na, ta), (sb, pb, nb, tb) = runs
+    for s in (0, 1):
+        for k in ("cell", "ox", "vx", "vy", "vz", "id"):
+            assert np.array_equal(sa[s][k].view(np.uint32), sb[s][k].view(np.uint32)), (s, k)
+    assert np.array_equal(pa, pb) and np.array_equal(na, nb)
+    assert na[2].sum() > 0  # ionisation happened
+    # the electrode value is set after each solve with t = (it-1) dt
+    # (11_rf_discharge.jl:95), so the last solve (step 400) used V(398 dt)
+    assert pa[0] == pytest.approx(200.0 * np.sin(2 * np.pi * 13.56e6 * 398 * dt), abs=1e-9)
+    assert ta["steps"] == 400
