@@ -1103,13 +1103,18 @@ int iskra_step(iskra_ctx *c, double dt, int64_t nsteps) {
     RC(isk_push_deposit_all(c, dt));
     int iters = 0;
     double rr = 0.0;
+    int E_done = 0;
     const bool async = c->ps.kind == ISKRA_POISSON_SPECTRAL;
     RC(isk_prof_begin(c, &ev));
-    RC(isk_poisson_solve(c, ISKRA_WARM_PREV, async ? nullptr : &iters, async ? nullptr : &rr));
+    /* the deposit kernel above packed the spectral right-hand side */
+    RC(isk_poisson_solve_ex(c, ISKRA_WARM_PREV, async ? nullptr : &iters,
+                            async ? nullptr : &rr, 1, &E_done, async ? 1 : 0));
     RC(isk_prof_end(c, ev, ISKRA_K_POISSON, 0));
-    RC(isk_prof_begin(c, &ev));
-    RC(isk_efield(c));
-    RC(isk_prof_end(c, ev, ISKRA_K_EFIELD, 0));
+    if (!E_done) {
+      RC(isk_prof_begin(c, &ev));
+      RC(isk_efield(c));
+      RC(isk_prof_end(c, ev, ISKRA_K_EFIELD, 0));
+    }
     /* after_loop: electrode values for the next step at t = (it-1) dt */
     double t = (double)c->step_count * dt;
     for (int k = 1; k <= ISK_MAX_TAGS; ++k)

@@ -270,6 +270,10 @@ int isk_poisson_setup(iskra_ctx *c);
 int isk_poisson_free(iskra_ctx *c);
 int isk_poisson_solve(iskra_ctx *c, int warm, int *iters, double *relres);
 int isk_poisson_check(iskra_ctx *c, double *relres);
+int isk_poisson_solve_ex(iskra_ctx *c, int warm, int *iters, double *relres, int with_E,
+                         int *E_done, int packed);
+struct IskSpPack;
+int isk_poisson_pack_desc(iskra_ctx *c, IskSpPack *pk);
 int isk_efield(iskra_ctx *c);
 int isk_mcc_setup(iskra_ctx *c, int id);
 int isk_mcc_run(iskra_ctx *c, int id, double dt, int64_t *events);
@@ -301,6 +305,60 @@ struct IskBLaunch {
   size_t shm = 0;         /* dynamic LDS bytes (maximum over instances) */
   long long *step_dev = nullptr; /* device step counter of the batch */
 };
+/* the finite-difference Poisson operator on the node grid: periodic
+ * couplings cxp / cyp, node tags (0 free, k > 0 Dirichlet with tagval[k]) */
+struct Sten {
+  int dim, nx, ny, per_x, per_y;
+  double cxp, cyp;
+  const int8_t *tags;
+  const double *tagval;
+};
+
+/* sum over the Dirichlet neighbours of c * value (moves known values to the rhs) */
+__device__ __forceinline__ double dirichlet_rhs(const Sten &s, int i, int j) {
+  int n = j * s.nx + i;
+  double acc = 0.0;
+#define DIRNB(cond, m, coef)                                      \
+  if (cond) {                                                    \
+    int t = s.tags[m];                                           \
+    if (t) acc += (coef) * s.tagval[t];                          \
+  }
+  DIRNB(i > 0, n - 1, 1.0)
+  DIRNB(i < s.nx - 1, n + 1, 1.0)
+  if (s.per_x) {
+    DIRNB(i == 0, n + s.nx - 1, s.cxp)
+    DIRNB(i == s.nx - 1, n - (s.nx - 1), s.cxp)
+  }
+  if (s.dim == 2) {
+    DIRNB(j > 0, n - s.nx, 1.0)
+    DIRNB(j < s.ny - 1, n + s.nx, 1.0)
+    if (s.per_y) {
+      DIRNB(j == 0, n + (s.ny - 1) * s.nx, s.cyp)
+      DIRNB(j == s.ny - 1, n - (s.ny - 1) * s.nx, s.cyp)
+    }
+  }
+#undef DIRNB
+  return acc;
+}
+
+/* the spectral solve's right-hand side, packed over the interior nodes
+ * (row-major [j-1][i-1]): F = rho scale + Dirichlet neighbours; written by
+ * the time loop's deposit kernel when F is not null */
+struct IskSpPack {
+  double *F;
+  Sten s;
+  double scale;
+};
+
+/* work-group barrier that orders LDS accesses only: global loads still in
+ * flight stay in flight across it (a __syncthreads after a global store
+ * waits for every outstanding global access of the wave, loads included) */
+__device__ __forceinline__ void isk_lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 __device__ __forceinline__ int isk_bfind(const int *bstart, int n, int g) {
   int lo = 0, hi = n; /* largest i with bstart[i] <= g */
   while (hi - lo > 1) {

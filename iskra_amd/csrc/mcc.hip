@@ -148,28 +148,72 @@ __device__ void randn3(Draws &d, double *n) {
   n[2] = r2 * c2;
 }
 
+/* Two phases per work-group.  Every live particle draws (Uc, U) from
+ * Philox block 0 of its stream; the candidates (Uc < candP) are listed in
+ * LDS in particle order, and the candidate work is then done by the first
+ * threads of the group, one candidate each, so that waves run it with all
+ * lanes busy instead of one or two lanes of each wave.  Each candidate's
+ * draws continue at Philox block 1, as they would in one pass, and products
+ * keep particle order (the list is in particle order). */
 __device__ __forceinline__ void mcc_body(const MccArgs &a, const int blk) {
   __shared__ int wcount[MCC_BLOCK / 64];
+  __shared__ int cand_t[MCC_BLOCK];
+  __shared__ double cand_u[MCC_BLOCK];
   const long long np = a.hdr[HDR_NP];
-  const long long p = (long long)blk * MCC_BLOCK + threadIdx.x;
-  int prod = 0;
-  float pe[3] = {0.f, 0.f, 0.f}, pi[3] = {0.f, 0.f, 0.f};
-  int32_t pcell = 0;
-  float pox = 0.f, poy = 0.f;
-  if (p < np) {
-    int32_t c = a.cell[p];
-    if (c >= 0) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const uint32_t step = a.step_dev ? (uint32_t)(a.step + *a.step_dev) : a.step;
+  bool cand = false;
+  double U1 = 0.0;
+  {
+    const long long p = (long long)blk * MCC_BLOCK + threadIdx.x;
+    if (p < np && a.cell[p] >= 0) {
       Draws d;
       d.id = a.id[p];
-      d.step = a.step_dev ? (uint32_t)(a.step + *a.step_dev) : a.step;
+      d.step = step;
       d.stream = a.stream;
       d.k0 = a.k0;
       d.k1 = a.k1;
       d.block = 0;
       d.half = 0;
-      double Uc = d.next();
-      double U = d.next();
-      if (Uc < a.candP) {
+      const double Uc = d.next();
+      U1 = d.next();
+      cand = Uc < a.candP;
+    }
+  }
+  {
+    const unsigned long long m = __ballot(cand);
+    const int rank = __popcll(m & ((1ull << lane) - 1ull));
+    if (lane == 0) wcount[w] = __popcll(m);
+    __syncthreads();
+    int off = 0;
+    for (int k = 0; k < w; ++k) off += wcount[k];
+    if (cand) {
+      cand_t[off + rank] = threadIdx.x;
+      cand_u[off + rank] = U1;
+    }
+  }
+  int ncand = 0;
+  for (int k = 0; k < MCC_BLOCK / 64; ++k) ncand += wcount[k];
+  __syncthreads();
+  int prod = 0;
+  float pe[3] = {0.f, 0.f, 0.f}, pi[3] = {0.f, 0.f, 0.f};
+  int32_t pcell = 0;
+  float pox = 0.f, poy = 0.f;
+  int evkey = -1; /* node * N + process of this thread's collision */
+  if ((int)threadIdx.x < ncand) {
+    const long long p = (long long)blk * MCC_BLOCK + cand_t[threadIdx.x];
+    const double U = cand_u[threadIdx.x];
+    const int32_t c = a.cell[p];
+    Draws d;
+    d.id = a.id[p];
+    d.step = step;
+    d.stream = a.stream;
+    d.k0 = a.k0;
+    d.k1 = a.k1;
+    d.block = 1;
+    d.half = 0;
+    {
+      {
         int i = c & 0xFFFF, j = c >> 16;
         int node = a.dim == 2 ? j * a.nx + i : i;
         double n = a.tn[node];
@@ -195,8 +239,7 @@ __device__ __forceinline__ void mcc_body(const MccArgs &a, const int blk) {
           P /= a.candP;
           if (P > 1.0) atomicOr(a.errflag, 1);
           if (U > (double)(k + 1) / (double)N - P) {
-            atomicAdd(&a.nu[(size_t)node * N + k], 1ull);
-            atomicAdd(&a.events[k], 1ull);
+            evkey = node * N + k;
             const int type = a.types[k];
             double sv[3] = {vx, vy, vz};
             double out[3];
@@ -263,9 +306,30 @@ __device__ __forceinline__ void mcc_body(const MccArgs &a, const int blk) {
       }
     }
   }
+  /* event counts: equal (node, process) keys of a wave are counted with one
+   * atomic, the per-process totals of the work-group go through LDS (integer
+   * sums, so the grouping does not change them) */
+  __shared__ unsigned int evl[ISK_MAX_COLL];
+  if (threadIdx.x < ISK_MAX_COLL) evl[threadIdx.x] = 0u;
+  __syncthreads();
+  {
+    unsigned long long todo = __ballot(evkey >= 0);
+    while (todo) {
+      const int leader = __ffsll((long long)todo) - 1;
+      const int lk = __shfl(evkey, leader);
+      const unsigned long long mk = __ballot(evkey == lk);
+      if (lane == leader) {
+        atomicAdd(&a.nu[lk], (unsigned long long)__popcll(mk));
+        atomicAdd(&evl[lk % a.ncoll], (unsigned int)__popcll(mk));
+      }
+      todo &= ~mk;
+    }
+  }
+  __syncthreads();
+  if ((int)threadIdx.x < a.ncoll && evl[threadIdx.x])
+    atomicAdd(&a.events[threadIdx.x], (unsigned long long)evl[threadIdx.x]);
   /* deterministic block compaction of the ionisation products */
   unsigned long long m = __ballot(prod != 0);
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   int rank = __popcll(m & ((1ull << lane) - 1ull));
   if (lane == 0) wcount[w] = __popcll(m);
   __syncthreads();

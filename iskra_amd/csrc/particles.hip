@@ -36,6 +36,7 @@
 #define SORT_PT 32
 #define SORT_CHUNK (SORT_BLOCK * SORT_PT)
 #define SORT_SEG 32
+static_assert(SORT_PT % 2 == 0, "the scatter handles rounds in pairs");
 
 /* ------------------------------------------------------------------------ */
 /* small utility kernels                                                     */
@@ -282,17 +283,30 @@ __device__ void sort_hist_body(const int32_t *cell, const long long *hdr, int *H
   const long long np = hdr[HDR_NP];
   const long long base = (long long)blk * SORT_CHUNK;
   const int lane = threadIdx.x & 63;
-  for (int r = 0; r < SORT_PT; ++r) {
-    long long p = base + (long long)r * SORT_BLOCK + threadIdx.x;
-    bool valid = p < np;
-    int key = valid ? sort_key(cell[p], kd) : -1;
-    unsigned long long todo = __ballot(valid);
-    while (todo) {
-      int leader = __ffsll((long long)todo) - 1;
-      int lk = __shfl(key, leader);
-      unsigned long long m = __ballot(valid && key == lk);
-      if (lane == leader) atomicAdd(&sh[lk], (int)__popcll(m));
-      todo &= ~m;
+  /* keys are loaded 16 rounds at a time (index clamped to the last particle,
+   * validity applied at use), so 16 loads per lane are in flight at once */
+  if (np > 0 && base < np) {
+    for (int r0 = 0; r0 < SORT_PT; r0 += 16) {
+      int cv[16];
+#pragma unroll
+      for (int u = 0; u < 16; ++u) {
+        const long long p = base + (long long)(r0 + u) * SORT_BLOCK + threadIdx.x;
+        cv[u] = cell[p < np ? p : np - 1];
+      }
+#pragma unroll
+      for (int u = 0; u < 16; ++u) {
+        const long long p = base + (long long)(r0 + u) * SORT_BLOCK + threadIdx.x;
+        const bool valid = p < np;
+        const int key = valid ? sort_key(cv[u], kd) : -1;
+        unsigned long long todo = __ballot(valid);
+        while (todo) {
+          int leader = __ffsll((long long)todo) - 1;
+          int lk = __shfl(key, leader);
+          unsigned long long m = __ballot(valid && key == lk);
+          if (lane == leader) atomicAdd(&sh[lk], (int)__popcll(m));
+          todo &= ~m;
+        }
+      }
     }
   }
   __syncthreads();
@@ -386,49 +400,100 @@ struct ScatterArgs {
   KeyDesc kd;
 };
 
-/* stable scatter: particles of a block are ranked in index order; within a
- * round of 256, lanes of a wave are matched by key with ballots and waves are
- * ordered through LDS group lists. */
+/* Stable scatter of one work-group's SORT_CHUNK particles, in three phases:
+ * (1) the keys of all its rounds go to registers and a work-group histogram;
+ * (2) rounds of SORT_BLOCK particles are ranked in index order (ballots
+ *     within a wave, LDS group lists across waves), which gives every kept
+ *     particle its slot in the work-group's key-ordered output: perm[slot] =
+ *     particle, skey[slot] = key;
+ * (3) the records are written slot by slot, so a key's run of the
+ *     work-group is written contiguously instead of a few elements per round.
+ * LDS: goff (global position of the work-group's first particle of a key,
+ * minus its local slot), lrun (local slot counter per key), perm, skey. */
+static inline size_t sort_scatter_shm(int nb1) {
+  return (size_t)2 * nb1 * sizeof(int) + (size_t)2 * SORT_CHUNK * sizeof(uint16_t);
+}
+
 __device__ void sort_scatter_body(const ScatterArgs &a, const int blk) {
-  extern __shared__ int base[];
+  extern __shared__ int sm[];
   __shared__ int gkey[4][64];
   __shared__ int gcnt[4][64];
   __shared__ int ng[4];
+  __shared__ int wsum[SORT_BLOCK / 64];
   const int nb1 = a.kd.nbins + 1;
+  int *goff = sm;
+  int *lrun = sm + nb1;
+  uint16_t *perm = reinterpret_cast<uint16_t *>(sm + 2 * nb1);
+  uint16_t *skey = perm + SORT_CHUNK;
   const int seg = blk / SORT_SEG;
-  for (int b = threadIdx.x; b < nb1; b += SORT_BLOCK)
-    base[b] = a.start[b] + a.S[(size_t)seg * nb1 + b] + a.H[(size_t)blk * nb1 + b];
+  for (int b = threadIdx.x; b < nb1; b += SORT_BLOCK) {
+    goff[b] = a.start[b] + a.S[(size_t)seg * nb1 + b] + a.H[(size_t)blk * nb1 + b];
+    lrun[b] = 0;
+  }
   __syncthreads();
   const long long np = a.hdr[HDR_NP];
   const long long b0 = (long long)blk * SORT_CHUNK;
+  if (np <= 0 || b0 >= np) return;
+  const int nloc = (int)min((long long)SORT_CHUNK, np - b0);
   const int lane = threadIdx.x & 63;
   const int w = threadIdx.x >> 6;
+  /* phase 1 */
+  int key[SORT_PT];
+#pragma unroll
   for (int r = 0; r < SORT_PT; ++r) {
-    long long p = b0 + (long long)r * SORT_BLOCK + threadIdx.x;
-    bool valid = p < np;
-    /* the payload is loaded together with the key, so its latency overlaps
-     * the ranking below instead of following it */
-    int c = -1;
-    float fox = 0.f, foy = 0.f, fvx = 0.f, fvy = 0.f, fvz = 0.f;
-    uint32_t fid = 0;
-    if (valid) {
-      c = a.cell[p];
-      fox = a.ox[p];
-      if (a.kd.dim == 2) foy = a.oy[p];
-      fvx = a.vx[p];
-      fvy = a.vy[p];
-      fvz = a.vz[p];
-      if (a.id) fid = a.id[p];
+    const int q = r * SORT_BLOCK + threadIdx.x;
+    key[r] = a.cell[b0 + (q < nloc ? q : nloc - 1)];
+  }
+#pragma unroll
+  for (int r = 0; r < SORT_PT; ++r) {
+    const int q = r * SORT_BLOCK + threadIdx.x;
+    const bool valid = q < nloc;
+    key[r] = valid ? sort_key(key[r], a.kd) : -1;
+    unsigned long long todo = __ballot(valid);
+    while (todo) {
+      const int leader = __ffsll((long long)todo) - 1;
+      const int lk = __shfl(key[r], leader);
+      const unsigned long long m = __ballot(valid && key[r] == lk);
+      if (lane == leader) atomicAdd(&lrun[lk], (int)__popcll(m));
+      todo &= ~m;
     }
-    int key = valid ? sort_key(c, a.kd) : -1;
+  }
+  __syncthreads();
+  /* exclusive scan of the counts in place (each thread a contiguous range) */
+  const int per = (nb1 + SORT_BLOCK - 1) / SORT_BLOCK;
+  const int lo = threadIdx.x * per, hi = min(lo + per, nb1);
+  int sum = 0;
+  for (int b = lo; b < hi; ++b) sum += lrun[b];
+  int inc = sum;
+  for (int off = 1; off < 64; off <<= 1) {
+    const int v = __shfl_up(inc, off);
+    if (lane >= off) inc += v;
+  }
+  if (lane == 63) wsum[w] = inc;
+  __syncthreads();
+  int run = inc - sum;
+  for (int k = 0; k < w; ++k) run += wsum[k];
+  for (int b = lo; b < hi; ++b) {
+    const int v = lrun[b];
+    lrun[b] = run;
+    goff[b] -= run;
+    run += v;
+  }
+  __syncthreads();
+  const int kept = lrun[a.kd.nbins]; /* discarded particles (last bin) are dropped */
+  /* phase 2 */
+#pragma unroll
+  for (int r = 0; r < SORT_PT; ++r) {
+    const int q = r * SORT_BLOCK + threadIdx.x;
+    const bool valid = q < nloc;
     unsigned long long todo = __ballot(valid);
     int myrank = 0;
     int ngr = 0;
     while (todo) {
-      int leader = __ffsll((long long)todo) - 1;
-      int lk = __shfl(key, leader);
-      unsigned long long m = __ballot(valid && key == lk);
-      if (valid && key == lk) myrank = __popcll(m & ((1ull << lane) - 1ull));
+      const int leader = __ffsll((long long)todo) - 1;
+      const int lk = __shfl(key[r], leader);
+      const unsigned long long m = __ballot(valid && key[r] == lk);
+      if (valid && key[r] == lk) myrank = __popcll(m & ((1ull << lane) - 1ull));
       if (lane == 0) {
         gkey[w][ngr] = lk;
         gcnt[w][ngr] = __popcll(m);
@@ -438,25 +503,32 @@ __device__ void sort_scatter_body(const ScatterArgs &a, const int blk) {
     }
     if (lane == 0) ng[w] = ngr;
     __syncthreads();
-    if (valid && key < a.kd.nbins) {
+    if (valid && key[r] < a.kd.nbins) {
       int pre = 0;
       for (int w2 = 0; w2 < w; ++w2) {
-        int n2 = ng[w2];
+        const int n2 = ng[w2];
         for (int g = 0; g < n2; ++g)
-          if (gkey[w2][g] == key) pre += gcnt[w2][g];
+          if (gkey[w2][g] == key[r]) pre += gcnt[w2][g];
       }
-      long long d = (long long)base[key] + pre + myrank;
-      a.cell_o[d] = c;
-      a.ox_o[d] = fox;
-      if (a.kd.dim == 2) a.oy_o[d] = foy;
-      a.vx_o[d] = fvx;
-      a.vy_o[d] = fvy;
-      a.vz_o[d] = fvz;
-      if (a.id) a.id_o[d] = fid;
+      const int slot = lrun[key[r]] + pre + myrank;
+      perm[slot] = (uint16_t)q;
+      skey[slot] = (uint16_t)key[r];
     }
     __syncthreads();
-    if (lane < ng[w]) atomicAdd(&base[gkey[w][lane]], gcnt[w][lane]);
+    if (lane < ng[w]) atomicAdd(&lrun[gkey[w][lane]], gcnt[w][lane]);
     __syncthreads();
+  }
+  /* phase 3 */
+  for (int s = threadIdx.x; s < kept; s += SORT_BLOCK) {
+    const long long src = b0 + perm[s];
+    const long long d = (long long)goff[skey[s]] + s;
+    a.cell_o[d] = a.cell[src];
+    a.ox_o[d] = a.ox[src];
+    if (a.kd.dim == 2) a.oy_o[d] = a.oy[src];
+    a.vx_o[d] = a.vx[src];
+    a.vy_o[d] = a.vy[src];
+    a.vz_o[d] = a.vz[src];
+    if (a.id) a.id_o[d] = a.id[src];
   }
 }
 
@@ -627,8 +699,17 @@ static int sort_pass(iskra_ctx *c, int sid, int mode, int build) {
   hipLaunchKernelGGL(k_sort_binscan, dim3(1), dim3(1024), 0, c->stream,
                      (const int *)A.tot, A.start, nb1);
   ISK_CHECK_LAUNCH();
-  hipLaunchKernelGGL(k_sort_scatter, dim3(nblk), dim3(SORT_BLOCK), shm,
-                     c->stream, A.sc);
+  const size_t sshm = sort_scatter_shm(nb1);
+  if (sshm > 65536) {
+    static bool raised = false;
+    if (!raised) {
+      ISK_HIP(hipFuncSetAttribute((const void *)k_sort_scatter,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)sort_scatter_shm(ISK_MAX_BINS + 1)));
+      raised = true;
+    }
+  }
+  hipLaunchKernelGGL(k_sort_scatter, dim3(nblk), dim3(SORT_BLOCK), sshm, c->stream, A.sc);
   ISK_CHECK_LAUNCH();
   hipLaunchKernelGGL(k_sort_finish, dim3(1), dim3(1024), 0, c->stream, A.hdr_w,
                      (const int *)A.tot, (const int *)A.start, A.kd.nbins, build, A.CH,
@@ -713,25 +794,27 @@ struct PushRegs {
   float hx[PUSH_UNROLL], hy[PUSH_UNROLL], vx[PUSH_UNROLL], vy[PUSH_UNROLL];
 };
 
+/* pointers are the work-group's first particle (uniform), indices 32-bit */
 template <int DIM>
-__device__ __forceinline__ void push_load(PushRegs<DIM> &r, const int32_t *__restrict__ cellp,
-                                          const float *__restrict__ oxp,
-                                          const float *__restrict__ oyp,
-                                          const float *__restrict__ vxp,
-                                          const float *__restrict__ vyp, long long start,
-                                          long long count, long long b0) {
-  /* branch-free: slots past the end load particle `start` and are marked -2 */
+__device__ __forceinline__ void push_load(PushRegs<DIM> &r, const int32_t *__restrict__ cb,
+                                          const float *__restrict__ oxb,
+                                          const float *__restrict__ oyb,
+                                          const float *__restrict__ vxb,
+                                          const float *__restrict__ vyb, unsigned count,
+                                          unsigned b0) {
+  /* branch-free: slots past the end load the first particle (the caller
+   * skips them); nothing here consumes a loaded value, so all loads of the
+   * batch are in flight together */
 #pragma unroll
   for (int u = 0; u < PUSH_UNROLL; ++u) {
-    long long k = b0 + u * PUSH_BLOCK + threadIdx.x;
-    bool in = k < count;
-    long long p = start + (in ? k : 0);
-    int32_t c = cellp[p];
-    r.c[u] = in ? c : -2;
-    r.hx[u] = oxp[p];
-    r.vx[u] = vxp[p];
-    r.hy[u] = DIM == 2 ? oyp[p] : 0.f;
-    r.vy[u] = DIM == 2 ? vyp[p] : 0.f;
+    const unsigned k = b0 + u * PUSH_BLOCK + threadIdx.x;
+    __builtin_assume(k < (1u << 24));
+    const unsigned q = k < count ? k : 0u;
+    r.c[u] = cb[q];
+    r.hx[u] = oxb[q];
+    r.vx[u] = vxb[q];
+    r.hy[u] = DIM == 2 ? oyb[q] : 0.f;
+    r.vy[u] = DIM == 2 ? vyb[q] : 0.f;
   }
 }
 
@@ -775,8 +858,14 @@ __device__ __forceinline__ void push_body(const PushArgs &a, const int blk) {
   }
   /* the first batch of particles is requested before the E window, so the
    * two memory latencies overlap instead of following each other */
+  int32_t *__restrict__ cb = cellp + start;
+  float *__restrict__ oxb = oxp + start;
+  float *__restrict__ oyb = DIM == 2 ? oyp + start : oyp;
+  float *__restrict__ vxb = vxp + start;
+  float *__restrict__ vyb = DIM == 2 ? vyp + start : vyp;
+  const unsigned ucount = (unsigned)count;
   PushRegs<DIM> cur;
-  push_load<DIM>(cur, cellp, oxp, oyp, vxp, vyp, start, count, 0);
+  push_load<DIM>(cur, cb, oxb, oyb, vxb, vyb, ucount, 0u);
   if (windowed) {
     for (int k = threadIdx.x; k < WW; k += PUSH_BLOCK) {
       sD[k] = 0ull;
@@ -801,9 +890,11 @@ __device__ __forceinline__ void push_body(const PushArgs &a, const int blk) {
   const long long per = (long long)PUSH_BLOCK * PUSH_UNROLL;
   for (long long b0 = 0; b0 < count; b0 += per) {
     /* a batch of PUSH_UNROLL particles per thread: all loads, then compute + stores */
-    if (b0 > 0) push_load<DIM>(cur, cellp, oxp, oyp, vxp, vyp, start, count, b0);
+    if (b0 > 0) push_load<DIM>(cur, cb, oxb, oyb, vxb, vyb, ucount, (unsigned)b0);
 #pragma unroll
     for (int u = 0; u < PUSH_UNROLL; ++u) {
+      const unsigned kslot = (unsigned)b0 + u * PUSH_BLOCK + threadIdx.x;
+      if (kslot >= ucount) continue;
       int32_t c = cur.c[u];
       if (c < 0) continue;
       int i = c & 0xFFFF;
@@ -891,13 +982,13 @@ __device__ __forceinline__ void push_body(const PushArgs &a, const int blk) {
         }
       }
       if (a.do_push) {
-        long long p = start + b0 + u * PUSH_BLOCK + threadIdx.x;
-        cellp[p] = alive ? (i | (j << 16)) : -1;
-        oxp[p] = hx;
-        vxp[p] = vx;
+        const unsigned k = (unsigned)b0 + u * PUSH_BLOCK + threadIdx.x;
+        cb[k] = alive ? (i | (j << 16)) : -1;
+        oxb[k] = hx;
+        vxb[k] = vx;
         if (DIM == 2) {
-          oyp[p] = hy;
-          vyp[p] = vy;
+          oyb[k] = hy;
+          vyb[k] = vy;
         }
       }
     }
@@ -987,6 +1078,48 @@ static int push_launch(iskra_ctx *c, int sid, double dt, int do_push, int fill) 
     if (rc2) return rc2;
   }
   return ISKRA_OK;
+}
+
+/* the pushes of all species in one launch: work-group g belongs to species
+ * s with bstart[s] <= g < bstart[s+1] and acts as work-group g - bstart[s]
+ * of that species' own launch (one launch boundary less per step, and the
+ * tail of one species overlaps the next) */
+struct PushMulti {
+  PushArgs a[ISK_MAX_SPECIES];
+  int bstart[ISK_MAX_SPECIES + 1];
+  int n;
+};
+
+template <int DIM>
+__global__ __launch_bounds__(PUSH_BLOCK) void k_push_multi(PushMulti m) {
+  int s = 0;
+  while (s + 1 < m.n && (int)blockIdx.x >= m.bstart[s + 1]) ++s;
+  push_body<DIM>(m.a[s], blockIdx.x - m.bstart[s]);
+}
+
+static int push_launch_all(iskra_ctx *c, double dt) {
+  if (c->nspecies == 0) return ISKRA_OK;
+  PushMulti m;
+  size_t shm = 0;
+  m.n = c->nspecies;
+  m.bstart[0] = 0;
+  for (int s = 0; s < c->nspecies; ++s) {
+    int grid;
+    size_t sh;
+    push_prepare(c, s, dt, 1, c->sp[s].cur, &m.a[s], &grid, &sh);
+    m.bstart[s + 1] = m.bstart[s] + grid;
+    shm = std::max(shm, sh);
+  }
+  if (m.bstart[c->nspecies] == 0) return ISKRA_OK;
+  hipEvent_t ev = nullptr;
+  int rc = isk_prof_begin(c, &ev);
+  if (rc) return rc;
+  if (c->g.dim == 2)
+    hipLaunchKernelGGL(k_push_multi<2>, dim3(m.bstart[m.n]), dim3(PUSH_BLOCK), shm, c->stream, m);
+  else
+    hipLaunchKernelGGL(k_push_multi<1>, dim3(m.bstart[m.n]), dim3(PUSH_BLOCK), shm, c->stream, m);
+  ISK_CHECK_LAUNCH();
+  return isk_prof_end(c, ev, ISKRA_K_PUSH, 0);
 }
 
 int isk_push_species(iskra_ctx *c, int sid, double dt, int do_push) {
@@ -1213,6 +1346,7 @@ __global__ __launch_bounds__(256) void k_merge_tiles_n(MergeTilesN a) {
 struct MergeRhoArgs {
   MergeArgs m[ISK_MAX_SPECIES];
   RhoArgs r;
+  IskSpPack pk; /* F != null: also the spectral solve's packed right-hand side */
 };
 
 template <int DIM>
@@ -1228,7 +1362,13 @@ __global__ __launch_bounds__(256) void k_merge_rho(MergeRhoArgs a) {
     a.m[s].cnt[node] = cnt;
     r = rho_term(r, cnt, a.r.qw[s], a.r.q[s], V);
   }
-  a.r.rho[node] = r + a.r.bg[node];
+  const double rho = r + a.r.bg[node];
+  a.r.rho[node] = rho;
+  if (DIM == 2 && a.pk.F) {
+    const int nx = a.pk.s.nx, i = node % nx, j = node / nx;
+    if (i > 0 && j > 0 && i < nx - 1 && j < a.pk.s.ny - 1)
+      a.pk.F[(j - 1) * (nx - 2) + (i - 1)] = rho * a.pk.scale + dirichlet_rhs(a.pk.s, i, j);
+  }
 }
 
 int isk_rho(iskra_ctx *c) {
@@ -1258,8 +1398,7 @@ int isk_rho(iskra_ctx *c) {
  * one launch and counts + rho in another */
 int isk_push_deposit_all(iskra_ctx *c, double dt) {
   int rc;
-  for (int s = 0; s < c->nspecies; ++s)
-    if ((rc = push_launch(c, s, dt, 1, 0))) return rc;
+  if ((rc = push_launch_all(c, dt))) return rc;
   hipEvent_t em = nullptr;
   if ((rc = isk_prof_begin(c, &em))) return rc;
   MergeRhoArgs a;
@@ -1276,6 +1415,7 @@ int isk_push_deposit_all(iskra_ctx *c, double dt) {
   a.r.bg = c->bg;
   a.r.rho = c->rho;
   a.r.nn = isk_nodes(c);
+  isk_poisson_pack_desc(c, &a.pk);
   if (c->g.dim == 2 && c->nspecies > 0) {
     MergeTilesN t;
     for (int s = 0; s < c->nspecies; ++s) {
@@ -1501,7 +1641,7 @@ int isk_bbuild_sort(iskra_ctx **cs, int n, int sid, const int *cur, IskBLaunch L
                                0)))
     return rc;
   if ((rc = isk_blaunch_upload(&L[4], ISK_B_SORT_SCATTER, A.data(), sizeof(SortArgs), bh,
-                               SORT_BLOCK, shm)))
+                               SORT_BLOCK, sort_scatter_shm((int)(shm / sizeof(int))))))
     return rc;
   return isk_blaunch_upload(&L[5], ISK_B_SORT_FINISH, A.data(), sizeof(SortArgs), one, 1024, 0);
 }
@@ -1535,6 +1675,15 @@ int isk_blaunch_particles(const IskBLaunch &L, hipStream_t st) {
       hipLaunchKernelGGL(kb_sort_binscan, g, b, 0, st, (const SortArgs *)L.args, L.bstart, L.n);
       break;
     case ISK_B_SORT_SCATTER:
+      if (L.shm > 65536) {
+        static bool raised = false;
+        if (!raised) {
+          ISK_HIP(hipFuncSetAttribute((const void *)kb_sort_scatter,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)sort_scatter_shm(ISK_MAX_BINS + 1)));
+          raised = true;
+        }
+      }
       hipLaunchKernelGGL(kb_sort_scatter, g, b, L.shm, st, (const SortArgs *)L.args, L.bstart,
                          L.n);
       break;

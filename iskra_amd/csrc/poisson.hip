@@ -44,12 +44,6 @@ enum { S_RZ = 0, S_PAP, S_ALPHA, S_BETA, S_RR, S_FF, S_TOL2, S_MEAN, S_RELRES, S
  * [4] this solve's pass met rtol, [5] sticky: an in-loop solve missed rtol */
 enum { IF_SP_DONE = 4, IF_SP_FAIL = 5 };
 
-struct Sten {
-  int dim, nx, ny, per_x, per_y;
-  double cxp, cyp;
-  const int8_t *tags;
-  const double *tagval;
-};
 
 template <typename T>
 __device__ __forceinline__ double apply_L(const Sten &s, const T *v, int i, int j,
@@ -74,32 +68,28 @@ __device__ __forceinline__ double apply_L(const Sten &s, const T *v, int i, int 
   return diag * (double)v[n] - acc;
 }
 
-/* sum over the Dirichlet neighbours of c * value (moves known values to the rhs) */
-__device__ __forceinline__ double dirichlet_rhs(const Sten &s, int i, int j) {
-  int n = j * s.nx + i;
-  double acc = 0.0;
-#define DIRNB(cond, m, coef)                                      \
-  if (cond) {                                                    \
-    int t = s.tags[m];                                           \
-    if (t) acc += (coef) * s.tagval[t];                          \
+/* E = -grad phi at node n: central differences inside, one-sided at the ends */
+__device__ __forceinline__ void efield_node(const double *phi, float *E, int dim, int nx, int ny,
+                                            double dx, double dy, int n) {
+  const int i = n % nx, j = n / nx;
+  double ex;
+  if (nx < 2) ex = 0.0;
+  else if (i == 0) ex = (phi[n] - phi[n + 1]) / dx;
+  else if (i == nx - 1) ex = (phi[n - 1] - phi[n]) / dx;
+  else ex = (phi[n - 1] - phi[n + 1]) / (2.0 * dx);
+  if (dim == 1) {
+    E[n] = (float)ex;
+    return;
   }
-  DIRNB(i > 0, n - 1, 1.0)
-  DIRNB(i < s.nx - 1, n + 1, 1.0)
-  if (s.per_x) {
-    DIRNB(i == 0, n + s.nx - 1, s.cxp)
-    DIRNB(i == s.nx - 1, n - (s.nx - 1), s.cxp)
-  }
-  if (s.dim == 2) {
-    DIRNB(j > 0, n - s.nx, 1.0)
-    DIRNB(j < s.ny - 1, n + s.nx, 1.0)
-    if (s.per_y) {
-      DIRNB(j == 0, n + (s.ny - 1) * s.nx, s.cyp)
-      DIRNB(j == s.ny - 1, n - (s.ny - 1) * s.nx, s.cyp)
-    }
-  }
-#undef DIRNB
-  return acc;
+  double ey;
+  if (ny < 2) ey = 0.0;
+  else if (j == 0) ey = (phi[n] - phi[n + nx]) / dy;
+  else if (j == ny - 1) ey = (phi[n - nx] - phi[n]) / dy;
+  else ey = (phi[n - nx] - phi[n + nx]) / (2.0 * dy);
+  E[2 * n] = (float)ex;
+  E[2 * n + 1] = (float)ey;
 }
+
 
 /* block sum of up to 2 doubles; the last block to finish reduces the per-block
  * partials in index order and returns true (deterministic). */
@@ -209,9 +199,50 @@ __device__ __forceinline__ void tri_solve_body(const TriArgs &t) {
   for (int i = 0; i < n; ++i) t.phi[i] = t.phi[i] - m2;
 }
 
-__global__ void k_tri_solve(TriArgs t) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  tri_solve_body(t);
+/* The same solve for a whole work-group of 64: with n <= TRI_LDS_MAX and
+ * Dirichlet nodes (not the singular ring), the coefficients and the
+ * right-hand side are staged in LDS by all lanes, lane 0 runs the two
+ * recurrences out of LDS (the same fma sequence as above, so the same bits),
+ * and all lanes store phi.  Otherwise lane 0 runs tri_solve_body. */
+#define TRI_LDS_MAX 2048
+__device__ __forceinline__ void tri_solve_group(const TriArgs &t) {
+  const int n = t.n;
+  if (t.singular || n > TRI_LDS_MAX) {
+    if (threadIdx.x == 0) tri_solve_body(t);
+    return;
+  }
+  __shared__ double sa[TRI_LDS_MAX], sinv[TRI_LDS_MAX], scp[TRI_LDS_MAX], sr[TRI_LDS_MAX];
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const int tg = t.tags[i];
+    sr[i] = tg ? t.tagval[tg] : -(t.rho[i] * t.scale);
+    sa[i] = t.a[i];
+    sinv[i] = t.inv[i];
+    scp[i] = t.cp[i];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double yprev = 0.0;
+#pragma unroll 8
+    for (int i = 0; i < n; ++i) {
+      const double y = __builtin_fma(-sa[i], yprev, sr[i]) * sinv[i];
+      sr[i] = y;
+      yprev = y;
+    }
+    double xn = 0.0;
+#pragma unroll 8
+    for (int i = n - 1; i >= 0; --i) {
+      const double x = __builtin_fma(-scp[i], xn, sr[i]);
+      sr[i] = x;
+      xn = x;
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < n; i += blockDim.x) t.phi[i] = sr[i];
+}
+
+__global__ __launch_bounds__(64) void k_tri_solve(TriArgs t) {
+  if (blockIdx.x != 0) return;
+  tri_solve_group(t);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -230,6 +261,7 @@ struct PcgArgs {
   double scale;
   int warm;
   const double *phi_in;
+  double *phi_out; /* spectral solve: the residual kernels write phi here */
   int mgpre;
   int max_iter;
 };
@@ -423,52 +455,57 @@ __global__ void k_pcg_out(PcgArgs a, double *phi, double mean_sub) {
 /* ------------------------------------------------------------------------ */
 /* spectral direct solve: phi = Sy ((Sy F Sx) / lambda) Sx                   */
 /* ------------------------------------------------------------------------ */
-/* F (interior, row-major [j-1][i-1]) = f_elim, or the residual r (correction) */
-__global__ void k_sp_pack(PcgArgs a, double *F, int mx, int my, int from_r, const int *skip) {
-  if (skip && *skip) return;
-  int e = blockIdx.x * blockDim.x + threadIdx.x;
+/* F (interior, row-major [j-1][i-1]) = f_elim = rho scale + Dirichlet
+ * neighbours (the time loop's deposit kernel writes the same values) */
+__global__ void k_sp_pack(IskSpPack pk, const double *rho) {
+  const int mx = pk.s.nx - 2, my = pk.s.ny - 2;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= mx * my) return;
-  int i = e % mx + 1, j = e / mx + 1;
-  int n = j * a.s.nx + i;
-  F[e] = from_r ? a.r[n] : a.rho[n] * a.scale + dirichlet_rhs(a.s, i, j);
+  const int i = e % mx + 1, j = e / mx + 1;
+  pk.F[e] = rho[j * pk.s.nx + i] * pk.scale + dirichlet_rhs(pk.s, i, j);
 }
 
 struct SpGemm {
-  const int *skip;         /* non-null and set: nothing to do (solve already converged) */
   const double *A, *B, *L; /* L: elementwise factor (mode 1) */
   double *C;               /* modes 0, 1 */
   double *x;               /* modes 2, 3: node array, interior (i, j) = (col+1, row+1) */
   int M, N, K, nx, mode;
+  Sten s;                  /* mode 2: the boundary ring's Dirichlet values */
 };
 
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 
-/* C = A B (row-major FP64) on the matrix cores (v_mfma_f64_16x16x4f64): one
- * 16 x 16 block of C per work-group, its K range split over the 4 waves, the
- * 4 partial blocks summed in LDS in wave order (deterministic).  Operand
- * lanes: A[row0 + l%16][k + l/16], B[k + l/16][col0 + l%16]; result lane l
- * holds C[row0 + l/16 + 4i][col0 + l%16], i = 0..3 (rows interleaved). */
-__global__ __launch_bounds__(256) void k_sp_gemm(SpGemm g) {
-  if (g.skip && *g.skip) return;
-  __shared__ double red[4][256];
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+/* B operand: the matrix B (RES = 0), or the residual node array read at the
+ * interior nodes, row k and column c = node (c+1, k+1) (RES = 1) */
+template <int RES>
+__device__ __forceinline__ double sp_bload(const SpGemm &g, int k, int col) {
+  if (RES) return g.B[(size_t)(k + 1) * g.nx + (col + 1)];
+  return g.B[(size_t)k * g.N + col];
+}
+
+/* C = A B (row-major FP64) on the matrix cores (v_mfma_f64_16x16x4f64) for
+ * one 16 x 16 block of C: the K range is split over 4 waves (w = 0..3) and
+ * each wave's partial block goes to red[w]; sp_tile_store sums them in wave
+ * order (deterministic).  Operand lanes: A[row0 + l%16][k + l/16],
+ * B[k + l/16][col0 + l%16]; result lane l holds C[row0 + l/16 + 4i][col0 +
+ * l%16], i = 0..3 (rows interleaved).  All loads of a 64-wide k block are
+ * issued before its MFMAs; the accumulation order is k ascending. */
+template <int RES>
+__device__ __forceinline__ void sp_tile_mfma(const SpGemm &g, int row0, int col0, int w, int lane,
+                                             double (*red)[256]) {
   const int lr = lane & 15, kq = lane >> 4;
-  const int row0 = blockIdx.y * 16, col0 = blockIdx.x * 16;
   const int arow = row0 + lr, bcol = col0 + lr;
   const int kper = ((g.K + 15) / 16) * 4;
   const int kbeg = w * kper;
   const int kend = min(g.K, kbeg + kper);
   f64x4 acc = {0.0, 0.0, 0.0, 0.0};
-  /* all loads of a 64-wide k block are issued before its MFMAs, so the wave
-   * waits for one memory latency per block instead of one per k step; the
-   * accumulation order (k ascending) is the same either way */
   for (int k0 = kbeg; k0 < kend; k0 += 64) {
     double av[16], bv[16];
 #pragma unroll
     for (int u = 0; u < 16; ++u) {
       const int kk = k0 + 4 * u + kq;
       av[u] = (arow < g.M && kk < kend) ? g.A[(size_t)arow * g.K + kk] : 0.0;
-      bv[u] = (kk < kend && bcol < g.N) ? g.B[(size_t)kk * g.N + bcol] : 0.0;
+      bv[u] = (kk < kend && bcol < g.N) ? sp_bload<RES>(g, kk, bcol) : 0.0;
     }
 #pragma unroll
     for (int u = 0; u < 16; ++u) {
@@ -478,8 +515,12 @@ __global__ __launch_bounds__(256) void k_sp_gemm(SpGemm g) {
   }
 #pragma unroll
   for (int i = 0; i < 4; ++i) red[w][(kq + 4 * i) * 16 + lr] = acc[i];
-  __syncthreads();
-  if (w != 0) return;
+}
+
+/* wave 0 of the group, after a barrier: sum the 4 partials, apply the mode */
+__device__ __forceinline__ void sp_tile_store(const SpGemm &g, int row0, int col0, int lane,
+                                              double (*red)[256]) {
+  const int lr = lane & 15, kq = lane >> 4;
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
     const int e = (kq + 4 * i) * 16 + lr;
@@ -497,72 +538,164 @@ __global__ __launch_bounds__(256) void k_sp_gemm(SpGemm g) {
   }
 }
 
-static int sp_gemm(iskra_ctx *c, const double *A, const double *B, const double *L, double *C,
-                   double *x, int M, int N, int K, int mode, const int *skip) {
+/* the boundary ring of x = its Dirichlet values */
+__device__ __forceinline__ void sp_ring_setx(const SpGemm &g, int tid, int nthr) {
+  const int nx = g.s.nx, ny = g.s.ny;
+  const int ring = 2 * nx + 2 * (ny - 2);
+  for (int q = tid; q < ring; q += nthr) {
+    int n;
+    if (q < nx) n = q;                                     /* row 0 */
+    else if (q < 2 * nx) n = (ny - 1) * nx + (q - nx);     /* row ny-1 */
+    else {
+      const int r = q - 2 * nx;                            /* columns 0, nx-1 */
+      n = (1 + (r >> 1)) * nx + ((r & 1) ? nx - 1 : 0);
+    }
+    const int t = g.s.tags[n];
+    g.x[n] = t ? g.s.tagval[t] : 0.0;
+  }
+}
+
+/* one block of C per work-group; mode 2 (first pass, x = result) also sets
+ * the boundary ring of x (work-group (0, 0)) */
+__global__ __launch_bounds__(256) void k_sp_gemm(SpGemm g) {
+  __shared__ double red[4][256];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int row0 = blockIdx.y * 16, col0 = blockIdx.x * 16;
+  sp_tile_mfma<0>(g, row0, col0, w, lane, red);
+  __syncthreads();
+  if (w == 0) sp_tile_store(g, row0, col0, lane, red);
+  if (g.mode == 2 && blockIdx.x == 0 && blockIdx.y == 0) sp_ring_setx(g, threadIdx.x, 256);
+}
+
+static SpGemm sp_desc(iskra_ctx *c, const PcgArgs &a, const double *A, const double *B,
+                      const double *L, double *C, int M, int N, int K, int mode) {
   SpGemm g;
-  g.skip = skip;
   g.A = A;
   g.B = B;
   g.L = L;
   g.C = C;
-  g.x = x;
+  g.x = a.x;
   g.M = M;
   g.N = N;
   g.K = K;
   g.nx = c->g.nx;
   g.mode = mode;
-  hipLaunchKernelGGL(k_sp_gemm, dim3((N + 15) / 16, (M + 15) / 16), dim3(256), 0, c->stream, g);
-  ISK_CHECK_LAUNCH();
-  return ISKRA_OK;
+  g.s = a.s;
+  return g;
 }
 
-/* r = f_elim - L x at free nodes, FP64 relative residual -> scal[S_RELRES],
- * iflag[IF_SP_DONE] = (relres <= rtol).  A correction pass (second != 0) does
- * nothing when the first pass already met rtol, and marks iflag[IF_SP_FAIL]
- * if it still misses it. */
-__global__ void k_sp_resid(PcgArgs a, double rtol, int second) {
-  if (second && a.iflag[IF_SP_DONE]) return;
-  int n = blockIdx.x * blockDim.x + threadIdx.x;
-  int nn = a.s.nx * a.s.ny;
-  double rr = 0.0, ff = 0.0;
-  if (n < nn) {
-    int i = n % a.s.nx, j = n / a.s.nx;
-    if (a.s.tags[n] == 0) {
-      double f = a.rho[n] * a.scale;
-      double Lx = apply_L(a.s, a.x, i, j, nullptr);
-      double fe = f + dirichlet_rhs(a.s, i, j);
-      double r = f - Lx;
-      a.r[n] = r;
-      rr = r * r;
-      ff = fe * fe;
-    } else {
-      a.r[n] = 0.0;
-    }
-  }
-  double srr, sff;
-  if (block_reduce2(rr, ff, a.partials, a.counter, &srr, &sff) && threadIdx.x == 0) {
-    double rel = sff > 0.0 ? sqrt(srr / sff) : 0.0;
-    a.scal[S_RELRES] = rel;
-    int done = rel <= rtol;
-    a.iflag[IF_SP_DONE] = done;
-    if (second && !done) a.iflag[IF_SP_FAIL] = 1;
-  }
-}
-
-/* one transform pass: x = L^-1 f_elim (first) or x += L^-1 r (correction) */
-static int sp_pass(iskra_ctx *c, const PcgArgs &a, int correction, const int *skip) {
+/* the four GEMMs of one transform pass: x = L^-1 F (first, with the
+ * boundary ring) or x += L^-1 r (correction, r read at the interior nodes) */
+static void sp_pass_desc(iskra_ctx *c, const PcgArgs &a, int correction, SpGemm g[4]) {
   IskPoisson &p = c->ps;
   const int mx = p.sp_nx, my = p.sp_ny;
-  hipLaunchKernelGGL(k_sp_pack, dim3((mx * my + 255) / 256), dim3(256), 0, c->stream, a, p.sp_F,
-                     mx, my, correction, skip);
-  ISK_CHECK_LAUNCH();
-  int rc;
-  if ((rc = sp_gemm(c, p.sp_Sy, p.sp_F, nullptr, p.sp_T1, nullptr, my, mx, my, 0, skip))) return rc;
-  if ((rc = sp_gemm(c, p.sp_T1, p.sp_Sx, p.sp_Li, p.sp_T2, nullptr, my, mx, mx, 1, skip)))
-    return rc;
-  if ((rc = sp_gemm(c, p.sp_Sy, p.sp_T2, nullptr, p.sp_T1, nullptr, my, mx, my, 0, skip))) return rc;
-  return sp_gemm(c, p.sp_T1, p.sp_Sx, nullptr, nullptr, p.x, my, mx, mx, correction ? 3 : 2,
-                 skip);
+  g[0] = sp_desc(c, a, p.sp_Sy, correction ? a.r : p.sp_F, nullptr, p.sp_T1, my, mx, my, 0);
+  g[1] = sp_desc(c, a, p.sp_T1, p.sp_Sx, p.sp_Li, p.sp_T2, my, mx, mx, 1);
+  g[2] = sp_desc(c, a, p.sp_Sy, p.sp_T2, nullptr, p.sp_T1, my, mx, my, 0);
+  g[3] = sp_desc(c, a, p.sp_T1, p.sp_Sx, nullptr, nullptr, my, mx, mx, correction ? 3 : 2);
+}
+
+/* residual of node n (free nodes: r = f - L x, contributions to ||r||^2 and
+ * ||f_elim||^2; Dirichlet nodes: r = 0); also phi = x and, with E, the field
+ * of node n from x (the arithmetic of efield_body) */
+__device__ __forceinline__ void sp_resid_node(const PcgArgs &a, int n, float *E, double dx,
+                                              double dy, double *rr, double *ff) {
+  const int i = n % a.s.nx, j = n / a.s.nx;
+  if (a.s.tags[n] == 0) {
+    const double f = a.rho[n] * a.scale;
+    const double Lx = apply_L(a.s, a.x, i, j, nullptr);
+    const double fe = f + dirichlet_rhs(a.s, i, j);
+    const double r = f - Lx;
+    a.r[n] = r;
+    *rr += r * r;
+    *ff += fe * fe;
+  } else {
+    a.r[n] = 0.0;
+  }
+  a.phi_out[n] = a.x[n];
+  if (E) efield_node(a.x, E, 2, a.s.nx, a.s.ny, dx, dy, n);
+}
+
+/* r = f_elim - L x, FP64 relative residual -> scal[S_RELRES], iflag[IF_SP_DONE]
+ * = (relres <= rtol); phi = x; with E != null also E = -grad phi */
+__global__ void k_sp_resid(PcgArgs a, double rtol, float *E, double dx, double dy) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  double rr = 0.0, ff = 0.0;
+  if (n < a.s.nx * a.s.ny) sp_resid_node(a, n, E, dx, dy, &rr, &ff);
+  double srr, sff;
+  if (block_reduce2(rr, ff, a.partials, a.counter, &srr, &sff) && threadIdx.x == 0) {
+    const double rel = sff > 0.0 ? sqrt(srr / sff) : 0.0;
+    a.scal[S_RELRES] = rel;
+    a.iflag[IF_SP_DONE] = rel <= rtol;
+  }
+}
+
+/* A correction pass in one work-group of 16 waves (4 groups of 4 waves, each
+ * group computing 16 x 16 blocks with the same tile code as k_sp_gemm, so the
+ * results are those of the multi-work-group GEMMs), then the residual, phi and
+ * E of the corrected x.  It returns at once when iflag[IF_SP_DONE] is set,
+ * which is the usual case: one transform pass is exact to FP64 round-off, and
+ * a correction is only needed when rtol is below that (then it takes about a
+ * millisecond at 257^2).  second != 0: a miss after the correction sets
+ * iflag[IF_SP_FAIL]. */
+struct SpCorrect {
+  SpGemm g[4];
+  PcgArgs a;
+  double rtol, dx, dy;
+  float *E;
+  int second;
+};
+
+#define SPC_THREADS 1024
+__global__ __launch_bounds__(SPC_THREADS) void k_sp_correct(SpCorrect c) {
+  if (c.a.iflag[IF_SP_DONE]) return;
+  __shared__ double red[4][4][256];
+  __shared__ double sred[2][SPC_THREADS / 64];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wv = tid >> 6, grp = wv >> 2, w = wv & 3;
+  for (int st = 0; st < 4; ++st) {
+    const SpGemm &g = c.g[st];
+    const int tx = (g.N + 15) / 16, ntile = tx * ((g.M + 15) / 16);
+    for (int t0 = 0; t0 < ntile; t0 += 4) {
+      const int t = t0 + grp;
+      const bool on = t < ntile;
+      const int row0 = (t / tx) * 16, col0 = (t % tx) * 16;
+      if (on) {
+        if (st == 0) sp_tile_mfma<1>(g, row0, col0, w, lane, red[grp]);
+        else sp_tile_mfma<0>(g, row0, col0, w, lane, red[grp]);
+      }
+      __syncthreads();
+      if (on && w == 0) sp_tile_store(g, row0, col0, lane, red[grp]);
+      __syncthreads();
+    }
+    /* the next stage reads what this one wrote (global memory) */
+    __threadfence();
+    __syncthreads();
+  }
+  const int nn = c.a.s.nx * c.a.s.ny;
+  double rr = 0.0, ff = 0.0;
+  for (int n = tid; n < nn; n += SPC_THREADS) sp_resid_node(c.a, n, c.E, c.dx, c.dy, &rr, &ff);
+  for (int off = 32; off > 0; off >>= 1) {
+    rr += __shfl_xor(rr, off);
+    ff += __shfl_xor(ff, off);
+  }
+  if (lane == 0) {
+    sred[0][wv] = rr;
+    sred[1][wv] = ff;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    double srr = 0.0, sff = 0.0;
+    for (int k = 0; k < SPC_THREADS / 64; ++k) {
+      srr += sred[0][k];
+      sff += sred[1][k];
+    }
+    const double rel = sff > 0.0 ? sqrt(srr / sff) : 0.0;
+    c.a.scal[S_RELRES] = rel;
+    const int done = rel <= c.rtol;
+    c.a.iflag[IF_SP_DONE] = done;
+    if (c.second && !done) c.a.iflag[IF_SP_FAIL] = 1;
+  }
 }
 
 /* ------------------------------------------------------------------------ */
@@ -978,23 +1111,7 @@ __device__ __forceinline__ void efield_body(const double *phi, float *E, int dim
                                             double dx, double dy, const int blk) {
   int n = blk * blockDim.x + threadIdx.x;
   if (n >= nx * ny) return;
-  int i = n % nx, j = n / nx;
-  double ex;
-  if (nx < 2) ex = 0.0;
-  else if (i == 0) ex = (phi[n] - phi[n + 1]) / dx;
-  else if (i == nx - 1) ex = (phi[n - 1] - phi[n]) / dx;
-  else ex = (phi[n - 1] - phi[n + 1]) / (2.0 * dx);
-  if (dim == 1) {
-    E[n] = (float)ex;
-    return;
-  }
-  double ey;
-  if (ny < 2) ey = 0.0;
-  else if (j == 0) ey = (phi[n] - phi[n + nx]) / dy;
-  else if (j == ny - 1) ey = (phi[n - nx] - phi[n]) / dy;
-  else ey = (phi[n - nx] - phi[n + nx]) / (2.0 * dy);
-  E[2 * n] = (float)ex;
-  E[2 * n + 1] = (float)ey;
+  efield_node(phi, E, dim, nx, ny, dx, dy, n);
 }
 
 __global__ void k_efield(const double *phi, float *E, int dim, int nx, int ny, double dx,
@@ -1265,6 +1382,7 @@ static PcgArgs make_pcg(iskra_ctx *c, int mgpre) {
   a.scale = (c->g.dx * c->g.dx) / p.eps0;
   a.warm = 0;
   a.phi_in = c->phi;
+  a.phi_out = c->phi;
   a.mgpre = mgpre;
   a.max_iter = p.max_iter;
   return a;
@@ -1362,7 +1480,27 @@ static void tri_prepare(iskra_ctx *c, TriArgs *pt) {
 }
 
 int isk_poisson_solve(iskra_ctx *c, int warm, int *iters, double *relres) {
+  return isk_poisson_solve_ex(c, warm, iters, relres, 0, nullptr, 0);
+}
+
+/* the spectral right-hand side packing, for the deposit kernel of the time
+ * loop; returns 0 (and F = null) for the other solvers */
+int isk_poisson_pack_desc(iskra_ctx *c, IskSpPack *pk) {
   IskPoisson &p = c->ps;
+  pk->F = nullptr;
+  if (p.kind != ISKRA_POISSON_SPECTRAL || !p.sp_F) return 0;
+  pk->F = p.sp_F;
+  pk->s = make_sten(c);
+  pk->scale = (c->g.dx * c->g.dx) / p.eps0;
+  return 1;
+}
+
+/* with_E != 0: a solver that can form E in the same pass does so and sets
+ * *E_done (the spectral solve); otherwise *E_done stays 0 */
+int isk_poisson_solve_ex(iskra_ctx *c, int warm, int *iters, double *relres, int with_E,
+                         int *E_done, int packed) {
+  IskPoisson &p = c->ps;
+  if (E_done) *E_done = 0;
   const int nn = isk_nodes(c);
   if (p.kind == ISKRA_POISSON_DIRECT) {
     TriArgs t;
@@ -1381,48 +1519,66 @@ int isk_poisson_solve(iskra_ctx *c, int warm, int *iters, double *relres) {
   }
   const int nblk = (nn + RED_BLOCK - 1) / RED_BLOCK;
   if (p.kind == ISKRA_POISSON_SPECTRAL) {
-    /* transform pass, FP64 residual; if it misses rtol, solve the residual
-     * and add it (defect correction).  Without iters/relres (the time loop)
-     * the correction pass is enqueued unconditionally and skips itself on
-     * the device when the first pass converged; a miss after it is reported
-     * by iskra_sync.  With them, the residual is read back after each pass. */
+    /* transform pass (F packed by the deposit kernel in the time loop, else
+     * here; 4 GEMM launches, the last also sets the boundary ring of x),
+     * FP64 residual (also phi, and E when asked), then
+     * the correction kernel, which does nothing when the pass met rtol.
+     * Without iters/relres (the time loop) one correction is enqueued and
+     * a miss after it is reported by iskra_sync; with them the residual is
+     * read back after each pass and up to 3 corrections are made. */
     PcgArgs a = make_pcg(c, 0);
-    a.warm = 0;
-    hipLaunchKernelGGL(k_pcg_setx, dim3(nblk), dim3(RED_BLOCK), 0, c->stream, a);
+    float *E = with_E ? c->E : nullptr;
+    if (!packed) {
+      IskSpPack pk;
+      isk_poisson_pack_desc(c, &pk);
+      const int m2 = p.sp_nx * p.sp_ny;
+      hipLaunchKernelGGL(k_sp_pack, dim3((m2 + 255) / 256), dim3(256), 0, c->stream, pk, c->rho);
+      ISK_CHECK_LAUNCH();
+    }
+    SpGemm g[4];
+    sp_pass_desc(c, a, 0, g);
+    for (int k = 0; k < 4; ++k) {
+      hipLaunchKernelGGL(k_sp_gemm, dim3((g[k].N + 15) / 16, (g[k].M + 15) / 16), dim3(256), 0,
+                         c->stream, g[k]);
+      ISK_CHECK_LAUNCH();
+    }
+    hipLaunchKernelGGL(k_sp_resid, dim3(nblk), dim3(RED_BLOCK), 0, c->stream, a, p.rtol, E,
+                       c->g.dx, c->g.dy);
     ISK_CHECK_LAUNCH();
-    const int *done = p.iflag + IF_SP_DONE;
-    int rc;
+    SpCorrect sc;
+    sp_pass_desc(c, a, 1, sc.g);
+    sc.a = a;
+    sc.rtol = p.rtol;
+    sc.dx = c->g.dx;
+    sc.dy = c->g.dy;
+    sc.E = E;
     int rounds = 1;
     double rel = 0.0;
-    if ((rc = sp_pass(c, a, 0, nullptr))) return rc;
-    hipLaunchKernelGGL(k_sp_resid, dim3(nblk), dim3(RED_BLOCK), 0, c->stream, a, p.rtol, 0);
-    ISK_CHECK_LAUNCH();
     if (!iters && !relres) {
-      if ((rc = sp_pass(c, a, 1, done))) return rc;
-      hipLaunchKernelGGL(k_sp_resid, dim3(nblk), dim3(RED_BLOCK), 0, c->stream, a, p.rtol, 1);
+      sc.second = 1;
+      hipLaunchKernelGGL(k_sp_correct, dim3(1), dim3(SPC_THREADS), 0, c->stream, sc);
       ISK_CHECK_LAUNCH();
     } else {
+      sc.second = 0;
       while (true) {
-        double sc[S_WORDS];
+        double scal[S_WORDS];
         int fl[8];
-        ISK_HIP(hipMemcpyAsync(sc, p.scal, sizeof(sc), hipMemcpyDeviceToHost, c->stream));
+        ISK_HIP(hipMemcpyAsync(scal, p.scal, sizeof(scal), hipMemcpyDeviceToHost, c->stream));
         ISK_HIP(hipMemcpyAsync(fl, p.iflag, sizeof(fl), hipMemcpyDeviceToHost, c->stream));
         ISK_HIP(hipStreamSynchronize(c->stream));
-        rel = sc[S_RELRES];
+        rel = scal[S_RELRES];
         if (fl[IF_SP_DONE]) break;
         if (rounds >= 4) {
           isk_set_error("poisson: spectral solve reached relative residual %.3g > %.3g", rel,
                         p.rtol);
           return ISKRA_E_CONVERGE;
         }
-        if ((rc = sp_pass(c, a, 1, nullptr))) return rc;
-        hipLaunchKernelGGL(k_sp_resid, dim3(nblk), dim3(RED_BLOCK), 0, c->stream, a, p.rtol, 0);
+        hipLaunchKernelGGL(k_sp_correct, dim3(1), dim3(SPC_THREADS), 0, c->stream, sc);
         ISK_CHECK_LAUNCH();
         ++rounds;
       }
     }
-    hipLaunchKernelGGL(k_pcg_out, dim3(nblk), dim3(RED_BLOCK), 0, c->stream, a, c->phi, 0.0);
-    ISK_CHECK_LAUNCH();
+    if (E_done) *E_done = with_E;
     p.last_iters = rounds;
     p.last_relres = rel;
     if (iters) *iters = rounds;
@@ -1533,9 +1689,9 @@ int isk_poisson_solve(iskra_ctx *c, int warm, int *iters, double *relres) {
 }
 
 /* ---- batched launches ---- */
-__global__ void kb_tri(const TriArgs *A, const int *bstart, int n) {
+__global__ __launch_bounds__(64) void kb_tri(const TriArgs *A, const int *bstart, int n) {
   const int i = isk_bfind(bstart, n, blockIdx.x);
-  if (threadIdx.x == 0) tri_solve_body(A[i]);
+  tri_solve_group(A[i]);
 }
 
 __global__ void kb_efield(const EfieldArgs *A, const int *bstart, int n) {

@@ -48,7 +48,7 @@ def two_stream_1d(cells=64, particles=20000, eps=1e-3, v0=1.0, kv0_wp=0.9, wp_dt
     order = np.empty(particles, dtype=np.int64)
     order[0::2] = np.arange(half)
     order[1::2] = np.arange(half, particles)
-    v = v[np.argsort(order, kind="stable")]
+    v = v[order]
     return dict(nodes=cells + 1, dx=dx, L=L, x=x[:, None], v=v, q=q, m=m, w0=w0, eps0=eps0,
                 dt=dt, bg=-q * n0, k=k, wp=wp)
 

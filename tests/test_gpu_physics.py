@@ -41,13 +41,14 @@ def test_two_stream_growth_rate_matches_linear_theory(gpu_ready):
     t = (np.arange(len(amp)) + 1) * p["dt"]
     la = np.log(amp)
     sat = int(np.argmax(la))
-    # linear phase: from half an e-fold above the start to 1.5 e-folds below the peak
-    lo = int(np.argmax(la > la[:10].mean() + 0.5))
+    # linear phase: the 3.5 e-folds from 5 to 1.5 e-folds below the peak (the
+    # first ~100 steps mix the seeded mode with the oscillating roots)
+    lo = int(np.argmax(la > la[sat] - 5.0))
     hi = int(np.argmax(la > la[sat] - 1.5))
-    assert hi - lo > 20, (lo, hi, sat)
+    assert hi - lo > 30, (lo, hi, sat)
     g = np.polyfit(t[lo:hi], la[lo:hi], 1)[0]
     print("growth rate", g, "theory", gamma, "window", lo, hi, "peak", sat)
-    assert abs(g / gamma - 1.0) < 0.15
+    assert abs(g / gamma - 1.0) < 0.10
 
 
 def test_two_stream_energy_drift(gpu_ready):
@@ -57,8 +58,9 @@ def test_two_stream_energy_drift(gpu_ready):
     # field energy grows from ~0 to a sizeable fraction of the beam energy
     assert wf.max() > 0.01 * wk[0]
     print("max relative energy drift", drift, "peak field/kinetic", wf.max() / wk[0])
-    # explicit PIC with cold beams: a few per cent of exchange error; bound 10 %
-    assert drift < 0.10
+    # leapfrog with the kinetic energy taken at the half step: ~2 % exchange
+    # error through saturation for this set-up; bound 5 %
+    assert drift < 0.05
 
 
 def test_ccp_runs_are_bitwise_reproducible(gpu_ready):
