@@ -404,12 +404,32 @@ struct AppendArgs {
   int dim;
 };
 
+/* one work-group appends the products of MCC_APPEND_GROUP consecutive MCC
+ * work-groups (their products are contiguous in the global order); product
+ * g comes from the last of them whose offset is <= g */
+#define MCC_APPEND_GROUP 16
+static inline int mcc_append_blocks(int nblk) {
+  return (nblk + MCC_APPEND_GROUP - 1) / MCC_APPEND_GROUP;
+}
+
+__device__ __forceinline__ void mcc_append_one(const AppendArgs &a, int g, int b, int k);
+
 __device__ __forceinline__ void mcc_append_body(const AppendArgs &a, const int b) {
-  const int k = threadIdx.x;
-  const int base = a.blk_off[b];
-  const int n = a.blk_off[b + 1] - base;
-  if (k >= n) return;
-  const int g = base + k;
+  const int m0 = b * MCC_APPEND_GROUP, m1 = min(m0 + MCC_APPEND_GROUP, a.nblk);
+  const int g1 = a.blk_off[m1];
+  for (int g = a.blk_off[m0] + (int)threadIdx.x; g < g1; g += blockDim.x) {
+    int lo = m0, hi = m1 - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (a.blk_off[mid] <= g) lo = mid;
+      else hi = mid - 1;
+    }
+    mcc_append_one(a, g, lo, g - a.blk_off[lo]);
+  }
+}
+
+/* product g = product k of MCC work-group b */
+__device__ __forceinline__ void mcc_append_one(const AppendArgs &a, int g, int b, int k) {
   const size_t s = (size_t)b * MCC_BLOCK + k;
   const float *st = a.stage + s * 8;
   const int32_t c = a.stage_cell[s];
@@ -672,7 +692,8 @@ int isk_mcc_run(iskra_ctx *c, int idx, double dt, int64_t *events) {
   ISK_CHECK_LAUNCH();
   hipLaunchKernelGGL(k_mcc_scan, dim3(1), dim3(1024), 0, c->stream, m.blk_cnt, nblk);
   ISK_CHECK_LAUNCH();
-  hipLaunchKernelGGL(k_mcc_append, dim3(nblk), dim3(MCC_BLOCK), 0, c->stream, p);
+  hipLaunchKernelGGL(k_mcc_append, dim3(mcc_append_blocks(nblk)), dim3(MCC_BLOCK), 0,
+                     c->stream, p);
   ISK_CHECK_LAUNCH();
   hipLaunchKernelGGL(k_mcc_commit, dim3(1), dim3(1), 0, c->stream, cm.blk_off, cm.nblk,
                      cm.ehdr, cm.ecap, cm.ihdr, cm.icap, cm.R);
@@ -725,11 +746,12 @@ int isk_bbuild_mcc(iskra_ctx **cs, int n, int m, double dt, const int *cur_src,
   std::vector<MccArgs> A(n);
   std::vector<AppendArgs> P(n);
   std::vector<CommitArgs> CM(n);
-  std::vector<int> blocks(n), one(n, 1);
+  std::vector<int> blocks(n), ablocks(n), one(n, 1);
   for (int i = 0; i < n; ++i) {
     int rc = mcc_prepare(cs[i], m, dt, cur_src[i], cur_ion[i], &A[i], &P[i], &CM[i], &blocks[i]);
     if (rc) return rc;
     A[i].step_dev = step_dev;
+    ablocks[i] = mcc_append_blocks(blocks[i]);
   }
   int rc;
   if ((rc = isk_blaunch_upload(&L[0], ISK_B_MCC, A.data(), sizeof(MccArgs), blocks, MCC_BLOCK,
@@ -738,7 +760,7 @@ int isk_bbuild_mcc(iskra_ctx **cs, int n, int m, double dt, const int *cur_src,
   if ((rc = isk_blaunch_upload(&L[1], ISK_B_MCC_SCAN, CM.data(), sizeof(CommitArgs), one, 1024,
                                0)))
     return rc;
-  if ((rc = isk_blaunch_upload(&L[2], ISK_B_MCC_APPEND, P.data(), sizeof(AppendArgs), blocks,
+  if ((rc = isk_blaunch_upload(&L[2], ISK_B_MCC_APPEND, P.data(), sizeof(AppendArgs), ablocks,
                                MCC_BLOCK, 0)))
     return rc;
   return isk_blaunch_upload(&L[3], ISK_B_MCC_COMMIT, CM.data(), sizeof(CommitArgs), one, 64, 0);

@@ -326,12 +326,15 @@ __device__ void sort_colscan_body(int *H, int *S, int nb1, int nblk, int bx, int
   int b = bx * blockDim.x + threadIdx.x;
   if (b >= nb1) return;
   int run = 0;
-  int end = min((seg + 1) * SORT_SEG, nblk);
-  for (int k = seg * SORT_SEG; k < end; ++k) {
-    size_t at = (size_t)k * nb1 + b;
-    int v = H[at];
-    H[at] = run;
-    run += v;
+  const int k0 = seg * SORT_SEG, n = min(SORT_SEG, nblk - k0);
+  /* all loads of the column before its stores (independent, in flight together) */
+  int v[SORT_SEG];
+#pragma unroll
+  for (int k = 0; k < SORT_SEG; ++k) v[k] = k < n ? H[(size_t)(k0 + k) * nb1 + b] : 0;
+#pragma unroll
+  for (int k = 0; k < SORT_SEG; ++k) {
+    if (k < n) H[(size_t)(k0 + k) * nb1 + b] = run;
+    run += v[k];
   }
   S[(size_t)seg * nb1 + b] = run;
 }
@@ -345,11 +348,17 @@ __device__ void sort_segscan_body(int *S, int *tot, int nb1, int nseg, int bx) {
   int b = bx * blockDim.x + threadIdx.x;
   if (b >= nb1) return;
   int run = 0;
-  for (int s = 0; s < nseg; ++s) {
-    size_t at = (size_t)s * nb1 + b;
-    int v = S[at];
-    S[at] = run;
-    run += v;
+  /* 32 segments at a time: loads first, then the stores */
+  for (int s0 = 0; s0 < nseg; s0 += 32) {
+    const int n = min(32, nseg - s0);
+    int v[32];
+#pragma unroll
+    for (int k = 0; k < 32; ++k) v[k] = k < n ? S[(size_t)(s0 + k) * nb1 + b] : 0;
+#pragma unroll
+    for (int k = 0; k < 32; ++k) {
+      if (k < n) S[(size_t)(s0 + k) * nb1 + b] = run;
+      run += v[k];
+    }
   }
   tot[b] = run;
 }
@@ -400,135 +409,99 @@ struct ScatterArgs {
   KeyDesc kd;
 };
 
-/* Stable scatter of one work-group's SORT_CHUNK particles, in three phases:
- * (1) the keys of all its rounds go to registers and a work-group histogram;
- * (2) rounds of SORT_BLOCK particles are ranked in index order (ballots
- *     within a wave, LDS group lists across waves), which gives every kept
- *     particle its slot in the work-group's key-ordered output: perm[slot] =
- *     particle, skey[slot] = key;
- * (3) the records are written slot by slot, so a key's run of the
- *     work-group is written contiguously instead of a few elements per round.
- * LDS: goff (global position of the work-group's first particle of a key,
- * minus its local slot), lrun (local slot counter per key), perm, skey. */
-static inline size_t sort_scatter_shm(int nb1) {
-  return (size_t)2 * nb1 * sizeof(int) + (size_t)2 * SORT_CHUNK * sizeof(uint16_t);
+/* stable scatter: particles of a block are ranked in index order; within a
+ * round of 256, lanes of a wave are matched by key with ballots and waves are
+ * ordered through LDS group lists. */
+struct SortPay {
+  int c;
+  float ox, oy, vx, vy, vz;
+  uint32_t id;
+};
+
+/* one particle's record; the index is clamped to the last particle so the
+ * loads carry no condition on their results and stay in flight */
+__device__ __forceinline__ void scatter_load(const ScatterArgs &a, long long p, long long np,
+                                             SortPay &q) {
+  const long long pc = p < np ? p : np - 1;
+  q.c = a.cell[pc];
+  q.ox = a.ox[pc];
+  q.oy = a.kd.dim == 2 ? a.oy[pc] : 0.f;
+  q.vx = a.vx[pc];
+  q.vy = a.vy[pc];
+  q.vz = a.vz[pc];
+  q.id = a.id ? a.id[pc] : 0u;
 }
 
+__device__ __forceinline__ void scatter_round(const ScatterArgs &a, const SortPay &q, bool valid,
+                                              int *base, int (*gkey)[64], int (*gcnt)[64],
+                                              int *ng, int lane, int w) {
+  const int key = valid ? sort_key(q.c, a.kd) : -1;
+  unsigned long long todo = __ballot(valid);
+  int myrank = 0;
+  int ngr = 0;
+  while (todo) {
+    int leader = __ffsll((long long)todo) - 1;
+    int lk = __shfl(key, leader);
+    unsigned long long m = __ballot(valid && key == lk);
+    if (valid && key == lk) myrank = __popcll(m & ((1ull << lane) - 1ull));
+    if (lane == 0) {
+      gkey[w][ngr] = lk;
+      gcnt[w][ngr] = __popcll(m);
+    }
+    ++ngr;
+    todo &= ~m;
+  }
+  if (lane == 0) ng[w] = ngr;
+  isk_lds_barrier();
+  if (valid && key < a.kd.nbins) {
+    int pre = 0;
+    for (int w2 = 0; w2 < w; ++w2) {
+      int n2 = ng[w2];
+      for (int g = 0; g < n2; ++g)
+        if (gkey[w2][g] == key) pre += gcnt[w2][g];
+    }
+    long long d = (long long)base[key] + pre + myrank;
+    a.cell_o[d] = q.c;
+    a.ox_o[d] = q.ox;
+    if (a.kd.dim == 2) a.oy_o[d] = q.oy;
+    a.vx_o[d] = q.vx;
+    a.vy_o[d] = q.vy;
+    a.vz_o[d] = q.vz;
+    if (a.id) a.id_o[d] = q.id;
+  }
+  isk_lds_barrier();
+  if (lane < ng[w]) atomicAdd(&base[gkey[w][lane]], gcnt[w][lane]);
+  isk_lds_barrier();
+}
+
+static inline size_t sort_scatter_shm(int nb1) { return (size_t)nb1 * sizeof(int); }
+
 __device__ void sort_scatter_body(const ScatterArgs &a, const int blk) {
-  extern __shared__ int sm[];
+  extern __shared__ int base[];
   __shared__ int gkey[4][64];
   __shared__ int gcnt[4][64];
   __shared__ int ng[4];
-  __shared__ int wsum[SORT_BLOCK / 64];
   const int nb1 = a.kd.nbins + 1;
-  int *goff = sm;
-  int *lrun = sm + nb1;
-  uint16_t *perm = reinterpret_cast<uint16_t *>(sm + 2 * nb1);
-  uint16_t *skey = perm + SORT_CHUNK;
   const int seg = blk / SORT_SEG;
-  for (int b = threadIdx.x; b < nb1; b += SORT_BLOCK) {
-    goff[b] = a.start[b] + a.S[(size_t)seg * nb1 + b] + a.H[(size_t)blk * nb1 + b];
-    lrun[b] = 0;
-  }
+  for (int b = threadIdx.x; b < nb1; b += SORT_BLOCK)
+    base[b] = a.start[b] + a.S[(size_t)seg * nb1 + b] + a.H[(size_t)blk * nb1 + b];
   __syncthreads();
   const long long np = a.hdr[HDR_NP];
   const long long b0 = (long long)blk * SORT_CHUNK;
   if (np <= 0 || b0 >= np) return;
-  const int nloc = (int)min((long long)SORT_CHUNK, np - b0);
   const int lane = threadIdx.x & 63;
   const int w = threadIdx.x >> 6;
-  /* phase 1 */
-  int key[SORT_PT];
-#pragma unroll
-  for (int r = 0; r < SORT_PT; ++r) {
-    const int q = r * SORT_BLOCK + threadIdx.x;
-    key[r] = a.cell[b0 + (q < nloc ? q : nloc - 1)];
-  }
-#pragma unroll
-  for (int r = 0; r < SORT_PT; ++r) {
-    const int q = r * SORT_BLOCK + threadIdx.x;
-    const bool valid = q < nloc;
-    key[r] = valid ? sort_key(key[r], a.kd) : -1;
-    unsigned long long todo = __ballot(valid);
-    while (todo) {
-      const int leader = __ffsll((long long)todo) - 1;
-      const int lk = __shfl(key[r], leader);
-      const unsigned long long m = __ballot(valid && key[r] == lk);
-      if (lane == leader) atomicAdd(&lrun[lk], (int)__popcll(m));
-      todo &= ~m;
-    }
-  }
-  __syncthreads();
-  /* exclusive scan of the counts in place (each thread a contiguous range) */
-  const int per = (nb1 + SORT_BLOCK - 1) / SORT_BLOCK;
-  const int lo = threadIdx.x * per, hi = min(lo + per, nb1);
-  int sum = 0;
-  for (int b = lo; b < hi; ++b) sum += lrun[b];
-  int inc = sum;
-  for (int off = 1; off < 64; off <<= 1) {
-    const int v = __shfl_up(inc, off);
-    if (lane >= off) inc += v;
-  }
-  if (lane == 63) wsum[w] = inc;
-  __syncthreads();
-  int run = inc - sum;
-  for (int k = 0; k < w; ++k) run += wsum[k];
-  for (int b = lo; b < hi; ++b) {
-    const int v = lrun[b];
-    lrun[b] = run;
-    goff[b] -= run;
-    run += v;
-  }
-  __syncthreads();
-  const int kept = lrun[a.kd.nbins]; /* discarded particles (last bin) are dropped */
-  /* phase 2 */
-#pragma unroll
-  for (int r = 0; r < SORT_PT; ++r) {
-    const int q = r * SORT_BLOCK + threadIdx.x;
-    const bool valid = q < nloc;
-    unsigned long long todo = __ballot(valid);
-    int myrank = 0;
-    int ngr = 0;
-    while (todo) {
-      const int leader = __ffsll((long long)todo) - 1;
-      const int lk = __shfl(key[r], leader);
-      const unsigned long long m = __ballot(valid && key[r] == lk);
-      if (valid && key[r] == lk) myrank = __popcll(m & ((1ull << lane) - 1ull));
-      if (lane == 0) {
-        gkey[w][ngr] = lk;
-        gcnt[w][ngr] = __popcll(m);
-      }
-      ++ngr;
-      todo &= ~m;
-    }
-    if (lane == 0) ng[w] = ngr;
-    __syncthreads();
-    if (valid && key[r] < a.kd.nbins) {
-      int pre = 0;
-      for (int w2 = 0; w2 < w; ++w2) {
-        const int n2 = ng[w2];
-        for (int g = 0; g < n2; ++g)
-          if (gkey[w2][g] == key[r]) pre += gcnt[w2][g];
-      }
-      const int slot = lrun[key[r]] + pre + myrank;
-      perm[slot] = (uint16_t)q;
-      skey[slot] = (uint16_t)key[r];
-    }
-    __syncthreads();
-    if (lane < ng[w]) atomicAdd(&lrun[gkey[w][lane]], gcnt[w][lane]);
-    __syncthreads();
-  }
-  /* phase 3 */
-  for (int s = threadIdx.x; s < kept; s += SORT_BLOCK) {
-    const long long src = b0 + perm[s];
-    const long long d = (long long)goff[skey[s]] + s;
-    a.cell_o[d] = a.cell[src];
-    a.ox_o[d] = a.ox[src];
-    if (a.kd.dim == 2) a.oy_o[d] = a.oy[src];
-    a.vx_o[d] = a.vx[src];
-    a.vy_o[d] = a.vy[src];
-    a.vz_o[d] = a.vz[src];
-    if (a.id) a.id_o[d] = a.id[src];
+  /* two records in registers: round r+1 is requested before round r is
+   * ranked and written, so its loads overlap that work (SORT_PT is even) */
+  SortPay A, B;
+  scatter_load(a, b0 + threadIdx.x, np, A);
+  for (int r = 0; r < SORT_PT; r += 2) {
+    const long long pa = b0 + (long long)r * SORT_BLOCK + threadIdx.x;
+    const long long pb = pa + SORT_BLOCK;
+    scatter_load(a, pb, np, B);
+    scatter_round(a, A, pa < np, base, gkey, gcnt, ng, lane, w);
+    if (r + 2 < SORT_PT) scatter_load(a, pb + SORT_BLOCK, np, A);
+    scatter_round(a, B, pb < np, base, gkey, gcnt, ng, lane, w);
   }
 }
 
@@ -699,17 +672,8 @@ static int sort_pass(iskra_ctx *c, int sid, int mode, int build) {
   hipLaunchKernelGGL(k_sort_binscan, dim3(1), dim3(1024), 0, c->stream,
                      (const int *)A.tot, A.start, nb1);
   ISK_CHECK_LAUNCH();
-  const size_t sshm = sort_scatter_shm(nb1);
-  if (sshm > 65536) {
-    static bool raised = false;
-    if (!raised) {
-      ISK_HIP(hipFuncSetAttribute((const void *)k_sort_scatter,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  (int)sort_scatter_shm(ISK_MAX_BINS + 1)));
-      raised = true;
-    }
-  }
-  hipLaunchKernelGGL(k_sort_scatter, dim3(nblk), dim3(SORT_BLOCK), sshm, c->stream, A.sc);
+  hipLaunchKernelGGL(k_sort_scatter, dim3(nblk), dim3(SORT_BLOCK), sort_scatter_shm(nb1),
+                     c->stream, A.sc);
   ISK_CHECK_LAUNCH();
   hipLaunchKernelGGL(k_sort_finish, dim3(1), dim3(1024), 0, c->stream, A.hdr_w,
                      (const int *)A.tot, (const int *)A.start, A.kd.nbins, build, A.CH,
@@ -749,14 +713,24 @@ struct PushArgs {
   int *live_cnt; /* per work-group live particles pushed */
 };
 
-/* Wx = rn(hx 2^32), Wy = rn(hy 2^32), Wxy = rn(hx hy 2^32) (exact products,
- * integer arithmetic on the float bits: isk_fix32 / isk_fixprod32) */
+/* rn(a 2^32) and rn(a b 2^32) for offsets a, b in [0, 1): the FP64
+ * products are exact (24 x 24 bit mantissas), rint rounds half to even, and
+ * the results are below 2^32, so they convert to uint32 exactly */
+__device__ __forceinline__ long long fix32_dev(float a) {
+  return (long long)(unsigned int)__builtin_rint((double)a * 4294967296.0);
+}
+__device__ __forceinline__ long long fixprod32_dev(float a, float b) {
+  return (long long)(unsigned int)__builtin_rint(((double)a * (double)b) * 4294967296.0);
+}
+
+/* CIC weights in 2^-32 units: Wx = rn(hx 2^32), Wy = rn(hy 2^32), Wxy =
+ * rn(hx hy 2^32); the four corners sum to exactly 2^32 */
 __device__ __forceinline__ void fix_weights2(float hx, float hy, long long *w00,
                                              long long *w10, long long *w01,
                                              long long *w11) {
-  long long Wx = isk_fix32(hx);
-  long long Wy = isk_fix32(hy);
-  long long Wxy = isk_fixprod32(hx, hy);
+  long long Wx = fix32_dev(hx);
+  long long Wy = fix32_dev(hy);
+  long long Wxy = fixprod32_dev(hx, hy);
   *w11 = Wxy;
   *w10 = Wx - Wxy;
   *w01 = Wy - Wxy;
@@ -969,7 +943,7 @@ __device__ __forceinline__ void push_body(const PushArgs &a, const int blk) {
             atomicAdd(&a.gfall[q + a.nx + 1], (unsigned long long)w11);
           }
         } else {
-          long long w1 = isk_fix32(hx);
+          long long w1 = fix32_dev(hx);
           long long w0 = (1LL << 32) - w1;
           int li = i - wx0;
           if (windowed && (unsigned)li < (unsigned)(W - 1)) {
@@ -1675,15 +1649,6 @@ int isk_blaunch_particles(const IskBLaunch &L, hipStream_t st) {
       hipLaunchKernelGGL(kb_sort_binscan, g, b, 0, st, (const SortArgs *)L.args, L.bstart, L.n);
       break;
     case ISK_B_SORT_SCATTER:
-      if (L.shm > 65536) {
-        static bool raised = false;
-        if (!raised) {
-          ISK_HIP(hipFuncSetAttribute((const void *)kb_sort_scatter,
-                                      hipFuncAttributeMaxDynamicSharedMemorySize,
-                                      (int)sort_scatter_shm(ISK_MAX_BINS + 1)));
-          raised = true;
-        }
-      }
       hipLaunchKernelGGL(kb_sort_scatter, g, b, L.shm, st, (const SortArgs *)L.args, L.bstart,
                          L.n);
       break;
