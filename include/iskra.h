@@ -76,6 +76,12 @@ typedef struct {
 } iskra_tuning;
 int iskra_tuning_set(iskra_ctx *ctx, const iskra_tuning *t);
 int iskra_tuning_get(iskra_ctx *ctx, iskra_tuning *t);
+/* Steps between counting sorts of one species inside iskra_step (0: the
+ * context's sort_every).  Slow species (heavy, cold ions) stay tile-ordered
+ * for many more steps than electrons.  No reference counterpart: the
+ * device-side particle order is this library's (the reference keeps its
+ * particles unsorted).  Batches need every species on the context value. */
+int iskra_species_sort_every(iskra_ctx *ctx, int sid, int32_t every);
 
 /* ---- species ------------------------------------------------------------ */
 /* KineticSpecies{D,3} + create_kinetic_species

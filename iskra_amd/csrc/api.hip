@@ -532,6 +532,17 @@ int iskra_species_set_boundary(iskra_ctx *c, int sid, const int32_t mode[2]) {
   return ISKRA_OK;
 }
 
+int iskra_species_sort_every(iskra_ctx *c, int sid, int32_t every) {
+  CHECK_CTX(c);
+  if (!isk_species_ok(c, sid)) return ISKRA_E_ARG;
+  if (every < 0) {
+    isk_set_error("species_sort_every: %d < 0", every);
+    return ISKRA_E_ARG;
+  }
+  c->sp[sid].sort_every = every;
+  return ISKRA_OK;
+}
+
 int iskra_species_upload(iskra_ctx *c, int sid, int64_t np, const double *x, const double *v,
                          const uint32_t *id) {
   CHECK_CTX(c);
@@ -1125,8 +1136,10 @@ int iskra_step(iskra_ctx *c, double dt, int64_t nsteps) {
       }
     c->step_count++;
     c->stats.steps++;
-    if (c->t.sort_every > 0 && c->step_count % c->t.sort_every == 0)
-      for (int s = 0; s < c->nspecies; ++s) RC(isk_sort_species(c, s, 0));
+    for (int s = 0; s < c->nspecies; ++s) {
+      const int every = c->sp[s].sort_every > 0 ? c->sp[s].sort_every : c->t.sort_every;
+      if (every > 0 && c->step_count % every == 0) RC(isk_sort_species(c, s, 0));
+    }
   }
   return iskra_sync(c);
 }

@@ -257,6 +257,11 @@ int iskra_batch_create(iskra_ctx **ctxs, int n, iskra_batch **out) {
                     "interaction count, sort interval or step count", i);
       return ISKRA_E_ARG;
     }
+    for (int s = 0; s < c->nspecies; ++s)
+      if (c->sp[s].sort_every > 0 && c->sp[s].sort_every != c->t.sort_every) {
+        isk_set_error("batch_create: instance %d: species %d has its own sort interval", i, s);
+        return ISKRA_E_UNSUPPORTED;
+      }
     for (int m = 0; m < c->nmcc; ++m)
       if (c->mcc[m].src != c0->mcc[m].src || c->mcc[m].ion != c0->mcc[m].ion) {
         isk_set_error("batch_create: instance %d: interaction %d has other species", i, m);

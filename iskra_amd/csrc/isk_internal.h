@@ -74,6 +74,7 @@ struct IskSpecies {
   double q, m, w0;
   int64_t cap;
   int track_ids;
+  int sort_every; /* 0: the context's */
   int bnd[2];
   int cur;
   int32_t *cell[2];

@@ -217,3 +217,44 @@ def test_density_and_gather_operators(gpu_ready, pic_module):
     xp = st[0]
     assert np.allclose(Ep[:, 0], 2.0 * xp[:, 0] - 0.5 * xp[:, 1], atol=1e-5)
     assert np.allclose(Ep[:, 1], 1.0 + xp[:, 1], atol=1e-5)
+
+
+def test_sort_schedule_does_not_change_results(gpu_ready):
+    """The counting sort only orders particles: electrons sorted every 4 steps
+    and ions every 4, 16 or 1000 steps (iskra_species_sort_every) give the
+    same rho, phi and particle states, bit for bit, compared by id."""
+    from iskra_amd import Context, _lib as L, problems as P
+    p = P.slab_2d(cells=64, ppc=16)
+    n = p["nodes"]
+    out = []
+    for ion_every in (0, 16, 1000):
+        ctx = Context()
+        ctx.grid(2, (n, n), (p["dx"], p["dx"]))
+        ctx.tuning(sort_every=4)
+        tags = np.zeros((n, n), np.int8)
+        tags[0, :] = tags[-1, :] = tags[:, 0] = tags[:, -1] = 1
+        ctx.poisson(L.POISSON_SPECTRAL, p["eps0"], tags.ravel(), rtol=1e-10)
+        ctx.tag_value(1, 0.0)
+        e = ctx.species(-p["q"], p["me"], p["w0"], p["n_part"] + 64, track_ids=True)
+        i = ctx.species(p["q"], p["mi"], p["w0"], p["n_part"] + 64, track_ids=True)
+        for s in (e, i):
+            ctx.boundary(s, [L.BND_DISCARD, L.BND_DISCARD])
+        ctx.species_sort_every(i, ion_every)
+        ctx.sample_maxwellian(e, p["n_part"], [p["L"]] * 2, [p["ve"]] * 3, seed=3, stream=0)
+        ctx.sample_maxwellian(i, p["n_part"], [p["L"]] * 2, [p["vi"]] * 3, seed=3, stream=1)
+        ctx.step(p["dt"], 40)
+        st = []
+        for s in (e, i):
+            g = ctx.get_state(s)
+            live = g["cell"] >= 0
+            o = np.argsort(g["id"][live], kind="stable")
+            st.append({k: v[live][o] for k, v in g.items()})
+        out.append((ctx.field(L.F_RHO), ctx.field(L.F_PHI), st))
+        ctx.close()
+    r0, f0, s0 = out[0]
+    for r, f, s in out[1:]:
+        assert np.array_equal(r.view(np.uint64), r0.view(np.uint64))
+        assert np.array_equal(f.view(np.uint64), f0.view(np.uint64))
+        for a, b in zip(s, s0):
+            for k in a:
+                assert np.array_equal(a[k].view(np.uint32), b[k].view(np.uint32)), k
