@@ -195,79 +195,15 @@ __device__ __forceinline__ void mcc_body(const MccArgs &a, const int blk) {
   int ncand = 0;
   for (int k = 0; k < MCC_BLOCK / 64; ++k) ncand += wcount[k];
   __syncthreads();
-  /* phase 2a: acceptance of each candidate (one per thread); the accepted
-   * ones are listed by process, so that phase 2b runs each collision kind
-   * with whole waves */
-  __shared__ int acc_list[MCC_BLOCK];   /* local particle | process << 16 */
-  __shared__ int acc_cnt[ISK_MAX_COLL][MCC_BLOCK / 64];
-  __shared__ int prod_f[MCC_BLOCK];     /* by local particle: product made */
-  __shared__ float prod_v[MCC_BLOCK][9];
-  const int N = a.ncoll;
-  int acc_k = -1, acc_lt = 0;
-  prod_f[threadIdx.x] = 0;
+  int prod = 0;
+  float pe[3] = {0.f, 0.f, 0.f}, pi[3] = {0.f, 0.f, 0.f};
+  int32_t pcell = 0;
+  float pox = 0.f, poy = 0.f;
+  int evkey = -1; /* node * N + process of this thread's collision */
   if ((int)threadIdx.x < ncand) {
-    const int lt = cand_t[threadIdx.x];
-    const long long p = (long long)blk * MCC_BLOCK + lt;
+    const long long p = (long long)blk * MCC_BLOCK + cand_t[threadIdx.x];
     const double U = cand_u[threadIdx.x];
     const int32_t c = a.cell[p];
-    const int i = c & 0xFFFF, j = c >> 16;
-    const int node = a.dim == 2 ? j * a.nx + i : i;
-    const double n = a.tn[node];
-    if (n >= 0.0) {
-      int k = (int)floor((double)N * U);
-      if (k > N - 1) k = N - 1;
-      const double vx = a.vx[p], vy = a.vy[p], vz = a.vz[p];
-      double tvx = 0.0, tvy = 0.0;
-      if (a.tqm != 0.0) {
-        if (a.dim == 2) {
-          tvx = a.tqm * (double)a.E[2 * node] * a.dt;
-          tvy = a.tqm * (double)a.E[2 * node + 1] * a.dt;
-        } else {
-          tvx = a.tqm * (double)a.E[node] * a.dt;
-        }
-      }
-      const double gx = tvx - vx, gy = tvy - vy, gz = 0.0 - vz;
-      const double g = sqrt(gx * gx + gy * gy + gz * gz);
-      const double eps = (0.5 * a.m_eV) * (g * g);
-      const double sg = interp_sigma(a, k, eps) * g;
-      double P = 1.0 - isk_exp(((-n) * sg) * a.dt);
-      P /= a.candP;
-      if (P > 1.0) atomicOr(a.errflag, 1);
-      if (U > (double)(k + 1) / (double)N - P) {
-        acc_k = k;
-        acc_lt = lt;
-      }
-    }
-  }
-  int nacc = 0;
-  {
-    int mypos = 0;
-    for (int kk = 0; kk < N; ++kk) {
-      const unsigned long long mk = __ballot(acc_k == kk);
-      if (acc_k == kk) mypos = __popcll(mk & ((1ull << lane) - 1ull));
-      if (lane == 0) acc_cnt[kk][w] = __popcll(mk);
-    }
-    __syncthreads();
-    int before = 0; /* accepted of lower processes, then of this process in earlier waves */
-    for (int kk = 0; kk < N; ++kk)
-      for (int w2 = 0; w2 < MCC_BLOCK / 64; ++w2) {
-        const int v = acc_cnt[kk][w2];
-        if (kk < acc_k || (kk == acc_k && w2 < w)) before += v;
-        nacc += v;
-      }
-    if (acc_k >= 0) acc_list[before + mypos] = acc_lt | (acc_k << 16);
-  }
-  __syncthreads();
-  /* phase 2b: the collisions */
-  int evkey = -1; /* node * N + process of this thread's collision */
-  if ((int)threadIdx.x < nacc) {
-    const int e = acc_list[threadIdx.x];
-    const int lt = e & 0xFFFF, k = e >> 16;
-    const long long p = (long long)blk * MCC_BLOCK + lt;
-    const int32_t c = a.cell[p];
-    const int i = c & 0xFFFF, j = c >> 16;
-    const int node = a.dim == 2 ? j * a.nx + i : i;
-    const double vx = a.vx[p], vy = a.vy[p], vz = a.vz[p];
     Draws d;
     d.id = a.id[p];
     d.step = step;
@@ -276,68 +212,97 @@ __device__ __forceinline__ void mcc_body(const MccArgs &a, const int blk) {
     d.k1 = a.k1;
     d.block = 1;
     d.half = 0;
-    evkey = node * N + k;
-    const int type = a.types[k];
-    double sv[3] = {vx, vy, vz};
-    double out[3];
-    if (type == ISKRA_COLL_ELASTIC_ISOTROPIC || type == ISKRA_COLL_ELASTIC_BACKWARD) {
-      double mr1 = a.ms / (a.ms + a.mt), mr2 = a.mt / (a.ms + a.mt);
-      double tv[3];
-      randn3(d, tv);
-      for (int q = 0; q < 3; ++q) tv[q] *= a.vth_t;
-      double vr[3], wv[3];
-      for (int q = 0; q < 3; ++q) {
-        vr[q] = sv[q] - tv[q];
-        wv[q] = mr1 * sv[q] + mr2 * tv[q];
-      }
-      double nvr = sqrt(vr[0] * vr[0] + vr[1] * vr[1] + vr[2] * vr[2]);
-      double dir[3];
-      if (type == ISKRA_COLL_ELASTIC_ISOTROPIC)
-        isotropic_scattering(vr[0], vr[1], vr[2], d, dir);
-      else
-        diffuse_reflection(vr[0], vr[1], vr[2], d, dir);
-      for (int q = 0; q < 3; ++q) out[q] = wv[q] + mr2 * (nvr * dir[q]);
-      a.vx[p] = (float)out[0];
-      a.vy[p] = (float)out[1];
-      a.vz[p] = (float)out[2];
-    } else if (type == ISKRA_COLL_EXCITATION) {
-      double sE = (0.5 * a.m_eV) * (vx * vx + vy * vy + vz * vz) - a.energy[k];
-      if (sE >= 0.0) {
-        double al = sqrt(2.0 / a.m_eV);
-        double ev = al * sqrt(sE);
-        double dir[3];
-        isotropic_scattering(vx, vy, vz, d, dir);
-        a.vx[p] = (float)(ev * dir[0]);
-        a.vy[p] = (float)(ev * dir[1]);
-        a.vz[p] = (float)(ev * dir[2]);
-      }
-    } else if (type == ISKRA_COLL_IONIZATION) {
-      double sE = (0.5 * a.m_eV) * (vx * vx + vy * vy + vz * vz) - a.energy[k];
-      if (sE >= 0.0) {
-        double e1E = sE * d.next();
-        double e2E = sE - e1E;
-        double al = sqrt(2.0 / a.m_eV);
-        double e1v = al * sqrt(e1E), e2v = al * sqrt(e2E);
-        double dir[3];
-        diffuse_reflection(vx, vy, vz, d, dir);
-        double nv[3] = {e1v * dir[0], e1v * dir[1], e1v * dir[2]};
-        a.vx[p] = (float)nv[0];
-        a.vy[p] = (float)nv[1];
-        a.vz[p] = (float)nv[2];
-        double dir2[3];
-        diffuse_reflection(nv[0], nv[1], nv[2], d, dir2);
-        double tv[3];
-        randn3(d, tv);
-        /* the product goes to the parent's slot: products keep particle order */
-        float *pv = prod_v[lt];
-        pv[0] = __int_as_float(c);
-        pv[1] = a.ox[p];
-        pv[2] = a.dim == 2 ? a.oy[p] : 0.f;
-        for (int q = 0; q < 3; ++q) {
-          pv[3 + q] = (float)(e2v * dir2[q]);
-          pv[6 + q] = (float)(tv[q] * a.vth_t);
+    {
+      {
+        int i = c & 0xFFFF, j = c >> 16;
+        int node = a.dim == 2 ? j * a.nx + i : i;
+        double n = a.tn[node];
+        if (n >= 0.0) {
+          const int N = a.ncoll;
+          int k = (int)floor((double)N * U);
+          if (k > N - 1) k = N - 1;
+          double vx = a.vx[p], vy = a.vy[p], vz = a.vz[p];
+          double tvx = 0.0, tvy = 0.0;
+          if (a.tqm != 0.0) {
+            if (a.dim == 2) {
+              tvx = a.tqm * (double)a.E[2 * node] * a.dt;
+              tvy = a.tqm * (double)a.E[2 * node + 1] * a.dt;
+            } else {
+              tvx = a.tqm * (double)a.E[node] * a.dt;
+            }
+          }
+          double gx = tvx - vx, gy = tvy - vy, gz = 0.0 - vz;
+          double g = sqrt(gx * gx + gy * gy + gz * gz);
+          double eps = (0.5 * a.m_eV) * (g * g);
+          double sg = interp_sigma(a, k, eps) * g;
+          double P = 1.0 - isk_exp(((-n) * sg) * a.dt);
+          P /= a.candP;
+          if (P > 1.0) atomicOr(a.errflag, 1);
+          if (U > (double)(k + 1) / (double)N - P) {
+            evkey = node * N + k;
+            const int type = a.types[k];
+            double sv[3] = {vx, vy, vz};
+            double out[3];
+            if (type == ISKRA_COLL_ELASTIC_ISOTROPIC || type == ISKRA_COLL_ELASTIC_BACKWARD) {
+              double mr1 = a.ms / (a.ms + a.mt), mr2 = a.mt / (a.ms + a.mt);
+              double tv[3];
+              randn3(d, tv);
+              for (int q = 0; q < 3; ++q) tv[q] *= a.vth_t;
+              double vr[3], w[3];
+              for (int q = 0; q < 3; ++q) {
+                vr[q] = sv[q] - tv[q];
+                w[q] = mr1 * sv[q] + mr2 * tv[q];
+              }
+              double nvr = sqrt(vr[0] * vr[0] + vr[1] * vr[1] + vr[2] * vr[2]);
+              double dir[3];
+              if (type == ISKRA_COLL_ELASTIC_ISOTROPIC)
+                isotropic_scattering(vr[0], vr[1], vr[2], d, dir);
+              else
+                diffuse_reflection(vr[0], vr[1], vr[2], d, dir);
+              for (int q = 0; q < 3; ++q) out[q] = w[q] + mr2 * (nvr * dir[q]);
+              a.vx[p] = (float)out[0];
+              a.vy[p] = (float)out[1];
+              a.vz[p] = (float)out[2];
+            } else if (type == ISKRA_COLL_EXCITATION) {
+              double sE = (0.5 * a.m_eV) * (vx * vx + vy * vy + vz * vz) - a.energy[k];
+              if (sE >= 0.0) {
+                double al = sqrt(2.0 / a.m_eV);
+                double ev = al * sqrt(sE);
+                double dir[3];
+                isotropic_scattering(vx, vy, vz, d, dir);
+                a.vx[p] = (float)(ev * dir[0]);
+                a.vy[p] = (float)(ev * dir[1]);
+                a.vz[p] = (float)(ev * dir[2]);
+              }
+            } else if (type == ISKRA_COLL_IONIZATION) {
+              double sE = (0.5 * a.m_eV) * (vx * vx + vy * vy + vz * vz) - a.energy[k];
+              if (sE >= 0.0) {
+                double e1E = sE * d.next();
+                double e2E = sE - e1E;
+                double al = sqrt(2.0 / a.m_eV);
+                double e1v = al * sqrt(e1E), e2v = al * sqrt(e2E);
+                double dir[3];
+                diffuse_reflection(vx, vy, vz, d, dir);
+                double nv[3] = {e1v * dir[0], e1v * dir[1], e1v * dir[2]};
+                a.vx[p] = (float)nv[0];
+                a.vy[p] = (float)nv[1];
+                a.vz[p] = (float)nv[2];
+                double dir2[3];
+                diffuse_reflection(nv[0], nv[1], nv[2], d, dir2);
+                double tv[3];
+                randn3(d, tv);
+                prod = 1;
+                pcell = c;
+                pox = a.ox[p];
+                poy = a.dim == 2 ? a.oy[p] : 0.f;
+                for (int q = 0; q < 3; ++q) {
+                  pe[q] = (float)(e2v * dir2[q]);
+                  pi[q] = (float)(tv[q] * a.vth_t);
+                }
+              }
+            }
+          }
         }
-        prod_f[lt] = 1;
       }
     }
   }
@@ -363,9 +328,7 @@ __device__ __forceinline__ void mcc_body(const MccArgs &a, const int blk) {
   __syncthreads();
   if ((int)threadIdx.x < a.ncoll && evl[threadIdx.x])
     atomicAdd(&a.events[threadIdx.x], (unsigned long long)evl[threadIdx.x]);
-  /* deterministic block compaction of the ionisation products, in the
-   * order of their parents (thread t reads the slot of local particle t) */
-  const int prod = prod_f[threadIdx.x];
+  /* deterministic block compaction of the ionisation products */
   unsigned long long m = __ballot(prod != 0);
   int rank = __popcll(m & ((1ull << lane) - 1ull));
   if (lane == 0) wcount[w] = __popcll(m);
@@ -373,11 +336,17 @@ __device__ __forceinline__ void mcc_body(const MccArgs &a, const int blk) {
   int off = 0;
   for (int k = 0; k < w; ++k) off += wcount[k];
   if (prod) {
-    const float *pv = prod_v[threadIdx.x];
     size_t s = (size_t)blk * MCC_BLOCK + off + rank;
-    a.stage_cell[s] = __float_as_int(pv[0]);
+    a.stage_cell[s] = pcell;
     float *st = a.stage + s * 8;
-    for (int q = 0; q < 8; ++q) st[q] = pv[1 + q];
+    st[0] = pox;
+    st[1] = poy;
+    st[2] = pe[0];
+    st[3] = pe[1];
+    st[4] = pe[2];
+    st[5] = pi[0];
+    st[6] = pi[1];
+    st[7] = pi[2];
   }
   if (threadIdx.x == 0) {
     int tot = 0;
