@@ -731,7 +731,8 @@ __global__ __launch_bounds__(1024) void kb_mcc_scan(const CommitArgs *A, const i
 __global__ __launch_bounds__(MCC_BLOCK) void kb_mcc_append(const AppendArgs *A, const int *bstart,
                                                             int n) {
   const int i = isk_bfind(bstart, n, blockIdx.x);
-  mcc_append_body(A[i], blockIdx.x - bstart[i]);
+  const AppendArgs a = A[i];
+  mcc_append_body(a, blockIdx.x - bstart[i]);
 }
 
 __global__ void kb_mcc_commit(const CommitArgs *A, const int *bstart, int n) {

@@ -1068,7 +1068,10 @@ template <int DIM>
 __global__ __launch_bounds__(PUSH_BLOCK) void k_push_multi(PushMulti m) {
   int s = 0;
   while (s + 1 < m.n && (int)blockIdx.x >= m.bstart[s + 1]) ++s;
-  push_body<DIM>(m.a[s], blockIdx.x - m.bstart[s]);
+  /* a copy in registers: read through a reference into the argument block,
+   * the fields would be re-read from memory after every particle's stores */
+  const PushArgs a = m.a[s];
+  push_body<DIM>(a, blockIdx.x - m.bstart[s]);
 }
 
 static int push_launch_all(iskra_ctx *c, double dt) {
@@ -1485,12 +1488,14 @@ __global__ void kb_fill(const FillArgs *A, const int *bstart, int n) {
 __global__ __launch_bounds__(PUSH_BLOCK) void kb_push1(const PushArgs *A, const int *bstart,
                                                         int n) {
   const int i = isk_bfind(bstart, n, blockIdx.x);
-  push_body<1>(A[i], blockIdx.x - bstart[i]);
+  const PushArgs a = A[i]; /* in registers, not re-read after each store */
+  push_body<1>(a, blockIdx.x - bstart[i]);
 }
 
 __global__ __launch_bounds__(256) void kb_merge1(const MergeArgs *A, const int *bstart, int n) {
   const int i = isk_bfind(bstart, n, blockIdx.x);
-  merge_body<1>(A[i], blockIdx.x - bstart[i], bstart[i + 1] - bstart[i]);
+  const MergeArgs a = A[i];
+  merge_body<1>(a, blockIdx.x - bstart[i], bstart[i + 1] - bstart[i]);
 }
 
 __global__ void kb_rho(const RhoArgs *A, const int *bstart, int n) {
