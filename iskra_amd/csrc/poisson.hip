@@ -41,8 +41,9 @@
 
 enum { S_RZ = 0, S_PAP, S_ALPHA, S_BETA, S_RR, S_FF, S_TOL2, S_MEAN, S_RELRES, S_RZNEW, S_WORDS = 16 };
 /* iflag words: [0] done, [1] iterations, [2] singular (PCG/MG); spectral solve:
- * [4] this solve's pass met rtol, [5] sticky: an in-loop solve missed rtol */
-enum { IF_SP_DONE = 4, IF_SP_FAIL = 5 };
+ * [4] this solve met rtol, [5] sticky: an in-loop solve missed rtol,
+ * [6] transform passes of this solve */
+enum { IF_SP_DONE = 4, IF_SP_FAIL = 5, IF_SP_PASSES = 6 };
 
 
 template <typename T>
@@ -616,17 +617,29 @@ __device__ __forceinline__ void sp_resid_node(const PcgArgs &a, int n, float *E,
   if (E) efield_node(a.x, E, 2, a.s.nx, a.s.ny, dx, dy, n);
 }
 
-/* r = f_elim - L x, FP64 relative residual -> scal[S_RELRES], iflag[IF_SP_DONE]
- * = (relres <= rtol); phi = x; with E != null also E = -grad phi */
-__global__ void k_sp_resid(PcgArgs a, double rtol, float *E, double dx, double dy) {
+/* r = f_elim - L x; phi = x; with E != null also E = -grad phi.  Each
+ * work-group writes its sums of r^2 and f_elim^2 to partials[2b], [2b+1]
+ * (summed in a fixed order by k_sp_correct, which runs next: no device-wide
+ * fence and counter here) */
+__global__ __launch_bounds__(RED_BLOCK) void k_sp_resid(PcgArgs a, float *E, double dx,
+                                                        double dy) {
+  __shared__ double ra[RED_BLOCK], rb[RED_BLOCK];
   const int n = blockIdx.x * blockDim.x + threadIdx.x;
   double rr = 0.0, ff = 0.0;
   if (n < a.s.nx * a.s.ny) sp_resid_node(a, n, E, dx, dy, &rr, &ff);
-  double srr, sff;
-  if (block_reduce2(rr, ff, a.partials, a.counter, &srr, &sff) && threadIdx.x == 0) {
-    const double rel = sff > 0.0 ? sqrt(srr / sff) : 0.0;
-    a.scal[S_RELRES] = rel;
-    a.iflag[IF_SP_DONE] = rel <= rtol;
+  ra[threadIdx.x] = rr;
+  rb[threadIdx.x] = ff;
+  __syncthreads();
+  for (int off = RED_BLOCK / 2; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off) {
+      ra[threadIdx.x] += ra[threadIdx.x + off];
+      rb[threadIdx.x] += rb[threadIdx.x + off];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    a.partials[2 * blockIdx.x] = ra[0];
+    a.partials[2 * blockIdx.x + 1] = rb[0];
   }
 }
 
@@ -638,21 +651,63 @@ __global__ void k_sp_resid(PcgArgs a, double rtol, float *E, double dx, double d
  * a correction is only needed when rtol is below that (then it takes about a
  * millisecond at 257^2).  second != 0: a miss after the correction sets
  * iflag[IF_SP_FAIL]. */
+#define SPC_THREADS 1024
 struct SpCorrect {
   SpGemm g[4];
   PcgArgs a;
   double rtol, dx, dy;
   float *E;
   int second;
+  int nparts; /* > 0: first sum k_sp_resid's partials (the first pass's residual) */
 };
 
-#define SPC_THREADS 1024
+/* fixed-order sum over the work-group of two values per thread; every
+ * thread gets the totals */
+__device__ __forceinline__ void spc_sum2(double *x, double *y, double (*sred)[SPC_THREADS / 64]) {
+  double rr = *x, ff = *y;
+  for (int off = 32; off > 0; off >>= 1) {
+    rr += __shfl_xor(rr, off);
+    ff += __shfl_xor(ff, off);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    sred[0][threadIdx.x >> 6] = rr;
+    sred[1][threadIdx.x >> 6] = ff;
+  }
+  __syncthreads();
+  double srr = 0.0, sff = 0.0;
+  for (int k = 0; k < SPC_THREADS / 64; ++k) {
+    srr += sred[0][k];
+    sff += sred[1][k];
+  }
+  __syncthreads();
+  *x = srr;
+  *y = sff;
+}
+
 __global__ __launch_bounds__(SPC_THREADS) void k_sp_correct(SpCorrect c) {
-  if (c.a.iflag[IF_SP_DONE]) return;
   __shared__ double red[4][4][256];
   __shared__ double sred[2][SPC_THREADS / 64];
   const int tid = threadIdx.x;
   const int lane = tid & 63, wv = tid >> 6, grp = wv >> 2, w = wv & 3;
+  bool done;
+  if (c.nparts > 0) {
+    double rr = 0.0, ff = 0.0;
+    for (int k = tid; k < c.nparts; k += SPC_THREADS) {
+      rr += c.a.partials[2 * k];
+      ff += c.a.partials[2 * k + 1];
+    }
+    spc_sum2(&rr, &ff, sred);
+    const double rel = ff > 0.0 ? sqrt(rr / ff) : 0.0;
+    done = rel <= c.rtol;
+    if (tid == 0) {
+      c.a.scal[S_RELRES] = rel;
+      c.a.iflag[IF_SP_DONE] = done;
+      c.a.iflag[IF_SP_PASSES] = 1;
+    }
+  } else {
+    done = c.a.iflag[IF_SP_DONE] != 0;
+  }
+  if (done) return;
   for (int st = 0; st < 4; ++st) {
     const SpGemm &g = c.g[st];
     const int tx = (g.N + 15) / 16, ntile = tx * ((g.M + 15) / 16);
@@ -675,26 +730,14 @@ __global__ __launch_bounds__(SPC_THREADS) void k_sp_correct(SpCorrect c) {
   const int nn = c.a.s.nx * c.a.s.ny;
   double rr = 0.0, ff = 0.0;
   for (int n = tid; n < nn; n += SPC_THREADS) sp_resid_node(c.a, n, c.E, c.dx, c.dy, &rr, &ff);
-  for (int off = 32; off > 0; off >>= 1) {
-    rr += __shfl_xor(rr, off);
-    ff += __shfl_xor(ff, off);
-  }
-  if (lane == 0) {
-    sred[0][wv] = rr;
-    sred[1][wv] = ff;
-  }
-  __syncthreads();
+  spc_sum2(&rr, &ff, sred);
   if (tid == 0) {
-    double srr = 0.0, sff = 0.0;
-    for (int k = 0; k < SPC_THREADS / 64; ++k) {
-      srr += sred[0][k];
-      sff += sred[1][k];
-    }
-    const double rel = sff > 0.0 ? sqrt(srr / sff) : 0.0;
+    const double rel = ff > 0.0 ? sqrt(rr / ff) : 0.0;
     c.a.scal[S_RELRES] = rel;
-    const int done = rel <= c.rtol;
-    c.a.iflag[IF_SP_DONE] = done;
-    if (c.second && !done) c.a.iflag[IF_SP_FAIL] = 1;
+    const int ok = rel <= c.rtol;
+    c.a.iflag[IF_SP_DONE] = ok;
+    c.a.iflag[IF_SP_PASSES] += 1;
+    if (c.second && !ok) c.a.iflag[IF_SP_FAIL] = 1;
   }
 }
 
@@ -1542,8 +1585,8 @@ int isk_poisson_solve_ex(iskra_ctx *c, int warm, int *iters, double *relres, int
                          c->stream, g[k]);
       ISK_CHECK_LAUNCH();
     }
-    hipLaunchKernelGGL(k_sp_resid, dim3(nblk), dim3(RED_BLOCK), 0, c->stream, a, p.rtol, E,
-                       c->g.dx, c->g.dy);
+    hipLaunchKernelGGL(k_sp_resid, dim3(nblk), dim3(RED_BLOCK), 0, c->stream, a, E, c->g.dx,
+                       c->g.dy);
     ISK_CHECK_LAUNCH();
     SpCorrect sc;
     sp_pass_desc(c, a, 1, sc.g);
@@ -1552,14 +1595,14 @@ int isk_poisson_solve_ex(iskra_ctx *c, int warm, int *iters, double *relres, int
     sc.dx = c->g.dx;
     sc.dy = c->g.dy;
     sc.E = E;
+    sc.nparts = nblk; /* the first call sums the residual of the first pass */
+    sc.second = (!iters && !relres) ? 1 : 0;
+    hipLaunchKernelGGL(k_sp_correct, dim3(1), dim3(SPC_THREADS), 0, c->stream, sc);
+    ISK_CHECK_LAUNCH();
     int rounds = 1;
     double rel = 0.0;
-    if (!iters && !relres) {
-      sc.second = 1;
-      hipLaunchKernelGGL(k_sp_correct, dim3(1), dim3(SPC_THREADS), 0, c->stream, sc);
-      ISK_CHECK_LAUNCH();
-    } else {
-      sc.second = 0;
+    if (iters || relres) {
+      sc.nparts = 0;
       while (true) {
         double scal[S_WORDS];
         int fl[8];
@@ -1567,6 +1610,7 @@ int isk_poisson_solve_ex(iskra_ctx *c, int warm, int *iters, double *relres, int
         ISK_HIP(hipMemcpyAsync(fl, p.iflag, sizeof(fl), hipMemcpyDeviceToHost, c->stream));
         ISK_HIP(hipStreamSynchronize(c->stream));
         rel = scal[S_RELRES];
+        rounds = fl[IF_SP_PASSES];
         if (fl[IF_SP_DONE]) break;
         if (rounds >= 4) {
           isk_set_error("poisson: spectral solve reached relative residual %.3g > %.3g", rel,
@@ -1575,7 +1619,6 @@ int isk_poisson_solve_ex(iskra_ctx *c, int warm, int *iters, double *relres, int
         }
         hipLaunchKernelGGL(k_sp_correct, dim3(1), dim3(SPC_THREADS), 0, c->stream, sc);
         ISK_CHECK_LAUNCH();
-        ++rounds;
       }
     }
     if (E_done) *E_done = with_E;

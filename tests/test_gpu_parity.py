@@ -392,3 +392,38 @@ def test_spectral_in_loop_equals_direct_call(gpu_ready):
     assert ei.value.code == L.E_CONVERGE
     ctx.sync()
     ctx.close()
+
+
+def test_poisson_spectral_correction_pass(gpu_ready, orc):
+    """rtol below one transform pass's FP64 round-off (~6e-15 at 129^2): the
+    solve makes a correction pass (the one-work-group kernel) and meets rtol,
+    matches the banded direct solve of the reference system, and the time
+    loop (same kernels, no read-back) leaves the same potential bit for bit."""
+    from iskra_amd import Context, _lib as L
+    rng = np.random.default_rng(12)
+    nodes, dx, eps0 = 129, 1e-4, 8.854e-12
+    tags = np.zeros((nodes, nodes), np.int8)
+    tags[0, :] = tags[-1, :] = tags[:, 0] = 1
+    tags[:, -1] = 2
+    bg = rng.standard_normal(nodes * nodes) * 1e-6
+    phis = []
+    for in_loop in (False, True):
+        ctx = Context()
+        ctx.grid(2, (nodes, nodes), (dx, dx))
+        ctx.poisson(L.POISSON_SPECTRAL, eps0, tags.ravel(), rtol=2e-15)
+        ctx.tag_value(2, 1.5)
+        ctx.set_field(L.F_BG, bg)
+        if in_loop:
+            ctx.step(1e-12, 1)  # no particles: rho = background
+            s = ctx.stats()
+            assert s["last_relres"] <= 2e-15, s
+        else:
+            ctx.step(1e-12, 1)
+            it, rr = ctx.solve(L.WARM_ZERO)
+            assert it >= 2 and rr <= 2e-15, (it, rr)
+            rho = ctx.field(L.F_RHO)
+            ref = orc.potential_banded(nodes, nodes, dx, tags, [0.0, 0.0, 1.5], rho, eps0)
+            assert np.max(np.abs(ctx.field(L.F_PHI) - ref)) <= 1e-10 * np.max(np.abs(ref))
+        phis.append(ctx.field(L.F_PHI))
+        ctx.close()
+    assert np.array_equal(phis[0].view(np.uint64), phis[1].view(np.uint64))
