@@ -211,7 +211,8 @@ def main():
             if t.get("cells") == args.cells and t.get("particles_per_species") == n_species:
                 traffic = t.get("hbm_bytes_per_launch")
         out["roofline"] = {
-            "bound": "hbm", "kernel": "k_push<2> (fused gather+push+boundary+deposit)",
+            "bound": "hbm",
+            "kernel": "k_push_multi<2> (fused gather+push+boundary+deposit, both species per launch)",
             "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
             "frac": achieved / PEAK_HBM_GBS, "traffic": traffic,
             "bytes_per_particle_step": BYTES_PER_PSTEP_2D,

@@ -351,6 +351,14 @@ struct IskSpPack {
   double scale;
 };
 
+/* F at interior node (i, j).  The spectral solve's Dirichlet nodes are the
+ * boundary ring, so only ring neighbours contribute: nodes two or more away
+ * from it add +0.0, which is what dirichlet_rhs returns for them */
+__device__ __forceinline__ double isk_sp_pack_value(const IskSpPack &pk, double rho, int i, int j) {
+  const bool near = i == 1 || j == 1 || i == pk.s.nx - 2 || j == pk.s.ny - 2;
+  return rho * pk.scale + (near ? dirichlet_rhs(pk.s, i, j) : 0.0);
+}
+
 /* work-group barrier that orders LDS accesses only: global loads still in
  * flight stay in flight across it (a __syncthreads after a global store
  * waits for every outstanding global access of the wave, loads included) */

@@ -1344,7 +1344,7 @@ __global__ __launch_bounds__(256) void k_merge_rho(MergeRhoArgs a) {
   if (DIM == 2 && a.pk.F) {
     const int nx = a.pk.s.nx, i = node % nx, j = node / nx;
     if (i > 0 && j > 0 && i < nx - 1 && j < a.pk.s.ny - 1)
-      a.pk.F[(j - 1) * (nx - 2) + (i - 1)] = rho * a.pk.scale + dirichlet_rhs(a.pk.s, i, j);
+      a.pk.F[(j - 1) * (nx - 2) + (i - 1)] = isk_sp_pack_value(a.pk, rho, i, j);
   }
 }
 

@@ -463,7 +463,7 @@ __global__ void k_sp_pack(IskSpPack pk, const double *rho) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= mx * my) return;
   const int i = e % mx + 1, j = e / mx + 1;
-  pk.F[e] = rho[j * pk.s.nx + i] * pk.scale + dirichlet_rhs(pk.s, i, j);
+  pk.F[e] = isk_sp_pack_value(pk, rho[j * pk.s.nx + i], i, j);
 }
 
 struct SpGemm {
