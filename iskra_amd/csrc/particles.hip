@@ -432,6 +432,10 @@ __device__ __forceinline__ void scatter_load(const ScatterArgs &a, long long p, 
   q.id = a.id ? a.id[pc] : 0u;
 }
 
+/* The group lists are double-buffered by round parity, so a round needs two
+ * barriers: after the lists are written (then base is read), and after base
+ * has been read (then it is advanced); the next round's first barrier orders
+ * the advance before the next reads. */
 __device__ __forceinline__ void scatter_round(const ScatterArgs &a, const SortPay &q, bool valid,
                                               int *base, int (*gkey)[64], int (*gcnt)[64],
                                               int *ng, int lane, int w) {
@@ -471,16 +475,15 @@ __device__ __forceinline__ void scatter_round(const ScatterArgs &a, const SortPa
   }
   isk_lds_barrier();
   if (lane < ng[w]) atomicAdd(&base[gkey[w][lane]], gcnt[w][lane]);
-  isk_lds_barrier();
 }
 
 static inline size_t sort_scatter_shm(int nb1) { return (size_t)nb1 * sizeof(int); }
 
 __device__ void sort_scatter_body(const ScatterArgs &a, const int blk) {
   extern __shared__ int base[];
-  __shared__ int gkey[4][64];
-  __shared__ int gcnt[4][64];
-  __shared__ int ng[4];
+  __shared__ int gkey[2][4][64];
+  __shared__ int gcnt[2][4][64];
+  __shared__ int ng[2][4];
   const int nb1 = a.kd.nbins + 1;
   const int seg = blk / SORT_SEG;
   for (int b = threadIdx.x; b < nb1; b += SORT_BLOCK)
@@ -492,16 +495,17 @@ __device__ void sort_scatter_body(const ScatterArgs &a, const int blk) {
   const int lane = threadIdx.x & 63;
   const int w = threadIdx.x >> 6;
   /* two records in registers: round r+1 is requested before round r is
-   * ranked and written, so its loads overlap that work (SORT_PT is even) */
+   * ranked and written, so its loads overlap that work (SORT_PT is even);
+   * rounds r and r+1 use the two group-list buffers */
   SortPay A, B;
   scatter_load(a, b0 + threadIdx.x, np, A);
   for (int r = 0; r < SORT_PT; r += 2) {
     const long long pa = b0 + (long long)r * SORT_BLOCK + threadIdx.x;
     const long long pb = pa + SORT_BLOCK;
     scatter_load(a, pb, np, B);
-    scatter_round(a, A, pa < np, base, gkey, gcnt, ng, lane, w);
+    scatter_round(a, A, pa < np, base, gkey[0], gcnt[0], ng[0], lane, w);
     if (r + 2 < SORT_PT) scatter_load(a, pb + SORT_BLOCK, np, A);
-    scatter_round(a, B, pb < np, base, gkey, gcnt, ng, lane, w);
+    scatter_round(a, B, pb < np, base, gkey[1], gcnt[1], ng[1], lane, w);
   }
 }
 
