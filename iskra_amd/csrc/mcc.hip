@@ -27,6 +27,10 @@
 #include "isk_internal.h"
 
 #define MCC_BLOCK 256
+/* particles per MCC work-group: MCC_RPT rounds of MCC_BLOCK (the loads of
+ * all rounds are issued together) */
+#define MCC_RPT 4
+#define MCC_PPB (MCC_BLOCK * MCC_RPT)
 #define KB_MCC 1.3806503e-23      /* mcc.jl:75 */
 #define QE_MCC 1.60217646e-19     /* mcc.jl:26 */
 #define TWO_PI 6.283185307179586
@@ -148,27 +152,43 @@ __device__ void randn3(Draws &d, double *n) {
   n[2] = r2 * c2;
 }
 
-/* Two phases per work-group.  Every live particle draws (Uc, U) from
- * Philox block 0 of its stream; the candidates (Uc < candP) are listed in
- * LDS in particle order, and the candidate work is then done by the first
- * threads of the group, one candidate each, so that waves run it with all
- * lanes busy instead of one or two lanes of each wave.  Each candidate's
- * draws continue at Philox block 1, as they would in one pass, and products
- * keep particle order (the list is in particle order). */
+/* Two phases per work-group of MCC_PPB particles.  Every live particle
+ * draws (Uc, U) from Philox block 0 of its stream (the cell and id loads of
+ * the work-group's MCC_RPT rounds are issued together); the candidates
+ * (Uc < candP) are listed in LDS in particle order, and the candidate work
+ * is then done one candidate per thread, MCC_BLOCK at a time, so that waves
+ * run it with all lanes busy instead of one or two lanes of each wave.  Each
+ * candidate's draws continue at Philox block 1, as they would in one pass,
+ * and products keep particle order (the list is in particle order). */
 __device__ __forceinline__ void mcc_body(const MccArgs &a, const int blk) {
   __shared__ int wcount[MCC_BLOCK / 64];
-  __shared__ int cand_t[MCC_BLOCK];
-  __shared__ double cand_u[MCC_BLOCK];
+  __shared__ int cand_t[MCC_PPB];
+  __shared__ double cand_u[MCC_PPB];
+  __shared__ unsigned int evl[ISK_MAX_COLL];
   const long long np = a.hdr[HDR_NP];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const uint32_t step = a.step_dev ? (uint32_t)(a.step + *a.step_dev) : a.step;
-  bool cand = false;
-  double U1 = 0.0;
-  {
-    const long long p = (long long)blk * MCC_BLOCK + threadIdx.x;
-    if (p < np && a.cell[p] >= 0) {
+  const long long b0 = (long long)blk * MCC_PPB;
+  if (threadIdx.x < ISK_MAX_COLL) evl[threadIdx.x] = 0u;
+  /* phase 1: cell and id of every round first (loads in flight together) */
+  int32_t cr[MCC_RPT];
+  uint32_t idr[MCC_RPT];
+#pragma unroll
+  for (int r = 0; r < MCC_RPT; ++r) {
+    const long long p = b0 + r * MCC_BLOCK + threadIdx.x;
+    const long long pc = p < np ? p : (np > 0 ? np - 1 : 0);
+    cr[r] = np > 0 ? a.cell[pc] : -1;
+    idr[r] = np > 0 ? a.id[pc] : 0u;
+  }
+  int ncand = 0;
+#pragma unroll
+  for (int r = 0; r < MCC_RPT; ++r) {
+    const int lt = r * MCC_BLOCK + threadIdx.x;
+    bool cand = false;
+    double U1 = 0.0;
+    if (b0 + lt < np && cr[r] >= 0) {
       Draws d;
-      d.id = a.id[p];
+      d.id = idr[r];
       d.step = step;
       d.stream = a.stream;
       d.k0 = a.k0;
@@ -179,30 +199,36 @@ __device__ __forceinline__ void mcc_body(const MccArgs &a, const int blk) {
       U1 = d.next();
       cand = Uc < a.candP;
     }
-  }
-  {
+    /* candidates listed in particle order */
     const unsigned long long m = __ballot(cand);
     const int rank = __popcll(m & ((1ull << lane) - 1ull));
     if (lane == 0) wcount[w] = __popcll(m);
     __syncthreads();
-    int off = 0;
+    int off = ncand;
     for (int k = 0; k < w; ++k) off += wcount[k];
     if (cand) {
-      cand_t[off + rank] = threadIdx.x;
+      cand_t[off + rank] = lt;
       cand_u[off + rank] = U1;
     }
+    for (int k = 0; k < MCC_BLOCK / 64; ++k) ncand += wcount[k];
+    __syncthreads();
   }
-  int ncand = 0;
-  for (int k = 0; k < MCC_BLOCK / 64; ++k) ncand += wcount[k];
-  __syncthreads();
+  /* phase 2: one candidate per thread, MCC_BLOCK at a time; the list is in
+   * particle order, so ionisation products, ranked per pass in candidate
+   * order, come out in particle order */
+  int nprod = 0;
+  for (int t0 = 0; t0 < ncand; t0 += MCC_BLOCK) {
+  const int tcand = t0 + (int)threadIdx.x;
   int prod = 0;
   float pe[3] = {0.f, 0.f, 0.f}, pi[3] = {0.f, 0.f, 0.f};
   int32_t pcell = 0;
   float pox = 0.f, poy = 0.f;
   int evkey = -1; /* node * N + process of this thread's collision */
-  if ((int)threadIdx.x < ncand) {
-    const long long p = (long long)blk * MCC_BLOCK + cand_t[threadIdx.x];
-    const double U = cand_u[threadIdx.x];
+  int lt = 0;
+  if (tcand < ncand) {
+    lt = cand_t[tcand];
+    const long long p = b0 + lt;
+    const double U = cand_u[tcand];
     const int32_t c = a.cell[p];
     Draws d;
     d.id = a.id[p];
@@ -306,12 +332,32 @@ __device__ __forceinline__ void mcc_body(const MccArgs &a, const int blk) {
       }
     }
   }
+  {
+    const unsigned long long m = __ballot(prod != 0);
+    const int rank = __popcll(m & ((1ull << lane) - 1ull));
+    if (lane == 0) wcount[w] = __popcll(m);
+    __syncthreads();
+    int off = nprod;
+    for (int k = 0; k < w; ++k) off += wcount[k];
+    if (prod) {
+      const size_t sidx = (size_t)b0 + off + rank;
+      a.stage_cell[sidx] = pcell;
+      float *st = a.stage + sidx * 8;
+      st[0] = pox;
+      st[1] = poy;
+      st[2] = pe[0];
+      st[3] = pe[1];
+      st[4] = pe[2];
+      st[5] = pi[0];
+      st[6] = pi[1];
+      st[7] = pi[2];
+    }
+    for (int k = 0; k < MCC_BLOCK / 64; ++k) nprod += wcount[k];
+    __syncthreads();
+  }
   /* event counts: equal (node, process) keys of a wave are counted with one
    * atomic, the per-process totals of the work-group go through LDS (integer
    * sums, so the grouping does not change them) */
-  __shared__ unsigned int evl[ISK_MAX_COLL];
-  if (threadIdx.x < ISK_MAX_COLL) evl[threadIdx.x] = 0u;
-  __syncthreads();
   {
     unsigned long long todo = __ballot(evkey >= 0);
     while (todo) {
@@ -325,34 +371,11 @@ __device__ __forceinline__ void mcc_body(const MccArgs &a, const int blk) {
       todo &= ~mk;
     }
   }
+  } /* phase 2 */
   __syncthreads();
   if ((int)threadIdx.x < a.ncoll && evl[threadIdx.x])
     atomicAdd(&a.events[threadIdx.x], (unsigned long long)evl[threadIdx.x]);
-  /* deterministic block compaction of the ionisation products */
-  unsigned long long m = __ballot(prod != 0);
-  int rank = __popcll(m & ((1ull << lane) - 1ull));
-  if (lane == 0) wcount[w] = __popcll(m);
-  __syncthreads();
-  int off = 0;
-  for (int k = 0; k < w; ++k) off += wcount[k];
-  if (prod) {
-    size_t s = (size_t)blk * MCC_BLOCK + off + rank;
-    a.stage_cell[s] = pcell;
-    float *st = a.stage + s * 8;
-    st[0] = pox;
-    st[1] = poy;
-    st[2] = pe[0];
-    st[3] = pe[1];
-    st[4] = pe[2];
-    st[5] = pi[0];
-    st[6] = pi[1];
-    st[7] = pi[2];
-  }
-  if (threadIdx.x == 0) {
-    int tot = 0;
-    for (int k = 0; k < MCC_BLOCK / 64; ++k) tot += wcount[k];
-    a.blk_cnt[blk] = tot;
-  }
+  if (threadIdx.x == 0) a.blk_cnt[blk] = nprod;
 }
 
 __global__ __launch_bounds__(MCC_BLOCK) void k_mcc(MccArgs a) { mcc_body(a, blockIdx.x); }
@@ -430,7 +453,7 @@ __device__ __forceinline__ void mcc_append_body(const AppendArgs &a, const int b
 
 /* product g = product k of MCC work-group b */
 __device__ __forceinline__ void mcc_append_one(const AppendArgs &a, int g, int b, int k) {
-  const size_t s = (size_t)b * MCC_BLOCK + k;
+  const size_t s = (size_t)b * MCC_PPB + k;
   const float *st = a.stage + s * 8;
   const int32_t c = a.stage_cell[s];
   long long enp = a.ehdr[HDR_NP];
@@ -575,7 +598,7 @@ static int mcc_prepare(iskra_ctx *c, int idx, double dt, int b_src, int b_ion, M
     return ISKRA_E_PROB;
   }
   int64_t cap = s.cap;
-  int nblk = (int)((cap + MCC_BLOCK - 1) / MCC_BLOCK);
+  int nblk = (int)((cap + MCC_PPB - 1) / MCC_PPB);
   if (m.stage_cap < cap) {
     isk_free(m.stage);
     isk_free(m.blk_cnt);
@@ -583,9 +606,9 @@ static int mcc_prepare(iskra_ctx *c, int idx, double dt, int b_src, int b_ion, M
     m.stage = nullptr;
     m.blk_cnt = nullptr;
     m.prod_flag = nullptr;
-    if (isk_alloc((void **)&m.stage, (size_t)nblk * MCC_BLOCK * 8 * 4) ||
+    if (isk_alloc((void **)&m.stage, (size_t)nblk * MCC_PPB * 8 * 4) ||
         isk_alloc((void **)&m.blk_cnt, (size_t)(nblk + 1) * 4) ||
-        isk_alloc((void **)&m.prod_flag, (size_t)nblk * MCC_BLOCK * 4))
+        isk_alloc((void **)&m.prod_flag, (size_t)nblk * MCC_PPB * 4))
       return ISKRA_E_HIP;
     m.stage_cap = cap;
   }
