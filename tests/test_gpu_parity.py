@@ -427,3 +427,56 @@ def test_poisson_spectral_correction_pass(gpu_ready, orc):
         phis.append(ctx.field(L.F_PHI))
         ctx.close()
     assert np.array_equal(phis[0].view(np.uint64), phis[1].view(np.uint64))
+
+
+def test_step_deposit_2d_two_species_bitexact(gpu_ready, orc):
+    """The time loop's 2-D path: one push launch for both species, the tile
+    windows of both summed in one launch, counts + rho in another.  After one
+    iskra_step in a preset E, particle states, per-species densities and rho
+    equal the oracle mirror bit for bit (rho = 0 + q_e n_e + q_i n_i + bg in
+    FP64, the kernel's order)."""
+    from iskra_amd import Context, _lib as L
+    rng = np.random.default_rng(31)
+    nx, ny = 65, 49
+    dx, dy = 1e-3, 1e-3
+    dt = 1e-11
+    sp = [(-1.6e-19, 9.1e-31, 1e6, 0.6, (L.BND_DISCARD, L.BND_WRAP)),
+          (1.6e-19, 6.6e-26, 2e6, 0.02, (L.BND_DISCARD, L.BND_DISCARD))]
+    ctx = Context()
+    ctx.grid(2, (nx, ny), (dx, dy))
+    ctx.tuning(tile=8, chunk=1024, margin=2, sort_every=1000)
+    tags = np.zeros((ny, nx), np.int8)
+    tags[0, :] = tags[-1, :] = tags[:, 0] = tags[:, -1] = 1
+    ctx.poisson(L.POISSON_SPECTRAL, 8.854e-12, tags.ravel(), rtol=1e-8)
+    ctx.tag_value(1, 0.0)
+    bg = rng.standard_normal(nx * ny) * 1e-9
+    ctx.set_field(L.F_BG, bg)
+    sids, st0 = [], []
+    for k, (q, m, w0, vc, bnd) in enumerate(sp):
+        n = 40000 + 7 * k
+        sid = ctx.species(q, m, w0, n + 16, track_ids=True)
+        ctx.boundary(sid, list(bnd))
+        ctx.set_state(sid, *_rand_state(rng, n, nx - 1, ny - 1, vc * dx / dt, vc * dy / dt))
+        sids.append(sid)
+        st0.append(ctx.get_state(sid))
+    Ex = rng.standard_normal(nx * ny) * 3e4
+    Ey = rng.standard_normal(nx * ny) * 3e4
+    ctx.set_field(L.F_EX, Ex)
+    ctx.set_field(L.F_EY, Ey)
+    E32 = np.empty(2 * nx * ny, dtype=np.float32)
+    E32[0::2] = Ex.astype(np.float32)
+    E32[1::2] = Ey.astype(np.float32)
+    V = orc.cell_volume(2, nx, ny, dx, dy)
+    ctx.step(dt, 1)
+    r = np.zeros(nx * ny)
+    for k, (q, m, w0, vc, bnd) in enumerate(sp):
+        ms, cnt, live = _mirror_push(orc, 2, st0[k], E32, nx, nx - 1, ny - 1, q, m, dt, dx, dy,
+                                     bnd)
+        st1 = ctx.get_state(sids[k])
+        for f in ("cell", "ox", "oy", "vx", "vy"):
+            assert np.array_equal(st1[f].view(np.uint32), ms[f].view(np.uint32)), (k, f)
+        dens = (cnt.astype(np.float64) * FIX) * w0 / V
+        assert np.array_equal(ctx.field(L.F_DENSITY, sids[k]), dens), k
+        r = r + q * dens
+    assert np.array_equal(ctx.field(L.F_RHO).view(np.uint64), (r + bg).view(np.uint64))
+    ctx.close()
