@@ -151,6 +151,7 @@ static void batch_free_launches(iskra_batch *b) {
 static int batch_launch(const IskBLaunch &L, hipStream_t st) {
   switch (L.kind) {
     case ISK_B_MCC:
+    case ISK_B_MCC_COLL:
     case ISK_B_MCC_SCAN:
     case ISK_B_MCC_APPEND:
     case ISK_B_MCC_COMMIT:
@@ -176,7 +177,7 @@ static int batch_build(iskra_batch *b, double dt) {
   iskra_ctx **cs = b->cs.data();
   std::vector<int> cur(n), cur_ion(n);
   for (int ph = 0; ph < 2; ++ph) {
-    b->mcc[ph].resize(4 * b->nmcc);
+    b->mcc[ph].resize(5 * b->nmcc);
     for (int m = 0; m < b->nmcc; ++m) {
       for (int i = 0; i < n; ++i) {
         const IskMcc &mm = cs[i]->mcc[m];
@@ -184,7 +185,7 @@ static int batch_build(iskra_batch *b, double dt) {
         cur_ion[i] = (mm.ion >= 0 && mm.ion < cs[i]->nspecies) ? (cs[i]->sp[mm.ion].cur ^ ph) : 0;
       }
       BRC(isk_bbuild_mcc(cs, n, m, dt, cur.data(), cur_ion.data(), b->step_dev,
-                         &b->mcc[ph][4 * m]));
+                         &b->mcc[ph][5 * m]));
     }
     b->push[ph].resize(b->nspecies);
     b->sort[ph].resize(6 * b->nspecies);

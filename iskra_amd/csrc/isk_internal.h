@@ -113,6 +113,9 @@ struct IskMcc {
   int *prod_flag;            /* per source particle: 1 if it ionised */
   float *stage;              /* per source particle: 2 offsets + 3 v of the new electron + 3 v of the ions */
   int *blk_cnt;              /* per block product count, scanned in place */
+  int *cand_i;               /* per block: candidate particles (local index), in order */
+  double *cand_u;            /* and their second uniform */
+  int *cand_cnt;             /* per block candidate count */
   int64_t stage_cap;
   uint32_t stream;
   uint32_t step;
@@ -294,7 +297,7 @@ enum IskBKind {
   ISK_B_FILL = 0, ISK_B_PUSH1, ISK_B_MERGE1, ISK_B_RHO,
   ISK_B_SORT_HIST, ISK_B_SORT_COLSCAN, ISK_B_SORT_SEGSCAN, ISK_B_SORT_BINSCAN,
   ISK_B_SORT_SCATTER, ISK_B_SORT_FINISH,
-  ISK_B_MCC, ISK_B_MCC_SCAN, ISK_B_MCC_APPEND, ISK_B_MCC_COMMIT,
+  ISK_B_MCC, ISK_B_MCC_COLL, ISK_B_MCC_SCAN, ISK_B_MCC_APPEND, ISK_B_MCC_COMMIT,
   ISK_B_TRI, ISK_B_EFIELD, ISK_B_WAVEFORM, ISK_B_STEPINC
 };
 struct IskBLaunch {
@@ -388,7 +391,7 @@ int isk_bbuild_merge(iskra_ctx **cs, int n, int sid, IskBLaunch *L);
 int isk_bbuild_rho(iskra_ctx **cs, int n, IskBLaunch *L);
 int isk_bbuild_sort(iskra_ctx **cs, int n, int sid, const int *cur, IskBLaunch L[6]);
 int isk_bbuild_mcc(iskra_ctx **cs, int n, int m, double dt, const int *cur_src,
-                   const int *cur_ion, long long *step_dev, IskBLaunch L[4]);
+                   const int *cur_ion, long long *step_dev, IskBLaunch L[5]);
 int isk_bbuild_tri(iskra_ctx **cs, int n, IskBLaunch *L);
 int isk_bbuild_efield(iskra_ctx **cs, int n, IskBLaunch *L);
 int isk_bbuild_waveform(iskra_ctx **cs, int n, double dt, long long *step_dev, IskBLaunch *L);

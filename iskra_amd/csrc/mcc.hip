@@ -26,11 +26,17 @@
 #include "isk_fp64math.h"
 #include "isk_internal.h"
 
+/* the pointers in the argument block point to global memory: said so, the
+ * batched kernel (arguments read from a table in memory) loads and stores
+ * with global instructions instead of flat ones */
+#define ISK_GL(ptr) ((__attribute__((address_space(1))) __typeof__(*(ptr)) *)(ptr))
 #define MCC_BLOCK 256
 /* particles per MCC work-group: MCC_RPT rounds of MCC_BLOCK (the loads of
  * all rounds are issued together) */
 #define MCC_RPT 4
 #define MCC_PPB (MCC_BLOCK * MCC_RPT)
+/* cross-section tables up to this many entries are staged in LDS */
+#define MCC_TAB_LDS 1024
 #define KB_MCC 1.3806503e-23      /* mcc.jl:75 */
 #define QE_MCC 1.60217646e-19     /* mcc.jl:26 */
 #define TWO_PI 6.283185307179586
@@ -61,6 +67,9 @@ struct MccArgs {
   int *blk_cnt;
   float *stage;
   int32_t *stage_cell;
+  int *cand_i;    /* candidates of each work-group (local particle index) */
+  double *cand_u; /* and their second uniform */
+  int *cand_cnt;  /* per work-group candidate count */
 };
 
 struct Draws {
@@ -81,17 +90,21 @@ struct Draws {
   }
 };
 
-__device__ double interp_sigma(const MccArgs &a, int k, double e) {
-  int lo = a.toff[k], hi = a.toff[k + 1] - 1;
-  if (e <= a.teps[lo]) return a.tsig[lo];
-  if (e >= a.teps[hi]) return a.tsig[hi];
+/* piecewise-linear sigma(e) of process k, flat outside the table
+ * (cross_section.jl:3-17); the tables are read from LDS when the work-group
+ * staged them, else from global memory */
+__device__ __forceinline__ double interp_sigma(const int *toff, const double *teps,
+                                              const double *tsig, int k, double e) {
+  int lo = toff[k], hi = toff[k + 1] - 1;
+  if (e <= teps[lo]) return tsig[lo];
+  if (e >= teps[hi]) return tsig[hi];
   while (hi - lo > 1) {
     int mid = (lo + hi) >> 1;
-    if (a.teps[mid] <= e) lo = mid;
+    if (teps[mid] <= e) lo = mid;
     else hi = mid;
   }
-  double t = (e - a.teps[lo]) / (a.teps[hi] - a.teps[lo]);
-  return a.tsig[lo] * (1.0 - t) + a.tsig[hi] * t;
+  double t = (e - teps[lo]) / (teps[hi] - teps[lo]);
+  return tsig[lo] * (1.0 - t) + tsig[hi] * t;
 }
 
 struct Ang {
@@ -152,24 +165,20 @@ __device__ void randn3(Draws &d, double *n) {
   n[2] = r2 * c2;
 }
 
-/* Two phases per work-group of MCC_PPB particles.  Every live particle
- * draws (Uc, U) from Philox block 0 of its stream (the cell and id loads of
- * the work-group's MCC_RPT rounds are issued together); the candidates
- * (Uc < candP) are listed in LDS in particle order, and the candidate work
- * is then done one candidate per thread, MCC_BLOCK at a time, so that waves
- * run it with all lanes busy instead of one or two lanes of each wave.  Each
- * candidate's draws continue at Philox block 1, as they would in one pass,
- * and products keep particle order (the list is in particle order). */
-__device__ __forceinline__ void mcc_body(const MccArgs &a, const int blk) {
+/* Two kernels per MCC run, on work-groups of MCC_PPB particles.  Phase 1
+ * (k_mcc_cand): every live particle draws (Uc, U) from Philox block 0 of its
+ * stream (the cell and id loads of the work-group's MCC_RPT rounds are
+ * issued together) and the candidates (Uc < candP) are listed in particle
+ * order.  Phase 2 (k_mcc_coll): one wave per phase-1 work-group runs the
+ * candidates 64 at a time, all lanes busy; each candidate's draws continue
+ * at Philox block 1, as they would in one pass, and products keep particle
+ * order (the list is in particle order). */
+__device__ __forceinline__ void mcc_cand_body(const MccArgs &a, const int blk) {
   __shared__ int wcount[MCC_BLOCK / 64];
-  __shared__ int cand_t[MCC_PPB];
-  __shared__ double cand_u[MCC_PPB];
-  __shared__ unsigned int evl[ISK_MAX_COLL];
   const long long np = a.hdr[HDR_NP];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const uint32_t step = a.step_dev ? (uint32_t)(a.step + *a.step_dev) : a.step;
   const long long b0 = (long long)blk * MCC_PPB;
-  if (threadIdx.x < ISK_MAX_COLL) evl[threadIdx.x] = 0u;
   /* phase 1: cell and id of every round first (loads in flight together) */
   int32_t cr[MCC_RPT];
   uint32_t idr[MCC_RPT];
@@ -177,11 +186,10 @@ __device__ __forceinline__ void mcc_body(const MccArgs &a, const int blk) {
   for (int r = 0; r < MCC_RPT; ++r) {
     const long long p = b0 + r * MCC_BLOCK + threadIdx.x;
     const long long pc = p < np ? p : (np > 0 ? np - 1 : 0);
-    cr[r] = np > 0 ? a.cell[pc] : -1;
-    idr[r] = np > 0 ? a.id[pc] : 0u;
+    cr[r] = np > 0 ? ISK_GL(a.cell)[pc] : -1;
+    idr[r] = np > 0 ? ISK_GL(a.id)[pc] : 0u;
   }
   int ncand = 0;
-#pragma unroll
   for (int r = 0; r < MCC_RPT; ++r) {
     const int lt = r * MCC_BLOCK + threadIdx.x;
     bool cand = false;
@@ -207,17 +215,48 @@ __device__ __forceinline__ void mcc_body(const MccArgs &a, const int blk) {
     int off = ncand;
     for (int k = 0; k < w; ++k) off += wcount[k];
     if (cand) {
-      cand_t[off + rank] = lt;
-      cand_u[off + rank] = U1;
+      ISK_GL(a.cand_i)[b0 + off + rank] = lt;
+      ISK_GL(a.cand_u)[b0 + off + rank] = U1;
     }
     for (int k = 0; k < MCC_BLOCK / 64; ++k) ncand += wcount[k];
     __syncthreads();
   }
-  /* phase 2: one candidate per thread, MCC_BLOCK at a time; the list is in
-   * particle order, so ionisation products, ranked per pass in candidate
-   * order, come out in particle order */
+  if (threadIdx.x == 0) ISK_GL(a.cand_cnt)[blk] = ncand;
+}
+
+/* phase 2: one wave per phase-1 work-group works through its candidates,
+ * 64 at a time (the heavy FP64 collision code runs in its own kernel, so the
+ * light phase 1 keeps a high occupancy) */
+#define MCC_COLL 64
+__device__ __forceinline__ void mcc_coll_body(const MccArgs &a, const int blk) {
+  __shared__ unsigned int evl[ISK_MAX_COLL];
+  const int lane = threadIdx.x;
+  const uint32_t step = a.step_dev ? (uint32_t)(a.step + *a.step_dev) : a.step;
+  const long long b0 = (long long)blk * MCC_PPB;
+  const int ncand = ISK_GL(a.cand_cnt)[blk];
+  if (ncand == 0) {
+    if (lane == 0) a.blk_cnt[blk] = 0;
+    return;
+  }
+  if (lane < ISK_MAX_COLL) evl[lane] = 0u;
+  /* the cross-section tables in LDS (the binary search then waits on LDS,
+   * not on a chain of global loads) */
+  __shared__ double s_eps[MCC_TAB_LDS], s_sig[MCC_TAB_LDS];
+  __shared__ int s_off[ISK_MAX_COLL + 1];
+  const int ntab = a.toff[a.ncoll];
+  const bool tab_lds = ntab <= MCC_TAB_LDS;
+  if ((int)threadIdx.x <= a.ncoll) s_off[threadIdx.x] = a.toff[threadIdx.x];
+  if (tab_lds)
+    for (int q = threadIdx.x; q < ntab; q += MCC_COLL) {
+      s_eps[q] = ISK_GL(a.teps)[q];
+      s_sig[q] = ISK_GL(a.tsig)[q];
+    }
+  __syncthreads();
+  /* one candidate per lane, MCC_COLL at a time; the list is in particle
+   * order, so ionisation products, ranked per pass in candidate order, come
+   * out in particle order */
   int nprod = 0;
-  for (int t0 = 0; t0 < ncand; t0 += MCC_BLOCK) {
+  for (int t0 = 0; t0 < ncand; t0 += MCC_COLL) {
   const int tcand = t0 + (int)threadIdx.x;
   int prod = 0;
   float pe[3] = {0.f, 0.f, 0.f}, pi[3] = {0.f, 0.f, 0.f};
@@ -226,12 +265,12 @@ __device__ __forceinline__ void mcc_body(const MccArgs &a, const int blk) {
   int evkey = -1; /* node * N + process of this thread's collision */
   int lt = 0;
   if (tcand < ncand) {
-    lt = cand_t[tcand];
+    lt = ISK_GL(a.cand_i)[b0 + tcand];
     const long long p = b0 + lt;
-    const double U = cand_u[tcand];
-    const int32_t c = a.cell[p];
+    const double U = ISK_GL(a.cand_u)[b0 + tcand];
+    const int32_t c = ISK_GL(a.cell)[p];
     Draws d;
-    d.id = a.id[p];
+    d.id = ISK_GL(a.id)[p];
     d.step = step;
     d.stream = a.stream;
     d.k0 = a.k0;
@@ -242,25 +281,26 @@ __device__ __forceinline__ void mcc_body(const MccArgs &a, const int blk) {
       {
         int i = c & 0xFFFF, j = c >> 16;
         int node = a.dim == 2 ? j * a.nx + i : i;
-        double n = a.tn[node];
+        double n = ISK_GL(a.tn)[node];
         if (n >= 0.0) {
           const int N = a.ncoll;
           int k = (int)floor((double)N * U);
           if (k > N - 1) k = N - 1;
-          double vx = a.vx[p], vy = a.vy[p], vz = a.vz[p];
+          double vx = ISK_GL(a.vx)[p], vy = ISK_GL(a.vy)[p], vz = ISK_GL(a.vz)[p];
           double tvx = 0.0, tvy = 0.0;
           if (a.tqm != 0.0) {
             if (a.dim == 2) {
-              tvx = a.tqm * (double)a.E[2 * node] * a.dt;
-              tvy = a.tqm * (double)a.E[2 * node + 1] * a.dt;
+              tvx = a.tqm * (double)ISK_GL(a.E)[2 * node] * a.dt;
+              tvy = a.tqm * (double)ISK_GL(a.E)[2 * node + 1] * a.dt;
             } else {
-              tvx = a.tqm * (double)a.E[node] * a.dt;
+              tvx = a.tqm * (double)ISK_GL(a.E)[node] * a.dt;
             }
           }
           double gx = tvx - vx, gy = tvy - vy, gz = 0.0 - vz;
           double g = sqrt(gx * gx + gy * gy + gz * gz);
           double eps = (0.5 * a.m_eV) * (g * g);
-          double sg = interp_sigma(a, k, eps) * g;
+          double sg = (tab_lds ? interp_sigma(s_off, s_eps, s_sig, k, eps)
+                               : interp_sigma(s_off, a.teps, a.tsig, k, eps)) * g;
           double P = 1.0 - isk_exp(((-n) * sg) * a.dt);
           P /= a.candP;
           if (P > 1.0) atomicOr(a.errflag, 1);
@@ -286,9 +326,9 @@ __device__ __forceinline__ void mcc_body(const MccArgs &a, const int blk) {
               else
                 diffuse_reflection(vr[0], vr[1], vr[2], d, dir);
               for (int q = 0; q < 3; ++q) out[q] = w[q] + mr2 * (nvr * dir[q]);
-              a.vx[p] = (float)out[0];
-              a.vy[p] = (float)out[1];
-              a.vz[p] = (float)out[2];
+              ISK_GL(a.vx)[p] = (float)out[0];
+              ISK_GL(a.vy)[p] = (float)out[1];
+              ISK_GL(a.vz)[p] = (float)out[2];
             } else if (type == ISKRA_COLL_EXCITATION) {
               double sE = (0.5 * a.m_eV) * (vx * vx + vy * vy + vz * vz) - a.energy[k];
               if (sE >= 0.0) {
@@ -296,9 +336,9 @@ __device__ __forceinline__ void mcc_body(const MccArgs &a, const int blk) {
                 double ev = al * sqrt(sE);
                 double dir[3];
                 isotropic_scattering(vx, vy, vz, d, dir);
-                a.vx[p] = (float)(ev * dir[0]);
-                a.vy[p] = (float)(ev * dir[1]);
-                a.vz[p] = (float)(ev * dir[2]);
+                ISK_GL(a.vx)[p] = (float)(ev * dir[0]);
+                ISK_GL(a.vy)[p] = (float)(ev * dir[1]);
+                ISK_GL(a.vz)[p] = (float)(ev * dir[2]);
               }
             } else if (type == ISKRA_COLL_IONIZATION) {
               double sE = (0.5 * a.m_eV) * (vx * vx + vy * vy + vz * vz) - a.energy[k];
@@ -310,17 +350,17 @@ __device__ __forceinline__ void mcc_body(const MccArgs &a, const int blk) {
                 double dir[3];
                 diffuse_reflection(vx, vy, vz, d, dir);
                 double nv[3] = {e1v * dir[0], e1v * dir[1], e1v * dir[2]};
-                a.vx[p] = (float)nv[0];
-                a.vy[p] = (float)nv[1];
-                a.vz[p] = (float)nv[2];
+                ISK_GL(a.vx)[p] = (float)nv[0];
+                ISK_GL(a.vy)[p] = (float)nv[1];
+                ISK_GL(a.vz)[p] = (float)nv[2];
                 double dir2[3];
                 diffuse_reflection(nv[0], nv[1], nv[2], d, dir2);
                 double tv[3];
                 randn3(d, tv);
                 prod = 1;
                 pcell = c;
-                pox = a.ox[p];
-                poy = a.dim == 2 ? a.oy[p] : 0.f;
+                pox = ISK_GL(a.ox)[p];
+                poy = a.dim == 2 ? ISK_GL(a.oy)[p] : 0.f;
                 for (int q = 0; q < 3; ++q) {
                   pe[q] = (float)(e2v * dir2[q]);
                   pi[q] = (float)(tv[q] * a.vth_t);
@@ -335,14 +375,11 @@ __device__ __forceinline__ void mcc_body(const MccArgs &a, const int blk) {
   {
     const unsigned long long m = __ballot(prod != 0);
     const int rank = __popcll(m & ((1ull << lane) - 1ull));
-    if (lane == 0) wcount[w] = __popcll(m);
-    __syncthreads();
-    int off = nprod;
-    for (int k = 0; k < w; ++k) off += wcount[k];
+    const int off = nprod;
     if (prod) {
       const size_t sidx = (size_t)b0 + off + rank;
-      a.stage_cell[sidx] = pcell;
-      float *st = a.stage + sidx * 8;
+      ISK_GL(a.stage_cell)[sidx] = pcell;
+      auto *st = ISK_GL(a.stage) + sidx * 8;
       st[0] = pox;
       st[1] = poy;
       st[2] = pe[0];
@@ -352,8 +389,7 @@ __device__ __forceinline__ void mcc_body(const MccArgs &a, const int blk) {
       st[6] = pi[1];
       st[7] = pi[2];
     }
-    for (int k = 0; k < MCC_BLOCK / 64; ++k) nprod += wcount[k];
-    __syncthreads();
+    nprod += __popcll(m);
   }
   /* event counts: equal (node, process) keys of a wave are counted with one
    * atomic, the per-process totals of the work-group go through LDS (integer
@@ -378,7 +414,8 @@ __device__ __forceinline__ void mcc_body(const MccArgs &a, const int blk) {
   if (threadIdx.x == 0) a.blk_cnt[blk] = nprod;
 }
 
-__global__ __launch_bounds__(MCC_BLOCK) void k_mcc(MccArgs a) { mcc_body(a, blockIdx.x); }
+__global__ __launch_bounds__(MCC_BLOCK) void k_mcc_cand(MccArgs a) { mcc_cand_body(a, blockIdx.x); }
+__global__ __launch_bounds__(MCC_COLL) void k_mcc_coll(MccArgs a) { mcc_coll_body(a, blockIdx.x); }
 
 /* exclusive scan of the block product counts (one work-group); cnt[nblk] = total */
 __device__ __forceinline__ void mcc_scan_body(int *cnt, int nblk) {
@@ -571,6 +608,9 @@ int isk_mcc_setup(iskra_ctx *c, int idx) {
   m.prod_flag = nullptr;
   m.stage = nullptr;
   m.blk_cnt = nullptr;
+  m.cand_i = nullptr;
+  m.cand_u = nullptr;
+  m.cand_cnt = nullptr;
   return ISKRA_OK;
 }
 
@@ -584,7 +624,7 @@ struct CommitArgs {
   int R;
 };
 
-/* arguments of the four launches of one MCC run, the source species reading
+/* arguments of the five launches of one MCC run, the source species reading
  * buffer b_src and the ion species buffer b_ion; allocates the staging area */
 static int mcc_prepare(iskra_ctx *c, int idx, double dt, int b_src, int b_ion, MccArgs *pa,
                        AppendArgs *pp, CommitArgs *pc, int *nblk_out) {
@@ -603,12 +643,21 @@ static int mcc_prepare(iskra_ctx *c, int idx, double dt, int b_src, int b_ion, M
     isk_free(m.stage);
     isk_free(m.blk_cnt);
     isk_free(m.prod_flag);
+    isk_free(m.cand_i);
+    isk_free(m.cand_u);
+    isk_free(m.cand_cnt);
+    m.cand_i = nullptr;
+    m.cand_u = nullptr;
+    m.cand_cnt = nullptr;
     m.stage = nullptr;
     m.blk_cnt = nullptr;
     m.prod_flag = nullptr;
     if (isk_alloc((void **)&m.stage, (size_t)nblk * MCC_PPB * 8 * 4) ||
         isk_alloc((void **)&m.blk_cnt, (size_t)(nblk + 1) * 4) ||
-        isk_alloc((void **)&m.prod_flag, (size_t)nblk * MCC_PPB * 4))
+        isk_alloc((void **)&m.prod_flag, (size_t)nblk * MCC_PPB * 4) ||
+        isk_alloc((void **)&m.cand_i, (size_t)nblk * MCC_PPB * 4) ||
+        isk_alloc((void **)&m.cand_u, (size_t)nblk * MCC_PPB * 8) ||
+        isk_alloc((void **)&m.cand_cnt, (size_t)nblk * 4))
       return ISKRA_E_HIP;
     m.stage_cap = cap;
   }
@@ -652,6 +701,9 @@ static int mcc_prepare(iskra_ctx *c, int idx, double dt, int b_src, int b_ion, M
   a.blk_cnt = m.blk_cnt;
   a.stage = m.stage;
   a.stage_cell = m.prod_flag;
+  a.cand_i = m.cand_i;
+  a.cand_u = m.cand_u;
+  a.cand_cnt = m.cand_cnt;
   AppendArgs &p = *pp;
   p.blk_off = m.blk_cnt;
   p.stage = m.stage;
@@ -711,7 +763,9 @@ int isk_mcc_run(iskra_ctx *c, int idx, double dt, int64_t *events) {
   int rc = mcc_prepare(c, idx, dt, s.cur, b_ion, &a, &p, &cm, &nblk);
   if (rc) return rc;
   if (events) ISK_HIP(hipMemsetAsync(m.events, 0, ISK_MAX_COLL * 8, c->stream));
-  hipLaunchKernelGGL(k_mcc, dim3(nblk), dim3(MCC_BLOCK), 0, c->stream, a);
+  hipLaunchKernelGGL(k_mcc_cand, dim3(nblk), dim3(MCC_BLOCK), 0, c->stream, a);
+  ISK_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_mcc_coll, dim3(nblk), dim3(MCC_COLL), 0, c->stream, a);
   ISK_CHECK_LAUNCH();
   hipLaunchKernelGGL(k_mcc_scan, dim3(1), dim3(1024), 0, c->stream, m.blk_cnt, nblk);
   ISK_CHECK_LAUNCH();
@@ -740,9 +794,16 @@ int isk_mcc_run(iskra_ctx *c, int idx, double dt, int64_t *events) {
 }
 
 /* ---- batched launches ---- */
-__global__ __launch_bounds__(MCC_BLOCK) void kb_mcc(const MccArgs *A, const int *bstart, int n) {
+__global__ __launch_bounds__(MCC_BLOCK) void kb_mcc_cand(const MccArgs *A, const int *bstart,
+                                                         int n) {
   const int i = isk_bfind(bstart, n, blockIdx.x);
-  mcc_body(A[i], blockIdx.x - bstart[i]);
+  mcc_cand_body(A[i], blockIdx.x - bstart[i]);
+}
+
+__global__ __launch_bounds__(MCC_COLL) void kb_mcc_coll(const MccArgs *A, const int *bstart,
+                                                        int n) {
+  const int i = isk_bfind(bstart, n, blockIdx.x);
+  mcc_coll_body(A[i], blockIdx.x - bstart[i]);
 }
 
 __global__ __launch_bounds__(1024) void kb_mcc_scan(const CommitArgs *A, const int *bstart,
@@ -766,7 +827,7 @@ __global__ void kb_mcc_commit(const CommitArgs *A, const int *bstart, int n) {
 }
 
 int isk_bbuild_mcc(iskra_ctx **cs, int n, int m, double dt, const int *cur_src,
-                   const int *cur_ion, long long *step_dev, IskBLaunch L[4]) {
+                   const int *cur_ion, long long *step_dev, IskBLaunch L[5]) {
   std::vector<MccArgs> A(n);
   std::vector<AppendArgs> P(n);
   std::vector<CommitArgs> CM(n);
@@ -781,13 +842,16 @@ int isk_bbuild_mcc(iskra_ctx **cs, int n, int m, double dt, const int *cur_src,
   if ((rc = isk_blaunch_upload(&L[0], ISK_B_MCC, A.data(), sizeof(MccArgs), blocks, MCC_BLOCK,
                                0)))
     return rc;
-  if ((rc = isk_blaunch_upload(&L[1], ISK_B_MCC_SCAN, CM.data(), sizeof(CommitArgs), one, 1024,
+  if ((rc = isk_blaunch_upload(&L[1], ISK_B_MCC_COLL, A.data(), sizeof(MccArgs), blocks,
+                               MCC_COLL, 0)))
+    return rc;
+  if ((rc = isk_blaunch_upload(&L[2], ISK_B_MCC_SCAN, CM.data(), sizeof(CommitArgs), one, 1024,
                                0)))
     return rc;
-  if ((rc = isk_blaunch_upload(&L[2], ISK_B_MCC_APPEND, P.data(), sizeof(AppendArgs), ablocks,
+  if ((rc = isk_blaunch_upload(&L[3], ISK_B_MCC_APPEND, P.data(), sizeof(AppendArgs), ablocks,
                                MCC_BLOCK, 0)))
     return rc;
-  return isk_blaunch_upload(&L[3], ISK_B_MCC_COMMIT, CM.data(), sizeof(CommitArgs), one, 64, 0);
+  return isk_blaunch_upload(&L[4], ISK_B_MCC_COMMIT, CM.data(), sizeof(CommitArgs), one, 64, 0);
 }
 
 int isk_blaunch_mcc(const IskBLaunch &L, hipStream_t st) {
@@ -795,7 +859,10 @@ int isk_blaunch_mcc(const IskBLaunch &L, hipStream_t st) {
   dim3 g((unsigned)L.total), b((unsigned)L.block);
   switch (L.kind) {
     case ISK_B_MCC:
-      hipLaunchKernelGGL(kb_mcc, g, b, 0, st, (const MccArgs *)L.args, L.bstart, L.n);
+      hipLaunchKernelGGL(kb_mcc_cand, g, b, 0, st, (const MccArgs *)L.args, L.bstart, L.n);
+      break;
+    case ISK_B_MCC_COLL:
+      hipLaunchKernelGGL(kb_mcc_coll, g, b, 0, st, (const MccArgs *)L.args, L.bstart, L.n);
       break;
     case ISK_B_MCC_SCAN:
       hipLaunchKernelGGL(kb_mcc_scan, g, b, 0, st, (const CommitArgs *)L.args, L.bstart, L.n);
