@@ -30,8 +30,15 @@
 #include "isk_fp64math.h"
 #include "isk_internal.h"
 
-#define PUSH_BLOCK 256
-#define PUSH_UNROLL 8
+/* push work-group shape: threads, and particles per thread per batch (one
+ * batch covers a 2048-particle chunk). 2-D: 512 x 4 keeps the kernel at 59
+ * VGPRs (8 waves per SIMD), 2 % faster than 256 x 8 at 96 VGPRs on the
+ * 256^2-cell slab; 1-D keeps 256 x 8 */
+template <int DIM>
+struct PushShape {
+  static constexpr int block = DIM == 2 ? 512 : 256;
+  static constexpr int unroll = DIM == 2 ? 4 : 8;
+};
 #define SORT_BLOCK 256
 #define SORT_PT 32
 #define SORT_CHUNK (SORT_BLOCK * SORT_PT)
@@ -768,8 +775,9 @@ __device__ __forceinline__ bool apply_bnd(int *i, int n, int mode) {
 
 template <int DIM>
 struct PushRegs {
-  int32_t c[PUSH_UNROLL];
-  float hx[PUSH_UNROLL], hy[PUSH_UNROLL], vx[PUSH_UNROLL], vy[PUSH_UNROLL];
+  static constexpr int U = PushShape<DIM>::unroll;
+  int32_t c[U];
+  float hx[U], hy[U], vx[U], vy[U];
 };
 
 /* pointers are the work-group's first particle (uniform), indices 32-bit */
@@ -784,8 +792,8 @@ __device__ __forceinline__ void push_load(PushRegs<DIM> &r, const int32_t *__res
    * skips them); nothing here consumes a loaded value, so all loads of the
    * batch are in flight together */
 #pragma unroll
-  for (int u = 0; u < PUSH_UNROLL; ++u) {
-    const unsigned k = b0 + u * PUSH_BLOCK + threadIdx.x;
+  for (int u = 0; u < PushShape<DIM>::unroll; ++u) {
+    const unsigned k = b0 + u * PushShape<DIM>::block + threadIdx.x;
     __builtin_assume(k < (1u << 24));
     const unsigned q = k < count ? k : 0u;
     r.c[u] = cb[q];
@@ -798,6 +806,7 @@ __device__ __forceinline__ void push_load(PushRegs<DIM> &r, const int32_t *__res
 
 template <int DIM>
 __device__ __forceinline__ void push_body(const PushArgs &a, const int blk) {
+  constexpr int BLK = PushShape<DIM>::block, UNR = PushShape<DIM>::unroll;
   extern __shared__ unsigned long long sD[]; /* WW deposit counters, then E */
   const int W = a.W;
   const int WW = DIM == 2 ? W * W : W;
@@ -845,7 +854,7 @@ __device__ __forceinline__ void push_body(const PushArgs &a, const int blk) {
   PushRegs<DIM> cur;
   push_load<DIM>(cur, cb, oxb, oyb, vxb, vyb, ucount, 0u);
   if (windowed) {
-    for (int k = threadIdx.x; k < WW; k += PUSH_BLOCK) {
+    for (int k = threadIdx.x; k < WW; k += BLK) {
       sD[k] = 0ull;
       if (a.do_push) {
         int li = DIM == 2 ? k % W : k;
@@ -865,13 +874,13 @@ __device__ __forceinline__ void push_body(const PushArgs &a, const int blk) {
     __syncthreads();
   }
   int live = 0;
-  const long long per = (long long)PUSH_BLOCK * PUSH_UNROLL;
+  const long long per = (long long)BLK * UNR;
   for (long long b0 = 0; b0 < count; b0 += per) {
-    /* a batch of PUSH_UNROLL particles per thread: all loads, then compute + stores */
+    /* a batch of UNR particles per thread: all loads, then compute + stores */
     if (b0 > 0) push_load<DIM>(cur, cb, oxb, oyb, vxb, vyb, ucount, (unsigned)b0);
 #pragma unroll
-    for (int u = 0; u < PUSH_UNROLL; ++u) {
-      const unsigned kslot = (unsigned)b0 + u * PUSH_BLOCK + threadIdx.x;
+    for (int u = 0; u < UNR; ++u) {
+      const unsigned kslot = (unsigned)b0 + u * BLK + threadIdx.x;
       if (kslot >= ucount) continue;
       int32_t c = cur.c[u];
       if (c < 0) continue;
@@ -960,7 +969,7 @@ __device__ __forceinline__ void push_body(const PushArgs &a, const int blk) {
         }
       }
       if (a.do_push) {
-        const unsigned k = (unsigned)b0 + u * PUSH_BLOCK + threadIdx.x;
+        const unsigned k = (unsigned)b0 + u * BLK + threadIdx.x;
         cb[k] = alive ? (i | (j << 16)) : -1;
         oxb[k] = hx;
         vxb[k] = vx;
@@ -972,23 +981,23 @@ __device__ __forceinline__ void push_body(const PushArgs &a, const int blk) {
     }
   }
   /* per-work-group live count (summed by the merge kernel, no atomics) */
-  __shared__ int wl[PUSH_BLOCK / 64];
+  __shared__ int wl[BLK / 64];
   for (int off = 32; off > 0; off >>= 1) live += __shfl_xor(live, off);
   if ((threadIdx.x & 63) == 0) wl[threadIdx.x >> 6] = live;
   __syncthreads();
   if (threadIdx.x == 0) {
     int t = 0;
-    for (int k = 0; k < PUSH_BLOCK / 64; ++k) t += wl[k];
+    for (int k = 0; k < BLK / 64; ++k) t += wl[k];
     a.live_cnt[blk] = t;
   }
   if (windowed) {
     long long *dst = a.partial + (size_t)blk * WW;
-    for (int k = threadIdx.x; k < WW; k += PUSH_BLOCK) dst[k] = (long long)sD[k];
+    for (int k = threadIdx.x; k < WW; k += BLK) dst[k] = (long long)sD[k];
   }
 }
 
 template <int DIM>
-__global__ __launch_bounds__(PUSH_BLOCK) void k_push(PushArgs a) {
+__global__ __launch_bounds__(PushShape<DIM>::block) void k_push(PushArgs a) {
   push_body<DIM>(a, blockIdx.x);
 }
 
@@ -1047,9 +1056,9 @@ static int push_launch(iskra_ctx *c, int sid, double dt, int do_push, int fill) 
     if (rc2) return rc2;
   }
   if (c->g.dim == 2)
-    hipLaunchKernelGGL(k_push<2>, dim3(grid), dim3(PUSH_BLOCK), shm, c->stream, a);
+    hipLaunchKernelGGL(k_push<2>, dim3(grid), dim3(PushShape<2>::block), shm, c->stream, a);
   else
-    hipLaunchKernelGGL(k_push<1>, dim3(grid), dim3(PUSH_BLOCK), shm, c->stream, a);
+    hipLaunchKernelGGL(k_push<1>, dim3(grid), dim3(PushShape<1>::block), shm, c->stream, a);
   ISK_CHECK_LAUNCH();
   if (do_push) {
     int rc2 = isk_prof_end(c, ev, ISKRA_K_PUSH, 0);
@@ -1069,7 +1078,7 @@ struct PushMulti {
 };
 
 template <int DIM>
-__global__ __launch_bounds__(PUSH_BLOCK) void k_push_multi(PushMulti m) {
+__global__ __launch_bounds__(PushShape<DIM>::block) void k_push_multi(PushMulti m) {
   int s = 0;
   while (s + 1 < m.n && (int)blockIdx.x >= m.bstart[s + 1]) ++s;
   /* a copy in registers: read through a reference into the argument block,
@@ -1096,9 +1105,11 @@ static int push_launch_all(iskra_ctx *c, double dt) {
   int rc = isk_prof_begin(c, &ev);
   if (rc) return rc;
   if (c->g.dim == 2)
-    hipLaunchKernelGGL(k_push_multi<2>, dim3(m.bstart[m.n]), dim3(PUSH_BLOCK), shm, c->stream, m);
+    hipLaunchKernelGGL(k_push_multi<2>, dim3(m.bstart[m.n]), dim3(PushShape<2>::block), shm,
+                       c->stream, m);
   else
-    hipLaunchKernelGGL(k_push_multi<1>, dim3(m.bstart[m.n]), dim3(PUSH_BLOCK), shm, c->stream, m);
+    hipLaunchKernelGGL(k_push_multi<1>, dim3(m.bstart[m.n]), dim3(PushShape<1>::block), shm,
+                       c->stream, m);
   ISK_CHECK_LAUNCH();
   return isk_prof_end(c, ev, ISKRA_K_PUSH, 0);
 }
@@ -1489,7 +1500,7 @@ __global__ void kb_fill(const FillArgs *A, const int *bstart, int n) {
   if (k < A[i].n) A[i].p[k] = 0;
 }
 
-__global__ __launch_bounds__(PUSH_BLOCK) void kb_push1(const PushArgs *A, const int *bstart,
+__global__ __launch_bounds__(PushShape<1>::block) void kb_push1(const PushArgs *A, const int *bstart,
                                                         int n) {
   const int i = isk_bfind(bstart, n, blockIdx.x);
   const PushArgs a = A[i]; /* in registers, not re-read after each store */
@@ -1566,7 +1577,8 @@ int isk_bbuild_push(iskra_ctx **cs, int n, int sid, double dt, const int *cur, I
     blocks[i] = grid;
     shm = std::max(shm, sh);
   }
-  return isk_blaunch_upload(L, ISK_B_PUSH1, A.data(), sizeof(PushArgs), blocks, PUSH_BLOCK, shm);
+  return isk_blaunch_upload(L, ISK_B_PUSH1, A.data(), sizeof(PushArgs), blocks, PushShape<1>::block,
+                            shm);
 }
 
 int isk_bbuild_merge(iskra_ctx **cs, int n, int sid, IskBLaunch *L) {
