@@ -39,7 +39,10 @@
 #define MG_BATCH 4
 #define MG_COARSE_MAX 65
 
-enum { S_RZ = 0, S_PAP, S_ALPHA, S_BETA, S_RR, S_FF, S_TOL2, S_MEAN, S_RELRES, S_RZNEW, S_WORDS = 16 };
+/* scalars: S_RZ0/S_RZ1 hold (r,z) of even/odd CG iterations, S_RTOL2 = rtol^2,
+ * S_TOL2 = rtol^2 (f,f) */
+enum { S_PAP = 0, S_ALPHA, S_BETA, S_RR, S_FF, S_TOL2, S_MEAN, S_RELRES, S_RZ0, S_RZ1, S_RTOL2,
+       S_WORDS = 16 };
 /* iflag words: [0] done, [1] iterations, [2] singular (PCG/MG); spectral solve:
  * [4] this solve met rtol, [5] sticky: an in-loop solve missed rtol,
  * [6] transform passes of this solve */
@@ -253,12 +256,14 @@ struct PcgArgs {
   Sten s;
   const double *rho;
   double *x, *r, *Ap;
-  float *z, *pold, *pnew;
+  float *z, *p0, *p1; /* search directions of even / odd iterations */
   float *f0;  /* MG preconditioner input (MGPCG) */
   float *e0;  /* MG finest-level correction, zeroed before each V-cycle */
   double *scal, *partials;
   unsigned int *counter;
   int *iflag;
+  double *cgp; /* CG partials, 7 per work-group */
+  int nblk;    /* work-groups of the CG kernels */
   double scale;
   int warm;
   const double *phi_in;
@@ -277,7 +282,8 @@ __global__ void k_pcg_setx(PcgArgs a) {
 
 /* scalars and flags of a new solve */
 __global__ void k_pcg_init(double *scal, int *iflag, double tol2, int singular) {
-  if (threadIdx.x < S_WORDS) scal[threadIdx.x] = threadIdx.x == S_TOL2 ? tol2 : 0.0;
+  if (threadIdx.x < S_WORDS)
+    scal[threadIdx.x] = (threadIdx.x == S_TOL2 || threadIdx.x == S_RTOL2) ? tol2 : 0.0;
   if (threadIdx.x < 4) iflag[threadIdx.x] = threadIdx.x == 2 ? singular : 0;
 }
 
@@ -291,11 +297,65 @@ __global__ void k_pcg_mean(PcgArgs a) {
     a.scal[S_MEAN] = s / (double)(a.s.nx * a.s.ny);
 }
 
-/* r = f_elim - L x at free nodes; ||f_elim||^2; z = M^-1 r (Jacobi) */
-__global__ void k_pcg_resid(PcgArgs a) {
+/* CG without device-wide reductions: a kernel that needs a dot product sums
+ * the per-work-group partials that the previous kernel wrote, in block order,
+ * in every work-group (the same data summed the same way gives the same bits
+ * everywhere), so no kernel waits on the others through fences or counters.
+ * Partials, RED_BLOCK-node work-groups: two slots by iteration parity of
+ * {(r,r), (r,z), (f,f)} per work-group, then one (p,Ap) per work-group.
+ * Iteration it: A(it) takes (r,r) and (r,z) of x_it, decides convergence,
+ * forms p_it = z + beta p_{it-1} in p[it & 1] and (p, Ap); B(it) updates
+ * x, r, z and writes the partials of x_{it+1} into slot (it + 1) & 1. */
+
+/* sums of 3 doubles over the work-group: an xor butterfly within each wave
+ * (every lane ends with the same bits), then the waves in index order */
+__device__ __forceinline__ void wg_sum3(double &a, double &b, double &c) {
+  __shared__ double w3[3][RED_BLOCK / 64];
+  for (int off = 32; off > 0; off >>= 1) {
+    a += __shfl_xor(a, off);
+    b += __shfl_xor(b, off);
+    c += __shfl_xor(c, off);
+  }
+  __syncthreads(); /* a previous call's readers are done with w3 */
+  if ((threadIdx.x & 63) == 0) {
+    const int w = threadIdx.x >> 6;
+    w3[0][w] = a;
+    w3[1][w] = b;
+    w3[2][w] = c;
+  }
+  __syncthreads();
+  double sa = 0.0, sb = 0.0, sc = 0.0;
+  for (int k = 0; k < RED_BLOCK / 64; ++k) {
+    sa += w3[0][k];
+    sb += w3[1][k];
+    sc += w3[2][k];
+  }
+  a = sa;
+  b = sb;
+  c = sc;
+}
+
+__device__ __forceinline__ double *cg_slot(const PcgArgs &a, int q) {
+  return a.cgp + (size_t)q * 3 * a.nblk;
+}
+__device__ __forceinline__ double *cg_pap(const PcgArgs &a) { return a.cgp + (size_t)6 * a.nblk; }
+
+/* this work-group's partials (thread 0 holds the work-group sums) */
+__device__ __forceinline__ void cg_put3(double *slot, double x, double y, double z) {
+  wg_sum3(x, y, z);
+  if (threadIdx.x == 0) {
+    slot[3 * blockIdx.x] = x;
+    slot[3 * blockIdx.x + 1] = y;
+    slot[3 * blockIdx.x + 2] = z;
+  }
+}
+
+/* r = f_elim - L x at free nodes; partials of (r,r), (r,z) (Jacobi z = r/diag;
+ * the MG preconditioner's z comes later) and (f,f) into slot 0 */
+__global__ __launch_bounds__(RED_BLOCK) void k_pcg_resid(PcgArgs a) {
   int n = blockIdx.x * blockDim.x + threadIdx.x;
   int nn = a.s.nx * a.s.ny;
-  double rr = 0.0, ff = 0.0;
+  double rr = 0.0, rz = 0.0, ff = 0.0;
   if (n < nn) {
     int i = n % a.s.nx, j = n / a.s.nx;
     if (a.s.tags[n] == 0) {
@@ -307,9 +367,14 @@ __global__ void k_pcg_resid(PcgArgs a) {
       double fe = f + dirichlet_rhs(a.s, i, j);
       double r = f - Lx;
       a.r[n] = r;
-      if (!a.mgpre) a.z[n] = (float)(r / diag);
-      else a.f0[n] = (float)r;
-      if (a.mgpre) a.e0[n] = 0.0f;
+      if (!a.mgpre) {
+        float z = (float)(r / diag);
+        a.z[n] = z;
+        rz = r * (double)z;
+      } else {
+        a.f0[n] = (float)r;
+        a.e0[n] = 0.0f;
+      }
       rr = r * r;
       ff = fe * fe;
     } else {
@@ -320,47 +385,74 @@ __global__ void k_pcg_resid(PcgArgs a) {
         a.e0[n] = 0.0f;
       }
     }
-    a.pold[n] = 0.0f;
-    a.pnew[n] = 0.0f;
+    a.p0[n] = 0.0f;
+    a.p1[n] = 0.0f;
   }
-  double srr, sff;
-  if (block_reduce2(rr, ff, a.partials, a.counter, &srr, &sff) && threadIdx.x == 0) {
-    a.scal[S_RR] = srr;
-    a.scal[S_FF] = sff;
-    a.scal[S_TOL2] = a.scal[S_TOL2] * sff; /* rtol^2 preset by the host */
-    a.scal[S_BETA] = 0.0;
-    a.scal[S_RELRES] = sff > 0.0 ? sqrt(srr / sff) : 0.0;
-    if (srr <= a.scal[S_TOL2]) a.iflag[0] = 1;
-  }
+  cg_put3(cg_slot(a, 0), rr, rz, ff);
 }
 
-/* rz = (r, z) after the preconditioner */
-__global__ void k_pcg_rz(PcgArgs a) {
+/* MG-preconditioned CG: z = the V-cycle's finest correction, (r,z) partials
+ * into slot q */
+__global__ __launch_bounds__(RED_BLOCK) void k_cg_toz(PcgArgs a, int q) {
   if (a.iflag[0]) return;
   int n = blockIdx.x * blockDim.x + threadIdx.x;
-  double v = 0.0;
-  if (n < a.s.nx * a.s.ny) v = a.r[n] * (double)a.z[n];
-  double s, dummy;
-  if (block_reduce2(v, 0.0, a.partials, a.counter, &s, &dummy) && threadIdx.x == 0) {
-    double old = a.scal[S_RZ];
-    if (a.iflag[3]) a.scal[S_BETA] = old != 0.0 ? s / old : 0.0;
-    else a.scal[S_BETA] = 0.0;
-    a.scal[S_RZ] = s;
-    a.iflag[3] = 1;
+  double rz = 0.0, d1 = 0.0, d2 = 0.0;
+  if (n < a.s.nx * a.s.ny) {
+    float z = a.e0[n];
+    a.z[n] = z;
+    rz = a.r[n] * (double)z;
   }
+  wg_sum3(rz, d1, d2);
+  if (threadIdx.x == 0) cg_slot(a, q)[3 * blockIdx.x + 1] = rz;
 }
 
-/* p_new = z + beta p_old (recomputed at the neighbours), Ap = L p_new, (p, Ap) */
-__global__ void k_pcg_a(PcgArgs a) {
-  if (a.iflag[0]) return;
+/* A(it): scalars of x_it, convergence, p_it = z + beta p_{it-1} (recomputed
+ * at the neighbours), Ap = L p_it, (p, Ap) partials */
+__global__ __launch_bounds__(RED_BLOCK) void k_cg_a(PcgArgs a, int it) {
+  __shared__ int s_stop;
+  if (threadIdx.x == 0) s_stop = a.iflag[0];
+  double rr = 0.0, rz = 0.0, ff = 0.0;
+  {
+    const double *sl = cg_slot(a, it & 1);
+    for (int k = threadIdx.x; k < a.nblk; k += RED_BLOCK) {
+      rr += sl[3 * k];
+      rz += sl[3 * k + 1];
+      ff += sl[3 * k + 2];
+    }
+  }
+  wg_sum3(rr, rz, ff); /* its barriers also publish s_stop */
+  if (s_stop) return;
+  const double sff = it == 0 ? ff : a.scal[S_FF];
+  const double tol2 = it == 0 ? a.scal[S_RTOL2] * ff : a.scal[S_TOL2];
+  const bool done = rr <= tol2 || it >= a.max_iter;
+  double beta = 0.0;
+  if (it > 0) {
+    const double old = a.scal[S_RZ0 + ((it - 1) & 1)];
+    beta = old != 0.0 ? rz / old : 0.0;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (it == 0) {
+      a.scal[S_FF] = ff;
+      a.scal[S_TOL2] = tol2;
+    }
+    a.scal[S_RR] = rr;
+    a.scal[S_RELRES] = sff > 0.0 ? sqrt(rr / sff) : 0.0;
+    a.scal[S_RZ0 + (it & 1)] = rz;
+    a.scal[S_BETA] = beta;
+    a.iflag[1] = it;
+    if (done) a.iflag[0] = 1;
+  }
+  if (done) return;
+  const float *pold = (it & 1) ? a.p0 : a.p1;
+  float *pnew = (it & 1) ? a.p1 : a.p0;
+  const float fb = (float)beta;
   int n = blockIdx.x * blockDim.x + threadIdx.x;
   int nn = a.s.nx * a.s.ny;
-  float beta = (float)a.scal[S_BETA];
   double pap = 0.0;
   if (n < nn) {
     int i = n % a.s.nx, j = n / a.s.nx;
     if (a.s.tags[n] == 0) {
-#define PN(m) (a.s.tags[m] ? 0.0 : (double)fmaf(beta, a.pold[m], a.z[m]))
+#define PN(m) (a.s.tags[m] ? 0.0 : (double)fmaf(fb, pold[m], a.z[m]))
       double pc = PN(n);
       double acc = 0.0, diag = 0.0;
       if (i > 0) { acc += PN(n - 1); diag += 1.0; }
@@ -379,31 +471,42 @@ __global__ void k_pcg_a(PcgArgs a) {
       }
 #undef PN
       double ap = diag * pc - acc;
-      a.pnew[n] = (float)pc;
+      pnew[n] = (float)pc;
       a.Ap[n] = ap;
       pap = pc * ap;
     } else {
-      a.pnew[n] = 0.0f;
+      pnew[n] = 0.0f;
       a.Ap[n] = 0.0;
     }
   }
-  double s, dummy;
-  if (block_reduce2(pap, 0.0, a.partials, a.counter, &s, &dummy) && threadIdx.x == 0) {
-    a.scal[S_PAP] = s;
-    a.scal[S_ALPHA] = s != 0.0 ? a.scal[S_RZ] / s : 0.0;
-  }
+  double d1 = 0.0, d2 = 0.0;
+  wg_sum3(pap, d1, d2);
+  if (threadIdx.x == 0) cg_pap(a)[blockIdx.x] = pap;
 }
 
-/* x += alpha p, r -= alpha Ap, z = M^-1 r, (r,r), (r,z) */
-__global__ void k_pcg_b(PcgArgs a) {
-  if (a.iflag[0]) return;
+/* B(it): alpha = (r,z) / (p,Ap); x += alpha p, r -= alpha Ap, z = M^-1 r;
+ * partials of x_{it+1} into slot (it + 1) & 1 */
+__global__ __launch_bounds__(RED_BLOCK) void k_cg_b(PcgArgs a, int it) {
+  if (a.iflag[0]) return; /* set by an earlier kernel: the same for every thread */
+  double pap = 0.0, d1 = 0.0, d2 = 0.0;
+  {
+    const double *pp = cg_pap(a);
+    for (int k = threadIdx.x; k < a.nblk; k += RED_BLOCK) pap += pp[k];
+  }
+  wg_sum3(pap, d1, d2);
+  const double rzc = a.scal[S_RZ0 + (it & 1)];
+  const double alpha = pap != 0.0 ? rzc / pap : 0.0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    a.scal[S_PAP] = pap;
+    a.scal[S_ALPHA] = alpha;
+  }
+  const float *pc = (it & 1) ? a.p1 : a.p0;
   int n = blockIdx.x * blockDim.x + threadIdx.x;
   int nn = a.s.nx * a.s.ny;
-  double alpha = a.scal[S_ALPHA];
   double rr = 0.0, rz = 0.0;
   if (n < nn && a.s.tags[n] == 0) {
     int i = n % a.s.nx, j = n / a.s.nx;
-    double p = (double)a.pnew[n];
+    double p = (double)pc[n];
     a.x[n] = __builtin_fma(alpha, p, a.x[n]);
     double r = __builtin_fma(-alpha, a.Ap[n], a.r[n]);
     a.r[n] = r;
@@ -426,25 +529,7 @@ __global__ void k_pcg_b(PcgArgs a) {
     }
   }
   if (a.mgpre && n < nn) a.e0[n] = 0.0f;
-  double srr, srz;
-  if (block_reduce2(rr, rz, a.partials, a.counter, &srr, &srz) && threadIdx.x == 0) {
-    a.scal[S_RR] = srr;
-    a.scal[S_RELRES] = a.scal[S_FF] > 0.0 ? sqrt(srr / a.scal[S_FF]) : 0.0;
-    a.iflag[1] += 1;
-    if (!a.mgpre) {
-      double old = a.scal[S_RZ];
-      a.scal[S_BETA] = old != 0.0 ? srz / old : 0.0;
-      a.scal[S_RZ] = srz;
-    }
-    if (srr <= a.scal[S_TOL2] || a.iflag[1] >= a.max_iter) a.iflag[0] = 1;
-  }
-}
-
-/* pold <- pnew (swap by copy keeps kernel arguments fixed) */
-__global__ void k_pcg_swap(PcgArgs a) {
-  if (a.iflag[0]) return;
-  int n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n < a.s.nx * a.s.ny) a.pold[n] = a.pnew[n];
+  cg_put3(cg_slot(a, (it + 1) & 1), rr, rz, 0.0);
 }
 
 /* final: mean removal for singular systems, copy to phi */
@@ -1177,7 +1262,7 @@ int isk_poisson_free(iskra_ctx *c) {
   IskPoisson &p = c->ps;
   void *ptrs[] = {p.d_tags, p.d_tag_val, p.d_tri_a, p.d_tri_cp, p.d_tri_inv,
                   p.d_work, p.x, p.r, p.Ap, p.z, p.p[0], p.p[1], p.scal,
-                  p.partials, p.counter, p.iflag, p.sp_Sx, p.sp_Sy, p.sp_Li,
+                  p.partials, p.cgp, p.counter, p.iflag, p.sp_Sx, p.sp_Sy, p.sp_Li,
                   p.sp_F, p.sp_T1, p.sp_T2};
   for (void *q : ptrs) isk_free(q);
   p.sp_Sx = p.sp_Sy = p.sp_Li = p.sp_F = p.sp_T1 = p.sp_T2 = nullptr;
@@ -1191,7 +1276,7 @@ int isk_poisson_free(iskra_ctx *c) {
   p.d_tri_a = p.d_tri_cp = p.d_tri_inv = p.d_work = nullptr;
   p.x = p.r = p.Ap = nullptr;
   p.z = p.p[0] = p.p[1] = nullptr;
-  p.scal = p.partials = nullptr;
+  p.scal = p.partials = p.cgp = nullptr;
   p.counter = nullptr;
   p.iflag = nullptr;
   p.nlev = 0;
@@ -1301,6 +1386,7 @@ int isk_poisson_setup(iskra_ctx *c) {
       isk_alloc((void **)&p.p[0], nn * 4) || isk_alloc((void **)&p.p[1], nn * 4) ||
       isk_alloc((void **)&p.scal, S_WORDS * 8) ||
       isk_alloc((void **)&p.partials, (size_t)p.partials_blocks * 2 * 8) ||
+      isk_alloc((void **)&p.cgp, (size_t)nblk * 7 * 8) ||
       isk_alloc((void **)&p.counter, 64) || isk_alloc((void **)&p.iflag, 64))
     return ISKRA_E_HIP;
   ISK_HIP(hipMemset(p.counter, 0, 64));
@@ -1414,8 +1500,10 @@ static PcgArgs make_pcg(iskra_ctx *c, int mgpre) {
   a.r = p.r;
   a.Ap = p.Ap;
   a.z = p.z;
-  a.pold = p.p[0];
-  a.pnew = p.p[1];
+  a.p0 = p.p[0];
+  a.p1 = p.p[1];
+  a.cgp = p.cgp;
+  a.nblk = (isk_nodes(c) + RED_BLOCK - 1) / RED_BLOCK;
   a.f0 = p.nlev ? p.mf[0] : nullptr;
   a.e0 = p.nlev ? p.me[0] : nullptr;
   a.scal = p.scal;
@@ -1674,30 +1762,29 @@ int isk_poisson_solve_ex(iskra_ctx *c, int warm, int *iters, double *relres, int
     if (mgpre) {
       int rc = mg_vcycle(c, 0, 0, 0);
       if (rc) return rc;
-      hipLaunchKernelGGL(k_mg_toz, dim3(nblk), dim3(RED_BLOCK), 0, c->stream, p.me[0], p.z, nn, p.iflag);
+      hipLaunchKernelGGL(k_cg_toz, dim3(nblk), dim3(RED_BLOCK), 0, c->stream, a, 0);
     }
-    hipLaunchKernelGGL(k_pcg_rz, dim3(nblk), dim3(RED_BLOCK), 0, c->stream, a);
+    /* every batch ends with an A kernel, which decides convergence */
+    hipLaunchKernelGGL(k_cg_a, dim3(nblk), dim3(RED_BLOCK), 0, c->stream, a, 0);
     ISK_CHECK_LAUNCH();
-    int launched = 0;
+    int it = 0;
     while (true) {
-      for (int b = 0; b < (mgpre ? MG_BATCH : PCG_BATCH); ++b) {
-        hipLaunchKernelGGL(k_pcg_a, dim3(nblk), dim3(RED_BLOCK), 0, c->stream, a);
-        hipLaunchKernelGGL(k_pcg_b, dim3(nblk), dim3(RED_BLOCK), 0, c->stream, a);
+      for (int b = 0; b < (mgpre ? MG_BATCH : PCG_BATCH); ++b, ++it) {
+        hipLaunchKernelGGL(k_cg_b, dim3(nblk), dim3(RED_BLOCK), 0, c->stream, a, it);
         if (mgpre) {
           int rc = mg_vcycle(c, 0, 0, 0);
           if (rc) return rc;
-          hipLaunchKernelGGL(k_mg_toz, dim3(nblk), dim3(RED_BLOCK), 0, c->stream, p.me[0], p.z, nn, p.iflag);
-          hipLaunchKernelGGL(k_pcg_rz, dim3(nblk), dim3(RED_BLOCK), 0, c->stream, a);
+          hipLaunchKernelGGL(k_cg_toz, dim3(nblk), dim3(RED_BLOCK), 0, c->stream, a,
+                             (it + 1) & 1);
         }
-        hipLaunchKernelGGL(k_pcg_swap, dim3(nblk), dim3(RED_BLOCK), 0, c->stream, a);
+        hipLaunchKernelGGL(k_cg_a, dim3(nblk), dim3(RED_BLOCK), 0, c->stream, a, it + 1);
         ISK_CHECK_LAUNCH();
       }
       ISK_HIP(hipMemcpyAsync(hflag, p.iflag, sizeof(hflag), hipMemcpyDeviceToHost, c->stream));
       ISK_HIP(hipStreamSynchronize(c->stream));
       if (hflag[0]) break;
-      launched += mgpre ? MG_BATCH : PCG_BATCH;
-      if (launched > p.max_iter + 2 * PCG_BATCH) {
-        isk_set_error("poisson: CG stopped after %d host iterations without a device verdict", launched);
+      if (it > p.max_iter + 2 * PCG_BATCH) {
+        isk_set_error("poisson: CG stopped after %d host iterations without a device verdict", it);
         return ISKRA_E_CONVERGE;
       }
     }
