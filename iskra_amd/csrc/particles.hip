@@ -43,6 +43,7 @@ struct PushShape {
 #define SORT_PT 32
 #define SORT_CHUNK (SORT_BLOCK * SORT_PT)
 #define SORT_SEG 32
+#define SORT_DEPTH 8 /* scatter rounds whose loads are in flight together */
 static_assert(SORT_PT % 2 == 0, "the scatter handles rounds in pairs");
 
 /* ------------------------------------------------------------------------ */
@@ -300,19 +301,21 @@ __device__ void sort_hist_body(const int32_t *cell, const long long *hdr, int *H
         const long long p = base + (long long)(r0 + u) * SORT_BLOCK + threadIdx.x;
         cv[u] = cell[p < np ? p : np - 1];
       }
+      /* valid lanes are a prefix of the wave, so lane 0 is valid when any
+       * lane is; lane 0's key (in cell-ordered data mostly the key of the
+       * whole wave) is counted with one ballot, the other lanes add 1 each:
+       * a fixed instruction count per round instead of one pass per
+       * distinct key */
 #pragma unroll
       for (int u = 0; u < 16; ++u) {
         const long long p = base + (long long)(r0 + u) * SORT_BLOCK + threadIdx.x;
         const bool valid = p < np;
         const int key = valid ? sort_key(cv[u], kd) : -1;
-        unsigned long long todo = __ballot(valid);
-        while (todo) {
-          int leader = __ffsll((long long)todo) - 1;
-          int lk = __shfl(key, leader);
-          unsigned long long m = __ballot(valid && key == lk);
-          if (lane == leader) atomicAdd(&sh[lk], (int)__popcll(m));
-          todo &= ~m;
-        }
+        if (__ballot(valid) == 0ull) continue;
+        const int k0 = __builtin_amdgcn_readlane(key, 0);
+        const unsigned long long m0 = __ballot(key == k0);
+        if (lane == 0) atomicAdd(&sh[k0], (int)__popcll(m0));
+        if (valid && key != k0) atomicAdd(&sh[key], 1);
       }
     }
   }
@@ -501,18 +504,22 @@ __device__ void sort_scatter_body(const ScatterArgs &a, const int blk) {
   if (np <= 0 || b0 >= np) return;
   const int lane = threadIdx.x & 63;
   const int w = threadIdx.x >> 6;
-  /* two records in registers: round r+1 is requested before round r is
-   * ranked and written, so its loads overlap that work (SORT_PT is even);
-   * rounds r and r+1 use the two group-list buffers */
-  SortPay A, B;
-  scatter_load(a, b0 + threadIdx.x, np, A);
-  for (int r = 0; r < SORT_PT; r += 2) {
-    const long long pa = b0 + (long long)r * SORT_BLOCK + threadIdx.x;
-    const long long pb = pa + SORT_BLOCK;
-    scatter_load(a, pb, np, B);
-    scatter_round(a, A, pa < np, base, gkey[0], gcnt[0], ng[0], lane, w);
-    if (r + 2 < SORT_PT) scatter_load(a, pb + SORT_BLOCK, np, A);
-    scatter_round(a, B, pb < np, base, gkey[1], gcnt[1], ng[1], lane, w);
+  /* SORT_DEPTH records in registers: the loads of round r + SORT_DEPTH are
+   * issued as soon as round r is written, so SORT_DEPTH rounds of loads are
+   * in flight per work-group; round parity (SORT_DEPTH and SORT_PT are even)
+   * picks the group-list buffer */
+  SortPay R[SORT_DEPTH];
+#pragma unroll
+  for (int u = 0; u < SORT_DEPTH; ++u)
+    scatter_load(a, b0 + (long long)u * SORT_BLOCK + threadIdx.x, np, R[u]);
+  for (int r = 0; r < SORT_PT; r += SORT_DEPTH) {
+#pragma unroll
+    for (int u = 0; u < SORT_DEPTH; ++u) {
+      const long long pr = b0 + (long long)(r + u) * SORT_BLOCK + threadIdx.x;
+      scatter_round(a, R[u], pr < np, base, gkey[u & 1], gcnt[u & 1], ng[u & 1], lane, w);
+      if (r + u + SORT_DEPTH < SORT_PT)
+        scatter_load(a, pr + (long long)SORT_DEPTH * SORT_BLOCK, np, R[u]);
+    }
   }
 }
 
