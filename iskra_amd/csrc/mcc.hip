@@ -224,13 +224,28 @@ __device__ __forceinline__ void mcc_cand_body(const MccArgs &a, const int blk) {
   if (threadIdx.x == 0) ISK_GL(a.cand_cnt)[blk] = ncand;
 }
 
-/* phase 2: one wave per phase-1 work-group works through its candidates,
- * 64 at a time (the heavy FP64 collision code runs in its own kernel, so the
- * light phase 1 keeps a high occupancy) */
+/* phase 2: one wave per phase-1 work-group works through its candidates
+ * (the heavy FP64 collision code runs in its own kernel, so the light phase 1
+ * keeps a high occupancy).  Pass A tests every candidate, 64 at a time with
+ * all lanes busy: the process k, the acceptance, the event counts, and for an
+ * ionisation the product slot, ranked in candidate order so the products keep
+ * particle order.  The accepted candidates are listed in LDS by collision
+ * type; pass B runs each type's list 64 at a time, so the lanes of a pass run
+ * the same code instead of every type's branch in turn under masks.  Pass A
+ * uses no draws, so each candidate's draws in pass B are the ones it would
+ * take in a single pass. */
 #define MCC_COLL 64
+#define MCC_NTYPE 4
+#define MCC_SEG 256 /* candidates per pass A + pass B segment */
 __device__ __forceinline__ void mcc_coll_body(const MccArgs &a, const int blk) {
   __shared__ unsigned int evl[ISK_MAX_COLL];
+  /* accepted candidates of a segment by type: candidate index | process << 11 */
+  __shared__ unsigned short s_list[MCC_NTYPE][MCC_SEG];
+  /* product slot of the ionisation list's entries */
+  __shared__ unsigned short s_slot[MCC_SEG];
+  __shared__ int s_cnt[MCC_NTYPE];
   const int lane = threadIdx.x;
+  const unsigned long long below = (1ull << lane) - 1ull;
   const uint32_t step = a.step_dev ? (uint32_t)(a.step + *a.step_dev) : a.step;
   const long long b0 = (long long)blk * MCC_PPB;
   const int ncand = ISK_GL(a.cand_cnt)[blk];
@@ -239,12 +254,14 @@ __device__ __forceinline__ void mcc_coll_body(const MccArgs &a, const int blk) {
     return;
   }
   if (lane < ISK_MAX_COLL) evl[lane] = 0u;
-  /* the cross-section tables in LDS (the binary search then waits on LDS,
-   * not on a chain of global loads) */
-  __shared__ double s_eps[MCC_TAB_LDS], s_sig[MCC_TAB_LDS];
+  /* the cross-section tables in LDS when they fit (dynamic LDS of
+   * mcc_tab_shm bytes): the binary search then waits on LDS, not on a chain
+   * of global loads */
+  extern __shared__ double s_tab[];
   __shared__ int s_off[ISK_MAX_COLL + 1];
   const int ntab = a.toff[a.ncoll];
   const bool tab_lds = ntab <= MCC_TAB_LDS;
+  double *s_eps = s_tab, *s_sig = s_tab + ntab;
   if ((int)threadIdx.x <= a.ncoll) s_off[threadIdx.x] = a.toff[threadIdx.x];
   if (tab_lds)
     for (int q = threadIdx.x; q < ntab; q += MCC_COLL) {
@@ -252,149 +269,73 @@ __device__ __forceinline__ void mcc_coll_body(const MccArgs &a, const int blk) {
       s_sig[q] = ISK_GL(a.tsig)[q];
     }
   __syncthreads();
-  /* one candidate per lane, MCC_COLL at a time; the list is in particle
-   * order, so ionisation products, ranked per pass in candidate order, come
-   * out in particle order */
+  const int N = a.ncoll;
   int nprod = 0;
-  for (int t0 = 0; t0 < ncand; t0 += MCC_COLL) {
-  const int tcand = t0 + (int)threadIdx.x;
-  int prod = 0;
-  float pe[3] = {0.f, 0.f, 0.f}, pi[3] = {0.f, 0.f, 0.f};
-  int32_t pcell = 0;
-  float pox = 0.f, poy = 0.f;
-  int evkey = -1; /* node * N + process of this thread's collision */
-  int lt = 0;
-  if (tcand < ncand) {
-    lt = ISK_GL(a.cand_i)[b0 + tcand];
-    const long long p = b0 + lt;
-    const double U = ISK_GL(a.cand_u)[b0 + tcand];
-    const int32_t c = ISK_GL(a.cell)[p];
-    Draws d;
-    d.id = ISK_GL(a.id)[p];
-    d.step = step;
-    d.stream = a.stream;
-    d.k0 = a.k0;
-    d.k1 = a.k1;
-    d.block = 1;
-    d.half = 0;
-    {
-      {
-        int i = c & 0xFFFF, j = c >> 16;
-        int node = a.dim == 2 ? j * a.nx + i : i;
-        double n = ISK_GL(a.tn)[node];
-        if (n >= 0.0) {
-          const int N = a.ncoll;
-          int k = (int)floor((double)N * U);
-          if (k > N - 1) k = N - 1;
-          double vx = ISK_GL(a.vx)[p], vy = ISK_GL(a.vy)[p], vz = ISK_GL(a.vz)[p];
-          double tvx = 0.0, tvy = 0.0;
-          if (a.tqm != 0.0) {
-            if (a.dim == 2) {
-              tvx = a.tqm * (double)ISK_GL(a.E)[2 * node] * a.dt;
-              tvy = a.tqm * (double)ISK_GL(a.E)[2 * node + 1] * a.dt;
-            } else {
-              tvx = a.tqm * (double)ISK_GL(a.E)[node] * a.dt;
-            }
+  for (int s0 = 0; s0 < ncand; s0 += MCC_SEG) {
+  const int s1 = min(ncand, s0 + MCC_SEG);
+  int lcnt[MCC_NTYPE] = {0, 0, 0, 0};
+  /* pass A */
+  for (int t0 = s0; t0 < s1; t0 += MCC_COLL) {
+    const int tcand = t0 + lane;
+    int evkey = -1; /* node * N + process of this thread's collision */
+    int type = -1, k = 0, prod = 0;
+    if (tcand < s1) {
+      const int lt = ISK_GL(a.cand_i)[b0 + tcand];
+      const long long p = b0 + lt;
+      const double U = ISK_GL(a.cand_u)[b0 + tcand];
+      const int32_t c = ISK_GL(a.cell)[p];
+      const int i = c & 0xFFFF, j = c >> 16;
+      const int node = a.dim == 2 ? j * a.nx + i : i;
+      const double n = ISK_GL(a.tn)[node];
+      if (n >= 0.0) {
+        k = (int)floor((double)N * U);
+        if (k > N - 1) k = N - 1;
+        const double vx = ISK_GL(a.vx)[p], vy = ISK_GL(a.vy)[p], vz = ISK_GL(a.vz)[p];
+        double tvx = 0.0, tvy = 0.0;
+        if (a.tqm != 0.0) {
+          if (a.dim == 2) {
+            tvx = a.tqm * (double)ISK_GL(a.E)[2 * node] * a.dt;
+            tvy = a.tqm * (double)ISK_GL(a.E)[2 * node + 1] * a.dt;
+          } else {
+            tvx = a.tqm * (double)ISK_GL(a.E)[node] * a.dt;
           }
-          double gx = tvx - vx, gy = tvy - vy, gz = 0.0 - vz;
-          double g = sqrt(gx * gx + gy * gy + gz * gz);
-          double eps = (0.5 * a.m_eV) * (g * g);
-          double sg = (tab_lds ? interp_sigma(s_off, s_eps, s_sig, k, eps)
-                               : interp_sigma(s_off, a.teps, a.tsig, k, eps)) * g;
-          double P = 1.0 - isk_exp(((-n) * sg) * a.dt);
-          P /= a.candP;
-          if (P > 1.0) atomicOr(a.errflag, 1);
-          if (U > (double)(k + 1) / (double)N - P) {
-            evkey = node * N + k;
-            const int type = a.types[k];
-            double sv[3] = {vx, vy, vz};
-            double out[3];
-            if (type == ISKRA_COLL_ELASTIC_ISOTROPIC || type == ISKRA_COLL_ELASTIC_BACKWARD) {
-              double mr1 = a.ms / (a.ms + a.mt), mr2 = a.mt / (a.ms + a.mt);
-              double tv[3];
-              randn3(d, tv);
-              for (int q = 0; q < 3; ++q) tv[q] *= a.vth_t;
-              double vr[3], w[3];
-              for (int q = 0; q < 3; ++q) {
-                vr[q] = sv[q] - tv[q];
-                w[q] = mr1 * sv[q] + mr2 * tv[q];
-              }
-              double nvr = sqrt(vr[0] * vr[0] + vr[1] * vr[1] + vr[2] * vr[2]);
-              double dir[3];
-              if (type == ISKRA_COLL_ELASTIC_ISOTROPIC)
-                isotropic_scattering(vr[0], vr[1], vr[2], d, dir);
-              else
-                diffuse_reflection(vr[0], vr[1], vr[2], d, dir);
-              for (int q = 0; q < 3; ++q) out[q] = w[q] + mr2 * (nvr * dir[q]);
-              ISK_GL(a.vx)[p] = (float)out[0];
-              ISK_GL(a.vy)[p] = (float)out[1];
-              ISK_GL(a.vz)[p] = (float)out[2];
-            } else if (type == ISKRA_COLL_EXCITATION) {
-              double sE = (0.5 * a.m_eV) * (vx * vx + vy * vy + vz * vz) - a.energy[k];
-              if (sE >= 0.0) {
-                double al = sqrt(2.0 / a.m_eV);
-                double ev = al * sqrt(sE);
-                double dir[3];
-                isotropic_scattering(vx, vy, vz, d, dir);
-                ISK_GL(a.vx)[p] = (float)(ev * dir[0]);
-                ISK_GL(a.vy)[p] = (float)(ev * dir[1]);
-                ISK_GL(a.vz)[p] = (float)(ev * dir[2]);
-              }
-            } else if (type == ISKRA_COLL_IONIZATION) {
-              double sE = (0.5 * a.m_eV) * (vx * vx + vy * vy + vz * vz) - a.energy[k];
-              if (sE >= 0.0) {
-                double e1E = sE * d.next();
-                double e2E = sE - e1E;
-                double al = sqrt(2.0 / a.m_eV);
-                double e1v = al * sqrt(e1E), e2v = al * sqrt(e2E);
-                double dir[3];
-                diffuse_reflection(vx, vy, vz, d, dir);
-                double nv[3] = {e1v * dir[0], e1v * dir[1], e1v * dir[2]};
-                ISK_GL(a.vx)[p] = (float)nv[0];
-                ISK_GL(a.vy)[p] = (float)nv[1];
-                ISK_GL(a.vz)[p] = (float)nv[2];
-                double dir2[3];
-                diffuse_reflection(nv[0], nv[1], nv[2], d, dir2);
-                double tv[3];
-                randn3(d, tv);
-                prod = 1;
-                pcell = c;
-                pox = ISK_GL(a.ox)[p];
-                poy = a.dim == 2 ? ISK_GL(a.oy)[p] : 0.f;
-                for (int q = 0; q < 3; ++q) {
-                  pe[q] = (float)(e2v * dir2[q]);
-                  pi[q] = (float)(tv[q] * a.vth_t);
-                }
-              }
-            }
+        }
+        const double gx = tvx - vx, gy = tvy - vy, gz = 0.0 - vz;
+        const double g = sqrt(gx * gx + gy * gy + gz * gz);
+        const double eps = (0.5 * a.m_eV) * (g * g);
+        const double sg = (tab_lds ? interp_sigma(s_off, s_eps, s_sig, k, eps)
+                                   : interp_sigma(s_off, a.teps, a.tsig, k, eps)) * g;
+        double P = 1.0 - isk_exp(((-n) * sg) * a.dt);
+        P /= a.candP;
+        if (P > 1.0) atomicOr(a.errflag, 1);
+        if (U > (double)(k + 1) / (double)N - P) {
+          evkey = node * N + k;
+          type = a.types[k];
+          if (type == ISKRA_COLL_IONIZATION) {
+            const double sE = (0.5 * a.m_eV) * (vx * vx + vy * vy + vz * vz) - a.energy[k];
+            prod = sE >= 0.0;
           }
         }
       }
     }
-  }
-  {
-    const unsigned long long m = __ballot(prod != 0);
-    const int rank = __popcll(m & ((1ull << lane) - 1ull));
-    const int off = nprod;
-    if (prod) {
-      const size_t sidx = (size_t)b0 + off + rank;
-      ISK_GL(a.stage_cell)[sidx] = pcell;
-      auto *st = ISK_GL(a.stage) + sidx * 8;
-      st[0] = pox;
-      st[1] = poy;
-      st[2] = pe[0];
-      st[3] = pe[1];
-      st[4] = pe[2];
-      st[5] = pi[0];
-      st[6] = pi[1];
-      st[7] = pi[2];
+    /* product slots in candidate order */
+    const unsigned long long mp = __ballot(prod != 0);
+    const int slot = nprod + __popcll(mp & below);
+    nprod += __popcll(mp);
+    /* the accepted candidates, listed by type in candidate order */
+#pragma unroll
+    for (int T = 0; T < MCC_NTYPE; ++T) {
+      const unsigned long long mt = __ballot(type == T);
+      if (type == T) {
+        const int q = lcnt[T] + __popcll(mt & below);
+        s_list[T][q] = (unsigned short)((tcand - s0) | (k << 11));
+        if (T == ISKRA_COLL_IONIZATION) s_slot[q] = (unsigned short)(prod ? slot : 0xFFFF);
+      }
+      lcnt[T] += __popcll(mt);
     }
-    nprod += __popcll(m);
-  }
-  /* event counts: equal (node, process) keys of a wave are counted with one
-   * atomic, the per-process totals of the work-group go through LDS (integer
-   * sums, so the grouping does not change them) */
-  {
+    /* event counts: equal (node, process) keys of a wave are counted with one
+     * atomic, the per-process totals of the work-group go through LDS
+     * (integer sums, so the grouping does not change them) */
     unsigned long long todo = __ballot(evkey >= 0);
     while (todo) {
       const int leader = __ffsll((long long)todo) - 1;
@@ -402,20 +343,104 @@ __device__ __forceinline__ void mcc_coll_body(const MccArgs &a, const int blk) {
       const unsigned long long mk = __ballot(evkey == lk);
       if (lane == leader) {
         atomicAdd(&a.nu[lk], (unsigned long long)__popcll(mk));
-        atomicAdd(&evl[lk % a.ncoll], (unsigned int)__popcll(mk));
+        atomicAdd(&evl[lk % N], (unsigned int)__popcll(mk));
       }
       todo &= ~mk;
     }
   }
-  } /* phase 2 */
+  if (lane < MCC_NTYPE) s_cnt[lane] = lane == 0 ? lcnt[0] : lane == 1 ? lcnt[1] : lane == 2 ? lcnt[2] : lcnt[3];
   __syncthreads();
+  /* pass B (T is uniform: the type branches below are scalar branches) */
+#pragma unroll 1
+  for (int T = 0; T < MCC_NTYPE; ++T) {
+    const int nT = s_cnt[T];
+    for (int q0 = 0; q0 < nT; q0 += MCC_COLL) {
+      const int q = q0 + lane;
+      if (q >= nT) continue;
+      const int rec = s_list[T][q];
+      const int tcand = s0 + (rec & 0x7FF), k = rec >> 11;
+      const int lt = ISK_GL(a.cand_i)[b0 + tcand];
+      const long long p = b0 + lt;
+      Draws d;
+      d.id = ISK_GL(a.id)[p];
+      d.step = step;
+      d.stream = a.stream;
+      d.k0 = a.k0;
+      d.k1 = a.k1;
+      d.block = 1;
+      d.half = 0;
+      const double vx = ISK_GL(a.vx)[p], vy = ISK_GL(a.vy)[p], vz = ISK_GL(a.vz)[p];
+      if (T == ISKRA_COLL_ELASTIC_ISOTROPIC || T == ISKRA_COLL_ELASTIC_BACKWARD) {
+        const double sv[3] = {vx, vy, vz};
+        double mr1 = a.ms / (a.ms + a.mt), mr2 = a.mt / (a.ms + a.mt);
+        double tv[3];
+        randn3(d, tv);
+        for (int r = 0; r < 3; ++r) tv[r] *= a.vth_t;
+        double vr[3], w[3];
+        for (int r = 0; r < 3; ++r) {
+          vr[r] = sv[r] - tv[r];
+          w[r] = mr1 * sv[r] + mr2 * tv[r];
+        }
+        double nvr = sqrt(vr[0] * vr[0] + vr[1] * vr[1] + vr[2] * vr[2]);
+        double dir[3];
+        if (T == ISKRA_COLL_ELASTIC_ISOTROPIC)
+          isotropic_scattering(vr[0], vr[1], vr[2], d, dir);
+        else
+          diffuse_reflection(vr[0], vr[1], vr[2], d, dir);
+        ISK_GL(a.vx)[p] = (float)(w[0] + mr2 * (nvr * dir[0]));
+        ISK_GL(a.vy)[p] = (float)(w[1] + mr2 * (nvr * dir[1]));
+        ISK_GL(a.vz)[p] = (float)(w[2] + mr2 * (nvr * dir[2]));
+      } else if (T == ISKRA_COLL_EXCITATION) {
+        double sE = (0.5 * a.m_eV) * (vx * vx + vy * vy + vz * vz) - a.energy[k];
+        if (sE >= 0.0) {
+          double al = sqrt(2.0 / a.m_eV);
+          double ev = al * sqrt(sE);
+          double dir[3];
+          isotropic_scattering(vx, vy, vz, d, dir);
+          ISK_GL(a.vx)[p] = (float)(ev * dir[0]);
+          ISK_GL(a.vy)[p] = (float)(ev * dir[1]);
+          ISK_GL(a.vz)[p] = (float)(ev * dir[2]);
+        }
+      } else {
+        const int slot = s_slot[q];
+        if (slot != 0xFFFF) { /* sE >= 0, decided in pass A */
+          double sE = (0.5 * a.m_eV) * (vx * vx + vy * vy + vz * vz) - a.energy[k];
+          double e1E = sE * d.next();
+          double e2E = sE - e1E;
+          double al = sqrt(2.0 / a.m_eV);
+          double e1v = al * sqrt(e1E), e2v = al * sqrt(e2E);
+          double dir[3];
+          diffuse_reflection(vx, vy, vz, d, dir);
+          double nv[3] = {e1v * dir[0], e1v * dir[1], e1v * dir[2]};
+          ISK_GL(a.vx)[p] = (float)nv[0];
+          ISK_GL(a.vy)[p] = (float)nv[1];
+          ISK_GL(a.vz)[p] = (float)nv[2];
+          double dir2[3];
+          diffuse_reflection(nv[0], nv[1], nv[2], d, dir2);
+          double tv[3];
+          randn3(d, tv);
+          const size_t sidx = (size_t)b0 + slot;
+          ISK_GL(a.stage_cell)[sidx] = ISK_GL(a.cell)[p];
+          auto *st = ISK_GL(a.stage) + sidx * 8;
+          st[0] = ISK_GL(a.ox)[p];
+          st[1] = a.dim == 2 ? ISK_GL(a.oy)[p] : 0.f;
+          for (int r = 0; r < 3; ++r) {
+            st[2 + r] = (float)(e2v * dir2[r]);
+            st[5 + r] = (float)(tv[r] * a.vth_t);
+          }
+        }
+      }
+    }
+  }
+  __syncthreads(); /* the next segment rewrites the lists */
+  } /* segments */
   if ((int)threadIdx.x < a.ncoll && evl[threadIdx.x])
     atomicAdd(&a.events[threadIdx.x], (unsigned long long)evl[threadIdx.x]);
   if (threadIdx.x == 0) a.blk_cnt[blk] = nprod;
 }
 
 __global__ __launch_bounds__(MCC_BLOCK) void k_mcc_cand(MccArgs a) { mcc_cand_body(a, blockIdx.x); }
-__global__ __launch_bounds__(MCC_COLL) void k_mcc_coll(MccArgs a) { mcc_coll_body(a, blockIdx.x); }
+__global__ __launch_bounds__(MCC_COLL) __attribute__((amdgpu_waves_per_eu(3, 8))) void k_mcc_coll(MccArgs a) { mcc_coll_body(a, blockIdx.x); }
 
 /* exclusive scan of the block product counts (one work-group); cnt[nblk] = total */
 __device__ __forceinline__ void mcc_scan_body(int *cnt, int nblk) {
@@ -751,6 +776,12 @@ static int mcc_prepare(iskra_ctx *c, int idx, double dt, int b_src, int b_ion, M
   return ISKRA_OK;
 }
 
+/* dynamic LDS of the collision kernel: the cross-section tables when they fit */
+static size_t mcc_tab_shm(const IskMcc &m) {
+  const size_t nt = m.tab_eps.size();
+  return nt <= MCC_TAB_LDS ? nt * 2 * sizeof(double) : 0;
+}
+
 int isk_mcc_run(iskra_ctx *c, int idx, double dt, int64_t *events) {
   IskMcc &m = c->mcc[idx];
   IskSpecies &s = c->sp[m.src];
@@ -765,7 +796,7 @@ int isk_mcc_run(iskra_ctx *c, int idx, double dt, int64_t *events) {
   if (events) ISK_HIP(hipMemsetAsync(m.events, 0, ISK_MAX_COLL * 8, c->stream));
   hipLaunchKernelGGL(k_mcc_cand, dim3(nblk), dim3(MCC_BLOCK), 0, c->stream, a);
   ISK_CHECK_LAUNCH();
-  hipLaunchKernelGGL(k_mcc_coll, dim3(nblk), dim3(MCC_COLL), 0, c->stream, a);
+  hipLaunchKernelGGL(k_mcc_coll, dim3(nblk), dim3(MCC_COLL), mcc_tab_shm(m), c->stream, a);
   ISK_CHECK_LAUNCH();
   hipLaunchKernelGGL(k_mcc_scan, dim3(1), dim3(1024), 0, c->stream, m.blk_cnt, nblk);
   ISK_CHECK_LAUNCH();
@@ -800,8 +831,9 @@ __global__ __launch_bounds__(MCC_BLOCK) void kb_mcc_cand(const MccArgs *A, const
   mcc_cand_body(A[i], blockIdx.x - bstart[i]);
 }
 
-__global__ __launch_bounds__(MCC_COLL) void kb_mcc_coll(const MccArgs *A, const int *bstart,
-                                                        int n) {
+/* at most 168 VGPRs: three waves per SIMD */
+__global__ __launch_bounds__(MCC_COLL) __attribute__((amdgpu_waves_per_eu(3, 8))) void kb_mcc_coll(
+    const MccArgs *A, const int *bstart, int n) {
   const int i = isk_bfind(bstart, n, blockIdx.x);
   mcc_coll_body(A[i], blockIdx.x - bstart[i]);
 }
@@ -832,9 +864,11 @@ int isk_bbuild_mcc(iskra_ctx **cs, int n, int m, double dt, const int *cur_src,
   std::vector<AppendArgs> P(n);
   std::vector<CommitArgs> CM(n);
   std::vector<int> blocks(n), ablocks(n), one(n, 1);
+  size_t tab_shm = 0;
   for (int i = 0; i < n; ++i) {
     int rc = mcc_prepare(cs[i], m, dt, cur_src[i], cur_ion[i], &A[i], &P[i], &CM[i], &blocks[i]);
     if (rc) return rc;
+    tab_shm = std::max(tab_shm, mcc_tab_shm(cs[i]->mcc[m]));
     A[i].step_dev = step_dev;
     ablocks[i] = mcc_append_blocks(blocks[i]);
   }
@@ -843,7 +877,7 @@ int isk_bbuild_mcc(iskra_ctx **cs, int n, int m, double dt, const int *cur_src,
                                0)))
     return rc;
   if ((rc = isk_blaunch_upload(&L[1], ISK_B_MCC_COLL, A.data(), sizeof(MccArgs), blocks,
-                               MCC_COLL, 0)))
+                               MCC_COLL, tab_shm)))
     return rc;
   if ((rc = isk_blaunch_upload(&L[2], ISK_B_MCC_SCAN, CM.data(), sizeof(CommitArgs), one, 1024,
                                0)))
@@ -862,7 +896,7 @@ int isk_blaunch_mcc(const IskBLaunch &L, hipStream_t st) {
       hipLaunchKernelGGL(kb_mcc_cand, g, b, 0, st, (const MccArgs *)L.args, L.bstart, L.n);
       break;
     case ISK_B_MCC_COLL:
-      hipLaunchKernelGGL(kb_mcc_coll, g, b, 0, st, (const MccArgs *)L.args, L.bstart, L.n);
+      hipLaunchKernelGGL(kb_mcc_coll, g, b, L.shm, st, (const MccArgs *)L.args, L.bstart, L.n);
       break;
     case ISK_B_MCC_SCAN:
       hipLaunchKernelGGL(kb_mcc_scan, g, b, 0, st, (const CommitArgs *)L.args, L.bstart, L.n);
