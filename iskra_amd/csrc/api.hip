@@ -299,7 +299,7 @@ static void release_fields(iskra_ctx *c) {
   for (int m = 0; m < c->nmcc; ++m) {
     IskMcc &q = c->mcc[m];
     void *ptrs[] = {q.d_eps, q.d_sig, q.d_off, q.target_n, q.nu, q.events, q.errflag,
-                    q.prod_flag, q.stage, q.blk_cnt, q.cand_i, q.cand_u, q.cand_cnt};
+                    q.prod_flag, q.stage, q.blk_cnt, q.cand_i, q.cand_cnt};
     for (void *p : ptrs) isk_free(p);
     q.tab_off.clear();
     q.tab_eps.clear();

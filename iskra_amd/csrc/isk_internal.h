@@ -113,9 +113,8 @@ struct IskMcc {
   int *prod_flag;            /* per source particle: 1 if it ionised */
   float *stage;              /* per source particle: 2 offsets + 3 v of the new electron + 3 v of the ions */
   int *blk_cnt;              /* per block product count, scanned in place */
-  int *cand_i;               /* per block: candidate particles (local index), in order */
-  double *cand_u;            /* and their second uniform */
-  int *cand_cnt;             /* per block candidate count */
+  int *cand_i;               /* per block: accepted candidates (list records), in order */
+  int *cand_cnt;             /* per block accepted count */
   int64_t stage_cap;
   uint32_t stream;
   uint32_t step;

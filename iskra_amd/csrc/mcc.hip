@@ -67,9 +67,8 @@ struct MccArgs {
   int *blk_cnt;
   float *stage;
   int32_t *stage_cell;
-  int *cand_i;    /* candidates of each work-group (local particle index) */
-  double *cand_u; /* and their second uniform */
-  int *cand_cnt;  /* per work-group candidate count */
+  int *cand_i;   /* accepted candidates of each work-group (list records) */
+  int *cand_cnt; /* per work-group accepted count */
 };
 
 struct Draws {
@@ -165,21 +164,47 @@ __device__ void randn3(Draws &d, double *n) {
   n[2] = r2 * c2;
 }
 
-/* Two kernels per MCC run, on work-groups of MCC_PPB particles.  Phase 1
- * (k_mcc_cand): every live particle draws (Uc, U) from Philox block 0 of its
- * stream (the cell and id loads of the work-group's MCC_RPT rounds are
- * issued together) and the candidates (Uc < candP) are listed in particle
- * order.  Phase 2 (k_mcc_coll): one wave per phase-1 work-group runs the
- * candidates 64 at a time, all lanes busy; each candidate's draws continue
- * at Philox block 1, as they would in one pass, and products keep particle
- * order (the list is in particle order). */
+/* Two kernels per MCC run, on work-groups of MCC_PPB particles.
+ * k_mcc_cand: every live particle draws (Uc, U) from Philox block 0 of its
+ * stream (the cell and id loads of the work-group's MCC_RPT rounds are issued
+ * together); the candidates (Uc < candP) are compacted in LDS in particle
+ * order and then tested 256 at a time with all lanes busy: process k,
+ * acceptance, event counts, and for an ionisation the product slot, ranked
+ * in candidate order so the products keep particle order.  The accepted
+ * candidates go to the block's list (local index | process << 11 |
+ * (slot + 1) << 16).  k_mcc_coll: one wave per list groups the accepted
+ * candidates by collision type in LDS and runs each type's FP64 collision
+ * code 64 at a time, so a wave runs one type's code instead of every type's
+ * branch in turn; each candidate's draws continue at Philox block 1, as they
+ * would in one pass. */
+#define MCC_NTYPE 4
 __device__ __forceinline__ void mcc_cand_body(const MccArgs &a, const int blk) {
   __shared__ int wcount[MCC_BLOCK / 64];
+  __shared__ int wprod[MCC_BLOCK / 64];
+  __shared__ int s_ci[MCC_PPB];    /* candidates: local particle index */
+  __shared__ double s_cu[MCC_PPB]; /* and their second uniform */
+  __shared__ unsigned int evl[ISK_MAX_COLL];
+  __shared__ int s_off[ISK_MAX_COLL + 1];
+  /* the cross-section tables in LDS when they fit (dynamic LDS of
+   * mcc_tab_shm bytes): the binary search then waits on LDS, not on a chain
+   * of global loads */
+  extern __shared__ double s_tab[];
   const long long np = a.hdr[HDR_NP];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const unsigned long long below = (1ull << lane) - 1ull;
   const uint32_t step = a.step_dev ? (uint32_t)(a.step + *a.step_dev) : a.step;
   const long long b0 = (long long)blk * MCC_PPB;
-  /* phase 1: cell and id of every round first (loads in flight together) */
+  if ((int)threadIdx.x < ISK_MAX_COLL) evl[threadIdx.x] = 0u;
+  const int ntab = a.toff[a.ncoll];
+  const bool tab_lds = ntab <= MCC_TAB_LDS;
+  double *s_eps = s_tab, *s_sig = s_tab + ntab;
+  if ((int)threadIdx.x <= a.ncoll) s_off[threadIdx.x] = a.toff[threadIdx.x];
+  if (tab_lds)
+    for (int q = threadIdx.x; q < ntab; q += MCC_BLOCK) {
+      s_eps[q] = ISK_GL(a.teps)[q];
+      s_sig[q] = ISK_GL(a.tsig)[q];
+    }
+  /* cell and id of every round first (loads in flight together) */
   int32_t cr[MCC_RPT];
   uint32_t idr[MCC_RPT];
 #pragma unroll
@@ -207,82 +232,32 @@ __device__ __forceinline__ void mcc_cand_body(const MccArgs &a, const int blk) {
       U1 = d.next();
       cand = Uc < a.candP;
     }
-    /* candidates listed in particle order */
+    /* candidates compacted in particle order */
     const unsigned long long m = __ballot(cand);
-    const int rank = __popcll(m & ((1ull << lane) - 1ull));
     if (lane == 0) wcount[w] = __popcll(m);
     __syncthreads();
     int off = ncand;
     for (int k = 0; k < w; ++k) off += wcount[k];
     if (cand) {
-      ISK_GL(a.cand_i)[b0 + off + rank] = lt;
-      ISK_GL(a.cand_u)[b0 + off + rank] = U1;
+      const int q = off + __popcll(m & below);
+      s_ci[q] = lt;
+      s_cu[q] = U1;
     }
     for (int k = 0; k < MCC_BLOCK / 64; ++k) ncand += wcount[k];
     __syncthreads();
   }
-  if (threadIdx.x == 0) ISK_GL(a.cand_cnt)[blk] = ncand;
-}
-
-/* phase 2: one wave per phase-1 work-group works through its candidates
- * (the heavy FP64 collision code runs in its own kernel, so the light phase 1
- * keeps a high occupancy).  Pass A tests every candidate, 64 at a time with
- * all lanes busy: the process k, the acceptance, the event counts, and for an
- * ionisation the product slot, ranked in candidate order so the products keep
- * particle order.  The accepted candidates are listed in LDS by collision
- * type; pass B runs each type's list 64 at a time, so the lanes of a pass run
- * the same code instead of every type's branch in turn under masks.  Pass A
- * uses no draws, so each candidate's draws in pass B are the ones it would
- * take in a single pass. */
-#define MCC_COLL 64
-#define MCC_NTYPE 4
-#define MCC_SEG 256 /* candidates per pass A + pass B segment */
-__device__ __forceinline__ void mcc_coll_body(const MccArgs &a, const int blk) {
-  __shared__ unsigned int evl[ISK_MAX_COLL];
-  /* accepted candidates of a segment by type: candidate index | process << 11 */
-  __shared__ unsigned short s_list[MCC_NTYPE][MCC_SEG];
-  /* product slot of the ionisation list's entries */
-  __shared__ unsigned short s_slot[MCC_SEG];
-  __shared__ int s_cnt[MCC_NTYPE];
-  const int lane = threadIdx.x;
-  const unsigned long long below = (1ull << lane) - 1ull;
-  const uint32_t step = a.step_dev ? (uint32_t)(a.step + *a.step_dev) : a.step;
-  const long long b0 = (long long)blk * MCC_PPB;
-  const int ncand = ISK_GL(a.cand_cnt)[blk];
-  if (ncand == 0) {
-    if (lane == 0) a.blk_cnt[blk] = 0;
-    return;
-  }
-  if (lane < ISK_MAX_COLL) evl[lane] = 0u;
-  /* the cross-section tables in LDS when they fit (dynamic LDS of
-   * mcc_tab_shm bytes): the binary search then waits on LDS, not on a chain
-   * of global loads */
-  extern __shared__ double s_tab[];
-  __shared__ int s_off[ISK_MAX_COLL + 1];
-  const int ntab = a.toff[a.ncoll];
-  const bool tab_lds = ntab <= MCC_TAB_LDS;
-  double *s_eps = s_tab, *s_sig = s_tab + ntab;
-  if ((int)threadIdx.x <= a.ncoll) s_off[threadIdx.x] = a.toff[threadIdx.x];
-  if (tab_lds)
-    for (int q = threadIdx.x; q < ntab; q += MCC_COLL) {
-      s_eps[q] = ISK_GL(a.teps)[q];
-      s_sig[q] = ISK_GL(a.tsig)[q];
-    }
-  __syncthreads();
+  /* acceptance, MCC_BLOCK candidates at a time */
   const int N = a.ncoll;
-  int nprod = 0;
-  for (int s0 = 0; s0 < ncand; s0 += MCC_SEG) {
-  const int s1 = min(ncand, s0 + MCC_SEG);
-  int lcnt[MCC_NTYPE] = {0, 0, 0, 0};
-  /* pass A */
-  for (int t0 = s0; t0 < s1; t0 += MCC_COLL) {
-    const int tcand = t0 + lane;
+  int nacc = 0, nprod = 0;
+  for (int t0 = 0; t0 < ncand; t0 += MCC_BLOCK) {
+    const int tcand = t0 + (int)threadIdx.x;
     int evkey = -1; /* node * N + process of this thread's collision */
-    int type = -1, k = 0, prod = 0;
-    if (tcand < s1) {
-      const int lt = ISK_GL(a.cand_i)[b0 + tcand];
+    int k = 0, lt = 0;
+    bool acc = false, prod = false;
+    if (tcand < ncand) {
+      lt = s_ci[tcand];
+      const double U = s_cu[tcand];
       const long long p = b0 + lt;
-      const double U = ISK_GL(a.cand_u)[b0 + tcand];
       const int32_t c = ISK_GL(a.cell)[p];
       const int i = c & 0xFFFF, j = c >> 16;
       const int node = a.dim == 2 ? j * a.nx + i : i;
@@ -309,29 +284,34 @@ __device__ __forceinline__ void mcc_coll_body(const MccArgs &a, const int blk) {
         P /= a.candP;
         if (P > 1.0) atomicOr(a.errflag, 1);
         if (U > (double)(k + 1) / (double)N - P) {
+          acc = true;
           evkey = node * N + k;
-          type = a.types[k];
-          if (type == ISKRA_COLL_IONIZATION) {
+          if (a.types[k] == ISKRA_COLL_IONIZATION) {
             const double sE = (0.5 * a.m_eV) * (vx * vx + vy * vy + vz * vz) - a.energy[k];
             prod = sE >= 0.0;
           }
         }
       }
     }
-    /* product slots in candidate order */
-    const unsigned long long mp = __ballot(prod != 0);
-    const int slot = nprod + __popcll(mp & below);
-    nprod += __popcll(mp);
-    /* the accepted candidates, listed by type in candidate order */
-#pragma unroll
-    for (int T = 0; T < MCC_NTYPE; ++T) {
-      const unsigned long long mt = __ballot(type == T);
-      if (type == T) {
-        const int q = lcnt[T] + __popcll(mt & below);
-        s_list[T][q] = (unsigned short)((tcand - s0) | (k << 11));
-        if (T == ISKRA_COLL_IONIZATION) s_slot[q] = (unsigned short)(prod ? slot : 0xFFFF);
-      }
-      lcnt[T] += __popcll(mt);
+    /* accepted candidates and product slots, both in candidate order */
+    const unsigned long long ma = __ballot(acc), mp = __ballot(prod);
+    if (lane == 0) {
+      wcount[w] = __popcll(ma);
+      wprod[w] = __popcll(mp);
+    }
+    __syncthreads();
+    int aoff = nacc, poff = nprod;
+    for (int q = 0; q < w; ++q) {
+      aoff += wcount[q];
+      poff += wprod[q];
+    }
+    if (acc) {
+      const int slot = prod ? poff + __popcll(mp & below) : -1;
+      ISK_GL(a.cand_i)[b0 + aoff + __popcll(ma & below)] = lt | (k << 11) | ((slot + 1) << 16);
+    }
+    for (int q = 0; q < MCC_BLOCK / 64; ++q) {
+      nacc += wcount[q];
+      nprod += wprod[q];
     }
     /* event counts: equal (node, process) keys of a wave are counted with one
      * atomic, the per-process totals of the work-group go through LDS
@@ -347,63 +327,94 @@ __device__ __forceinline__ void mcc_coll_body(const MccArgs &a, const int blk) {
       }
       todo &= ~mk;
     }
+    __syncthreads(); /* wcount and wprod are rewritten next round */
   }
-  if (lane < MCC_NTYPE) s_cnt[lane] = lane == 0 ? lcnt[0] : lane == 1 ? lcnt[1] : lane == 2 ? lcnt[2] : lcnt[3];
-  __syncthreads();
-  /* pass B (T is uniform: the type branches below are scalar branches) */
+  if ((int)threadIdx.x < N && evl[threadIdx.x])
+    atomicAdd(&a.events[threadIdx.x], (unsigned long long)evl[threadIdx.x]);
+  if (threadIdx.x == 0) {
+    ISK_GL(a.cand_cnt)[blk] = nacc;
+    a.blk_cnt[blk] = nprod;
+  }
+}
+
+/* k_mcc_coll: one wave per work-group's list of accepted candidates */
+#define MCC_COLL 64
+#define MCC_SEG 256 /* accepted candidates grouped by type per segment */
+__device__ __forceinline__ void mcc_coll_body(const MccArgs &a, const int blk) {
+  /* accepted candidates of a segment by type: list position */
+  __shared__ unsigned short s_list[MCC_NTYPE][MCC_SEG];
+  __shared__ int s_cnt[MCC_NTYPE];
+  const int lane = threadIdx.x;
+  const unsigned long long below = (1ull << lane) - 1ull;
+  const uint32_t step = a.step_dev ? (uint32_t)(a.step + *a.step_dev) : a.step;
+  const long long b0 = (long long)blk * MCC_PPB;
+  const int nacc = ISK_GL(a.cand_cnt)[blk];
+  for (int s0 = 0; s0 < nacc; s0 += MCC_SEG) {
+    const int s1 = min(nacc, s0 + MCC_SEG);
+    int lcnt[MCC_NTYPE] = {0, 0, 0, 0};
+    for (int t0 = s0; t0 < s1; t0 += MCC_COLL) {
+      const int q = t0 + lane;
+      const int type = q < s1 ? a.types[(ISK_GL(a.cand_i)[b0 + q] >> 11) & 0x1F] : -1;
+#pragma unroll
+      for (int T = 0; T < MCC_NTYPE; ++T) {
+        const unsigned long long mt = __ballot(type == T);
+        if (type == T) s_list[T][lcnt[T] + __popcll(mt & below)] = (unsigned short)(q - s0);
+        lcnt[T] += __popcll(mt);
+      }
+    }
+    if (lane < MCC_NTYPE)
+      s_cnt[lane] = lane == 0 ? lcnt[0] : lane == 1 ? lcnt[1] : lane == 2 ? lcnt[2] : lcnt[3];
+    __syncthreads();
+    /* T is uniform: the type branches below are scalar branches */
 #pragma unroll 1
-  for (int T = 0; T < MCC_NTYPE; ++T) {
-    const int nT = s_cnt[T];
-    for (int q0 = 0; q0 < nT; q0 += MCC_COLL) {
-      const int q = q0 + lane;
-      if (q >= nT) continue;
-      const int rec = s_list[T][q];
-      const int tcand = s0 + (rec & 0x7FF), k = rec >> 11;
-      const int lt = ISK_GL(a.cand_i)[b0 + tcand];
-      const long long p = b0 + lt;
-      Draws d;
-      d.id = ISK_GL(a.id)[p];
-      d.step = step;
-      d.stream = a.stream;
-      d.k0 = a.k0;
-      d.k1 = a.k1;
-      d.block = 1;
-      d.half = 0;
-      const double vx = ISK_GL(a.vx)[p], vy = ISK_GL(a.vy)[p], vz = ISK_GL(a.vz)[p];
-      if (T == ISKRA_COLL_ELASTIC_ISOTROPIC || T == ISKRA_COLL_ELASTIC_BACKWARD) {
-        const double sv[3] = {vx, vy, vz};
-        double mr1 = a.ms / (a.ms + a.mt), mr2 = a.mt / (a.ms + a.mt);
-        double tv[3];
-        randn3(d, tv);
-        for (int r = 0; r < 3; ++r) tv[r] *= a.vth_t;
-        double vr[3], w[3];
-        for (int r = 0; r < 3; ++r) {
-          vr[r] = sv[r] - tv[r];
-          w[r] = mr1 * sv[r] + mr2 * tv[r];
-        }
-        double nvr = sqrt(vr[0] * vr[0] + vr[1] * vr[1] + vr[2] * vr[2]);
-        double dir[3];
-        if (T == ISKRA_COLL_ELASTIC_ISOTROPIC)
-          isotropic_scattering(vr[0], vr[1], vr[2], d, dir);
-        else
-          diffuse_reflection(vr[0], vr[1], vr[2], d, dir);
-        ISK_GL(a.vx)[p] = (float)(w[0] + mr2 * (nvr * dir[0]));
-        ISK_GL(a.vy)[p] = (float)(w[1] + mr2 * (nvr * dir[1]));
-        ISK_GL(a.vz)[p] = (float)(w[2] + mr2 * (nvr * dir[2]));
-      } else if (T == ISKRA_COLL_EXCITATION) {
-        double sE = (0.5 * a.m_eV) * (vx * vx + vy * vy + vz * vz) - a.energy[k];
-        if (sE >= 0.0) {
-          double al = sqrt(2.0 / a.m_eV);
-          double ev = al * sqrt(sE);
+    for (int T = 0; T < MCC_NTYPE; ++T) {
+      const int nT = s_cnt[T];
+      for (int q0 = 0; q0 < nT; q0 += MCC_COLL) {
+        if (q0 + lane >= nT) continue;
+        const int rec = ISK_GL(a.cand_i)[b0 + s0 + s_list[T][q0 + lane]];
+        const int lt = rec & 0x7FF, k = (rec >> 11) & 0x1F, slot = (rec >> 16) - 1;
+        const long long p = b0 + lt;
+        Draws d;
+        d.id = ISK_GL(a.id)[p];
+        d.step = step;
+        d.stream = a.stream;
+        d.k0 = a.k0;
+        d.k1 = a.k1;
+        d.block = 1;
+        d.half = 0;
+        const double vx = ISK_GL(a.vx)[p], vy = ISK_GL(a.vy)[p], vz = ISK_GL(a.vz)[p];
+        if (T == ISKRA_COLL_ELASTIC_ISOTROPIC || T == ISKRA_COLL_ELASTIC_BACKWARD) {
+          const double sv[3] = {vx, vy, vz};
+          double mr1 = a.ms / (a.ms + a.mt), mr2 = a.mt / (a.ms + a.mt);
+          double tv[3];
+          randn3(d, tv);
+          for (int r = 0; r < 3; ++r) tv[r] *= a.vth_t;
+          double vr[3], wv[3];
+          for (int r = 0; r < 3; ++r) {
+            vr[r] = sv[r] - tv[r];
+            wv[r] = mr1 * sv[r] + mr2 * tv[r];
+          }
+          double nvr = sqrt(vr[0] * vr[0] + vr[1] * vr[1] + vr[2] * vr[2]);
           double dir[3];
-          isotropic_scattering(vx, vy, vz, d, dir);
-          ISK_GL(a.vx)[p] = (float)(ev * dir[0]);
-          ISK_GL(a.vy)[p] = (float)(ev * dir[1]);
-          ISK_GL(a.vz)[p] = (float)(ev * dir[2]);
-        }
-      } else {
-        const int slot = s_slot[q];
-        if (slot != 0xFFFF) { /* sE >= 0, decided in pass A */
+          if (T == ISKRA_COLL_ELASTIC_ISOTROPIC)
+            isotropic_scattering(vr[0], vr[1], vr[2], d, dir);
+          else
+            diffuse_reflection(vr[0], vr[1], vr[2], d, dir);
+          ISK_GL(a.vx)[p] = (float)(wv[0] + mr2 * (nvr * dir[0]));
+          ISK_GL(a.vy)[p] = (float)(wv[1] + mr2 * (nvr * dir[1]));
+          ISK_GL(a.vz)[p] = (float)(wv[2] + mr2 * (nvr * dir[2]));
+        } else if (T == ISKRA_COLL_EXCITATION) {
+          double sE = (0.5 * a.m_eV) * (vx * vx + vy * vy + vz * vz) - a.energy[k];
+          if (sE >= 0.0) {
+            double al = sqrt(2.0 / a.m_eV);
+            double ev = al * sqrt(sE);
+            double dir[3];
+            isotropic_scattering(vx, vy, vz, d, dir);
+            ISK_GL(a.vx)[p] = (float)(ev * dir[0]);
+            ISK_GL(a.vy)[p] = (float)(ev * dir[1]);
+            ISK_GL(a.vz)[p] = (float)(ev * dir[2]);
+          }
+        } else if (slot >= 0) { /* ionisation with sE >= 0, decided by k_mcc_cand */
           double sE = (0.5 * a.m_eV) * (vx * vx + vy * vy + vz * vz) - a.energy[k];
           double e1E = sE * d.next();
           double e2E = sE - e1E;
@@ -431,16 +442,12 @@ __device__ __forceinline__ void mcc_coll_body(const MccArgs &a, const int blk) {
         }
       }
     }
+    __syncthreads(); /* the next segment rewrites the lists */
   }
-  __syncthreads(); /* the next segment rewrites the lists */
-  } /* segments */
-  if ((int)threadIdx.x < a.ncoll && evl[threadIdx.x])
-    atomicAdd(&a.events[threadIdx.x], (unsigned long long)evl[threadIdx.x]);
-  if (threadIdx.x == 0) a.blk_cnt[blk] = nprod;
 }
 
 __global__ __launch_bounds__(MCC_BLOCK) void k_mcc_cand(MccArgs a) { mcc_cand_body(a, blockIdx.x); }
-__global__ __launch_bounds__(MCC_COLL) __attribute__((amdgpu_waves_per_eu(3, 8))) void k_mcc_coll(MccArgs a) { mcc_coll_body(a, blockIdx.x); }
+__global__ __launch_bounds__(MCC_COLL) void k_mcc_coll(MccArgs a) { mcc_coll_body(a, blockIdx.x); }
 
 /* exclusive scan of the block product counts (one work-group); cnt[nblk] = total */
 __device__ __forceinline__ void mcc_scan_body(int *cnt, int nblk) {
@@ -634,7 +641,6 @@ int isk_mcc_setup(iskra_ctx *c, int idx) {
   m.stage = nullptr;
   m.blk_cnt = nullptr;
   m.cand_i = nullptr;
-  m.cand_u = nullptr;
   m.cand_cnt = nullptr;
   return ISKRA_OK;
 }
@@ -669,10 +675,8 @@ static int mcc_prepare(iskra_ctx *c, int idx, double dt, int b_src, int b_ion, M
     isk_free(m.blk_cnt);
     isk_free(m.prod_flag);
     isk_free(m.cand_i);
-    isk_free(m.cand_u);
     isk_free(m.cand_cnt);
     m.cand_i = nullptr;
-    m.cand_u = nullptr;
     m.cand_cnt = nullptr;
     m.stage = nullptr;
     m.blk_cnt = nullptr;
@@ -681,7 +685,6 @@ static int mcc_prepare(iskra_ctx *c, int idx, double dt, int b_src, int b_ion, M
         isk_alloc((void **)&m.blk_cnt, (size_t)(nblk + 1) * 4) ||
         isk_alloc((void **)&m.prod_flag, (size_t)nblk * MCC_PPB * 4) ||
         isk_alloc((void **)&m.cand_i, (size_t)nblk * MCC_PPB * 4) ||
-        isk_alloc((void **)&m.cand_u, (size_t)nblk * MCC_PPB * 8) ||
         isk_alloc((void **)&m.cand_cnt, (size_t)nblk * 4))
       return ISKRA_E_HIP;
     m.stage_cap = cap;
@@ -727,7 +730,6 @@ static int mcc_prepare(iskra_ctx *c, int idx, double dt, int b_src, int b_ion, M
   a.stage = m.stage;
   a.stage_cell = m.prod_flag;
   a.cand_i = m.cand_i;
-  a.cand_u = m.cand_u;
   a.cand_cnt = m.cand_cnt;
   AppendArgs &p = *pp;
   p.blk_off = m.blk_cnt;
@@ -776,7 +778,7 @@ static int mcc_prepare(iskra_ctx *c, int idx, double dt, int b_src, int b_ion, M
   return ISKRA_OK;
 }
 
-/* dynamic LDS of the collision kernel: the cross-section tables when they fit */
+/* dynamic LDS of the candidate kernel: the cross-section tables when they fit */
 static size_t mcc_tab_shm(const IskMcc &m) {
   const size_t nt = m.tab_eps.size();
   return nt <= MCC_TAB_LDS ? nt * 2 * sizeof(double) : 0;
@@ -794,9 +796,9 @@ int isk_mcc_run(iskra_ctx *c, int idx, double dt, int64_t *events) {
   int rc = mcc_prepare(c, idx, dt, s.cur, b_ion, &a, &p, &cm, &nblk);
   if (rc) return rc;
   if (events) ISK_HIP(hipMemsetAsync(m.events, 0, ISK_MAX_COLL * 8, c->stream));
-  hipLaunchKernelGGL(k_mcc_cand, dim3(nblk), dim3(MCC_BLOCK), 0, c->stream, a);
+  hipLaunchKernelGGL(k_mcc_cand, dim3(nblk), dim3(MCC_BLOCK), mcc_tab_shm(m), c->stream, a);
   ISK_CHECK_LAUNCH();
-  hipLaunchKernelGGL(k_mcc_coll, dim3(nblk), dim3(MCC_COLL), mcc_tab_shm(m), c->stream, a);
+  hipLaunchKernelGGL(k_mcc_coll, dim3(nblk), dim3(MCC_COLL), 0, c->stream, a);
   ISK_CHECK_LAUNCH();
   hipLaunchKernelGGL(k_mcc_scan, dim3(1), dim3(1024), 0, c->stream, m.blk_cnt, nblk);
   ISK_CHECK_LAUNCH();
@@ -831,9 +833,8 @@ __global__ __launch_bounds__(MCC_BLOCK) void kb_mcc_cand(const MccArgs *A, const
   mcc_cand_body(A[i], blockIdx.x - bstart[i]);
 }
 
-/* at most 168 VGPRs: three waves per SIMD */
-__global__ __launch_bounds__(MCC_COLL) __attribute__((amdgpu_waves_per_eu(3, 8))) void kb_mcc_coll(
-    const MccArgs *A, const int *bstart, int n) {
+__global__ __launch_bounds__(MCC_COLL) void kb_mcc_coll(const MccArgs *A, const int *bstart,
+                                                        int n) {
   const int i = isk_bfind(bstart, n, blockIdx.x);
   mcc_coll_body(A[i], blockIdx.x - bstart[i]);
 }
@@ -874,10 +875,10 @@ int isk_bbuild_mcc(iskra_ctx **cs, int n, int m, double dt, const int *cur_src,
   }
   int rc;
   if ((rc = isk_blaunch_upload(&L[0], ISK_B_MCC, A.data(), sizeof(MccArgs), blocks, MCC_BLOCK,
-                               0)))
+                               tab_shm)))
     return rc;
   if ((rc = isk_blaunch_upload(&L[1], ISK_B_MCC_COLL, A.data(), sizeof(MccArgs), blocks,
-                               MCC_COLL, tab_shm)))
+                               MCC_COLL, 0)))
     return rc;
   if ((rc = isk_blaunch_upload(&L[2], ISK_B_MCC_SCAN, CM.data(), sizeof(CommitArgs), one, 1024,
                                0)))
@@ -893,10 +894,10 @@ int isk_blaunch_mcc(const IskBLaunch &L, hipStream_t st) {
   dim3 g((unsigned)L.total), b((unsigned)L.block);
   switch (L.kind) {
     case ISK_B_MCC:
-      hipLaunchKernelGGL(kb_mcc_cand, g, b, 0, st, (const MccArgs *)L.args, L.bstart, L.n);
+      hipLaunchKernelGGL(kb_mcc_cand, g, b, L.shm, st, (const MccArgs *)L.args, L.bstart, L.n);
       break;
     case ISK_B_MCC_COLL:
-      hipLaunchKernelGGL(kb_mcc_coll, g, b, L.shm, st, (const MccArgs *)L.args, L.bstart, L.n);
+      hipLaunchKernelGGL(kb_mcc_coll, g, b, 0, st, (const MccArgs *)L.args, L.bstart, L.n);
       break;
     case ISK_B_MCC_SCAN:
       hipLaunchKernelGGL(kb_mcc_scan, g, b, 0, st, (const CommitArgs *)L.args, L.bstart, L.n);
