@@ -96,8 +96,10 @@ def cpu_baseline(cells, budget_s=20.0, sample=1 << 22, max_steps=100):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=5)
+    # 64 timed steps: one full ion sort cycle (every 64 steps) and eight
+    # electron sort cycles (every 8), so the sorts enter at their average rate
+    ap.add_argument("--steps", type=int, default=64)
+    ap.add_argument("--warmup", type=int, default=8)
     ap.add_argument("--cells", type=int, default=256)
     ap.add_argument("--ppc", type=int, default=256)
     ap.add_argument("--solver", default="spectral", choices=["spectral", "mg", "mgpcg", "pcg"])
