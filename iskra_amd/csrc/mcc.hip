@@ -194,6 +194,14 @@ __device__ __forceinline__ void mcc_cand_body(const MccArgs &a, const int blk) {
   const unsigned long long below = (1ull << lane) - 1ull;
   const uint32_t step = a.step_dev ? (uint32_t)(a.step + *a.step_dev) : a.step;
   const long long b0 = (long long)blk * MCC_PPB;
+  /* blocks past the live particles (the grid covers the capacity) */
+  if (b0 >= np) {
+    if (threadIdx.x == 0) {
+      ISK_GL(a.cand_cnt)[blk] = 0;
+      a.blk_cnt[blk] = 0;
+    }
+    return;
+  }
   if ((int)threadIdx.x < ISK_MAX_COLL) evl[threadIdx.x] = 0u;
   const int ntab = a.toff[a.ncoll];
   const bool tab_lds = ntab <= MCC_TAB_LDS;
@@ -210,9 +218,9 @@ __device__ __forceinline__ void mcc_cand_body(const MccArgs &a, const int blk) {
 #pragma unroll
   for (int r = 0; r < MCC_RPT; ++r) {
     const long long p = b0 + r * MCC_BLOCK + threadIdx.x;
-    const long long pc = p < np ? p : (np > 0 ? np - 1 : 0);
-    cr[r] = np > 0 ? ISK_GL(a.cell)[pc] : -1;
-    idr[r] = np > 0 ? ISK_GL(a.id)[pc] : 0u;
+    const long long pc = p < np ? p : np - 1; /* np > b0 >= 0 here */
+    cr[r] = ISK_GL(a.cell)[pc];
+    idr[r] = ISK_GL(a.id)[pc];
   }
   int ncand = 0;
   for (int r = 0; r < MCC_RPT; ++r) {
