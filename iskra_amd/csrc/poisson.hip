@@ -309,8 +309,9 @@ __global__ void k_pcg_mean(PcgArgs a) {
 
 /* sums of 3 doubles over the work-group: an xor butterfly within each wave
  * (every lane ends with the same bits), then the waves in index order */
+template <int NT = RED_BLOCK>
 __device__ __forceinline__ void wg_sum3(double &a, double &b, double &c) {
-  __shared__ double w3[3][RED_BLOCK / 64];
+  __shared__ double w3[3][NT / 64];
   for (int off = 32; off > 0; off >>= 1) {
     a += __shfl_xor(a, off);
     b += __shfl_xor(b, off);
@@ -325,7 +326,7 @@ __device__ __forceinline__ void wg_sum3(double &a, double &b, double &c) {
   }
   __syncthreads();
   double sa = 0.0, sb = 0.0, sc = 0.0;
-  for (int k = 0; k < RED_BLOCK / 64; ++k) {
+  for (int k = 0; k < NT / 64; ++k) {
     sa += w3[0][k];
     sb += w3[1][k];
     sc += w3[2][k];
@@ -904,7 +905,44 @@ struct MgCoarseArgs {
   int nx[12], ny[12], off[12];
   int total;
   const int *flag;
+  /* decide != 0: this kernel takes the cycle's convergence verdict */
+  int decide, nparts, dfirst, max_iter;
+  const double *partials;
+  double *scal;
+  int *iflag;
 };
+
+/* defect-correction multigrid: the convergence verdict on the FP64 residual
+ * of x, taken by the kernel after the one that wrote the per-work-group
+ * (r,r), (f,f) partials, in every work-group alike (the same sums in the
+ * same order give the same bits); work-group 0 records it.  Returns true
+ * when the cycle stops here.  The iteration limit is applied by work-group 0
+ * alone through the flag, which the later kernels of the cycle read. */
+template <int NT>
+__device__ bool mg_verdict(const double *partials, int nparts, int first, double *scal,
+                           int *iflag, int max_iter) {
+  double rr = 0.0, ff = 0.0, d = 0.0;
+  for (int k = threadIdx.x; k < nparts; k += NT) {
+    rr += partials[2 * k];
+    ff += partials[2 * k + 1];
+  }
+  wg_sum3<NT>(rr, ff, d);
+  const double sff = first ? ff : scal[S_FF];
+  const double tol2 = first ? scal[S_RTOL2] * ff : scal[S_TOL2];
+  const bool done = rr <= tol2;
+  if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
+    if (first) {
+      scal[S_FF] = ff;
+      scal[S_TOL2] = tol2;
+    }
+    scal[S_RR] = rr;
+    scal[S_RELRES] = sff > 0.0 ? sqrt(rr / sff) : 0.0;
+    const int it = iflag[1];
+    if (done || it >= max_iter) iflag[0] = 1;
+    else iflag[1] = it + 1;
+  }
+  return done;
+}
 
 /* threads as a 32 x 32 grid; interior nodes of one colour, no divisions */
 __device__ __forceinline__ void lds_rbgs(float *e, const float *f, int nx, int ny, int color) {
@@ -921,7 +959,11 @@ __device__ __forceinline__ void lds_rbgs(float *e, const float *f, int nx, int n
 }
 
 __global__ __launch_bounds__(1024) void k_mg_coarse(MgCoarseArgs a) {
-  if (a.flag[0]) return;
+  if (a.decide) {
+    if (mg_verdict<1024>(a.partials, a.nparts, a.dfirst, a.scal, a.iflag, a.max_iter)) return;
+  } else if (a.flag[0]) {
+    return;
+  }
   extern __shared__ float sm[];
   float *E = sm;
   float *Fv = sm + a.total;
@@ -1028,6 +1070,9 @@ struct MgFused {
   unsigned int *counter;
   int *iflag;
   int first, max_iter;
+  /* decide != 0: this kernel takes the cycle's convergence verdict from the
+   * nparts partials of the finest residual (dfirst: first cycle of the solve) */
+  int decide, nparts, dfirst;
 };
 
 __device__ __forceinline__ void tile_sweep(float *se, const float *sf, int ox0, int oy0, int nx,
@@ -1046,7 +1091,11 @@ __device__ __forceinline__ void tile_sweep(float *se, const float *sf, int ox0, 
 
 /* pre-smooth from e = 0, restrict the residual, write owned e (and f) */
 __global__ __launch_bounds__(256) void k_mgf_down(MgFused a) {
-  if (!a.first && a.flag[0]) return;
+  if (a.decide) {
+    if (mg_verdict<256>(a.partials, a.nparts, a.dfirst, a.scal, a.iflag, a.max_iter)) return;
+  } else if (!a.first && a.flag[0]) {
+    return;
+  }
   __shared__ float se[MGT_S * MGT_S], sf[MGT_S * MGT_S];
   const int nx = a.F.nx, ny = a.F.ny;
   const int I0 = blockIdx.x * MGT_CT, J0 = blockIdx.y * MGT_CT;
@@ -1118,16 +1167,13 @@ __global__ __launch_bounds__(256) void k_mgf_down(MgFused a) {
     if (a.mode_in == 1) a.F.f[m] = sf[k];
   }
   if (a.mode_in == 1) {
-    double srr, sff;
-    if (block_reduce2(rr, ff, a.partials, a.counter, &srr, &sff) && threadIdx.x == 0) {
-      if (a.first) {
-        a.scal[S_FF] = sff;
-        a.scal[S_TOL2] = a.scal[S_TOL2] * sff;
-      }
-      a.scal[S_RR] = srr;
-      a.scal[S_RELRES] = a.scal[S_FF] > 0.0 ? sqrt(srr / a.scal[S_FF]) : 0.0;
-      if (srr <= a.scal[S_TOL2] || a.iflag[1] >= a.max_iter) a.iflag[0] = 1;
-      else a.iflag[1] += 1;
+    /* partials of the residual of x; the next kernel takes the verdict */
+    double d = 0.0;
+    wg_sum3<256>(rr, ff, d);
+    if (threadIdx.x == 0) {
+      const int bid = blockIdx.x + gridDim.x * blockIdx.y;
+      a.partials[2 * bid] = rr;
+      a.partials[2 * bid + 1] = ff;
     }
   }
 }
@@ -1180,7 +1226,7 @@ __global__ __launch_bounds__(256) void k_mgf_up(MgFused a) {
 }
 
 /* FP64 defect-correction residual: f0 = r = f_elim - L x, e0 = 0 */
-__global__ void k_mg_resid0(PcgArgs a, float *e0, int first) {
+__global__ __launch_bounds__(RED_BLOCK) void k_mg_resid0(PcgArgs a, float *e0, int first) {
   if (!first && a.iflag[0]) return;
   int n = blockIdx.x * blockDim.x + threadIdx.x;
   int nn = a.s.nx * a.s.ny;
@@ -1199,16 +1245,12 @@ __global__ void k_mg_resid0(PcgArgs a, float *e0, int first) {
       a.f0[n] = 0.0f;
     }
   }
-  double srr, sff;
-  if (block_reduce2(rr, ff, a.partials, a.counter, &srr, &sff) && threadIdx.x == 0) {
-    if (first) {
-      a.scal[S_FF] = sff;
-      a.scal[S_TOL2] = a.scal[S_TOL2] * sff;
-    }
-    a.scal[S_RR] = srr;
-    a.scal[S_RELRES] = a.scal[S_FF] > 0.0 ? sqrt(srr / a.scal[S_FF]) : 0.0;
-    if (srr <= a.scal[S_TOL2] || a.iflag[1] >= a.max_iter) a.iflag[0] = 1;
-    else a.iflag[1] += 1;
+  /* partials of the residual of x; the next kernel takes the verdict */
+  double d = 0.0;
+  wg_sum3<RED_BLOCK>(rr, ff, d);
+  if (threadIdx.x == 0) {
+    a.partials[2 * blockIdx.x] = rr;
+    a.partials[2 * blockIdx.x + 1] = ff;
   }
 }
 
@@ -1547,15 +1589,25 @@ static int mg_vcycle(iskra_ctx *c, int mode_in0, int mode_out0, int first) {
   const int nn0 = (p.ln[0] & 0xFFFF) * (p.ln[0] >> 16);
   const int nblk0 = (nn0 + RED_BLOCK - 1) / RED_BLOCK;
   PcgArgs pa = make_pcg(c, 0);
+  /* with mode_in0 = 1 the finest residual kernel writes per-work-group
+   * partials and the kernel after it takes the convergence verdict: the
+   * second down kernel, or the single-work-group kernel */
+  int nparts = 0;
+  a.decide = 0;
+  a.nparts = 0;
+  a.dfirst = first;
   if (p.coarse_lev == 0 && mode_in0 == 1) {
     /* the whole grid is solved by the single work-group kernel: FP64 residual here */
     hipLaunchKernelGGL(k_mg_resid0, dim3(nblk0), dim3(RED_BLOCK), 0, c->stream, pa, p.me[0], first);
+    nparts = nblk0;
   }
   for (int l = 0; l < p.coarse_lev; ++l) {
     a.F = lev(l);
     a.C = lev(l + 1);
     a.mode_in = l == 0 ? mode_in0 : 0;
     a.first = l == 0 ? first : 0;
+    a.decide = (l == 1 && nparts > 0) ? 1 : 0;
+    a.nparts = nparts;
     dim3 g((a.C.nx + MGT_CT - 1) / MGT_CT, (a.C.ny + MGT_CT - 1) / MGT_CT);
     if (a.mode_in == 1 && (int)(g.x * g.y) > p.partials_blocks) {
       isk_set_error("poisson: reduction scratch holds %d blocks, kernel needs %d",
@@ -1563,6 +1615,7 @@ static int mg_vcycle(iskra_ctx *c, int mode_in0, int mode_out0, int first) {
       return ISKRA_E_STATE;
     }
     hipLaunchKernelGGL(k_mgf_down, g, dim3(256), 0, c->stream, a);
+    if (a.mode_in == 1) nparts = (int)(g.x * g.y);
   }
   {
     MgCoarseArgs ca;
@@ -1579,6 +1632,14 @@ static int mg_vcycle(iskra_ctx *c, int mode_in0, int mode_out0, int first) {
     ca.e_out = p.me[p.coarse_lev];
     ca.f_in = p.mf[p.coarse_lev];
     ca.flag = flag;
+    /* the verdict here when no second down kernel took it */
+    ca.decide = (nparts > 0 && p.coarse_lev <= 1) ? 1 : 0;
+    ca.nparts = nparts;
+    ca.dfirst = first;
+    ca.max_iter = p.max_iter;
+    ca.partials = p.partials;
+    ca.scal = p.scal;
+    ca.iflag = p.iflag;
     hipLaunchKernelGGL(k_mg_coarse, dim3(1), dim3(1024), (size_t)tot * 2 * 4, c->stream, ca);
   }
   for (int l = p.coarse_lev - 1; l >= 0; --l) {
