@@ -145,7 +145,6 @@ struct IskPoisson {
   double *partials = nullptr;  /* per-block partial sums (2 per block) */
   double *cgp = nullptr;       /* CG per-work-group partials (7 per work-group) */
   int partials_blocks = 0;     /* blocks the partials array can hold */
-  unsigned int *counter = nullptr;
   int *iflag = nullptr;        /* [0] done, [1] iterations */
   /* multigrid */
   int nlev = 0;

@@ -15,8 +15,9 @@
  *  DIRECT  1-D tridiagonal elimination (the reference's A\b is a direct solve),
  *          FP64, factor built once on the host, substitution on the device.
  *  PCG     Jacobi-preconditioned conjugate gradients: x, r, Ap and all inner
- *          products in FP64, search direction p and z in FP32; reductions are
- *          block partial sums combined in a fixed order by the last block.
+ *          products in FP64, search direction p and z in FP32; a dot product
+ *          is per-work-group partials, summed in block order by the kernel
+ *          that needs it (every work-group of it gets the same bits).
  *  MG      geometric multigrid V(2,2) cycles with red-black Gauss-Seidel on FP32
  *          levels as a defect correction of the FP64 iterate (residual, norm
  *          and update in FP64); levels <= 65^2 run in one work-group from LDS.
@@ -95,55 +96,6 @@ __device__ __forceinline__ void efield_node(const double *phi, float *E, int dim
 }
 
 
-/* block sum of up to 2 doubles; the last block to finish reduces the per-block
- * partials in index order and returns true (deterministic). */
-__device__ bool block_reduce2(double a, double b, double *partials,
-                              unsigned int *counter, double *out_a, double *out_b) {
-  __shared__ double ra[RED_BLOCK], rb[RED_BLOCK];
-  __shared__ bool last;
-  ra[threadIdx.x] = a;
-  rb[threadIdx.x] = b;
-  __syncthreads();
-  for (int off = RED_BLOCK / 2; off > 0; off >>= 1) {
-    if (threadIdx.x < off) {
-      ra[threadIdx.x] += ra[threadIdx.x + off];
-      rb[threadIdx.x] += rb[threadIdx.x + off];
-    }
-    __syncthreads();
-  }
-  /* linear block id over a 1-D or 2-D grid */
-  const int bid = blockIdx.x + gridDim.x * blockIdx.y;
-  const int nb = gridDim.x * gridDim.y;
-  if (threadIdx.x == 0) {
-    partials[2 * bid] = ra[0];
-    partials[2 * bid + 1] = rb[0];
-    __threadfence();
-    unsigned int t = atomicAdd(counter, 1u);
-    last = (t == (unsigned int)nb - 1u);
-  }
-  __syncthreads();
-  if (!last) return false;
-  __threadfence();
-  double sa = 0.0, sb = 0.0;
-  for (int k = threadIdx.x; k < nb; k += RED_BLOCK) {
-    sa += __hip_atomic_load(&partials[2 * k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    sb += __hip_atomic_load(&partials[2 * k + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
-  ra[threadIdx.x] = sa;
-  rb[threadIdx.x] = sb;
-  __syncthreads();
-  for (int off = RED_BLOCK / 2; off > 0; off >>= 1) {
-    if (threadIdx.x < off) {
-      ra[threadIdx.x] += ra[threadIdx.x + off];
-      rb[threadIdx.x] += rb[threadIdx.x + off];
-    }
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) *counter = 0u;
-  *out_a = ra[0];
-  *out_b = rb[0];
-  return true;
-}
 
 /* ------------------------------------------------------------------------ */
 /* 1-D direct solve                                                          */
@@ -260,7 +212,6 @@ struct PcgArgs {
   float *f0;  /* MG preconditioner input (MGPCG) */
   float *e0;  /* MG finest-level correction, zeroed before each V-cycle */
   double *scal, *partials;
-  unsigned int *counter;
   int *iflag;
   double *cgp; /* CG partials, 7 per work-group */
   int nblk;    /* work-groups of the CG kernels */
@@ -285,16 +236,6 @@ __global__ void k_pcg_init(double *scal, int *iflag, double tol2, int singular) 
   if (threadIdx.x < S_WORDS)
     scal[threadIdx.x] = (threadIdx.x == S_TOL2 || threadIdx.x == S_RTOL2) ? tol2 : 0.0;
   if (threadIdx.x < 4) iflag[threadIdx.x] = threadIdx.x == 2 ? singular : 0;
-}
-
-/* sum of the eliminated rhs (for the singular projection) */
-__global__ void k_pcg_mean(PcgArgs a) {
-  int n = blockIdx.x * blockDim.x + threadIdx.x;
-  double v = 0.0;
-  if (n < a.s.nx * a.s.ny) v = a.rho[n] * a.scale;
-  double s, dummy;
-  if (block_reduce2(v, 0.0, a.partials, a.counter, &s, &dummy) && threadIdx.x == 0)
-    a.scal[S_MEAN] = s / (double)(a.s.nx * a.s.ny);
 }
 
 /* CG without device-wide reductions: a kernel that needs a dot product sums
@@ -351,17 +292,37 @@ __device__ __forceinline__ void cg_put3(double *slot, double x, double y, double
   }
 }
 
+/* singular systems: per-work-group partial sums of the right-hand side (the
+ * residual kernel that follows sums them for the zero-mean projection) */
+__global__ __launch_bounds__(RED_BLOCK) void k_pcg_mean(PcgArgs a) {
+  int n = blockIdx.x * blockDim.x + threadIdx.x;
+  double v = 0.0, d1 = 0.0, d2 = 0.0;
+  if (n < a.s.nx * a.s.ny) v = a.rho[n] * a.scale;
+  wg_sum3(v, d1, d2);
+  if (threadIdx.x == 0) a.partials[blockIdx.x] = v;
+}
+
 /* r = f_elim - L x at free nodes; partials of (r,r), (r,z) (Jacobi z = r/diag;
  * the MG preconditioner's z comes later) and (f,f) into slot 0 */
 __global__ __launch_bounds__(RED_BLOCK) void k_pcg_resid(PcgArgs a) {
   int n = blockIdx.x * blockDim.x + threadIdx.x;
   int nn = a.s.nx * a.s.ny;
   double rr = 0.0, rz = 0.0, ff = 0.0;
+  /* singular system (flag set by an earlier kernel, the same for every
+   * thread): the mean of the right-hand side from k_pcg_mean's partials */
+  double mean = 0.0;
+  if (a.iflag[2]) {
+    double d1 = 0.0, d2 = 0.0;
+    for (int k = threadIdx.x; k < a.nblk; k += RED_BLOCK) mean += a.partials[k];
+    wg_sum3(mean, d1, d2);
+    mean /= (double)nn;
+    if (blockIdx.x == 0 && threadIdx.x == 0) a.scal[S_MEAN] = mean;
+  }
   if (n < nn) {
     int i = n % a.s.nx, j = n / a.s.nx;
     if (a.s.tags[n] == 0) {
       double f = a.rho[n] * a.scale;
-      if (a.iflag[2]) f -= a.scal[S_MEAN];
+      if (a.iflag[2]) f -= mean;
       double diag;
       /* x holds the Dirichlet values, so L x already carries them */
       double Lx = apply_L(a.s, a.x, i, j, &diag);
@@ -1067,7 +1028,6 @@ struct MgFused {
   const double *rho;
   double scale;
   double *scal, *partials;
-  unsigned int *counter;
   int *iflag;
   int first, max_iter;
   /* decide != 0: this kernel takes the cycle's convergence verdict from the
@@ -1304,7 +1264,7 @@ int isk_poisson_free(iskra_ctx *c) {
   IskPoisson &p = c->ps;
   void *ptrs[] = {p.d_tags, p.d_tag_val, p.d_tri_a, p.d_tri_cp, p.d_tri_inv,
                   p.d_work, p.x, p.r, p.Ap, p.z, p.p[0], p.p[1], p.scal,
-                  p.partials, p.cgp, p.counter, p.iflag, p.sp_Sx, p.sp_Sy, p.sp_Li,
+                  p.partials, p.cgp, p.iflag, p.sp_Sx, p.sp_Sy, p.sp_Li,
                   p.sp_F, p.sp_T1, p.sp_T2};
   for (void *q : ptrs) isk_free(q);
   p.sp_Sx = p.sp_Sy = p.sp_Li = p.sp_F = p.sp_T1 = p.sp_T2 = nullptr;
@@ -1319,7 +1279,6 @@ int isk_poisson_free(iskra_ctx *c) {
   p.x = p.r = p.Ap = nullptr;
   p.z = p.p[0] = p.p[1] = nullptr;
   p.scal = p.partials = p.cgp = nullptr;
-  p.counter = nullptr;
   p.iflag = nullptr;
   p.nlev = 0;
   return ISKRA_OK;
@@ -1429,9 +1388,8 @@ int isk_poisson_setup(iskra_ctx *c) {
       isk_alloc((void **)&p.scal, S_WORDS * 8) ||
       isk_alloc((void **)&p.partials, (size_t)p.partials_blocks * 2 * 8) ||
       isk_alloc((void **)&p.cgp, (size_t)nblk * 7 * 8) ||
-      isk_alloc((void **)&p.counter, 64) || isk_alloc((void **)&p.iflag, 64))
+      isk_alloc((void **)&p.iflag, 64))
     return ISKRA_E_HIP;
-  ISK_HIP(hipMemset(p.counter, 0, 64));
   ISK_HIP(hipMemset(p.iflag, 0, 64));
   p.nlev = 0;
   if (p.kind == ISKRA_POISSON_SPECTRAL) {
@@ -1550,7 +1508,6 @@ static PcgArgs make_pcg(iskra_ctx *c, int mgpre) {
   a.e0 = p.nlev ? p.me[0] : nullptr;
   a.scal = p.scal;
   a.partials = p.partials;
-  a.counter = p.counter;
   a.iflag = p.iflag;
   a.scale = (c->g.dx * c->g.dx) / p.eps0;
   a.warm = 0;
@@ -1583,7 +1540,6 @@ static int mg_vcycle(iskra_ctx *c, int mode_in0, int mode_out0, int first) {
   a.scale = (c->g.dx * c->g.dx) / p.eps0;
   a.scal = p.scal;
   a.partials = p.partials;
-  a.counter = p.counter;
   a.iflag = p.iflag;
   a.max_iter = p.max_iter;
   const int nn0 = (p.ln[0] & 0xFFFF) * (p.ln[0] >> 16);

@@ -173,3 +173,34 @@ def test_ionisation_beyond_capacity(gpu_ready):
     st = ctx.get_state(e)
     assert len(st["cell"]) == cap and np.all((st["cell"] >= 0) & (st["cell"] < 64))
     ctx.close()
+
+
+@pytest.mark.parametrize("nx,ny", [(17, 17), (33, 9)])
+def test_singular_periodic_pcg(gpu_ready, orc, nx, ny):
+    """A fully periodic 2-D system has no Dirichlet node and is singular
+    (generalized_poisson.jl:202; the reference's A\\b fixes the constant by
+    round-off).  The solve projects the right-hand side to zero mean and
+    returns the mean-free solution: it must equal the minimum-norm
+    least-squares solution of the reference matrix (assembled with its
+    periodic-ring quirks) for the projected density, which is mean-free
+    because the constants span the null space."""
+    from iskra_amd import Context, _lib as L
+    rng = np.random.default_rng(nx * ny)
+    dx = dy = 0.01
+    eps0 = 8.854e-12
+    ctx = Context()
+    ctx.grid(2, (nx, ny), (dx, dy))
+    ctx.poisson(L.POISSON_PCG, eps0, None, periodic_x=True, periodic_y=True, rtol=1e-10)
+    rho = rng.standard_normal(nx * ny) * 1e-9 + 3e-10  # nonzero mean on purpose
+    ctx.set_field(L.F_RHO, rho)
+    it, rr = ctx.solve()
+    assert rr <= 1e-10
+    phi = ctx.field(L.F_PHI)
+    A, b, isdof = orc.poisson_matrix(2, nx, ny, dx, dy, periodic=[1, 2])
+    assert np.all(isdof == 1)
+    bb = -(rho - rho.mean()) / eps0
+    x = np.linalg.lstsq(A.reshape(nx * ny, nx * ny), bb, rcond=None)[0]
+    assert abs(x.mean()) <= 1e-10 * np.abs(x).max()
+    assert abs(phi.mean()) <= 1e-9 * np.abs(x).max()
+    assert np.max(np.abs(phi - x)) <= 1e-7 * np.abs(x).max()
+    ctx.close()
