@@ -1,4 +1,4 @@
-"""Learned warm start for the 2-D Poisson solve: host driver.
+"""Learned warm start for the 2-D Poisson solve: host side.
 
 The network, its training and its inference run in libiskra_hip.so
 (iskra_ws_*, hand-written BF16 MFMA kernels); this module initialises
