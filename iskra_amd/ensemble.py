@@ -37,9 +37,11 @@ def draw_instances(count=256, seed=0, cells=(128, 1024), particles=(1e4, 5e5),
     return out
 
 
-def ccp_context(inst, tile=0, chunk=0, sort_every=8, gap=0.025, freq=13.56e6, Te_eV=2.5,
+def ccp_context(inst, tile=0, chunk=0, sort_every=16, gap=0.025, freq=13.56e6, Te_eV=2.5,
                 Tg=300.0, n0=1e15, cap_factor=2.0, device=0):
-    """One instance as an ordinary context.  Returns (ctx, dt)."""
+    """One instance as an ordinary context.  Returns (ctx, dt).  Both species
+    are re-sorted every `sort_every` steps (16: 3 % faster per batched step
+    than 8 on the 256-member ensemble, the same as 32)."""
     cells = inst["cells"]
     npart = inst["particles"]
     p = P.ccp_1d(cells=cells, gap=gap, electrons=npart, ions=npart, Te=Te_eV,
