@@ -467,7 +467,7 @@ __device__ __forceinline__ void mcc_scan_body(int *cnt, int nblk) {
   sh[threadIdx.x] = sum;
   __syncthreads();
   for (int o = 1; o < 1024; o <<= 1) {
-    int v = threadIdx.x >= o ? sh[threadIdx.x - o] : 0;
+    int v = (int)threadIdx.x >= o ? sh[threadIdx.x - o] : 0;
     __syncthreads();
     sh[threadIdx.x] += v;
     __syncthreads();

@@ -388,7 +388,7 @@ __device__ void sort_binscan_body(const int *tot, int *start, int nb1) {
   sh[threadIdx.x] = sum;
   __syncthreads();
   for (int off = 1; off < 1024; off <<= 1) {
-    int v = threadIdx.x >= off ? sh[threadIdx.x - off] : 0;
+    int v = (int)threadIdx.x >= off ? sh[threadIdx.x - off] : 0;
     __syncthreads();
     sh[threadIdx.x] += v;
     __syncthreads();
@@ -548,7 +548,7 @@ __device__ void sort_finish_body(long long *hdr, const int *tot, const int *star
   sh[threadIdx.x] = sum;
   __syncthreads();
   for (int off = 1; off < 1024; off <<= 1) {
-    int v = threadIdx.x >= off ? sh[threadIdx.x - off] : 0;
+    int v = (int)threadIdx.x >= off ? sh[threadIdx.x - off] : 0;
     __syncthreads();
     sh[threadIdx.x] += v;
     __syncthreads();

@@ -944,7 +944,6 @@ __global__ __launch_bounds__(1024) void k_mg_coarse(MgCoarseArgs a) {
     lds_rbgs(e, f, nx, ny, 1);
     lds_rbgs(e, f, nx, ny, 0);
     lds_rbgs(e, f, nx, ny, 1);
-    float *ec = E + a.off[l + 1];
     float *fc = Fv + a.off[l + 1];
     int cx = a.nx[l + 1], cy = a.ny[l + 1];
     for (int J = 1 + ty; J < cy - 1; J += 32)
