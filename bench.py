@@ -113,6 +113,8 @@ def main():
     # ions at 0.03 eV move ~3e-4 cells per step here: they stay tile-ordered
     # for hundreds of steps, so they are re-sorted far less often
     ap.add_argument("--ion-sort-every", type=int, default=64)
+    # and their deposit windows need no margin (-1: the electrons' margin)
+    ap.add_argument("--ion-margin", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--no-warmstart", action="store_true")
@@ -147,6 +149,7 @@ def main():
     ctx.boundary(e, [L.BND_DISCARD, L.BND_DISCARD])
     ctx.boundary(i, [L.BND_DISCARD, L.BND_DISCARD])
     ctx.species_sort_every(i, args.ion_sort_every)
+    ctx.species_margin(i, args.ion_margin)
     Lx = p["L"]
     ctx.sample_maxwellian(e, n_species, [Lx, Lx], [p["ve"]] * 3, seed=1, stream=0)
     ctx.sample_maxwellian(i, n_species, [Lx, Lx], [p["vi"]] * 3, seed=1, stream=1)
@@ -190,7 +193,8 @@ def main():
             "cells": args.cells,
             "particles_per_species": n_species,
             "poisson": args.solver,
-            "tuning": dict(ctx.get_tuning(), ion_sort_every=args.ion_sort_every),
+            "tuning": dict(ctx.get_tuning(), ion_sort_every=args.ion_sort_every,
+                           ion_margin=args.ion_margin),
         },
         "poisson": {"solver": args.solver, "iters_per_solve": iters / max(solves, 1),
                     "rtol": 1e-8},

@@ -82,6 +82,12 @@ int iskra_tuning_get(iskra_ctx *ctx, iskra_tuning *t);
  * device-side particle order is this library's (the reference keeps its
  * particles unsorted).  Batches need every species on the context value. */
 int iskra_species_sort_every(iskra_ctx *ctx, int sid, int32_t every);
+/* Deposit-window margin of one species in cells (-1: the context's margin;
+ * values above it are clipped to it).  A chunk's window spans its tile plus
+ * the margin, so a species that moves well under a cell between sorts
+ * (cold ions) needs margin 0 and its push and merge move less window data.
+ * No reference counterpart (the windows are this library's). */
+int iskra_species_margin(iskra_ctx *ctx, int sid, int32_t margin);
 
 /* ---- species ------------------------------------------------------------ */
 /* KineticSpecies{D,3} + create_kinetic_species

@@ -24,7 +24,8 @@ COLL_ELASTIC_ISOTROPIC, COLL_ELASTIC_BACKWARD, COLL_EXCITATION, COLL_IONIZATION 
 EXPORTS = [
     "iskra_last_error", "iskra_version", "iskra_ctx_create", "iskra_ctx_destroy", "iskra_sync",
     "iskra_grid_set", "iskra_tuning_set", "iskra_tuning_get", "iskra_species_add",
-    "iskra_species_set_boundary", "iskra_species_sort_every", "iskra_species_upload",
+    "iskra_species_set_boundary", "iskra_species_sort_every", "iskra_species_margin",
+    "iskra_species_upload",
     "iskra_species_download",
     "iskra_species_count", "iskra_species_get_state", "iskra_species_set_state",
     "iskra_species_sample_maxwellian", "iskra_sort", "iskra_gather",
@@ -89,6 +90,7 @@ def lib():
             "iskra_species_add": (I, [P, D, D, D, I64, I, C.POINTER(I)]),
             "iskra_species_set_boundary": (I, [P, I, P]),
             "iskra_species_sort_every": (I, [P, I, I]),
+            "iskra_species_margin": (I, [P, I, I]),
             "iskra_species_upload": (I, [P, I, I64, P, P, P]),
             "iskra_species_download": (I, [P, I, I64, C.POINTER(I64), P, P, P]),
             "iskra_species_count": (I, [P, I, C.POINTER(I64)]),
@@ -243,6 +245,10 @@ class Context:
     def species_sort_every(self, sid, every):
         """Steps between this species' sorts in step() (0: the context's)."""
         check(lib().iskra_species_sort_every(self.h, sid, int(every)))
+
+    def species_margin(self, sid, margin):
+        """Deposit-window margin of one species in cells (-1: the context's)."""
+        check(lib().iskra_species_margin(self.h, sid, int(margin)))
 
     def boundary(self, sid, modes):
         md = np.asarray(list(modes) + [BND_WRAP] * (2 - len(modes)), dtype=np.int32)

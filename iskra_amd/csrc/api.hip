@@ -507,6 +507,7 @@ int iskra_species_add(iskra_ctx *c, double q, double m, double w0, int64_t capac
   s->cap = capacity;
   s->track_ids = track_ids ? 1 : 0;
   s->bnd[0] = s->bnd[1] = ISKRA_BND_WRAP;
+  s->margin = -1;
   s->cur = 0;
   int rc = isk_species_alloc(c, s);
   if (rc) {
@@ -529,6 +530,17 @@ int iskra_species_set_boundary(iskra_ctx *c, int sid, const int32_t mode[2]) {
     }
     c->sp[sid].bnd[d] = mode[d];
   }
+  return ISKRA_OK;
+}
+
+int iskra_species_margin(iskra_ctx *c, int sid, int32_t margin) {
+  CHECK_CTX(c);
+  if (!isk_species_ok(c, sid)) return ISKRA_E_ARG;
+  if (margin < -1) {
+    isk_set_error("species_margin: %d < -1", margin);
+    return ISKRA_E_ARG;
+  }
+  c->sp[sid].margin = margin;
   return ISKRA_OK;
 }
 

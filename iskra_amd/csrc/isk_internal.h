@@ -75,6 +75,7 @@ struct IskSpecies {
   int64_t cap;
   int track_ids;
   int sort_every; /* 0: the context's */
+  int margin;     /* window margin in cells, -1: the context's (never above it) */
   int bnd[2];
   int cur;
   int32_t *cell[2];
@@ -254,6 +255,14 @@ __host__ __device__ inline float isk_u24(uint32_t a) {
 int isk_alloc(void **p, size_t bytes);
 int isk_free(void *p);
 int isk_nodes(const iskra_ctx *c);
+/* deposit-window margin of a species and the window width T + 2M + 1 */
+inline int isk_sp_margin(const iskra_ctx *c, int sid) {
+  const int m = c->sp[sid].margin;
+  return (m >= 0 && m < c->t.M) ? m : c->t.M;
+}
+inline int isk_sp_window(const iskra_ctx *c, int sid) {
+  return c->t.T + 2 * isk_sp_margin(c, sid) + 1;
+}
 int isk_species_ok(iskra_ctx *c, int sid);
 
 /* ---- kernels exposed across translation units ---- */

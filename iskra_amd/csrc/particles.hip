@@ -1030,8 +1030,8 @@ static void push_prepare(iskra_ctx *c, int sid, double dt, int do_push, int b, P
   a.ncy = g.ncy;
   a.T = c->t.T;
   a.log2T = c->t.log2T;
-  a.M = c->t.M;
-  a.W = c->t.W;
+  a.M = isk_sp_margin(c, sid);
+  a.W = isk_sp_window(c, sid);
   a.CH = c->t.CH;
   a.ntx = c->ntx;
   a.qmdt = (float)(s->q / s->m * dt);
@@ -1269,8 +1269,8 @@ static void merge_prepare(iskra_ctx *c, int sid, int count_live, MergeArgs *pa, 
   a.nx = c->g.nx;
   a.ny = c->g.ny;
   a.T = c->t.T;
-  a.M = c->t.M;
-  a.W = c->t.W;
+  a.M = isk_sp_margin(c, sid);
+  a.W = isk_sp_window(c, sid);
   a.ntx = c->ntx;
   a.nty = c->nty;
   a.tilewin = nullptr;
@@ -1285,7 +1285,7 @@ int isk_merge_species(iskra_ctx *c, int sid, int count_live) {
     IskSpecies *s = &c->sp[sid];
     hipLaunchKernelGGL(k_merge_tiles, dim3(c->ntiles), dim3(256), 0, c->stream,
                        (const long long *)s->partial, (const int *)s->tile_chunk_begin,
-                       (const long long *)s->hdr, s->tilewin, c->t.W * c->t.W);
+                       (const long long *)s->hdr, s->tilewin, isk_sp_window(c, sid) * isk_sp_window(c, sid));
     ISK_CHECK_LAUNCH();
     a.tilewin = s->tilewin;
     hipLaunchKernelGGL(k_merge<2>, dim3(grid), dim3(256), 0, c->stream, a);
@@ -1334,12 +1334,12 @@ struct MergeTilesN {
   const int *tcb[ISK_MAX_SPECIES];
   const long long *hdr[ISK_MAX_SPECIES];
   long long *tilewin[ISK_MAX_SPECIES];
-  int WW;
+  int WW[ISK_MAX_SPECIES]; /* window nodes of each species */
 };
 
 __global__ __launch_bounds__(256) void k_merge_tiles_n(MergeTilesN a) {
   const int s = blockIdx.y;
-  merge_tile_body(a.partial[s], a.tcb[s], a.hdr[s], a.tilewin[s], a.WW, blockIdx.x);
+  merge_tile_body(a.partial[s], a.tcb[s], a.hdr[s], a.tilewin[s], a.WW[s], blockIdx.x);
 }
 
 struct MergeRhoArgs {
@@ -1422,8 +1422,8 @@ int isk_push_deposit_all(iskra_ctx *c, double dt) {
       t.tcb[s] = c->sp[s].tile_chunk_begin;
       t.hdr[s] = c->sp[s].hdr;
       t.tilewin[s] = c->sp[s].tilewin;
+      t.WW[s] = isk_sp_window(c, s) * isk_sp_window(c, s);
     }
-    t.WW = c->t.W * c->t.W;
     hipLaunchKernelGGL(k_merge_tiles_n, dim3(c->ntiles, c->nspecies), dim3(256), 0, c->stream,
                        t);
     ISK_CHECK_LAUNCH();
