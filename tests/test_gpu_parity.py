@@ -429,14 +429,15 @@ def test_poisson_spectral_correction_pass(gpu_ready, orc):
     assert np.array_equal(phis[0].view(np.uint64), phis[1].view(np.uint64))
 
 
-@pytest.mark.parametrize("ion_margin", [None, 0, 1])
+@pytest.mark.parametrize("ion_margin", [None, 0, 1, 5])
 def test_step_deposit_2d_two_species_bitexact(gpu_ready, orc, ion_margin):
     """The time loop's 2-D path: one push launch for both species, the tile
     windows of both summed in one launch, counts + rho in another.  After one
     iskra_step in a preset E, particle states, per-species densities and rho
     equal the oracle mirror bit for bit (rho = 0 + q_e n_e + q_i n_i + bg in
     FP64, the kernel's order), also with the ions on narrower deposit
-    windows than the electrons (per-species margin 0 or 1 against 2)."""
+    windows than the electrons (per-species margin 0 or 1 against 2; 5 is
+    clipped to the context's 2)."""
     from iskra_amd import Context, _lib as L
     rng = np.random.default_rng(31)
     nx, ny = 65, 49
