@@ -246,6 +246,21 @@ int iskra_rng_set(iskra_ctx *ctx, uint64_t seed);
  * words per block -> out[n][nblocks][4].  The CPU oracle consumes these. */
 int iskra_rng_export(iskra_ctx *ctx, uint32_t stream, uint32_t step,
                      const uint32_t *ids, int64_t n, int nblocks, uint32_t *out);
+/* iskra_rng_export with no device allocation.  No counterpart in the
+ * reference (which has no device memory): it is for callers that cannot
+ * allocate device memory -- code under stream or graph capture, tight
+ * serving or test loops, harnesses that forbid hipMalloc.  Same arguments
+ * and the same draws as iskra_rng_export; the ids and draws are staged in a
+ * fixed per-GPU scratch array of ISKRA_NOALLOC_SCRATCH_BYTES, held by each
+ * GPU the library runs on until the process ends, so
+ * round_up(4 n, 256) + 16 n nblocks must not exceed it (ISKRA_E_ARG
+ * otherwise, nothing launched).  Every context on one GPU shares that array,
+ * which keeps the last call's contents: calls that overlap on one GPU (other
+ * host threads or contexts) race. */
+#define ISKRA_NOALLOC_SCRATCH_BYTES (4 << 20)
+int iskra_rng_export_noalloc(iskra_ctx *ctx, uint32_t stream, uint32_t step,
+                             const uint32_t *ids, int64_t n, int nblocks,
+                             uint32_t *out);
 
 /* ---- the time loop ------------------------------------------------------- */
 /* nsteps iterations of ParticleInCell.solve's loop body

@@ -20,6 +20,7 @@ F_RHO, F_PHI, F_EX, F_EY, F_BG, F_DENSITY, F_TARGET = range(7)
 POISSON_DIRECT, POISSON_PCG, POISSON_MG, POISSON_MGPCG, POISSON_SPECTRAL = range(5)
 WARM_ZERO, WARM_PREV = 0, 1
 COLL_ELASTIC_ISOTROPIC, COLL_ELASTIC_BACKWARD, COLL_EXCITATION, COLL_IONIZATION = range(4)
+NOALLOC_SCRATCH_BYTES = 4 << 20
 
 EXPORTS = [
     "iskra_last_error", "iskra_version", "iskra_ctx_create", "iskra_ctx_destroy", "iskra_sync",
@@ -32,7 +33,8 @@ EXPORTS = [
     "iskra_push", "iskra_deposit_species", "iskra_deposit", "iskra_field_download",
     "iskra_field_upload", "iskra_poisson_set", "iskra_poisson_set_tag_value",
     "iskra_poisson_set_tag_waveform", "iskra_poisson_solve", "iskra_efield", "iskra_mcc_add",
-    "iskra_mcc_run", "iskra_mcc_counts", "iskra_rng_set", "iskra_rng_export", "iskra_step",
+    "iskra_mcc_run", "iskra_mcc_counts", "iskra_rng_set", "iskra_rng_export",
+    "iskra_rng_export_noalloc", "iskra_step",
     "iskra_stats_get", "iskra_profile", "iskra_bench_copy",
     "iskra_ws_create", "iskra_ws_destroy", "iskra_ws_param_count", "iskra_ws_set_params",
     "iskra_ws_get_params", "iskra_ws_set_gain", "iskra_ws_forward", "iskra_ws_backward",
@@ -114,6 +116,7 @@ def lib():
             "iskra_mcc_counts": (I, [P, I, I, P]),
             "iskra_rng_set": (I, [P, U64]),
             "iskra_rng_export": (I, [P, U32, U32, P, I64, I, P]),
+            "iskra_rng_export_noalloc": (I, [P, U32, U32, P, I64, I, P]),
             "iskra_step": (I, [P, D, I64]),
             "iskra_stats_get": (I, [P, C.POINTER(Stats)]),
             "iskra_profile": (I, [P, I]),
@@ -410,6 +413,15 @@ class Context:
         ids = np.ascontiguousarray(ids, dtype=np.uint32)
         out = np.zeros(len(ids) * nblocks * 4, dtype=np.uint32)
         check(lib().iskra_rng_export(self.h, stream, step, _p(ids), len(ids), nblocks, _p(out)))
+        return out.reshape(len(ids), nblocks, 4)
+
+    def rng_export_noalloc(self, stream, step, ids, nblocks):
+        """rng_export staged in the library's fixed per-GPU scratch (no device
+        allocation; at most NOALLOC_SCRATCH_BYTES of ids and draws)."""
+        ids = np.ascontiguousarray(ids, dtype=np.uint32)
+        out = np.zeros(len(ids) * nblocks * 4, dtype=np.uint32)
+        check(lib().iskra_rng_export_noalloc(self.h, stream, step, _p(ids), len(ids), nblocks,
+                                             _p(out)))
         return out.reshape(len(ids), nblocks, 4)
 
     # loop

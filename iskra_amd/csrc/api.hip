@@ -267,6 +267,9 @@ int iskra_ctx_create(int device, iskra_ctx **out) {
     return ISKRA_E_ARG;
   }
   ISK_HIP(hipSetDevice(device));
+  /* once per device, with the device current: the address of its copy of
+   * the no-allocation scratch array */
+  RC(isk_noalloc_init(device));
   iskra_ctx *c = new (std::nothrow) iskra_ctx();
   if (!c) {
     isk_set_error("out of host memory");
@@ -1093,6 +1096,16 @@ int iskra_rng_export(iskra_ctx *c, uint32_t stream, uint32_t step, const uint32_
   CHECK_CTX(c);
   if ((n > 0 && (!ids || !out)) || nblocks < 0) return ISKRA_E_ARG;
   return isk_rng_export(c, stream, step, ids, n, nblocks, out);
+}
+
+int iskra_rng_export_noalloc(iskra_ctx *c, uint32_t stream, uint32_t step, const uint32_t *ids,
+                             int64_t n, int nblocks, uint32_t *out) {
+  CHECK_CTX(c);
+  if (n < 0 || nblocks < 0 || (n > 0 && nblocks > 0 && (!ids || !out))) {
+    isk_set_error("rng export: negative size or null array");
+    return ISKRA_E_ARG;
+  }
+  return isk_rng_export_noalloc(c, stream, step, ids, n, nblocks, out);
 }
 
 /* ---- the time loop ---- */

@@ -291,6 +291,9 @@ int isk_mcc_setup(iskra_ctx *c, int id);
 int isk_mcc_run(iskra_ctx *c, int id, double dt, int64_t *events);
 int isk_rng_export(iskra_ctx *c, uint32_t stream, uint32_t step,
                    const uint32_t *ids, int64_t n, int nblocks, uint32_t *out);
+int isk_noalloc_init(int device);
+int isk_rng_export_noalloc(iskra_ctx *c, uint32_t stream, uint32_t step,
+                           const uint32_t *ids, int64_t n, int nblocks, uint32_t *out);
 int isk_species_alloc(iskra_ctx *c, IskSpecies *s);
 void isk_species_release(IskSpecies *s);
 int isk_species_reset_chunks(iskra_ctx *c, int sid);

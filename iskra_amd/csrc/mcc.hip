@@ -21,6 +21,7 @@
 #include <math.h>
 
 #include <algorithm>
+#include <mutex>
 #include <vector>
 
 #include "isk_fp64math.h"
@@ -957,5 +958,70 @@ int isk_rng_export(iskra_ctx *c, uint32_t stream, uint32_t step, const uint32_t 
   ISK_HIP(hipMemcpy(out, dout, n * nblocks * 16, hipMemcpyDeviceToHost));
   isk_free(dids);
   isk_free(dout);
+  return ISKRA_OK;
+}
+
+/* ---- the draw export without device allocation ----
+ * The staging of iskra_rng_export_noalloc (the ids at offset 0, the draws
+ * after them at the next 256-byte boundary) is this file-scope array: every
+ * GPU holds its own copy until the process ends.  In host code the array's
+ * name is only a stand-in; the device address of each GPU's copy comes from
+ * hipGetSymbolAddress, taken once per device at context creation with that
+ * device current, and only that address is handed to copies and kernels. */
+#define ISK_NOALLOC_MAX_DEVICES 64
+static __device__ __align__(256) unsigned char scratch[ISKRA_NOALLOC_SCRATCH_BYTES];
+static std::mutex g_scratch_mu;
+static unsigned char *g_scratch_ptr[ISK_NOALLOC_MAX_DEVICES];
+
+int isk_noalloc_init(int device) {
+  if (device < 0 || device >= ISK_NOALLOC_MAX_DEVICES) {
+    isk_set_error("device %d: the no-allocation scratch supports devices 0..%d", device,
+                  ISK_NOALLOC_MAX_DEVICES - 1);
+    return ISKRA_E_ARG;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[device]) return ISKRA_OK;
+  void *p = nullptr;
+  hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) return isk_hip_fail(e, "hipGetSymbolAddress(scratch)", __FILE__, __LINE__);
+  if (!p) {
+    isk_set_error("hipGetSymbolAddress(scratch) returned a null address on device %d", device);
+    return ISKRA_E_HIP;
+  }
+  g_scratch_ptr[device] = (unsigned char *)p;
+  return ISKRA_OK;
+}
+
+int isk_rng_export_noalloc(iskra_ctx *c, uint32_t stream, uint32_t step, const uint32_t *ids,
+                           int64_t n, int nblocks, uint32_t *out) {
+  if (n <= 0 || nblocks <= 0) return ISKRA_OK;
+  const int64_t cap = ISKRA_NOALLOC_SCRATCH_BYTES;
+  /* the size test cannot overflow: n <= cap/4 and nblocks <= cap/16 bound the
+   * product by 2^38 */
+  const int64_t ids_bytes = (n * 4 + 255) / 256 * 256;
+  if (n > cap / 4 || nblocks > cap / 16 || ids_bytes + n * (int64_t)nblocks * 16 > cap) {
+    isk_set_error("rng export without allocation: %lld ids x %d blocks exceed the %lld-byte "
+                  "scratch (it needs round_up(4 n, 256) + 16 n nblocks bytes)",
+                  (long long)n, nblocks, (long long)cap);
+    return ISKRA_E_ARG;
+  }
+  unsigned char *base = nullptr;
+  {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (c->device >= 0 && c->device < ISK_NOALLOC_MAX_DEVICES) base = g_scratch_ptr[c->device];
+  }
+  if (!base) {
+    isk_set_error("no-allocation scratch of device %d was not set up", c->device);
+    return ISKRA_E_STATE;
+  }
+  uint32_t *dids = (uint32_t *)base, *dout = (uint32_t *)(base + ids_bytes);
+  ISK_HIP(hipMemcpyAsync(dids, ids, n * 4, hipMemcpyHostToDevice, c->stream));
+  const long long tot = n * nblocks;
+  hipLaunchKernelGGL(k_rng_export, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0,
+                     c->stream, dids, (long long)n, nblocks, step, stream,
+                     (uint32_t)c->seed, (uint32_t)(c->seed >> 32), dout);
+  ISK_CHECK_LAUNCH();
+  ISK_HIP(hipMemcpyAsync(out, dout, tot * 16, hipMemcpyDeviceToHost, c->stream));
+  ISK_HIP(hipStreamSynchronize(c->stream));
   return ISKRA_OK;
 }
