@@ -55,11 +55,11 @@ int iskra_ctx_destroy(iskra_ctx *ctx);
 int iskra_sync(iskra_ctx *ctx);
 
 /* ---- grid --------------------------------------------------------------- */
-/* RegularGrids.create_uniform_grid (RegularGrids/src/RegularGrids.jl:135-162).
+/* RegularGrids.create_uniform_grid (RegularGrids/src/RegularGrids.jl:55-82).
  * dim 1 or 2; nodes[d] node counts (cells = nodes-1); dh[d] spacing;
  * origin[d]; vol_periodic[4] = {left,right,bottom,top} edges declared
  * :periodic, which disables the edge-volume halving of cell_volume
- * (RegularGrids.jl:106-118).  Resets species, solver and fields. */
+ * (RegularGrids.jl:26-38).  Resets species, solver and fields. */
 int iskra_grid_set(iskra_ctx *ctx, int dim, const int64_t nodes[2],
                    const double dh[2], const double origin[2],
                    const int32_t vol_periodic[4]);
@@ -223,7 +223,8 @@ enum {
   ISKRA_COLL_ELASTIC_ISOTROPIC = 0,
   ISKRA_COLL_ELASTIC_BACKWARD = 1,
   ISKRA_COLL_EXCITATION = 2,
-  ISKRA_COLL_IONIZATION = 3
+  ISKRA_COLL_IONIZATION = 3,
+  ISKRA_COLL_INELASTIC_BACKWARD = 4 /* MCC.InelasticBackward (mcc.jl:144-157) */
 };
 int iskra_mcc_add(iskra_ctx *ctx, int src_sid, int n_coll, const int32_t *types,
                   const double *energy, const int32_t *tab_off,

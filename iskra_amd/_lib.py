@@ -20,6 +20,7 @@ F_RHO, F_PHI, F_EX, F_EY, F_BG, F_DENSITY, F_TARGET = range(7)
 POISSON_DIRECT, POISSON_PCG, POISSON_MG, POISSON_MGPCG, POISSON_SPECTRAL = range(5)
 WARM_ZERO, WARM_PREV = 0, 1
 COLL_ELASTIC_ISOTROPIC, COLL_ELASTIC_BACKWARD, COLL_EXCITATION, COLL_IONIZATION = range(4)
+COLL_INELASTIC_BACKWARD = 4
 NOALLOC_SCRATCH_BYTES = 4 << 20
 
 EXPORTS = [

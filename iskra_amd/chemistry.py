@@ -42,6 +42,13 @@ class ElasticBackward:
     energy = 0.0
 
 
+class InelasticBackward:
+    """Backward scattering that keeps a uniform fraction of the relative
+    speed (mcc.jl:144-157)."""
+    code = L.COLL_INELASTIC_BACKWARD
+    energy = 0.0
+
+
 class Excitation:
     code = L.COLL_EXCITATION
 

@@ -302,7 +302,8 @@ static void release_fields(iskra_ctx *c) {
   for (int m = 0; m < c->nmcc; ++m) {
     IskMcc &q = c->mcc[m];
     void *ptrs[] = {q.d_eps, q.d_sig, q.d_off, q.target_n, q.nu, q.events, q.errflag,
-                    q.prod_flag, q.stage, q.blk_cnt, q.cand_i, q.cand_cnt};
+                    q.prod_flag, q.stage, q.blk_cnt, q.cand_i, q.cand_cnt,
+                    q.stage_pid, q.pid_c, q.snap};
     for (void *p : ptrs) isk_free(p);
     q.tab_off.clear();
     q.tab_eps.clear();
@@ -418,7 +419,7 @@ int iskra_grid_set(iskra_ctx *c, int dim, const int64_t nodes[2], const double d
   ISK_HIP(hipMemset(c->phi, 0, nn * 8));
   ISK_HIP(hipMemset(c->bg, 0, nn * 8));
   ISK_HIP(hipMemset(c->E, 0, (size_t)nn * 4 * dim));
-  /* cell_volume (RegularGrids.jl:106-118); 1-D by analogy: dx, halved at
+  /* cell_volume (RegularGrids.jl:26-38); 1-D by analogy: dx, halved at
    * non-periodic ends */
   std::vector<double> V(nn);
   double V0 = dim == 2 ? g.dx * g.dy : g.dx;
@@ -748,6 +749,25 @@ int iskra_species_sample_maxwellian(iskra_ctx *c, int sid, int64_t n, const doub
     isk_set_error("sample: null wx/wv or negative n");
     return ISKRA_E_ARG;
   }
+  CHECK_GRID(c);
+  /* positions are a cell and an offset inside it, so the box [dx, dx + wx)
+   * must lie in the grid: a particle outside it has no representation (the
+   * reference would wrap or discard it only after the next push,
+   * pic/surfaces/wrap.jl:1-33) */
+  for (int d = 0; d < c->g.dim; ++d) {
+    const double lo = dx ? dx[d] : 0.0, hi = lo + wx[d];
+    const double g0 = d == 0 ? c->g.x0 : c->g.y0;
+    const double g1 = g0 + (d == 0 ? c->g.ncx * c->g.dx : c->g.ncy * c->g.dy);
+    /* slack for a box given as the domain length, which can differ from
+     * cells * spacing by round-off (the kernel keeps such positions in the
+     * edge cells) */
+    const double tol = 1e-12 * std::max(fabs(g0), fabs(g1));
+    if (!(wx[d] >= 0.0) || !(lo >= g0 - tol) || !(hi <= g1 + tol)) {
+      isk_set_error("sample: source box [%g, %g) of dimension %d leaves the grid [%g, %g]", lo,
+                    hi, d + 1, g0, g1);
+      return ISKRA_E_ARG;
+    }
+  }
   RC(isk_sample_maxwellian(c, sid, n, wx, dx, wv, dv, seed, stream));
   c->sp[sid].deposited = 0;
   ISK_HIP(hipStreamSynchronize(c->stream));
@@ -1022,7 +1042,7 @@ int iskra_mcc_add(iskra_ctx *c, int src_sid, int n_coll, const int32_t *types,
   m.ion = ion_sid;
   m.ncoll = n_coll;
   for (int k = 0; k < n_coll; ++k) {
-    if (types[k] < 0 || types[k] > ISKRA_COLL_IONIZATION) {
+    if (types[k] < 0 || types[k] > ISKRA_COLL_INELASTIC_BACKWARD) {
       isk_set_error("mcc_add: bad collision type %d", types[k]);
       return ISKRA_E_ARG;
     }

@@ -116,6 +116,9 @@ struct IskMcc {
   int *blk_cnt;              /* per block product count, scanned in place */
   int *cand_i;               /* per block: accepted candidates (list records), in order */
   int *cand_cnt;             /* per block accepted count */
+  uint32_t *stage_pid;       /* per staged product: the parent's id */
+  uint32_t *pid_c;           /* parent ids of the products in append order */
+  long long *snap;           /* species counts and next ids before the append */
   int64_t stage_cap;
   uint32_t stream;
   uint32_t step;

@@ -70,6 +70,7 @@ struct MccArgs {
   int32_t *stage_cell;
   int *cand_i;   /* accepted candidates of each work-group (list records) */
   int *cand_cnt; /* per work-group accepted count */
+  uint32_t *stage_pid; /* parent id of each staged product */
 };
 
 struct Draws {
@@ -178,7 +179,7 @@ __device__ void randn3(Draws &d, double *n) {
  * code 64 at a time, so a wave runs one type's code instead of every type's
  * branch in turn; each candidate's draws continue at Philox block 1, as they
  * would in one pass. */
-#define MCC_NTYPE 4
+#define MCC_NTYPE 5
 __device__ __forceinline__ void mcc_cand_body(const MccArgs &a, const int blk) {
   __shared__ int wcount[MCC_BLOCK / 64];
   __shared__ int wprod[MCC_BLOCK / 64];
@@ -360,7 +361,7 @@ __device__ __forceinline__ void mcc_coll_body(const MccArgs &a, const int blk) {
   const int nacc = ISK_GL(a.cand_cnt)[blk];
   for (int s0 = 0; s0 < nacc; s0 += MCC_SEG) {
     const int s1 = min(nacc, s0 + MCC_SEG);
-    int lcnt[MCC_NTYPE] = {0, 0, 0, 0};
+    int lcnt[MCC_NTYPE] = {0, 0, 0, 0, 0};
     for (int t0 = s0; t0 < s1; t0 += MCC_COLL) {
       const int q = t0 + lane;
       const int type = q < s1 ? a.types[(ISK_GL(a.cand_i)[b0 + q] >> 11) & 0x1F] : -1;
@@ -372,7 +373,11 @@ __device__ __forceinline__ void mcc_coll_body(const MccArgs &a, const int blk) {
       }
     }
     if (lane < MCC_NTYPE)
-      s_cnt[lane] = lane == 0 ? lcnt[0] : lane == 1 ? lcnt[1] : lane == 2 ? lcnt[2] : lcnt[3];
+      s_cnt[lane] = lane == 0   ? lcnt[0]
+                    : lane == 1 ? lcnt[1]
+                    : lane == 2 ? lcnt[2]
+                    : lane == 3 ? lcnt[3]
+                                : lcnt[4];
     __syncthreads();
     /* T is uniform: the type branches below are scalar branches */
 #pragma unroll 1
@@ -392,7 +397,8 @@ __device__ __forceinline__ void mcc_coll_body(const MccArgs &a, const int blk) {
         d.block = 1;
         d.half = 0;
         const double vx = ISK_GL(a.vx)[p], vy = ISK_GL(a.vy)[p], vz = ISK_GL(a.vz)[p];
-        if (T == ISKRA_COLL_ELASTIC_ISOTROPIC || T == ISKRA_COLL_ELASTIC_BACKWARD) {
+        if (T == ISKRA_COLL_ELASTIC_ISOTROPIC || T == ISKRA_COLL_ELASTIC_BACKWARD ||
+            T == ISKRA_COLL_INELASTIC_BACKWARD) {
           const double sv[3] = {vx, vy, vz};
           double mr1 = a.ms / (a.ms + a.mt), mr2 = a.mt / (a.ms + a.mt);
           double tv[3];
@@ -405,10 +411,14 @@ __device__ __forceinline__ void mcc_coll_body(const MccArgs &a, const int blk) {
           }
           double nvr = sqrt(vr[0] * vr[0] + vr[1] * vr[1] + vr[2] * vr[2]);
           double dir[3];
-          if (T == ISKRA_COLL_ELASTIC_ISOTROPIC)
+          if (T == ISKRA_COLL_ELASTIC_ISOTROPIC) {
             isotropic_scattering(vr[0], vr[1], vr[2], d, dir);
-          else
+          } else {
+            /* InelasticBackward (mcc.jl:144-157): |vr| scaled by a uniform
+             * drawn before the direction */
+            if (T == ISKRA_COLL_INELASTIC_BACKWARD) nvr = d.next() * nvr;
             diffuse_reflection(vr[0], vr[1], vr[2], d, dir);
+          }
           ISK_GL(a.vx)[p] = (float)(wv[0] + mr2 * (nvr * dir[0]));
           ISK_GL(a.vy)[p] = (float)(wv[1] + mr2 * (nvr * dir[1]));
           ISK_GL(a.vz)[p] = (float)(wv[2] + mr2 * (nvr * dir[2]));
@@ -441,6 +451,7 @@ __device__ __forceinline__ void mcc_coll_body(const MccArgs &a, const int blk) {
           randn3(d, tv);
           const size_t sidx = (size_t)b0 + slot;
           ISK_GL(a.stage_cell)[sidx] = ISK_GL(a.cell)[p];
+          ISK_GL(a.stage_pid)[sidx] = d.id;
           auto *st = ISK_GL(a.stage) + sidx * 8;
           st[0] = ISK_GL(a.ox)[p];
           st[1] = a.dim == 2 ? ISK_GL(a.oy)[p] : 0.f;
@@ -503,11 +514,18 @@ struct AppendArgs {
   long long icap;
   int R;
   int dim;
+  const uint32_t *stage_pid; /* parent id per staged product */
+  uint32_t *pid_c;           /* out: parent id of product g */
+  long long *snap;           /* out (work-group 0): counts and next ids */
 };
 
 /* one work-group appends the products of MCC_APPEND_GROUP consecutive MCC
  * work-groups (their products are contiguous in the global order); product
- * g comes from the last of them whose offset is <= g */
+ * g comes from the last of them whose offset is <= g.  Positions and
+ * velocities go to slot count + g of each species; the ids are handed out
+ * afterwards by the commit kernel in the order of the parents' ids, so a
+ * product's id (and with it every later draw of it) does not depend on the
+ * order the particles happen to be stored in. */
 #define MCC_APPEND_GROUP 16
 static inline int mcc_append_blocks(int nblk) {
   return (nblk + MCC_APPEND_GROUP - 1) / MCC_APPEND_GROUP;
@@ -518,6 +536,14 @@ __device__ __forceinline__ void mcc_append_one(const AppendArgs &a, int g, int b
 __device__ __forceinline__ void mcc_append_body(const AppendArgs &a, const int b) {
   const int m0 = b * MCC_APPEND_GROUP, m1 = min(m0 + MCC_APPEND_GROUP, a.nblk);
   const int g1 = a.blk_off[m1];
+  /* the counts and next ids the commit kernel hands ids out from (it must
+   * not read the headers it rewrites) */
+  if (b == 0 && threadIdx.x == 0) {
+    a.snap[0] = a.ehdr[HDR_NP];
+    a.snap[1] = a.ehdr[HDR_NEXT_ID];
+    a.snap[2] = a.ihdr ? a.ihdr[HDR_NP] : 0;
+    a.snap[3] = a.ihdr ? a.ihdr[HDR_NEXT_ID] : 0;
+  }
   for (int g = a.blk_off[m0] + (int)threadIdx.x; g < g1; g += blockDim.x) {
     int lo = m0, hi = m1 - 1;
     while (lo < hi) {
@@ -543,8 +569,8 @@ __device__ __forceinline__ void mcc_append_one(const AppendArgs &a, int g, int b
     a.evx[pe] = st[2];
     a.evy[pe] = st[3];
     a.evz[pe] = st[4];
-    if (a.eid) a.eid[pe] = (uint32_t)(a.ehdr[HDR_NEXT_ID] + g);
   }
+  a.pid_c[g] = a.stage_pid[s];
   if (a.ic && a.R > 0) {
     long long inp = a.ihdr[HDR_NP];
     for (int r = 0; r < a.R; ++r) {
@@ -556,39 +582,86 @@ __device__ __forceinline__ void mcc_append_one(const AppendArgs &a, int g, int b
       a.ivx[pi] = st[5];
       a.ivy[pi] = st[6];
       a.ivz[pi] = st[7];
-      if (a.iid) a.iid[pi] = (uint32_t)(a.ihdr[HDR_NEXT_ID] + (long long)g * a.R + r);
     }
   }
 }
 
 __global__ void k_mcc_append(AppendArgs a) { mcc_append_body(a, blockIdx.x); }
 
-__device__ __forceinline__ void mcc_commit_body(const int *blk_off, int nblk, long long *ehdr,
-                                                long long ecap, long long *ihdr, long long icap,
-                                                int R) {
-  long long tot = blk_off[nblk];
+struct CommitArgs {
+  const int *blk_off;
+  int nblk;
+  long long *ehdr;
+  long long ecap;
+  long long *ihdr;
+  long long icap;
+  int R;
+  uint32_t *eid, *iid;     /* id arrays of the buffers the products went to */
+  const uint32_t *pid_c;   /* parent id of product g */
+  const long long *snap;   /* counts and next ids before the append */
+};
+
+/* work-groups of the commit kernel when the interaction ionises (ranking the
+ * products is O(products^2), spread over them) */
+#define MCC_RANK_GROUPS 64
+#define MCC_RANK_TILE 1024
+
+/* Ids of the products and the species headers.  Product g receives
+ * next_id + rank(g), rank = the number of products whose parent id is below
+ * its parent's (parent ids are unique, so ranks are 0..P-1), and its R ions
+ * next_ion_id + rank R + r.  Work-group 0 then updates the headers; no
+ * work-group reads them (they use the append kernel's snapshot). */
+__device__ __forceinline__ void mcc_commit_body(const CommitArgs &a, const int blk,
+                                                const int nblk_grid) {
+  __shared__ uint32_t tile[MCC_RANK_TILE];
+  const long long tot = a.blk_off[a.nblk];
   if (tot == 0) return;
-  long long ne = ehdr[HDR_NP] + tot;
-  if (ne > ecap) {
-    ehdr[HDR_OVERFLOW] = 1;
-    ne = ecap;
-  }
-  ehdr[HDR_NP] = ne;
-  ehdr[HDR_NEXT_ID] += tot;
-  if (ihdr && R > 0) {
-    long long ni = ihdr[HDR_NP] + tot * R;
-    if (ni > icap) {
-      ihdr[HDR_OVERFLOW] = 1;
-      ni = icap;
+  const int P = (int)tot;
+  if (a.eid || (a.iid && a.R > 0)) {
+    const long long enp = a.snap[0], enext = a.snap[1], inp = a.snap[2], inext = a.snap[3];
+    for (int g0 = blk * (int)blockDim.x; g0 < P; g0 += nblk_grid * (int)blockDim.x) {
+      const int g = g0 + (int)threadIdx.x;
+      const uint32_t mine = g < P ? a.pid_c[g] : 0u;
+      int rank = 0;
+      for (int t0 = 0; t0 < P; t0 += MCC_RANK_TILE) {
+        const int tn = min(MCC_RANK_TILE, P - t0);
+        __syncthreads(); /* the previous tile has been read */
+        for (int q = threadIdx.x; q < tn; q += blockDim.x) tile[q] = a.pid_c[t0 + q];
+        __syncthreads();
+        for (int q = 0; q < tn; ++q) rank += tile[q] < mine ? 1 : 0;
+      }
+      if (g < P) {
+        if (a.eid && enp + g < a.ecap) a.eid[enp + g] = (uint32_t)(enext + rank);
+        if (a.iid)
+          for (int r = 0; r < a.R; ++r) {
+            const long long pi = inp + (long long)g * a.R + r;
+            if (pi >= a.icap) break;
+            a.iid[pi] = (uint32_t)(inext + (long long)rank * a.R + r);
+          }
+      }
     }
-    ihdr[HDR_NP] = ni;
-    ihdr[HDR_NEXT_ID] += tot * R;
+  }
+  if (blk != 0 || threadIdx.x != 0) return;
+  long long ne = a.snap[0] + tot;
+  if (ne > a.ecap) {
+    a.ehdr[HDR_OVERFLOW] = 1;
+    ne = a.ecap;
+  }
+  a.ehdr[HDR_NP] = ne;
+  a.ehdr[HDR_NEXT_ID] = a.snap[1] + tot;
+  if (a.ihdr && a.R > 0) {
+    long long ni = a.snap[2] + tot * a.R;
+    if (ni > a.icap) {
+      a.ihdr[HDR_OVERFLOW] = 1;
+      ni = a.icap;
+    }
+    a.ihdr[HDR_NP] = ni;
+    a.ihdr[HDR_NEXT_ID] = a.snap[3] + tot * a.R;
   }
 }
 
-__global__ void k_mcc_commit(const int *blk_off, int nblk, long long *ehdr, long long ecap,
-                             long long *ihdr, long long icap, int R) {
-  mcc_commit_body(blk_off, nblk, ehdr, ecap, ihdr, icap, R);
+__global__ __launch_bounds__(MCC_BLOCK) void k_mcc_commit(CommitArgs a) {
+  mcc_commit_body(a, blockIdx.x, gridDim.x);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -654,16 +727,6 @@ int isk_mcc_setup(iskra_ctx *c, int idx) {
   return ISKRA_OK;
 }
 
-struct CommitArgs {
-  const int *blk_off;
-  int nblk;
-  long long *ehdr;
-  long long ecap;
-  long long *ihdr;
-  long long icap;
-  int R;
-};
-
 /* arguments of the five launches of one MCC run, the source species reading
  * buffer b_src and the ion species buffer b_ion; allocates the staging area */
 static int mcc_prepare(iskra_ctx *c, int idx, double dt, int b_src, int b_ion, MccArgs *pa,
@@ -685,16 +748,25 @@ static int mcc_prepare(iskra_ctx *c, int idx, double dt, int b_src, int b_ion, M
     isk_free(m.prod_flag);
     isk_free(m.cand_i);
     isk_free(m.cand_cnt);
+    isk_free(m.stage_pid);
+    isk_free(m.pid_c);
+    isk_free(m.snap);
     m.cand_i = nullptr;
     m.cand_cnt = nullptr;
     m.stage = nullptr;
     m.blk_cnt = nullptr;
     m.prod_flag = nullptr;
+    m.stage_pid = nullptr;
+    m.pid_c = nullptr;
+    m.snap = nullptr;
     if (isk_alloc((void **)&m.stage, (size_t)nblk * MCC_PPB * 8 * 4) ||
         isk_alloc((void **)&m.blk_cnt, (size_t)(nblk + 1) * 4) ||
         isk_alloc((void **)&m.prod_flag, (size_t)nblk * MCC_PPB * 4) ||
         isk_alloc((void **)&m.cand_i, (size_t)nblk * MCC_PPB * 4) ||
-        isk_alloc((void **)&m.cand_cnt, (size_t)nblk * 4))
+        isk_alloc((void **)&m.cand_cnt, (size_t)nblk * 4) ||
+        isk_alloc((void **)&m.stage_pid, (size_t)nblk * MCC_PPB * 4) ||
+        isk_alloc((void **)&m.pid_c, (size_t)nblk * MCC_PPB * 4) ||
+        isk_alloc((void **)&m.snap, 64))
       return ISKRA_E_HIP;
     m.stage_cap = cap;
   }
@@ -740,7 +812,11 @@ static int mcc_prepare(iskra_ctx *c, int idx, double dt, int b_src, int b_ion, M
   a.stage_cell = m.prod_flag;
   a.cand_i = m.cand_i;
   a.cand_cnt = m.cand_cnt;
+  a.stage_pid = m.stage_pid;
   AppendArgs &p = *pp;
+  p.stage_pid = m.stage_pid;
+  p.pid_c = m.pid_c;
+  p.snap = m.snap;
   p.blk_off = m.blk_cnt;
   p.stage = m.stage;
   p.stage_cell = m.prod_flag;
@@ -783,8 +859,20 @@ static int mcc_prepare(iskra_ctx *c, int idx, double dt, int b_src, int b_ion, M
   pc->ihdr = p.ihdr;
   pc->icap = (long long)p.icap;
   pc->R = R;
+  pc->eid = p.eid;
+  pc->iid = p.ic ? p.iid : nullptr;
+  pc->pid_c = m.pid_c;
+  pc->snap = m.snap;
   *nblk_out = nblk;
   return ISKRA_OK;
+}
+
+/* work-groups of the commit kernel: one when nothing is appended, else enough
+ * to spread the ranking of the products */
+static int mcc_commit_groups(const IskMcc &m, int nblk) {
+  bool ionises = false;
+  for (int k = 0; k < m.ncoll; ++k) ionises |= m.types[k] == ISKRA_COLL_IONIZATION;
+  return ionises ? std::max(1, std::min(MCC_RANK_GROUPS, mcc_append_blocks(nblk))) : 1;
 }
 
 /* dynamic LDS of the candidate kernel: the cross-section tables when they fit */
@@ -814,8 +902,8 @@ int isk_mcc_run(iskra_ctx *c, int idx, double dt, int64_t *events) {
   hipLaunchKernelGGL(k_mcc_append, dim3(mcc_append_blocks(nblk)), dim3(MCC_BLOCK), 0,
                      c->stream, p);
   ISK_CHECK_LAUNCH();
-  hipLaunchKernelGGL(k_mcc_commit, dim3(1), dim3(1), 0, c->stream, cm.blk_off, cm.nblk,
-                     cm.ehdr, cm.ecap, cm.ihdr, cm.icap, cm.R);
+  hipLaunchKernelGGL(k_mcc_commit, dim3(mcc_commit_groups(m, nblk)), dim3(MCC_BLOCK), 0,
+                     c->stream, cm);
   ISK_CHECK_LAUNCH();
   if (p.ihdr) c->sp[m.ion].deposited = 0;
   s.deposited = 0;
@@ -861,11 +949,11 @@ __global__ __launch_bounds__(MCC_BLOCK) void kb_mcc_append(const AppendArgs *A, 
   mcc_append_body(a, blockIdx.x - bstart[i]);
 }
 
-__global__ void kb_mcc_commit(const CommitArgs *A, const int *bstart, int n) {
+__global__ __launch_bounds__(MCC_BLOCK) void kb_mcc_commit(const CommitArgs *A,
+                                                            const int *bstart, int n) {
   const int i = isk_bfind(bstart, n, blockIdx.x);
-  if (threadIdx.x == 0)
-    mcc_commit_body(A[i].blk_off, A[i].nblk, A[i].ehdr, A[i].ecap, A[i].ihdr, A[i].icap,
-                    A[i].R);
+  const CommitArgs a = A[i];
+  mcc_commit_body(a, blockIdx.x - bstart[i], bstart[i + 1] - bstart[i]);
 }
 
 int isk_bbuild_mcc(iskra_ctx **cs, int n, int m, double dt, const int *cur_src,
@@ -873,7 +961,7 @@ int isk_bbuild_mcc(iskra_ctx **cs, int n, int m, double dt, const int *cur_src,
   std::vector<MccArgs> A(n);
   std::vector<AppendArgs> P(n);
   std::vector<CommitArgs> CM(n);
-  std::vector<int> blocks(n), ablocks(n), one(n, 1);
+  std::vector<int> blocks(n), ablocks(n), cblocks(n), one(n, 1);
   size_t tab_shm = 0;
   for (int i = 0; i < n; ++i) {
     int rc = mcc_prepare(cs[i], m, dt, cur_src[i], cur_ion[i], &A[i], &P[i], &CM[i], &blocks[i]);
@@ -881,6 +969,7 @@ int isk_bbuild_mcc(iskra_ctx **cs, int n, int m, double dt, const int *cur_src,
     tab_shm = std::max(tab_shm, mcc_tab_shm(cs[i]->mcc[m]));
     A[i].step_dev = step_dev;
     ablocks[i] = mcc_append_blocks(blocks[i]);
+    cblocks[i] = mcc_commit_groups(cs[i]->mcc[m], blocks[i]);
   }
   int rc;
   if ((rc = isk_blaunch_upload(&L[0], ISK_B_MCC, A.data(), sizeof(MccArgs), blocks, MCC_BLOCK,
@@ -895,7 +984,8 @@ int isk_bbuild_mcc(iskra_ctx **cs, int n, int m, double dt, const int *cur_src,
   if ((rc = isk_blaunch_upload(&L[3], ISK_B_MCC_APPEND, P.data(), sizeof(AppendArgs), ablocks,
                                MCC_BLOCK, 0)))
     return rc;
-  return isk_blaunch_upload(&L[4], ISK_B_MCC_COMMIT, CM.data(), sizeof(CommitArgs), one, 64, 0);
+  return isk_blaunch_upload(&L[4], ISK_B_MCC_COMMIT, CM.data(), sizeof(CommitArgs), cblocks,
+                            MCC_BLOCK, 0);
 }
 
 int isk_blaunch_mcc(const IskBLaunch &L, hipStream_t st) {

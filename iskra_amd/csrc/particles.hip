@@ -18,7 +18,7 @@
  *  push_in_cartesian! (B=0) ParticleInCell/src/pic/pushers.jl:37-50
  *  wrap! / discard!         ParticleInCell/src/pic/surfaces/wrap.jl:1-33
  *  particle_to_grid (CIC)   ParticleInCell/src/pic/cloud_in_cell.jl:1-18
- *  density, cell_volume     pic/kinetic.jl:53, RegularGrids/src/RegularGrids.jl:106-118
+ *  density, cell_volume     pic/kinetic.jl:53, RegularGrids/src/RegularGrids.jl:26-38
  *  rho accumulation         ParticleInCell/src/ParticleInCell.jl:118-124
  */
 #include <math.h>
@@ -39,6 +39,8 @@ struct PushShape {
   static constexpr int block = DIM == 2 ? 512 : 256;
   static constexpr int unroll = DIM == 2 ? 4 : 8;
 };
+/* the largest float below 1 (1 - 2^-24): an offset on the upper edge of its cell */
+#define ISK_BELOW_ONE 0x1.fffffep-1f
 #define SORT_BLOCK 256
 #define SORT_PT 32
 #define SORT_CHUNK (SORT_BLOCK * SORT_PT)
@@ -183,7 +185,10 @@ __global__ __launch_bounds__(256) void k_sample(SampleArgs a) {
   int i = (int)fl;
   float of = (float)(s - fl);
   if (of >= 1.0f) { of = 0.0f; i += 1; }
-  i = min(max(i, 0), a.ncx - 1);
+  /* the host keeps the box inside the grid; these only catch a position
+   * that rounding put just past an edge of it */
+  if (i < 0) { i = 0; of = 0.0f; }
+  if (i > a.ncx - 1) { i = a.ncx - 1; of = ISK_BELOW_ONE; }
   int j = 0;
   if (a.dim == 2) {
     double uy = isk_u01(r0.z, r0.w);
@@ -193,7 +198,8 @@ __global__ __launch_bounds__(256) void k_sample(SampleArgs a) {
     j = (int)fly;
     float ofy = (float)(sy - fly);
     if (ofy >= 1.0f) { ofy = 0.0f; j += 1; }
-    j = min(max(j, 0), a.ncy - 1);
+    if (j < 0) { j = 0; ofy = 0.0f; }
+    if (j > a.ncy - 1) { j = a.ncy - 1; ofy = ISK_BELOW_ONE; }
     a.oy[p] = ofy;
   }
   /* velocities: Box-Muller on (r1.xy, r1.zw) and (r2.xy, r2.zw) */

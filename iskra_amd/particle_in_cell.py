@@ -128,8 +128,9 @@ class FluidSpecies:
     """FluidSpecies{D} (pic/fluid.jl:1-11): a background with density n (per
     node), mobility mu, charge q, mass m, temperature T.  It enters rho as
     q*n (no volume division, fluid.jl:11) and is the MCC target gas.  Its
-    density is held constant (the reference's advection-diffusion gives
-    dn = 0 for uniform n, advection_diffusion.jl:4-19)."""
+    density is held constant, which is what the reference's
+    advection-diffusion update gives for the fluids `bind` accepts
+    (advection_diffusion.jl:4-38); others are refused there."""
 
     def __init__(self, name, mu, q, m, n, T):
         self.name = name
@@ -269,16 +270,48 @@ def _kinetic_species(config):
     return [s for s in config.species if isinstance(s, KineticSpecies)]
 
 
+def _fluid_is_static(f, g):
+    """True when the reference's fluid update leaves n unchanged at every step
+    (advance!(::FluidSpecies), ParticleInCell.jl:74-82, with
+    calculate_advection_diffusion, advection_diffusion.jl:4-38).  1-D: only
+    diffusion mu (n[i-1] - 2 n[i] + n[i+1]) at interior nodes (advection is
+    commented out there).  2-D: diffusion by the 5-point sum plus advection by
+    v = E dt q/m, over interior nodes only, so a grid with fewer than three
+    nodes along an axis has no update at all."""
+    n = np.asarray(np.broadcast_to(f.n, g.n), dtype=np.float64)
+    if g.dim == 1:
+        if g.n[0] < 3 or f.mu == 0.0:
+            return True
+        return not np.any(n[:-2] - 2.0 * n[1:-1] + n[2:])
+    if g.n[0] < 3 or g.n[1] < 3:
+        return True
+    if f.q != 0.0:
+        return False
+    if f.mu == 0.0:
+        return True
+    lap = n[:-2, 1:-1] + n[1:-1, :-2] - 4.0 * n[1:-1, 1:-1] + n[2:, 1:-1] + n[1:-1, 2:]
+    return not np.any(lap)
+
+
 def bind(config):
     """Create the device context for `config` and move grid, solver,
     species, fluids and interactions onto it.  Called by `solve`; callable
-    directly to use the operators below outside the loop."""
-    from ._lib import Context
+    directly to use the operators below outside the loop.  Fluid species are
+    held constant on the device, so a fluid whose density the reference would
+    update (a mobile fluid with non-uniform density, or a charged fluid on a
+    2-D grid) is refused with ISKRA_E_UNSUPPORTED before anything runs."""
+    from ._lib import Context, IskraError
     from . import chemistry
 
     g = config.grid
     if not isinstance(g, UniformGrid):
         raise TypeError("config.grid must come from create_uniform_grid")
+    for f in _fluid_species(config):
+        if not _fluid_is_static(f, g):
+            raise IskraError(L.E_UNSUPPORTED,
+                             f"{f.name}: the fluid's density would change every step "
+                             "(advection_diffusion.jl:4-38); only fluids the reference "
+                             "leaves constant are supported")
     ctx = Context(config.device)
     ctx.grid(g.dim, g.n, g.dh, g.origin if g.dim == 2 else (g.origin[0], 0.0), g.vol_periodic())
     ctx.rng(config.seed)
@@ -331,19 +364,25 @@ def init(src, species, dt):
     """init(src, species, dt) = sample!(src, species, dt) (ParticleInCell.jl:47,
     pic/sources.jl:24-34): append min(free capacity, floor(rate*dt))
     particles drawn on the device (Philox, keyed by the species' source
-    count)."""
+    count).  The draw stream is (species << 24) | call count: species below
+    64 and up to 2^24 calls per species keep every call's stream distinct,
+    and below the MCC streams (0x4D43xxxx)."""
     if species._ctx is None:
         raise RuntimeError("init: the species is not bound to a simulation yet (call it from "
                            "enter_loop, or bind(config) first)")
     species._push()
     n = int(math.floor(src.rate * dt))
     ctx = species._ctx
-    stream = getattr(species, "_nsample", 0)
-    species._nsample = stream + 1
+    count = getattr(species, "_nsample", 0)
+    if species._sid >= 64 or count >= 1 << 24:
+        raise L.IskraError(L.E_UNSUPPORTED,
+                           f"{species.name}: sample! call {count} of species {species._sid} has "
+                           "no distinct draw stream (at most 2^24 calls for 64 species)")
+    species._nsample = count + 1
     wx = src.wx[:species.D]
     dx = src.dx[:species.D]
     ctx.sample_maxwellian(species._sid, n, wx, src.wv, dx=dx, dv=src.dv,
-                          seed=ctx_seed(ctx), stream=(species._sid << 16) | stream)
+                          seed=ctx_seed(ctx), stream=(species._sid << 24) | count)
     species._host_stale = True
 
 

@@ -34,6 +34,20 @@ How each oracle function is pinned (checks live in tests/):
   orc_fix32/fixprod32    (the integer fixed-point CIC weights of the particle
                          kernels) against rint of the exact FP64 products on
                          edge cases and random offsets.
+  mir_scatter            the reference's unrotated matrix literal applied as
+                         written (row vector times matrix, numpy); chi = 0
+                         returns the incident direction; |r|^2 and r.v_hat
+                         against their closed forms for that (non-orthogonal)
+                         matrix; unit length and r.v_hat = cos chi where cos
+                         eta = 0; the chi distributions by KS tests against
+                         their closed-form CDFs.
+  mir_coll_kin           closed forms per collision type with the draws
+                         injected: Box-Muller target velocity, centre-of-mass
+                         velocity fixed, |v' - w| = mr2 |v_r| |r|, energy
+                         conserved by elastic scattering where |r| = 1,
+                         excitation eps' = (eps - E)|r|^2, ionisation energy
+                         split e1 + e2 = eps - E, secondary relative to the
+                         updated primary (tests/test_oracle_mcc_kinematics.py).
   mir_*                  against the ref_* functions within FP32 tolerance
                          (tests/test_oracle.py), and bit-exactly against the GPU.
   orc_ws_run             (learned warm start, oracle/ws_oracle.c; no reference
@@ -113,7 +127,9 @@ def _declare(L):
         "mir_tri_solve": (None, [C.c_int, P, P, P, P, P, P, D, C.c_int, P]),
         "mir_efield": (None, [C.c_int, C.c_int, C.c_int, D, D, P, P]),
         "mir_mcc": (I64, [C.POINTER(MirMcc), I64, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P,
-                          P]),
+                          P, P]),
+        "mir_scatter": (None, [C.c_int, D, D, D, D, D, P]),
+        "mir_coll_kin": (C.c_int, [C.c_int, D, D, D, D, D, P, P, P]),
         "mir_sample": (None, [C.c_int, I64, I64, D, D, D, D, C.c_int, C.c_int, P, P, P, P, U64,
                               U32, P, P, P, P, P, P, P]),
         "mir_sort_tile": (I64, [C.c_int, I64, C.c_int, C.c_int, P, P, P, P, P, P, P]),
@@ -285,6 +301,26 @@ QE_MCC = 1.60217646e-19
 KB_MCC = 1.3806503e-23
 
 
+def scatter(kind, v, u1, u2):
+    """mir_scatter: direction of a particle scattered from velocity v
+    (kind 0 the reference's "isotropic" angles, 1 its diffuse reflection)."""
+    r = np.zeros(3)
+    lib().mir_scatter(int(kind), float(v[0]), float(v[1]), float(v[2]), float(u1), float(u2),
+                      ptr(r))
+    return r
+
+
+def coll_kin(type_, ms, mt, m_eV, energy, vth_t, v, u):
+    """mir_coll_kin: one collision (FP64) from velocity v and the uniforms u.
+    Returns (uniforms used, velocity after, target/ion velocity, secondary)."""
+    vv = np.ascontiguousarray(v, dtype=np.float64)
+    uu = np.zeros(10)
+    uu[:len(u)] = u
+    out = np.zeros(9)
+    k = lib().mir_coll_kin(int(type_), ms, mt, m_eV, energy, vth_t, ptr(vv), ptr(uu), ptr(out))
+    return k, out[0:3].copy(), out[3:6].copy(), out[6:9].copy()
+
+
 class MirSpecies:
     def __init__(self, dim, cap, q, m, w0, track_ids=True, bnd=(1, 1)):
         self.dim, self.cap, self.q, self.m, self.w0 = dim, cap, q, m, w0
@@ -328,7 +364,8 @@ class MirSim:
     """CPU twin of iskra_step for grids whose Poisson solve is the 1-D direct
     solver: same number format, same operation order, same particle order
     (stable tile sort every sort_every steps, products appended in parent
-    order), same Philox draws."""
+    order and numbered in the order of their parents' ids), same Philox
+    draws."""
 
     def __init__(self, nx, dx, x0=0.0, volper=(0, 0, 0, 0), T=64, sort_every=4, seed=0):
         self.dim, self.nx, self.ny = 1, nx, 1
@@ -418,11 +455,12 @@ class MirSim:
         poy = np.zeros(max(n, 1), dtype=np.float32)
         pe = np.zeros(3 * max(n, 1), dtype=np.float32)
         pi = np.zeros(3 * max(n, 1), dtype=np.float32)
+        ppid = np.zeros(max(n, 1), dtype=np.uint32)
         ev = np.zeros(8, dtype=np.int64)
         nprod = lib().mir_mcc(C.byref(a), n, ptr(sp.cell), ptr(sp.ox), ptr(sp.oy), ptr(sp.vx),
                               ptr(sp.vy), ptr(sp.vz), ptr(sp.id), ptr(self.E), ptr(m["tn"]),
                               ptr(m["nu"]), ptr(ev), ptr(pcell), ptr(pox), ptr(poy), ptr(pe),
-                              ptr(pi))
+                              ptr(pi), ptr(ppid))
         if nprod < 0:
             raise RuntimeError("mirror mcc: probability above 1")
         m["events"] += ev
@@ -432,6 +470,10 @@ class MirSim:
             R = 0
             if m["ion"] is not None and m["ion"] >= 0:
                 R = int(np.rint(sp.w0 / self.species[m["ion"]].w0))
+            # products are stored in particle order but numbered in the order
+            # of their parents' ids (the rank of the parent id)
+            rank = np.empty(nprod, dtype=np.int64)
+            rank[np.argsort(ppid[:nprod], kind="stable")] = np.arange(nprod)
             # electrons appended to the source species
             s0 = sp.np
             k = min(nprod, sp.cap - s0)
@@ -441,7 +483,7 @@ class MirSim:
             sp.vx[s0:s0 + k] = pe[0:3 * k:3]
             sp.vy[s0:s0 + k] = pe[1:3 * k:3]
             sp.vz[s0:s0 + k] = pe[2:3 * k:3]
-            sp.id[s0:s0 + k] = (sp.next_id + np.arange(k)).astype(np.uint32)
+            sp.id[s0:s0 + k] = (sp.next_id + rank[:k]).astype(np.uint32)
             sp.np += k
             sp.next_id += nprod
             if R > 0:
@@ -456,7 +498,8 @@ class MirSim:
                 ion.vx[i0:i0 + kk] = pi[3 * g]
                 ion.vy[i0:i0 + kk] = pi[3 * g + 1]
                 ion.vz[i0:i0 + kk] = pi[3 * g + 2]
-                ion.id[i0:i0 + kk] = (ion.next_id + np.arange(kk)).astype(np.uint32)
+                r_of = np.tile(np.arange(R), nprod)[:kk]
+                ion.id[i0:i0 + kk] = (ion.next_id + rank[g] * R + r_of).astype(np.uint32)
                 ion.np += kk
                 ion.next_id += nprod * R
         return m["last_events"]

@@ -93,7 +93,7 @@ EXPORT void orc_sincos2pi(double u, double *s, double *c) { isk_sincos2pi(u, s, 
 /* reference mode (FP64)                                                    */
 /* ======================================================================== */
 
-/* cell_volume (RegularGrids/src/RegularGrids.jl:106-118); the 1-D version is
+/* cell_volume (RegularGrids/src/RegularGrids.jl:26-38); the 1-D version is
  * inferred by analogy (no 1-D method exists in the reference). */
 EXPORT void ref_cell_volume(int dim, int nx, int ny, double dx, double dy,
                             const int32_t volper[4], double *V) {
@@ -732,35 +732,104 @@ static void m_rotate(double st, double ct, double sp, double cp, double sX, doub
   r[2] = a0 * (-st) + a1 * 0.0 + a2 * ct;
 }
 
-/* isotropic_scattering (mcc.jl:53-62,108-113): chi = 2 pi U, eta = 2 pi U */
-static void m_iso(double vx, double vy, double vz, MirDraws *d, double *r) {
+/* scattering direction [sin chi cos eta, sin chi sin eta, cos chi] *
+ * unrotated(euler_angles(v)) (mcc.jl:53-72,83-127) from two uniforms:
+ * kind 0 is the reference's "isotropic_distribution" (chi = 2 pi u1),
+ * kind 1 its "cosine_distribution" of diffuse_reflection (sin chi = sqrt(u1),
+ * cos chi = -sqrt(1 - sin chi^2)); eta = 2 pi u2 for both.  unrotated is the
+ * reference's matrix as written, which is not orthogonal (its first row holds
+ * -sin phi cos theta), so r is a unit vector only in special cases. */
+EXPORT void mir_scatter(int kind, double vx, double vy, double vz, double u1, double u2,
+                        double *r) {
   double st, ct, sp, cp, sX, cX, sE, cE;
   m_angles(vx, vy, vz, &st, &ct, &sp, &cp);
-  isk_sincos2pi(md_next(d), &sX, &cX);
-  isk_sincos2pi(md_next(d), &sE, &cE);
+  if (kind == 0) {
+    isk_sincos2pi(u1, &sX, &cX);
+  } else {
+    sX = sqrt(u1);
+    cX = -sqrt(1.0 - sX * sX);
+  }
+  isk_sincos2pi(u2, &sE, &cE);
   m_rotate(st, ct, sp, cp, sX, cX, sE, cE, r);
 }
 
-/* diffuse_reflection (mcc.jl:64-72,122-127): sin chi = sqrt(U), cos chi < 0 */
-static void m_diffuse(double vx, double vy, double vz, MirDraws *d, double *r) {
-  double st, ct, sp, cp, sE, cE;
-  m_angles(vx, vy, vz, &st, &ct, &sp, &cp);
-  double sX = sqrt(md_next(d));
-  double cX = -sqrt(1.0 - sX * sX);
-  isk_sincos2pi(md_next(d), &sE, &cE);
-  m_rotate(st, ct, sp, cp, sX, cX, sE, cE, r);
-}
-
-/* maxwellian_velocity / randn(3) (mcc.jl:79-81) via Box-Muller */
-static void m_randn3(MirDraws *d, double *n) {
-  double u1 = md_next(d), u2 = md_next(d), u3 = md_next(d), u4 = md_next(d);
-  double r1 = sqrt(-2.0 * isk_log(u1)), r2 = sqrt(-2.0 * isk_log(u3));
+/* maxwellian_velocity / randn(3) (mcc.jl:79-81) from four uniforms by
+ * Box-Muller: (r1 cos, r1 sin) of the first pair, r2 cos of the second */
+static void m_randn3(const double *u, double *n) {
+  double r1 = sqrt(-2.0 * isk_log(u[0])), r2 = sqrt(-2.0 * isk_log(u[2]));
   double s1, c1, s2, c2;
-  isk_sincos2pi(u2, &s1, &c1);
-  isk_sincos2pi(u4, &s2, &c2);
+  isk_sincos2pi(u[1], &s1, &c1);
+  isk_sincos2pi(u[3], &s2, &c2);
   n[0] = r1 * c1;
   n[1] = r1 * s1;
   n[2] = r2 * c2;
+}
+
+/* One collision of the reference's kernels (mcc.jl:129-229), FP64, with the
+ * uniforms u[] that follow the candidate and process draws, consumed in the
+ * reference's order:
+ *  ELASTIC_ISOTROPIC (0) / ELASTIC_BACKWARD (1) (mcc.jl:129-142,159-172):
+ *     u[0..3] target velocity tv = vth_t randn(3), u[4..5] the direction of
+ *     the relative velocity vr = v - tv; v' = w + mr2 |vr| dir with the
+ *     centre-of-mass velocity w = mr1 v + mr2 tv;
+ *  EXCITATION (2) (mcc.jl:215-229): v' = sqrt(2 (eps - E)/m) iso(v), u[0..1];
+ *  IONIZATION (3) (mcc.jl:174-213): split fraction u[0]; primary
+ *     e1v diffuse(v) with u[1..2]; secondary e2v diffuse(v') with u[3..4]
+ *     (relative to the updated primary); ions' velocity vth_t randn(3) from
+ *     u[5..8];
+ *  INELASTIC_BACKWARD (4) (mcc.jl:144-157): as ELASTIC_BACKWARD with
+ *     |vr| scaled by the uniform u[4] (drawn before the direction, u[5..6]).
+ * out[0..2] velocity after the collision; out[3..5] the target velocity drawn
+ * (elastic kinds) or the ions' velocity (ionisation); out[6..8] the secondary
+ * electron.  Returns the number of uniforms used, or 0 when the energy is
+ * below the threshold (mcc.jl:178-182,219-223: velocity unchanged). */
+EXPORT int mir_coll_kin(int type, double ms, double mt, double m_eV, double energy,
+                        double vth_t, const double *v, const double *u, double *out) {
+  const double v0 = v[0], v1 = v[1], v2 = v[2];
+  if (type == 0 || type == 1 || type == 4) {
+    double mr1 = ms / (ms + mt), mr2 = mt / (ms + mt);
+    double tv[3], vr[3], w[3], dir[3];
+    m_randn3(u, tv);
+    for (int q = 0; q < 3; ++q) tv[q] *= vth_t;
+    for (int q = 0; q < 3; ++q) { vr[q] = v[q] - tv[q]; w[q] = mr1 * v[q] + mr2 * tv[q]; }
+    double nvr = sqrt(vr[0] * vr[0] + vr[1] * vr[1] + vr[2] * vr[2]);
+    int used = 6;
+    if (type == 4) {
+      nvr = u[4] * nvr;
+      mir_scatter(1, vr[0], vr[1], vr[2], u[5], u[6], dir);
+      used = 7;
+    } else {
+      mir_scatter(type, vr[0], vr[1], vr[2], u[4], u[5], dir);
+    }
+    for (int q = 0; q < 3; ++q) {
+      out[q] = w[q] + mr2 * (nvr * dir[q]);
+      out[3 + q] = tv[q];
+    }
+    return used;
+  }
+  double sE = (0.5 * m_eV) * (v0 * v0 + v1 * v1 + v2 * v2) - energy;
+  if (sE < 0.0) return 0;
+  if (type == 2) {
+    double ev = sqrt(2.0 / m_eV) * sqrt(sE);
+    double dir[3];
+    mir_scatter(0, v0, v1, v2, u[0], u[1], dir);
+    for (int q = 0; q < 3; ++q) out[q] = ev * dir[q];
+    return 2;
+  }
+  double e1E = sE * u[0];
+  double e2E = sE - e1E;
+  double al = sqrt(2.0 / m_eV);
+  double e1v = al * sqrt(e1E), e2v = al * sqrt(e2E);
+  double dir[3], dir2[3], tv[3];
+  mir_scatter(1, v0, v1, v2, u[1], u[2], dir);
+  for (int q = 0; q < 3; ++q) out[q] = e1v * dir[q];
+  mir_scatter(1, out[0], out[1], out[2], u[3], u[4], dir2);
+  m_randn3(u + 5, tv);
+  for (int q = 0; q < 3; ++q) {
+    out[6 + q] = e2v * dir2[q];
+    out[3 + q] = tv[q] * vth_t;
+  }
+  return 9;
 }
 
 typedef struct {
@@ -777,12 +846,14 @@ typedef struct {
 /* PIC.perform!(::MonteCarloCollisions) with per-particle candidates
  * (mcc.jl:231-289; kinematics mcc.jl:129-229).  Products (ionisation) are
  * returned in particle order: pcell/pox/poy/pe (3 per product)/pi (3 per
- * product).  nu: nodes x ncoll counts (accumulated), events: per process.
- * Returns the number of products, or -1 if some P_k > 1. */
+ * product)/ppid (the parent's id; nullable).  nu: nodes x ncoll counts
+ * (accumulated), events: per process.  Returns the number of products, or -1
+ * if some P_k > 1. */
 EXPORT int64_t mir_mcc(const MirMcc *a, int64_t np, const int32_t *cell, const float *ox,
                        const float *oy, float *vx, float *vy, float *vz, const uint32_t *id,
                        const float *E, const double *tn, int64_t *nu, int64_t *events,
-                       int32_t *pcell, float *pox, float *poy, float *pe, float *pi) {
+                       int32_t *pcell, float *pox, float *poy, float *pe, float *pi,
+                       uint32_t *ppid) {
   int64_t nprod = 0;
   int err = 0;
   const int N = a->ncoll;
@@ -819,53 +890,25 @@ EXPORT int64_t mir_mcc(const MirMcc *a, int64_t np, const int32_t *cell, const f
     if (!(U > (double)(k + 1) / (double)N - P)) continue;
     nu[(size_t)node * N + k] += 1;
     events[k] += 1;
+    /* the collision's uniforms: Philox blocks 1..5 of the particle */
+    double u[10], sv[3] = {v0, v1, v2}, o[9];
+    for (int q = 0; q < 10; ++q) u[q] = md_next(&d);
     int type = a->types[k];
-    if (type == 0 || type == 1) {
-      double mr1 = a->ms / (a->ms + a->mt), mr2 = a->mt / (a->ms + a->mt);
-      double tv[3], sv[3] = {v0, v1, v2}, vr[3], w[3], dir[3];
-      m_randn3(&d, tv);
-      for (int q = 0; q < 3; ++q) tv[q] *= a->vth_t;
-      for (int q = 0; q < 3; ++q) { vr[q] = sv[q] - tv[q]; w[q] = mr1 * sv[q] + mr2 * tv[q]; }
-      double nvr = sqrt(vr[0] * vr[0] + vr[1] * vr[1] + vr[2] * vr[2]);
-      if (type == 0) m_iso(vr[0], vr[1], vr[2], &d, dir);
-      else m_diffuse(vr[0], vr[1], vr[2], &d, dir);
-      vx[p] = (float)(w[0] + mr2 * (nvr * dir[0]));
-      vy[p] = (float)(w[1] + mr2 * (nvr * dir[1]));
-      vz[p] = (float)(w[2] + mr2 * (nvr * dir[2]));
-    } else if (type == 2) {
-      double sE = (0.5 * a->m_eV) * (v0 * v0 + v1 * v1 + v2 * v2) - a->energy[k];
-      if (sE >= 0.0) {
-        double ev = sqrt(2.0 / a->m_eV) * sqrt(sE);
-        double dir[3];
-        m_iso(v0, v1, v2, &d, dir);
-        vx[p] = (float)(ev * dir[0]);
-        vy[p] = (float)(ev * dir[1]);
-        vz[p] = (float)(ev * dir[2]);
+    if (mir_coll_kin(type, a->ms, a->mt, a->m_eV, a->energy[k], a->vth_t, sv, u, o) == 0)
+      continue;
+    vx[p] = (float)o[0];
+    vy[p] = (float)o[1];
+    vz[p] = (float)o[2];
+    if (type == 3) {
+      pcell[nprod] = c;
+      if (ppid) ppid[nprod] = id[p];
+      pox[nprod] = ox[p];
+      poy[nprod] = a->dim == 2 ? oy[p] : 0.f;
+      for (int q = 0; q < 3; ++q) {
+        pe[3 * nprod + q] = (float)o[6 + q];
+        pi[3 * nprod + q] = (float)o[3 + q];
       }
-    } else if (type == 3) {
-      double sE = (0.5 * a->m_eV) * (v0 * v0 + v1 * v1 + v2 * v2) - a->energy[k];
-      if (sE >= 0.0) {
-        double e1E = sE * md_next(&d);
-        double e2E = sE - e1E;
-        double al = sqrt(2.0 / a->m_eV);
-        double e1v = al * sqrt(e1E), e2v = al * sqrt(e2E);
-        double dir[3], dir2[3], tv[3];
-        m_diffuse(v0, v1, v2, &d, dir);
-        double nv[3] = {e1v * dir[0], e1v * dir[1], e1v * dir[2]};
-        vx[p] = (float)nv[0];
-        vy[p] = (float)nv[1];
-        vz[p] = (float)nv[2];
-        m_diffuse(nv[0], nv[1], nv[2], &d, dir2);
-        m_randn3(&d, tv);
-        pcell[nprod] = c;
-        pox[nprod] = ox[p];
-        poy[nprod] = a->dim == 2 ? oy[p] : 0.f;
-        for (int q = 0; q < 3; ++q) {
-          pe[3 * nprod + q] = (float)(e2v * dir2[q]);
-          pi[3 * nprod + q] = (float)(tv[q] * a->vth_t);
-        }
-        ++nprod;
-      }
+      ++nprod;
     }
   }
   return err ? -1 : nprod;
@@ -892,8 +935,8 @@ EXPORT void mir_sample(int dim, int64_t n, int64_t first_id, double x0, double y
     int i = (int)fl;
     float of = (float)(s - fl);
     if (of >= 1.0f) { of = 0.0f; i += 1; }
-    if (i < 0) i = 0;
-    if (i > ncx - 1) i = ncx - 1;
+    if (i < 0) { i = 0; of = 0.0f; }
+    if (i > ncx - 1) { i = ncx - 1; of = 0x1.fffffep-1f; }
     int j = 0;
     if (dim == 2) {
       double uy = orc_u01(r0[2], r0[3]);
@@ -903,8 +946,8 @@ EXPORT void mir_sample(int dim, int64_t n, int64_t first_id, double x0, double y
       j = (int)fly;
       float ofy = (float)(sy - fly);
       if (ofy >= 1.0f) { ofy = 0.0f; j += 1; }
-      if (j < 0) j = 0;
-      if (j > ncy - 1) j = ncy - 1;
+      if (j < 0) { j = 0; ofy = 0.0f; }
+      if (j > ncy - 1) { j = ncy - 1; ofy = 0x1.fffffep-1f; }
       oy[k] = ofy;
     }
     double u1 = orc_u01(r1[0], r1[1]), u2 = orc_u01(r1[2], r1[3]);

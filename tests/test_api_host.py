@@ -144,3 +144,30 @@ def test_bench_refuses_more_than_one_gpu():
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 2
     assert "one GPU" in r.stderr and r.stdout == ""
+
+
+def test_fluids_the_reference_would_update_are_refused():
+    """advection_diffusion.jl:4-38 (called by advance!(::FluidSpecies),
+    ParticleInCell.jl:74-82) leaves n unchanged for a uniform 1-D fluid, any
+    fluid with mu = 0 in 1-D, and a neutral uniform 2-D fluid; every other
+    fluid would change and is refused before any device work."""
+    g1 = rg.create_uniform_grid(np.linspace(0, 1, 17))
+    g2 = rg.create_uniform_grid(np.linspace(0, 1, 9), np.linspace(0, 1, 9))
+    g2q = rg.create_uniform_grid(np.linspace(0, 1, 9), np.linspace(0, 1, 2))
+    ramp = np.linspace(1.0, 2.0, 17) ** 2
+    static = [(g1, pic.FluidSpecies("bg", 0.0, 1.0, 1.0, ramp, 300.0)),
+              (g1, pic.FluidSpecies("bg", 1e-3, 1.0, 1.0, np.full(17, 3.0), 300.0)),
+              (g2, pic.FluidSpecies("gas", 1e-3, 0.0, 1.0, np.full((9, 9), 3.0), 300.0)),
+              (g2q, pic.FluidSpecies("ions", 1e-3, 1.0, 1.0, np.full((9, 2), 3.0), 300.0))]
+    moving = [(g1, pic.FluidSpecies("bg", 1e-3, 1.0, 1.0, ramp, 300.0)),
+              (g2, pic.FluidSpecies("ions", 0.0, 1.0, 1.0, np.full((9, 9), 3.0), 300.0)),
+              (g2, pic.FluidSpecies("gas", 1e-3, 0.0, 1.0,
+                                    np.outer(np.arange(9.0), np.arange(9.0)) ** 2, 300.0))]
+    for g, f in static:
+        assert pic._fluid_is_static(f, g), f.name
+    for g, f in moving:
+        assert not pic._fluid_is_static(f, g), f.name
+        cfg = pic.Config(grid=g, species=[f])
+        with pytest.raises(L.IskraError) as ei:
+            pic.bind(cfg)
+        assert ei.value.code == L.E_UNSUPPORTED and cfg.ctx is None

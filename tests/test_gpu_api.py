@@ -258,3 +258,63 @@ def test_sort_schedule_does_not_change_results(gpu_ready):
         for a, b in zip(s, s0):
             for k in a:
                 assert np.array_equal(a[k].view(np.uint32), b[k].view(np.uint32)), k
+
+
+def _ccp_run(tile, chunk, sort_every, steps=60, ne=30000):
+    """A 1-D RF discharge with MCC (elastic, excitation, ionisation; ion
+    backscatter) under the given work decomposition; states by id."""
+    from iskra_amd import Context, _lib as L, problems as P
+    p = P.ccp_1d(cells=256, electrons=ne, ions=ne)
+    nodes = p["nodes"]
+    rng = np.random.default_rng(5)
+    ctx = Context()
+    ctx.grid(1, (nodes,), (p["dx"],))
+    ctx.tuning(tile=tile, chunk=chunk, sort_every=sort_every)
+    ctx.rng(9)
+    e = ctx.species(-p["q"], p["me"], p["w0"], 4 * ne, track_ids=True)
+    i = ctx.species(p["q"], p["mi"], p["w0"], 4 * ne, track_ids=True)
+    for s in (e, i):
+        ctx.boundary(s, [L.BND_DISCARD])
+    ve = P.thermal_speed(15.0 * P.QE / P.KB, P.ME)
+    ctx.upload(e, rng.random(ne)[:, None] * p["L"], rng.standard_normal((ne, 3)) * ve)
+    ctx.upload(i, rng.random(ne)[:, None] * p["L"], rng.standard_normal((ne, 3)) * p["vi"])
+    xs = p["xs"]
+    me = ctx.mcc(e, [L.COLL_ELASTIC_ISOTROPIC, L.COLL_EXCITATION, L.COLL_IONIZATION],
+                 xs["energies"], xs["electron"], ion=i, target_m=p["mi"], target_T=p["Tg"])
+    mi = ctx.mcc(i, [L.COLL_ELASTIC_BACKWARD], [0.0], xs["ion"], target_m=p["mi"],
+                 target_T=p["Tg"])
+    for m in (me, mi):
+        ctx.set_field(L.F_TARGET, np.full(nodes, p["ng"]), index=m)
+    tags = np.zeros(nodes, np.int8)
+    tags[0], tags[-1] = 1, 2
+    ctx.poisson(L.POISSON_DIRECT, p["eps0"], tags)
+    ctx.tag_waveform(1, 0.0, p["volts"], p["freq"], 0.0)
+    ctx.step(p["dt"], steps)
+    st = []
+    for s in (e, i):
+        g = ctx.get_state(s)
+        live = g["cell"] >= 0
+        o = np.argsort(g["id"][live], kind="stable")
+        st.append({k: v[live][o] for k, v in g.items()})
+    out = (ctx.field(L.F_RHO), ctx.field(L.F_PHI), st, ctx.mcc_counts(me, 3))
+    ctx.close()
+    return out
+
+
+def test_work_decomposition_does_not_change_results_with_ionisation(gpu_ready):
+    """Ionisation products are numbered in the order of their parents' ids, so
+    neither the sort interval nor the tile and chunk shapes (which change the
+    order particles are stored in) change any result: rho, phi, every
+    particle's state compared by id, and the collision counts, bit for bit."""
+    runs = [_ccp_run(64, 2048, 4), _ccp_run(32, 1024, 16), _ccp_run(128, 4096, 7)]
+    r0, f0, s0, n0 = runs[0]
+    assert len(s0[0]["id"]) > 30000  # ionisation added electrons
+    assert n0[2].sum() > 100
+    for r, f, s, nu in runs[1:]:
+        assert np.array_equal(r.view(np.uint64), r0.view(np.uint64))
+        assert np.array_equal(f.view(np.uint64), f0.view(np.uint64))
+        assert np.array_equal(nu, n0)
+        for a, b in zip(s, s0):
+            assert len(a["id"]) == len(b["id"])
+            for k in a:
+                assert np.array_equal(a[k].view(np.uint32), b[k].view(np.uint32)), k

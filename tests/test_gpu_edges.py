@@ -204,3 +204,29 @@ def test_singular_periodic_pcg(gpu_ready, orc, nx, ny):
     assert abs(phi.mean()) <= 1e-9 * np.abs(x).max()
     assert np.max(np.abs(phi - x)) <= 1e-7 * np.abs(x).max()
     ctx.close()
+
+
+def test_source_box_outside_the_grid_is_refused(gpu_ready):
+    """sample!(::MaxwellianSource) draws x = U wx + dx (sources.jl:24-34); a
+    box reaching past the grid has no cell/offset representation and is
+    refused with ISKRA_E_ARG before anything is drawn; a box of exactly the
+    domain stays inside it."""
+    from iskra_amd import Context, _lib as L
+    ctx = Context()
+    try:
+        ncx, L_ = 300, 0.025
+        dx = L_ / ncx
+        ctx.grid(1, (ncx + 1,), (dx,))
+        s = ctx.species(-1.0, 1.0, 1.0, 10000, track_ids=True)
+        for wx, x0 in (([L_ * 1.01], [0.0]), ([L_ / 2], [L_ * 0.6]), ([L_], [-dx])):
+            with pytest.raises(L.IskraError) as ei:
+                ctx.sample_maxwellian(s, 100, wx, [1.0, 1.0, 1.0], dx=x0, seed=1, stream=0)
+            assert ei.value.code == L.E_ARG
+            assert ctx.count(s) == 0
+        ctx.sample_maxwellian(s, 5000, [L_], [1.0, 1.0, 1.0], seed=1, stream=0)
+        st = ctx.get_state(s)
+        assert len(st["cell"]) == 5000
+        assert st["cell"].min() >= 0 and st["cell"].max() <= ncx - 1
+        assert st["ox"].min() >= 0.0 and st["ox"].max() < 1.0
+    finally:
+        ctx.close()
