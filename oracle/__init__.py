@@ -50,6 +50,17 @@ How each oracle function is pinned (checks live in tests/):
                          updated primary (tests/test_oracle_mcc_kinematics.py).
   mir_*                  against the ref_* functions within FP32 tolerance
                          (tests/test_oracle.py), and bit-exactly against the GPU.
+  poisson_sparse         equal to the dense reference assembly on small grids;
+                         its direct solve reproduces a linear potential.
+  RefSim                 the reference's FP64 time loop composed of the ref_*
+                         operators above; the FP32 twin stays within the stated
+                         per-field bounds of it on the two-stream, slab and
+                         discharge configurations, and a run with a planted
+                         boundary error does not (tests/test_oracle_fp64_loop.py).
+  MirSim (2-D)           the 1-D twin's operators on a 2-D grid; bit for bit
+                         against the device over 40 steps with the device's
+                         potential handed over, that potential within 1e-12 of
+                         the FP64 banded solve (tests/test_gpu_loop.py).
   orc_ws_run             (learned warm start, oracle/ws_oracle.c; no reference
                          counterpart) FP64 gradients against central finite
                          differences; convolution, pooling and upsampling
@@ -224,6 +235,34 @@ def poisson_matrix(dim, nx, ny, dx, dy, periodic=(), dirichlet=()):
     return A, b, isdof
 
 
+def poisson_sparse(nodes, dx, tags, tagval):
+    """The reference system of create_poisson_solver + apply_dirichlet
+    (generalized_poisson.jl:34-68,205-215) on a square grid, as a sparse
+    matrix for grids the dense assembly cannot hold: natural 5-point rows
+    scaled by 1/dx^2, Dirichlet rows replaced by the identity with b = the
+    tag's value; node k = i + nodes*j.  Returns (A, Dirichlet mask, values)."""
+    import scipy.sparse as sps
+    N = nodes * nodes
+    k = np.arange(N)
+    i, j = k % nodes, k // nodes
+    t = np.asarray(tags, np.int8).ravel()
+    free = t == 0
+    rows, cols, vals = [k[~free]], [k[~free]], [np.ones((~free).sum())]
+    diag = np.zeros(N)
+    for off, ok in ((1, i < nodes - 1), (-1, i > 0), (nodes, j < nodes - 1), (-nodes, j > 0)):
+        m = free & ok
+        rows.append(k[m])
+        cols.append(k[m] + off)
+        vals.append(np.full(m.sum(), 1.0 / dx ** 2))
+        diag[m] -= 1.0 / dx ** 2
+    rows.append(k[free])
+    cols.append(k[free])
+    vals.append(diag[free])
+    A = sps.csc_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
+                       shape=(N, N))
+    return A, ~free, np.asarray(tagval)[t]
+
+
 def lu_solve(A, b):
     n = len(b)
     x = np.zeros(n)
@@ -361,24 +400,30 @@ class MirSpecies:
 
 
 class MirSim:
-    """CPU twin of iskra_step for grids whose Poisson solve is the 1-D direct
-    solver: same number format, same operation order, same particle order
-    (stable tile sort every sort_every steps, products appended in parent
-    order and numbered in the order of their parents' ids), same Philox
-    draws."""
+    """CPU twin of iskra_step: same number format, same operation order, same
+    particle order (stable tile sort every sort_every steps, per species when
+    a species sets its own interval; products appended in parent order and
+    numbered in the order of their parents' ids), same Philox draws.  In 1-D
+    the Poisson solve is the library's direct solver, bit for bit.  In 2-D
+    (ny > 1) the potential comes from the reference matrix solved directly in
+    FP64 (banded LU: set_poisson_2d) -- or is set from outside between
+    advance() and end_step() -- and E is formed from it as the device does."""
 
-    def __init__(self, nx, dx, x0=0.0, volper=(0, 0, 0, 0), T=64, sort_every=4, seed=0):
-        self.dim, self.nx, self.ny = 1, nx, 1
-        self.ncx, self.ncy = nx - 1, 1
-        self.dx, self.dy, self.x0 = dx, 1.0, x0
-        self.vol = cell_volume(1, nx, 1, dx, 1.0, volper)
+    def __init__(self, nx, dx, x0=0.0, volper=(0, 0, 0, 0), T=64, sort_every=4, seed=0,
+                 ny=1, dy=1.0):
+        self.dim = 2 if ny > 1 else 1
+        self.nx, self.ny = nx, ny
+        self.ncx, self.ncy = nx - 1, (ny - 1 if self.dim == 2 else 1)
+        self.dx, self.dy, self.x0 = dx, dy, x0
+        self.vol = cell_volume(self.dim, nx, ny, dx, dy, volper)
         self.T, self.sort_every = T, sort_every
         self.ntx = (self.ncx + T - 1) // T
+        nn = nx * ny
         self.species = []
-        self.bg = np.zeros(nx)
-        self.rho = np.zeros(nx)
-        self.phi = np.zeros(nx)
-        self.E = np.zeros(nx, dtype=np.float32)
+        self.bg = np.zeros(nn)
+        self.rho = np.zeros(nn)
+        self.phi = np.zeros(nn)
+        self.E = np.zeros(nn * self.dim, dtype=np.float32)  # (Ex, Ey) per node in 2-D
         self.seed = seed
         self.step_count = 0
         self.mccs = []
@@ -403,6 +448,14 @@ class MirSim:
         rc = lib().mir_tri_factor(n, ptr(self.tags), self.singular, ptr(self.a), ptr(self.cp),
                                   ptr(self.inv))
         assert rc == 0
+
+    def set_poisson_2d(self, eps0, tags, tagval):
+        """Dirichlet tags per node (x fastest) and tag values: the reference
+        system (generalized_poisson.jl:34-68,205-215) solved by banded LU."""
+        self.eps0 = eps0
+        self.tags = np.ascontiguousarray(tags, dtype=np.int8).ravel()
+        self.tagval = np.zeros(17)
+        self.tagval[:len(tagval)] = tagval
 
     def add_mcc(self, src, ion, types, energy, tables, target_m, target_q, target_T, tn):
         off = [0]
@@ -526,10 +579,19 @@ class MirSim:
                       ptr(self.bg), ptr(self.rho))
 
     def solve(self):
+        if self.dim == 2:
+            self.set_phi(potential_banded(self.nx, self.ny, self.dx, self.tags, self.tagval,
+                                          self.rho, self.eps0))
+            return
         lib().mir_tri_solve(self.nx, ptr(self.rho), ptr(self.tags), ptr(self.tagval), ptr(self.a),
                             ptr(self.cp), ptr(self.inv), (self.dx * self.dx) / self.eps0,
                             self.singular, ptr(self.phi))
         lib().mir_efield(1, self.nx, 1, self.dx, 1.0, ptr(self.phi), ptr(self.E))
+
+    def set_phi(self, phi):
+        """Take phi (FP64, x fastest) and form E from it as the device does."""
+        self.phi = np.ascontiguousarray(phi, dtype=np.float64).copy()
+        lib().mir_efield(self.dim, self.nx, self.ny, self.dx, self.dy, ptr(self.phi), ptr(self.E))
 
     def sort(self, s):
         sp = self.species[s]
@@ -546,19 +608,149 @@ class MirSim:
             lib().orc_sincos2pi(u, C.byref(s), C.byref(c))
             self.tagval[tag] = off + amp * s.value
 
+    def advance(self, dt):
+        """The part of one step before the Poisson solve: MCC, push and
+        deposit of every species, rho."""
+        for k in range(len(self.mccs)):
+            self.mcc_run(k, dt)
+        for s in range(len(self.species)):
+            self.push(s, dt)
+        self.compute_rho()
+
+    def end_step(self, dt):
+        """The part after the solve: electrode waveforms, step count, sorts."""
+        self.waveform(self.step_count * dt)
+        self.step_count += 1
+        for s, sp in enumerate(self.species):
+            every = getattr(sp, "sort_every", 0) or self.sort_every
+            if every > 0 and self.step_count % every == 0:
+                self.sort(s)
+
     def step(self, dt, nsteps=1):
         for _ in range(nsteps):
-            for k in range(len(self.mccs)):
-                self.mcc_run(k, dt)
-            for s in range(len(self.species)):
-                self.push(s, dt)
-            self.compute_rho()
+            self.advance(dt)
             self.solve()
-            self.waveform(self.step_count * dt)
+            self.end_step(dt)
+
+
+class RefSim:
+    """The reference's time loop in FP64 (ParticleInCell.jl:84-139) built from
+    the ref_* restatements, with the reference's data model: absolute FP64
+    positions x (D x np), FP64 velocities, weights wg.  One step: for every
+    kinetic species grid_to_particle of E (cloud_in_cell.jl:20-36), the Boris
+    push with B = 0 (pushers.jl:37-50), the after_push boundary (wrap!/discard!
+    with swap-remove, wrap.jl:1-33, kinetic.jl:20-27); then density =
+    particle_to_grid(wg) / cell_volume (kinetic.jl:53), rho = sum q n + the
+    fluid background (ParticleInCell.jl:118-124), calculate_electric_potential
+    (generalized_poisson.jl:372-378) as a direct solve of the reference matrix,
+    calculate_electric_field (:385-410) and the electrode waveform of the
+    after_loop hook at t = (it-1) dt (problem/11_rf_discharge.jl:95).
+
+    Poisson: set_ring_1d() for the 1-D periodic ring (A1; the singular system's
+    minimum-norm solution, i.e. mean-free phi, A2), set_tags() for Dirichlet
+    tags solved by banded LU (any grid size).  No MCC (the reference's
+    collisions draw from Julia's MersenneTwister, not reproducible here)."""
+
+    def __init__(self, dim, nx, dx, ny=1, dy=1.0, eps0=1.0, volper=(0, 0, 0, 0)):
+        self.dim, self.nx, self.ny, self.dx, self.dy, self.eps0 = dim, nx, ny, dx, dy, eps0
+        self.vol = cell_volume(dim, nx, ny, dx, dy, volper)
+        nn = nx * ny
+        self.bg = np.zeros(nn)
+        self.rho = np.zeros(nn)
+        self.phi = np.zeros(nn)
+        self.E = np.zeros(3 * nn)  # component-major, as ref_efield writes it
+        self.species = []
+        self.pinv = None
+        self.tags = None
+        self.tagval = np.zeros(17)
+        self.wf = {}
+        self.step_count = 0
+
+    def set_ring_1d(self):
+        A, _, _ = poisson_matrix(1, self.nx, 1, self.dx, 1.0, periodic=[1])
+        self.pinv = np.linalg.pinv(A.reshape(self.nx, self.nx, order="F"))
+
+    def set_tags(self, tags, tagval=()):
+        self.tags = np.ascontiguousarray(tags, dtype=np.int8).ravel()
+        self.tagval[:] = 0.0
+        self.tagval[:len(tagval)] = tagval
+
+    def add_species(self, q, m, w0, x, v, bnd):
+        """x: (np, D) FP64 positions, v: (np, 3); bnd per dimension (0 none,
+        1 wrap, 2 discard)."""
+        x = np.asarray(x, dtype=np.float64).reshape(len(v), -1)
+        n = len(v)
+        self.species.append(dict(
+            q=q, m=m, w0=w0, np=n, bnd=tuple(bnd),
+            x=np.ascontiguousarray(x.T), v=np.ascontiguousarray(np.asarray(v, np.float64).T),
+            wg=np.full(n, float(w0)), id=np.arange(1, n + 1, dtype=np.uint32)))
+        return len(self.species) - 1
+
+    def _advance_species(self, sp, dt):
+        D, n = self.dim, sp["np"]
+        if n == 0:
+            return
+        x = np.ascontiguousarray(sp["x"][:, :n])
+        v = np.ascontiguousarray(sp["v"][:, :n])
+        pE = grid_to_particle(D, x.ravel(), self.nx, self.ny, self.dx, self.dy, self.E)
+        B = np.zeros(3 * n)
+        xf, vf = x.ravel().copy(), v.ravel().copy()
+        ref_push(xf, vf, pE, B, sp["q"] / sp["m"], dt, D)
+        x, v = xf.reshape(D, n), vf.reshape(3, n)
+        nodes = np.array([self.nx, self.ny], dtype=np.int64)
+        dh = np.array([self.dx, self.dy])
+        org = np.zeros(2)
+        wrapd = np.array([d for d in range(D) if sp["bnd"][d] == 1], dtype=np.int32)
+        if len(wrapd):
+            xf = np.ascontiguousarray(x).ravel()
+            lib().ref_wrap(n, D, ptr(xf), ptr(nodes), ptr(dh), ptr(org), len(wrapd), ptr(wrapd))
+            x = xf.reshape(D, n)
+        disd = np.array([d for d in range(D) if sp["bnd"][d] == 2], dtype=np.int32)
+        if len(disd):
+            xf = np.ascontiguousarray(x).ravel()
+            vf = np.ascontiguousarray(v).ravel()
+            wg = np.ascontiguousarray(sp["wg"][:n])
+            ids = np.ascontiguousarray(sp["id"][:n])
+            npr = C.c_int64(n)
+            lib().ref_discard(C.byref(npr), n, D, ptr(xf), ptr(vf), ptr(wg), ptr(ids), sp["w0"],
+                              ptr(nodes), ptr(dh), ptr(org), len(disd), ptr(disd))
+            n = npr.value
+            x, v = xf.reshape(D, -1)[:, :n], vf.reshape(3, -1)[:, :n]
+            sp["wg"], sp["id"] = wg[:n].copy(), ids[:n].copy()
+        sp["x"], sp["v"], sp["np"] = np.ascontiguousarray(x), np.ascontiguousarray(v), n
+
+    def density(self, s):
+        sp = self.species[s]
+        n = sp["np"]
+        x = np.ascontiguousarray(sp["x"][:, :n]).ravel()
+        return particle_to_grid(self.dim, x, sp["wg"][:n], self.nx, self.ny, self.dx,
+                                self.dy) / self.vol
+
+    def step(self, dt, nsteps=1):
+        for _ in range(nsteps):
+            for sp in self.species:
+                self._advance_species(sp, dt)
+            rho = np.zeros(self.nx * self.ny)
+            for s, sp in enumerate(self.species):
+                rho += sp["q"] * self.density(s)
+            self.rho = rho + self.bg
+            if self.pinv is not None:
+                self.phi = self.pinv @ (-self.rho / self.eps0)
+            else:
+                self.phi = potential_banded(self.nx, self.ny, self.dx, self.tags, self.tagval,
+                                            self.rho, self.eps0)
+            self.E = efield(self.dim, self.nx, self.ny, self.dx, self.dy, self.phi)
+            t = self.step_count * dt
+            for tag, (off, amp, f, ph) in self.wf.items():
+                self.tagval[tag] = off + amp * np.sin(2.0 * np.pi * (f * t + ph))
             self.step_count += 1
-            if self.sort_every > 0 and self.step_count % self.sort_every == 0:
-                for s in range(len(self.species)):
-                    self.sort(s)
+
+    def moments(self, s):
+        """np, sum x (per dimension), sum v, sum |v|^2 of species s."""
+        sp = self.species[s]
+        n = sp["np"]
+        return dict(np=n, x=sp["x"][:, :n].sum(axis=1), v=sp["v"][:, :n].sum(axis=1),
+                    v2=float((sp["v"][:, :n] ** 2).sum()))
 
 
 # --------------------------------------------------------------------------

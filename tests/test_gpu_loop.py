@@ -123,3 +123,67 @@ def test_two_stream_loop_bitexact(gpu_ready, orc):
         assert np.array_equal(ctx.field(L.F_RHO), sim.rho)
     s = ctx.stats()
     assert s["steps"] == 100 and s["particle_steps"] == 100 * n
+
+
+def test_slab_2d_loop_bitexact(gpu_ready, orc):
+    """The 2-D time loop over 40 steps against the CPU twin: each step the
+    device's rho must equal the twin's bit for bit; the device's potential
+    (spectral solve) must match the FP64 banded solve of that rho to 1e-12
+    and is then handed to the twin, whose E = -grad phi must equal the
+    device's bit for bit; every 8 steps the particle states (electrons
+    sorted every 4 steps, ions every 16) must be equal bit for bit."""
+    from iskra_amd import Context, _lib as L, problems as P
+    p = P.slab_2d(cells=64, ppc=8)
+    nodes, dx, n = p["nodes"], p["dx"], p["n_part"]
+    ctx = Context()
+    ctx.grid(2, (nodes, nodes), (dx, dx))
+    ctx.tuning(tile=8, chunk=1024, margin=2, sort_every=4)
+    tags = np.zeros((nodes, nodes), np.int8)
+    tags[0, :] = tags[-1, :] = tags[:, 0] = tags[:, -1] = 1
+    ctx.poisson(L.POISSON_SPECTRAL, p["eps0"], tags.ravel(), rtol=1e-10)
+    ctx.tag_value(1, 0.0)
+    sim = O_MirSim(orc, nodes, dx)
+    sim.set_poisson_2d(p["eps0"], tags, [0.0, 0.0])
+    sids = []
+    try:
+        for k, (q, m, vth) in enumerate([(-p["q"], p["me"], p["ve"]), (p["q"], p["mi"], p["vi"])]):
+            s = ctx.species(q, m, p["w0"], n, track_ids=True)
+            ctx.boundary(s, [L.BND_DISCARD, L.BND_DISCARD])
+            ctx.sample_maxwellian(s, n, [p["L"]] * 2, [vth] * 3, seed=11, stream=k)
+            st = ctx.get_state(s)
+            sp = orc.MirSpecies(2, n, q, m, p["w0"], bnd=(2, 2))
+            sp.set_state(st["cell"], st["ox"], st["oy"], st["vx"], st["vy"], st["vz"], st["id"])
+            sim.add_species(sp)
+            sids.append(s)
+        ctx.species_sort_every(sids[1], 16)
+        sim.species[1].sort_every = 16
+        for it in range(40):
+            ctx.step(p["dt"], 1)
+            sim.advance(p["dt"])
+            rho = ctx.field(L.F_RHO)
+            assert np.array_equal(rho.view(np.uint64), sim.rho.view(np.uint64)), it
+            phi = ctx.field(L.F_PHI)
+            ref = orc.potential_banded(nodes, nodes, dx, tags, [0.0, 0.0], rho, p["eps0"])
+            assert np.max(np.abs(phi - ref)) <= 1e-12 * np.max(np.abs(ref)), it
+            sim.set_phi(phi)
+            assert np.array_equal(ctx.field(L.F_EX), sim.E[0::2].astype(np.float64)), it
+            assert np.array_equal(ctx.field(L.F_EY), sim.E[1::2].astype(np.float64)), it
+            sim.end_step(p["dt"])
+            if it % 8 == 7:
+                for k in range(2):
+                    _assert_species_equal_2d(ctx, sim, k, f"2-D step {it + 1}")
+    finally:
+        ctx.close()
+
+
+def O_MirSim(orc, nodes, dx):
+    return orc.MirSim(nodes, dx, T=8, sort_every=4, ny=nodes, dy=dx)
+
+
+def _assert_species_equal_2d(ctx, sim, sid, what):
+    st = ctx.get_state(sid)
+    sp = sim.species[sid]
+    assert len(st["cell"]) == sp.np, (what, len(st["cell"]), sp.np)
+    for k in ("cell", "ox", "oy", "vx", "vy", "vz", "id"):
+        a, b = st[k], getattr(sp, k)[:sp.np]
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), (what, sid, k)

@@ -485,3 +485,38 @@ def test_step_deposit_2d_two_species_bitexact(gpu_ready, orc, ion_margin):
         r = r + q * dens
     assert np.array_equal(ctx.field(L.F_RHO).view(np.uint64), (r + bg).view(np.uint64))
     ctx.close()
+
+
+@pytest.mark.parametrize("kind", [1, 2, 3, 4])
+def test_poisson_513_vs_sparse_direct(gpu_ready, orc, kind):
+    """Every 2-D solver at the 513^2-node grid of the learned-warm-start
+    configuration, to relative residual 1e-8 (FP64), against a sparse direct
+    solve (SuperLU) of the reference system, with a particle-noise charge
+    density and a biased wall."""
+    import scipy.sparse.linalg as spla
+    from iskra_amd import Context, _lib as L
+    nodes, dx, eps0 = 513, 1e-4, 8.854e-12
+    rng = np.random.default_rng(513 + kind)
+    tags = np.zeros((nodes, nodes), np.int8)
+    tags[0, :] = tags[-1, :] = tags[:, 0] = 1
+    tags[:, -1] = 2
+    vals = [0.0, 0.0, -3.0]
+    rho = rng.standard_normal(nodes * nodes) * 1e-6 + 2e-6 * np.sin(
+        np.linspace(0, 3 * np.pi, nodes * nodes))
+    A, dmask, dval = orc.poisson_sparse(nodes, dx, tags, vals)
+    b = np.where(dmask, dval, -rho / eps0)
+    ref = spla.spsolve(A, b)
+    ctx = Context()
+    try:
+        ctx.grid(2, (nodes, nodes), (dx, dx))
+        ctx.poisson(kind, eps0, tags.ravel(), rtol=1e-8, max_iter=20000)
+        ctx.tag_value(2, -3.0)
+        ctx.set_field(L.F_RHO, rho)
+        it, rr = ctx.solve(L.WARM_ZERO)
+        print(f"kind {kind}: {it} iterations, relres {rr:.2e}", flush=True)
+        assert rr <= 1e-8, (it, rr)
+        phi = ctx.field(L.F_PHI)
+        scale = np.max(np.abs(ref))
+        assert np.max(np.abs(phi - ref)) <= 1e-6 * scale, np.max(np.abs(phi - ref)) / scale
+    finally:
+        ctx.close()

@@ -101,3 +101,42 @@ def test_ws_training_cuts_pcg_iterations(gpu_ready):
     assert np.mean(learned) < np.mean(zero)
     ws.close()
     ctx.close()
+
+
+def test_ws_cfg5_network_against_oracle(gpu_ready):
+    """The learned warm start's own architecture (4 levels, 32-256 channels,
+    3x3 convolutions, BF16 activations and weights, FP32 accumulation):
+    forward and every layer's weight gradient at 256^2 pixels, and the forward
+    pass at 512^2 (the 513^2-node grid of the configuration), against the FP64
+    oracle in its BF16-rounding mode."""
+    import oracle as O
+    from iskra_amd import Context, _lib as Lb
+    from iskra_amd import warmstart as W
+    L, base = 4, 32
+    p = W.he_init(L, base, 5)
+    for S, grads in ((256, True), (512, False)):
+        ctx = Context(0)
+        ctx.grid(2, (S + 1, S + 1), (1.0, 1.0))
+        ws = Lb.WarmStart(ctx, levels=L, base=base, side=S, max_batch=1)
+        try:
+            ws.set_params(p)
+            rng = np.random.default_rng(S)
+            x = rng.standard_normal((1, S, S)).astype(np.float32)
+            t = rng.standard_normal((1, S, S)).astype(np.float32)
+            out = ws.forward(x)
+            ref, lref, gref = O.ws_run(L, base, p.astype(np.float64), x.astype(np.float64),
+                                       t.astype(np.float64) if grads else None, round_bf16=True,
+                                       grads=grads)
+            e = _relerr(out, ref)
+            print(f"{S}^2 forward relerr {e:.2e}", flush=True)
+            assert e < 5e-3
+            if grads:
+                loss, g = ws.backward(x, t)
+                assert loss == pytest.approx(lref, rel=1e-3)
+                for k, (off, n, cin, cout) in enumerate(W.layer_slices(L, base)):
+                    ek = _relerr(g[off:off + n], gref[off:off + n])
+                    print(f"  layer {k} {cin}->{cout} grad relerr {ek:.2e}", flush=True)
+                    assert ek < 3e-2, (k, ek)
+        finally:
+            ws.close()
+            ctx.close()

@@ -440,3 +440,31 @@ def test_fixed_point_weights_equal_fp64_rounding(orc):
         for b in vals[1::7]:
             exact = np.float64(a) * np.float64(b)  # 48-bit product: exact in FP64
             assert lib.orc_fixprod32(float(a), float(b)) == int(np.rint(exact * 2.0 ** 32))
+
+
+def test_sparse_reference_matrix_is_the_dense_assembly(orc):
+    """poisson_sparse (used where the dense matrix is too large) equals the
+    dense assembly of the reference matrix with its Dirichlet rows
+    (generalized_poisson.jl:34-68,205-215) on a small grid, and its direct
+    solve satisfies the manufactured closed form of a linear potential."""
+    nodes, dx = 9, 0.3
+    tags = np.zeros((nodes, nodes), np.int8)
+    tags[0, :] = tags[-1, :] = tags[:, 0] = 1
+    tags[:, -1] = 2
+    A, dmask, dval = orc.poisson_sparse(nodes, dx, tags, [0.0, 0.0, 2.0])
+    Ad, b, isdof = orc.poisson_matrix(2, nodes, nodes, dx, dx, dirichlet=[
+        (tags.T == 1, 0.0), (tags.T == 2, 2.0)])
+    N = nodes * nodes
+    assert np.array_equal(A.toarray(), Ad.reshape(N, N, order="F"))
+    assert np.array_equal(isdof == 0, dmask) and np.array_equal(b[dmask], dval[dmask])
+    # phi = 2 j/(n-1) on walls x = 0 / x = L grounded ... a linear ramp in j
+    # between j = 0 (0 V) and j = n-1 (2 V) with the side walls tagged along
+    import scipy.sparse.linalg as spla
+    jj = np.repeat(np.arange(nodes), nodes)
+    ramp = 2.0 * jj / (nodes - 1)
+    tags2 = np.zeros((nodes, nodes), np.int8)
+    tags2[0, :], tags2[-1, :] = 1, 2
+    A2, m2, _ = orc.poisson_sparse(nodes, dx, tags2, [0.0, 0.0, 2.0])
+    b2 = np.where(m2, np.where(tags2.ravel() == 2, 2.0, 0.0), 0.0)
+    phi = spla.spsolve(A2, b2)
+    assert np.allclose(phi, ramp, atol=1e-12)
