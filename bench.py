@@ -37,10 +37,56 @@ PEAK_HBM_GBS = 8000.0
 BYTES_PER_PSTEP_2D = 40  # read+write cell, ox, oy, vx, vy (4 B each)
 
 
-def cpu_baseline(cells, budget_s=20.0, sample=1 << 22, max_steps=100):
+def cpu_baseline(cells, budget_s=10.0):
     """The oracle's FP64 restatement of the reference's per-step particle work
     (grid_to_particle, push_in_cartesian! with B = 0, discard!, particle_to_grid)
-    on one host thread, for a bounded sample of electrons on the same grid."""
+    for a bounded sample of electrons on the same grid: on all the host cores
+    this process may use (OpenMP, ref_particles_omp), and on one thread (the
+    reference is single-threaded)."""
+    threads = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
+    allc = cpu_baseline_omp(cells, threads, budget_s)
+    one = cpu_baseline_1thread(cells, budget_s)
+    allc["single_thread"] = one
+    return allc
+
+
+def cpu_baseline_omp(cells, threads, budget_s, sample=1 << 23, max_steps=100):
+    import oracle as O
+    from iskra_amd import problems as P
+    p = P.slab_2d(cells=cells)
+    nx = cells + 1
+    dx = p["dx"]
+    rng = np.random.default_rng(0)
+    L = cells * dx
+    x = np.ascontiguousarray(np.concatenate([rng.random(sample) * L, rng.random(sample) * L]))
+    v = np.ascontiguousarray(rng.standard_normal(3 * sample) * p["ve"])
+    E = np.ascontiguousarray(rng.standard_normal(3 * nx * nx) * 10.0)
+    E[2 * nx * nx:] = 0.0
+    wg = np.full(sample, p["w0"])
+    u = np.zeros(nx * nx)
+    qm = -P.QE / P.ME
+    lib = O.lib()
+    n, steps, psteps = sample, 0, 0
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < budget_s and steps < max_steps:
+        m = lib.ref_particles_omp(n, O.ptr(x), O.ptr(v), O.ptr(wg), nx, nx, dx, dx, O.ptr(E), qm,
+                                  p["dt"], O.ptr(u), threads)
+        psteps += n
+        steps += 1
+        if m != n:  # keep leading dimension = count for the next step
+            x = np.ascontiguousarray(np.concatenate([x[:m], x[n:n + m]]))
+            v = np.ascontiguousarray(np.concatenate([v[:m], v[n:n + m], v[2 * n:2 * n + m]]))
+            wg = wg[:m].copy()
+            n = m
+    dt = time.perf_counter() - t0
+    return dict(value=psteps / dt, unit="particle-steps/s", cores=threads, kind="port",
+                sample=(f"{sample} electrons on the {cells}x{cells}-cell grid, {steps} steps of "
+                        f"the FP64 reference algorithm (gather, push B=0, discard, CIC deposit) "
+                        f"on {threads} OpenMP threads, {dt:.1f} s; Poisson excluded (the "
+                        f"reference's dense A\\b at 257^2 nodes is infeasible)"))
+
+
+def cpu_baseline_1thread(cells, budget_s, sample=1 << 21, max_steps=100):
     import oracle as O
     from iskra_amd import problems as P
     p = P.slab_2d(cells=cells)
@@ -96,23 +142,26 @@ def cpu_baseline(cells, budget_s=20.0, sample=1 << 22, max_steps=100):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    # 192 timed steps: three ion sort cycles (every 64 steps) and sixteen
-    # electron sort cycles (every 12), so the sorts enter at their average rate
-    ap.add_argument("--steps", type=int, default=192)
+    ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=8)
     ap.add_argument("--cells", type=int, default=256)
     ap.add_argument("--ppc", type=int, default=256)
     ap.add_argument("--solver", default="spectral", choices=["spectral", "mg", "mgpcg", "pcg"])
     ap.add_argument("--tile", type=int, default=0)
     ap.add_argument("--chunk", type=int, default=0)
-    # electrons re-sorted every 12 steps with a 3-cell window margin: 2.6 %
-    # faster per step than every 8 steps with 2 cells (the sort costs less
-    # than the slightly larger deposit windows add); 0 / -1 = library default
+    # electrons re-sorted about every 12 steps with a 3-cell window margin:
+    # 2.6 % faster per step than every 8 steps with 2 cells (the sort costs
+    # less than the slightly larger deposit windows add).  The interval used
+    # is the largest divisor of --steps not above this, so the timed steps
+    # always hold whole sort cycles (--steps 20: every 10); 0 / -1 = library
+    # default
     ap.add_argument("--margin", type=int, default=3)
     ap.add_argument("--sort-every", type=int, default=12)
-    # ions at 0.03 eV move ~3e-4 cells per step here: they stay tile-ordered
-    # for hundreds of steps, so they are re-sorted far less often
-    ap.add_argument("--ion-sort-every", type=int, default=64)
+    # ions at 0.03 eV drift ~4e-4 cells per step here (0.04 cells over the
+    # configuration's 100 steps), so they keep the order of the initial sort
+    # for the whole run (0: not re-sorted; those crossing a tile edge deposit
+    # through the fallback grid)
+    ap.add_argument("--ion-sort-every", type=int, default=0)
     # and their deposit windows need no margin (-1: the electrons' margin)
     ap.add_argument("--ion-margin", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -131,11 +180,17 @@ def main():
     p = P.slab_2d(cells=args.cells, ppc=args.ppc)
     nodes = p["nodes"]
     n_species = p["n_part"]
+    # whole electron sort cycles in the timed steps: the largest divisor of
+    # --steps not above --sort-every (any window of --steps steps then holds
+    # exactly --steps / interval sorts, whatever the warm-up)
+    sort_every = args.sort_every
+    if sort_every > 0 and args.steps > 0:
+        sort_every = max(d for d in range(1, sort_every + 1) if args.steps % d == 0)
+    ion_every = args.ion_sort_every if args.ion_sort_every > 0 else 1 << 30
     t_setup = time.perf_counter()
     ctx = Context(0)
     ctx.grid(2, (nodes, nodes), (p["dx"], p["dx"]))
-    ctx.tuning(tile=args.tile, chunk=args.chunk, margin=args.margin,
-               sort_every=args.sort_every)
+    ctx.tuning(tile=args.tile, chunk=args.chunk, margin=args.margin, sort_every=sort_every)
     tags = np.zeros((nodes, nodes), np.int8)
     tags[0, :] = tags[-1, :] = tags[:, 0] = tags[:, -1] = 1
     kinds = dict(spectral=L.POISSON_SPECTRAL, mg=L.POISSON_MG, mgpcg=L.POISSON_MGPCG,
@@ -148,7 +203,7 @@ def main():
     i = ctx.species(p["q"], p["mi"], p["w0"], cap, track_ids=False)
     ctx.boundary(e, [L.BND_DISCARD, L.BND_DISCARD])
     ctx.boundary(i, [L.BND_DISCARD, L.BND_DISCARD])
-    ctx.species_sort_every(i, args.ion_sort_every)
+    ctx.species_sort_every(i, ion_every)
     ctx.species_margin(i, args.ion_margin)
     Lx = p["L"]
     ctx.sample_maxwellian(e, n_species, [Lx, Lx], [p["ve"]] * 3, seed=1, stream=0)
@@ -193,8 +248,11 @@ def main():
             "cells": args.cells,
             "particles_per_species": n_species,
             "poisson": args.solver,
-            "tuning": dict(ctx.get_tuning(), ion_sort_every=args.ion_sort_every,
-                           ion_margin=args.ion_margin),
+            "tuning": dict(ctx.get_tuning(),
+                           ion_sort_every=args.ion_sort_every if args.ion_sort_every > 0
+                           else "not re-sorted in the run", ion_margin=args.ion_margin),
+            "sorts_in_timed_steps": {"electrons": args.steps // sort_every if sort_every > 0
+                                     else 0, "ions": args.steps // ion_every},
         },
         "poisson": {"solver": args.solver, "iters_per_solve": iters / max(solves, 1),
                     "rtol": 1e-8},

@@ -139,6 +139,8 @@ def _declare(L):
         "mir_efield": (None, [C.c_int, C.c_int, C.c_int, D, D, P, P]),
         "mir_mcc": (I64, [C.POINTER(MirMcc), I64, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P,
                           P, P]),
+        "ref_particles_omp": (I64, [I64, P, P, P, C.c_int, C.c_int, D, D, P, D, D, P,
+                                    C.c_int]),
         "mir_scatter": (None, [C.c_int, D, D, D, D, D, P]),
         "mir_coll_kin": (C.c_int, [C.c_int, D, D, D, D, D, P, P, P]),
         "mir_sample": (None, [C.c_int, I64, I64, D, D, D, D, C.c_int, C.c_int, P, P, P, P, U64,

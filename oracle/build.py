@@ -2,7 +2,7 @@
 
 Compiles oracle/oracle.c and oracle/ws_oracle.c into oracle/liboracle.so with gcc, floating-point
 contraction off so that FP32/FP64 operation order is exactly what the source
-says.  The reference implementation itself is Julia and cannot be built here
+says, and OpenMP for the all-cores CPU baseline (ref_particles_omp).  The reference implementation itself is Julia and cannot be built here
 (no Julia toolchain), so there is no oracle/_ref build.
 """
 import os
@@ -16,7 +16,8 @@ LIB = os.path.join(HERE, "liboracle.so")
 def build(verbose=False):
     tmp = LIB + ".tmp"
     cmd = ["gcc", "-O2", "-std=c11", "-fPIC", "-shared", "-ffp-contract=off",
-           "-fno-fast-math", "-Wall", "-Wno-unused-function", "-o", tmp] + SRC + ["-lm"]
+           "-fno-fast-math", "-fopenmp", "-Wall", "-Wno-unused-function", "-o", tmp] + SRC + [
+               "-lm"]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)

@@ -32,6 +32,9 @@
 #include <string.h>
 
 #include "../iskra_amd/csrc/isk_fp64math.h"
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 #define EXPORT __attribute__((visibility("default")))
 #define FIX_ONE 4294967296.0
@@ -194,6 +197,93 @@ EXPORT void ref_push(int64_t np, int D, double *x, double *v, const double *E, c
 
 /* Julia fld(a, b) for b > 0 */
 static inline double jl_fld(double a, double b) { return floor(a / b); }
+
+/* The per-step particle work of the reference on a 2-D grid -- grid_to_particle
+ * of E (cloud_in_cell.jl:20-36), push_in_cartesian! with B = 0
+ * (pushers.jl:37-50), discard! on both axes (wrap.jl:1-18) and
+ * particle_to_grid of wg (cloud_in_cell.jl:1-18) -- with the same FP64
+ * arithmetic per particle as ref_grid_to_particle / ref_push / ref_discard /
+ * ref_particle_to_grid, spread over nthreads OpenMP threads: particles in
+ * parallel, the deposit into one grid per thread summed afterwards, discarded
+ * particles replaced by survivors from the end of the arrays (the set the
+ * reference's swap-remove leaves; the order may differ).  CPU baseline only.
+ * x: 2 x np
+ * and v: 3 x np column-major with leading dimension np; E: nx x ny x 3; u
+ * (nx x ny) receives the deposit.  Returns the particles left. */
+EXPORT int64_t ref_particles_omp(int64_t np, double *x, double *v, double *wg, int nx, int ny,
+                                 double dx, double dy, const double *E, double qm, double dt,
+                                 double *u, int nthreads) {
+  const size_t nn = (size_t)nx * ny;
+  const double Lx = (double)(nx - 1) * dx, Ly = (double)(ny - 1) * dy;
+  unsigned char *keep = (unsigned char *)malloc((size_t)(np > 0 ? np : 1));
+  double *part = (double *)calloc(nn * (size_t)nthreads, sizeof(double));
+  if (!keep || !part) {
+    free(keep);
+    free(part);
+    return -1;
+  }
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+  for (int64_t p = 0; p < np; ++p) {
+    int i, j;
+    double hx, hy;
+    ref_particle_cell(x[p], dx, &i, &hx);
+    ref_particle_cell(x[np + p], dy, &j, &hy);
+    double pe[3];
+    for (int c = 0; c < 3; ++c) {
+      const double *g = E + c * nn;
+      pe[c] = (1.0 - hx) * (1.0 - hy) * g[j * nx + i] + (hx) * (1.0 - hy) * g[j * nx + i + 1] +
+              (1.0 - hx) * (hy)*g[(j + 1) * nx + i] + (hx) * (hy)*g[(j + 1) * nx + i + 1];
+    }
+    for (int c = 0; c < 3; ++c) {
+      double vm = 0.5 * dt * qm * pe[c] + v[c * np + p];
+      v[c * np + p] = dt * pe[c] * qm * 0.5 + vm;
+    }
+    for (int c = 0; c < 2; ++c) x[c * np + p] = dt * v[c * np + p] + x[c * np + p];
+    keep[p] = jl_fld(x[p], Lx) == 0.0 && jl_fld(x[np + p], Ly) == 0.0;
+  }
+  /* removal as remove! does it (kinetic.jl:20-27): a discarded slot below
+   * the new count takes a surviving particle from above it */
+  int64_t m = 0;
+#pragma omp parallel for num_threads(nthreads) reduction(+ : m) schedule(static)
+  for (int64_t p = 0; p < np; ++p) m += keep[p];
+  for (int64_t h = 0, t = m; h < m; ++h) {
+    if (keep[h]) continue;
+    while (!keep[t]) ++t;
+    for (int c = 0; c < 2; ++c) x[c * np + h] = x[c * np + t];
+    for (int c = 0; c < 3; ++c) v[c * np + h] = v[c * np + t];
+    wg[h] = wg[t];
+    ++t;
+  }
+#pragma omp parallel num_threads(nthreads)
+  {
+#ifdef _OPENMP
+    const int t = omp_get_thread_num();
+#else
+    const int t = 0;
+#endif
+    double *ut = part + (size_t)t * nn;
+#pragma omp for schedule(static)
+    for (int64_t p = 0; p < m; ++p) {
+      int i, j;
+      double hx, hy;
+      ref_particle_cell(x[p], dx, &i, &hx);
+      ref_particle_cell(x[np + p], dy, &j, &hy);
+      const double w = wg[p];
+      ut[j * nx + i] += (1.0 - hx) * (1.0 - hy) * w;
+      ut[j * nx + i + 1] += (hx) * (1.0 - hy) * w;
+      ut[(j + 1) * nx + i] += (1.0 - hx) * (hy)*w;
+      ut[(j + 1) * nx + i + 1] += (hx) * (hy)*w;
+    }
+  }
+  for (size_t k = 0; k < nn; ++k) {
+    double s = 0.0;
+    for (int t = 0; t < nthreads; ++t) s += part[(size_t)t * nn + k];
+    u[k] = s;
+  }
+  free(keep);
+  free(part);
+  return m;
+}
 
 /* wrap! (ParticleInCell/src/pic/surfaces/wrap.jl:20-33) over the listed dims */
 EXPORT void ref_wrap(int64_t np, int D, double *x, const int64_t *n, const double *dh,

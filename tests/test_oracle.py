@@ -468,3 +468,56 @@ def test_sparse_reference_matrix_is_the_dense_assembly(orc):
     b2 = np.where(m2, np.where(tags2.ravel() == 2, 2.0, 0.0), 0.0)
     phi = spla.spsolve(A2, b2)
     assert np.allclose(phi, ramp, atol=1e-12)
+
+
+def test_openmp_baseline_is_the_reference_particle_step(orc):
+    """ref_particles_omp (the all-cores CPU baseline) does per particle what
+    ref_grid_to_particle + ref_push (B = 0) + ref_discard + ref_particle_to_grid
+    do: the same particles kept with the same positions and velocities bit for
+    bit (compared as sets: both remove by moving survivors from the end), and
+    the deposit equal up to the order of the per-thread sums."""
+    from oracle import ptr
+    rng = np.random.default_rng(8)
+    nx = ny = 33
+    dx = dy = 0.1
+    Lx = (nx - 1) * dx
+    n = 20000
+    E = np.ascontiguousarray(rng.standard_normal(3 * nx * ny))
+    E[2 * nx * ny:] = 0.0
+    x0 = np.concatenate([rng.random(n) * Lx, rng.random(n) * Lx])
+    v0 = rng.standard_normal(3 * n) * 2.0
+    qm, dt = -1.7, 0.05
+    outs = []
+    for threads in (1, 4):
+        x, v, wg = x0.copy(), v0.copy(), np.full(n, 0.5)
+        u = np.zeros(nx * ny)
+        m = orc.lib().ref_particles_omp(n, ptr(x), ptr(v), ptr(wg), nx, ny, dx, dy, ptr(E), qm,
+                                        dt, ptr(u), threads)
+        outs.append((m, x[:m].copy(), x[n:n + m].copy(), v.reshape(3, n)[:, :m].copy(), u))
+    # the sequential reference path
+    x, v = x0.copy(), v0.copy()
+    pE = np.zeros(3 * n)
+    orc.lib().ref_grid_to_particle(2, n, ptr(x), nx, ny, dx, dy, ptr(E), ptr(pE))
+    orc.ref_push(x, v, pE, np.zeros(3 * n), qm, dt, 2)
+    keep = (np.floor(x[:n] / Lx) == 0) & (np.floor(x[n:] / Lx) == 0)
+    assert 0 < keep.sum() < n
+    xk, yk = x[:n][keep], x[n:][keep]
+    vk = v.reshape(3, n)[:, keep]
+    u_ref = orc.particle_to_grid(2, np.concatenate([xk, yk]), np.full(keep.sum(), 0.5), nx, ny,
+                                 dx, dy)
+    ok = np.lexsort((yk, xk))
+    for m, xo, yo, vo, u in outs:
+        assert m == keep.sum()
+        o = np.lexsort((yo, xo))
+        assert np.array_equal(xo[o], xk[ok]) and np.array_equal(yo[o], yk[ok])
+        assert np.array_equal(vo[:, o], vk[:, ok])
+        assert np.allclose(u, u_ref, rtol=1e-13, atol=1e-13 * u_ref.max())
+    # the reference's own discard keeps the same set (in swap-remove order)
+    xs, vs = x.copy(), v.copy()
+    wg, ids = np.full(n, 0.5), np.arange(n, dtype=np.uint32)
+    npr = np.array([n], dtype=np.int64)
+    orc.lib().ref_discard(ptr(npr), n, 2, ptr(xs), ptr(vs), ptr(wg), ptr(ids), 0.5,
+                          ptr(np.array([nx, ny], dtype=np.int64)), ptr(np.array([dx, dy])),
+                          ptr(np.zeros(2)), 2, ptr(np.array([0, 1], dtype=np.int32)))
+    assert npr[0] == keep.sum()
+    assert np.array_equal(np.sort(xs[:npr[0]]), np.sort(xk))
