@@ -155,48 +155,89 @@ __device__ __forceinline__ void tri_solve_body(const TriArgs &t) {
   for (int i = 0; i < n; ++i) t.phi[i] = t.phi[i] - m2;
 }
 
-/* The same solve for a whole work-group of 64: with n <= TRI_LDS_MAX and
- * Dirichlet nodes (not the singular ring), the coefficients and the
- * right-hand side are staged in LDS by all lanes, lane 0 runs the two
- * recurrences out of LDS (the same fma sequence as above, so the same bits),
- * and all lanes store phi.  Otherwise lane 0 runs tri_solve_body. */
-#define TRI_LDS_MAX 2048
+/* The same system solved by a whole work-group of TRI_BLOCK threads when it
+ * has Dirichlet nodes (not the singular ring) and n <= TRI_BLOCK * TRI_K.
+ * Both recurrences of the factored system are first-order affine maps,
+ * forward y_i = A_i y_(i-1) + B_i with A_i = -a_i inv_i, B_i = r_i inv_i, and
+ * backward x_i = C_i x_(i+1) + D_i with C_i = -cp_i, D_i = y_i, so each is a
+ * prefix composition of maps, (A2, B2) after (A1, B1) = (A2 A1, fma(A2, B1,
+ * B2)): every thread composes its TRI_K consecutive maps, a Hillis-Steele
+ * scan over the TRI_BLOCK segment maps gives each segment its incoming value,
+ * and every element is then one fma away.  log2(TRI_BLOCK) + 2 TRI_K
+ * dependent steps instead of 3n.  The operations are fixed, so the oracle
+ * restates them bit for bit.  Otherwise thread 0 runs tri_solve_body. */
+#define TRI_BLOCK 256
+#define TRI_K 8
+
+/* one prefix composition of n affine maps (ma[i], mb[i]) applied to 0;
+ * rev != 0 scans from the last element down.  out[i] = value at element i. */
+__device__ __forceinline__ void tri_scan(const double *ma, const double *mb, int n, int rev,
+                                         double *out, double *ga, double *gb) {
+  const int tid = threadIdx.x, j0 = tid * TRI_K;
+  double la[TRI_K], lb[TRI_K];
+  double ca = 1.0, cb = 0.0;
+#pragma unroll
+  for (int k = 0; k < TRI_K; ++k) {
+    const int j = j0 + k;
+    if (j < n) {
+      const int i = rev ? n - 1 - j : j;
+      const double A = ma[i], B = mb[i];
+      cb = __builtin_fma(A, cb, B);
+      ca = A * ca;
+    }
+    la[k] = ca;
+    lb[k] = cb;
+  }
+  ga[tid] = ca;
+  gb[tid] = cb;
+  __syncthreads();
+  for (int d = 1; d < TRI_BLOCK; d <<= 1) {
+    double pa = 1.0, pb = 0.0;
+    if (tid >= d) {
+      pa = ga[tid - d];
+      pb = gb[tid - d];
+    }
+    __syncthreads();
+    if (tid >= d) {
+      const double a2 = ga[tid], b2 = gb[tid];
+      gb[tid] = __builtin_fma(a2, pb, b2);
+      ga[tid] = a2 * pa;
+    }
+    __syncthreads();
+  }
+  const double vin = tid > 0 ? gb[tid - 1] : 0.0;
+#pragma unroll
+  for (int k = 0; k < TRI_K; ++k) {
+    const int j = j0 + k;
+    if (j < n) out[rev ? n - 1 - j : j] = __builtin_fma(la[k], vin, lb[k]);
+  }
+  __syncthreads(); /* out, ga and gb are read or rewritten next */
+}
+
 __device__ __forceinline__ void tri_solve_group(const TriArgs &t) {
   const int n = t.n;
-  if (t.singular || n > TRI_LDS_MAX) {
+  if (t.singular || n > TRI_BLOCK * TRI_K) {
     if (threadIdx.x == 0) tri_solve_body(t);
     return;
   }
-  __shared__ double sa[TRI_LDS_MAX], sinv[TRI_LDS_MAX], scp[TRI_LDS_MAX], sr[TRI_LDS_MAX];
+  __shared__ double sA[TRI_BLOCK * TRI_K], sB[TRI_BLOCK * TRI_K], sy[TRI_BLOCK * TRI_K];
+  __shared__ double ga[TRI_BLOCK], gb[TRI_BLOCK];
   for (int i = threadIdx.x; i < n; i += blockDim.x) {
     const int tg = t.tags[i];
-    sr[i] = tg ? t.tagval[tg] : -(t.rho[i] * t.scale);
-    sa[i] = t.a[i];
-    sinv[i] = t.inv[i];
-    scp[i] = t.cp[i];
+    const double r = tg ? t.tagval[tg] : -(t.rho[i] * t.scale);
+    const double inv = t.inv[i];
+    sA[i] = -t.a[i] * inv;
+    sB[i] = r * inv;
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
-    double yprev = 0.0;
-#pragma unroll 8
-    for (int i = 0; i < n; ++i) {
-      const double y = __builtin_fma(-sa[i], yprev, sr[i]) * sinv[i];
-      sr[i] = y;
-      yprev = y;
-    }
-    double xn = 0.0;
-#pragma unroll 8
-    for (int i = n - 1; i >= 0; --i) {
-      const double x = __builtin_fma(-scp[i], xn, sr[i]);
-      sr[i] = x;
-      xn = x;
-    }
-  }
+  tri_scan(sA, sB, n, 0, sy, ga, gb);
+  for (int i = threadIdx.x; i < n; i += blockDim.x) sA[i] = -t.cp[i];
   __syncthreads();
-  for (int i = threadIdx.x; i < n; i += blockDim.x) t.phi[i] = sr[i];
+  tri_scan(sA, sy, n, 1, sB, ga, gb);
+  for (int i = threadIdx.x; i < n; i += blockDim.x) t.phi[i] = sB[i];
 }
 
-__global__ __launch_bounds__(64) void k_tri_solve(TriArgs t) {
+__global__ __launch_bounds__(TRI_BLOCK) void k_tri_solve(TriArgs t) {
   if (blockIdx.x != 0) return;
   tri_solve_group(t);
 }
@@ -1652,7 +1693,7 @@ int isk_poisson_solve_ex(iskra_ctx *c, int warm, int *iters, double *relres, int
   if (p.kind == ISKRA_POISSON_DIRECT) {
     TriArgs t;
     tri_prepare(c, &t);
-    hipLaunchKernelGGL(k_tri_solve, dim3(1), dim3(64), 0, c->stream, t);
+    hipLaunchKernelGGL(k_tri_solve, dim3(1), dim3(TRI_BLOCK), 0, c->stream, t);
     ISK_CHECK_LAUNCH();
     p.last_iters = 1;
     p.last_relres = 0.0;
@@ -1835,7 +1876,7 @@ int isk_poisson_solve_ex(iskra_ctx *c, int warm, int *iters, double *relres, int
 }
 
 /* ---- batched launches ---- */
-__global__ __launch_bounds__(64) void kb_tri(const TriArgs *A, const int *bstart, int n) {
+__global__ __launch_bounds__(TRI_BLOCK) void kb_tri(const TriArgs *A, const int *bstart, int n) {
   const int i = isk_bfind(bstart, n, blockIdx.x);
   tri_solve_group(A[i]);
 }
@@ -1856,7 +1897,7 @@ int isk_bbuild_tri(iskra_ctx **cs, int n, IskBLaunch *L) {
     }
     tri_prepare(cs[i], &A[i]);
   }
-  return isk_blaunch_upload(L, ISK_B_TRI, A.data(), sizeof(TriArgs), one, 64, 0);
+  return isk_blaunch_upload(L, ISK_B_TRI, A.data(), sizeof(TriArgs), one, TRI_BLOCK, 0);
 }
 
 int isk_bbuild_efield(iskra_ctx **cs, int n, IskBLaunch *L) {

@@ -49,7 +49,7 @@ def _timed(step, sync, nsteps):
     return time.perf_counter() - t0
 
 
-def run(name, make, nsteps, warmup=8):
+def run(name, make, nsteps, warmup=8, batch=True):
     out = {"config": name, "steps": nsteps}
     ctx, dt = make()
     ctx.step(dt, warmup)
@@ -58,6 +58,10 @@ def run(name, make, nsteps, warmup=8):
     ps = ctx.stats()["particle_steps"] - s0
     out["eager"] = {"s": t, "ms_per_step": 1e3 * t / nsteps, "particle_steps_per_s": ps / t}
     counts = [ctx.count(s) for s in range(2) if s < _nspecies(ctx)]
+    if not batch:
+        out["particles_final"] = counts
+        ctx.close()
+        return out
     ctx2, _ = make()
     try:
         b = Batch([ctx2])
@@ -92,11 +96,16 @@ def _nspecies(ctx):
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--out", default=None)
+    ap.add_argument("--no-batch", action="store_true",
+                    help="time the single-context path only (no one-member batch)")
+    ap.add_argument("--steps", type=int, default=0, help="steps of each run (0: 500 / 2000)")
     args = ap.parse_args(argv)
-    res = [run("two-stream 1-D, 64 cells, 20,000 electrons", two_stream_context, 500),
+    b = not args.no_batch
+    res = [run("two-stream 1-D, 64 cells, 20,000 electrons", two_stream_context,
+               args.steps or 500, batch=b),
            run("RF discharge 1-D with MCC, 512 cells, 1M e- + 1M ions",
                lambda: ccp_context(dict(cells=512, particles=1_000_000, volts=200.0, torr=1.0,
-                                        seed=1)), 2000)]
+                                        seed=1)), args.steps or 2000, batch=b)]
     for r in res:
         print(json.dumps(r), flush=True)
     if args.out:

@@ -718,11 +718,70 @@ EXPORT int mir_tri_factor(int n, const int8_t *tags, int singular, double *a, do
   return 0;
 }
 
+/* The device's work-group solve of the factored system (systems with
+ * Dirichlet nodes and n <= MIR_TRI_T * MIR_TRI_K): each recurrence is a
+ * prefix composition of affine maps (A, B), (A2, B2) after (A1, B1) =
+ * (A2 A1, fma(A2, B1, B2)), computed as segments of MIR_TRI_K maps per
+ * thread, an inclusive Hillis-Steele scan over the MIR_TRI_T segment maps
+ * (every step reads the previous step's values), and one fma per element. */
+#define MIR_TRI_T 256
+#define MIR_TRI_K 8
+static void mir_tri_scan(const double *ma, const double *mb, int n, int rev, double *out) {
+  double la[MIR_TRI_T][MIR_TRI_K], lb[MIR_TRI_T][MIR_TRI_K];
+  double ga[MIR_TRI_T], gb[MIR_TRI_T], pa[MIR_TRI_T], pb[MIR_TRI_T];
+  for (int t = 0; t < MIR_TRI_T; ++t) {
+    double ca = 1.0, cb = 0.0;
+    for (int k = 0; k < MIR_TRI_K; ++k) {
+      int j = t * MIR_TRI_K + k;
+      if (j < n) {
+        int i = rev ? n - 1 - j : j;
+        cb = fma(ma[i], cb, mb[i]);
+        ca = ma[i] * ca;
+      }
+      la[t][k] = ca;
+      lb[t][k] = cb;
+    }
+    ga[t] = ca;
+    gb[t] = cb;
+  }
+  for (int d = 1; d < MIR_TRI_T; d <<= 1) {
+    for (int t = 0; t < MIR_TRI_T; ++t) {
+      pa[t] = t >= d ? ga[t - d] : 1.0;
+      pb[t] = t >= d ? gb[t - d] : 0.0;
+    }
+    for (int t = d; t < MIR_TRI_T; ++t) {
+      double a2 = ga[t], b2 = gb[t];
+      gb[t] = fma(a2, pb[t], b2);
+      ga[t] = a2 * pa[t];
+    }
+  }
+  for (int t = 0; t < MIR_TRI_T; ++t) {
+    double vin = t > 0 ? gb[t - 1] : 0.0;
+    for (int k = 0; k < MIR_TRI_K; ++k) {
+      int j = t * MIR_TRI_K + k;
+      if (j < n) out[rev ? n - 1 - j : j] = fma(la[t][k], vin, lb[t][k]);
+    }
+  }
+}
+
 EXPORT void mir_tri_solve(int n, const double *rho, const int8_t *tags, const double *tagval,
                           const double *a, const double *cp, const double *inv, double scale,
                           int singular, double *phi) {
   double *y = (double *)malloc(sizeof(double) * n);
-  if (!singular) {
+  if (!singular && n <= MIR_TRI_T * MIR_TRI_K) {
+    double *mA = (double *)malloc(sizeof(double) * n), *mB = (double *)malloc(sizeof(double) * n);
+    for (int i = 0; i < n; ++i) {
+      int tg = tags[i];
+      double r = tg ? tagval[tg] : -(rho[i] * scale);
+      mA[i] = -a[i] * inv[i];
+      mB[i] = r * inv[i];
+    }
+    mir_tri_scan(mA, mB, n, 0, y);
+    for (int i = 0; i < n; ++i) mA[i] = -cp[i];
+    mir_tri_scan(mA, y, n, 1, phi);
+    free(mA);
+    free(mB);
+  } else if (!singular) {
     double yprev = 0.0;
     for (int i = 0; i < n; ++i) {
       int tg = tags[i];
