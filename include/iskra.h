@@ -63,6 +63,26 @@ int iskra_sync(iskra_ctx *ctx);
 int iskra_grid_set(iskra_ctx *ctx, int dim, const int64_t nodes[2],
                    const double dh[2], const double origin[2],
                    const int32_t vol_periodic[4]);
+/* RegularGrids.create_axial_grid (RegularGrids/src/RegularGrids.jl:84-97):
+ * a 2-D (r, z) grid, axis 0 radial with the symmetry axis at node i = 0
+ * (r = i dr), axis 1 axial.  Node volumes are the annuli of
+ * cell_volume(::AxialGrid{2}) (RegularGrids.jl:40-53), halved on a z edge
+ * unless vol_periodic_z[0] (bottom) / [1] (top) declares it :periodic.
+ * Particles, deposit and gather work in (r, z) exactly as in (x, y)
+ * (cloud_in_cell.jl:38-73); iskra_poisson_set assembles the axisymmetric
+ * operator of generalized_poisson.jl:70-199.  Resets species, solver and
+ * fields, and selects the RZ pusher. */
+int iskra_grid_set_axial(iskra_ctx *ctx, const int64_t nodes[2],
+                         const double dh[2], const double origin[2],
+                         const int32_t vol_periodic_z[2]);
+/* Boris pusher geometry (ParticleInCell/src/pic/pushers.jl:4-17): XY is
+ * create_boris_pusher (push_in_cartesian!), RZ is create_axial_boris_pusher
+ * (push_in_cartesian! followed by transform_from_cartesian_to_cylindrical!,
+ * pushers.jl:52-66: the particle moves in the plane through its position
+ * and the axis, then is rotated back into the (r, z) half plane together with
+ * (v_r, v_theta) = (v_x, v_z)).  RZ needs a 2-D grid. */
+enum { ISKRA_PUSHER_XY = 0, ISKRA_PUSHER_RZ = 1 };
+int iskra_pusher_set(iskra_ctx *ctx, int kind);
 
 /* Work-decomposition parameters (the autotuner's knobs).  tile: cells per
  * tile edge (power of two); chunk: particles per work-group; margin: window
@@ -183,7 +203,15 @@ int iskra_field_upload(iskra_ctx *ctx, int which, int index, const double *in);
  * sine transform, which diagonalises the operator when all four walls are
  * Dirichlet and no interior node is: FP64 dense transforms, then the FP64
  * residual is checked and corrected until it meets rtol).  eps0 in F/m; rtol on ||b-A phi||2/||b||2
- * evaluated in FP64. */
+ * evaluated in FP64.
+ * On an axial grid (iskra_grid_set_axial) the operator is the reference's
+ * axisymmetric one (generalized_poisson.jl:70-199), which becomes symmetric
+ * once row i is multiplied by w_i = i (w_0 = 1/2 on the axis, w_{nr-1} =
+ * nr - 3/2 on the outer wall): radial links of weight (i + 1/2)/dr^2 between
+ * i and i+1, axial links of weight w_i/dz^2.  Only PCG solves it (on that
+ * weighted system, relres measured there); it needs a Dirichlet node and no
+ * periodic link (the reference's apply_periodic for RZ cannot run:
+ * generalized_poisson.jl:326-361 reads undefined nr, nz, dr, dz). */
 enum {
   ISKRA_POISSON_DIRECT = 0,
   ISKRA_POISSON_PCG = 1,
@@ -369,6 +397,21 @@ int iskra_ws_train(iskra_ws *net, int64_t first, int64_t count, int batch,
 int iskra_ws_eval(iskra_ws *net, int64_t first, int64_t count, double *mse);
 /* phi at the interior nodes of the context <- prediction from its rho */
 int iskra_ws_predict(iskra_ws *net);
+/* calculate_electric_potential (generalized_poisson.jl:372-378) with the
+ * network inside the iteration: flexible conjugate gradients on the
+ * context's system (each new direction A-orthogonalised against the previous
+ * one), every iteration preconditioned by z = the network applied to the
+ * residual (scaled by its RMS and the gain), followed when omega > 0 by one
+ * damped Jacobi sweep z += omega/4 (r - A z) for the high frequencies the
+ * network leaves.  The system must be the network's: phi = 0 on all four
+ * walls, no other Dirichlet node and no periodic link (ISKRA_E_UNSUPPORTED
+ * otherwise).  Iterates until the FP64 relative residual meets the solver's
+ * rtol (ISKRA_E_CONVERGE after max_iter iterations; 0: the solver's
+ * max_iter); warm: ISKRA_WARM_ZERO or ISKRA_WARM_PREV (the current phi, for
+ * example the network's own prediction).  Leaves phi like
+ * iskra_poisson_solve. */
+int iskra_ws_solve(iskra_ws *net, int warm, double omega, int max_iter,
+                   int *iters, double *relres);
 
 /* ---- diagnostics ---------------------------------------------------------- */
 /* Device-to-device copy of `bytes` bytes, `iters` times, with the library's

@@ -18,6 +18,7 @@ OK, E_ARG, E_HIP, E_CAPACITY, E_PROB, E_STATE, E_CONVERGE, E_UNSUPPORTED = range
 BND_NONE, BND_WRAP, BND_DISCARD = 0, 1, 2
 F_RHO, F_PHI, F_EX, F_EY, F_BG, F_DENSITY, F_TARGET = range(7)
 POISSON_DIRECT, POISSON_PCG, POISSON_MG, POISSON_MGPCG, POISSON_SPECTRAL = range(5)
+PUSHER_XY, PUSHER_RZ = 0, 1
 WARM_ZERO, WARM_PREV = 0, 1
 COLL_ELASTIC_ISOTROPIC, COLL_ELASTIC_BACKWARD, COLL_EXCITATION, COLL_IONIZATION = range(4)
 COLL_INELASTIC_BACKWARD = 4
@@ -25,7 +26,7 @@ NOALLOC_SCRATCH_BYTES = 4 << 20
 
 EXPORTS = [
     "iskra_last_error", "iskra_version", "iskra_ctx_create", "iskra_ctx_destroy", "iskra_sync",
-    "iskra_grid_set", "iskra_tuning_set", "iskra_tuning_get", "iskra_species_add",
+    "iskra_grid_set", "iskra_grid_set_axial", "iskra_pusher_set", "iskra_tuning_set", "iskra_tuning_get", "iskra_species_add",
     "iskra_species_set_boundary", "iskra_species_sort_every", "iskra_species_margin",
     "iskra_species_upload",
     "iskra_species_download",
@@ -41,6 +42,7 @@ EXPORTS = [
     "iskra_ws_get_params", "iskra_ws_set_gain", "iskra_ws_forward", "iskra_ws_backward",
     "iskra_ws_dataset_reserve", "iskra_ws_capture", "iskra_ws_dataset_size",
     "iskra_ws_dataset_rms", "iskra_ws_train", "iskra_ws_eval", "iskra_ws_predict",
+    "iskra_ws_solve",
     "iskra_batch_create", "iskra_batch_destroy", "iskra_batch_set_graph", "iskra_batch_step",
     "iskra_batch_graphs_launched",
 ]
@@ -88,6 +90,8 @@ def lib():
             "iskra_ctx_destroy": (I, [P]),
             "iskra_sync": (I, [P]),
             "iskra_grid_set": (I, [P, I, P, P, P, P]),
+            "iskra_grid_set_axial": (I, [P, P, P, P, P]),
+            "iskra_pusher_set": (I, [P, I]),
             "iskra_tuning_set": (I, [P, C.POINTER(Tuning)]),
             "iskra_tuning_get": (I, [P, C.POINTER(Tuning)]),
             "iskra_species_add": (I, [P, D, D, D, I64, I, C.POINTER(I)]),
@@ -137,6 +141,7 @@ def lib():
             "iskra_ws_train": (I, [P, I64, I64, I, I, C.c_float, U64, P]),
             "iskra_ws_eval": (I, [P, I64, I64, C.POINTER(D)]),
             "iskra_ws_predict": (I, [P]),
+            "iskra_ws_solve": (I, [P, I, D, I, C.POINTER(I), C.POINTER(D)]),
             "iskra_batch_create": (I, [P, I, C.POINTER(P)]),
             "iskra_batch_destroy": (I, [P]),
             "iskra_batch_set_graph": (I, [P, I]),
@@ -225,6 +230,19 @@ class Context:
         check(lib().iskra_grid_set(self.h, dim, _p(n), _p(d), _p(o), _p(vp)))
         self.dim = dim
         self.nodes = (int(n[0]), int(n[1]) if dim == 2 else 1)
+
+    def grid_axial(self, nodes, dh, origin=(0.0, 0.0), vol_periodic_z=(0, 0)):
+        """(r, z) grid: nodes (nr, nz), dh (dr, dz); selects the RZ pusher."""
+        n = np.asarray(nodes, dtype=np.int64)
+        d = np.asarray(dh, dtype=np.float64)
+        o = np.asarray(origin, dtype=np.float64)
+        vp = np.asarray(vol_periodic_z, dtype=np.int32)
+        check(lib().iskra_grid_set_axial(self.h, _p(n), _p(d), _p(o), _p(vp)))
+        self.dim = 2
+        self.nodes = (int(n[0]), int(n[1]))
+
+    def pusher(self, kind):
+        check(lib().iskra_pusher_set(self.h, int(kind)))
 
     @property
     def nn(self):
@@ -537,6 +555,14 @@ class WarmStart:
 
     def predict(self):
         check(lib().iskra_ws_predict(self.h))
+
+    def solve(self, warm=WARM_ZERO, omega=0.8, max_iter=0):
+        """Flexible CG on the context's system preconditioned by the network
+        (plus one damped Jacobi sweep when omega > 0); returns (iters, relres)."""
+        it, rr = C.c_int(), C.c_double()
+        check(lib().iskra_ws_solve(self.h, int(warm), float(omega), int(max_iter), C.byref(it),
+                                   C.byref(rr)))
+        return it.value, rr.value
 
 
 class Batch:

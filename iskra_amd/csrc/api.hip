@@ -363,9 +363,8 @@ static int ilog2(int v) {
   return l;
 }
 
-int iskra_grid_set(iskra_ctx *c, int dim, const int64_t nodes[2], const double dh[2],
-                   const double origin[2], const int32_t vol_periodic[4]) {
-  CHECK_CTX(c);
+static int grid_setup(iskra_ctx *c, int dim, const int64_t nodes[2], const double dh[2],
+                      const double origin[2], const int32_t volper[4], int axial) {
   if (dim != 1 && dim != 2) {
     isk_set_error("grid: dim must be 1 or 2");
     return ISKRA_E_ARG;
@@ -391,7 +390,9 @@ int iskra_grid_set(iskra_ctx *c, int dim, const int64_t nodes[2], const double d
   g.dy = dim == 2 ? dh[1] : 1.0;
   g.x0 = origin ? origin[0] : 0.0;
   g.y0 = (origin && dim == 2) ? origin[1] : 0.0;
-  for (int k = 0; k < 4; ++k) g.volper[k] = vol_periodic ? vol_periodic[k] : 0;
+  for (int k = 0; k < 4; ++k) g.volper[k] = volper ? volper[k] : 0;
+  g.axial = axial;
+  c->pusher = axial ? ISKRA_PUSHER_RZ : ISKRA_PUSHER_XY;
   /* default tiling */
   if (dim == 2) {
     c->t.T = 8;
@@ -420,14 +421,31 @@ int iskra_grid_set(iskra_ctx *c, int dim, const int64_t nodes[2], const double d
   ISK_HIP(hipMemset(c->bg, 0, nn * 8));
   ISK_HIP(hipMemset(c->E, 0, (size_t)nn * 4 * dim));
   /* cell_volume (RegularGrids.jl:26-38); 1-D by analogy: dx, halved at
-   * non-periodic ends */
+   * non-periodic ends.  Axial grid (RegularGrids.jl:40-53): the annulus
+   * pi dz ((i dr - dr/2)^2 - (i dr - 3dr/2)^2) of 1-based node i, a disc of
+   * radius dr/2 on the axis, the outer half annulus from (nr-3/2) dr to
+   * (nr-1) dr, each halved on a non-periodic z edge */
   std::vector<double> V(nn);
   double V0 = dim == 2 ? g.dx * g.dy : g.dx;
   for (int j = 0; j < g.ny; ++j)
     for (int i = 0; i < g.nx; ++i) {
       double v = V0;
-      if (i == 0 && !g.volper[0]) v *= 0.5;
-      if (i == g.nx - 1 && !g.volper[1]) v *= 0.5;
+      if (axial) {
+        const double dr = g.dx, dz = g.dy, pdz = M_PI * dz;
+        const int i1 = i + 1, nr = g.nx;
+        if (i == 0) {
+          v = pdz * ((0.5 * dr) * (0.5 * dr));
+        } else if (i == nr - 1) {
+          const double a = nr * dr - 1.0 * dr, b = nr * dr - 1.5 * dr;
+          v = pdz * (a * a - b * b);
+        } else {
+          const double a = i1 * dr - 0.5 * dr, b = i1 * dr - 1.5 * dr;
+          v = pdz * (a * a - b * b);
+        }
+      } else {
+        if (i == 0 && !g.volper[0]) v *= 0.5;
+        if (i == g.nx - 1 && !g.volper[1]) v *= 0.5;
+      }
       if (dim == 2) {
         if (j == 0 && !g.volper[2]) v *= 0.5;
         if (j == g.ny - 1 && !g.volper[3]) v *= 0.5;
@@ -439,6 +457,40 @@ int iskra_grid_set(iskra_ctx *c, int dim, const int64_t nodes[2], const double d
   c->step_count = 0;
   c->t_now = 0.0;
   memset(&c->stats, 0, sizeof(c->stats));
+  return ISKRA_OK;
+}
+
+int iskra_grid_set(iskra_ctx *c, int dim, const int64_t nodes[2], const double dh[2],
+                   const double origin[2], const int32_t vol_periodic[4]) {
+  CHECK_CTX(c);
+  return grid_setup(c, dim, nodes, dh, origin, vol_periodic, 0);
+}
+
+int iskra_grid_set_axial(iskra_ctx *c, const int64_t nodes[2], const double dh[2],
+                         const double origin[2], const int32_t vol_periodic_z[2]) {
+  CHECK_CTX(c);
+  if (!nodes || !dh) {
+    isk_set_error("grid_set_axial: nodes and dh are required");
+    return ISKRA_E_ARG;
+  }
+  /* the radial bcs are (:other, :other): no radial halving flags */
+  int32_t vp[4] = {0, 0, vol_periodic_z ? vol_periodic_z[0] : 0,
+                   vol_periodic_z ? vol_periodic_z[1] : 0};
+  return grid_setup(c, 2, nodes, dh, origin, vp, 1);
+}
+
+int iskra_pusher_set(iskra_ctx *c, int kind) {
+  CHECK_CTX(c);
+  CHECK_GRID(c);
+  if (kind != ISKRA_PUSHER_XY && kind != ISKRA_PUSHER_RZ) {
+    isk_set_error("pusher_set: unknown kind %d", kind);
+    return ISKRA_E_ARG;
+  }
+  if (kind == ISKRA_PUSHER_RZ && c->g.dim != 2) {
+    isk_set_error("pusher_set: the RZ pusher needs a 2-D grid");
+    return ISKRA_E_ARG;
+  }
+  c->pusher = kind;
   return ISKRA_OK;
 }
 

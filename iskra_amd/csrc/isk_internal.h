@@ -63,6 +63,7 @@ struct IskGrid {
   double dx, dy;
   double x0, y0;
   int volper[4];
+  int axial; /* (r, z) grid: axis 0 radial, symmetry axis at i = 0 */
 };
 
 struct IskTune {
@@ -169,6 +170,7 @@ struct iskra_ctx {
   hipStream_t stream = nullptr;
   int have_grid = 0;
   IskGrid g;
+  int pusher = ISKRA_PUSHER_XY;
   IskTune t;
   int ntx = 1, nty = 1, ntiles = 1;
   int nspecies = 0;
@@ -330,7 +332,17 @@ struct Sten {
   double cxp, cyp;
   const int8_t *tags;
   const double *tagval;
+  /* axisymmetric operator (rows multiplied by w_i): radial link i -- i+1 of
+   * weight i + 1/2, axial links of weight w_i az, az = dr^2 / dz^2, with
+   * w_0 = 1/2, w_i = i, w_{nx-1} = nx - 3/2 */
+  int axial;
+  double az;
 };
+
+/* row weight w_i of the axisymmetric operator */
+__host__ __device__ __forceinline__ double isk_rz_w(int i, int nx) {
+  return i == 0 ? 0.5 : (i == nx - 1 ? (double)nx - 1.5 : (double)i);
+}
 
 /* sum over the Dirichlet neighbours of c * value (moves known values to the rhs) */
 __device__ __forceinline__ double dirichlet_rhs(const Sten &s, int i, int j) {
@@ -340,6 +352,14 @@ __device__ __forceinline__ double dirichlet_rhs(const Sten &s, int i, int j) {
   if (cond) {                                                    \
     int t = s.tags[m];                                           \
     if (t) acc += (coef) * s.tagval[t];                          \
+  }
+  if (s.axial) {
+    const double cz = isk_rz_w(i, s.nx) * s.az;
+    DIRNB(i > 0, n - 1, (double)i - 0.5)
+    DIRNB(i < s.nx - 1, n + 1, (double)i + 0.5)
+    DIRNB(j > 0, n - s.nx, cz)
+    DIRNB(j < s.ny - 1, n + s.nx, cz)
+    return acc;
   }
   DIRNB(i > 0, n - 1, 1.0)
   DIRNB(i < s.nx - 1, n + 1, 1.0)

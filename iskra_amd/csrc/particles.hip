@@ -724,6 +724,8 @@ int isk_sort_species(iskra_ctx *c, int sid, int exact_cell) {
 struct PushArgs {
   int32_t *cell;
   float *ox, *oy, *vx, *vy;
+  float *vz;     /* RZ pusher: v_theta, rotated with v_r */
+  double dtdr;   /* RZ pusher: dt / dr */
   long long *hdr;
   const IskChunk *chunks;
   long long *partial;
@@ -774,6 +776,44 @@ __device__ __forceinline__ int advance_coord(float v, float dtd, float *o) {
   return (int)f;
 }
 
+/* transform_from_cartesian_to_cylindrical! (pushers.jl:52-66) in cell units,
+ * after the Cartesian step of the radial coordinate: the particle sits at
+ * X = i + hx + v_r dt/dr in the plane through its old position and the axis
+ * and has moved Y = v_theta dt/dr out of it; it is rotated back to radius
+ * R = sqrt(X^2 + Y^2) with sin = Y/R (0 when R = 0, the reference's
+ * r .~ 0 test) and cos = +sqrt(1 - sin^2) (also for X < 0, as the reference
+ * takes it), and (v_r, v_theta) rotated by the same angle.  FP64 throughout
+ * (correctly rounded sqrt and division); R beyond 2^30 cells maps to a cell
+ * index no grid has. */
+__device__ __forceinline__ void rz_transform(int *i, float *hx, float *vr, float *vt,
+                                             double dtdr) {
+  const double X = (double)*i + ((double)*hx + (double)*vr * dtdr);
+  const double Y = (double)*vt * dtdr;
+  const double R = sqrt(X * X + Y * Y);
+  const double sn = R == 0.0 ? 0.0 : Y / R;
+  const double cs = sqrt(1.0 - sn * sn);
+  const double nvr = cs * (double)*vr + sn * (double)*vt;
+  const double nvt = cs * (double)*vt - sn * (double)*vr;
+  int ii;
+  float h;
+  if (R < 1073741824.0) {
+    const double fl = floor(R);
+    ii = (int)fl;
+    h = (float)(R - fl);
+    if (h >= 1.0f) {
+      h = 0.0f;
+      ++ii;
+    }
+  } else {
+    ii = 1 << 30;
+    h = 0.0f;
+  }
+  *i = ii;
+  *hx = h;
+  *vr = (float)nvr;
+  *vt = (float)nvt;
+}
+
 /* boundary on one axis; returns false if the particle is discarded */
 __device__ __forceinline__ bool apply_bnd(int *i, int n, int mode) {
   if (*i >= 0 && *i < n) return true;
@@ -817,7 +857,7 @@ __device__ __forceinline__ void push_load(PushRegs<DIM> &r, const int32_t *__res
   }
 }
 
-template <int DIM>
+template <int DIM, bool RZ = false>
 __device__ __forceinline__ void push_body(const PushArgs &a, const int blk) {
   constexpr int BLK = PushShape<DIM>::block, UNR = PushShape<DIM>::unroll;
   extern __shared__ unsigned long long sD[]; /* WW deposit counters, then E */
@@ -900,6 +940,7 @@ __device__ __forceinline__ void push_body(const PushArgs &a, const int blk) {
       int i = c & 0xFFFF;
       int j = c >> 16;
       float hx = cur.hx[u], hy = cur.hy[u], vx = cur.vx[u], vy = cur.vy[u];
+      float vt = 0.f;
       bool alive = true;
       if (a.do_push) {
         /* gather (cloud_in_cell.jl:20-36) */
@@ -940,7 +981,12 @@ __device__ __forceinline__ void push_body(const PushArgs &a, const int blk) {
         }
         /* push with B = 0 (pushers.jl:37-50): v += (q/m) E dt, x += v dt */
         vx = fmaf(a.qmdt, Ex, vx);
-        i += advance_coord(vx, a.dtdx, &hx);
+        if (RZ) {
+          vt = a.vz[start + kslot];
+          rz_transform(&i, &hx, &vx, &vt, a.dtdr);
+        } else {
+          i += advance_coord(vx, a.dtdx, &hx);
+        }
         alive = apply_bnd(&i, a.ncx, a.bndx);
         if (DIM == 2) {
           vy = fmaf(a.qmdt, Ey, vy);
@@ -990,6 +1036,7 @@ __device__ __forceinline__ void push_body(const PushArgs &a, const int blk) {
           oyb[k] = hy;
           vyb[k] = vy;
         }
+        if (RZ) a.vz[start + k] = vt;
       }
     }
   }
@@ -1009,9 +1056,9 @@ __device__ __forceinline__ void push_body(const PushArgs &a, const int blk) {
   }
 }
 
-template <int DIM>
+template <int DIM, bool RZ = false>
 __global__ __launch_bounds__(PushShape<DIM>::block) void k_push(PushArgs a) {
-  push_body<DIM>(a, blockIdx.x);
+  push_body<DIM, RZ>(a, blockIdx.x);
 }
 
 /* arguments of one push launch reading buffer `b`; grid and LDS size */
@@ -1025,6 +1072,8 @@ static void push_prepare(iskra_ctx *c, int sid, double dt, int do_push, int b, P
   a.oy = s->oy[b];
   a.vx = s->vx[b];
   a.vy = s->vy[b];
+  a.vz = s->vz[b];
+  a.dtdr = dt / g.dx;
   a.hdr = s->hdr;
   a.chunks = s->chunks;
   a.partial = s->partial;
@@ -1068,7 +1117,10 @@ static int push_launch(iskra_ctx *c, int sid, double dt, int do_push, int fill) 
     int rc2 = isk_prof_begin(c, &ev);
     if (rc2) return rc2;
   }
-  if (c->g.dim == 2)
+  if (c->g.dim == 2 && c->pusher == ISKRA_PUSHER_RZ && do_push)
+    hipLaunchKernelGGL((k_push<2, true>), dim3(grid), dim3(PushShape<2>::block), shm, c->stream,
+                       a);
+  else if (c->g.dim == 2)
     hipLaunchKernelGGL(k_push<2>, dim3(grid), dim3(PushShape<2>::block), shm, c->stream, a);
   else
     hipLaunchKernelGGL(k_push<1>, dim3(grid), dim3(PushShape<1>::block), shm, c->stream, a);
@@ -1090,14 +1142,14 @@ struct PushMulti {
   int n;
 };
 
-template <int DIM>
+template <int DIM, bool RZ = false>
 __global__ __launch_bounds__(PushShape<DIM>::block) void k_push_multi(PushMulti m) {
   int s = 0;
   while (s + 1 < m.n && (int)blockIdx.x >= m.bstart[s + 1]) ++s;
   /* a copy in registers: read through a reference into the argument block,
    * the fields would be re-read from memory after every particle's stores */
   const PushArgs a = m.a[s];
-  push_body<DIM>(a, blockIdx.x - m.bstart[s]);
+  push_body<DIM, RZ>(a, blockIdx.x - m.bstart[s]);
 }
 
 static int push_launch_all(iskra_ctx *c, double dt) {
@@ -1117,7 +1169,10 @@ static int push_launch_all(iskra_ctx *c, double dt) {
   hipEvent_t ev = nullptr;
   int rc = isk_prof_begin(c, &ev);
   if (rc) return rc;
-  if (c->g.dim == 2)
+  if (c->g.dim == 2 && c->pusher == ISKRA_PUSHER_RZ)
+    hipLaunchKernelGGL((k_push_multi<2, true>), dim3(m.bstart[m.n]), dim3(PushShape<2>::block),
+                       shm, c->stream, m);
+  else if (c->g.dim == 2)
     hipLaunchKernelGGL(k_push_multi<2>, dim3(m.bstart[m.n]), dim3(PushShape<2>::block), shm,
                        c->stream, m);
   else

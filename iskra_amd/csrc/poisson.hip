@@ -55,6 +55,15 @@ __device__ __forceinline__ double apply_L(const Sten &s, const T *v, int i, int 
                                           double *diag_out) {
   int n = j * s.nx + i;
   double acc = 0.0, diag = 0.0;
+  if (s.axial) {
+    const double cz = isk_rz_w(i, s.nx) * s.az;
+    if (i > 0) { acc += ((double)i - 0.5) * (double)v[n - 1]; diag += (double)i - 0.5; }
+    if (i < s.nx - 1) { acc += ((double)i + 0.5) * (double)v[n + 1]; diag += (double)i + 0.5; }
+    if (j > 0) { acc += cz * (double)v[n - s.nx]; diag += cz; }
+    if (j < s.ny - 1) { acc += cz * (double)v[n + s.nx]; diag += cz; }
+    if (diag_out) *diag_out = diag;
+    return diag * (double)v[n] - acc;
+  }
   if (i > 0) { acc += (double)v[n - 1]; diag += 1.0; }
   if (i < s.nx - 1) { acc += (double)v[n + 1]; diag += 1.0; }
   if (s.per_x) {
@@ -363,6 +372,7 @@ __global__ __launch_bounds__(RED_BLOCK) void k_pcg_resid(PcgArgs a) {
     int i = n % a.s.nx, j = n / a.s.nx;
     if (a.s.tags[n] == 0) {
       double f = a.rho[n] * a.scale;
+      if (a.s.axial) f *= isk_rz_w(i, a.s.nx);
       if (a.iflag[2]) f -= mean;
       double diag;
       /* x holds the Dirichlet values, so L x already carries them */
@@ -458,6 +468,13 @@ __global__ __launch_bounds__(RED_BLOCK) void k_cg_a(PcgArgs a, int it) {
 #define PN(m) (a.s.tags[m] ? 0.0 : (double)fmaf(fb, pold[m], a.z[m]))
       double pc = PN(n);
       double acc = 0.0, diag = 0.0;
+      if (a.s.axial) {
+        const double cz = isk_rz_w(i, a.s.nx) * a.s.az;
+        if (i > 0) { acc += ((double)i - 0.5) * PN(n - 1); diag += (double)i - 0.5; }
+        if (i < a.s.nx - 1) { acc += ((double)i + 0.5) * PN(n + 1); diag += (double)i + 0.5; }
+        if (j > 0) { acc += cz * PN(n - a.s.nx); diag += cz; }
+        if (j < a.s.ny - 1) { acc += cz * PN(n + a.s.nx); diag += cz; }
+      } else {
       if (i > 0) { acc += PN(n - 1); diag += 1.0; }
       if (i < a.s.nx - 1) { acc += PN(n + 1); diag += 1.0; }
       if (a.s.per_x) {
@@ -471,6 +488,7 @@ __global__ __launch_bounds__(RED_BLOCK) void k_cg_a(PcgArgs a, int it) {
           if (j == 0) { acc += a.s.cyp * PN(n + (a.s.ny - 1) * a.s.nx); diag += a.s.cyp; }
           if (j == a.s.ny - 1) { acc += a.s.cyp * PN(n - (a.s.ny - 1) * a.s.nx); diag += a.s.cyp; }
         }
+      }
       }
 #undef PN
       double ap = diag * pc - acc;
@@ -516,13 +534,21 @@ __global__ __launch_bounds__(RED_BLOCK) void k_cg_b(PcgArgs a, int it) {
     rr = r * r;
     if (!a.mgpre) {
       double diag = 0.0;
-      if (i > 0) diag += 1.0;
-      if (i < a.s.nx - 1) diag += 1.0;
-      if (a.s.per_x && (i == 0 || i == a.s.nx - 1)) diag += a.s.cxp;
-      if (a.s.dim == 2) {
-        if (j > 0) diag += 1.0;
-        if (j < a.s.ny - 1) diag += 1.0;
-        if (a.s.per_y && (j == 0 || j == a.s.ny - 1)) diag += a.s.cyp;
+      if (a.s.axial) {
+        const double cz = isk_rz_w(i, a.s.nx) * a.s.az;
+        if (i > 0) diag += (double)i - 0.5;
+        if (i < a.s.nx - 1) diag += (double)i + 0.5;
+        if (j > 0) diag += cz;
+        if (j < a.s.ny - 1) diag += cz;
+      } else {
+        if (i > 0) diag += 1.0;
+        if (i < a.s.nx - 1) diag += 1.0;
+        if (a.s.per_x && (i == 0 || i == a.s.nx - 1)) diag += a.s.cxp;
+        if (a.s.dim == 2) {
+          if (j > 0) diag += 1.0;
+          if (j < a.s.ny - 1) diag += 1.0;
+          if (a.s.per_y && (j == 0 || j == a.s.ny - 1)) diag += a.s.cyp;
+        }
       }
       float z = (float)(r / diag);
       a.z[n] = z;
@@ -1338,6 +1364,8 @@ static Sten make_sten(iskra_ctx *c) {
   s.cyp = 1.0;
   s.tags = c->ps.d_tags;
   s.tagval = c->ps.d_tag_val;
+  s.axial = c->g.axial;
+  s.az = c->g.axial ? (c->g.dx * c->g.dx) / (c->g.dy * c->g.dy) : 1.0;
   return s;
 }
 
@@ -1355,6 +1383,24 @@ int isk_poisson_setup(iskra_ctx *c) {
     }
   }
   p.singular = ndir == 0;
+  if (c->g.axial) {
+    /* the axisymmetric operator (generalized_poisson.jl:70-199) is solved by
+     * CG on its symmetric weighted form only */
+    if (p.kind != ISKRA_POISSON_PCG) {
+      isk_set_error("poisson: an axial grid is solved by PCG only");
+      return ISKRA_E_UNSUPPORTED;
+    }
+    if (p.per_x || p.per_y) {
+      isk_set_error("poisson: no periodic link on an axial grid (the reference's "
+                    "apply_periodic for RZ cannot run)");
+      return ISKRA_E_UNSUPPORTED;
+    }
+    if (p.singular) {
+      isk_set_error("poisson: an axial grid needs a Dirichlet node (the reference's "
+                    "matrix is singular without one)");
+      return ISKRA_E_UNSUPPORTED;
+    }
+  }
   if (isk_alloc((void **)&p.d_tags, nn)) return ISKRA_E_HIP;
   ISK_HIP(hipMemcpy(p.d_tags, p.tags.data(), nn, hipMemcpyHostToDevice));
   if (isk_alloc((void **)&p.d_tag_val, sizeof(double) * (ISK_MAX_TAGS + 1))) return ISKRA_E_HIP;

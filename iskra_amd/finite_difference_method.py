@@ -42,6 +42,8 @@ class PoissonSolver:
             return self.kind
         if self.grid.dim == 1:
             return L.POISSON_DIRECT
+        if getattr(self.grid, "kind", "xy") == "rz":
+            return L.POISSON_PCG  # the axisymmetric operator (generalized_poisson.jl:70-199)
         walls = (self.tags[0, :].all() and self.tags[-1, :].all() and self.tags[:, 0].all()
                  and self.tags[:, -1].all())
         interior = self.tags[1:-1, 1:-1].any()
@@ -69,10 +71,9 @@ class PoissonSolver:
 
     def _context(self):
         if self._ctx is None:
+            from .particle_in_cell import _grid_to
             ctx = Context(0)
-            g = self.grid
-            ctx.grid(g.dim, g.n, g.dh, g.origin if g.dim == 2 else (g.origin[0], 0.0),
-                     g.vol_periodic())
+            _grid_to(ctx, self.grid)
             self._own = True
             self._configure(ctx)
         elif self._dirty:
@@ -112,7 +113,11 @@ def apply_periodic(ps, axis=None):
     """apply_periodic(ps[, axis]) (generalized_poisson.jl:271-324): link the
     first and last node of an axis as neighbours.  In 2-D the reference's
     axis = 1 links the second index (j) and axis = 2 the first (i); that
-    labelling is kept.  1-D takes no axis."""
+    labelling is kept.  1-D takes no axis.  On an axial grid the reference
+    cannot apply it (generalized_poisson.jl:326-361 asserts axis != 1 and
+    reads undefined names for axis 2): refused."""
+    if getattr(ps.grid, "kind", "xy") == "rz":
+        raise IskraError(L.E_UNSUPPORTED, "apply_periodic: not available on an axial grid")
     if ps.grid.dim == 1:
         if axis not in (None, 1):
             raise ValueError("apply_periodic: a 1-D solver has no axis argument")

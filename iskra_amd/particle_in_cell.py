@@ -186,11 +186,20 @@ def create_thermalized_beam(species, x, vb, dx=None, T=300.0, rate=1.0):
 
 
 class BorisPusher:
-    """BorisPusher{:xy} (pic/pushers.jl:4-6); E only (B = 0 on this path)."""
+    """BorisPusher{:xy} / BorisPusher{:rz} (pic/pushers.jl:4-17); E only
+    (B = 0 on this path).  The :rz pusher follows the Cartesian step by
+    transform_from_cartesian_to_cylindrical! (pushers.jl:52-66)."""
+
+    def __init__(self, kind="xy"):
+        self.kind = kind
 
 
 def create_boris_pusher():
-    return BorisPusher()
+    return BorisPusher("xy")
+
+
+def create_axial_boris_pusher():
+    return BorisPusher("rz")
 
 
 # ---------------------------------------------------------------------------
@@ -313,7 +322,11 @@ def bind(config):
                              "(advection_diffusion.jl:4-38); only fluids the reference "
                              "leaves constant are supported")
     ctx = Context(config.device)
-    ctx.grid(g.dim, g.n, g.dh, g.origin if g.dim == 2 else (g.origin[0], 0.0), g.vol_periodic())
+    _grid_to(ctx, g)
+    if config.pusher is not None and getattr(config.pusher, "kind", "xy") == "rz":
+        ctx.pusher(L.PUSHER_RZ)
+    elif config.pusher is not None:
+        ctx.pusher(L.PUSHER_XY)
     ctx.rng(config.seed)
     ctx._seed = config.seed
     config.ctx = ctx
@@ -344,6 +357,16 @@ def bind(config):
         if isinstance(inter, chemistry.MonteCarloCollisions):
             inter._bind(ctx, g)
     return ctx
+
+
+def _grid_to(ctx, g):
+    """The grid on the device: create_uniform_grid or create_axial_grid."""
+    if g.kind == "rz":
+        vp = g.vol_periodic()
+        ctx.grid_axial(g.n, g.dh, g.origin, (vp[2], vp[3]))
+    else:
+        ctx.grid(g.dim, g.n, g.dh, g.origin if g.dim == 2 else (g.origin[0], 0.0),
+                 g.vol_periodic())
 
 
 def _sync_to_device(config):

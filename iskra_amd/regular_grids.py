@@ -1,4 +1,5 @@
-"""RegularGrids: uniform Cartesian grids (RegularGrids/src/RegularGrids.jl).
+"""RegularGrids: uniform Cartesian and axial (r, z) grids
+(RegularGrids/src/RegularGrids.jl).
 
 Grid objects are host metadata (node counts, spacing, boundary symbols,
 origin); the device copy is made when a solver or a simulation binds the grid
@@ -9,6 +10,7 @@ import numpy as np
 
 OPEN = "open"
 PERIODIC = "periodic"
+OTHER = "other"
 
 
 class UniformGrid:
@@ -16,7 +18,8 @@ class UniformGrid:
     dh, boundary symbols bcs = ((left, right), (bottom, top)), origin, node
     coordinates."""
 
-    def __init__(self, n, dh, bcs, origin, coords):
+    def __init__(self, n, dh, bcs, origin, coords, kind="xy"):
+        self.kind = kind  # "xy": CartesianGrid, "rz": AxialGrid (RegularGrids.jl:17-18)
         self.n = tuple(int(k) for k in n)
         self.dh = tuple(float(h) for h in dh)
         self.bcs = bcs
@@ -69,6 +72,19 @@ def create_uniform_grid(xx, yy=None, left=OPEN, right=OPEN, bottom=OPEN, top=OPE
                        (xx[0], yy[0]), (x, y))
 
 
+def create_axial_grid(rr, zz, bottom=OPEN, top=OPEN):
+    """create_axial_grid(rr, zz; bottom, top) (RegularGrids.jl:84-97): radial
+    node coordinates rr (the axis at rr[0]) and axial zz; the radial bcs are
+    (:other, :other)."""
+    rr = np.asarray(rr, dtype=np.float64)
+    zz = np.asarray(zz, dtype=np.float64)
+    nr, nz = len(rr), len(zz)
+    r = np.repeat(rr[:, None], nz, axis=1)
+    z = np.repeat(zz[None, :], nr, axis=0)
+    return UniformGrid((nr, nz), (_spacing(rr), _spacing(zz)), ((OTHER, OTHER), (bottom, top)),
+                       (rr[0], zz[0]), (r, z), kind="rz")
+
+
 def cell_volume(g):
     """cell_volume(::CartesianGrid{2}) (RegularGrids.jl:26-38): dx*dy per node,
     halved on each edge that is not :periodic (corners quartered).  The 1-D
@@ -82,6 +98,20 @@ def cell_volume(g):
             V[-1] *= 0.5
         return V
     (left, right), (bottom, top) = g.bcs
+    if g.kind == "rz":
+        # cell_volume(::AxialGrid{2}) (RegularGrids.jl:40-53), 1-based i
+        dr, dz = g.dh
+        nr, nz = g.n
+        V = np.zeros(g.n)
+        for i in range(2, nr):
+            V[i - 1, :] = np.pi * dz * ((i * dr - 0.5 * dr) ** 2 - (i * dr - 1.5 * dr) ** 2)
+        V[0, :] = np.pi * dz * (0.5 * dr) ** 2
+        V[nr - 1, :] = np.pi * dz * ((nr * dr - 1.0 * dr) ** 2 - (nr * dr - 1.5 * dr) ** 2)
+        if bottom != PERIODIC:
+            V[:, 0] *= 0.5
+        if top != PERIODIC:
+            V[:, -1] *= 0.5
+        return V
     V = np.full(g.n, g.dh[0] * g.dh[1])
     if left != PERIODIC:
         V[0, :] *= 0.5

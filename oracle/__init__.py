@@ -61,6 +61,21 @@ How each oracle function is pinned (checks live in tests/):
                          against the device over 40 steps with the device's
                          potential handed over, that potential within 1e-12 of
                          the FP64 banded solve (tests/test_gpu_loop.py).
+  ref_cell_volume_axial  closed forms: the annulus volumes 2 pi r_i dr dz
+                         (interior), pi (dr/2)^2 dz (axis), pi dz (r^2 -
+                         (r - dr/2)^2) (outer wall); they sum to the cylinder
+                         pi R^2 L (tests/test_oracle_rz.py).
+  ref_poisson_assemble_rz manufactured solutions: the axisymmetric operator
+                         is exact on phi = r^2 (Laplacian 4) and z^2 (2) on
+                         interior rows, and row by row on the axis and wall
+                         rows' mirrored forms; equal to poisson_sparse_rz;
+                         W A symmetric with the stated row weights.
+  ref_rz_transform       closed forms: r' = sqrt(x^2 + (dt v_theta)^2),
+                         v_r'^2 + v_theta'^2 = v_r^2 + v_theta^2, a particle
+                         with v_theta = 0 keeps its Cartesian step; the
+                         axis-crossing sign quirk (cos >= 0) by hand.
+  mir_push_deposit_rz    against ref_push + ref_rz_transform in FP32
+                         tolerance, bit for bit against the device.
   orc_ws_run             (learned warm start, oracle/ws_oracle.c; no reference
                          counterpart) FP64 gradients against central finite
                          differences; convolution, pooling and upsampling
@@ -122,6 +137,11 @@ def _declare(L):
         "ref_wrap": (None, [I64, C.c_int, P, P, P, P, C.c_int, P]),
         "ref_discard": (I64, [P, I64, C.c_int, P, P, P, P, D, P, P, P, C.c_int, P]),
         "ref_poisson_assemble": (None, [C.c_int, C.c_int, D, P]),
+        "ref_cell_volume_axial": (None, [C.c_int, C.c_int, D, D, P, P]),
+        "ref_poisson_assemble_rz": (None, [C.c_int, C.c_int, D, D, P]),
+        "ref_rz_transform": (None, [I64, P, P, D]),
+        "mir_push_deposit_rz": (I64, [I64, P, P, P, P, P, P, P, C.c_int, C.c_int, C.c_int, F, D,
+                                      F, C.c_int, C.c_int, P]),
         "ref_apply_periodic_1d": (None, [C.c_int, D, P]),
         "ref_apply_periodic_2d": (None, [C.c_int, C.c_int, D, D, C.c_int, P]),
         "ref_apply_dirichlet": (None, [C.c_int, P, D, P, P, P]),
@@ -196,6 +216,91 @@ def cell_volume(dim, nx, ny, dx, dy, volper=(0, 0, 0, 0)):
     vp = np.asarray(volper, dtype=np.int32)
     lib().ref_cell_volume(dim, nx, ny, dx, dy, ptr(vp), ptr(V))
     return V
+
+
+def cell_volume_axial(nr, nz, dr, dz, volper_z=(0, 0)):
+    """cell_volume(::AxialGrid{2}) (RegularGrids.jl:40-53), r fastest."""
+    V = np.zeros(nr * nz)
+    vp = np.asarray(volper_z, dtype=np.int32)
+    lib().ref_cell_volume_axial(nr, nz, dr, dz, ptr(vp), ptr(V))
+    return V
+
+
+def poisson_matrix_rz(nr, nz, dr, dz, dirichlet=()):
+    """The reference's dense axisymmetric A (generalized_poisson.jl:70-199)
+    with apply_dirichlet rows; returns A (column-major), b, isdof."""
+    nn = nr * nz
+    A = np.zeros(nn * nn)
+    lib().ref_poisson_assemble_rz(nr, nz, dr, dz, ptr(A))
+    b = np.zeros(nn)
+    isdof = np.ones(nn, dtype=np.uint8)
+    for mask, val in dirichlet:
+        m = np.ascontiguousarray(mask, dtype=np.uint8).ravel(order="F")
+        lib().ref_apply_dirichlet(nn, ptr(m), float(val), ptr(A), ptr(b), ptr(isdof))
+    return A, b, isdof
+
+
+def poisson_sparse_rz(nr, nz, dr, dz, tags, tagval):
+    """The same reference system as poisson_matrix_rz for grids the dense
+    assembly cannot hold (sparse, node k = i + nr j): radial second
+    difference and +-0.5/(dr r) derivative terms on 0 < i < nr-1, axial second
+    difference, a missing neighbour's link mirrored onto the diagonal;
+    Dirichlet rows are the identity.  Returns (A, Dirichlet mask, values)."""
+    import scipy.sparse as sps
+    N = nr * nz
+    k = np.arange(N)
+    i, j = k % nr, k // nr
+    t = np.asarray(tags, np.int8).ravel()
+    free = t == 0
+    cr, cz = 1.0 / dr ** 2, 1.0 / dz ** 2
+    rows, cols, vals = [k[~free]], [k[~free]], [np.ones((~free).sum())]
+    diag = np.where(free, -2.0 * cr - 2.0 * cz, 0.0)
+    mid = (i > 0) & (i < nr - 1)
+    rinv = np.where(mid, 0.5 / dr / np.maximum(i, 1) / dr, 0.0)
+    for off, ok, c, extra in ((1, i < nr - 1, cr, rinv), (-1, i > 0, cr, -rinv),
+                              (nr, j < nz - 1, cz, 0.0), (-nr, j > 0, cz, 0.0)):
+        m = free & ok
+        rows.append(k[m])
+        cols.append(k[m] + off)
+        vals.append((c + np.broadcast_to(extra, (N,)))[m])
+        mirror = free & ~ok
+        diag[mirror] += c
+    rows.append(k[free])
+    cols.append(k[free])
+    vals.append(diag[free])
+    A = sps.csc_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
+                       shape=(N, N))
+    return A, ~free, np.asarray(tagval)[t]
+
+
+def potential_rz(nr, nz, dr, dz, tags, tagval, rho, eps0):
+    """calculate_electric_potential on the axisymmetric reference system,
+    solved by sparse LU (FP64)."""
+    import scipy.sparse.linalg as spla
+    A, dmask, dval = poisson_sparse_rz(nr, nz, dr, dz, tags, tagval)
+    b = np.where(dmask, dval, -np.asarray(rho, np.float64) / eps0)
+    # the identity Dirichlet rows next to 1/dr^2 rows make a direct solve of
+    # the full system lose ~1e-8 of max|phi|; the same solution from the free
+    # rows with the Dirichlet columns moved to the right-hand side does not
+    A = A.tocsr()
+    free = ~dmask
+    x = np.where(dmask, dval, 0.0).astype(np.float64)
+    Aff = A[free][:, free].tocsc()
+    rhs = b[free] - A[free][:, dmask] @ x[dmask]
+    x[free] = spla.splu(Aff).solve(rhs)
+    return x
+
+
+def rz_transform(x, v, dt):
+    """transform_from_cartesian_to_cylindrical! (pushers.jl:52-66) on
+    reference-layout FP64 arrays (x: np x 2, v: np x 3, column-major), in place."""
+    n = x.shape[0]
+    xf = np.asfortranarray(x, dtype=np.float64)
+    vf = np.asfortranarray(v, dtype=np.float64)
+    lib().ref_rz_transform(n, xf.ctypes.data_as(C.c_void_p), vf.ctypes.data_as(C.c_void_p),
+                           float(dt))
+    x[...] = xf
+    v[...] = vf
 
 
 def particle_to_grid(dim, x, wg, nx, ny, dx, dy):
@@ -412,12 +517,16 @@ class MirSim:
     advance() and end_step() -- and E is formed from it as the device does."""
 
     def __init__(self, nx, dx, x0=0.0, volper=(0, 0, 0, 0), T=64, sort_every=4, seed=0,
-                 ny=1, dy=1.0):
+                 ny=1, dy=1.0, axial=False):
         self.dim = 2 if ny > 1 else 1
         self.nx, self.ny = nx, ny
         self.ncx, self.ncy = nx - 1, (ny - 1 if self.dim == 2 else 1)
         self.dx, self.dy, self.x0 = dx, dy, x0
-        self.vol = cell_volume(self.dim, nx, ny, dx, dy, volper)
+        self.axial = axial  # (r, z) grid with the RZ pusher
+        if axial:
+            self.vol = cell_volume_axial(nx, ny, dx, dy, volper[2:4])
+        else:
+            self.vol = cell_volume(self.dim, nx, ny, dx, dy, volper)
         self.T, self.sort_every = T, sort_every
         self.ntx = (self.ncx + T - 1) // T
         nn = nx * ny
@@ -563,6 +672,15 @@ class MirSim:
         sp = self.species[s]
         cnt = np.zeros(self.nx * self.ny, dtype=np.int64)
         qmdt = np.float32(sp.q / sp.m * dt)
+        if self.axial and do_push:
+            live = lib().mir_push_deposit_rz(sp.np, ptr(sp.cell), ptr(sp.ox), ptr(sp.oy),
+                                             ptr(sp.vx), ptr(sp.vy), ptr(sp.vz), ptr(self.E),
+                                             self.nx, self.ncx, self.ncy, qmdt, dt / self.dx,
+                                             np.float32(dt / self.dy), sp.bnd[0], sp.bnd[1],
+                                             ptr(cnt))
+            self.pushed += live
+            sp.cnt = cnt
+            return
         live = lib().mir_push_deposit(self.dim, sp.np, ptr(sp.cell), ptr(sp.ox), ptr(sp.oy),
                                       ptr(sp.vx), ptr(sp.vy), ptr(self.E), self.nx, self.ncx,
                                       self.ncy, qmdt, np.float32(dt / self.dx),
@@ -581,6 +699,10 @@ class MirSim:
                       ptr(self.bg), ptr(self.rho))
 
     def solve(self):
+        if self.axial:
+            self.set_phi(potential_rz(self.nx, self.ny, self.dx, self.dy, self.tags, self.tagval,
+                                      self.rho, self.eps0))
+            return
         if self.dim == 2:
             self.set_phi(potential_banded(self.nx, self.ny, self.dx, self.tags, self.tagval,
                                           self.rho, self.eps0))

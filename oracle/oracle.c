@@ -27,6 +27,9 @@
  * Build: oracle/build.py (gcc -O2 -ffp-contract=off).
  */
 #include <math.h>
+#ifndef M_PI
+#define M_PI 3.14159265358979323846 /* pi rounded to double, as Julia converts it */
+#endif
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -115,6 +118,29 @@ EXPORT void ref_cell_volume(int dim, int nx, int ny, double dx, double dy,
   }
 }
 
+/* cell_volume(::AxialGrid{2}) (RegularGrids/src/RegularGrids.jl:40-53), in
+ * the reference's 1-based node numbers and operation order: annuli
+ * pi dz ((i dr - dr/2)^2 - (i dr - 3dr/2)^2) for i = 2..nr-1, the axis disc
+ * pi dz (dr/2)^2, the outer node pi dz ((nr dr - dr)^2 - (nr dr - 3dr/2)^2);
+ * rows on a non-periodic z edge halved (volper_z = {bottom, top}). */
+EXPORT void ref_cell_volume_axial(int nr, int nz, double dr, double dz, const int32_t volper_z[2],
+                                  double *V) {
+  for (int j = 1; j <= nz; ++j) {
+    for (int i = 2; i <= nr - 1; ++i) {
+      double a = i * dr - 0.5 * dr, b = i * dr - 1.5 * dr;
+      V[(j - 1) * nr + (i - 1)] = M_PI * dz * (a * a - b * b);
+    }
+    double h = 0.5 * dr;
+    V[(j - 1) * nr] = M_PI * dz * (h * h);
+    double a = nr * dr - 1.0 * dr, b = nr * dr - 1.5 * dr;
+    V[(j - 1) * nr + nr - 1] = M_PI * dz * (a * a - b * b);
+  }
+  for (int i = 0; i < nr; ++i) {
+    if (!volper_z[0]) V[i] *= 0.5;
+    if (!volper_z[1]) V[(nz - 1) * nr + i] *= 0.5;
+  }
+}
+
 /* particle_cell (ParticleInCell/src/ParticleInCell.jl:28-35): f = 1 + x/dh,
  * i = floor(f) (1-based), h = f - i.  Returns the 0-based i. */
 static inline void ref_particle_cell(double x, double dh, int *i, double *h) {
@@ -192,6 +218,26 @@ EXPORT void ref_push(int64_t np, int D, double *x, double *v, const double *E, c
     vpl[2] = vm[2] + (vp[0] * s[1] - vp[1] * s[0]);
     for (int c = 0; c < 3; ++c) v[c * np + p] = dt * E[c * np + p] * qm * 0.5 + vpl[c];
     for (int c = 0; c < D; ++c) x[c * np + p] = dt * v[c * np + p] + x[c * np + p];
+  }
+}
+
+/* transform_from_cartesian_to_cylindrical! (ParticleInCell/src/pic/
+ * pushers.jl:52-66) after push_in_cartesian! (the RZ pusher, :13-17):
+ * y = dt v_3; r = sqrt(x_1^2 + y^2); sin = y/r (0 where r == 0, the
+ * reference's r .~ 0.0), cos = sqrt(1 - sin^2); x_1 = r and
+ * (v_1, v_3) = (cos v_1 + sin v_3, -sin v_1 + cos v_3).  x: np x 2, v: np x 3. */
+EXPORT void ref_rz_transform(int64_t np, double *x, double *v, double dt) {
+  for (int64_t p = 0; p < np; ++p) {
+    double y = dt * v[2 * np + p];
+    double r = sqrt(x[p] * x[p] + y * y);
+    double sn = y / r;
+    if (r == 0.0) sn = 0.0;
+    double cs = sqrt(1.0 - sn * sn);
+    double vr = cs * v[p] + sn * v[2 * np + p];
+    double vy = -sn * v[p] + cs * v[2 * np + p];
+    x[p] = r;
+    v[p] = vr;
+    v[2 * np + p] = vy;
   }
 }
 
@@ -347,6 +393,37 @@ EXPORT void ref_poisson_assemble(int nx, int ny, double dx, double *A) {
     }
   double d2 = dx * dx;
   for (size_t q = 0; q < nn * nn; ++q) A[q] /= d2;
+}
+
+/* create_poisson_solver(::AxialGrid{2}, eps0) (generalized_poisson.jl:70-199):
+ * dense nn x nn A (column-major, node k = i + nr j), entries 1/dr^2, 1/dz^2
+ * and the first-derivative terms +-0.5/(dr r) with r = i dr (0-based i) on
+ * the rows off the axis and off the outer wall.  The reference writes nine
+ * loops (interior, bottom, top, axis, side, four corners); they are one rule:
+ * a missing neighbour's link is mirrored back onto the diagonal
+ * (+1/dh^2 on top of the -2/dh^2), and the derivative terms exist exactly on
+ * rows 0 < i < nr-1. */
+EXPORT void ref_poisson_assemble_rz(int nr, int nz, double dr, double dz, double *A) {
+  size_t nn = (size_t)nr * nz;
+  memset(A, 0, sizeof(double) * nn * nn);
+  double cr = 1.0 / (dr * dr), cz = 1.0 / (dz * dz);
+  for (int j = 0; j < nz; ++j)
+    for (int i = 0; i < nr; ++i) {
+      int k = i + nr * j;
+      /* radial second difference */
+      AE(k, (i < nr - 1) ? k + 1 : k) += cr;
+      AE(k, k) -= 2.0 * cr;
+      AE(k, (i > 0) ? k - 1 : k) += cr;
+      /* axial second difference */
+      AE(k, (j < nz - 1) ? k + nr : k) += cz;
+      AE(k, k) -= 2.0 * cz;
+      AE(k, (j > 0) ? k - nr : k) += cz;
+      if (i > 0 && i < nr - 1) {
+        double r = i * dr;
+        AE(k, k + 1) += 0.5 / dr / r;
+        AE(k, k - 1) -= 0.5 / dr / r;
+      }
+    }
 }
 
 /* apply_periodic 1-D (generalized_poisson.jl:271-284) */
@@ -668,6 +745,64 @@ EXPORT int64_t mir_push_deposit(int dim, int64_t np, int32_t *cell, float *ox, f
     if (alive) {
       ++live;
       mir_deposit1(dim, nx, i, j, hx, hy, cnt);
+    }
+  }
+  return live;
+}
+
+/* the RZ transform in cell units as the device computes it (FP64 from the
+ * pushed FP32 velocity; see mir_push_deposit_rz) */
+static inline void mir_rz_transform(int *i, float *hx, float *vr, float *vt, double dtdr) {
+  const double X = (double)*i + ((double)*hx + (double)*vr * dtdr);
+  const double Y = (double)*vt * dtdr;
+  const double R = sqrt(X * X + Y * Y);
+  const double sn = R == 0.0 ? 0.0 : Y / R;
+  const double cs = sqrt(1.0 - sn * sn);
+  const double nvr = cs * (double)*vr + sn * (double)*vt;
+  const double nvt = cs * (double)*vt - sn * (double)*vr;
+  int ii;
+  float h;
+  if (R < 1073741824.0) {
+    const double fl = floor(R);
+    ii = (int)fl;
+    h = (float)(R - fl);
+    if (h >= 1.0f) { h = 0.0f; ++ii; }
+  } else {
+    ii = 1 << 30;
+    h = 0.0f;
+  }
+  *i = ii;
+  *hx = h;
+  *vr = (float)nvr;
+  *vt = (float)nvt;
+}
+
+/* mir_push_deposit for the RZ pusher on a 2-D (r, z) grid: the radial
+ * coordinate goes through the transform instead of the Cartesian step, and
+ * v_theta (vz) turns with v_r. */
+EXPORT int64_t mir_push_deposit_rz(int64_t np, int32_t *cell, float *ox, float *oy, float *vx,
+                                   float *vy, float *vz, const float *E, int nx, int ncx, int ncy,
+                                   float qmdt, double dtdr, float dtdz, int bndr, int bndz,
+                                   int64_t *cnt) {
+  int64_t live = 0;
+  for (int64_t p = 0; p < np; ++p) {
+    if (cell[p] < 0) continue;
+    int i = cell[p] & 0xFFFF, j = cell[p] >> 16;
+    float hx = ox[p], hy = oy[p];
+    float Ex, Ey;
+    mir_gather1(2, E, nx, i, j, hx, hy, &Ex, &Ey);
+    vx[p] = fmaf(qmdt, Ex, vx[p]);
+    mir_rz_transform(&i, &hx, &vx[p], &vz[p], dtdr);
+    int alive = mir_bnd(&i, ncx, bndr);
+    vy[p] = fmaf(qmdt, Ey, vy[p]);
+    j += mir_advance(vy[p], dtdz, &hy);
+    alive = mir_bnd(&j, ncy, bndz) && alive;
+    cell[p] = alive ? (i | (j << 16)) : -1;
+    ox[p] = hx;
+    oy[p] = hy;
+    if (alive) {
+      ++live;
+      mir_deposit1(2, nx, i, j, hx, hy, cnt);
     }
   }
   return live;

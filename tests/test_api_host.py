@@ -171,3 +171,24 @@ def test_fluids_the_reference_would_update_are_refused():
         with pytest.raises(L.IskraError) as ei:
             pic.bind(cfg)
         assert ei.value.code == L.E_UNSUPPORTED and cfg.ctx is None
+
+
+def test_axial_grid_pusher_and_solver_host_side(orc):
+    """create_axial_grid (RegularGrids.jl:84-97): r fastest node layout,
+    radial bcs (:other, :other), node volumes equal to the oracle's
+    restatement of cell_volume(::AxialGrid{2}); the axial pusher and the
+    PCG default; apply_periodic refused (the reference's RZ version cannot
+    run)."""
+    g = rg.create_axial_grid(np.arange(0, 9) * 0.5e-3, np.arange(0, 6) * 1e-3, top=rg.PERIODIC)
+    assert g.kind == "rz" and g.n == (9, 6) and g.dh == (0.5e-3, 1e-3)
+    assert g.bcs == (("other", "other"), ("open", "periodic"))
+    V = rg.cell_volume(g)
+    want = orc.cell_volume_axial(9, 6, 0.5e-3, 1e-3, (0, 1)).reshape(6, 9).T
+    assert np.array_equal(V, want)
+    assert pic.create_axial_boris_pusher().kind == "rz"
+    assert pic.create_boris_pusher().kind == "xy"
+    ps = fdm.create_poisson_solver(g, 8.854e-12)
+    assert ps._solver_kind() == L.POISSON_PCG
+    with pytest.raises(L.IskraError) as ei:
+        fdm.apply_periodic(ps, 1)
+    assert ei.value.code == L.E_UNSUPPORTED
