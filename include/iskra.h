@@ -267,6 +267,31 @@ int iskra_mcc_run(iskra_ctx *ctx, int mcc_id, double dt, int64_t *events);
  * (mcc.jl:283-288), out: nodes x n_coll col-major. */
 int iskra_mcc_counts(iskra_ctx *ctx, int mcc_id, int reset, int64_t *out);
 
+/* ---- direct simulation Monte Carlo ------------------------------------- */
+/* dsmc(@reactions ...) with one DSMC.ElasticCollision
+ * (Chemistry/src/dsmc.jl:1-21,144-167) between two kinetic species of equal
+ * weight (src_sid may equal tgt_sid): the rate is a CrossSection over the
+ * relative speed g (cross_section.jl:3-17): n_tab nodes tab_g (m/s,
+ * increasing) and tab_sigma (m^2).  2-D grids only (the reference's
+ * perform! destructures a 2-D grid).  ISKRA_E_UNSUPPORTED for unequal
+ * weights: the reference's update for them writes the target velocity into
+ * the particle with the source's index (dsmc.jl:71-77).  One reaction per
+ * interaction: add one interaction per reaction (the reference's lists are
+ * shared between the reactions of one interaction, dsmc.jl:94-96). */
+int iskra_dsmc_add(iskra_ctx *ctx, int src_sid, int tgt_sid, int n_tab,
+                   const double *tab_g, const double *tab_sigma, int *dsmc_id);
+/* PIC.perform!(::DirectSimulationMonteCarlo, E, dt, config)
+ * (dsmc.jl:81-142): both species are sorted by exact cell (their device
+ * order changes), then every cell runs its floor(Nc) pairing trials; the
+ * fractional remainder of Nc is carried per cell to the next call.
+ * events (nullable) receives this call's accepted collisions.  iskra_step
+ * runs every DSMC interaction after the MCC ones, before the push. */
+int iskra_dsmc_run(iskra_ctx *ctx, int dsmc_id, double dt, int64_t *events);
+/* Accepted collisions per cell since the last reset, at the cell's lower-left
+ * node (the reference's nuDSMC field, dsmc.jl:124-141): out has one entry
+ * per node (nodes x 1, col-major). */
+int iskra_dsmc_counts(iskra_ctx *ctx, int dsmc_id, int reset, int64_t *out);
+
 /* ---- random numbers ------------------------------------------------------ */
 /* The seed of all device draws (Random.seed!, src/iskra.jl:22-26). */
 int iskra_rng_set(iskra_ctx *ctx, uint64_t seed);

@@ -26,7 +26,8 @@ NOALLOC_SCRATCH_BYTES = 4 << 20
 
 EXPORTS = [
     "iskra_last_error", "iskra_version", "iskra_ctx_create", "iskra_ctx_destroy", "iskra_sync",
-    "iskra_grid_set", "iskra_grid_set_axial", "iskra_pusher_set", "iskra_tuning_set", "iskra_tuning_get", "iskra_species_add",
+    "iskra_grid_set", "iskra_grid_set_axial", "iskra_pusher_set", "iskra_dsmc_add",
+    "iskra_dsmc_run", "iskra_dsmc_counts", "iskra_tuning_set", "iskra_tuning_get", "iskra_species_add",
     "iskra_species_set_boundary", "iskra_species_sort_every", "iskra_species_margin",
     "iskra_species_upload",
     "iskra_species_download",
@@ -92,6 +93,9 @@ def lib():
             "iskra_grid_set": (I, [P, I, P, P, P, P]),
             "iskra_grid_set_axial": (I, [P, P, P, P, P]),
             "iskra_pusher_set": (I, [P, I]),
+            "iskra_dsmc_add": (I, [P, I, I, I, P, P, C.POINTER(I)]),
+            "iskra_dsmc_run": (I, [P, I, D, C.POINTER(I64)]),
+            "iskra_dsmc_counts": (I, [P, I, I, P]),
             "iskra_tuning_set": (I, [P, C.POINTER(Tuning)]),
             "iskra_tuning_get": (I, [P, C.POINTER(Tuning)]),
             "iskra_species_add": (I, [P, D, D, D, I64, I, C.POINTER(I)]),
@@ -425,6 +429,25 @@ class Context:
         return out.reshape(ncoll, self.nn)
 
     # RNG
+    def dsmc(self, src, tgt, tab_g, tab_sigma):
+        """One DSMC.ElasticCollision interaction; returns its id."""
+        g = np.ascontiguousarray(tab_g, dtype=np.float64)
+        s = np.ascontiguousarray(tab_sigma, dtype=np.float64)
+        out = C.c_int()
+        check(lib().iskra_dsmc_add(self.h, int(src), int(tgt), len(g), _p(g), _p(s),
+                                   C.byref(out)))
+        return out.value
+
+    def dsmc_run(self, did, dt):
+        ev = C.c_int64(0)
+        check(lib().iskra_dsmc_run(self.h, int(did), float(dt), C.byref(ev)))
+        return ev.value
+
+    def dsmc_counts(self, did, reset=False):
+        out = np.zeros(self.nn, dtype=np.int64)
+        check(lib().iskra_dsmc_counts(self.h, int(did), int(reset), _p(out)))
+        return out
+
     def rng(self, seed):
         check(lib().iskra_rng_set(self.h, int(seed)))
 

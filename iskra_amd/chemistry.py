@@ -176,3 +176,80 @@ def _accept(r):
 def mcc(reaction_list):
     """mcc(reactions) (mcc.jl:314-320)."""
     return MonteCarloCollisions([_accept(r) for r in reaction_list])
+
+
+# ---------------------------------------------------------------------------
+# direct simulation Monte Carlo (Chemistry/src/dsmc.jl)
+
+class DSMCElasticCollision:
+    """DSMC.ElasticCollision (dsmc.jl:2-6): rate is a CrossSection over the
+    relative speed (m/s)."""
+
+    def __init__(self, rate, source, target):
+        self.rate = rate
+        self.source = source
+        self.target = target
+
+
+class DSMCIonizationCollision:
+    """DSMC.IonizationCollision (dsmc.jl:8-13).  The reference defines no
+    perform! for it, so a run that draws one fails there; binding it here is
+    refused."""
+
+    def __init__(self, rate, source, target, products):
+        self.rate = rate
+        self.source = source
+        self.target = target
+        self.products = products
+
+
+class DirectSimulationMonteCarlo:
+    """DirectSimulationMonteCarlo (dsmc.jl:16-23): pairing collisions between
+    two kinetic species per cell, run on the GPU inside the loop
+    (PIC.perform!, dsmc.jl:81-142).  One reaction per interaction (the
+    reference's candidate lists are shared between the reactions of one
+    interaction and only one reaction gives defined results)."""
+
+    def __init__(self, collisions):
+        self.collisions = list(collisions)
+        self._ctx = None
+        self._id = -1
+
+    def _bind(self, ctx, grid):
+        if len(self.collisions) != 1:
+            raise L.IskraError(L.E_UNSUPPORTED, "dsmc: one reaction per interaction (the "
+                               "reference appends every reaction's particles to the same "
+                               "cell lists, dsmc.jl:94-96)")
+        c = self.collisions[0]
+        if isinstance(c, DSMCIonizationCollision):
+            raise L.IskraError(L.E_UNSUPPORTED, "dsmc: the reference has no perform! for "
+                               "DSMC.IonizationCollision")
+        self._id = ctx.dsmc(c.source._sid, c.target._sid, c.rate.nodes[:, 0], c.rate.nodes[:, 1])
+        self._ctx = ctx
+
+    def counts(self, reset=False):
+        """nuDSMC per node (the cell's lower-left node), accumulated on the
+        device since the last reset."""
+        return self._ctx.dsmc_counts(self._id, reset)
+
+
+def dsmc(reaction_list):
+    """dsmc(reactions) (dsmc.jl:144-167): every reaction has exactly two
+    kinetic reactants; with products it is an ionisation, else elastic."""
+    out = []
+    for r in reaction_list:
+        if len(r.reactants) != 2:
+            raise ValueError("Direct Simulation Monte Carlo support only two reacting, "
+                             "kinetic species")
+        source, _ = r.reactants[0]
+        target, _ = r.reactants[1]
+        products = [s for s, c in r.stoichiometry if c > 0]
+        if source is None:
+            raise ValueError("Reaction without source species")
+        if target is None:
+            raise ValueError("Reaction without target species")
+        if products:
+            out.append(DSMCIonizationCollision(r.rate, source, target, products))
+        else:
+            out.append(DSMCElasticCollision(r.rate, source, target))
+    return DirectSimulationMonteCarlo(out)

@@ -310,6 +310,8 @@ static void release_fields(iskra_ctx *c) {
     q.tab_sig.clear();
   }
   c->nmcc = 0;
+  for (int m = 0; m < c->ndsmc; ++m) isk_dsmc_release(&c->dsmc[m]);
+  c->ndsmc = 0;
   isk_poisson_free(c);
   c->ps.kind = -1;
   void *ptrs[] = {c->rho, c->phi, c->bg, c->vol, c->E, c->sort_H, c->sort_S, c->sort_tot,
@@ -1156,6 +1158,43 @@ int iskra_mcc_counts(iskra_ctx *c, int mcc_id, int reset, int64_t *out) {
   return ISKRA_OK;
 }
 
+/* ---- DSMC ---- */
+int iskra_dsmc_add(iskra_ctx *c, int src_sid, int tgt_sid, int n_tab, const double *tab_g,
+                   const double *tab_sigma, int *dsmc_id) {
+  CHECK_CTX(c);
+  CHECK_GRID(c);
+  if (!isk_species_ok(c, src_sid) || !isk_species_ok(c, tgt_sid)) return ISKRA_E_ARG;
+  if (!dsmc_id) {
+    isk_set_error("dsmc_add: dsmc_id is required");
+    return ISKRA_E_ARG;
+  }
+  RC(isk_dsmc_add(c, src_sid, tgt_sid, n_tab, tab_g, tab_sigma, dsmc_id));
+  return ISKRA_OK;
+}
+
+int iskra_dsmc_run(iskra_ctx *c, int dsmc_id, double dt, int64_t *events) {
+  CHECK_CTX(c);
+  if (dsmc_id < 0 || dsmc_id >= c->ndsmc) {
+    isk_set_error("dsmc id %d out of range", dsmc_id);
+    return ISKRA_E_ARG;
+  }
+  if (!(dt > 0)) {
+    isk_set_error("dsmc_run: dt must be positive");
+    return ISKRA_E_ARG;
+  }
+  RC(isk_dsmc_run(c, dsmc_id, dt, events));
+  return iskra_sync(c);
+}
+
+int iskra_dsmc_counts(iskra_ctx *c, int dsmc_id, int reset, int64_t *out) {
+  CHECK_CTX(c);
+  if (dsmc_id < 0 || dsmc_id >= c->ndsmc) {
+    isk_set_error("dsmc id %d out of range", dsmc_id);
+    return ISKRA_E_ARG;
+  }
+  return isk_dsmc_counts(c, dsmc_id, reset, out);
+}
+
 /* ---- RNG ---- */
 int iskra_rng_set(iskra_ctx *c, uint64_t seed) {
   CHECK_CTX(c);
@@ -1208,6 +1247,7 @@ int iskra_step(iskra_ctx *c, double dt, int64_t nsteps) {
       for (int m = 0; m < c->nmcc; ++m) RC(isk_mcc_run(c, m, dt, nullptr));
       RC(isk_prof_end(c, ev, ISKRA_K_MCC, 0));
     }
+    for (int m = 0; m < c->ndsmc; ++m) RC(isk_dsmc_run(c, m, dt, nullptr));
     RC(isk_push_deposit_all(c, dt));
     int iters = 0;
     double rr = 0.0;

@@ -30,6 +30,7 @@
 
 #define ISK_MAX_SPECIES 8
 #define ISK_MAX_MCC 4
+#define ISK_MAX_DSMC 4
 #define ISK_MAX_COLL 8
 #define ISK_MAX_TAGS 16
 /* largest number of sort bins (tiles, or cells in a tile); LDS-resident */
@@ -125,6 +126,19 @@ struct IskMcc {
   uint32_t step;
 };
 
+/* one DSMC interaction (a single DSMC.ElasticCollision) */
+struct IskDsmc {
+  int src = -1, tgt = -1;
+  int ntab = 0;
+  double sgmax = 0.0;              /* maximum(rate) * argmax(rate) */
+  double *d_g = nullptr, *d_sig = nullptr;
+  double *rem = nullptr;           /* cells: collisions_remaining */
+  unsigned long long *nu = nullptr;     /* nodes: accepted events since reset */
+  unsigned long long *events = nullptr; /* accepted events of the last call */
+  int *rng = nullptr;              /* 4 x cells: cell ranges of source, target */
+  uint32_t stream = 0, step = 0;
+};
+
 struct IskPoisson {
   int kind = -1;
   double eps0 = 1.0;
@@ -177,6 +191,8 @@ struct iskra_ctx {
   IskSpecies sp[ISK_MAX_SPECIES];
   int nmcc = 0;
   IskMcc mcc[ISK_MAX_MCC];
+  int ndsmc = 0;
+  IskDsmc dsmc[ISK_MAX_DSMC];
   IskPoisson ps;
   /* fields */
   double *rho = nullptr, *phi = nullptr, *bg = nullptr, *vol = nullptr;
@@ -296,6 +312,11 @@ int isk_mcc_setup(iskra_ctx *c, int id);
 int isk_mcc_run(iskra_ctx *c, int id, double dt, int64_t *events);
 int isk_rng_export(iskra_ctx *c, uint32_t stream, uint32_t step,
                    const uint32_t *ids, int64_t n, int nblocks, uint32_t *out);
+int isk_dsmc_add(iskra_ctx *c, int src, int tgt, int ntab, const double *tab_g,
+                 const double *tab_sig, int *id);
+int isk_dsmc_run(iskra_ctx *c, int idx, double dt, int64_t *events);
+int isk_dsmc_counts(iskra_ctx *c, int idx, int reset, int64_t *out);
+void isk_dsmc_release(IskDsmc *m);
 int isk_noalloc_init(int device);
 int isk_rng_export_noalloc(iskra_ctx *c, uint32_t stream, uint32_t step,
                            const uint32_t *ids, int64_t n, int nblocks, uint32_t *out);

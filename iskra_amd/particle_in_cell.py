@@ -354,7 +354,8 @@ def bind(config):
         config.solver._ctx = ctx
         config.solver._configure(ctx)
     for inter in config.interactions:
-        if isinstance(inter, chemistry.MonteCarloCollisions):
+        if isinstance(inter, (chemistry.MonteCarloCollisions,
+                              chemistry.DirectSimulationMonteCarlo)):
             inter._bind(ctx, g)
     return ctx
 

@@ -76,6 +76,14 @@ How each oracle function is pinned (checks live in tests/):
                          axis-crossing sign quirk (cos >= 0) by hand.
   mir_push_deposit_rz    against ref_push + ref_rz_transform in FP32
                          tolerance, bit for bit against the device.
+  mir_dsmc               closed forms with the draws it takes: floor(Nc) trials
+                         per cell with the remainder carried (all accepted
+                         when sigma(g) g >= sgmax), momentum and kinetic
+                         energy of every cell conserved by each collision,
+                         post-collision relative velocity |g| (B, A cos C,
+                         A sin C) with B and C uniform (KS tests), a source
+                         index equal to the target index redrawn only when
+                         the species differ (tests/test_oracle_dsmc.py).
   orc_ws_run             (learned warm start, oracle/ws_oracle.c; no reference
                          counterpart) FP64 gradients against central finite
                          differences; convolution, pooling and upsampling
@@ -166,6 +174,9 @@ def _declare(L):
         "mir_sample": (None, [C.c_int, I64, I64, D, D, D, D, C.c_int, C.c_int, P, P, P, P, U64,
                               U32, P, P, P, P, P, P, P]),
         "mir_sort_tile": (I64, [C.c_int, I64, C.c_int, C.c_int, P, P, P, P, P, P, P]),
+        "mir_sort_cell": (I64, [C.c_int, I64, C.c_int, C.c_int, P, P, P, P, P, P, P]),
+        "mir_dsmc": (I64, [C.c_int, C.c_int, C.c_int, I64, P, P, P, P, I64, P, P, P, P, C.c_int,
+                           D, D, D, D, D, D, D, D, C.c_int, P, P, P, P, U32, U32, U32, U32]),
         "orc_ws_param_count": (I64, [C.c_int, C.c_int]),
         "orc_fix32": (I64, [F]),
         "orc_fixprod32": (I64, [F, F]),
@@ -568,6 +579,38 @@ class MirSim:
         self.tagval = np.zeros(17)
         self.tagval[:len(tagval)] = tagval
 
+    def add_dsmc(self, src, tgt, tab_g, tab_sigma):
+        """One DSMC.ElasticCollision interaction (equal weights), run after
+        the MCC interactions of each step, as iskra_step orders them."""
+        g = np.ascontiguousarray(tab_g, dtype=np.float64)
+        sg = np.ascontiguousarray(tab_sigma, dtype=np.float64)
+        km = int(np.argmax(sg))
+        d = dict(src=src, tgt=tgt, g=g, sig=sg, sgmax=float(sg[km] * g[km]),
+                 rem=np.zeros(self.ncx * self.ncy), nu=np.zeros(self.nx * self.ny, np.int64),
+                 step=0, stream=0x44530000 + len(getattr(self, "dsmcs", [])), last=0)
+        if not hasattr(self, "dsmcs"):
+            self.dsmcs = []
+        self.dsmcs.append(d)
+        return len(self.dsmcs) - 1
+
+    def dsmc_run(self, k, dt):
+        d = self.dsmcs[k]
+        A, B = self.species[d["src"]], self.species[d["tgt"]]
+        same = d["src"] == d["tgt"]
+        self.sort(d["src"], exact=True)
+        if not same:
+            self.sort(d["tgt"], exact=True)
+        k0, k1 = self.seed & 0xFFFFFFFF, (self.seed >> 32) & 0xFFFFFFFF
+        ev = lib().mir_dsmc(self.ncx, self.ncy, self.nx, A.np, ptr(A.cell), ptr(A.vx), ptr(A.vy),
+                            ptr(A.vz), B.np, ptr(B.cell), ptr(B.vx), ptr(B.vy), ptr(B.vz),
+                            int(same), A.w0, B.w0, self.dx, self.dy, dt, d["sgmax"],
+                            A.m / (A.m + B.m), B.m / (A.m + B.m), len(d["g"]), ptr(d["g"]),
+                            ptr(d["sig"]), ptr(d["rem"]), ptr(d["nu"]), d["step"], d["stream"],
+                            k0, k1)
+        d["step"] += 1
+        d["last"] = ev
+        return ev
+
     def add_mcc(self, src, ion, types, energy, tables, target_m, target_q, target_T, tn):
         off = [0]
         eps_all, sig_all = [], []
@@ -717,11 +760,11 @@ class MirSim:
         self.phi = np.ascontiguousarray(phi, dtype=np.float64).copy()
         lib().mir_efield(self.dim, self.nx, self.ny, self.dx, self.dy, ptr(self.phi), ptr(self.E))
 
-    def sort(self, s):
+    def sort(self, s, exact=False):
         sp = self.species[s]
-        sp.np = int(lib().mir_sort_tile(self.dim, sp.np, self.T, self.ntx, ptr(sp.cell),
-                                        ptr(sp.ox), ptr(sp.oy), ptr(sp.vx), ptr(sp.vy),
-                                        ptr(sp.vz), ptr(sp.id)))
+        f = lib().mir_sort_cell if exact else lib().mir_sort_tile
+        sp.np = int(f(self.dim, sp.np, self.T, self.ntx, ptr(sp.cell), ptr(sp.ox), ptr(sp.oy),
+                      ptr(sp.vx), ptr(sp.vy), ptr(sp.vz), ptr(sp.id)))
 
     def waveform(self, t):
         for tag, (off, amp, f, ph_turns) in self.wf.items():
@@ -737,6 +780,8 @@ class MirSim:
         deposit of every species, rho."""
         for k in range(len(self.mccs)):
             self.mcc_run(k, dt)
+        for k in range(len(getattr(self, "dsmcs", []))):
+            self.dsmc_run(k, dt)
         for s in range(len(self.species)):
             self.push(s, dt)
         self.compute_rho()

@@ -1252,8 +1252,11 @@ EXPORT void mir_sample(int dim, int64_t n, int64_t first_id, double x0, double y
 /* stable counting sort by key = tile of the cell (or the dead bin) with
  * compaction of discarded particles: the GPU's ordering rule.  Arrays are
  * permuted in place; returns the live count. */
-EXPORT int64_t mir_sort_tile(int dim, int64_t np, int T, int ntx, int32_t *cell, float *ox,
-                             float *oy, float *vx, float *vy, float *vz, uint32_t *id) {
+/* stable counting sort of the live particles by tile (exact = 0) or by
+ * (tile, cell inside the tile) (exact = 1: the device's two passes, cell in
+ * tile then tile, both stable); discarded particles are dropped */
+static int64_t mir_sort_key(int dim, int64_t np, int T, int ntx, int exact, int32_t *cell,
+                            float *ox, float *oy, float *vx, float *vy, float *vz, uint32_t *id) {
   int log2T = 0;
   while ((1 << log2T) < T) ++log2T;
   int64_t *perm = (int64_t *)malloc(sizeof(int64_t) * (np > 0 ? np : 1));
@@ -1264,6 +1267,10 @@ EXPORT int64_t mir_sort_tile(int dim, int64_t np, int T, int ntx, int32_t *cell,
     if (c < 0) { key[p] = -1; continue; }
     int i = c & 0xFFFF, j = c >> 16;
     key[p] = (i >> log2T) + ntx * (j >> log2T);
+    if (exact) {
+      int local = (i & (T - 1)) + (dim == 2 ? ((j & (T - 1)) << log2T) : 0);
+      key[p] = key[p] * (dim == 2 ? T * T : T) + local;
+    }
     if (key[p] > maxkey) maxkey = key[p];
   }
   int64_t *cnt = (int64_t *)calloc((size_t)maxkey + 2, sizeof(int64_t));
@@ -1290,4 +1297,107 @@ EXPORT int64_t mir_sort_tile(int dim, int64_t np, int T, int ntx, int32_t *cell,
   free(key);
   free(cnt);
   return live;
+}
+
+EXPORT int64_t mir_sort_tile(int dim, int64_t np, int T, int ntx, int32_t *cell, float *ox,
+                             float *oy, float *vx, float *vy, float *vz, uint32_t *id) {
+  return mir_sort_key(dim, np, T, ntx, 0, cell, ox, oy, vx, vy, vz, id);
+}
+
+EXPORT int64_t mir_sort_cell(int dim, int64_t np, int T, int ntx, int32_t *cell, float *ox,
+                             float *oy, float *vx, float *vy, float *vz, uint32_t *id) {
+  return mir_sort_key(dim, np, T, ntx, 1, cell, ox, oy, vx, vy, vz, id);
+}
+
+/* PIC.perform!(::DirectSimulationMonteCarlo) (Chemistry/src/dsmc.jl:81-142)
+ * with one DSMC.ElasticCollision of equal weights (dsmc.jl:32-79), as the
+ * device runs it: both species already sorted by exact cell (mir_sort_cell),
+ * cell t = i + ncx j owns the draws (t, step, stream, block); per cell the
+ * reference's Nc and remainder, then floor(Nc) trials in order.  rem: cells,
+ * nu: nodes (accumulated).  Returns the accepted collisions. */
+EXPORT int64_t mir_dsmc(int ncx, int ncy, int nx, int64_t npa, const int32_t *cella, float *avx,
+                        float *avy, float *avz, int64_t npb, const int32_t *cellb, float *bvx,
+                        float *bvy, float *bvz, int same, double Wa, double Wb, double dx,
+                        double dy, double dt, double sgmax, double mr1, double mr2, int ntab,
+                        const double *tg, const double *ts, double *rem, int64_t *nu,
+                        uint32_t step, uint32_t stream, uint32_t k0, uint32_t k1) {
+  const int ncell = ncx * ncy;
+  int *rng = (int *)calloc((size_t)4 * ncell, sizeof(int));
+  int *as = rng, *ae = rng + ncell, *bs = rng + 2 * ncell, *be = rng + 3 * ncell;
+  for (int64_t p = 0; p < npa; ++p) {
+    int c = cella[p];
+    if (c < 0) continue;
+    int key = (c & 0xFFFF) + ncx * (c >> 16);
+    if (p == 0 || cella[p - 1] != c) as[key] = (int)p;
+    if (p == npa - 1 || cella[p + 1] != c) ae[key] = (int)(p + 1);
+  }
+  if (same) {
+    bs = as;
+    be = ae;
+  } else {
+    for (int64_t p = 0; p < npb; ++p) {
+      int c = cellb[p];
+      if (c < 0) continue;
+      int key = (c & 0xFFFF) + ncx * (c >> 16);
+      if (p == 0 || cellb[p - 1] != c) bs[key] = (int)p;
+      if (p == npb - 1 || cellb[p + 1] != c) be[key] = (int)(p + 1);
+    }
+  }
+  double Pab, Pba;
+  if (Wa > Wb) { Pab = Wb / Wa; Pba = 1.0; } else { Pab = 1.0; Pba = Wa / Wb; }
+  int64_t total = 0;
+  for (int t = 0; t < ncell; ++t) {
+    const int a0 = as[t], Na = ae[t] - a0, b0 = bs[t], Nb = be[t] - b0;
+    if (Na < 2 || Nb < 2) continue;
+    const double na = (double)Na * Wa / (dx * dy);
+    double Nc = na * (double)Nb * dt * sgmax;
+    Nc /= Pab + (Wb / Wa) * Pba;
+    if (!same) Nc *= 2.0;
+    Nc += rem[t];
+    const double fl = floor(Nc);
+    rem[t] = Nc - fl;
+    const long long ntr = (long long)fl;
+    MirDraws d = {(uint32_t)t, step, stream, k0, k1, 0, 0, {0, 0, 0, 0}};
+    int64_t acc = 0;
+    for (long long k = 0; k < ntr; ++k) {
+      int ia = (int)floor((double)Na * md_next(&d));
+      if (ia > Na - 1) ia = Na - 1;
+      const int sR = a0 + ia;
+      int ib = (int)floor((double)Nb * md_next(&d));
+      if (ib > Nb - 1) ib = Nb - 1;
+      int tR = b0 + ib;
+      while (!same && sR == tR) {
+        ib = (int)floor((double)Nb * md_next(&d));
+        if (ib > Nb - 1) ib = Nb - 1;
+        tR = b0 + ib;
+      }
+      const double sx = avx[sR], sy = avy[sR], sz = avz[sR];
+      const double tx = bvx[tR], ty = bvy[tR], tz = bvz[tR];
+      const double gx = sx - tx, gy = sy - ty, gz = sz - tz;
+      const double gn = sqrt(gx * gx + gy * gy + gz * gz);
+      const double sg = ref_sigma(ntab, tg, ts, gn) * gn;
+      const double P = sg / sgmax;
+      const double R = md_next(&d);
+      if (P < R) continue;
+      const double B = 2.0 * md_next(&d) - 1.0;
+      const double A = sqrt(1.0 - B * B);
+      double sC, cC;
+      orc_sincos2pi(md_next(&d), &sC, &cC);
+      const double rx = gn * B, ry = gn * (A * cC), rz = gn * (A * sC);
+      const double cx = mr1 * sx + mr2 * tx, cy = mr1 * sy + mr2 * ty, cz = mr1 * sz + mr2 * tz;
+      avx[sR] = (float)(cx + mr2 * rx);
+      avy[sR] = (float)(cy + mr2 * ry);
+      avz[sR] = (float)(cz + mr2 * rz);
+      bvx[tR] = (float)(cx - mr1 * rx);
+      bvy[tR] = (float)(cy - mr1 * ry);
+      bvz[tR] = (float)(cz - mr1 * rz);
+      ++acc;
+    }
+    if (acc) {
+      nu[(t / ncx) * nx + (t % ncx)] += acc;
+      total += acc;
+    }
+  }
+  free(rng);
+  return total;
 }
