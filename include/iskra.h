@@ -292,6 +292,66 @@ int iskra_dsmc_run(iskra_ctx *ctx, int dsmc_id, double dt, int64_t *events);
  * per node (nodes x 1, col-major). */
 int iskra_dsmc_counts(iskra_ctx *ctx, int dsmc_id, int reset, int64_t *out);
 
+/* ---- surfaces, electrodes and the external circuit --------------------- */
+/* Surfaces of the cell-edge tracker (ParticleInCell/src/pic/surfaces/
+ * track.jl:9-25, hit.jl:30-56): absorbing (the particle is removed),
+ * reflective (the velocity component along the edge normal flips and the
+ * rest of the step is mirrored), periodic (a hit does nothing: the particle
+ * continues and the after_push boundary applies). */
+enum { ISKRA_SURF_ABSORBING = 1, ISKRA_SURF_REFLECTIVE = 2, ISKRA_SURF_PERIODIC = 3 };
+/* create_surface_tracker(grid, ds) (track.jl:101-114): every edge of the grid
+ * boundary gets default_surface.  From then on every push of a 2-D Cartesian
+ * grid runs track! + check! (ParticleInCell.jl:55-60; check.jl:17-68): a
+ * particle that starts the step in a cell next to a surface edge follows its
+ * straight path cell by cell (FP64, the reference's arithmetic with the grid
+ * spacing dx along both axes as check uses it) and meets every surface edge
+ * it crosses.  Absorbed particles are discarded (removed at the next sort,
+ * like discard!); the reference's "too fast" message is not printed. */
+int iskra_tracker_create(iskra_ctx *ctx, int default_surface);
+/* track_surface!(tracker, nodes, surface) (track.jl:116-118): every cell edge
+ * whose two end nodes are in the mask (one uint8 per node, x fastest) gets
+ * the surface (build_surface_lookup!, track.jl:45-60; later calls override). */
+int iskra_tracker_surface(iskra_ctx *ctx, const uint8_t *nodes, int surface);
+/* The tracker as the push kernels read it (no reference counterpart: the
+ * reference keeps a Dict of cell pairs): per cell (x fastest) four codes for
+ * the edges -x, +x, -y, +y (0 none, 1 absorbing or fixed electrode,
+ * 2 reflective, 3 periodic, 16 + k floating electrode with sigma dof k) and
+ * one tracked flag (the cell appears in a surface entry on either side). */
+int iskra_tracker_table(iskra_ctx *ctx, uint8_t *codes, uint8_t *tracked);
+/* create_electrode(nodes, config; fixed, sigma, phi) (problem/configuration.jl
+ * :17-76; circuit_coupling.jl:4-15,45-62).  Needs the Poisson solver set
+ * (iskra_poisson_set first; setting it again removes the electrodes) and
+ * creates the default absorbing tracker if none exists.  fixed != 0:
+ * apply_dirichlet(nodes, phi) under a Dirichlet tag of its own, and an
+ * absorbing surface on the mask's edges.  fixed == 0 (floating): a surface
+ * charge sigma (C/m^2, initial value `sigma`) and the rows apply_neumann
+ * writes for the mask (generalized_poisson.jl:222-269; the system is then
+ * not symmetric and is solved by BiCGStab, PCG kind only); every particle
+ * its edges absorb adds q w0 to the electrode's charge and q w0 / area to
+ * sigma (area from calculate_area, dz = 1). */
+int iskra_electrode_create(iskra_ctx *ctx, const uint8_t *nodes, int fixed, double sigma,
+                           double phi, int *electrode_id);
+/* sigma and collected charge (floating; 0 for fixed), phi (fixed: its value;
+ * floating: the potential at its reference node, the first mask node in
+ * column-major order), area.  Any argument may be NULL. */
+int iskra_electrode_state(iskra_ctx *ctx, int electrode_id, double *sigma, double *phi,
+                          double *dq, double *area);
+/* rlc(@netlist ...) (Circuit/src/Circuit.jl:58-66) with V(t) = v_offset +
+ * v_amplitude sin(2 pi v_freq t + v_phase) and, as external device, the
+ * PlasmaDevice of a floating (positive) and a fixed (negative) electrode
+ * (circuit_coupling.jl:17-33), or none (-1, -1: ShortedConnection, zero
+ * voltage).  Each iskra_step then runs advance!(::CircuitRLC)
+ * (circuit_coupling.jl:35-43) after the push: advance_circuit! (Circuit.jl
+ * :117-136) with the external voltage phi(positive) - phi(negative) of the
+ * previous solve, and sigma of the first floating electrode (the reference's
+ * get_rhs(solver, :sigma, 1)) += -dt i / area(positive).  i = q = t = 0 at
+ * the call. */
+int iskra_circuit_set(iskra_ctx *ctx, double R, double L, double C, double v_offset,
+                      double v_amplitude, double v_freq, double v_phase, int positive,
+                      int negative);
+/* the circuit's current, capacitor charge, time and last external voltage */
+int iskra_circuit_state(iskra_ctx *ctx, double *i, double *q, double *t, double *vext);
+
 /* ---- random numbers ------------------------------------------------------ */
 /* The seed of all device draws (Random.seed!, src/iskra.jl:22-26). */
 int iskra_rng_set(iskra_ctx *ctx, uint64_t seed);

@@ -19,6 +19,7 @@ BND_NONE, BND_WRAP, BND_DISCARD = 0, 1, 2
 F_RHO, F_PHI, F_EX, F_EY, F_BG, F_DENSITY, F_TARGET = range(7)
 POISSON_DIRECT, POISSON_PCG, POISSON_MG, POISSON_MGPCG, POISSON_SPECTRAL = range(5)
 PUSHER_XY, PUSHER_RZ = 0, 1
+SURF_ABSORBING, SURF_REFLECTIVE, SURF_PERIODIC = 1, 2, 3
 WARM_ZERO, WARM_PREV = 0, 1
 COLL_ELASTIC_ISOTROPIC, COLL_ELASTIC_BACKWARD, COLL_EXCITATION, COLL_IONIZATION = range(4)
 COLL_INELASTIC_BACKWARD = 4
@@ -27,7 +28,9 @@ NOALLOC_SCRATCH_BYTES = 4 << 20
 EXPORTS = [
     "iskra_last_error", "iskra_version", "iskra_ctx_create", "iskra_ctx_destroy", "iskra_sync",
     "iskra_grid_set", "iskra_grid_set_axial", "iskra_pusher_set", "iskra_dsmc_add",
-    "iskra_dsmc_run", "iskra_dsmc_counts", "iskra_tuning_set", "iskra_tuning_get", "iskra_species_add",
+    "iskra_dsmc_run", "iskra_dsmc_counts", "iskra_tracker_create", "iskra_tracker_surface",
+    "iskra_tracker_table", "iskra_electrode_create", "iskra_electrode_state",
+    "iskra_circuit_set", "iskra_circuit_state", "iskra_tuning_set", "iskra_tuning_get", "iskra_species_add",
     "iskra_species_set_boundary", "iskra_species_sort_every", "iskra_species_margin",
     "iskra_species_upload",
     "iskra_species_download",
@@ -96,6 +99,13 @@ def lib():
             "iskra_dsmc_add": (I, [P, I, I, I, P, P, C.POINTER(I)]),
             "iskra_dsmc_run": (I, [P, I, D, C.POINTER(I64)]),
             "iskra_dsmc_counts": (I, [P, I, I, P]),
+            "iskra_tracker_create": (I, [P, I]),
+            "iskra_tracker_surface": (I, [P, P, I]),
+            "iskra_tracker_table": (I, [P, P, P]),
+            "iskra_electrode_create": (I, [P, P, I, D, D, C.POINTER(I)]),
+            "iskra_electrode_state": (I, [P, I, P, P, P, P]),
+            "iskra_circuit_set": (I, [P, D, D, D, D, D, D, D, I, I]),
+            "iskra_circuit_state": (I, [P, P, P, P, P]),
             "iskra_tuning_set": (I, [P, C.POINTER(Tuning)]),
             "iskra_tuning_get": (I, [P, C.POINTER(Tuning)]),
             "iskra_species_add": (I, [P, D, D, D, I64, I, C.POINTER(I)]),
@@ -447,6 +457,46 @@ class Context:
         out = np.zeros(self.nn, dtype=np.int64)
         check(lib().iskra_dsmc_counts(self.h, int(did), int(reset), _p(out)))
         return out
+
+    # surfaces, electrodes, circuit
+    def tracker(self, default_surface=None):
+        check(lib().iskra_tracker_create(self.h, SURF_ABSORBING if default_surface is None
+                                         else int(default_surface)))
+
+    def tracker_surface(self, nodes, surface):
+        m = np.ascontiguousarray(np.asarray(nodes, dtype=np.uint8).ravel())
+        check(lib().iskra_tracker_surface(self.h, _p(m), int(surface)))
+
+    def tracker_table(self):
+        ncell = (self.nodes[0] - 1) * (self.nodes[1] - 1)
+        codes = np.zeros(4 * ncell, np.uint8)
+        tracked = np.zeros(ncell, np.uint8)
+        check(lib().iskra_tracker_table(self.h, _p(codes), _p(tracked)))
+        return codes, tracked
+
+    def electrode(self, nodes, fixed=False, sigma=0.0, phi=0.0):
+        """nodes: one flag per node, x fastest; returns the electrode id."""
+        m = np.ascontiguousarray(np.asarray(nodes, dtype=np.uint8).ravel())
+        out = C.c_int()
+        check(lib().iskra_electrode_create(self.h, _p(m), int(bool(fixed)), float(sigma),
+                                           float(phi), C.byref(out)))
+        return out.value
+
+    def electrode_state(self, eid):
+        v = [C.c_double() for _ in range(4)]
+        check(lib().iskra_electrode_state(self.h, int(eid), *[C.byref(x) for x in v]))
+        return dict(sigma=v[0].value, phi=v[1].value, dq=v[2].value, area=v[3].value)
+
+    def circuit(self, R, L, C_, v_offset=0.0, v_amplitude=0.0, v_freq=0.0, v_phase=0.0,
+                positive=-1, negative=-1):
+        check(lib().iskra_circuit_set(self.h, float(R), float(L), float(C_), float(v_offset),
+                                      float(v_amplitude), float(v_freq), float(v_phase),
+                                      int(positive), int(negative)))
+
+    def circuit_state(self):
+        v = [C.c_double() for _ in range(4)]
+        check(lib().iskra_circuit_state(self.h, *[C.byref(x) for x in v]))
+        return dict(i=v[0].value, q=v[1].value, t=v[2].value, vext=v[3].value)
 
     def rng(self, seed):
         check(lib().iskra_rng_set(self.h, int(seed)))

@@ -106,6 +106,8 @@ void isk_species_release(IskSpecies *s) {
   isk_free(s->live_cnt);
   isk_free(s->gfall);
   isk_free(s->cnt);
+  isk_free(s->ehit);
+  s->ehit = nullptr;
   s->hdr = nullptr;
   s->chunks = nullptr;
   s->tile_chunk_begin = nullptr;
@@ -151,8 +153,10 @@ int isk_species_alloc(iskra_ctx *c, IskSpecies *s) {
   int nn = isk_nodes(c);
   if (isk_alloc((void **)&s->hdr, HDR_WORDS * 8) ||
       isk_alloc((void **)&s->gfall, (size_t)nn * 8) ||
-      isk_alloc((void **)&s->cnt, (size_t)nn * 8))
+      isk_alloc((void **)&s->cnt, (size_t)nn * 8) ||
+      isk_alloc((void **)&s->ehit, ISK_MAX_ELECTRODES * 8))
     return ISKRA_E_HIP;
+  ISK_HIP(hipMemset(s->ehit, 0, ISK_MAX_ELECTRODES * 8));
   ISK_HIP(hipMemset(s->hdr, 0, HDR_WORDS * 8));
   ISK_HIP(hipMemset(s->cnt, 0, (size_t)nn * 8));
   ISK_HIP(hipMemset(s->gfall, 0, (size_t)nn * 8));
@@ -312,6 +316,7 @@ static void release_fields(iskra_ctx *c) {
   c->nmcc = 0;
   for (int m = 0; m < c->ndsmc; ++m) isk_dsmc_release(&c->dsmc[m]);
   c->ndsmc = 0;
+  isk_electrodes_release(c);
   isk_poisson_free(c);
   c->ps.kind = -1;
   void *ptrs[] = {c->rho, c->phi, c->bg, c->vol, c->E, c->sort_H, c->sort_S, c->sort_tot,
@@ -999,6 +1004,17 @@ int iskra_poisson_set(iskra_ctx *c, int kind, double eps0, const int8_t *tags, i
   isk_poisson_free(c);
   IskPoisson &p = c->ps;
   int nn = isk_nodes(c);
+  /* a new solver drops the electrodes built on the old one (their rows and
+   * tags); the tracker keeps its surfaces */
+  c->nelec = 0;
+  c->nfloat = 0;
+  c->circuit_on = 0;
+  p.h_srow.clear();
+  p.h_sd.clear();
+  p.h_sc.clear();
+  p.h_ssig.clear();
+  p.h_snb.clear();
+  p.h_sdof.clear();
   p.kind = kind;
   p.eps0 = eps0;
   p.rtol = rtol > 0 ? rtol : 1e-8;
@@ -1249,6 +1265,7 @@ int iskra_step(iskra_ctx *c, double dt, int64_t nsteps) {
     }
     for (int m = 0; m < c->ndsmc; ++m) RC(isk_dsmc_run(c, m, dt, nullptr));
     RC(isk_push_deposit_all(c, dt));
+    if (c->d_elec) RC(isk_electrodes_after_push(c, dt));
     int iters = 0;
     double rr = 0.0;
     int E_done = 0;

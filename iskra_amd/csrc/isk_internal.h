@@ -31,6 +31,14 @@
 #define ISK_MAX_SPECIES 8
 #define ISK_MAX_MCC 4
 #define ISK_MAX_DSMC 4
+/* floating electrodes (each one sigma degree of freedom) */
+#define ISK_MAX_ELECTRODES 8
+
+/* surface codes of the tracker's edge table (per cell, per direction -x, +x,
+ * -y, +y): none, absorbing (absorbing surfaces and fixed electrodes),
+ * reflective, a surface whose hit does nothing, floating electrode k */
+enum { ISK_SURF_NONE = 0, ISK_SURF_ABSORB = 1, ISK_SURF_REFLECT = 2, ISK_SURF_PASS = 3,
+       ISK_SURF_ELECTRODE = 16 };
 #define ISK_MAX_COLL 8
 #define ISK_MAX_TAGS 16
 /* largest number of sort bins (tiles, or cells in a tile); LDS-resident */
@@ -91,6 +99,7 @@ struct IskSpecies {
   unsigned long long *gfall; /* nodes */
   long long *cnt;            /* nodes: merged deposit (fixed point) */
   int *live_cnt;             /* max_chunks: live particles per push work-group */
+  unsigned long long *ehit;  /* ISK_MAX_ELECTRODES: particles absorbed by each floating electrode */
   int64_t max_chunks;
   int deposited;             /* cnt holds this step's deposit */
 };
@@ -139,6 +148,28 @@ struct IskDsmc {
   uint32_t stream = 0, step = 0;
 };
 
+/* d_elec layout (doubles): sigma, collected charge and area per sigma dof,
+ * then the RLC circuit: current, charge, time, last external voltage */
+enum { EL_SIGMA = 0, EL_DQ = ISK_MAX_ELECTRODES, EL_AREA = 2 * ISK_MAX_ELECTRODES,
+       EL_CIRC = 3 * ISK_MAX_ELECTRODES, EL_WORDS = 3 * ISK_MAX_ELECTRODES + 8 };
+
+/* the surface tracker: edge table and tracked cells on the device (host copy
+ * of the codes kept for updates) */
+struct IskTracker {
+  int on = 0;
+  std::vector<uint8_t> h_surf;
+  uint8_t *surf = nullptr, *tracked = nullptr;
+};
+
+/* electrodes (create_electrode, problem/configuration.jl:17-76) */
+struct IskElectrode {
+  int fixed = 0;
+  int tag = 0;        /* fixed: its Dirichlet tag */
+  int ref_node = 0;   /* first node of the mask (column-major order) */
+  double area = 0.0;
+  double sigma0 = 0.0;
+};
+
 struct IskPoisson {
   int kind = -1;
   double eps0 = 1.0;
@@ -177,6 +208,18 @@ struct IskPoisson {
          *sp_T1 = nullptr, *sp_T2 = nullptr;
   int last_iters = 0;
   double last_relres = 0.0;
+  /* floating electrodes: rows replaced as apply_neumann does
+   * (generalized_poisson.jl:222-269), kept in the solver's scaled form
+   * sd x_n - sum_q sc_q x_{nb_q} = rho_n dx^2/eps0 + ssig sigma[sdof]; every
+   * other row is the 5-point row.  Such a system is not symmetric: it is
+   * solved by BiCGStab (poisson.hip) */
+  std::vector<int> h_srow;           /* nodes: special row index or -1 */
+  std::vector<double> h_sd, h_sc, h_ssig;
+  std::vector<int> h_snb, h_sdof;
+  int *d_srow = nullptr, *d_snb = nullptr, *d_sdof = nullptr;
+  double *d_sd = nullptr, *d_sc = nullptr, *d_ssig = nullptr;
+  double *bi = nullptr;              /* BiCGStab vectors and partials */
+  double *d_dg = nullptr;            /* row diagonals (Jacobi) */
 };
 
 struct iskra_ctx {
@@ -193,6 +236,17 @@ struct iskra_ctx {
   IskMcc mcc[ISK_MAX_MCC];
   int ndsmc = 0;
   IskDsmc dsmc[ISK_MAX_DSMC];
+  IskTracker trk;
+  int nelec = 0;
+  IskElectrode elec[ISK_MAX_ELECTRODES];
+  int nfloat = 0;              /* floating electrodes = sigma degrees of freedom */
+  int float_elec[ISK_MAX_ELECTRODES]; /* electrode id of each sigma dof */
+  /* device state of the floating electrodes, per sigma dof: sigma (C/m^2),
+   * charge collected (C), area; the RLC circuit (RLC_WORDS doubles) */
+  double *d_elec = nullptr;
+  int circuit_on = 0;
+  double circ_R = 0, circ_L = 0, circ_C = 0, circ_v[4] = {0, 0, 0, 0};
+  int circ_pos = -1, circ_neg = -1;
   IskPoisson ps;
   /* fields */
   double *rho = nullptr, *phi = nullptr, *bg = nullptr, *vol = nullptr;
@@ -303,6 +357,9 @@ int isk_poisson_setup(iskra_ctx *c);
 int isk_poisson_free(iskra_ctx *c);
 int isk_poisson_solve(iskra_ctx *c, int warm, int *iters, double *relres);
 int isk_poisson_check(iskra_ctx *c, double *relres);
+int isk_bicgstab_solve(iskra_ctx *c, int warm, int *iters, double *relres);
+int isk_electrodes_after_push(iskra_ctx *c, double dt);
+void isk_electrodes_release(iskra_ctx *c);
 int isk_poisson_solve_ex(iskra_ctx *c, int warm, int *iters, double *relres, int with_E,
                          int *E_done, int packed);
 struct IskSpPack;

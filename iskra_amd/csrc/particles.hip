@@ -726,6 +726,12 @@ struct PushArgs {
   float *ox, *oy, *vx, *vy;
   float *vz;     /* RZ pusher: v_theta, rotated with v_r */
   double dtdr;   /* RZ pusher: dt / dr */
+  /* surface tracker (PM_TRACK): edge codes per cell and direction, tracked
+   * cells, per-electrode hit counters of this species, dt and the spacing
+   * the reference's check uses */
+  const uint8_t *surf, *tracked;
+  unsigned long long *ehit;
+  double dt_d, dh_d;
   long long *hdr;
   const IskChunk *chunks;
   long long *partial;
@@ -814,6 +820,103 @@ __device__ __forceinline__ void rz_transform(int *i, float *hx, float *vr, float
   *vt = (float)nvt;
 }
 
+/* pusher modes of the push kernels */
+enum { PM_XY = 0, PM_RZ = 1, PM_TRACK = 2 };
+
+/* track! + check! + hit! of the reference's surface tracker
+ * (pic/surfaces/track.jl:152-161, check.jl:17-68, hit.jl:1-56,
+ * circuit_coupling.jl:47-62) for one particle that starts in a tracked cell:
+ * from its offsets (hx, hy) before the push it follows the straight path
+ * with the pushed velocity (vx, vy) cell by cell for dt, in FP64 and in the
+ * reference's operation order (one spacing dh for both axes, as check uses
+ * st.dh).  At each edge crossed: no surface -> keep tracing from the next
+ * cell; absorbing surface or fixed electrode -> absorbed; floating electrode
+ * k -> absorbed and counted on electrode k; reflective surface -> the
+ * velocity components along the edge normal flip, the position is corrected
+ * by the reflected rest of the path (the reference's px -= v dt' ... px +=
+ * v' dt'), and tracing resumes in the cell the particle came from
+ * (scattered!); any other surface ends the tracing.  Returns 1 if absorbed;
+ * *cx, *cy receive the position corrections in cells. */
+#define ISK_TRACK_MAX 64
+__device__ __forceinline__ int track_path(const PushArgs &a, int i, int j, float hx0, float hy0,
+                                          float *vx, float *vy, double *cx, double *cy) {
+  const double h = a.dh_d;
+  double T = a.dt_d, hx = hx0, hy = hy0;
+  *cx = 0.0;
+  *cy = 0.0;
+  for (int it = 0; it < ISK_TRACK_MAX; ++it) {
+    const double vxd = *vx, vyd = *vy;
+    const double dxs = vxd > 0 ? h * (1.0 - hx) : h * hx;
+    const double dys = vyd > 0 ? h * (1.0 - hy) : h * hy;
+    const double tx = dxs / fabs(vxd), ty = dys / fabs(vyd);
+    if (T < tx && T < ty) return 0; /* stays in this cell */
+    int ni = i, nj = j, dir;
+    double nhx, nhy, Tn;
+    if (tx < ty) {
+      Tn = T - tx;
+      nhy = hy + vyd * tx / h;
+      if (vxd > 0) { ni = i + 1; nhx = 0.0; dir = 1; }
+      else { ni = i - 1; nhx = 1.0; dir = 0; }
+    } else {
+      Tn = T - ty;
+      nhx = hx + vxd * ty / h;
+      if (vyd > 0) { nj = j + 1; nhy = 0.0; dir = 3; }
+      else { nj = j - 1; nhy = 1.0; dir = 2; }
+    }
+    if (i < 0 || i >= a.ncx || j < 0 || j >= a.ncy) return 0; /* left the grid untracked */
+    const int code = a.surf[(i + a.ncx * j) * 4 + dir];
+    if (code == ISK_SURF_NONE) {
+      i = ni;
+      j = nj;
+      hx = nhx;
+      hy = nhy;
+      T = Tn;
+      continue;
+    }
+    if (code == ISK_SURF_ABSORB) return 1;
+    if (code >= ISK_SURF_ELECTRODE) {
+      atomicAdd(&a.ehit[code - ISK_SURF_ELECTRODE], 1ull);
+      return 1;
+    }
+    if (code != ISK_SURF_REFLECT) return 0; /* hit!(::Surface) does nothing */
+    if (ni != i) {
+      *vx = -*vx;
+      *cx += 2.0 * (double)*vx * Tn / h;
+    }
+    if (nj != j) {
+      *vy = -*vy;
+      *cy += 2.0 * (double)*vy * Tn / h;
+    }
+    /* scattered! (hit.jl:13-21) */
+    int si = ni, sj = nj;
+    double shx = nhx, shy = nhy;
+    if (nhx == 0.0) { si = ni - 1; shx = 1.0; }
+    if (nhx == 1.0) { si = ni + 1; shx = 0.0; }
+    if (nhy == 0.0) { sj = nj - 1; shy = 1.0; }
+    if (nhy == 1.0) { sj = nj + 1; shy = 0.0; }
+    i = si;
+    j = sj;
+    hx = shx;
+    hy = shy;
+    T = Tn;
+  }
+  return 0;
+}
+
+/* a position correction of c cells added to (cell, offset), FP64 */
+__device__ __forceinline__ int shift_coord(double c, float *o) {
+  const double t = (double)*o + c;
+  const double f = floor(t);
+  float h = (float)(t - f);
+  int k = (int)f;
+  if (h >= 1.0f) {
+    h = 0.0f;
+    ++k;
+  }
+  *o = h;
+  return k;
+}
+
 /* boundary on one axis; returns false if the particle is discarded */
 __device__ __forceinline__ bool apply_bnd(int *i, int n, int mode) {
   if (*i >= 0 && *i < n) return true;
@@ -857,8 +960,9 @@ __device__ __forceinline__ void push_load(PushRegs<DIM> &r, const int32_t *__res
   }
 }
 
-template <int DIM, bool RZ = false>
+template <int DIM, int PM = PM_XY>
 __device__ __forceinline__ void push_body(const PushArgs &a, const int blk) {
+  constexpr bool RZ = PM == PM_RZ, TRACK = PM == PM_TRACK;
   constexpr int BLK = PushShape<DIM>::block, UNR = PushShape<DIM>::unroll;
   extern __shared__ unsigned long long sD[]; /* WW deposit counters, then E */
   const int W = a.W;
@@ -981,17 +1085,34 @@ __device__ __forceinline__ void push_body(const PushArgs &a, const int blk) {
         }
         /* push with B = 0 (pushers.jl:37-50): v += (q/m) E dt, x += v dt */
         vx = fmaf(a.qmdt, Ex, vx);
-        if (RZ) {
-          vt = a.vz[start + kslot];
-          rz_transform(&i, &hx, &vx, &vt, a.dtdr);
-        } else {
-          i += advance_coord(vx, a.dtdx, &hx);
-        }
-        alive = apply_bnd(&i, a.ncx, a.bndx);
-        if (DIM == 2) {
+        if (TRACK && DIM == 2) {
+          /* track! before the push, check! after it (ParticleInCell.jl:55-60) */
           vy = fmaf(a.qmdt, Ey, vy);
-          j += advance_coord(vy, a.dtdy, &hy);
+          const float vx0 = vx, vy0 = vy;
+          double cx = 0.0, cy = 0.0;
+          int absorbed = 0;
+          if (a.tracked[i + a.ncx * j])
+            absorbed = track_path(a, i, j, hx, hy, &vx, &vy, &cx, &cy);
+          i += advance_coord(vx0, a.dtdx, &hx);
+          j += advance_coord(vy0, a.dtdy, &hy);
+          if (cx != 0.0) i += shift_coord(cx, &hx);
+          if (cy != 0.0) j += shift_coord(cy, &hy);
+          alive = apply_bnd(&i, a.ncx, a.bndx);
           alive = apply_bnd(&j, a.ncy, a.bndy) && alive;
+          if (absorbed) alive = false;
+        } else {
+          if (RZ) {
+            vt = a.vz[start + kslot];
+            rz_transform(&i, &hx, &vx, &vt, a.dtdr);
+          } else {
+            i += advance_coord(vx, a.dtdx, &hx);
+          }
+          alive = apply_bnd(&i, a.ncx, a.bndx);
+          if (DIM == 2) {
+            vy = fmaf(a.qmdt, Ey, vy);
+            j += advance_coord(vy, a.dtdy, &hy);
+            alive = apply_bnd(&j, a.ncy, a.bndy) && alive;
+          }
         }
       }
       if (alive) {
@@ -1056,9 +1177,9 @@ __device__ __forceinline__ void push_body(const PushArgs &a, const int blk) {
   }
 }
 
-template <int DIM, bool RZ = false>
+template <int DIM, int PM = PM_XY>
 __global__ __launch_bounds__(PushShape<DIM>::block) void k_push(PushArgs a) {
-  push_body<DIM, RZ>(a, blockIdx.x);
+  push_body<DIM, PM>(a, blockIdx.x);
 }
 
 /* arguments of one push launch reading buffer `b`; grid and LDS size */
@@ -1074,6 +1195,11 @@ static void push_prepare(iskra_ctx *c, int sid, double dt, int do_push, int b, P
   a.vy = s->vy[b];
   a.vz = s->vz[b];
   a.dtdr = dt / g.dx;
+  a.surf = c->trk.surf;
+  a.tracked = c->trk.tracked;
+  a.ehit = s->ehit;
+  a.dt_d = dt;
+  a.dh_d = g.dx;
   a.hdr = s->hdr;
   a.chunks = s->chunks;
   a.partial = s->partial;
@@ -1118,8 +1244,11 @@ static int push_launch(iskra_ctx *c, int sid, double dt, int do_push, int fill) 
     if (rc2) return rc2;
   }
   if (c->g.dim == 2 && c->pusher == ISKRA_PUSHER_RZ && do_push)
-    hipLaunchKernelGGL((k_push<2, true>), dim3(grid), dim3(PushShape<2>::block), shm, c->stream,
+    hipLaunchKernelGGL((k_push<2, PM_RZ>), dim3(grid), dim3(PushShape<2>::block), shm, c->stream,
                        a);
+  else if (c->g.dim == 2 && c->trk.on && do_push)
+    hipLaunchKernelGGL((k_push<2, PM_TRACK>), dim3(grid), dim3(PushShape<2>::block), shm,
+                       c->stream, a);
   else if (c->g.dim == 2)
     hipLaunchKernelGGL(k_push<2>, dim3(grid), dim3(PushShape<2>::block), shm, c->stream, a);
   else
@@ -1142,14 +1271,14 @@ struct PushMulti {
   int n;
 };
 
-template <int DIM, bool RZ = false>
+template <int DIM, int PM = PM_XY>
 __global__ __launch_bounds__(PushShape<DIM>::block) void k_push_multi(PushMulti m) {
   int s = 0;
   while (s + 1 < m.n && (int)blockIdx.x >= m.bstart[s + 1]) ++s;
   /* a copy in registers: read through a reference into the argument block,
    * the fields would be re-read from memory after every particle's stores */
   const PushArgs a = m.a[s];
-  push_body<DIM, RZ>(a, blockIdx.x - m.bstart[s]);
+  push_body<DIM, PM>(a, blockIdx.x - m.bstart[s]);
 }
 
 static int push_launch_all(iskra_ctx *c, double dt) {
@@ -1170,8 +1299,11 @@ static int push_launch_all(iskra_ctx *c, double dt) {
   int rc = isk_prof_begin(c, &ev);
   if (rc) return rc;
   if (c->g.dim == 2 && c->pusher == ISKRA_PUSHER_RZ)
-    hipLaunchKernelGGL((k_push_multi<2, true>), dim3(m.bstart[m.n]), dim3(PushShape<2>::block),
+    hipLaunchKernelGGL((k_push_multi<2, PM_RZ>), dim3(m.bstart[m.n]), dim3(PushShape<2>::block),
                        shm, c->stream, m);
+  else if (c->g.dim == 2 && c->trk.on)
+    hipLaunchKernelGGL((k_push_multi<2, PM_TRACK>), dim3(m.bstart[m.n]),
+                       dim3(PushShape<2>::block), shm, c->stream, m);
   else if (c->g.dim == 2)
     hipLaunchKernelGGL(k_push_multi<2>, dim3(m.bstart[m.n]), dim3(PushShape<2>::block), shm,
                        c->stream, m);

@@ -1326,13 +1326,339 @@ int isk_efield(iskra_ctx *c) {
 /* ------------------------------------------------------------------------ */
 /* host side                                                                 */
 /* ------------------------------------------------------------------------ */
+/* ------------------------------------------------------------------------ */
+/* BiCGStab for the systems with floating-electrode rows                     */
+/* ------------------------------------------------------------------------ */
+/* calculate_electric_potential (generalized_poisson.jl:372-378) when
+ * apply_neumann has replaced rows (floating electrodes): the system is not
+ * symmetric, so it is solved by right-preconditioned BiCGStab (van der
+ * Vorst 1992) with the Jacobi preconditioner, FP64 throughout.  Unknowns are
+ * the non-Dirichlet nodes; the sigma degrees of freedom are identity rows in
+ * the reference (b = sigma), so they enter as known right-hand-side terms.
+ * Like the CG kernels, every kernel sums the previous kernel's per-work-group
+ * partials in block order itself (no device-wide reduction); iteration it:
+ *   B1: beta, p = r + beta (p - omega v), v = A K^-1 p, partials (rh, v)
+ *   B2: alpha = rho / (rh, v), s = r - alpha v
+ *   B3: t = A K^-1 s, partials (t, s), (t, t)
+ *   B4: omega = (t, s) / (t, t), x += alpha K^-1 p + omega K^-1 s,
+ *       r = s - omega t, partials (rh, r), (r, r)
+ * p and v alternate between two buffers by iteration parity. */
+enum { BS_RHO0 = 0, BS_RHO1, BS_ALPHA, BS_OMEGA, BS_BB, BS_RR, BS_RELRES, BS_WORDS = 8 };
+
+static Sten make_sten(iskra_ctx *c);
+
+struct BiArgs {
+  Sten s;
+  const double *rho;
+  double scale;
+  const int *srow, *snb, *sdof;
+  const double *sd, *sc, *ssig, *sigma, *dg;
+  double *x, *r, *rh, *pp[2], *v[2], *sv, *tv;
+  double *p0, *pa, *pt; /* partials: 3 x nblk (init / B4), nblk (B1), 2 x nblk (B3) */
+  double *scal;
+  int *iflag;
+  int nblk, max_iter, warm;
+  const double *phi_in;
+  double rtol2;
+};
+
+/* row n of the unknowns' operator applied to get(m) (callers return 0 at
+ * Dirichlet nodes, whose values are on the right-hand side) */
+template <typename G>
+__device__ __forceinline__ double bi_row(const BiArgs &a, int n, G get) {
+  const int k = a.srow[n];
+  if (k >= 0) {
+    double acc = a.sd[k] * get(n);
+    for (int q = 0; q < 2; ++q) {
+      const int m = a.snb[2 * k + q];
+      if (m >= 0) acc -= a.sc[2 * k + q] * get(m);
+    }
+    return acc;
+  }
+  const Sten &s = a.s;
+  const int i = n % s.nx, j = n / s.nx;
+  double acc = 0.0, diag = 0.0;
+  if (i > 0) { acc += get(n - 1); diag += 1.0; }
+  if (i < s.nx - 1) { acc += get(n + 1); diag += 1.0; }
+  if (s.per_x) {
+    if (i == 0) { acc += s.cxp * get(n + s.nx - 1); diag += s.cxp; }
+    if (i == s.nx - 1) { acc += s.cxp * get(n - (s.nx - 1)); diag += s.cxp; }
+  }
+  if (s.dim == 2) {
+    if (j > 0) { acc += get(n - s.nx); diag += 1.0; }
+    if (j < s.ny - 1) { acc += get(n + s.nx); diag += 1.0; }
+    if (s.per_y) {
+      if (j == 0) { acc += s.cyp * get(n + (s.ny - 1) * s.nx); diag += s.cyp; }
+      if (j == s.ny - 1) { acc += s.cyp * get(n - (s.ny - 1) * s.nx); diag += s.cyp; }
+    }
+  }
+  return diag * get(n) - acc;
+}
+
+/* right-hand side of unknown n */
+__device__ __forceinline__ double bi_rhs(const BiArgs &a, int n) {
+  const double f = a.rho[n] * a.scale;
+  const int k = a.srow[n];
+  if (k >= 0) {
+    double b = f + a.ssig[k] * a.sigma[a.sdof[k]];
+    for (int q = 0; q < 2; ++q) {
+      const int m = a.snb[2 * k + q];
+      if (m >= 0 && a.s.tags[m] && a.srow[m] < 0) b += a.sc[2 * k + q] * a.s.tagval[a.s.tags[m]];
+    }
+    return b;
+  }
+  return f + dirichlet_rhs(a.s, n % a.s.nx, n / a.s.nx);
+}
+
+__global__ __launch_bounds__(RED_BLOCK) void k_bi_init(BiArgs a) {
+  const int n = blockIdx.x * RED_BLOCK + threadIdx.x, nn = a.s.nx * a.s.ny;
+  double rr = 0.0, bb = 0.0, rhor = 0.0;
+  if (blockIdx.x == 0 && threadIdx.x < BS_WORDS) a.scal[threadIdx.x] = 0.0;
+  if (n < nn) {
+    const int t = a.s.tags[n];
+    const bool unk = t == 0 || a.srow[n] >= 0;
+    if (!unk) {
+      a.x[n] = a.s.tagval[t];
+      a.r[n] = a.rh[n] = 0.0;
+    } else {
+      const double x0 = a.warm ? a.phi_in[n] : 0.0;
+      a.x[n] = x0;
+      auto gx = [&](int m) -> double {
+        return (a.s.tags[m] && a.srow[m] < 0) ? 0.0 : (a.warm ? a.phi_in[m] : 0.0);
+      };
+      const double b = bi_rhs(a, n);
+      const double r = b - bi_row(a, n, gx);
+      a.r[n] = r;
+      a.rh[n] = r;
+      rr = r * r;
+      bb = b * b;
+      rhor = r * r;
+    }
+    a.pp[1][n] = 0.0;
+    a.v[1][n] = 0.0;
+  }
+  wg_sum3(rhor, bb, rr);
+  if (threadIdx.x == 0) {
+    a.p0[3 * blockIdx.x] = rhor;
+    a.p0[3 * blockIdx.x + 1] = bb;
+    a.p0[3 * blockIdx.x + 2] = rr;
+  }
+}
+
+__global__ __launch_bounds__(RED_BLOCK) void k_bi_1(BiArgs a, int it) {
+  __shared__ int s_stop;
+  if (threadIdx.x == 0) s_stop = a.iflag[0];
+  double rho = 0.0, bb = 0.0, rr = 0.0;
+  for (int k = threadIdx.x; k < a.nblk; k += RED_BLOCK) {
+    rho += a.p0[3 * k];
+    bb += a.p0[3 * k + 1];
+    rr += a.p0[3 * k + 2];
+  }
+  wg_sum3(rho, bb, rr);
+  if (s_stop) return;
+  if (it > 0) bb = a.scal[BS_BB];
+  const bool done = rr <= a.rtol2 * bb || it >= a.max_iter || !(rr == rr);
+  double beta = 0.0;
+  const double omega = a.scal[BS_OMEGA];
+  if (it > 0) {
+    const double rho_old = a.scal[BS_RHO0 + ((it - 1) & 1)];
+    beta = (rho_old != 0.0 && omega != 0.0) ? (rho / rho_old) * (a.scal[BS_ALPHA] / omega) : 0.0;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (it == 0) a.scal[BS_BB] = bb;
+    a.scal[BS_RR] = rr;
+    a.scal[BS_RELRES] = bb > 0.0 ? sqrt(rr / bb) : 0.0;
+    a.scal[BS_RHO0 + (it & 1)] = rho;
+    a.iflag[1] = it;
+    if (done) a.iflag[0] = 1;
+  }
+  if (done) return;
+  const double *pold = a.pp[(it + 1) & 1], *vold = a.v[(it + 1) & 1];
+  double *pnew = a.pp[it & 1], *vnew = a.v[it & 1];
+  const int n = blockIdx.x * RED_BLOCK + threadIdx.x, nn = a.s.nx * a.s.ny;
+  double rv = 0.0, d1 = 0.0, d2 = 0.0;
+  if (n < nn) {
+    auto unk = [&](int m) { return a.s.tags[m] == 0 || a.srow[m] >= 0; };
+    auto gp = [&](int m) -> double {
+      if (!unk(m)) return 0.0;
+      const double p = a.r[m] + beta * (pold[m] - omega * vold[m]);
+      return p / a.dg[m];
+    };
+    if (unk(n)) {
+      const double p = a.r[n] + beta * (pold[n] - omega * vold[n]);
+      pnew[n] = p;
+      const double v = bi_row(a, n, gp);
+      vnew[n] = v;
+      rv = a.rh[n] * v;
+    } else {
+      pnew[n] = 0.0;
+      vnew[n] = 0.0;
+    }
+  }
+  wg_sum3(rv, d1, d2);
+  if (threadIdx.x == 0) a.pa[blockIdx.x] = rv;
+}
+
+__global__ __launch_bounds__(RED_BLOCK) void k_bi_2(BiArgs a, int it) {
+  if (a.iflag[0]) return;
+  double rv = 0.0, d1 = 0.0, d2 = 0.0;
+  for (int k = threadIdx.x; k < a.nblk; k += RED_BLOCK) rv += a.pa[k];
+  wg_sum3(rv, d1, d2);
+  const double rho = a.scal[BS_RHO0 + (it & 1)];
+  const double alpha = rv != 0.0 ? rho / rv : 0.0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    a.scal[BS_ALPHA] = alpha;
+    if (rv == 0.0) a.iflag[3] = 1; /* breakdown */
+  }
+  const int n = blockIdx.x * RED_BLOCK + threadIdx.x, nn = a.s.nx * a.s.ny;
+  if (n < nn) a.sv[n] = a.r[n] - alpha * a.v[it & 1][n];
+}
+
+__global__ __launch_bounds__(RED_BLOCK) void k_bi_3(BiArgs a, int it) {
+  if (a.iflag[0]) return;
+  (void)it;
+  const int n = blockIdx.x * RED_BLOCK + threadIdx.x, nn = a.s.nx * a.s.ny;
+  double ts = 0.0, tt = 0.0, d2 = 0.0;
+  if (n < nn) {
+    auto unk = [&](int m) { return a.s.tags[m] == 0 || a.srow[m] >= 0; };
+    if (unk(n)) {
+      auto gs = [&](int m) -> double { return unk(m) ? a.sv[m] / a.dg[m] : 0.0; };
+      const double t = bi_row(a, n, gs);
+      a.tv[n] = t;
+      ts = t * a.sv[n];
+      tt = t * t;
+    } else {
+      a.tv[n] = 0.0;
+    }
+  }
+  wg_sum3(ts, tt, d2);
+  if (threadIdx.x == 0) {
+    a.pt[2 * blockIdx.x] = ts;
+    a.pt[2 * blockIdx.x + 1] = tt;
+  }
+}
+
+__global__ __launch_bounds__(RED_BLOCK) void k_bi_4(BiArgs a, int it) {
+  if (a.iflag[0]) return;
+  double ts = 0.0, tt = 0.0, d2 = 0.0;
+  for (int k = threadIdx.x; k < a.nblk; k += RED_BLOCK) {
+    ts += a.pt[2 * k];
+    tt += a.pt[2 * k + 1];
+  }
+  wg_sum3(ts, tt, d2);
+  const double omega = tt != 0.0 ? ts / tt : 0.0;
+  const double alpha = a.scal[BS_ALPHA];
+  if (blockIdx.x == 0 && threadIdx.x == 0) a.scal[BS_OMEGA] = omega;
+  const int n = blockIdx.x * RED_BLOCK + threadIdx.x, nn = a.s.nx * a.s.ny;
+  double rhor = 0.0, rr = 0.0, z = 0.0;
+  if (n < nn && (a.s.tags[n] == 0 || a.srow[n] >= 0)) {
+    const double dg = a.dg[n];
+    a.x[n] += alpha * (a.pp[it & 1][n] / dg) + omega * (a.sv[n] / dg);
+    const double r = a.sv[n] - omega * a.tv[n];
+    a.r[n] = r;
+    rhor = a.rh[n] * r;
+    rr = r * r;
+  }
+  wg_sum3(rhor, z, rr);
+  if (threadIdx.x == 0) {
+    a.p0[3 * blockIdx.x] = rhor;
+    a.p0[3 * blockIdx.x + 1] = 0.0;
+    a.p0[3 * blockIdx.x + 2] = rr;
+  }
+}
+
+__global__ void k_bi_out(const double *x, double *phi, int nn) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n < nn) phi[n] = x[n];
+}
+
+int isk_bicgstab_solve(iskra_ctx *c, int warm, int *iters, double *relres) {
+  IskPoisson &p = c->ps;
+  const int nn = isk_nodes(c), nblk = (nn + RED_BLOCK - 1) / RED_BLOCK;
+  BiArgs a;
+  a.s = make_sten(c);
+  a.rho = c->rho;
+  a.scale = (c->g.dx * c->g.dx) / p.eps0;
+  a.srow = p.d_srow;
+  a.snb = p.d_snb;
+  a.sdof = p.d_sdof;
+  a.sd = p.d_sd;
+  a.sc = p.d_sc;
+  a.ssig = p.d_ssig;
+  a.sigma = c->d_elec + EL_SIGMA;
+  a.dg = p.d_dg;
+  double *b = p.bi;
+  a.x = p.x;
+  a.r = b;
+  a.rh = b + nn;
+  a.pp[0] = b + 2 * (size_t)nn;
+  a.pp[1] = b + 3 * (size_t)nn;
+  a.v[0] = b + 4 * (size_t)nn;
+  a.v[1] = b + 5 * (size_t)nn;
+  a.sv = b + 6 * (size_t)nn;
+  a.tv = b + 7 * (size_t)nn;
+  a.p0 = b + 8 * (size_t)nn;
+  a.pa = a.p0 + 3 * nblk;
+  a.pt = a.pa + nblk;
+  a.scal = a.pt + 2 * nblk;
+  a.iflag = p.iflag;
+  a.nblk = nblk;
+  a.max_iter = p.max_iter;
+  a.warm = warm == ISKRA_WARM_PREV ? 1 : 0;
+  a.phi_in = c->phi;
+  a.rtol2 = p.rtol * p.rtol;
+  hipLaunchKernelGGL(k_pcg_init, dim3(1), dim3(64), 0, c->stream, p.scal, p.iflag, 0.0, 0);
+  hipLaunchKernelGGL(k_bi_init, dim3(nblk), dim3(RED_BLOCK), 0, c->stream, a);
+  ISK_CHECK_LAUNCH();
+  int it = 0, hflag[4] = {0, 0, 0, 0};
+  while (true) {
+    for (int k = 0; k < 8; ++k, ++it) {
+      hipLaunchKernelGGL(k_bi_1, dim3(nblk), dim3(RED_BLOCK), 0, c->stream, a, it);
+      hipLaunchKernelGGL(k_bi_2, dim3(nblk), dim3(RED_BLOCK), 0, c->stream, a, it);
+      hipLaunchKernelGGL(k_bi_3, dim3(nblk), dim3(RED_BLOCK), 0, c->stream, a, it);
+      hipLaunchKernelGGL(k_bi_4, dim3(nblk), dim3(RED_BLOCK), 0, c->stream, a, it);
+      ISK_CHECK_LAUNCH();
+    }
+    ISK_HIP(hipMemcpyAsync(hflag, p.iflag, sizeof(hflag), hipMemcpyDeviceToHost, c->stream));
+    ISK_HIP(hipStreamSynchronize(c->stream));
+    if (hflag[0]) break;
+    if (it > p.max_iter + 16) {
+      isk_set_error("poisson: BiCGStab stopped after %d host iterations without a verdict", it);
+      return ISKRA_E_CONVERGE;
+    }
+  }
+  double sc[BS_WORDS];
+  ISK_HIP(hipMemcpy(sc, a.scal, sizeof(sc), hipMemcpyDeviceToHost));
+  hipLaunchKernelGGL(k_bi_out, dim3((nn + 255) / 256), dim3(256), 0, c->stream, p.x, c->phi, nn);
+  ISK_CHECK_LAUNCH();
+  const int n_it = hflag[1];
+  const double rel = sc[BS_RELRES];
+  p.last_iters = n_it;
+  p.last_relres = rel;
+  c->stats.poisson_solves++;
+  c->stats.poisson_iters += n_it;
+  c->stats.last_iters = n_it;
+  c->stats.last_relres = rel;
+  if (iters) *iters = n_it;
+  if (relres) *relres = rel;
+  if (!(rel <= p.rtol)) {
+    isk_set_error("poisson: BiCGStab reached relative residual %.3g > %.3g after %d iterations%s",
+                  rel, p.rtol, n_it, hflag[3] ? " (breakdown)" : "");
+    return ISKRA_E_CONVERGE;
+  }
+  return ISKRA_OK;
+}
+
 int isk_poisson_free(iskra_ctx *c) {
   IskPoisson &p = c->ps;
   void *ptrs[] = {p.d_tags, p.d_tag_val, p.d_tri_a, p.d_tri_cp, p.d_tri_inv,
                   p.d_work, p.x, p.r, p.Ap, p.z, p.p[0], p.p[1], p.scal,
                   p.partials, p.cgp, p.iflag, p.sp_Sx, p.sp_Sy, p.sp_Li,
-                  p.sp_F, p.sp_T1, p.sp_T2};
+                  p.sp_F, p.sp_T1, p.sp_T2, p.d_srow, p.d_snb, p.d_sdof, p.d_sd,
+                  p.d_sc, p.d_ssig, p.bi, p.d_dg};
   for (void *q : ptrs) isk_free(q);
+  p.d_srow = p.d_snb = p.d_sdof = nullptr;
+  p.d_sd = p.d_sc = p.d_ssig = p.bi = p.d_dg = nullptr;
   p.sp_Sx = p.sp_Sy = p.sp_Li = p.sp_F = p.sp_T1 = p.sp_T2 = nullptr;
   p.sp_nx = p.sp_ny = 0;
   for (int l = 0; l < p.nlev; ++l) {
@@ -1478,6 +1804,49 @@ int isk_poisson_setup(iskra_ctx *c) {
     return ISKRA_E_HIP;
   ISK_HIP(hipMemset(p.iflag, 0, 64));
   p.nlev = 0;
+  if (!p.h_sd.empty()) {
+    /* floating electrodes: apply_neumann rows, solved by BiCGStab */
+    if (c->g.dim != 2 || c->g.axial ||
+        (p.kind != ISKRA_POISSON_PCG && p.kind != ISKRA_POISSON_MGPCG)) {
+      isk_set_error("poisson: floating electrodes need a 2-D Cartesian grid and the PCG kind "
+                    "(their rows make the system non-symmetric: BiCGStab solves it)");
+      return ISKRA_E_UNSUPPORTED;
+    }
+    const int ns = (int)p.h_sd.size();
+    std::vector<double> dg(nn);
+    for (int k = 0; k < nn; ++k) {
+      if (p.h_srow[k] >= 0) {
+        dg[k] = p.h_sd[p.h_srow[k]];
+        continue;
+      }
+      const int i = k % c->g.nx, j = k / c->g.nx;
+      double d = 0.0;
+      if (i > 0) d += 1.0;
+      if (i < c->g.nx - 1) d += 1.0;
+      if (p.per_x && (i == 0 || i == c->g.nx - 1)) d += (c->g.dx * c->g.dx) / (c->g.dy * c->g.dy);
+      if (j > 0) d += 1.0;
+      if (j < c->g.ny - 1) d += 1.0;
+      if (p.per_y && (j == 0 || j == c->g.ny - 1)) d += 1.0;
+      dg[k] = d > 0.0 ? d : 1.0;
+    }
+    const size_t nbuf = (size_t)8 * nn + 6 * (size_t)nblk + 16;
+    if (isk_alloc((void **)&p.d_srow, (size_t)nn * 4) ||
+        isk_alloc((void **)&p.d_snb, (size_t)2 * ns * 4) ||
+        isk_alloc((void **)&p.d_sdof, (size_t)ns * 4) || isk_alloc((void **)&p.d_sd, (size_t)ns * 8) ||
+        isk_alloc((void **)&p.d_sc, (size_t)2 * ns * 8) ||
+        isk_alloc((void **)&p.d_ssig, (size_t)ns * 8) ||
+        isk_alloc((void **)&p.d_dg, (size_t)nn * 8) || isk_alloc((void **)&p.bi, nbuf * 8))
+      return ISKRA_E_HIP;
+    ISK_HIP(hipMemcpy(p.d_srow, p.h_srow.data(), (size_t)nn * 4, hipMemcpyHostToDevice));
+    ISK_HIP(hipMemcpy(p.d_snb, p.h_snb.data(), (size_t)2 * ns * 4, hipMemcpyHostToDevice));
+    ISK_HIP(hipMemcpy(p.d_sdof, p.h_sdof.data(), (size_t)ns * 4, hipMemcpyHostToDevice));
+    ISK_HIP(hipMemcpy(p.d_sd, p.h_sd.data(), (size_t)ns * 8, hipMemcpyHostToDevice));
+    ISK_HIP(hipMemcpy(p.d_sc, p.h_sc.data(), (size_t)2 * ns * 8, hipMemcpyHostToDevice));
+    ISK_HIP(hipMemcpy(p.d_ssig, p.h_ssig.data(), (size_t)ns * 8, hipMemcpyHostToDevice));
+    ISK_HIP(hipMemcpy(p.d_dg, dg.data(), (size_t)nn * 8, hipMemcpyHostToDevice));
+    ISK_HIP(hipMemset(p.bi, 0, nbuf * 8));
+    return ISKRA_OK;
+  }
   if (p.kind == ISKRA_POISSON_SPECTRAL) {
     if (c->g.dim != 2 || p.per_x || p.per_y || interior_tag) {
       isk_set_error("poisson: the spectral solve needs a 2-D grid with Dirichlet walls only");
@@ -1822,6 +2191,11 @@ int isk_poisson_solve_ex(iskra_ctx *c, int warm, int *iters, double *relres, int
     c->stats.last_iters = rounds;
     c->stats.last_relres = rel;
     return ISKRA_OK;
+  }
+  if (!p.h_sd.empty()) {
+    int rc = isk_electrodes_after_push(c, 0.0); /* fold in charge collected since */
+    if (rc) return rc;
+    return isk_bicgstab_solve(c, warm, iters, relres);
   }
   const int mgpre = p.kind == ISKRA_POISSON_MGPCG;
   PcgArgs a = make_pcg(c, mgpre);

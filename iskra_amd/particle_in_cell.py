@@ -203,6 +203,111 @@ def create_axial_boris_pusher():
 
 
 # ---------------------------------------------------------------------------
+# surfaces and electrodes (pic/surfaces/track.jl, pic/circuit_coupling.jl,
+# problem/configuration.jl:17-76)
+
+class Surface:
+    code = 0
+
+
+class PeriodicSurface(Surface):
+    """A surface whose hit does nothing (hit!(::Surface), hit.jl:23-28)."""
+    code = L.SURF_PERIODIC
+
+
+class AbsorbingSurface(Surface):
+    code = L.SURF_ABSORBING
+
+
+class ReflectiveSurface(Surface):
+    code = L.SURF_REFLECTIVE
+
+
+def create_periodic_surface():
+    return PeriodicSurface()
+
+
+def create_absorbing_surface():
+    return AbsorbingSurface()
+
+
+def create_reflective_surface():
+    return ReflectiveSurface()
+
+
+class SurfaceTracker:
+    """SurfaceTracker{2} (track.jl:14-19): the default surface of the grid's
+    boundary edges and, in call order, the surfaces and electrodes added to
+    it; the device builds the edge table from them when the configuration is
+    bound."""
+
+    def __init__(self, grid, ds):
+        self.grid = grid
+        self.default = ds
+        self.ops = []
+
+
+def create_surface_tracker(grid, ds=None):
+    """create_surface_tracker(grid, ds) (track.jl:101-114)."""
+    return SurfaceTracker(grid, AbsorbingSurface() if ds is None else ds)
+
+
+def track_surface(st, nodes, ss):
+    """track_surface!(tracker, nodes, surface) (track.jl:116-118)."""
+    st.ops.append(("surface", np.asarray(nodes, dtype=bool).copy(), ss))
+
+
+class FixedPotentialElectrode(Surface):
+    """FixedPotentialElectrode (circuit_coupling.jl:4-8)."""
+
+    def __init__(self, nodes, phi):
+        self.nodes, self.phi, self.fixed = nodes, float(phi), True
+        self._ctx, self._id = None, -1
+
+
+class FloatingPotentialElectrode(Surface):
+    """FloatingPotentialElectrode (circuit_coupling.jl:10-15): its surface
+    charge sigma (C/m^2, initial value) collects the charge of the particles
+    it absorbs."""
+
+    def __init__(self, nodes, sigma):
+        self.nodes, self.sigma0, self.fixed = nodes, float(sigma), False
+        self._ctx, self._id = None, -1
+
+    def state(self):
+        """sigma, dq, phi (at the reference node) and area on the device."""
+        return self._ctx.electrode_state(self._id)
+
+
+def create_electrode(nodes, config, fixed=False, sigma=0.0, phi=0.0):
+    """create_electrode(nodes, config; fixed, sigma, phi)
+    (problem/configuration.jl:17-35): a default tracker when the
+    configuration has none, then the electrode (Dirichlet nodes and an
+    absorbing surface when fixed; a surface charge with the reference's
+    apply_neumann rows when floating) added to it."""
+    if config.grid is None or config.solver is None:
+        raise ValueError("create_electrode: define the grid and the field solver first")
+    if config.tracker is None:
+        config.tracker = create_surface_tracker(config.grid)
+    mask = np.asarray(nodes, dtype=bool).copy()
+    e = FixedPotentialElectrode(mask, phi) if fixed else FloatingPotentialElectrode(mask, sigma)
+    config.tracker.ops.append(("electrode", mask, e))
+    return e
+
+
+def _bind_tracker(ctx, st):
+    ctx.tracker(st.default.code)
+    for op, mask, obj in st.ops:
+        m = np.ascontiguousarray(mask.T).ravel()
+        if op == "surface":
+            ctx.tracker_surface(m, obj.code)
+        else:
+            obj._id = ctx.electrode(m, fixed=obj.fixed, sigma=getattr(obj, "sigma0", 0.0),
+                                    phi=getattr(obj, "phi", 0.0))
+            obj._ctx = ctx
+
+
+# ---------------------------------------------------------------------------
 # configuration (problem/configuration.jl:5-15)
 
 class Config:
@@ -353,6 +458,10 @@ def bind(config):
     if isinstance(config.solver, PoissonSolver):
         config.solver._ctx = ctx
         config.solver._configure(ctx)
+    if config.tracker is not None:
+        _bind_tracker(ctx, config.tracker)
+    if config.circuit is not None:
+        config.circuit._bind(ctx)
     for inter in config.interactions:
         if isinstance(inter, (chemistry.MonteCarloCollisions,
                               chemistry.DirectSimulationMonteCarlo)):

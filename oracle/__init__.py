@@ -84,6 +84,18 @@ How each oracle function is pinned (checks live in tests/):
                          A sin C) with B and C uniform (KS tests), a source
                          index equal to the target index redrawn only when
                          the species differ (tests/test_oracle_dsmc.py).
+  surfaces.SurfaceTracker the reference's track!/check!/hit! restated on
+                         absolute FP64 positions: closed forms of absorption,
+                         mirror reflection and corner double reflection,
+                         untracked cells left alone (tests/test_oracle_surfaces.py).
+  mir_push_deposit_track against SurfaceTracker within FP32 tolerance, bit
+                         for bit against the device.
+  surfaces.electrode_system the reference's add_new_dof + apply_neumann rows:
+                         a floating wall electrode over a grounded wall with
+                         no charge gives the linear potential of slope -sigma
+                         its rows state; the segment-end rows agree with it.
+  surfaces.CircuitRLC    advance_circuit! converges to the closed-form RL
+                         step response and RC discharge as dt -> 0.
   orc_ws_run             (learned warm start, oracle/ws_oracle.c; no reference
                          counterpart) FP64 gradients against central finite
                          differences; convolution, pooling and upsampling
@@ -150,6 +162,8 @@ def _declare(L):
         "ref_rz_transform": (None, [I64, P, P, D]),
         "mir_push_deposit_rz": (I64, [I64, P, P, P, P, P, P, P, C.c_int, C.c_int, C.c_int, F, D,
                                       F, C.c_int, C.c_int, P]),
+        "mir_push_deposit_track": (I64, [I64, P, P, P, P, P, P, C.c_int, C.c_int, C.c_int, F, F, F,
+                                         C.c_int, C.c_int, P, P, D, D, P, P]),
         "ref_apply_periodic_1d": (None, [C.c_int, D, P]),
         "ref_apply_periodic_2d": (None, [C.c_int, C.c_int, D, D, C.c_int, P]),
         "ref_apply_dirichlet": (None, [C.c_int, P, D, P, P, P]),
@@ -579,6 +593,61 @@ class MirSim:
         self.tagval = np.zeros(17)
         self.tagval[:len(tagval)] = tagval
 
+    # -- surface tracker, electrodes, circuit (the device's arithmetic) --
+    def set_tracker(self, surf, tracked):
+        """Edge codes (4 per cell: -x, +x, -y, +y) and tracked cells as the
+        device holds them (0 none, 1 absorb, 2 reflect, 3 pass, 16 + k
+        floating electrode k)."""
+        self.surf = np.ascontiguousarray(surf, dtype=np.uint8)
+        self.tracked = np.ascontiguousarray(tracked, dtype=np.uint8)
+
+    def set_electrodes(self, sigma, area, circuit=None):
+        """Floating electrodes' initial sigma and area (per sigma dof); circuit:
+        dict(R, L, C, v0, v1, f, ph, pos_ref, pos_dof, neg_phi) or None."""
+        self.el_sigma = np.zeros(8)
+        self.el_dq = np.zeros(8)
+        self.el_area = np.zeros(8)
+        self.el_sigma[:len(sigma)] = sigma
+        self.el_area[:len(area)] = area
+        self.nfloat = len(sigma)
+        self.circuit = circuit
+        self.circ = np.zeros(4)  # i, q, t, vext
+
+    def electrode_update(self, dt, with_circuit):
+        """The device's k_el_update: hits -> charge, then the circuit."""
+        for k in range(self.nfloat):
+            dq = 0.0
+            for sp in self.species:
+                eh = getattr(sp, "ehit", None)
+                if eh is not None and eh[k]:
+                    dq += float(eh[k]) * (sp.q * sp.w0)
+                    eh[k] = 0
+            if dq != 0.0:
+                self.el_dq[k] += dq
+                self.el_sigma[k] += dq / self.el_area[k]
+        c = self.circuit
+        if not with_circuit or c is None:
+            return
+        i, q, t = self.circ[0], self.circ[1], self.circ[2]
+        vext = (self.phi[c["pos_ref"]] - c["neg_phi"]) if c["pos_ref"] >= 0 else 0.0
+        u = c["f"] * t + c["ph"] / 6.283185307179586
+        u = u - np.floor(u)
+        sn, cn = C.c_double(), C.c_double()
+        lib().orc_sincos2pi(u, C.byref(sn), C.byref(cn))
+        vt = c["v0"] + c["v1"] * sn.value
+        ni = (c["L"] / dt - c["R"] / 2.0) * i + vext - vt
+        nq = q
+        if c["C"] > 0.0:
+            ni -= q / c["C"]
+            ni /= (c["L"] / dt + c["R"] / 2.0)
+            nq = q + dt * i
+        else:
+            ni /= (c["L"] / dt + c["R"] / 2.0)
+        self.circ[:] = (ni, nq, t + dt, vext)
+        if c["pos_dof"] >= 0:
+            self.el_dq[c["pos_dof"]] = 0.0
+            self.el_sigma[0] += -dt * ni / self.el_area[c["pos_dof"]]
+
     def add_dsmc(self, src, tgt, tab_g, tab_sigma):
         """One DSMC.ElasticCollision interaction (equal weights), run after
         the MCC interactions of each step, as iskra_step orders them."""
@@ -715,6 +784,17 @@ class MirSim:
         sp = self.species[s]
         cnt = np.zeros(self.nx * self.ny, dtype=np.int64)
         qmdt = np.float32(sp.q / sp.m * dt)
+        if getattr(self, "surf", None) is not None and do_push:
+            if getattr(sp, "ehit", None) is None:
+                sp.ehit = np.zeros(8, dtype=np.uint64)
+            live = lib().mir_push_deposit_track(
+                sp.np, ptr(sp.cell), ptr(sp.ox), ptr(sp.oy), ptr(sp.vx), ptr(sp.vy), ptr(self.E),
+                self.nx, self.ncx, self.ncy, qmdt, np.float32(dt / self.dx),
+                np.float32(dt / self.dy), sp.bnd[0], sp.bnd[1], ptr(self.surf), ptr(self.tracked),
+                dt, self.dx, ptr(sp.ehit), ptr(cnt))
+            self.pushed += live
+            sp.cnt = cnt
+            return
         if self.axial and do_push:
             live = lib().mir_push_deposit_rz(sp.np, ptr(sp.cell), ptr(sp.ox), ptr(sp.oy),
                                              ptr(sp.vx), ptr(sp.vy), ptr(sp.vz), ptr(self.E),
@@ -784,6 +864,8 @@ class MirSim:
             self.dsmc_run(k, dt)
         for s in range(len(self.species)):
             self.push(s, dt)
+        if getattr(self, "el_area", None) is not None:
+            self.electrode_update(dt, True)
         self.compute_rho()
 
     def end_step(self, dt):

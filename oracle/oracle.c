@@ -808,6 +808,112 @@ EXPORT int64_t mir_push_deposit_rz(int64_t np, int32_t *cell, float *ox, float *
   return live;
 }
 
+/* the device's surface tracer (track_path in particles.hip): the reference's
+ * track!/check!/hit! (pic/surfaces/track.jl:152-161, check.jl:17-68,
+ * hit.jl:1-56, circuit_coupling.jl:47-62) in FP64 from the FP32 state.
+ * surf: 4 codes per cell (-x, +x, -y, +y): 0 none, 1 absorb, 2 reflect,
+ * 3 pass, 16 + k floating electrode k.  Returns 1 if absorbed (electrode k
+ * in *elec, else -1). */
+#define MIR_TRACK_MAX 64
+static int mir_track_path(const uint8_t *surf, int ncx, int ncy, double dt, double h, int i, int j,
+                          float hx0, float hy0, float *vx, float *vy, double *cx, double *cy,
+                          int *elec) {
+  double T = dt, hx = hx0, hy = hy0;
+  *cx = 0.0;
+  *cy = 0.0;
+  *elec = -1;
+  for (int it = 0; it < MIR_TRACK_MAX; ++it) {
+    const double vxd = *vx, vyd = *vy;
+    const double dxs = vxd > 0 ? h * (1.0 - hx) : h * hx;
+    const double dys = vyd > 0 ? h * (1.0 - hy) : h * hy;
+    const double tx = dxs / fabs(vxd), ty = dys / fabs(vyd);
+    if (T < tx && T < ty) return 0;
+    int ni = i, nj = j, dir;
+    double nhx, nhy, Tn;
+    if (tx < ty) {
+      Tn = T - tx;
+      nhy = hy + vyd * tx / h;
+      if (vxd > 0) { ni = i + 1; nhx = 0.0; dir = 1; }
+      else { ni = i - 1; nhx = 1.0; dir = 0; }
+    } else {
+      Tn = T - ty;
+      nhx = hx + vxd * ty / h;
+      if (vyd > 0) { nj = j + 1; nhy = 0.0; dir = 3; }
+      else { nj = j - 1; nhy = 1.0; dir = 2; }
+    }
+    if (i < 0 || i >= ncx || j < 0 || j >= ncy) return 0;
+    const int code = surf[(i + ncx * j) * 4 + dir];
+    if (code == 0) {
+      i = ni; j = nj; hx = nhx; hy = nhy; T = Tn;
+      continue;
+    }
+    if (code == 1) return 1;
+    if (code >= 16) { *elec = code - 16; return 1; }
+    if (code != 2) return 0;
+    if (ni != i) { *vx = -*vx; *cx += 2.0 * (double)*vx * Tn / h; }
+    if (nj != j) { *vy = -*vy; *cy += 2.0 * (double)*vy * Tn / h; }
+    int si = ni, sj = nj;
+    double shx = nhx, shy = nhy;
+    if (nhx == 0.0) { si = ni - 1; shx = 1.0; }
+    if (nhx == 1.0) { si = ni + 1; shx = 0.0; }
+    if (nhy == 0.0) { sj = nj - 1; shy = 1.0; }
+    if (nhy == 1.0) { sj = nj + 1; shy = 0.0; }
+    i = si; j = sj; hx = shx; hy = shy; T = Tn;
+  }
+  return 0;
+}
+
+static inline int mir_shift(double c, float *o) {
+  const double t = (double)*o + c;
+  const double f = floor(t);
+  float h = (float)(t - f);
+  int k = (int)f;
+  if (h >= 1.0f) { h = 0.0f; ++k; }
+  *o = h;
+  return k;
+}
+
+/* mir_push_deposit (2-D) with the surface tracker of the push kernels:
+ * particles starting in a tracked cell are traced; ehit[k] counts the
+ * particles floating electrode k absorbed. */
+EXPORT int64_t mir_push_deposit_track(int64_t np, int32_t *cell, float *ox, float *oy, float *vx,
+                                      float *vy, const float *E, int nx, int ncx, int ncy,
+                                      float qmdt, float dtdx, float dtdy, int bndx, int bndy,
+                                      const uint8_t *surf, const uint8_t *tracked, double dt,
+                                      double h, uint64_t *ehit, int64_t *cnt) {
+  int64_t live = 0;
+  for (int64_t p = 0; p < np; ++p) {
+    if (cell[p] < 0) continue;
+    int i = cell[p] & 0xFFFF, j = cell[p] >> 16;
+    float hx = ox[p], hy = oy[p];
+    float Ex, Ey;
+    mir_gather1(2, E, nx, i, j, hx, hy, &Ex, &Ey);
+    vx[p] = fmaf(qmdt, Ex, vx[p]);
+    vy[p] = fmaf(qmdt, Ey, vy[p]);
+    const float vx0 = vx[p], vy0 = vy[p];
+    double cx = 0.0, cy = 0.0;
+    int absorbed = 0, el = -1;
+    if (tracked[i + ncx * j])
+      absorbed = mir_track_path(surf, ncx, ncy, dt, h, i, j, hx, hy, &vx[p], &vy[p], &cx, &cy, &el);
+    if (absorbed && el >= 0) ehit[el]++;
+    i += mir_advance(vx0, dtdx, &hx);
+    j += mir_advance(vy0, dtdy, &hy);
+    if (cx != 0.0) i += mir_shift(cx, &hx);
+    if (cy != 0.0) j += mir_shift(cy, &hy);
+    int alive = mir_bnd(&i, ncx, bndx);
+    alive = mir_bnd(&j, ncy, bndy) && alive;
+    if (absorbed) alive = 0;
+    cell[p] = alive ? (i | (j << 16)) : -1;
+    ox[p] = hx;
+    oy[p] = hy;
+    if (alive) {
+      ++live;
+      mir_deposit1(2, nx, i, j, hx, hy, cnt);
+    }
+  }
+  return live;
+}
+
 /* rho = sum_s q_s n_s + bg, n_s = (cnt_s 2^-32) w0_s / V (GPU order) */
 EXPORT void mir_rho(int nn, int ns, const int64_t *cnt /* ns x nn */, const double *w0,
                     const double *q, const double *vol, const double *bg, double *rho) {
