@@ -482,22 +482,6 @@ int iskra_ws_train(iskra_ws *net, int64_t first, int64_t count, int batch,
 int iskra_ws_eval(iskra_ws *net, int64_t first, int64_t count, double *mse);
 /* phi at the interior nodes of the context <- prediction from its rho */
 int iskra_ws_predict(iskra_ws *net);
-/* calculate_electric_potential (generalized_poisson.jl:372-378) with the
- * network inside the iteration: flexible conjugate gradients on the
- * context's system (each new direction A-orthogonalised against the previous
- * one), every iteration preconditioned by z = the network applied to the
- * residual (scaled by its RMS and the gain), followed when omega > 0 by one
- * damped Jacobi sweep z += omega/4 (r - A z) for the high frequencies the
- * network leaves.  The system must be the network's: phi = 0 on all four
- * walls, no other Dirichlet node and no periodic link (ISKRA_E_UNSUPPORTED
- * otherwise).  Iterates until the FP64 relative residual meets the solver's
- * rtol (ISKRA_E_CONVERGE after max_iter iterations; 0: the solver's
- * max_iter); warm: ISKRA_WARM_ZERO or ISKRA_WARM_PREV (the current phi, for
- * example the network's own prediction).  Leaves phi like
- * iskra_poisson_solve. */
-int iskra_ws_solve(iskra_ws *net, int warm, double omega, int max_iter,
-                   int *iters, double *relres);
-
 /* ---- diagnostics ---------------------------------------------------------- */
 /* Device-to-device copy of `bytes` bytes, `iters` times, with the library's
  * own streaming copy kernel (nontemporal != 0: non-temporal loads/stores,

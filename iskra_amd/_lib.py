@@ -46,7 +46,6 @@ EXPORTS = [
     "iskra_ws_get_params", "iskra_ws_set_gain", "iskra_ws_forward", "iskra_ws_backward",
     "iskra_ws_dataset_reserve", "iskra_ws_capture", "iskra_ws_dataset_size",
     "iskra_ws_dataset_rms", "iskra_ws_train", "iskra_ws_eval", "iskra_ws_predict",
-    "iskra_ws_solve",
     "iskra_batch_create", "iskra_batch_destroy", "iskra_batch_set_graph", "iskra_batch_step",
     "iskra_batch_graphs_launched",
 ]
@@ -155,7 +154,6 @@ def lib():
             "iskra_ws_train": (I, [P, I64, I64, I, I, C.c_float, U64, P]),
             "iskra_ws_eval": (I, [P, I64, I64, C.POINTER(D)]),
             "iskra_ws_predict": (I, [P]),
-            "iskra_ws_solve": (I, [P, I, D, I, C.POINTER(I), C.POINTER(D)]),
             "iskra_batch_create": (I, [P, I, C.POINTER(P)]),
             "iskra_batch_destroy": (I, [P]),
             "iskra_batch_set_graph": (I, [P, I]),
@@ -628,14 +626,6 @@ class WarmStart:
 
     def predict(self):
         check(lib().iskra_ws_predict(self.h))
-
-    def solve(self, warm=WARM_ZERO, omega=0.8, max_iter=0):
-        """Flexible CG on the context's system preconditioned by the network
-        (plus one damped Jacobi sweep when omega > 0); returns (iters, relres)."""
-        it, rr = C.c_int(), C.c_double()
-        check(lib().iskra_ws_solve(self.h, int(warm), float(omega), int(max_iter), C.byref(it),
-                                   C.byref(rr)))
-        return it.value, rr.value
 
 
 class Batch:

@@ -1377,6 +1377,10 @@ __device__ __forceinline__ long long merge_node(const MergeArgs &a, const int no
   const int tlo = merge_tlo(i, a), thi = min((i + a.M) / a.T, a.ntx - 1);
   const int ulo = DIM == 2 ? merge_tlo(j, a) : 0;
   const int uhi = DIM == 2 ? min((j + a.M) / a.T, a.nty - 1) : 0;
+  if (DIM == 1 && a.tilewin) {
+    for (int ti = tlo; ti <= thi; ++ti) s += a.tilewin[(size_t)ti * WW + (i - (ti * a.T - a.M))];
+    return s;
+  }
   if (DIM == 2 && a.tilewin) {
     long long v[16];
 #pragma unroll
@@ -1425,6 +1429,30 @@ __device__ __forceinline__ void merge_tile_body(const long long *__restrict__ pa
                                                 const long long *__restrict__ hdr,
                                                 long long *__restrict__ tilewin, int WW, int t) {
   const int k0 = hdr[HDR_NCHUNKS] > 0 ? tcb[t] : 0, k1 = hdr[HDR_NCHUNKS] > 0 ? tcb[t + 1] : 0;
+  if (2 * WW <= (int)blockDim.x) {
+    /* narrow (1-D) windows: G = blockDim / WW thread groups take every G-th
+     * chunk of the tile, then the G partial sums are added in LDS (integer
+     * sums: any grouping gives the same result) */
+    __shared__ long long red[256];
+    const int G = (int)blockDim.x / WW, q = (int)threadIdx.x % WW, g = (int)threadIdx.x / WW;
+    long long s0 = 0, s1 = 0;
+    if (g < G) {
+      int k = k0 + g;
+      for (; k + G < k1; k += 2 * G) {
+        s0 += partial[(size_t)k * WW + q];
+        s1 += partial[(size_t)(k + G) * WW + q];
+      }
+      if (k < k1) s0 += partial[(size_t)k * WW + q];
+    }
+    red[threadIdx.x] = s0 + s1;
+    __syncthreads();
+    if (g == 0) {
+      long long v = red[threadIdx.x];
+      for (int h = 1; h < G; ++h) v += red[threadIdx.x + h * WW];
+      tilewin[(size_t)t * WW + q] = v;
+    }
+    return;
+  }
   for (int q = threadIdx.x; q < WW; q += blockDim.x) {
     /* four independent partial sums keep four loads in flight (integer
      * sums: the grouping does not change the result) */
@@ -1474,15 +1502,15 @@ int isk_merge_species(iskra_ctx *c, int sid, int count_live) {
   MergeArgs a;
   int grid;
   merge_prepare(c, sid, count_live, &a, &grid);
-  if (c->g.dim == 2) {
-    IskSpecies *s = &c->sp[sid];
-    hipLaunchKernelGGL(k_merge_tiles, dim3(c->ntiles), dim3(256), 0, c->stream,
-                       (const long long *)s->partial, (const int *)s->tile_chunk_begin,
-                       (const long long *)s->hdr, s->tilewin, isk_sp_window(c, sid) * isk_sp_window(c, sid));
-    ISK_CHECK_LAUNCH();
-    a.tilewin = s->tilewin;
+  IskSpecies *s = &c->sp[sid];
+  const int W = isk_sp_window(c, sid);
+  hipLaunchKernelGGL(k_merge_tiles, dim3(c->ntiles), dim3(256), 0, c->stream,
+                     (const long long *)s->partial, (const int *)s->tile_chunk_begin,
+                     (const long long *)s->hdr, s->tilewin, c->g.dim == 2 ? W * W : W);
+  ISK_CHECK_LAUNCH();
+  a.tilewin = s->tilewin;
+  if (c->g.dim == 2)
     hipLaunchKernelGGL(k_merge<2>, dim3(grid), dim3(256), 0, c->stream, a);
-  }
   else
     hipLaunchKernelGGL(k_merge<1>, dim3(grid), dim3(256), 0, c->stream, a);
   ISK_CHECK_LAUNCH();
@@ -1597,7 +1625,7 @@ int isk_push_deposit_all(iskra_ctx *c, double dt) {
   int grid = 0;
   for (int s = 0; s < c->nspecies; ++s) {
     merge_prepare(c, s, 1, &a.m[s], &grid);
-    if (c->g.dim == 2) a.m[s].tilewin = c->sp[s].tilewin;
+    a.m[s].tilewin = c->sp[s].tilewin;
     a.r.cnt[s] = c->sp[s].cnt;
     a.r.qw[s] = c->sp[s].w0;
     a.r.q[s] = c->sp[s].q;
@@ -1608,14 +1636,14 @@ int isk_push_deposit_all(iskra_ctx *c, double dt) {
   a.r.rho = c->rho;
   a.r.nn = isk_nodes(c);
   isk_poisson_pack_desc(c, &a.pk);
-  if (c->g.dim == 2 && c->nspecies > 0) {
+  if (c->nspecies > 0) {
     MergeTilesN t;
     for (int s = 0; s < c->nspecies; ++s) {
       t.partial[s] = c->sp[s].partial;
       t.tcb[s] = c->sp[s].tile_chunk_begin;
       t.hdr[s] = c->sp[s].hdr;
       t.tilewin[s] = c->sp[s].tilewin;
-      t.WW[s] = isk_sp_window(c, s) * isk_sp_window(c, s);
+      t.WW[s] = c->g.dim == 2 ? isk_sp_window(c, s) * isk_sp_window(c, s) : isk_sp_window(c, s);
     }
     hipLaunchKernelGGL(k_merge_tiles_n, dim3(c->ntiles, c->nspecies), dim3(256), 0, c->stream,
                        t);

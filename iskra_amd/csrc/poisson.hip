@@ -223,14 +223,65 @@ __device__ __forceinline__ void tri_scan(const double *ma, const double *mb, int
   __syncthreads(); /* out, ga and gb are read or rewritten next */
 }
 
+/* sum of v[0..n) in a fixed order: each thread adds its TRI_K consecutive
+ * elements, then a pairwise tree over the TRI_BLOCK partials (every thread
+ * returns the same bits) */
+__device__ __forceinline__ double tri_sum(const double *v, int n, double *g) {
+  const int tid = threadIdx.x, j0 = tid * TRI_K;
+  double s = 0.0;
+#pragma unroll
+  for (int k = 0; k < TRI_K; ++k)
+    if (j0 + k < n) s += v[j0 + k];
+  g[tid] = s;
+  __syncthreads();
+  for (int d = TRI_BLOCK / 2; d > 0; d >>= 1) {
+    if (tid < d) g[tid] += g[tid + d];
+    __syncthreads();
+  }
+  const double r = g[0];
+  __syncthreads();
+  return r;
+}
+
+/* the periodic ring without Dirichlet nodes (A1/A2): right-hand side
+ * projected to zero mean, node 0 pinned, rows 1..n-1 solved by the two
+ * scans, then the mean of phi removed; sums in tri_sum's order */
+__device__ __forceinline__ void tri_solve_ring_group(const TriArgs &t, double *sA, double *sB,
+                                                     double *sy, double *ga, double *gb) {
+  const int n = t.n, m = n - 1;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) sy[i] = -(t.rho[i] * t.scale);
+  __syncthreads();
+  const double mean = tri_sum(sy, n, ga) / (double)n;
+  for (int j = threadIdx.x; j < m; j += blockDim.x) {
+    const int i = j + 1;
+    const double inv = t.inv[i];
+    sA[j] = -t.a[i] * inv;
+    sB[j] = (sy[i] - mean) * inv;
+  }
+  __syncthreads();
+  tri_scan(sA, sB, m, 0, sy, ga, gb);
+  for (int j = threadIdx.x; j < m; j += blockDim.x) sA[j] = -t.cp[j + 1];
+  __syncthreads();
+  tri_scan(sA, sy, m, 1, sB, ga, gb);
+  /* phi over all n nodes in sy: node 0 pinned at 0 */
+  for (int i = threadIdx.x; i < n; i += blockDim.x) sy[i] = i == 0 ? 0.0 : sB[i - 1];
+  __syncthreads();
+  const double m2 = tri_sum(sy, n, ga) / (double)n;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) t.phi[i] = sy[i] - m2;
+}
+
 __device__ __forceinline__ void tri_solve_group(const TriArgs &t) {
   const int n = t.n;
+  __shared__ double sA[TRI_BLOCK * TRI_K], sB[TRI_BLOCK * TRI_K], sy[TRI_BLOCK * TRI_K];
+  __shared__ double ga[TRI_BLOCK], gb[TRI_BLOCK];
+  if (t.singular && n <= TRI_BLOCK * TRI_K) {
+    tri_solve_ring_group(t, sA, sB, sy, ga, gb);
+    return;
+  }
   if (t.singular || n > TRI_BLOCK * TRI_K) {
     if (threadIdx.x == 0) tri_solve_body(t);
     return;
   }
-  __shared__ double sA[TRI_BLOCK * TRI_K], sB[TRI_BLOCK * TRI_K], sy[TRI_BLOCK * TRI_K];
-  __shared__ double ga[TRI_BLOCK], gb[TRI_BLOCK];
   for (int i = threadIdx.x; i < n; i += blockDim.x) {
     const int tg = t.tags[i];
     const double r = tg ? t.tagval[tg] : -(t.rho[i] * t.scale);

@@ -499,164 +499,6 @@ __global__ __launch_bounds__(EW_BLOCK) void k_ws_sumsq(const float *x, long long
 }
 
 /* ------------------------------------------------------------------------ */
-/* the network inside the iteration: flexible CG on the interior nodes of     */
-/* the network's grid, A x = 4 x - (4 neighbours) = f = dx^2 rho / eps0, walls */
-/* 0 (the reference system with its Dirichlet columns eliminated and the rows  */
-/* multiplied by -dx^2).  Node k = j nx + i; vectors keep 0 on the walls.      */
-
-#define LS_BLOCK 256
-enum { LS_FF = 0, LS_RR = 1, LS_ZQ = 2, LS_PQ = 3, LS_RP = 4, LS_NS = 8 };
-
-__device__ __forceinline__ double ls_Ax(const double *x, int nx, long long k) {
-  return 4.0 * x[k] - x[k - 1] - x[k + 1] - x[k - nx] - x[k + nx];
-}
-
-/* interior node of flat index q < (nx-2)^2 */
-__device__ __forceinline__ long long ls_node(long long q, int nx) {
-  const int m = nx - 2;
-  const long long j = q / m;
-  return (j + 1) * nx + (q - j * m) + 1;
-}
-
-/* block sum, valid in thread 0 (wave shuffles, then the waves in order) */
-__device__ __forceinline__ double ls_block_sum(double v, double *red) {
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o);
-  __syncthreads();
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
-  __syncthreads();
-  double s = 0.0;
-  if (threadIdx.x == 0)
-    for (int q = 0; q < LS_BLOCK / 64; ++q) s += red[q];
-  return s;
-}
-
-/* x = phi (or 0), r = f - A x; partials of f.f and r.r */
-__global__ __launch_bounds__(LS_BLOCK) void k_ls_init(const double *rho, const double *phi,
-                                                      int nx, double fscale, double *x,
-                                                      double *r, double *part) {
-  __shared__ double red[LS_BLOCK / 64];
-  const long long n = (long long)(nx - 2) * (nx - 2);
-  double ff = 0.0, rr = 0.0;
-  for (long long q = blockIdx.x * (long long)LS_BLOCK + threadIdx.x; q < n;
-       q += (long long)gridDim.x * LS_BLOCK) {
-    const long long k = ls_node(q, nx);
-    const double f = rho[k] * fscale;
-    const double xv = phi ? phi[k] : 0.0;
-    const double rv = phi ? f - ls_Ax(phi, nx, k) : f;
-    x[k] = xv;
-    r[k] = rv;
-    ff += f * f;
-    rr += rv * rv;
-  }
-  ff = ls_block_sum(ff, red);
-  rr = ls_block_sum(rr, red);
-  if (threadIdx.x == 0) {
-    part[2 * blockIdx.x] = ff;
-    part[2 * blockIdx.x + 1] = rr;
-  }
-}
-
-/* out[v] = sum over blocks, in block order, of part[nv b + v] */
-__global__ void k_ls_reduce(const double *part, int nb, int nv, double *out) {
-  if (blockIdx.x != 0 || threadIdx.x >= (unsigned)nv) return;
-  double s = 0.0;
-  for (int b = 0; b < nb; ++b) s += part[nv * b + threadIdx.x];
-  out[threadIdx.x] = s;
-}
-
-/* zs = z + om4 (r - A z): one damped Jacobi sweep on what the network left */
-__global__ __launch_bounds__(LS_BLOCK) void k_ls_smooth(const double *r, const double *z,
-                                                        double *zs, int nx, double om4) {
-  const long long n = (long long)(nx - 2) * (nx - 2);
-  for (long long q = blockIdx.x * (long long)LS_BLOCK + threadIdx.x; q < n;
-       q += (long long)gridDim.x * LS_BLOCK) {
-    const long long k = ls_node(q, nx);
-    zs[k] = z[k] + om4 * (r[k] - ls_Ax(z, nx, k));
-  }
-}
-
-/* partials of a.b */
-__global__ __launch_bounds__(LS_BLOCK) void k_ls_dot(const double *a, const double *b, int nx,
-                                                     double *part) {
-  __shared__ double red[LS_BLOCK / 64];
-  const long long n = (long long)(nx - 2) * (nx - 2);
-  double s = 0.0;
-  for (long long q = blockIdx.x * (long long)LS_BLOCK + threadIdx.x; q < n;
-       q += (long long)gridDim.x * LS_BLOCK) {
-    const long long k = ls_node(q, nx);
-    s += a[k] * b[k];
-  }
-  s = ls_block_sum(s, red);
-  if (threadIdx.x == 0) part[blockIdx.x] = s;
-}
-
-/* p = z + beta p_prev, beta = -(z.q_prev)/(p_prev.q_prev) (none in the first
- * iteration): the new direction A-orthogonal to the previous one */
-__global__ __launch_bounds__(LS_BLOCK) void k_ls_dir(const double *z, const double *pprev,
-                                                     double *p, int nx, const double *s,
-                                                     int first) {
-  const long long n = (long long)(nx - 2) * (nx - 2);
-  const double beta = first ? 0.0 : -s[LS_ZQ] / s[LS_PQ];
-  for (long long q = blockIdx.x * (long long)LS_BLOCK + threadIdx.x; q < n;
-       q += (long long)gridDim.x * LS_BLOCK) {
-    const long long k = ls_node(q, nx);
-    p[k] = first ? z[k] : z[k] + beta * pprev[k];
-  }
-}
-
-/* q = A p; partials of p.q and r.p */
-__global__ __launch_bounds__(LS_BLOCK) void k_ls_Ap(const double *p, const double *r, double *qv,
-                                                    int nx, double *part) {
-  __shared__ double red[LS_BLOCK / 64];
-  const long long n = (long long)(nx - 2) * (nx - 2);
-  double pq = 0.0, rp = 0.0;
-  for (long long q = blockIdx.x * (long long)LS_BLOCK + threadIdx.x; q < n;
-       q += (long long)gridDim.x * LS_BLOCK) {
-    const long long k = ls_node(q, nx);
-    const double a = ls_Ax(p, nx, k);
-    qv[k] = a;
-    pq += p[k] * a;
-    rp += r[k] * p[k];
-  }
-  pq = ls_block_sum(pq, red);
-  rp = ls_block_sum(rp, red);
-  if (threadIdx.x == 0) {
-    part[2 * blockIdx.x] = pq;
-    part[2 * blockIdx.x + 1] = rp;
-  }
-}
-
-/* alpha = (r.p)/(p.q); x += alpha p, r -= alpha q; partials of r.r */
-__global__ __launch_bounds__(LS_BLOCK) void k_ls_xr(double *x, double *r, const double *p,
-                                                    const double *qv, int nx, const double *s,
-                                                    double *part) {
-  __shared__ double red[LS_BLOCK / 64];
-  const long long n = (long long)(nx - 2) * (nx - 2);
-  const double alpha = s[LS_RP] / s[LS_PQ];
-  double rr = 0.0;
-  for (long long q = blockIdx.x * (long long)LS_BLOCK + threadIdx.x; q < n;
-       q += (long long)gridDim.x * LS_BLOCK) {
-    const long long k = ls_node(q, nx);
-    x[k] = __builtin_fma(alpha, p[k], x[k]);
-    const double rv = __builtin_fma(-alpha, qv[k], r[k]);
-    r[k] = rv;
-    rr += rv * rv;
-  }
-  rr = ls_block_sum(rr, red);
-  if (threadIdx.x == 0) part[blockIdx.x] = rr;
-}
-
-/* phi (interior nodes) = x */
-__global__ __launch_bounds__(LS_BLOCK) void k_ls_out(const double *x, double *phi, int nx) {
-  const long long n = (long long)(nx - 2) * (nx - 2);
-  for (long long q = blockIdx.x * (long long)LS_BLOCK + threadIdx.x; q < n;
-       q += (long long)gridDim.x * LS_BLOCK) {
-    const long long k = ls_node(q, nx);
-    phi[k] = x[k];
-  }
-}
-
-/* ------------------------------------------------------------------------ */
 /* host side                                                                  */
 
 struct WsConv {
@@ -694,8 +536,6 @@ struct iskra_ws {
   float *ds_out = nullptr;
   long long ds_cap = 0, ds_n = 0;
   /* the network-preconditioned solve (node grid, FP64; walls stay 0) */
-  double *lx = nullptr, *lr = nullptr, *lz = nullptr, *lzs = nullptr, *lp[2] = {nullptr, nullptr},
-         *lq[2] = {nullptr, nullptr}, *lpart = nullptr, *ls = nullptr;
 };
 
 static int ws_side(const iskra_ws *w, int l) { return w->S >> l; }
@@ -962,10 +802,9 @@ static void ws_release(iskra_ws *w) {
     ws_free_tensor(&w->Da[l]);
     ws_free_tensor(&w->D[l]);
   }
-  void *ptrs[] = {w->params, w->grads, w->m,     w->v,     w->out,    w->tgt,   w->gout,
-                  w->gm,     w->part,  w->dpart, w->scal,  w->idx,    w->xin,   w->ds_in,
-                  w->ds_out, w->lx,    w->lr,    w->lz,    w->lzs,    w->lp[0], w->lp[1],
-                  w->lq[0],  w->lq[1], w->lpart, w->ls};
+  void *ptrs[] = {w->params, w->grads, w->m,     w->v,     w->out,    w->tgt,  w->gout,
+                  w->gm,     w->part,  w->dpart, w->scal,  w->idx,    w->xin,  w->ds_in,
+                  w->ds_out};
   for (void *p : ptrs) isk_free(p);
 }
 
@@ -1317,158 +1156,6 @@ int iskra_ws_eval(iskra_ws *w, int64_t first, int64_t count, double *mse) {
   return ISKRA_OK;
 }
 
-/* z = the network's estimate of A^-1 r (input r / rms(r), output scaled back
- * by gain * rms(r)); then, when om4 > 0, one damped Jacobi sweep */
-static int ls_precond(iskra_ws *w, int nx, double om4, const double **z) {
-  iskra_ctx *c = w->ctx;
-  const long long cnt = (long long)(w->S - 1) * (w->S - 1);
-  const unsigned nb = std::min(ws_grid(cnt), (unsigned)w->dpart_n);
-  hipLaunchKernelGGL(k_ws_rhs_sumsq, dim3(nb), dim3(EW_BLOCK), 0, c->stream,
-                     (const double *)w->lr, nx, 1.0, w->dpart);
-  ISK_CHECK_LAUNCH();
-  hipLaunchKernelGGL(k_ws_rhs_scale, dim3(1), dim3(64), 0, c->stream, (const double *)w->dpart,
-                     (int)nb, cnt, w->scal);
-  ISK_CHECK_LAUNCH();
-  const long long S2 = (long long)w->S * w->S;
-  hipLaunchKernelGGL(k_ws_rhs_store, dim3(ws_grid(S2)), dim3(EW_BLOCK), 0, c->stream,
-                     (const double *)w->lr, (const double *)nullptr, nx, 1.0,
-                     (const double *)w->scal, (uint16_t *)nullptr, w->A0.a, (float *)nullptr);
-  ISK_CHECK_LAUNCH();
-  WRC(ws_forward(w, 1));
-  hipLaunchKernelGGL(k_ws_phi_out, dim3(ws_grid(S2)), dim3(EW_BLOCK), 0, c->stream,
-                     (const float *)w->out, w->lz, nx, w->gain, (const double *)w->scal);
-  ISK_CHECK_LAUNCH();
-  *z = w->lz;
-  if (om4 > 0.0) {
-    const long long n = (long long)(nx - 2) * (nx - 2);
-    const unsigned g = (unsigned)std::min((n + LS_BLOCK - 1) / LS_BLOCK, 1024ll);
-    hipLaunchKernelGGL(k_ls_smooth, dim3(g), dim3(LS_BLOCK), 0, c->stream,
-                       (const double *)w->lr, (const double *)w->lz, w->lzs, nx, om4);
-    ISK_CHECK_LAUNCH();
-    *z = w->lzs;
-  }
-  return ISKRA_OK;
-}
-
-int iskra_ws_solve(iskra_ws *w, int warm, double omega, int max_iter, int *iters,
-                   double *relres) {
-  WRC(ws_check(w));
-  WRC(ws_check_grid(w));
-  iskra_ctx *c = w->ctx;
-  IskPoisson &p = c->ps;
-  if (p.kind < 0) {
-    isk_set_error("ws_solve: no Poisson solver configured");
-    return ISKRA_E_STATE;
-  }
-  if (warm != ISKRA_WARM_ZERO && warm != ISKRA_WARM_PREV) return ISKRA_E_ARG;
-  if (!(omega >= 0.0) || omega >= 2.0) {
-    isk_set_error("ws_solve: the Jacobi weight must lie in [0, 2)");
-    return ISKRA_E_ARG;
-  }
-  const int nx = c->g.nx;
-  /* the network's system: phi = 0 on the four walls, nothing else fixed */
-  bool ok = !p.per_x && !p.per_y && (int)p.tags.size() == nx * nx;
-  for (int j = 0; ok && j < nx; ++j)
-    for (int i = 0; i < nx; ++i) {
-      const int t = p.tags[(size_t)j * nx + i];
-      const bool wall = i == 0 || j == 0 || i == nx - 1 || j == nx - 1;
-      if (wall ? (t == 0 || p.tag_val[t] != 0.0) : t != 0) {
-        ok = false;
-        break;
-      }
-    }
-  if (!ok) {
-    isk_set_error("ws_solve: the network solves phi = 0 on all four walls with no other "
-                  "Dirichlet node or periodic link");
-    return ISKRA_E_UNSUPPORTED;
-  }
-  const long long nn = (long long)nx * nx;
-  if (!w->lx) {
-    double **bufs[] = {&w->lx, &w->lr, &w->lz, &w->lzs, &w->lp[0], &w->lp[1], &w->lq[0],
-                       &w->lq[1]};
-    for (double **b : bufs) {
-      WRC(isk_alloc((void **)b, nn * 8));
-      ISK_HIP(hipMemsetAsync(*b, 0, nn * 8, c->stream));
-    }
-    WRC(isk_alloc((void **)&w->lpart, 4096 * 8));
-    WRC(isk_alloc((void **)&w->ls, LS_NS * 8));
-    ISK_HIP(hipMemsetAsync(w->ls, 0, LS_NS * 8, c->stream));
-  }
-  hipEvent_t ev = nullptr;
-  WRC(isk_prof_begin(c, &ev));
-  const long long n = (long long)(nx - 2) * (nx - 2);
-  const int nb = (int)std::min((n + LS_BLOCK - 1) / LS_BLOCK, 1024ll);
-  const double fscale = c->g.dx * c->g.dx / p.eps0;
-  const double om4 = 0.25 * omega;
-  const int maxit = max_iter > 0 ? max_iter : p.max_iter;
-  hipLaunchKernelGGL(k_ls_init, dim3(nb), dim3(LS_BLOCK), 0, c->stream, (const double *)c->rho,
-                     warm == ISKRA_WARM_PREV ? (const double *)c->phi : (const double *)nullptr,
-                     nx, fscale, w->lx, w->lr, w->lpart);
-  ISK_CHECK_LAUNCH();
-  hipLaunchKernelGGL(k_ls_reduce, dim3(1), dim3(64), 0, c->stream, (const double *)w->lpart,
-                     nb, 2, w->ls + LS_FF);
-  ISK_CHECK_LAUNCH();
-  double s[2];
-  ISK_HIP(hipMemcpyAsync(s, w->ls, 16, hipMemcpyDeviceToHost, c->stream));
-  ISK_HIP(hipStreamSynchronize(c->stream));
-  double rr = s[0] > 0.0 ? sqrt(s[1] / s[0]) : 0.0;
-  int it = 0, cur = 0;
-  while (rr > p.rtol && it < maxit) {
-    ++it;
-    const double *z = nullptr;
-    WRC(ls_precond(w, nx, om4, &z));
-    const int prev = cur ^ 1;
-    if (it > 1) {
-      hipLaunchKernelGGL(k_ls_dot, dim3(nb), dim3(LS_BLOCK), 0, c->stream, z,
-                         (const double *)w->lq[prev], nx, w->lpart);
-      ISK_CHECK_LAUNCH();
-      hipLaunchKernelGGL(k_ls_reduce, dim3(1), dim3(64), 0, c->stream,
-                         (const double *)w->lpart, nb, 1, w->ls + LS_ZQ);
-      ISK_CHECK_LAUNCH();
-    }
-    hipLaunchKernelGGL(k_ls_dir, dim3(nb), dim3(LS_BLOCK), 0, c->stream, z,
-                       (const double *)w->lp[prev], w->lp[cur], nx, (const double *)w->ls,
-                       it == 1 ? 1 : 0);
-    ISK_CHECK_LAUNCH();
-    hipLaunchKernelGGL(k_ls_Ap, dim3(nb), dim3(LS_BLOCK), 0, c->stream,
-                       (const double *)w->lp[cur], (const double *)w->lr, w->lq[cur], nx,
-                       w->lpart);
-    ISK_CHECK_LAUNCH();
-    hipLaunchKernelGGL(k_ls_reduce, dim3(1), dim3(64), 0, c->stream, (const double *)w->lpart,
-                       nb, 2, w->ls + LS_PQ);
-    ISK_CHECK_LAUNCH();
-    hipLaunchKernelGGL(k_ls_xr, dim3(nb), dim3(LS_BLOCK), 0, c->stream, w->lx, w->lr,
-                       (const double *)w->lp[cur], (const double *)w->lq[cur], nx,
-                       (const double *)w->ls, w->lpart);
-    ISK_CHECK_LAUNCH();
-    hipLaunchKernelGGL(k_ls_reduce, dim3(1), dim3(64), 0, c->stream, (const double *)w->lpart,
-                       nb, 1, w->ls + LS_RR);
-    ISK_CHECK_LAUNCH();
-    ISK_HIP(hipMemcpyAsync(s, w->ls, 16, hipMemcpyDeviceToHost, c->stream));
-    ISK_HIP(hipStreamSynchronize(c->stream));
-    rr = s[0] > 0.0 ? sqrt(s[1] / s[0]) : 0.0;
-    cur ^= 1;
-  }
-  hipLaunchKernelGGL(k_ls_out, dim3(nb), dim3(LS_BLOCK), 0, c->stream, (const double *)w->lx,
-                     c->phi, nx);
-  ISK_CHECK_LAUNCH();
-  WRC(isk_prof_end(c, ev, ISKRA_K_POISSON, 0));
-  ISK_HIP(hipStreamSynchronize(c->stream));
-  p.last_iters = it;
-  p.last_relres = rr;
-  c->stats.poisson_solves++;
-  c->stats.poisson_iters += it;
-  c->stats.last_iters = it;
-  c->stats.last_relres = rr;
-  if (iters) *iters = it;
-  if (relres) *relres = rr;
-  if (rr > p.rtol) {
-    isk_set_error("ws_solve: relative residual %.3e above %.3e after %d iterations", rr, p.rtol,
-                  it);
-    return ISKRA_E_CONVERGE;
-  }
-  return ISKRA_OK;
-}
 
 /* phi at the context's interior nodes <- network prediction from its rho */
 int iskra_ws_predict(iskra_ws *w) {

@@ -1005,6 +1005,21 @@ static void mir_tri_scan(const double *ma, const double *mb, int n, int rev, dou
   }
 }
 
+/* the device's tri_sum: per-thread sums of MIR_TRI_K consecutive elements,
+ * then a pairwise tree over the MIR_TRI_T partials */
+static double mir_tri_sum(const double *v, int n) {
+  double g[MIR_TRI_T];
+  for (int t = 0; t < MIR_TRI_T; ++t) {
+    double s = 0.0;
+    for (int k = 0; k < MIR_TRI_K; ++k)
+      if (t * MIR_TRI_K + k < n) s += v[t * MIR_TRI_K + k];
+    g[t] = s;
+  }
+  for (int d = MIR_TRI_T / 2; d > 0; d >>= 1)
+    for (int t = 0; t < d; ++t) g[t] += g[t + d];
+  return g[0];
+}
+
 EXPORT void mir_tri_solve(int n, const double *rho, const int8_t *tags, const double *tagval,
                           const double *a, const double *cp, const double *inv, double scale,
                           int singular, double *phi) {
@@ -1020,6 +1035,28 @@ EXPORT void mir_tri_solve(int n, const double *rho, const int8_t *tags, const do
     mir_tri_scan(mA, mB, n, 0, y);
     for (int i = 0; i < n; ++i) mA[i] = -cp[i];
     mir_tri_scan(mA, y, n, 1, phi);
+    free(mA);
+    free(mB);
+  } else if (singular && n <= MIR_TRI_T * MIR_TRI_K) {
+    /* the device's ring scan: tree sums, node 0 pinned, rows 1..n-1 */
+    const int m = n - 1;
+    double *f = (double *)malloc(sizeof(double) * n);
+    double *mA = (double *)malloc(sizeof(double) * (m > 0 ? m : 1));
+    double *mB = (double *)malloc(sizeof(double) * (m > 0 ? m : 1));
+    for (int i = 0; i < n; ++i) f[i] = -(rho[i] * scale);
+    const double mean = mir_tri_sum(f, n) / (double)n;
+    for (int j = 0; j < m; ++j) {
+      mA[j] = -a[j + 1] * inv[j + 1];
+      mB[j] = (f[j + 1] - mean) * inv[j + 1];
+    }
+    mir_tri_scan(mA, mB, m, 0, y);
+    for (int j = 0; j < m; ++j) mA[j] = -cp[j + 1];
+    mir_tri_scan(mA, y, m, 1, mB);
+    f[0] = 0.0;
+    for (int i = 1; i < n; ++i) f[i] = mB[i - 1];
+    const double m2 = mir_tri_sum(f, n) / (double)n;
+    for (int i = 0; i < n; ++i) phi[i] = f[i] - m2;
+    free(f);
     free(mA);
     free(mB);
   } else if (!singular) {
