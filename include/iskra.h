@@ -444,7 +444,7 @@ int iskra_batch_graphs_launched(iskra_batch *batch, int64_t *count);
  * right-hand side f = dx^2 rho / eps0 of a square 2-D grid of (side+1)^2
  * nodes with phi = 0 on all four walls to phi at the interior nodes; the
  * iterative solver then starts from that phi (ISKRA_WARM_PREV) and iterates
- * to its usual tolerance.  levels encoder/decoder levels with base*2^l
+ * to its usual tolerance.  levels encoder/decoder levels with base*2^min(l,3)
  * channels, 3x3 convolutions, BF16 activations and weights on MFMA with FP32
  * accumulation, FP32 master weights trained with Adam on the device.
  * Parameters are float32, layer by layer in the order enc0a, enc0b, ...,
@@ -461,8 +461,29 @@ int iskra_ws_set_params(iskra_ws *net, const float *params);
 int iskra_ws_get_params(iskra_ws *net, float *params);
 /* phi = output * gain * rms(f) (the network sees f / rms(f)) */
 int iskra_ws_set_gain(iskra_ws *net, double gain);
+/* Training objective of backward/train/eval:
+ *   ISKRA_WS_PHI_MSE (default): mean squared error of the output against the
+ *     target phi / (rms(f) gain) over the interior (side-1)^2 pixels (the
+ *     warm-start objective; omega and sweeps unused);
+ *   ISKRA_WS_ENERGY: the network as the coarse solver of a two-level cycle
+ *     (iskra_ws_solve's preconditioner): k = sweeps damped Jacobi sweeps
+ *     from 0, the network on the residual they leave, k sweeps again.  With
+ *     A v = 4 v - the 4 neighbours on the interior pixels (0 outside) and
+ *     P = (I - omega/4 A)^k, each sample (x, t) becomes (P x, P t) (what
+ *     the pre-sweeps leave and its exact correction), the network sees
+ *     bf16(P x), y_0 = gain * output, y_{j+1} = y_j + omega/4 (P x - A y_j)
+ *     for j < k, e = y_k - gain P t (= the cycle's error), and the loss is
+ *     e.Ae / (gain P t).A(gain P t) summed over the batch: the squared
+ *     A-norm of the error relative to the correction's, the quantity that
+ *     sets flexible CG's rate.  omega in [0, 2), sweeps in [0, 16].
+ *   ISKRA_WS_ENERGY_NET: the same pairs (P x, P t), scored on the network's
+ *     own output: e = gain * output - gain P t, no post-sweeps in the loss,
+ *     so rough error the post-sweeps would hide is charged in full. */
+enum { ISKRA_WS_PHI_MSE = 0, ISKRA_WS_ENERGY = 1, ISKRA_WS_ENERGY_NET = 2 };
+int iskra_ws_set_objective(iskra_ws *net, int objective, double omega,
+                           int sweeps);
 /* Host batches (side x side planes per sample, already normalised input):
- * forward -> out; backward: forward, mean squared error over the interior
+ * forward -> out; backward: forward, the objective's loss over the interior
  * (side-1)^2 pixels against target, gradients of all parameters -> grads. */
 int iskra_ws_forward(iskra_ws *net, int batch, const float *in, float *out);
 int iskra_ws_backward(iskra_ws *net, int batch, const float *in,
@@ -482,6 +503,23 @@ int iskra_ws_train(iskra_ws *net, int64_t first, int64_t count, int batch,
 int iskra_ws_eval(iskra_ws *net, int64_t first, int64_t count, double *mse);
 /* phi at the interior nodes of the context <- prediction from its rho */
 int iskra_ws_predict(iskra_ws *net);
+/* calculate_electric_potential (generalized_poisson.jl:372-378) with the
+ * network inside the iteration: flexible conjugate gradients on the
+ * context's system (each new direction A-orthogonalised against the previous
+ * one), every iteration preconditioned by z = the network applied to the
+ * residual: when omega > 0, `sweeps` damped Jacobi sweeps
+ * z += omega/4 (r - A z) from z = 0, then z += the network applied to
+ * r - A z (scaled by its RMS and the gain), then `sweeps` sweeps again (train
+ * with ISKRA_WS_ENERGY and the same omega and sweeps; sweeps in [0, 16]).
+ * The system must be the network's: phi = 0 on all four
+ * walls, no other Dirichlet node and no periodic link (ISKRA_E_UNSUPPORTED
+ * otherwise).  Iterates until the FP64 relative residual ||f - A phi|| /
+ * ||f|| meets the solver's rtol (ISKRA_E_CONVERGE after max_iter
+ * iterations; 0: the solver's max_iter); warm: ISKRA_WARM_ZERO or
+ * ISKRA_WARM_PREV (the current phi).  Leaves phi like iskra_poisson_solve
+ * (E is not recomputed). */
+int iskra_ws_solve(iskra_ws *net, int warm, double omega, int sweeps,
+                   int max_iter, int *iters, double *relres);
 /* ---- diagnostics ---------------------------------------------------------- */
 /* Device-to-device copy of `bytes` bytes, `iters` times, with the library's
  * own streaming copy kernel (nontemporal != 0: non-temporal loads/stores,

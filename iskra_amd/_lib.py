@@ -21,6 +21,7 @@ POISSON_DIRECT, POISSON_PCG, POISSON_MG, POISSON_MGPCG, POISSON_SPECTRAL = range
 PUSHER_XY, PUSHER_RZ = 0, 1
 SURF_ABSORBING, SURF_REFLECTIVE, SURF_PERIODIC = 1, 2, 3
 WARM_ZERO, WARM_PREV = 0, 1
+WS_PHI_MSE, WS_ENERGY, WS_ENERGY_NET = 0, 1, 2
 COLL_ELASTIC_ISOTROPIC, COLL_ELASTIC_BACKWARD, COLL_EXCITATION, COLL_IONIZATION = range(4)
 COLL_INELASTIC_BACKWARD = 4
 NOALLOC_SCRATCH_BYTES = 4 << 20
@@ -46,6 +47,7 @@ EXPORTS = [
     "iskra_ws_get_params", "iskra_ws_set_gain", "iskra_ws_forward", "iskra_ws_backward",
     "iskra_ws_dataset_reserve", "iskra_ws_capture", "iskra_ws_dataset_size",
     "iskra_ws_dataset_rms", "iskra_ws_train", "iskra_ws_eval", "iskra_ws_predict",
+    "iskra_ws_set_objective", "iskra_ws_solve",
     "iskra_batch_create", "iskra_batch_destroy", "iskra_batch_set_graph", "iskra_batch_step",
     "iskra_batch_graphs_launched",
 ]
@@ -154,6 +156,8 @@ def lib():
             "iskra_ws_train": (I, [P, I64, I64, I, I, C.c_float, U64, P]),
             "iskra_ws_eval": (I, [P, I64, I64, C.POINTER(D)]),
             "iskra_ws_predict": (I, [P]),
+            "iskra_ws_set_objective": (I, [P, I, D, I]),
+            "iskra_ws_solve": (I, [P, I, D, I, I, C.POINTER(I), C.POINTER(D)]),
             "iskra_batch_create": (I, [P, I, C.POINTER(P)]),
             "iskra_batch_destroy": (I, [P]),
             "iskra_batch_set_graph": (I, [P, I]),
@@ -543,7 +547,7 @@ class Context:
 
 class WarmStart:
     """Learned warm start for the 2-D Poisson solve of `ctx` (iskra_ws_*):
-    encoder-decoder with `levels` levels and base*2^l channels on a
+    encoder-decoder with `levels` levels and base*2^min(l,3) channels on a
     side x side pixel grid (the context grid has side+1 nodes per axis)."""
 
     def __init__(self, ctx, levels=4, base=32, side=512, max_batch=8):
@@ -626,6 +630,20 @@ class WarmStart:
 
     def predict(self):
         check(lib().iskra_ws_predict(self.h))
+
+    def set_objective(self, objective, omega=0.0, sweeps=1):
+        """WS_PHI_MSE (warm start) or WS_ENERGY (preconditioner, with the
+        Jacobi weight and sweep count the solve will use)."""
+        check(lib().iskra_ws_set_objective(self.h, int(objective), float(omega), int(sweeps)))
+
+    def solve(self, warm=WARM_ZERO, omega=0.8, sweeps=1, max_iter=0):
+        """Flexible CG on the context's system preconditioned by the network
+        (then `sweeps` damped Jacobi sweeps when omega > 0); returns
+        (iters, relres)."""
+        it, rr = C.c_int(), C.c_double()
+        check(lib().iskra_ws_solve(self.h, int(warm), float(omega), int(sweeps), int(max_iter),
+                                   C.byref(it), C.byref(rr)))
+        return it.value, rr.value
 
 
 class Batch:

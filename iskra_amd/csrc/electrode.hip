@@ -1,6 +1,6 @@
 /*
  * electrode.hip -- the surface tracker's tables, electrodes and the RLC
- * circuit of the reference (ParticleInCell/src/pic/surfaces/*.jl,
+ * circuit of the reference (ParticleInCell/src/pic/surfaces/{track,check,hit}.jl,
  * pic/circuit_coupling.jl, problem/configuration.jl:17-76,
  * Circuit/src/Circuit.jl:117-136).
  *

@@ -5,11 +5,14 @@
  * of a square grid with phi = 0 on all four walls to phi at the interior
  * nodes; its output seeds the iterative solver (PCG or multigrid), which then
  * iterates to the same FP64 residual as from any other start, so the network
- * can only change the iteration count, never the answer.  There is no
+ * can only change the iteration count, never the answer.  Trained with the
+ * energy objective (the A-norm of the error left by the network plus one
+ * damped Jacobi sweep), the same network preconditions every iteration of a
+ * flexible CG on the FP64 residual (iskra_ws_solve).  There is no
  * counterpart in the reference (its Poisson solve is a dense direct solve,
  * FiniteDifferenceMethod/src/generalized_poisson.jl:201-203,372-378).
  *
- * Network (L levels, channels c_l = base * 2^l, 3x3 convolutions, zero
+ * Network (L levels, channels c_l = base * 2^min(l, 3), 3x3 convolutions, zero
  * padding, no biases, so the map is positively homogeneous and inputs are
  * normalised by their RMS):
  *   encoder  l = 0..L-1:  Ea_l = relu(conv(P_l)),  E_l = relu(conv(Ea_l)),
@@ -39,11 +42,12 @@ typedef uint16_t u16x8 __attribute__((ext_vector_type(8)));
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-#define WS_MAX_LEVELS 6
+#define WS_MAX_LEVELS 8
 #define WS_IN_CH 8       /* input channels (channel 0 carries f) */
 #define WS_OUT_CH 16     /* output channels computed (channel 0 is used) */
 #define WS_PART_BYTES (64ll << 20)
 #define EW_BLOCK 256
+#define WS_MAX_SWEEPS 16
 
 __host__ __device__ inline uint16_t ws_f2bf(float f) {
   uint32_t u = __builtin_bit_cast(uint32_t, f);
@@ -499,6 +503,310 @@ __global__ __launch_bounds__(EW_BLOCK) void k_ws_sumsq(const float *x, long long
 }
 
 /* ------------------------------------------------------------------------ */
+/* energy objective: the network (times gain) followed by k damped Jacobi     */
+/* sweeps, scored in the A-norm.  Plane pixels (h, w) < S-1 are the interior  */
+/* nodes; A v = 4 v - (4 neighbours), neighbours outside the interior are 0.  */
+
+__device__ __forceinline__ float eg_nb(const float *v, long long i, int h, int w, int S) {
+  float s = 0.f;
+  if (h > 0) s += v[i - S];
+  if (h < S - 2) s += v[i + S];
+  if (w > 0) s += v[i - 1];
+  if (w < S - 2) s += v[i + 1];
+  return s;
+}
+
+/* one sweep dst = sc src + om4 (x - A sc src), x = the network input (A0
+ * channel 0).  last: dst -= gain t (the error e) and part[nb + b] = the
+ * block's sum of (gain t).A(gain t) */
+__global__ __launch_bounds__(EW_BLOCK) void k_ws_eg_sweep(const float *src, float *dst,
+                                                          const uint16_t *A0, const float *t,
+                                                          int S, long long npix, float sc,
+                                                          float om4, float gain, int last,
+                                                          double *part, int nb) {
+  __shared__ double red[EW_BLOCK];
+  double acc = 0.0;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < npix;
+       i += (long long)gridDim.x * blockDim.x) {
+    const int pix = (int)(i % ((long long)S * S));
+    const int h = pix / S, w = pix - (pix / S) * S;
+    float v = 0.f;
+    if (h < S - 1 && w < S - 1) {
+      const float y = sc * src[i];
+      const float ay = 4.f * y - sc * eg_nb(src, i, h, w, S);
+      const float x = ws_bf2f(A0[i * WS_IN_CH]);
+      v = y + om4 * (x - ay);
+      if (last) {
+        const float tv = gain * t[i];
+        v -= tv;
+        acc += (double)tv * (double)(4.f * tv - gain * eg_nb(t, i, h, w, S));
+      }
+    }
+    dst[i] = v;
+  }
+  if (!last) return;
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = EW_BLOCK / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[nb + blockIdx.x] = red[0];
+}
+
+/* ae = A e; part[b] = sum e.Ae */
+__global__ __launch_bounds__(EW_BLOCK) void k_ws_eg_ae(const float *e, float *ae, int S,
+                                                       long long npix, double *part) {
+  __shared__ double red[EW_BLOCK];
+  double acc = 0.0;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < npix;
+       i += (long long)gridDim.x * blockDim.x) {
+    const int pix = (int)(i % ((long long)S * S));
+    const int h = pix / S, w = pix - (pix / S) * S;
+    float a = 0.f;
+    if (h < S - 1 && w < S - 1) {
+      a = 4.f * e[i] - eg_nb(e, i, h, w, S);
+      acc += (double)e[i] * (double)a;
+    }
+    ae[i] = a;
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = EW_BLOCK / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+/* back through one sweep: h' = coef (h - om4 A h); Gout (non-null, the last
+ * sweep) receives bf16(h') in channel 0 and 0 in the other channels, else
+ * dst = h' */
+__global__ void k_ws_eg_back(const float *hsrc, float *dst, uint16_t *Gout, int S,
+                             long long npix, float om4, float coef) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < npix;
+       i += (long long)gridDim.x * blockDim.x) {
+    const int pix = (int)(i % ((long long)S * S));
+    const int h = pix / S, w = pix - (pix / S) * S;
+    float g = 0.f;
+    if (h < S - 1 && w < S - 1) {
+      const float a = hsrc[i];
+      g = coef * (a - om4 * (4.f * a - eg_nb(hsrc, i, h, w, S)));
+    }
+    if (Gout) {
+      uint16_t *gp = Gout + i * WS_OUT_CH;
+      gp[0] = ws_f2bf(g);
+      for (int c = 1; c < WS_OUT_CH; ++c) gp[c] = 0;
+    } else {
+      dst[i] = g;
+    }
+  }
+}
+
+/* dst = src - om4 A src on the interior (0 outside): one factor of the
+ * pre-smoothing polynomial P = (I - om4 A)^k that maps a training pair (x, t)
+ * to the pair the network meets after k pre-sweeps, (P x, P t) */
+__global__ void k_ws_eg_poly(const float *src, float *dst, int S, long long npix, float om4) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < npix;
+       i += (long long)gridDim.x * blockDim.x) {
+    const int pix = (int)(i % ((long long)S * S));
+    const int h = pix / S, w = pix - (pix / S) * S;
+    float v = 0.f;
+    if (h < S - 1 && w < S - 1) {
+      const float a = src[i];
+      v = a - om4 * (4.f * a - eg_nb(src, i, h, w, S));
+    }
+    dst[i] = v;
+  }
+}
+
+/* plane <-> channel 0 of the network input */
+__global__ void k_ws_a0_get(const uint16_t *A0, float *x, long long npix) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < npix;
+       i += (long long)gridDim.x * blockDim.x)
+    x[i] = ws_bf2f(A0[i * WS_IN_CH]);
+}
+
+/* ------------------------------------------------------------------------ */
+/* the network inside the iteration: flexible CG on the interior nodes of     */
+/* the network's grid, A x = 4 x - (4 neighbours) = f = dx^2 rho / eps0, walls */
+/* 0 (the reference system with its Dirichlet columns eliminated and the rows  */
+/* multiplied by -dx^2).  Node k = j nx + i; vectors keep 0 on the walls.      */
+
+#define LS_BLOCK 256
+enum { LS_FF = 0, LS_RR = 1, LS_ZQ = 2, LS_PQ = 3, LS_RP = 4, LS_NS = 8 };
+
+__device__ __forceinline__ double ls_Ax(const double *x, int nx, long long k) {
+  return 4.0 * x[k] - x[k - 1] - x[k + 1] - x[k - nx] - x[k + nx];
+}
+
+/* interior node of flat index q < (nx-2)^2 */
+__device__ __forceinline__ long long ls_node(long long q, int nx) {
+  const int m = nx - 2;
+  const long long j = q / m;
+  return (j + 1) * nx + (q - j * m) + 1;
+}
+
+/* block sum, valid in thread 0 (wave shuffles, then the waves in order) */
+__device__ __forceinline__ double ls_block_sum(double v, double *red) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0)
+    for (int q = 0; q < LS_BLOCK / 64; ++q) s += red[q];
+  return s;
+}
+
+/* x = phi (or 0), r = f - A x; partials of f.f and r.r */
+__global__ __launch_bounds__(LS_BLOCK) void k_ls_init(const double *rho, const double *phi,
+                                                      int nx, double fscale, double *x,
+                                                      double *r, double *part) {
+  __shared__ double red[LS_BLOCK / 64];
+  const long long n = (long long)(nx - 2) * (nx - 2);
+  double ff = 0.0, rr = 0.0;
+  for (long long q = blockIdx.x * (long long)LS_BLOCK + threadIdx.x; q < n;
+       q += (long long)gridDim.x * LS_BLOCK) {
+    const long long k = ls_node(q, nx);
+    const double f = rho[k] * fscale;
+    const double xv = phi ? phi[k] : 0.0;
+    const double rv = phi ? f - ls_Ax(phi, nx, k) : f;
+    x[k] = xv;
+    r[k] = rv;
+    ff += f * f;
+    rr += rv * rv;
+  }
+  ff = ls_block_sum(ff, red);
+  rr = ls_block_sum(rr, red);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = ff;
+    part[2 * blockIdx.x + 1] = rr;
+  }
+}
+
+/* out[v] = sum over blocks, in block order, of part[nv b + v] */
+__global__ void k_ls_reduce(const double *part, int nb, int nv, double *out) {
+  if (blockIdx.x != 0 || threadIdx.x >= (unsigned)nv) return;
+  double s = 0.0;
+  for (int b = 0; b < nb; ++b) s += part[nv * b + threadIdx.x];
+  out[threadIdx.x] = s;
+}
+
+/* zs = z + om4 (r - A z): one damped Jacobi sweep (z null: from z = 0) */
+__global__ __launch_bounds__(LS_BLOCK) void k_ls_smooth(const double *r, const double *z,
+                                                        double *zs, int nx, double om4) {
+  const long long n = (long long)(nx - 2) * (nx - 2);
+  for (long long q = blockIdx.x * (long long)LS_BLOCK + threadIdx.x; q < n;
+       q += (long long)gridDim.x * LS_BLOCK) {
+    const long long k = ls_node(q, nx);
+    zs[k] = z ? z[k] + om4 * (r[k] - ls_Ax(z, nx, k)) : om4 * r[k];
+  }
+}
+
+/* rp = r - A z: what the pre-sweeps left for the network */
+__global__ __launch_bounds__(LS_BLOCK) void k_ls_resid(const double *r, const double *z,
+                                                       double *rp, int nx) {
+  const long long n = (long long)(nx - 2) * (nx - 2);
+  for (long long q = blockIdx.x * (long long)LS_BLOCK + threadIdx.x; q < n;
+       q += (long long)gridDim.x * LS_BLOCK) {
+    const long long k = ls_node(q, nx);
+    rp[k] = r[k] - ls_Ax(z, nx, k);
+  }
+}
+
+/* z += c: the network's correction added to the pre-sweeps' iterate */
+__global__ __launch_bounds__(LS_BLOCK) void k_ls_add(double *z, const double *c, int nx) {
+  const long long n = (long long)(nx - 2) * (nx - 2);
+  for (long long q = blockIdx.x * (long long)LS_BLOCK + threadIdx.x; q < n;
+       q += (long long)gridDim.x * LS_BLOCK) {
+    const long long k = ls_node(q, nx);
+    z[k] += c[k];
+  }
+}
+
+/* partials of a.b */
+__global__ __launch_bounds__(LS_BLOCK) void k_ls_dot(const double *a, const double *b, int nx,
+                                                     double *part) {
+  __shared__ double red[LS_BLOCK / 64];
+  const long long n = (long long)(nx - 2) * (nx - 2);
+  double s = 0.0;
+  for (long long q = blockIdx.x * (long long)LS_BLOCK + threadIdx.x; q < n;
+       q += (long long)gridDim.x * LS_BLOCK) {
+    const long long k = ls_node(q, nx);
+    s += a[k] * b[k];
+  }
+  s = ls_block_sum(s, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+/* p = z + beta p_prev, beta = -(z.q_prev)/(p_prev.q_prev) (none in the first
+ * iteration): the new direction A-orthogonal to the previous one */
+__global__ __launch_bounds__(LS_BLOCK) void k_ls_dir(const double *z, const double *pprev,
+                                                     double *p, int nx, const double *s,
+                                                     int first) {
+  const long long n = (long long)(nx - 2) * (nx - 2);
+  const double beta = first ? 0.0 : -s[LS_ZQ] / s[LS_PQ];
+  for (long long q = blockIdx.x * (long long)LS_BLOCK + threadIdx.x; q < n;
+       q += (long long)gridDim.x * LS_BLOCK) {
+    const long long k = ls_node(q, nx);
+    p[k] = first ? z[k] : z[k] + beta * pprev[k];
+  }
+}
+
+/* q = A p; partials of p.q and r.p */
+__global__ __launch_bounds__(LS_BLOCK) void k_ls_Ap(const double *p, const double *r, double *qv,
+                                                    int nx, double *part) {
+  __shared__ double red[LS_BLOCK / 64];
+  const long long n = (long long)(nx - 2) * (nx - 2);
+  double pq = 0.0, rp = 0.0;
+  for (long long q = blockIdx.x * (long long)LS_BLOCK + threadIdx.x; q < n;
+       q += (long long)gridDim.x * LS_BLOCK) {
+    const long long k = ls_node(q, nx);
+    const double a = ls_Ax(p, nx, k);
+    qv[k] = a;
+    pq += p[k] * a;
+    rp += r[k] * p[k];
+  }
+  pq = ls_block_sum(pq, red);
+  rp = ls_block_sum(rp, red);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = pq;
+    part[2 * blockIdx.x + 1] = rp;
+  }
+}
+
+/* alpha = (r.p)/(p.q); x += alpha p, r -= alpha q; partials of r.r */
+__global__ __launch_bounds__(LS_BLOCK) void k_ls_xr(double *x, double *r, const double *p,
+                                                    const double *qv, int nx, const double *s,
+                                                    double *part) {
+  __shared__ double red[LS_BLOCK / 64];
+  const long long n = (long long)(nx - 2) * (nx - 2);
+  const double alpha = s[LS_RP] / s[LS_PQ];
+  double rr = 0.0;
+  for (long long q = blockIdx.x * (long long)LS_BLOCK + threadIdx.x; q < n;
+       q += (long long)gridDim.x * LS_BLOCK) {
+    const long long k = ls_node(q, nx);
+    x[k] = __builtin_fma(alpha, p[k], x[k]);
+    const double rv = __builtin_fma(-alpha, qv[k], r[k]);
+    r[k] = rv;
+    rr += rv * rv;
+  }
+  rr = ls_block_sum(rr, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = rr;
+}
+
+/* phi (interior nodes) = x */
+__global__ __launch_bounds__(LS_BLOCK) void k_ls_out(const double *x, double *phi, int nx) {
+  const long long n = (long long)(nx - 2) * (nx - 2);
+  for (long long q = blockIdx.x * (long long)LS_BLOCK + threadIdx.x; q < n;
+       q += (long long)gridDim.x * LS_BLOCK) {
+    const long long k = ls_node(q, nx);
+    phi[k] = x[k];
+  }
+}
+
+/* ------------------------------------------------------------------------ */
 /* host side                                                                  */
 
 struct WsConv {
@@ -535,7 +843,15 @@ struct iskra_ws {
   uint16_t *ds_in = nullptr;
   float *ds_out = nullptr;
   long long ds_cap = 0, ds_n = 0;
+  /* training objective (ISKRA_WS_*), its Jacobi weight / 4 and planes */
+  int obj = ISKRA_WS_PHI_MSE;
+  double om4 = 0.0;
+  int sweeps = 1;
+  float *eg_e = nullptr, *eg_ae = nullptr;
   /* the network-preconditioned solve (node grid, FP64; walls stay 0) */
+  double *lx = nullptr, *lr = nullptr, *lz = nullptr, *lzs = nullptr, *lp[2] = {nullptr, nullptr},
+         *lq[2] = {nullptr, nullptr}, *lpart = nullptr, *ls = nullptr, *lpre[2] = {nullptr, nullptr},
+         *lrp = nullptr;
 };
 
 static int ws_side(const iskra_ws *w, int l) { return w->S >> l; }
@@ -564,7 +880,9 @@ static void ws_free_tensor(WsTensor *t) {
   t->g = nullptr;
 }
 
-static int ws_chan(const iskra_ws *w, int l) { return w->base << l; }
+/* channels of level l: base * 2^l, held at base * 8 from level 3 on (so a
+ * deep network keeps its coarse levels affordable) */
+static int ws_chan(const iskra_ws *w, int l) { return w->base << std::min(l, 3); }
 
 /* layer order: enc0A, enc0B, ..., enc(L-1)B, dec(L-2)A, dec(L-2)B, ..., dec0B, out */
 static int ws_enc(int l, int b) { return 2 * l + b; }
@@ -706,8 +1024,67 @@ static int ws_forward(iskra_ws *w, int b) {
   return ws_fwd_conv(w, ws_out(w), b, last, nullptr, nullptr, w->out, 2);
 }
 
+static int ws_pre(iskra_ws *w, int b);
+
+/* energy objective on a batch (x, t) already mapped to (P x, P t) by
+ * ws_pre: z = k post-sweeps from gain * out, e = z - gain t (= P (gain out
+ * - gain t), the error of the whole two-level cycle), loss = sum e.Ae /
+ * sum (gain t).A(gain t) over the batch; with_grad: Gout = its gradient with
+ * respect to the output plane, 2 gain (I - om4 A)^k A e / denominator.
+ * Planes ping-pong between eg_e and eg_ae. */
+static int ws_energy_loss(iskra_ws *w, int b, bool with_grad, double *loss) {
+  hipStream_t st = w->ctx->stream;
+  const long long npix = ws_npix(w, 0, b);
+  const unsigned nb = std::min(ws_grid(npix), (unsigned)(w->dpart_n / 2));
+  /* no post-sweeps (sweeps = 0, or the network's own error scored):
+   * one pass with om4 = 0 forms e = gain (o - t) */
+  const bool post = w->obj == ISKRA_WS_ENERGY && w->sweeps > 0;
+  const int ks = post ? w->sweeps : 1;
+  const float gain = (float)w->gain, om4 = post ? (float)w->om4 : 0.f;
+  float *buf[2] = {w->eg_e, w->eg_ae};
+  const float *src = w->out;
+  int cur = 0;
+  for (int k = 0; k < ks; ++k) {
+    const int last = k == ks - 1;
+    hipLaunchKernelGGL(k_ws_eg_sweep, dim3(nb), dim3(EW_BLOCK), 0, st, src, buf[cur],
+                       (const uint16_t *)w->A0.a, (const float *)w->tgt, w->S, npix,
+                       k == 0 ? gain : 1.f, om4, gain, last, w->dpart, (int)nb);
+    ISK_CHECK_LAUNCH();
+    src = buf[cur];
+    cur ^= 1;
+  }
+  /* e in buf[cur ^ 1]; A e into buf[cur] */
+  hipLaunchKernelGGL(k_ws_eg_ae, dim3(nb), dim3(EW_BLOCK), 0, st, src, buf[cur], w->S, npix,
+                     w->dpart);
+  ISK_CHECK_LAUNCH();
+  std::vector<double> h(2 * nb);
+  ISK_HIP(hipMemcpyAsync(h.data(), w->dpart, 2 * nb * 8, hipMemcpyDeviceToHost, st));
+  ISK_HIP(hipStreamSynchronize(st));
+  double num = 0.0, den = 0.0;
+  for (unsigned i = 0; i < nb; ++i) {
+    num += h[i];
+    den += h[nb + i];
+  }
+  if (!(den > 0.0)) den = 1.0;
+  if (loss) *loss = num / den;
+  if (with_grad) {
+    /* the sweeps' adjoint, applied k times (A is symmetric); the first
+     * carries the factor 2 gain / den */
+    for (int k = 0; k < ks; ++k) {
+      const int last = k == ks - 1;
+      hipLaunchKernelGGL(k_ws_eg_back, dim3(ws_grid(npix)), dim3(EW_BLOCK), 0, st,
+                         (const float *)buf[cur], buf[cur ^ 1], last ? w->gout : nullptr, w->S,
+                         npix, om4, k == 0 ? (float)(2.0 * w->gain / den) : 1.f);
+      ISK_CHECK_LAUNCH();
+      cur ^= 1;
+    }
+  }
+  return ISKRA_OK;
+}
+
 /* loss of w->out against w->tgt; with_grad: also Gout; returns the mean */
 static int ws_loss(iskra_ws *w, int b, bool with_grad, double *loss) {
+  if (w->obj != ISKRA_WS_PHI_MSE) return ws_energy_loss(w, b, with_grad, loss);
   const long long npix = ws_npix(w, 0, b);
   const long long count = (long long)b * (w->S - 1) * (w->S - 1);
   const unsigned nb = std::min(ws_grid(npix), (unsigned)w->dpart_n);
@@ -802,9 +1179,11 @@ static void ws_release(iskra_ws *w) {
     ws_free_tensor(&w->Da[l]);
     ws_free_tensor(&w->D[l]);
   }
-  void *ptrs[] = {w->params, w->grads, w->m,     w->v,     w->out,    w->tgt,  w->gout,
-                  w->gm,     w->part,  w->dpart, w->scal,  w->idx,    w->xin,  w->ds_in,
-                  w->ds_out};
+  void *ptrs[] = {w->params, w->grads, w->m,     w->v,     w->out,    w->tgt,   w->gout,
+                  w->gm,     w->part,  w->dpart, w->scal,  w->idx,    w->xin,   w->ds_in,
+                  w->ds_out, w->eg_e,  w->eg_ae, w->lx,    w->lr,     w->lz,    w->lzs,
+                  w->lp[0],  w->lp[1], w->lq[0], w->lq[1], w->lpart,  w->ls,    w->lpre[0],
+                  w->lpre[1], w->lrp};
   for (void *p : ptrs) isk_free(p);
 }
 
@@ -913,6 +1292,8 @@ int iskra_ws_create(iskra_ctx *ctx, int levels, int base, int side, int max_batc
     WTRY(isk_alloc((void **)&w->tgt, npix * 4));
     WTRY(isk_alloc((void **)&w->xin, npix * 4));
     WTRY(isk_alloc((void **)&w->gout, npix * WS_OUT_CH * 2));
+    WTRY(isk_alloc((void **)&w->eg_e, npix * 4));
+    WTRY(isk_alloc((void **)&w->eg_ae, npix * 4));
     WTRY(isk_alloc((void **)&w->gm, w->gm_elems * 2));
     WTRY(isk_alloc((void **)&w->part, WS_PART_BYTES));
     w->dpart_n = 8192;
@@ -987,6 +1368,22 @@ int iskra_ws_set_gain(iskra_ws *w, double gain) {
   return ISKRA_OK;
 }
 
+int iskra_ws_set_objective(iskra_ws *w, int objective, double omega, int sweeps) {
+  WRC(ws_check(w));
+  if ((objective != ISKRA_WS_PHI_MSE && objective != ISKRA_WS_ENERGY &&
+       objective != ISKRA_WS_ENERGY_NET) ||
+      !(omega >= 0.0) ||
+      omega >= 2.0 || sweeps < 0 || sweeps > WS_MAX_SWEEPS) {
+    isk_set_error("ws_set_objective: objective ISKRA_WS_PHI_MSE, _ENERGY or _ENERGY_NET, omega in "
+                  "[0, 2), sweeps in [0, %d]", WS_MAX_SWEEPS);
+    return ISKRA_E_ARG;
+  }
+  w->obj = objective;
+  w->om4 = 0.25 * omega;
+  w->sweeps = sweeps;
+  return ISKRA_OK;
+}
+
 static int ws_host_input(iskra_ws *w, int batch, const float *in) {
   const long long npix = ws_npix(w, 0, batch);
   ISK_HIP(hipMemcpyAsync(w->xin, in, npix * 4, hipMemcpyHostToDevice, w->ctx->stream));
@@ -1014,6 +1411,7 @@ int iskra_ws_backward(iskra_ws *w, int batch, const float *in, const float *targ
   WRC(ws_host_input(w, batch, in));
   ISK_HIP(hipMemcpyAsync(w->tgt, target, ws_npix(w, 0, batch) * 4, hipMemcpyHostToDevice,
                          w->ctx->stream));
+  WRC(ws_pre(w, batch));
   WRC(ws_forward(w, batch));
   WRC(ws_loss(w, batch, true, loss));
   WRC(ws_backward(w, batch));
@@ -1087,6 +1485,42 @@ int iskra_ws_dataset_rms(iskra_ws *w, int64_t first, int64_t count, double *rms)
   return ISKRA_OK;
 }
 
+/* energy objective with k > 0 sweeps: the batch pair (x in A0, t in tgt)
+ * becomes (P x, P t), P = (I - om4 A)^k, the residual and correction the
+ * network meets after k pre-sweeps from zero (A^-1 commutes with P) */
+static int ws_pre(iskra_ws *w, int b) {
+  if (w->obj == ISKRA_WS_PHI_MSE || w->sweeps == 0) return ISKRA_OK;
+  hipStream_t st = w->ctx->stream;
+  const long long npix = ws_npix(w, 0, b);
+  const unsigned g = ws_grid(npix);
+  float *buf[2] = {w->eg_e, w->eg_ae};
+  const float om4 = (float)w->om4;
+  for (int pass = 0; pass < 2; ++pass) {
+    const float *src = w->tgt;
+    if (pass == 0) {
+      hipLaunchKernelGGL(k_ws_a0_get, dim3(g), dim3(EW_BLOCK), 0, st, (const uint16_t *)w->A0.a,
+                         w->eg_ae, npix);
+      ISK_CHECK_LAUNCH();
+      src = w->eg_ae;
+    }
+    int cur = 0;
+    for (int k = 0; k < w->sweeps; ++k) {
+      hipLaunchKernelGGL(k_ws_eg_poly, dim3(g), dim3(EW_BLOCK), 0, st, src, buf[cur], w->S, npix,
+                         om4);
+      ISK_CHECK_LAUNCH();
+      src = buf[cur];
+      cur ^= 1;
+    }
+    if (pass == 0) {
+      hipLaunchKernelGGL(k_ws_input, dim3(g), dim3(EW_BLOCK), 0, st, src, w->A0.a, npix);
+      ISK_CHECK_LAUNCH();
+    } else {
+      ISK_HIP(hipMemcpyAsync(w->tgt, src, npix * 4, hipMemcpyDeviceToDevice, st));
+    }
+  }
+  return ISKRA_OK;
+}
+
 static int ws_load_batch(iskra_ws *w, const long long *idx, int b) {
   ISK_HIP(hipMemcpyAsync(w->idx, idx, (size_t)b * 8, hipMemcpyHostToDevice, w->ctx->stream));
   const long long S2 = (long long)w->S * w->S;
@@ -1095,6 +1529,7 @@ static int ws_load_batch(iskra_ws *w, const long long *idx, int b) {
                      (const long long *)w->idx, w->A0.a, w->tgt, (int)S2, b,
                      (float)(1.0 / w->gain));
   ISK_CHECK_LAUNCH();
+  WRC(ws_pre(w, b));
   /* the index upload reads host memory of this call: finish before returning */
   ISK_HIP(hipStreamSynchronize(w->ctx->stream));
   return ISKRA_OK;
@@ -1156,6 +1591,182 @@ int iskra_ws_eval(iskra_ws *w, int64_t first, int64_t count, double *mse) {
   return ISKRA_OK;
 }
 
+/* z = the two-level cycle applied to the residual r: `sweeps` damped Jacobi
+ * sweeps from z = 0, the network's estimate of A^-1 of what they leave
+ * (r - A z, scaled by its RMS and the gain) added to z, then `sweeps`
+ * sweeps again.  om4 = 0 or sweeps = 0: the network alone. */
+static int ls_precond(iskra_ws *w, int nx, double om4, int sweeps, const double **z) {
+  iskra_ctx *c = w->ctx;
+  hipStream_t st = c->stream;
+  const long long n = (long long)(nx - 2) * (nx - 2);
+  const unsigned g = (unsigned)std::min((n + LS_BLOCK - 1) / LS_BLOCK, 1024ll);
+  const int k = om4 > 0.0 ? sweeps : 0;
+  /* pre-sweeps: ping-pong between lpre[0] and lpre[1] */
+  const double *zpre = nullptr;
+  for (int i = 0; i < k; ++i) {
+    hipLaunchKernelGGL(k_ls_smooth, dim3(g), dim3(LS_BLOCK), 0, st, (const double *)w->lr, zpre,
+                       w->lpre[i & 1], nx, om4);
+    ISK_CHECK_LAUNCH();
+    zpre = w->lpre[i & 1];
+  }
+  const double *rin = w->lr;
+  if (zpre) {
+    hipLaunchKernelGGL(k_ls_resid, dim3(g), dim3(LS_BLOCK), 0, st, (const double *)w->lr, zpre,
+                       w->lrp, nx);
+    ISK_CHECK_LAUNCH();
+    rin = w->lrp;
+  }
+  const long long cnt = (long long)(w->S - 1) * (w->S - 1);
+  const unsigned nb = std::min(ws_grid(cnt), (unsigned)w->dpart_n);
+  hipLaunchKernelGGL(k_ws_rhs_sumsq, dim3(nb), dim3(EW_BLOCK), 0, st, rin, nx, 1.0, w->dpart);
+  ISK_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_ws_rhs_scale, dim3(1), dim3(64), 0, st, (const double *)w->dpart, (int)nb,
+                     cnt, w->scal);
+  ISK_CHECK_LAUNCH();
+  const long long S2 = (long long)w->S * w->S;
+  hipLaunchKernelGGL(k_ws_rhs_store, dim3(ws_grid(S2)), dim3(EW_BLOCK), 0, st, rin,
+                     (const double *)nullptr, nx, 1.0, (const double *)w->scal,
+                     (uint16_t *)nullptr, w->A0.a, (float *)nullptr);
+  ISK_CHECK_LAUNCH();
+  WRC(ws_forward(w, 1));
+  hipLaunchKernelGGL(k_ws_phi_out, dim3(ws_grid(S2)), dim3(EW_BLOCK), 0, st,
+                     (const float *)w->out, w->lz, nx, w->gain, (const double *)w->scal);
+  ISK_CHECK_LAUNCH();
+  if (zpre) {
+    hipLaunchKernelGGL(k_ls_add, dim3(g), dim3(LS_BLOCK), 0, st, w->lz, zpre, nx);
+    ISK_CHECK_LAUNCH();
+  }
+  *z = w->lz;
+  double *buf[2] = {w->lzs, w->lz};
+  for (int i = 0; i < k; ++i) {
+    hipLaunchKernelGGL(k_ls_smooth, dim3(g), dim3(LS_BLOCK), 0, st, (const double *)w->lr, *z,
+                       buf[i & 1], nx, om4);
+    ISK_CHECK_LAUNCH();
+    *z = buf[i & 1];
+  }
+  return ISKRA_OK;
+}
+
+int iskra_ws_solve(iskra_ws *w, int warm, double omega, int sweeps, int max_iter, int *iters,
+                   double *relres) {
+  WRC(ws_check(w));
+  WRC(ws_check_grid(w));
+  iskra_ctx *c = w->ctx;
+  IskPoisson &p = c->ps;
+  if (p.kind < 0) {
+    isk_set_error("ws_solve: no Poisson solver configured");
+    return ISKRA_E_STATE;
+  }
+  if (warm != ISKRA_WARM_ZERO && warm != ISKRA_WARM_PREV) return ISKRA_E_ARG;
+  if (!(omega >= 0.0) || omega >= 2.0 || sweeps < 0 || sweeps > WS_MAX_SWEEPS) {
+    isk_set_error("ws_solve: the Jacobi weight must lie in [0, 2), sweeps in [0, %d]",
+                  WS_MAX_SWEEPS);
+    return ISKRA_E_ARG;
+  }
+  const int nx = c->g.nx;
+  /* the network's system: phi = 0 on the four walls, nothing else fixed */
+  bool ok = !p.per_x && !p.per_y && (int)p.tags.size() == nx * nx;
+  for (int j = 0; ok && j < nx; ++j)
+    for (int i = 0; i < nx; ++i) {
+      const int t = p.tags[(size_t)j * nx + i];
+      const bool wall = i == 0 || j == 0 || i == nx - 1 || j == nx - 1;
+      if (wall ? (t == 0 || p.tag_val[t] != 0.0) : t != 0) {
+        ok = false;
+        break;
+      }
+    }
+  if (!ok) {
+    isk_set_error("ws_solve: the network solves phi = 0 on all four walls with no other "
+                  "Dirichlet node or periodic link");
+    return ISKRA_E_UNSUPPORTED;
+  }
+  const long long nn = (long long)nx * nx;
+  if (!w->lx) {
+    double **bufs[] = {&w->lx,    &w->lr,    &w->lz,      &w->lzs,     &w->lp[0], &w->lp[1],
+                       &w->lq[0], &w->lq[1], &w->lpre[0], &w->lpre[1], &w->lrp};
+    for (double **b : bufs) {
+      WRC(isk_alloc((void **)b, nn * 8));
+      ISK_HIP(hipMemsetAsync(*b, 0, nn * 8, c->stream));
+    }
+    WRC(isk_alloc((void **)&w->lpart, 4096 * 8));
+    WRC(isk_alloc((void **)&w->ls, LS_NS * 8));
+    ISK_HIP(hipMemsetAsync(w->ls, 0, LS_NS * 8, c->stream));
+  }
+  hipEvent_t ev = nullptr;
+  WRC(isk_prof_begin(c, &ev));
+  const long long n = (long long)(nx - 2) * (nx - 2);
+  const int nb = (int)std::min((n + LS_BLOCK - 1) / LS_BLOCK, 1024ll);
+  const double fscale = c->g.dx * c->g.dx / p.eps0;
+  const double om4 = 0.25 * omega;
+  const int maxit = max_iter > 0 ? max_iter : p.max_iter;
+  hipLaunchKernelGGL(k_ls_init, dim3(nb), dim3(LS_BLOCK), 0, c->stream, (const double *)c->rho,
+                     warm == ISKRA_WARM_PREV ? (const double *)c->phi : (const double *)nullptr,
+                     nx, fscale, w->lx, w->lr, w->lpart);
+  ISK_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_ls_reduce, dim3(1), dim3(64), 0, c->stream, (const double *)w->lpart,
+                     nb, 2, w->ls + LS_FF);
+  ISK_CHECK_LAUNCH();
+  double s[2];
+  ISK_HIP(hipMemcpyAsync(s, w->ls, 16, hipMemcpyDeviceToHost, c->stream));
+  ISK_HIP(hipStreamSynchronize(c->stream));
+  double rr = s[0] > 0.0 ? sqrt(s[1] / s[0]) : 0.0;
+  int it = 0, cur = 0;
+  while (rr > p.rtol && it < maxit) {
+    ++it;
+    const double *z = nullptr;
+    WRC(ls_precond(w, nx, om4, sweeps, &z));
+    const int prev = cur ^ 1;
+    if (it > 1) {
+      hipLaunchKernelGGL(k_ls_dot, dim3(nb), dim3(LS_BLOCK), 0, c->stream, z,
+                         (const double *)w->lq[prev], nx, w->lpart);
+      ISK_CHECK_LAUNCH();
+      hipLaunchKernelGGL(k_ls_reduce, dim3(1), dim3(64), 0, c->stream,
+                         (const double *)w->lpart, nb, 1, w->ls + LS_ZQ);
+      ISK_CHECK_LAUNCH();
+    }
+    hipLaunchKernelGGL(k_ls_dir, dim3(nb), dim3(LS_BLOCK), 0, c->stream, z,
+                       (const double *)w->lp[prev], w->lp[cur], nx, (const double *)w->ls,
+                       it == 1 ? 1 : 0);
+    ISK_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_ls_Ap, dim3(nb), dim3(LS_BLOCK), 0, c->stream,
+                       (const double *)w->lp[cur], (const double *)w->lr, w->lq[cur], nx,
+                       w->lpart);
+    ISK_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_ls_reduce, dim3(1), dim3(64), 0, c->stream, (const double *)w->lpart,
+                       nb, 2, w->ls + LS_PQ);
+    ISK_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_ls_xr, dim3(nb), dim3(LS_BLOCK), 0, c->stream, w->lx, w->lr,
+                       (const double *)w->lp[cur], (const double *)w->lq[cur], nx,
+                       (const double *)w->ls, w->lpart);
+    ISK_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_ls_reduce, dim3(1), dim3(64), 0, c->stream, (const double *)w->lpart,
+                       nb, 1, w->ls + LS_RR);
+    ISK_CHECK_LAUNCH();
+    ISK_HIP(hipMemcpyAsync(s, w->ls, 16, hipMemcpyDeviceToHost, c->stream));
+    ISK_HIP(hipStreamSynchronize(c->stream));
+    rr = s[0] > 0.0 ? sqrt(s[1] / s[0]) : 0.0;
+    cur ^= 1;
+  }
+  hipLaunchKernelGGL(k_ls_out, dim3(nb), dim3(LS_BLOCK), 0, c->stream, (const double *)w->lx,
+                     c->phi, nx);
+  ISK_CHECK_LAUNCH();
+  WRC(isk_prof_end(c, ev, ISKRA_K_POISSON, 0));
+  ISK_HIP(hipStreamSynchronize(c->stream));
+  p.last_iters = it;
+  p.last_relres = rr;
+  c->stats.poisson_solves++;
+  c->stats.poisson_iters += it;
+  c->stats.last_iters = it;
+  c->stats.last_relres = rr;
+  if (iters) *iters = it;
+  if (relres) *relres = rr;
+  if (rr > p.rtol) {
+    isk_set_error("ws_solve: relative residual %.3e above %.3e after %d iterations", rr, p.rtol,
+                  it);
+    return ISKRA_E_CONVERGE;
+  }
+  return ISKRA_OK;
+}
 
 /* phi at the context's interior nodes <- network prediction from its rho */
 int iskra_ws_predict(iskra_ws *w) {

@@ -7,7 +7,9 @@ and measures what the trained network gains: Poisson iterations and time to
 relative residual rtol from a zero start, from the previous step's phi and
 from the network's prediction, with the same solver and the same stopping
 rule (the solver always iterates to rtol, so the warm start cannot change the
-answer beyond that tolerance).
+answer beyond that tolerance).  Trained with the energy objective, the same
+network also preconditions every iteration of a flexible CG
+(iskra_ws_solve), measured beside Jacobi-PCG to the same FP64 residual.
 
 Snapshot source: a 2-D two-stream plasma (the 1-D two-stream instability of
 the 64-cell configuration, set on a side x side grid with phi = 0 walls):
@@ -26,15 +28,20 @@ from ._lib import Context, WarmStart
 WS_IN, WS_OUT = 8, 16
 
 
+def _ch(base, l):
+    """Channels of level l: base * 2^l, held at base * 8 from level 3 on."""
+    return base << min(l, 3)
+
+
 def layer_shapes(levels, base):
     """(cin, cout) of each convolution, in parameter order."""
     out = []
     for l in range(levels):
-        out.append((WS_IN if l == 0 else base << (l - 1), base << l))
-        out.append((base << l, base << l))
+        out.append((WS_IN if l == 0 else _ch(base, l - 1), _ch(base, l)))
+        out.append((_ch(base, l), _ch(base, l)))
     for l in range(levels - 2, -1, -1):
-        out.append((base << (l + 1), base << l))
-        out.append((base << l, base << l))
+        out.append((_ch(base, l + 1), _ch(base, l)))
+        out.append((_ch(base, l), _ch(base, l)))
     out.append((base, WS_OUT))
     return out
 
@@ -146,11 +153,15 @@ def train(ws, first, count, steps, batch=8, lr=1e-3, lr_end=1e-5, chunk=50, seed
     return np.asarray(losses), gain
 
 
-def solve_table(ctx, ws, snapshots, kinds=(L.POISSON_PCG, L.POISSON_MG), rtol=1e-8):
+def solve_table(ctx, ws, snapshots, kinds=(L.POISSON_PCG, L.POISSON_MG), rtol=1e-8,
+                fcg_omega=None, fcg_sweeps=1):
     """For each solver kind (configured once, then one untimed solve), the
     iterations and wall ms of solving every snapshot (rho, phi_prev) from a
     zero start, from phi_prev and from the network's prediction (inference
-    time included).  Every solve must reach rtol.  Returns
+    time included).  With fcg_omega set, the PCG row also holds "fcg": the
+    network inside the iteration (iskra_ws_solve from a zero start,
+    fcg_sweeps Jacobi sweeps of weight fcg_omega after the network).  Every
+    solve must reach rtol.  Returns
     {kind: {start: {"iters": [...], "ms": [...], "relres": [...]}}}."""
     n = ws.side + 1
     tags = np.zeros((n, n), np.int8)
@@ -161,16 +172,22 @@ def solve_table(ctx, ws, snapshots, kinds=(L.POISSON_PCG, L.POISSON_MG), rtol=1e
         ctx.tag_value(1, 0.0)
         ctx.set_field(L.F_RHO, snapshots[0][0])
         ctx.solve(L.WARM_ZERO)
-        row = {s: {"iters": [], "ms": [], "relres": []} for s in ("zero", "prev", "learned")}
+        starts = ["zero", "prev", "learned"]
+        if fcg_omega is not None and kind == L.POISSON_PCG:
+            starts.append("fcg")
+            ws.solve(L.WARM_ZERO, fcg_omega, fcg_sweeps)  # untimed: first-call allocations
+        row = {s: {"iters": [], "ms": [], "relres": []} for s in starts}
         for rho, prev in snapshots:
             ctx.set_field(L.F_RHO, rho)
-            for start in ("zero", "prev", "learned"):
+            for start in starts:
                 if start == "prev":
                     ctx.set_field(L.F_PHI, prev)
                 ctx.sync()
                 t0 = time.perf_counter()
                 if start == "zero":
                     it, rr = ctx.solve(L.WARM_ZERO)
+                elif start == "fcg":
+                    it, rr = ws.solve(L.WARM_ZERO, fcg_omega, fcg_sweeps)
                 else:
                     if start == "learned":
                         ws.predict()
@@ -184,9 +201,14 @@ def solve_table(ctx, ws, snapshots, kinds=(L.POISSON_PCG, L.POISSON_MG), rtol=1e
 
 
 def experiment(side=512, levels=4, base=32, runs=8, steps_per_run=500, train_steps=1000,
-               batch=8, lr=1e-3, held_steps=41, every=10, seed=0, log=print):
+               batch=8, lr=1e-3, held_steps=41, every=10, seed=0, log=print,
+               objective=L.WS_ENERGY, omega=0.8, sweeps=1):
     """Capture, train, evaluate; returns a dict of the measured figures.
-    Everything runs on device 0 in one context, closed before returning."""
+    objective: L.WS_ENERGY trains the network as the preconditioner of
+    iskra_ws_solve (`sweeps` Jacobi sweeps of weight omega after it), and the
+    evaluation then includes that solve; L.WS_PHI_MSE trains it as a warm
+    start only.  Everything runs on device 0 in one context, closed before
+    returning."""
     t0 = time.perf_counter()
 
     def say(m):
@@ -199,6 +221,7 @@ def experiment(side=512, levels=4, base=32, runs=8, steps_per_run=500, train_ste
         ctx.grid(2, (side + 1, side + 1), (1.0, 1.0))
         ws = WarmStart(ctx, levels=levels, base=base, side=side, max_batch=batch)
         ws.set_params(he_init(levels, base, seed))
+        ws.set_objective(objective, omega if objective == L.WS_ENERGY else 0.0, sweeps)
         n = runs * steps_per_run
         ws.reserve(n + held_steps)
         t = time.perf_counter()
@@ -217,7 +240,7 @@ def experiment(side=512, levels=4, base=32, runs=8, steps_per_run=500, train_ste
         ctx.sync()
         tr_s = time.perf_counter() - t
         e1 = ws.eval(n, held_steps)
-        say(f"held-out relative MSE {e0:.4e} -> {e1:.4e}")
+        say(f"held-out loss {e0:.4e} -> {e1:.4e}")
         ctx.set_field(L.F_RHO, held[0][0])
         ctx.profile(True)
         for _ in range(10):
@@ -226,8 +249,12 @@ def experiment(side=512, levels=4, base=32, runs=8, steps_per_run=500, train_ste
         ctx.profile(False)
         inf_ms = st["kernel_ms"]["warmstart"] / max(st["kernel_calls"]["warmstart"], 1)
         names = {L.POISSON_PCG: "pcg", L.POISSON_MG: "mg"}
+        objname = {L.WS_ENERGY: f"energy (A-norm after {sweeps} Jacobi sweeps, omega {omega})",
+                   L.WS_PHI_MSE: "phi mean squared error"}[objective]
         snaps = [(rho, prev) for rho, prev, _phi in held[1::every]]
-        table = solve_table(ctx, ws, snaps)
+        table = solve_table(ctx, ws, snaps,
+                            fcg_omega=omega if objective == L.WS_ENERGY else None,
+                            fcg_sweeps=sweeps)
         summary = {names[k]: {s: {"iters": float(np.mean(v["iters"])),
                                   "ms": float(np.median(v["ms"])),
                                   "max_relres": float(np.max(v["relres"]))}
@@ -237,7 +264,7 @@ def experiment(side=512, levels=4, base=32, runs=8, steps_per_run=500, train_ste
                    capture_s=cap_s, train_steps=train_steps, batch=batch, train_s=tr_s,
                    train_ms_per_step=1e3 * tr_s / max(train_steps, 1),
                    train_tflops=3 * flops * batch / (tr_s / max(train_steps, 1)) / 1e12,
-                   heldout_mse_before=e0, heldout_mse_after=e1, gain=gain,
+                   objective=objname, heldout_loss_before=e0, heldout_loss_after=e1, gain=gain,
                    inference_ms=inf_ms, inference_tflops=flops / (inf_ms * 1e-3) / 1e12,
                    solves=summary, evaluated=len(snaps))
         say(json.dumps(summary))
@@ -271,10 +298,15 @@ def main(argv=None):
     ap.add_argument("--train-steps", type=int, default=4000)
     ap.add_argument("--batch", type=int, default=8)
     ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--objective", choices=("energy", "mse"), default="energy")
+    ap.add_argument("--omega", type=float, default=0.8)
+    ap.add_argument("--sweeps", type=int, default=1)
     ap.add_argument("--out", default=None)
     a = ap.parse_args(argv)
     res = experiment(a.side, a.levels, a.base, a.runs, a.steps_per_run, a.train_steps, a.batch,
-                     a.lr, log=lambda m: print(m, flush=True))
+                     a.lr, log=lambda m: print(m, flush=True),
+                     objective=L.WS_ENERGY if a.objective == "energy" else L.WS_PHI_MSE,
+                     omega=a.omega, sweeps=a.sweeps)
     s = json.dumps(res)
     print(s, flush=True)
     if a.out:

@@ -195,6 +195,8 @@ def _declare(L):
         "orc_fix32": (I64, [F]),
         "orc_fixprod32": (I64, [F, F]),
         "orc_ws_run": (D, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, P, P, P, P, P]),
+        "orc_ws_run_obj": (D, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, D, D,
+                               C.c_int, P, P, P, P, P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -1011,8 +1013,12 @@ def ws_param_count(L, base):
     return int(lib().orc_ws_param_count(L, base))
 
 
-def ws_run(L, base, params, inputs, target=None, round_bf16=True, grads=True):
-    """inputs: (B, S, S); returns (out (B,S,S), loss, grads or None)."""
+def ws_run(L, base, params, inputs, target=None, round_bf16=True, grads=True, objective=0,
+           gain=1.0, omega=0.0, sweeps=1):
+    """inputs: (B, S, S); returns (out (B,S,S), loss, grads or None).
+    objective 0: mean squared error against target; 1: the energy objective
+    (network times gain, then `sweeps` damped Jacobi sweeps of weight omega,
+    scored in the A-norm relative to gain * target)."""
     x = np.ascontiguousarray(inputs, dtype=np.float64)
     B, S = x.shape[0], x.shape[1]
     p = np.ascontiguousarray(params, dtype=np.float64)
@@ -1020,6 +1026,7 @@ def ws_run(L, base, params, inputs, target=None, round_bf16=True, grads=True):
     out = np.zeros_like(x)
     t = None if target is None else np.ascontiguousarray(target, dtype=np.float64)
     g = np.zeros_like(p) if (grads and t is not None) else None
-    loss = lib().orc_ws_run(L, base, S, B, int(round_bf16), ptr(p), ptr(x), ptr(t), ptr(out),
-                            ptr(g))
+    loss = lib().orc_ws_run_obj(L, base, S, B, int(round_bf16), int(objective), float(gain),
+                                0.25 * float(omega), int(sweeps), ptr(p), ptr(x), ptr(t),
+                                ptr(out), ptr(g))
     return out, loss, g

@@ -22,7 +22,10 @@
 
 #define WSO_IN 8
 #define WSO_OUT 16
-#define WSO_MAXL 6
+#define WSO_MAXL 8
+
+/* channels of level l: base * 2^l, held at base * 8 from level 3 on */
+static int wso_ch(int base, int l) { return base << (l < 3 ? l : 3); }
 
 static double wso_bf(double x) {
   float f = (float)x;
@@ -159,12 +162,12 @@ static void wso_mask(const WsoNet *net, size_t cnt, const double *G, const doubl
 static int64_t wso_nparam(int L, int base) {
   int64_t n = 0;
   for (int l = 0; l < L; ++l) {
-    const int c = base << l, cin = l == 0 ? WSO_IN : (base << (l - 1));
+    const int c = wso_ch(base, l), cin = l == 0 ? WSO_IN : (wso_ch(base, l - 1));
     n += 9ll * c * cin + 9ll * c * c;
   }
   for (int l = L - 2; l >= 0; --l) {
-    const int c = base << l;
-    n += 9ll * c * (base << (l + 1)) + 9ll * c * c;
+    const int c = wso_ch(base, l);
+    n += 9ll * c * (wso_ch(base, l + 1)) + 9ll * c * c;
   }
   n += 9ll * WSO_OUT * base;
   return n;
@@ -172,12 +175,82 @@ static int64_t wso_nparam(int L, int base) {
 
 int64_t orc_ws_param_count(int L, int base) { return wso_nparam(L, base); }
 
+/* sum of the 4 neighbours of interior pixel (h, x) of plane v, neighbours
+ * outside the interior (h, x < S-1) counting 0 */
+static double wso_nb(const double *v, int S, int h, int x) {
+  double s = 0.0;
+  if (h > 0) s += v[(size_t)(h - 1) * S + x];
+  if (h < S - 2) s += v[(size_t)(h + 1) * S + x];
+  if (x > 0) s += v[(size_t)h * S + x - 1];
+  if (x < S - 2) s += v[(size_t)h * S + x + 1];
+  return s;
+}
+
+/* v <- A v restricted to the interior of each of B planes (w: scratch) */
+static void wso_apply_A(int S, int B, const double *v, double *out) {
+  const size_t S2 = (size_t)S * S;
+  for (int b = 0; b < B; ++b)
+    for (int h = 0; h < S; ++h)
+      for (int w = 0; w < S; ++w) {
+        const size_t i = b * S2 + (size_t)h * S + w;
+        out[i] = (h < S - 1 && w < S - 1) ? 4.0 * v[i] - wso_nb(v + b * S2, S, h, w) : 0.0;
+      }
+}
+
+/* Energy objective of one batch: y_0 = gain o, `sweeps` damped Jacobi sweeps
+ * y_{k+1} = y_k + om4 (x - A y_k), e = y_sweeps - gain t,
+ * loss = sum e.Ae / sum (gain t).A(gain t), and the output gradient
+ * G = 2 gain (I - om4 A)^sweeps A e / denominator on interior pixels (A v =
+ * 4 v - neighbours, 0 outside the interior).  x = the network input as it
+ * enters the first convolution. */
+static double wso_energy(int S, int B, double gain, double om4, int sweeps, const double *x,
+                         const double *o, const double *t, double *G) {
+  const size_t S2 = (size_t)S * S, n = B * S2;
+  double *y = (double *)calloc(n, sizeof(double));
+  double *ay = (double *)calloc(n, sizeof(double));
+  double *gt = (double *)calloc(n, sizeof(double));
+  double num = 0.0, den = 0.0;
+  for (int b = 0; b < B; ++b)
+    for (int h = 0; h < S - 1; ++h)
+      for (int w = 0; w < S - 1; ++w) {
+        const size_t i = b * S2 + (size_t)h * S + w;
+        y[i] = gain * o[i];
+        gt[i] = gain * t[i];
+      }
+  for (int k = 0; k < sweeps; ++k) {
+    wso_apply_A(S, B, y, ay);
+    for (size_t i = 0; i < n; ++i)
+      if (((i % S2) / S) < (size_t)S - 1 && (i % S) < (size_t)S - 1) y[i] += om4 * (x[i] - ay[i]);
+  }
+  for (size_t i = 0; i < n; ++i) y[i] -= gt[i]; /* e (0 off the interior) */
+  wso_apply_A(S, B, gt, ay);
+  for (size_t i = 0; i < n; ++i) den += gt[i] * ay[i];
+  wso_apply_A(S, B, y, ay);
+  for (size_t i = 0; i < n; ++i) num += y[i] * ay[i];
+  if (!(den > 0.0)) den = 1.0;
+  if (G) {
+    /* h = 2 gain A e / den, then h <- h - om4 A h, sweeps times */
+    for (size_t i = 0; i < n; ++i) G[i] = 2.0 * gain / den * ay[i];
+    for (int k = 0; k < sweeps; ++k) {
+      wso_apply_A(S, B, G, gt);
+      for (size_t i = 0; i < n; ++i) G[i] -= om4 * gt[i];
+    }
+  }
+  free(y);
+  free(ay);
+  free(gt);
+  return num / den;
+}
+
 /* in: B x S x S normalised inputs; target (nullable): B x S x S; out: B x S x S
- * network output o; grads (nullable): parameter gradients of the mean squared
- * error over interior pixels (h, w < S-1).  Returns the loss (0 without
- * target). */
-double orc_ws_run(int L, int base, int S, int B, int round, const double *params,
-                  const double *in, const double *target, double *out, double *grads) {
+ * network output o; grads (nullable): parameter gradients of the loss over
+ * interior pixels (h, w < S-1): obj 0 the mean squared error of o against
+ * the target, obj 1 the energy objective (wso_energy) with gain, om4 and
+ * sweeps.
+ * Returns the loss (0 without target). */
+double orc_ws_run_obj(int L, int base, int S, int B, int round, int obj, double gain, double om4,
+                      int sweeps, const double *params, const double *in, const double *target,
+                      double *out, double *grads) {
   WsoNet net = {L, base, S, B, round};
   const int64_t np = wso_nparam(L, base);
   double *w = (double *)malloc(np * sizeof(double));
@@ -200,12 +273,12 @@ double orc_ws_run(int L, int base, int S, int B, int round, const double *params
     ++nl;                                                                    \
   } while (0)
   for (int l = 0; l < L; ++l) {
-    ADD(l == 0 ? WSO_IN : base << (l - 1), base << l, l, 1);
-    ADD(base << l, base << l, l, 1);
+    ADD(l == 0 ? WSO_IN : wso_ch(base, l - 1), wso_ch(base, l), l, 1);
+    ADD(wso_ch(base, l), wso_ch(base, l), l, 1);
   }
   for (int l = L - 2; l >= 0; --l) {
-    ADD(base << (l + 1), base << l, l, 1);
-    ADD(base << l, base << l, l, 1);
+    ADD(wso_ch(base, l + 1), wso_ch(base, l), l, 1);
+    ADD(wso_ch(base, l), wso_ch(base, l), l, 1);
   }
   ADD(base, WSO_OUT, 0, 0);
 #undef ADD
@@ -225,21 +298,21 @@ double orc_ws_run(int L, int base, int S, int B, int round, const double *params
       A0[(b * S2 + p) * WSO_IN] = round ? wso_bf(v) : v;
     }
   for (int l = 0; l < L; ++l) {
-    const int c = base << l;
+    const int c = wso_ch(base, l);
     Ea[l] = wso_alloc(&net, l, c);
     E[l] = wso_alloc(&net, l, c);
     gEa[l] = wso_alloc(&net, l, c);
     gE[l] = wso_alloc(&net, l, c);
     if (l > 0) {
-      P[l] = wso_alloc(&net, l, base << (l - 1));
-      gP[l] = wso_alloc(&net, l, base << (l - 1));
-      wso_pool(&net, l - 1, base << (l - 1), E[l - 1], P[l]);
+      P[l] = wso_alloc(&net, l, wso_ch(base, l - 1));
+      gP[l] = wso_alloc(&net, l, wso_ch(base, l - 1));
+      wso_pool(&net, l - 1, wso_ch(base, l - 1), E[l - 1], P[l]);
     }
     wso_conv_fwd(&net, &cv[2 * l], l == 0 ? A0 : P[l], NULL, Ea[l]);
     wso_conv_fwd(&net, &cv[2 * l + 1], Ea[l], NULL, E[l]);
   }
   for (int l = L - 2; l >= 0; --l) {
-    const int c = base << l, cu = base << (l + 1);
+    const int c = wso_ch(base, l), cu = wso_ch(base, l + 1);
     const double *src = (l + 1 == L - 1) ? E[l + 1] : D[l + 1];
     U[l] = wso_alloc(&net, l, cu);
     gU[l] = wso_alloc(&net, l, cu);
@@ -265,18 +338,28 @@ double orc_ws_run(int L, int base, int S, int B, int round, const double *params
   if (target) {
     const double cnt = (double)B * (S - 1) * (S - 1);
     double *G = wso_alloc(&net, 0, WSO_OUT);
-    for (int b = 0; b < B; ++b)
-      for (int h = 0; h < S; ++h)
-        for (int x = 0; x < S; ++x) {
-          const size_t i = (size_t)b * S2 + (size_t)h * S + x;
-          double d = 0.0;
-          if (h < S - 1 && x < S - 1) d = out[i] - target[i];
-          loss += d * d;
-          double g = 2.0 * d / cnt;
-          if (round) g = wso_bf((double)(float)(2.0f * (float)d * (float)(1.0 / cnt)));
-          G[i * WSO_OUT] = g;
-        }
-    loss /= cnt;
+    if (obj == 1) {
+      double *x0 = (double *)malloc(B * S2 * sizeof(double));
+      double *g0 = (double *)calloc(B * S2, sizeof(double));
+      for (size_t i = 0; i < B * S2; ++i) x0[i] = A0[i * WSO_IN];
+      loss = wso_energy(S, B, gain, om4, sweeps, x0, out, target, g0);
+      for (size_t i = 0; i < B * S2; ++i) G[i * WSO_OUT] = round ? wso_bf(g0[i]) : g0[i];
+      free(x0);
+      free(g0);
+    } else {
+      for (int b = 0; b < B; ++b)
+        for (int h = 0; h < S; ++h)
+          for (int x = 0; x < S; ++x) {
+            const size_t i = (size_t)b * S2 + (size_t)h * S + x;
+            double d = 0.0;
+            if (h < S - 1 && x < S - 1) d = out[i] - target[i];
+            loss += d * d;
+            double g = 2.0 * d / cnt;
+            if (round) g = wso_bf((double)(float)(2.0f * (float)d * (float)(1.0 / cnt)));
+            G[i * WSO_OUT] = g;
+          }
+      loss /= cnt;
+    }
     if (gw) {
       /* backward */
       wso_conv_bwd(&net, &cv[nl - 1], last, G, gl);
@@ -291,7 +374,7 @@ double orc_ws_run(int L, int base, int S, int B, int round, const double *params
     }                                                               \
   } while (0)
       for (int l = 0; l <= L - 2; ++l) {
-        const int c = base << l, cu = base << (l + 1);
+        const int c = wso_ch(base, l), cu = wso_ch(base, l + 1);
         const int li = 2 * L + 2 * (L - 2 - l);
         const size_t n1 = wso_count(&net, l, c);
         GM(n1);
@@ -303,14 +386,14 @@ double orc_ws_run(int L, int base, int S, int B, int round, const double *params
         wso_up_bwd(&net, l, cu, gU[l], dst);
       }
       for (int l = L - 1; l >= 0; --l) {
-        const int c = base << l;
+        const int c = wso_ch(base, l);
         const size_t n1 = wso_count(&net, l, c);
         GM(n1);
         wso_mask(&net, n1, gE[l], E[l], gm, NULL);
         wso_conv_bwd(&net, &cv[2 * l + 1], Ea[l], gm, gEa[l]);
         wso_mask(&net, n1, gEa[l], Ea[l], gm, NULL);
         wso_conv_bwd(&net, &cv[2 * l], l == 0 ? A0 : P[l], gm, l > 0 ? gP[l] : NULL);
-        if (l > 0) wso_pool_bwd(&net, l - 1, base << (l - 1), gP[l], gE[l - 1]);
+        if (l > 0) wso_pool_bwd(&net, l - 1, wso_ch(base, l - 1), gP[l], gE[l - 1]);
       }
 #undef GM
       free(gm);
@@ -325,4 +408,9 @@ double orc_ws_run(int L, int base, int S, int B, int round, const double *params
   }
   free(w);
   return loss;
+}
+
+double orc_ws_run(int L, int base, int S, int B, int round, const double *params,
+                  const double *in, const double *target, double *out, double *grads) {
+  return orc_ws_run_obj(L, base, S, B, round, 0, 1.0, 0.0, 1, params, in, target, out, grads);
 }

@@ -69,6 +69,150 @@ def test_ws_gradients_match_oracle(gpu_ready):
     ctx.close()
 
 
+def _bf16(x):
+    """Round float32 values to bfloat16 (nearest even), returned as float64."""
+    u = np.ascontiguousarray(x, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def _presmooth(v, omega, k):
+    """(I - omega/4 A)^k on the interior (S-1)^2 pixels of each plane, 0
+    outside: the pair transform of the two-level objective."""
+    v = np.array(v, dtype=np.float64)
+    S = v.shape[-1]
+    v[:, S - 1, :] = 0.0
+    v[:, :, S - 1] = 0.0
+    for _ in range(k):
+        a = 4 * v
+        a[:, 1:, :] -= v[:, :-1, :]
+        a[:, :-1, :] -= v[:, 1:, :]
+        a[:, :, 1:] -= v[:, :, :-1]
+        a[:, :, :-1] -= v[:, :, 1:]
+        v = v - omega / 4 * a
+        v[:, S - 1, :] = 0.0
+        v[:, :, S - 1] = 0.0
+    return v
+
+
+def test_ws_energy_gradients_match_oracle(gpu_ready):
+    """The two-level objective's loss and parameter gradients (pair mapped by
+    two pre-sweeps, network times gain, two post-sweeps, A-norm) against the
+    oracle fed the same pre-smoothed pair."""
+    import oracle as O
+    from iskra_amd import Context, _lib as Lb
+    from iskra_amd import warmstart as W
+    ctx = Context(0)
+    ws = _net(ctx)
+    try:
+        p = _he_params(L_, BASE_, 4)
+        ws.set_params(p)
+        ws.set_gain(1.3)
+        ws.set_objective(Lb.WS_ENERGY, 0.8, 2)
+        rng = np.random.default_rng(13)
+        x = rng.standard_normal((2, S_, S_)).astype(np.float32)
+        t = (0.05 * rng.standard_normal((2, S_, S_))).astype(np.float32)
+        loss, g = ws.backward(x, t)
+        xp = _presmooth(_bf16(x), 0.8, 2)
+        tp = _presmooth(t, 0.8, 2)
+        _, lref, gref = O.ws_run(L_, BASE_, p.astype(np.float64), xp, tp, round_bf16=True,
+                                 objective=1, gain=1.3, omega=0.8, sweeps=2)
+        print("energy loss", loss, lref)
+        assert loss == pytest.approx(lref, rel=1e-3)
+        for k, (off, n, cin, cout) in enumerate(W.layer_slices(L_, BASE_)):
+            e = _relerr(g[off:off + n], gref[off:off + n])
+            print("layer", k, cin, cout, "grad relerr", e)
+            assert e < 3e-2, (k, e)
+    finally:
+        ws.close()
+        ctx.close()
+
+
+def _dst_solve(f):
+    """Exact solution of 4 x - (4 neighbours) = f on the interior nodes, 0 on
+    the walls (DST-I diagonalises the operator)."""
+    from scipy.fft import dstn, idstn
+    m = f.shape[0]
+    k = np.arange(1, m + 1)
+    lam1 = 2 - 2 * np.cos(np.pi * k / (m + 1))
+    return idstn(dstn(f, type=1) / (lam1[:, None] + lam1[None, :]), type=1)
+
+
+def test_ws_in_iteration_cuts_pcg_iterations_3x(gpu_ready):
+    """The network inside the iteration on a 65 x 65-node grid: trained on
+    300 two-stream snapshots with the two-level energy objective (two Jacobi
+    sweeps before and after the network), the cycle preconditions a
+    flexible CG (iskra_ws_solve) that reaches the same FP64 residual 1e-8 as
+    Jacobi-PCG in at least 3x fewer iterations on held-out snapshots of
+    another run; each solution is checked on the host (residual recomputed,
+    error against the exact DST-I solution)."""
+    from iskra_amd import Context, _lib as Lb
+    from iskra_amd import warmstart as W
+    ctx = Context(0)
+    ctx.grid(2, (S_ + 1, S_ + 1), (1.0, 1.0))
+    ws = Lb.WarmStart(ctx, levels=L_, base=BASE_, side=S_, max_batch=8)
+    try:
+        ws.set_params(W.he_init(L_, BASE_, 3))
+        ws.set_objective(Lb.WS_ENERGY, 0.8, 2)
+        ws.reserve(400)
+        W.capture_two_stream(ws, runs=2, steps=150, seed=1)
+        held = []
+        W.capture_two_stream(ws, runs=1, steps=40, seed=2, record=held)
+        ws.set_gain(ws.dataset_rms(0, 300))
+        before = ws.eval(300, 40)
+        losses, _ = W.train(ws, 0, 300, steps=1500, batch=8, lr=2e-3)
+        after = ws.eval(300, 40)
+        print("held-out energy loss", before, "->", after, flush=True)
+        assert np.all(np.isfinite(losses)) and after < 0.2 * before
+        snaps = [(rho, prev) for rho, prev, _phi in held[::8]]
+        table = W.solve_table(ctx, ws, snaps, kinds=(Lb.POISSON_PCG,),
+                              fcg_omega=0.8, fcg_sweeps=2)[Lb.POISSON_PCG]
+        print({k: v["iters"] for k, v in table.items()}, flush=True)
+        for v in table.values():
+            assert max(v["relres"]) <= 1e-8
+        plain, fcg = np.mean(table["zero"]["iters"]), np.mean(table["fcg"]["iters"])
+        assert plain >= 3.0 * fcg, (plain, fcg)
+        n = S_ + 1
+        for rho, _prev in snaps:
+            ctx.set_field(Lb.F_RHO, rho)
+            it, rr = ws.solve(Lb.WARM_ZERO, 0.8, 2)
+            phi = ctx.field(Lb.F_PHI).reshape(n, n)
+            f = rho.reshape(n, n)[1:-1, 1:-1]
+            x = phi[1:-1, 1:-1]
+            assert np.all(phi[0] == 0) and np.all(phi[-1] == 0)
+            assert np.all(phi[:, 0] == 0) and np.all(phi[:, -1] == 0)
+            r = f - (4 * x - phi[:-2, 1:-1] - phi[2:, 1:-1] - phi[1:-1, :-2] - phi[1:-1, 2:])
+            assert np.linalg.norm(r) / np.linalg.norm(f) <= 1.01e-8, (it, rr)
+            ex = _dst_solve(f)
+            assert np.linalg.norm(x - ex) / np.linalg.norm(ex) <= 1e-6
+    finally:
+        ws.close()
+        ctx.close()
+
+
+def test_ws_solve_refuses_other_systems(gpu_ready):
+    """iskra_ws_solve solves only the network's system (phi = 0 on the four
+    walls): an extra Dirichlet node is refused with ISKRA_E_UNSUPPORTED."""
+    from iskra_amd import Context, _lib as Lb
+    ctx = Context(0)
+    ctx.grid(2, (S_ + 1, S_ + 1), (1.0, 1.0))
+    ws = Lb.WarmStart(ctx, levels=L_, base=BASE_, side=S_, max_batch=1)
+    try:
+        n = S_ + 1
+        tags = np.zeros((n, n), np.int8)
+        tags[0, :] = tags[-1, :] = tags[:, 0] = tags[:, -1] = 1
+        tags[n // 2, n // 2] = 2
+        ctx.poisson(Lb.POISSON_PCG, 1.0, tags.ravel(), rtol=1e-8)
+        ctx.tag_value(1, 0.0)
+        ctx.tag_value(2, 1.0)
+        with pytest.raises(Lb.IskraError) as ei:
+            ws.solve(Lb.WARM_ZERO, 0.8)
+        assert ei.value.code == Lb.E_UNSUPPORTED
+    finally:
+        ws.close()
+        ctx.close()
+
+
 def test_ws_training_cuts_pcg_iterations(gpu_ready):
     """Train on snapshots of a 64 x 64 two-stream run, then solve held-out
     snapshots of another run: the learned start must need fewer PCG
