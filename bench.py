@@ -11,10 +11,16 @@ solve, multigrid and PCG timed beside it on the same charge density), E-field,
 and the counting sort every `sort_every` steps.  Inputs are generated on the
 device (Philox loader), so they are resident in HBM before the timed region.
 
-After the timed step, a reduced learned-warm-start leg (513^2 nodes: capture
-two-stream snapshots, train the encoder-decoder on the device, compare PCG
-and multigrid iterations from zero, previous and learned starts) reports the
-"learned vs plain" part of the metric; --no-warmstart skips it.
+After the timed step, a reduced learned leg (513^2 nodes, the 4-level
+32-256-channel network, 1000 two-stream snapshots instead of 20,000 and
+1000 + 3 x 1000 training steps instead of 4000 + 3 x 1000; the full size is
+`python -m iskra_amd.warmstart`) trains the network on the device as the
+coarse solver of the two-level cycle that preconditions a flexible CG
+(iskra_ws_solve), then compares, on held-out snapshots, Jacobi-PCG from
+zero, from the previous phi and from the network's prediction, multigrid
+likewise, and the network-preconditioned solve, all to the same FP64
+relative residual 1e-8: the "learned vs plain" part of the metric;
+--no-warmstart skips it.
 
 Prints one JSON line (see the keys below).  --gpus above 1 exits: this build
 runs on one GPU.
@@ -168,7 +174,7 @@ def main():
     ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--no-warmstart", action="store_true")
     ap.add_argument("--ws-runs", type=int, default=4)
-    ap.add_argument("--ws-train-steps", type=int, default=800)
+    ap.add_argument("--ws-train-steps", type=int, default=1000)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -322,16 +328,23 @@ def main():
         from iskra_amd import warmstart as W
         ws = W.experiment(side=512, levels=4, base=32, runs=args.ws_runs, steps_per_run=250,
                           train_steps=args.ws_train_steps, held_steps=41, every=10, log=None)
+        pcg = ws["solves"]["pcg"]
         out["learned_warm_start"] = {
             "grid": "513x513 nodes, phi = 0 walls, 2-D two-stream snapshots",
+            "reduced": f"{ws['snapshots']} of 20,000 snapshots, {ws['train_steps']} training "
+                       "steps (full size: python -m iskra_amd.warmstart)",
             "network": f"{ws['levels']}-level conv encoder-decoder, {ws['base']}-"
-                       f"{ws['base'] << (ws['levels'] - 1)} channels, {ws['params']} parameters",
+                       f"{min(ws['base'] << (ws['levels'] - 1), ws['base'] * 8)} channels, "
+                       f"{ws['params']} parameters",
+            "objective": ws["objective"], "rounds": ws["rounds"],
+            "iterate_pairs": ws["iterate_pairs"],
             "snapshots": ws["snapshots"], "train_steps": ws["train_steps"],
-            "train_ms_per_step": ws["train_ms_per_step"], "train_tflops": ws["train_tflops"],
-            "inference_ms": ws["inference_ms"], "heldout_rel_mse": ws["heldout_mse_after"],
+            "train_ms_per_step": ws["train_ms_per_step"],
+            "inference_ms": ws["inference_ms"], "heldout_loss": ws["heldout_loss_after"],
             "solves": ws["solves"],
-            "pcg_iters_plain_over_learned": ws["solves"]["pcg"]["zero"]["iters"] /
-            ws["solves"]["pcg"]["learned"]["iters"],
+            "pcg_iters_plain_over_learned": pcg["zero"]["iters"] / pcg["fcg"]["iters"],
+            "pcg_ms_plain_over_learned": pcg["zero"]["ms"] / pcg["fcg"]["ms"],
+            "pcg_iters_plain_over_warm_start": pcg["zero"]["iters"] / pcg["learned"]["iters"],
         }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cells)

@@ -520,6 +520,18 @@ int iskra_ws_predict(iskra_ws *net);
  * (E is not recomputed). */
 int iskra_ws_solve(iskra_ws *net, int warm, double omega, int sweeps,
                    int max_iter, int *iters, double *relres);
+/* Training pairs from the solve's own iterates (the residuals the
+ * preconditioner meets inside the iteration): the context's phi must hold
+ * the solution for its rho (call after an exact solve); flexible CG as in
+ * iskra_ws_solve runs from phi = 0, and after iters[k] iterations (n_iters
+ * increasing counts) the pair (r / rms(r), (phi - x) / rms(r)) of iterate x
+ * and residual r = f - A x is appended to the dataset; stops early when r
+ * vanishes.  *captured (nullable) = pairs appended.  phi is left as it was. */
+int iskra_ws_capture_iterates(iskra_ws *net, double omega, int sweeps,
+                              const int *iters, int n_iters, int64_t *captured);
+/* sample index of the dataset: in = the stored (bf16) input plane, out =
+ * the target plane, side x side floats each */
+int iskra_ws_dataset_get(iskra_ws *net, int64_t index, float *in, float *out);
 /* ---- diagnostics ---------------------------------------------------------- */
 /* Device-to-device copy of `bytes` bytes, `iters` times, with the library's
  * own streaming copy kernel (nontemporal != 0: non-temporal loads/stores,

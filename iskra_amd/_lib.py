@@ -47,7 +47,8 @@ EXPORTS = [
     "iskra_ws_get_params", "iskra_ws_set_gain", "iskra_ws_forward", "iskra_ws_backward",
     "iskra_ws_dataset_reserve", "iskra_ws_capture", "iskra_ws_dataset_size",
     "iskra_ws_dataset_rms", "iskra_ws_train", "iskra_ws_eval", "iskra_ws_predict",
-    "iskra_ws_set_objective", "iskra_ws_solve",
+    "iskra_ws_set_objective", "iskra_ws_solve", "iskra_ws_capture_iterates",
+    "iskra_ws_dataset_get",
     "iskra_batch_create", "iskra_batch_destroy", "iskra_batch_set_graph", "iskra_batch_step",
     "iskra_batch_graphs_launched",
 ]
@@ -158,6 +159,8 @@ def lib():
             "iskra_ws_predict": (I, [P]),
             "iskra_ws_set_objective": (I, [P, I, D, I]),
             "iskra_ws_solve": (I, [P, I, D, I, I, C.POINTER(I), C.POINTER(D)]),
+            "iskra_ws_capture_iterates": (I, [P, D, I, P, I, C.POINTER(I64)]),
+            "iskra_ws_dataset_get": (I, [P, I64, P, P]),
             "iskra_batch_create": (I, [P, I, C.POINTER(P)]),
             "iskra_batch_destroy": (I, [P]),
             "iskra_batch_set_graph": (I, [P, I]),
@@ -644,6 +647,23 @@ class WarmStart:
         check(lib().iskra_ws_solve(self.h, int(warm), float(omega), int(sweeps), int(max_iter),
                                    C.byref(it), C.byref(rr)))
         return it.value, rr.value
+
+    def capture_iterates(self, iters, omega=0.8, sweeps=2):
+        """Append the pairs (r, phi - x) of the solve's iterates after each
+        count in `iters` (increasing); the context's phi must be the exact
+        solution of its rho.  Returns the number appended."""
+        a = np.ascontiguousarray(iters, dtype=np.int32)
+        n = C.c_int64()
+        check(lib().iskra_ws_capture_iterates(self.h, float(omega), int(sweeps), _p(a), a.size,
+                                              C.byref(n)))
+        return n.value
+
+    def dataset_get(self, index):
+        """(input plane, target plane) of dataset sample `index`."""
+        a = np.zeros((self.side, self.side), np.float32)
+        b = np.zeros((self.side, self.side), np.float32)
+        check(lib().iskra_ws_dataset_get(self.h, int(index), _p(a), _p(b)))
+        return a, b
 
 
 class Batch:

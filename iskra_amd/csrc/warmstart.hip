@@ -806,6 +806,22 @@ __global__ __launch_bounds__(LS_BLOCK) void k_ls_out(const double *x, double *ph
   }
 }
 
+/* a training pair from an FCG iterate: input r / s, target (phi - x) / s at
+ * the interior nodes (s = scal[0], the RMS of r), plane layout of capture */
+__global__ void k_ws_pair_store(const double *r, const double *phi, const double *x, int nx,
+                                const double *scal, uint16_t *in_plane, float *out_plane) {
+  const int S = nx - 1;
+  const double inv = scal[1];
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < (long long)S * S;
+       i += (long long)gridDim.x * blockDim.x) {
+    const int h = (int)(i / S), w = (int)(i - (long long)h * S);
+    const bool in = h < S - 1 && w < S - 1;
+    const long long node = (long long)(h + 1) * nx + (w + 1);
+    in_plane[i] = ws_f2bf(in ? (float)(r[node] * inv) : 0.f);
+    out_plane[i] = in ? (float)((phi[node] - x[node]) * inv) : 0.f;
+  }
+}
+
 /* ------------------------------------------------------------------------ */
 /* host side                                                                  */
 
@@ -1647,24 +1663,23 @@ static int ls_precond(iskra_ws *w, int nx, double om4, int sweeps, const double 
   return ISKRA_OK;
 }
 
-int iskra_ws_solve(iskra_ws *w, int warm, double omega, int sweeps, int max_iter, int *iters,
-                   double *relres) {
-  WRC(ws_check(w));
+/* the context must hold the network's system: a configured solver with
+ * phi = 0 on the four walls and no other Dirichlet node or periodic link;
+ * allocates the iteration vectors on first use */
+static int ls_setup(iskra_ws *w, const char *who, double omega, int sweeps) {
   WRC(ws_check_grid(w));
   iskra_ctx *c = w->ctx;
   IskPoisson &p = c->ps;
   if (p.kind < 0) {
-    isk_set_error("ws_solve: no Poisson solver configured");
+    isk_set_error("%s: no Poisson solver configured", who);
     return ISKRA_E_STATE;
   }
-  if (warm != ISKRA_WARM_ZERO && warm != ISKRA_WARM_PREV) return ISKRA_E_ARG;
   if (!(omega >= 0.0) || omega >= 2.0 || sweeps < 0 || sweeps > WS_MAX_SWEEPS) {
-    isk_set_error("ws_solve: the Jacobi weight must lie in [0, 2), sweeps in [0, %d]",
+    isk_set_error("%s: the Jacobi weight must lie in [0, 2), sweeps in [0, %d]", who,
                   WS_MAX_SWEEPS);
     return ISKRA_E_ARG;
   }
   const int nx = c->g.nx;
-  /* the network's system: phi = 0 on the four walls, nothing else fixed */
   bool ok = !p.per_x && !p.per_y && (int)p.tags.size() == nx * nx;
   for (int j = 0; ok && j < nx; ++j)
     for (int i = 0; i < nx; ++i) {
@@ -1676,8 +1691,8 @@ int iskra_ws_solve(iskra_ws *w, int warm, double omega, int sweeps, int max_iter
       }
     }
   if (!ok) {
-    isk_set_error("ws_solve: the network solves phi = 0 on all four walls with no other "
-                  "Dirichlet node or periodic link");
+    isk_set_error("%s: the network solves phi = 0 on all four walls with no other Dirichlet "
+                  "node or periodic link", who);
     return ISKRA_E_UNSUPPORTED;
   }
   const long long nn = (long long)nx * nx;
@@ -1692,13 +1707,41 @@ int iskra_ws_solve(iskra_ws *w, int warm, double omega, int sweeps, int max_iter
     WRC(isk_alloc((void **)&w->ls, LS_NS * 8));
     ISK_HIP(hipMemsetAsync(w->ls, 0, LS_NS * 8, c->stream));
   }
-  hipEvent_t ev = nullptr;
-  WRC(isk_prof_begin(c, &ev));
+  return ISKRA_OK;
+}
+
+/* training pair (r, phi - x) of the current iterate appended to the dataset */
+static int ls_capture_pair(iskra_ws *w) {
+  iskra_ctx *c = w->ctx;
+  const int nx = c->g.nx;
+  const long long cnt = (long long)(w->S - 1) * (w->S - 1);
+  const unsigned nb = std::min(ws_grid(cnt), (unsigned)w->dpart_n);
+  hipLaunchKernelGGL(k_ws_rhs_sumsq, dim3(nb), dim3(EW_BLOCK), 0, c->stream,
+                     (const double *)w->lr, nx, 1.0, w->dpart);
+  ISK_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_ws_rhs_scale, dim3(1), dim3(64), 0, c->stream, (const double *)w->dpart,
+                     (int)nb, cnt, w->scal);
+  ISK_CHECK_LAUNCH();
+  const long long S2 = (long long)w->S * w->S;
+  hipLaunchKernelGGL(k_ws_pair_store, dim3(ws_grid(S2)), dim3(EW_BLOCK), 0, c->stream,
+                     (const double *)w->lr, (const double *)c->phi, (const double *)w->lx, nx,
+                     (const double *)w->scal, w->ds_in + w->ds_n * S2, w->ds_out + w->ds_n * S2);
+  ISK_CHECK_LAUNCH();
+  ++w->ds_n;
+  return ISKRA_OK;
+}
+
+/* Flexible CG from phi (warm) or 0 on f = dx^2 rho / eps0 until the relative
+ * residual meets rtol or maxit iterations; with caps (ascending), the
+ * iteration also stops after the last cap and the pair of every capped
+ * iterate is captured.  Iterate in lx, residual in lr. */
+static int ls_iterate(iskra_ws *w, int warm, double om4, int sweeps, int maxit, double rtol,
+                      const int *caps, int ncaps, int *it_out, double *rr_out) {
+  iskra_ctx *c = w->ctx;
+  const int nx = c->g.nx;
   const long long n = (long long)(nx - 2) * (nx - 2);
   const int nb = (int)std::min((n + LS_BLOCK - 1) / LS_BLOCK, 1024ll);
-  const double fscale = c->g.dx * c->g.dx / p.eps0;
-  const double om4 = 0.25 * omega;
-  const int maxit = max_iter > 0 ? max_iter : p.max_iter;
+  const double fscale = c->g.dx * c->g.dx / c->ps.eps0;
   hipLaunchKernelGGL(k_ls_init, dim3(nb), dim3(LS_BLOCK), 0, c->stream, (const double *)c->rho,
                      warm == ISKRA_WARM_PREV ? (const double *)c->phi : (const double *)nullptr,
                      nx, fscale, w->lx, w->lr, w->lpart);
@@ -1710,8 +1753,9 @@ int iskra_ws_solve(iskra_ws *w, int warm, double omega, int sweeps, int max_iter
   ISK_HIP(hipMemcpyAsync(s, w->ls, 16, hipMemcpyDeviceToHost, c->stream));
   ISK_HIP(hipStreamSynchronize(c->stream));
   double rr = s[0] > 0.0 ? sqrt(s[1] / s[0]) : 0.0;
-  int it = 0, cur = 0;
-  while (rr > p.rtol && it < maxit) {
+  int it = 0, cur = 0, next_cap = 0;
+  if (caps) maxit = std::min(maxit, caps[ncaps - 1]);
+  while (rr > rtol && it < maxit) {
     ++it;
     const double *z = nullptr;
     WRC(ls_precond(w, nx, om4, sweeps, &z));
@@ -1742,11 +1786,36 @@ int iskra_ws_solve(iskra_ws *w, int warm, double omega, int sweeps, int max_iter
     hipLaunchKernelGGL(k_ls_reduce, dim3(1), dim3(64), 0, c->stream, (const double *)w->lpart,
                        nb, 1, w->ls + LS_RR);
     ISK_CHECK_LAUNCH();
+    if (caps && next_cap < ncaps && it == caps[next_cap]) {
+      WRC(ls_capture_pair(w));
+      ++next_cap;
+    }
     ISK_HIP(hipMemcpyAsync(s, w->ls, 16, hipMemcpyDeviceToHost, c->stream));
     ISK_HIP(hipStreamSynchronize(c->stream));
     rr = s[0] > 0.0 ? sqrt(s[1] / s[0]) : 0.0;
     cur ^= 1;
   }
+  *it_out = it;
+  *rr_out = rr;
+  return ISKRA_OK;
+}
+
+int iskra_ws_solve(iskra_ws *w, int warm, double omega, int sweeps, int max_iter, int *iters,
+                   double *relres) {
+  WRC(ws_check(w));
+  if (warm != ISKRA_WARM_ZERO && warm != ISKRA_WARM_PREV) return ISKRA_E_ARG;
+  WRC(ls_setup(w, "ws_solve", omega, sweeps));
+  iskra_ctx *c = w->ctx;
+  IskPoisson &p = c->ps;
+  const int nx = c->g.nx;
+  hipEvent_t ev = nullptr;
+  WRC(isk_prof_begin(c, &ev));
+  int it = 0;
+  double rr = 0.0;
+  WRC(ls_iterate(w, warm, 0.25 * omega, sweeps, max_iter > 0 ? max_iter : p.max_iter, p.rtol,
+                 nullptr, 0, &it, &rr));
+  const long long n = (long long)(nx - 2) * (nx - 2);
+  const int nb = (int)std::min((n + LS_BLOCK - 1) / LS_BLOCK, 1024ll);
   hipLaunchKernelGGL(k_ls_out, dim3(nb), dim3(LS_BLOCK), 0, c->stream, (const double *)w->lx,
                      c->phi, nx);
   ISK_CHECK_LAUNCH();
@@ -1765,6 +1834,51 @@ int iskra_ws_solve(iskra_ws *w, int warm, double omega, int sweeps, int max_iter
                   it);
     return ISKRA_E_CONVERGE;
   }
+  return ISKRA_OK;
+}
+
+int iskra_ws_capture_iterates(iskra_ws *w, double omega, int sweeps, const int *iters,
+                              int n_iters, int64_t *captured) {
+  WRC(ws_check(w));
+  if (!iters || n_iters < 1 || iters[0] < 1) {
+    isk_set_error("ws_capture_iterates: n_iters >= 1 iteration counts, the first >= 1");
+    return ISKRA_E_ARG;
+  }
+  for (int k = 1; k < n_iters; ++k)
+    if (iters[k] <= iters[k - 1]) {
+      isk_set_error("ws_capture_iterates: iteration counts must increase");
+      return ISKRA_E_ARG;
+    }
+  if (w->ds_n + n_iters > w->ds_cap) {
+    isk_set_error("ws_capture_iterates: dataset full (%lld of %lld samples)", (long long)w->ds_n,
+                  (long long)w->ds_cap);
+    return ISKRA_E_CAPACITY;
+  }
+  WRC(ls_setup(w, "ws_capture_iterates", omega, sweeps));
+  const long long n0 = w->ds_n;
+  int it = 0;
+  double rr = 0.0;
+  WRC(ls_iterate(w, ISKRA_WARM_ZERO, 0.25 * omega, sweeps, iters[n_iters - 1], 0.0, iters,
+                 n_iters, &it, &rr));
+  ISK_HIP(hipStreamSynchronize(w->ctx->stream));
+  if (captured) *captured = w->ds_n - n0;
+  return ISKRA_OK;
+}
+
+int iskra_ws_dataset_get(iskra_ws *w, int64_t index, float *in, float *out) {
+  WRC(ws_check(w));
+  if (index < 0 || index >= w->ds_n || !in || !out) {
+    isk_set_error("ws_dataset_get: index outside the dataset or null buffer");
+    return ISKRA_E_ARG;
+  }
+  const long long S2 = (long long)w->S * w->S;
+  std::vector<uint16_t> h(S2);
+  ISK_HIP(hipMemcpyAsync(h.data(), w->ds_in + index * S2, S2 * 2, hipMemcpyDeviceToHost,
+                         w->ctx->stream));
+  ISK_HIP(hipMemcpyAsync(out, w->ds_out + index * S2, S2 * 4, hipMemcpyDeviceToHost,
+                         w->ctx->stream));
+  ISK_HIP(hipStreamSynchronize(w->ctx->stream));
+  for (long long i = 0; i < S2; ++i) in[i] = ws_bf2f(h[i]);
   return ISKRA_OK;
 }
 

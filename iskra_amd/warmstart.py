@@ -115,10 +115,11 @@ def two_stream_setup(ctx, side, ppc=16, seed=0, v0=None, amp=0.02, modes=4, solv
     return sid, 0.1
 
 
-def capture_two_stream(ws, runs, steps, skip=20, ppc=16, seed=0, record=None):
+def capture_two_stream(ws, runs, steps, skip=20, ppc=16, seed=0, record=None, record_every=1):
     """Run `runs` two-stream simulations on the network's context, `steps`
     steps each, capturing (f, phi) after every step beyond `skip`.  `record`
-    (list, optional) receives (rho, phi_prev, phi) of every captured step."""
+    (list, optional) receives (rho, phi_prev, phi) of every record_every-th
+    captured step."""
     ctx = ws.ctx
     for r in range(runs):
         sid, dt = two_stream_setup(ctx, ws.side, ppc=ppc, seed=seed * 1000 + r)
@@ -127,10 +128,32 @@ def capture_two_stream(ws, runs, steps, skip=20, ppc=16, seed=0, record=None):
         for s in range(steps):
             ctx.step(dt, 1)
             ws.capture()
-            if record is not None:
+            if record is not None and (record_every == 1 or (s + 1) % record_every == 0):
                 phi = ctx.field(L.F_PHI)
                 record.append((ctx.field(L.F_RHO), prev, phi))
                 prev = phi
+            elif record is not None and (s + 2) % record_every == 0:
+                prev = ctx.field(L.F_PHI)
+
+
+def capture_iterate_pairs(ws, rhos, iters=(1, 2, 4, 8, 16), omega=0.8, sweeps=2):
+    """For each rho of `rhos`: the exact phi by the spectral direct solve,
+    then the pairs (r_k, phi - x_k) of iskra_ws_solve's iterates after each
+    count of `iters` appended to the dataset (the residuals the network meets
+    inside the iteration, whose distribution depends on the network itself).
+    Returns the number of pairs appended."""
+    ctx = ws.ctx
+    n = ws.side + 1
+    tags = np.zeros((n, n), np.int8)
+    tags[0, :] = tags[-1, :] = tags[:, 0] = tags[:, -1] = 1
+    ctx.poisson(L.POISSON_SPECTRAL, 1.0, tags.ravel(), rtol=1e-11)
+    ctx.tag_value(1, 0.0)
+    got = 0
+    for rho in rhos:
+        ctx.set_field(L.F_RHO, rho)
+        ctx.solve(L.WARM_ZERO)
+        got += ws.capture_iterates(iters, omega, sweeps)
+    return got
 
 
 def train(ws, first, count, steps, batch=8, lr=1e-3, lr_end=1e-5, chunk=50, seed=1, log=None):
@@ -202,44 +225,72 @@ def solve_table(ctx, ws, snapshots, kinds=(L.POISSON_PCG, L.POISSON_MG), rtol=1e
 
 def experiment(side=512, levels=4, base=32, runs=8, steps_per_run=500, train_steps=1000,
                batch=8, lr=1e-3, held_steps=41, every=10, seed=0, log=print,
-               objective=L.WS_ENERGY, omega=0.8, sweeps=1):
+               objective=L.WS_ENERGY_NET, omega=0.8, sweeps=2, rounds=3, round_snaps=200,
+               round_iters=(1, 2, 4, 8, 16), round_steps=1000, round_lr=3e-4):
     """Capture, train, evaluate; returns a dict of the measured figures.
-    objective: L.WS_ENERGY trains the network as the preconditioner of
-    iskra_ws_solve (`sweeps` Jacobi sweeps of weight omega after it), and the
-    evaluation then includes that solve; L.WS_PHI_MSE trains it as a warm
-    start only.  Everything runs on device 0 in one context, closed before
-    returning."""
+
+    objective L.WS_ENERGY_NET or L.WS_ENERGY trains the network as the
+    coarse solver of iskra_ws_solve's two-level cycle (`sweeps` Jacobi
+    sweeps of weight omega on either side); then `rounds` times: the pairs
+    of the solve's own iterates (`round_iters` iterations) on `round_snaps`
+    training snapshots are added to the dataset and training continues
+    (`round_steps` steps at `round_lr`), so the network learns the residuals
+    it meets inside the iteration.  The evaluation then includes that solve.
+    L.WS_PHI_MSE trains a warm start only (no rounds).  The held-out
+    snapshots come from a run the training never sees.  Everything runs on
+    device 0 in one context, closed before returning."""
     t0 = time.perf_counter()
 
     def say(m):
         if log is not None:
             log(f"[{time.perf_counter() - t0:7.1f} s] {m}")
 
+    energy = objective in (L.WS_ENERGY, L.WS_ENERGY_NET)
+    rounds = rounds if energy else 0
     ctx = Context(0)
     ws = None
     try:
         ctx.grid(2, (side + 1, side + 1), (1.0, 1.0))
         ws = WarmStart(ctx, levels=levels, base=base, side=side, max_batch=batch)
         ws.set_params(he_init(levels, base, seed))
-        ws.set_objective(objective, omega if objective == L.WS_ENERGY else 0.0, sweeps)
+        ws.set_objective(objective, omega if energy else 0.0, sweeps if energy else 0)
         n = runs * steps_per_run
-        ws.reserve(n + held_steps)
-        t = time.perf_counter()
-        for r in range(runs):
-            capture_two_stream(ws, runs=1, steps=steps_per_run, seed=seed * 100 + 10 + r)
-        ctx.sync()
-        cap_s = time.perf_counter() - t
-        say(f"captured {ws.size()} snapshots in {cap_s:.1f} s")
+        ws.reserve(held_steps + n + rounds * round_snaps * len(round_iters))
         held = []
         capture_two_stream(ws, runs=1, steps=held_steps, seed=seed * 100 + 99, record=held)
-        ws.set_gain(ws.dataset_rms(0, n))
-        e0 = ws.eval(n, held_steps)
+        pool = []
+        pool_every = max(1, (n // max(round_snaps, 1)) // 2) if rounds else 0
         t = time.perf_counter()
-        losses, gain = train(ws, 0, n, steps=train_steps, batch=batch, lr=lr,
+        for r in range(runs):
+            capture_two_stream(ws, runs=1, steps=steps_per_run, seed=seed * 100 + 10 + r,
+                               record=pool if rounds else None, record_every=pool_every or 1)
+        ctx.sync()
+        cap_s = time.perf_counter() - t
+        say(f"captured {n} training and {held_steps} held-out snapshots in {cap_s:.1f} s")
+        h0 = held_steps
+        ws.set_gain(ws.dataset_rms(h0, n))
+        e0 = ws.eval(0, held_steps)
+        t = time.perf_counter()
+        losses, gain = train(ws, h0, n, steps=train_steps, batch=batch, lr=lr,
                              chunk=max(1, train_steps // 10), log=say)
+        ws.set_gain(gain)
+        pairs = 0
+        rng = np.random.default_rng(seed + 17)
+        for rd in range(rounds):
+            pick = rng.choice(len(pool), min(round_snaps, len(pool)), replace=False)
+            pairs += capture_iterate_pairs(ws, [pool[j][0] for j in pick], round_iters, omega,
+                                           sweeps)
+            k = ws.size() - h0
+            done = 0
+            while done < round_steps:
+                c = min(max(1, round_steps // 5), round_steps - done)
+                ws.train(h0, k, batch, c, round_lr, seed=seed + 7919 * (rd + 1) + done)
+                done += c
+            say(f"round {rd + 1}: {pairs} iterate pairs, {round_steps} more steps")
         ctx.sync()
         tr_s = time.perf_counter() - t
-        e1 = ws.eval(n, held_steps)
+        all_steps = train_steps + rounds * round_steps
+        e1 = ws.eval(0, held_steps)
         say(f"held-out loss {e0:.4e} -> {e1:.4e}")
         ctx.set_field(L.F_RHO, held[0][0])
         ctx.profile(True)
@@ -249,11 +300,13 @@ def experiment(side=512, levels=4, base=32, runs=8, steps_per_run=500, train_ste
         ctx.profile(False)
         inf_ms = st["kernel_ms"]["warmstart"] / max(st["kernel_calls"]["warmstart"], 1)
         names = {L.POISSON_PCG: "pcg", L.POISSON_MG: "mg"}
-        objname = {L.WS_ENERGY: f"energy (A-norm after {sweeps} Jacobi sweeps, omega {omega})",
+        objname = {L.WS_ENERGY: f"energy of the cycle ({sweeps} Jacobi sweeps each side, "
+                                f"omega {omega})",
+                   L.WS_ENERGY_NET: f"energy of the network's own correction inside the cycle "
+                                    f"({sweeps} Jacobi sweeps each side, omega {omega})",
                    L.WS_PHI_MSE: "phi mean squared error"}[objective]
         snaps = [(rho, prev) for rho, prev, _phi in held[1::every]]
-        table = solve_table(ctx, ws, snaps,
-                            fcg_omega=omega if objective == L.WS_ENERGY else None,
+        table = solve_table(ctx, ws, snaps, fcg_omega=omega if energy else None,
                             fcg_sweeps=sweeps)
         summary = {names[k]: {s: {"iters": float(np.mean(v["iters"])),
                                   "ms": float(np.median(v["ms"])),
@@ -261,9 +314,9 @@ def experiment(side=512, levels=4, base=32, runs=8, steps_per_run=500, train_ste
                               for s, v in row.items()} for k, row in table.items()}
         flops = forward_flops(levels, base, side)
         out = dict(side=side, levels=levels, base=base, params=ws.nparam, snapshots=n,
-                   capture_s=cap_s, train_steps=train_steps, batch=batch, train_s=tr_s,
-                   train_ms_per_step=1e3 * tr_s / max(train_steps, 1),
-                   train_tflops=3 * flops * batch / (tr_s / max(train_steps, 1)) / 1e12,
+                   capture_s=cap_s, train_steps=all_steps, batch=batch, train_s=tr_s,
+                   rounds=rounds, iterate_pairs=pairs,
+                   train_ms_per_step=1e3 * tr_s / max(all_steps, 1),
                    objective=objname, heldout_loss_before=e0, heldout_loss_after=e1, gain=gain,
                    inference_ms=inf_ms, inference_tflops=flops / (inf_ms * 1e-3) / 1e12,
                    solves=summary, evaluated=len(snaps))
@@ -298,15 +351,20 @@ def main(argv=None):
     ap.add_argument("--train-steps", type=int, default=4000)
     ap.add_argument("--batch", type=int, default=8)
     ap.add_argument("--lr", type=float, default=1e-3)
-    ap.add_argument("--objective", choices=("energy", "mse"), default="energy")
+    ap.add_argument("--objective", choices=("net", "cycle", "mse"), default="net")
     ap.add_argument("--omega", type=float, default=0.8)
-    ap.add_argument("--sweeps", type=int, default=1)
+    ap.add_argument("--sweeps", type=int, default=2)
+    ap.add_argument("--rounds", type=int, default=3)
+    ap.add_argument("--round-snaps", type=int, default=200)
+    ap.add_argument("--round-steps", type=int, default=1000)
     ap.add_argument("--out", default=None)
     a = ap.parse_args(argv)
     res = experiment(a.side, a.levels, a.base, a.runs, a.steps_per_run, a.train_steps, a.batch,
                      a.lr, log=lambda m: print(m, flush=True),
-                     objective=L.WS_ENERGY if a.objective == "energy" else L.WS_PHI_MSE,
-                     omega=a.omega, sweeps=a.sweeps)
+                     objective={"net": L.WS_ENERGY_NET, "cycle": L.WS_ENERGY,
+                                "mse": L.WS_PHI_MSE}[a.objective],
+                     omega=a.omega, sweeps=a.sweeps, rounds=a.rounds, round_snaps=a.round_snaps,
+                     round_steps=a.round_steps)
     s = json.dumps(res)
     print(s, flush=True)
     if a.out:

@@ -123,6 +123,109 @@ def test_ws_energy_gradients_match_oracle(gpu_ready):
             e = _relerr(g[off:off + n], gref[off:off + n])
             print("layer", k, cin, cout, "grad relerr", e)
             assert e < 3e-2, (k, e)
+        # the network's own error (no post-sweeps in the loss), same pairs
+        ws.set_objective(Lb.WS_ENERGY_NET, 0.8, 2)
+        loss, g = ws.backward(x, t)
+        _, lref, gref = O.ws_run(L_, BASE_, p.astype(np.float64), xp, tp, round_bf16=True,
+                                 objective=1, gain=1.3, omega=0.8, sweeps=0)
+        print("network energy loss", loss, lref)
+        assert loss == pytest.approx(lref, rel=1e-3)
+        # its output gradient A e is not smoothed by post-sweeps, so BF16
+        # rounding flips of the gradient plane weigh more (a wrong stencil or
+        # sign gives errors of order 1)
+        for k, (off, n, cin, cout) in enumerate(W.layer_slices(L_, BASE_)):
+            e = _relerr(g[off:off + n], gref[off:off + n])
+            print("layer", k, cin, cout, "network-objective grad relerr", e)
+            assert e < 6e-2, (k, e)
+    finally:
+        ws.close()
+        ctx.close()
+
+
+def _A_interior(v):
+    y = 4 * v
+    y[1:] -= v[:-1]
+    y[:-1] -= v[1:]
+    y[:, 1:] -= v[:, :-1]
+    y[:, :-1] -= v[:, 1:]
+    return y
+
+
+def test_ws_capture_iterates_pairs_are_the_solve_iterates(gpu_ready):
+    """iskra_ws_capture_iterates: each stored pair is (r_k, phi - x_k) / rms(r_k)
+    of iskra_ws_solve's k-th iterate (checked against a separate solve capped
+    at k iterations) and is exact: A target = input within the BF16 rounding
+    of the input; the context's phi is left as it was."""
+    from iskra_amd import Context, _lib as Lb
+    from iskra_amd import warmstart as W
+    ctx = Context(0)
+    ctx.grid(2, (S_ + 1, S_ + 1), (1.0, 1.0))
+    ws = Lb.WarmStart(ctx, levels=L_, base=BASE_, side=S_, max_batch=2)
+    try:
+        ws.set_params(W.he_init(L_, BASE_, 6))
+        ws.set_gain(0.05)
+        ws.reserve(8)
+        n = S_ + 1
+        m = S_ - 1
+        rng = np.random.default_rng(21)
+        rho = np.zeros((n, n))
+        rho[1:-1, 1:-1] = rng.standard_normal((m, m))
+        got = W.capture_iterate_pairs(ws, [rho.ravel()], iters=(1, 2, 5), omega=0.8, sweeps=2)
+        assert got == 3 and ws.size() == 3
+        phi_exact = ctx.field(Lb.F_PHI).copy()
+        f = rho[1:-1, 1:-1]
+        for idx, k in enumerate((1, 2, 5)):
+            xin, tgt = ws.dataset_get(idx)
+            xin, tgt = xin[:m, :m].astype(np.float64), tgt[:m, :m].astype(np.float64)
+            assert np.linalg.norm(_A_interior(tgt) - xin) <= 5e-3 * np.linalg.norm(xin)
+            ctx.set_field(Lb.F_RHO, rho.ravel())
+            with pytest.raises(Lb.IskraError):
+                ws.solve(Lb.WARM_ZERO, 0.8, 2, max_iter=k)
+            x = ctx.field(Lb.F_PHI).reshape(n, n)[1:-1, 1:-1]
+            r = f - _A_interior(x)
+            s = np.sqrt(np.mean(r * r))
+            assert np.allclose(xin, r / s, rtol=0, atol=8e-3 * np.max(np.abs(r / s))), k
+            ex = phi_exact.reshape(n, n)[1:-1, 1:-1]
+            assert np.allclose(tgt, (ex - x) / s, rtol=1e-5, atol=1e-6 * np.max(np.abs(tgt))), k
+            ctx.set_field(Lb.F_PHI, phi_exact)
+        with pytest.raises(Lb.IskraError) as ei:
+            ws.capture_iterates([3, 2])
+        assert ei.value.code == Lb.E_ARG
+    finally:
+        ws.close()
+        ctx.close()
+
+
+def test_ws_network_objective_with_iterate_rounds_cuts_iterations_3x(gpu_ready):
+    """The training recipe of the learned solve on a 65 x 65-node grid: the
+    network-error objective on two-stream snapshots, then one round of the
+    solve's own iterate pairs; held-out snapshots of another run reach the
+    FP64 residual 1e-8 in at least 3x fewer iterations than Jacobi-PCG."""
+    from iskra_amd import Context, _lib as Lb
+    from iskra_amd import warmstart as W
+    ctx = Context(0)
+    ctx.grid(2, (S_ + 1, S_ + 1), (1.0, 1.0))
+    ws = Lb.WarmStart(ctx, levels=L_, base=BASE_, side=S_, max_batch=8)
+    try:
+        ws.set_params(W.he_init(L_, BASE_, 3))
+        ws.set_objective(Lb.WS_ENERGY_NET, 0.8, 2)
+        ws.reserve(600)
+        held = []
+        W.capture_two_stream(ws, runs=1, steps=40, seed=2, record=held)
+        pool = []
+        W.capture_two_stream(ws, runs=2, steps=150, seed=1, record=pool, record_every=5)
+        _, gain = W.train(ws, 40, 300, steps=1000, batch=8, lr=2e-3)
+        ws.set_gain(gain)
+        W.capture_iterate_pairs(ws, [p[0] for p in pool[:40]], iters=(1, 2, 4, 8, 16))
+        ws.train(40, ws.size() - 40, 8, 500, 5e-4, seed=9)
+        snaps = [(rho, prev) for rho, prev, _phi in held[::8]]
+        table = W.solve_table(ctx, ws, snaps, kinds=(Lb.POISSON_PCG,), fcg_omega=0.8,
+                              fcg_sweeps=2)[Lb.POISSON_PCG]
+        print({k: v["iters"] for k, v in table.items()}, flush=True)
+        for v in table.values():
+            assert max(v["relres"]) <= 1e-8
+        plain, fcg = np.mean(table["zero"]["iters"]), np.mean(table["fcg"]["iters"])
+        assert plain >= 3.0 * fcg, (plain, fcg)
     finally:
         ws.close()
         ctx.close()
