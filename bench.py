@@ -342,8 +342,11 @@ def main():
             "train_ms_per_step": ws["train_ms_per_step"],
             "inference_ms": ws["inference_ms"], "heldout_loss": ws["heldout_loss_after"],
             "solves": ws["solves"],
-            "pcg_iters_plain_over_learned": pcg["zero"]["iters"] / pcg["fcg"]["iters"],
-            "pcg_ms_plain_over_learned": pcg["zero"]["ms"] / pcg["fcg"]["ms"],
+            "learned_converged": pcg["fcg"]["converged"],
+            "pcg_iters_plain_over_learned": (pcg["zero"]["iters"] / pcg["fcg"]["iters"]
+                                             if pcg["fcg"]["converged"] else None),
+            "pcg_ms_plain_over_learned": (pcg["zero"]["ms"] / pcg["fcg"]["ms"]
+                                          if pcg["fcg"]["converged"] else None),
             "pcg_iters_plain_over_warm_start": pcg["zero"]["iters"] / pcg["learned"]["iters"],
         }
     if not args.no_cpu:

@@ -639,13 +639,16 @@ class WarmStart:
         Jacobi weight and sweep count the solve will use)."""
         check(lib().iskra_ws_set_objective(self.h, int(objective), float(omega), int(sweeps)))
 
-    def solve(self, warm=WARM_ZERO, omega=0.8, sweeps=1, max_iter=0):
+    def solve(self, warm=WARM_ZERO, omega=0.8, sweeps=1, max_iter=0, raise_unconverged=True):
         """Flexible CG on the context's system preconditioned by the network
         (then `sweeps` damped Jacobi sweeps when omega > 0); returns
-        (iters, relres)."""
+        (iters, relres).  raise_unconverged=False returns them also when
+        max_iter ran out (ISKRA_E_CONVERGE)."""
         it, rr = C.c_int(), C.c_double()
-        check(lib().iskra_ws_solve(self.h, int(warm), float(omega), int(sweeps), int(max_iter),
-                                   C.byref(it), C.byref(rr)))
+        rc = lib().iskra_ws_solve(self.h, int(warm), float(omega), int(sweeps), int(max_iter),
+                                  C.byref(it), C.byref(rr))
+        if rc != E_CONVERGE or raise_unconverged:
+            check(rc)
         return it.value, rr.value
 
     def capture_iterates(self, iters, omega=0.8, sweeps=2):

@@ -177,14 +177,16 @@ def train(ws, first, count, steps, batch=8, lr=1e-3, lr_end=1e-5, chunk=50, seed
 
 
 def solve_table(ctx, ws, snapshots, kinds=(L.POISSON_PCG, L.POISSON_MG), rtol=1e-8,
-                fcg_omega=None, fcg_sweeps=1):
+                fcg_omega=None, fcg_sweeps=1, fcg_max_iter=3000):
     """For each solver kind (configured once, then one untimed solve), the
     iterations and wall ms of solving every snapshot (rho, phi_prev) from a
     zero start, from phi_prev and from the network's prediction (inference
     time included).  With fcg_omega set, the PCG row also holds "fcg": the
     network inside the iteration (iskra_ws_solve from a zero start,
     fcg_sweeps Jacobi sweeps of weight fcg_omega after the network).  Every
-    solve must reach rtol.  Returns
+    solve must reach rtol, except the network-preconditioned one, which stops
+    after fcg_max_iter iterations (its row then shows a relres above rtol).
+    Returns
     {kind: {start: {"iters": [...], "ms": [...], "relres": [...]}}}."""
     n = ws.side + 1
     tags = np.zeros((n, n), np.int8)
@@ -198,7 +200,9 @@ def solve_table(ctx, ws, snapshots, kinds=(L.POISSON_PCG, L.POISSON_MG), rtol=1e
         starts = ["zero", "prev", "learned"]
         if fcg_omega is not None and kind == L.POISSON_PCG:
             starts.append("fcg")
-            ws.solve(L.WARM_ZERO, fcg_omega, fcg_sweeps)  # untimed: first-call allocations
+            # untimed: first-call allocations
+            ws.solve(L.WARM_ZERO, fcg_omega, fcg_sweeps, max_iter=fcg_max_iter,
+                     raise_unconverged=False)
         row = {s: {"iters": [], "ms": [], "relres": []} for s in starts}
         for rho, prev in snapshots:
             ctx.set_field(L.F_RHO, rho)
@@ -210,7 +214,8 @@ def solve_table(ctx, ws, snapshots, kinds=(L.POISSON_PCG, L.POISSON_MG), rtol=1e
                 if start == "zero":
                     it, rr = ctx.solve(L.WARM_ZERO)
                 elif start == "fcg":
-                    it, rr = ws.solve(L.WARM_ZERO, fcg_omega, fcg_sweeps)
+                    it, rr = ws.solve(L.WARM_ZERO, fcg_omega, fcg_sweeps,
+                                      max_iter=fcg_max_iter, raise_unconverged=False)
                 else:
                     if start == "learned":
                         ws.predict()
@@ -226,7 +231,8 @@ def solve_table(ctx, ws, snapshots, kinds=(L.POISSON_PCG, L.POISSON_MG), rtol=1e
 def experiment(side=512, levels=4, base=32, runs=8, steps_per_run=500, train_steps=1000,
                batch=8, lr=1e-3, held_steps=41, every=10, seed=0, log=print,
                objective=L.WS_ENERGY_NET, omega=0.8, sweeps=2, rounds=3, round_snaps=200,
-               round_iters=(1, 2, 4, 8, 16), round_steps=1000, round_lr=3e-4):
+               round_iters=(1, 2, 4, 8, 16, 32, 64), round_steps=1000, round_lr=3e-4,
+               round_keep=1000):
     """Capture, train, evaluate; returns a dict of the measured figures.
 
     objective L.WS_ENERGY_NET or L.WS_ENERGY trains the network as the
@@ -234,8 +240,9 @@ def experiment(side=512, levels=4, base=32, runs=8, steps_per_run=500, train_ste
     sweeps of weight omega on either side); then `rounds` times: the pairs
     of the solve's own iterates (`round_iters` iterations) on `round_snaps`
     training snapshots are added to the dataset and training continues
-    (`round_steps` steps at `round_lr`), so the network learns the residuals
-    it meets inside the iteration.  The evaluation then includes that solve.
+    (`round_steps` steps at `round_lr`) on the iterate pairs and the last
+    `round_keep` snapshots, so the network learns the residuals it meets
+    inside the iteration without them being diluted by the snapshots.  The evaluation then includes that solve.
     L.WS_PHI_MSE trains a warm start only (no rounds).  The held-out
     snapshots come from a run the training never sees.  Everything runs on
     device 0 in one context, closed before returning."""
@@ -280,11 +287,12 @@ def experiment(side=512, levels=4, base=32, runs=8, steps_per_run=500, train_ste
             pick = rng.choice(len(pool), min(round_snaps, len(pool)), replace=False)
             pairs += capture_iterate_pairs(ws, [pool[j][0] for j in pick], round_iters, omega,
                                            sweeps)
-            k = ws.size() - h0
+            first = h0 + max(0, n - round_keep)
+            k = ws.size() - first
             done = 0
             while done < round_steps:
                 c = min(max(1, round_steps // 5), round_steps - done)
-                ws.train(h0, k, batch, c, round_lr, seed=seed + 7919 * (rd + 1) + done)
+                ws.train(first, k, batch, c, round_lr, seed=seed + 7919 * (rd + 1) + done)
                 done += c
             say(f"round {rd + 1}: {pairs} iterate pairs, {round_steps} more steps")
         ctx.sync()
@@ -310,7 +318,8 @@ def experiment(side=512, levels=4, base=32, runs=8, steps_per_run=500, train_ste
                             fcg_sweeps=sweeps)
         summary = {names[k]: {s: {"iters": float(np.mean(v["iters"])),
                                   "ms": float(np.median(v["ms"])),
-                                  "max_relres": float(np.max(v["relres"]))}
+                                  "max_relres": float(np.max(v["relres"])),
+                                  "converged": bool(np.max(v["relres"]) <= 1e-8)}
                               for s, v in row.items()} for k, row in table.items()}
         flops = forward_flops(levels, base, side)
         out = dict(side=side, levels=levels, base=base, params=ws.nparam, snapshots=n,
@@ -348,7 +357,7 @@ def main(argv=None):
     ap.add_argument("--base", type=int, default=32)
     ap.add_argument("--runs", type=int, default=40)
     ap.add_argument("--steps-per-run", type=int, default=500)
-    ap.add_argument("--train-steps", type=int, default=4000)
+    ap.add_argument("--train-steps", type=int, default=1000)
     ap.add_argument("--batch", type=int, default=8)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--objective", choices=("net", "cycle", "mse"), default="net")
@@ -357,6 +366,7 @@ def main(argv=None):
     ap.add_argument("--rounds", type=int, default=3)
     ap.add_argument("--round-snaps", type=int, default=200)
     ap.add_argument("--round-steps", type=int, default=1000)
+    ap.add_argument("--round-iters", default="1,2,4,8,16,32,64")
     ap.add_argument("--out", default=None)
     a = ap.parse_args(argv)
     res = experiment(a.side, a.levels, a.base, a.runs, a.steps_per_run, a.train_steps, a.batch,
@@ -364,7 +374,8 @@ def main(argv=None):
                      objective={"net": L.WS_ENERGY_NET, "cycle": L.WS_ENERGY,
                                 "mse": L.WS_PHI_MSE}[a.objective],
                      omega=a.omega, sweeps=a.sweeps, rounds=a.rounds, round_snaps=a.round_snaps,
-                     round_steps=a.round_steps)
+                     round_steps=a.round_steps,
+                     round_iters=tuple(int(v) for v in a.round_iters.split(",")))
     s = json.dumps(res)
     print(s, flush=True)
     if a.out:
