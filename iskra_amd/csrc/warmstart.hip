@@ -48,6 +48,9 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 #define WS_PART_BYTES (64ll << 20)
 #define EW_BLOCK 256
 #define WS_MAX_SWEEPS 16
+#ifndef WS_CONV_MIN_WG
+#define WS_CONV_MIN_WG 256 /* work-groups a conv tile should give (1 per CU) */
+#endif
 
 __host__ __device__ inline uint16_t ws_f2bf(float f) {
   uint32_t u = __builtin_bit_cast(uint32_t, f);
@@ -78,71 +81,92 @@ struct ConvArgs {
   int H, W, Cin, Cout, Kpad, relu, mode;
 };
 
-template <int TM>
+/* A wave computes TM output channels x NP groups of 16 pixels: per 32-wide
+ * K block it loads TM/16 weight fragments once and reuses each for the NP
+ * pixel groups (TM/16 x NP MFMAs per block).  Work-group = 4 waves =
+ * 64 NP pixels. */
+template <int TM, int NP>
 __global__ __launch_bounds__(256) void k_ws_conv(ConvArgs a) {
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int col = lane & 15, grp = lane >> 4;
-  const long long p = (long long)blockIdx.x * 64 + wv * 16 + col;
   const int HW = a.H * a.W;
-  const int n = (int)(p / HW);
-  const int rem = (int)(p - (long long)n * HW);
-  const int h = rem / a.W, w = rem - h * a.W;
-  const int co0 = blockIdx.y * TM;
-  f32x4 acc[TM / 16];
+  long long p[NP];
+  int n[NP], h[NP], w[NP];
 #pragma unroll
-  for (int j = 0; j < TM / 16; ++j) acc[j] = (f32x4){0.f, 0.f, 0.f, 0.f};
+  for (int g = 0; g < NP; ++g) {
+    p[g] = (long long)blockIdx.x * (64 * NP) + wv * (16 * NP) + g * 16 + col;
+    n[g] = (int)(p[g] / HW);
+    const int rem = (int)(p[g] - (long long)n[g] * HW);
+    h[g] = rem / a.W;
+    w[g] = rem - h[g] * a.W;
+  }
+  const int co0 = blockIdx.y * TM;
+  f32x4 acc[NP][TM / 16];
+#pragma unroll
+  for (int g = 0; g < NP; ++g)
+#pragma unroll
+    for (int j = 0; j < TM / 16; ++j) acc[g][j] = (f32x4){0.f, 0.f, 0.f, 0.f};
   const int nkb = a.Kpad >> 5;
   for (int kb = 0; kb < nkb; ++kb) {
     const int k0 = kb * 32 + grp * 8;
     const int tap = k0 / a.Cin;
     const int ci0 = k0 - tap * a.Cin;
-    u32x4 bv = (u32x4){0u, 0u, 0u, 0u};
-    if (tap < 9) {
-      const int hh = h + tap / 3 - 1, ww = w + tap % 3 - 1;
-      if (hh >= 0 && hh < a.H && ww >= 0 && ww < a.W)
-        bv = *(const u32x4 *)(a.X + ((long long)(n * a.H + hh) * a.W + ww) * a.Cin + ci0);
+    u32x4 av[TM / 16];
+#pragma unroll
+    for (int j = 0; j < TM / 16; ++j)
+      av[j] = *(const u32x4 *)(a.Wt + (long long)(co0 + j * 16 + col) * a.Kpad + k0);
+    u32x4 bv[NP];
+#pragma unroll
+    for (int g = 0; g < NP; ++g) {
+      bv[g] = (u32x4){0u, 0u, 0u, 0u};
+      if (tap < 9) {
+        const int hh = h[g] + tap / 3 - 1, ww = w[g] + tap % 3 - 1;
+        if (hh >= 0 && hh < a.H && ww >= 0 && ww < a.W)
+          bv[g] = *(const u32x4 *)(a.X + ((long long)(n[g] * a.H + hh) * a.W + ww) * a.Cin + ci0);
+      }
     }
-    const bf16x8 B = __builtin_bit_cast(bf16x8, bv);
+#pragma unroll
+    for (int g = 0; g < NP; ++g)
+#pragma unroll
+      for (int j = 0; j < TM / 16; ++j)
+        acc[g][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+            __builtin_bit_cast(bf16x8, av[j]), __builtin_bit_cast(bf16x8, bv[g]), acc[g][j], 0, 0,
+            0);
+  }
+#pragma unroll
+  for (int g = 0; g < NP; ++g)
 #pragma unroll
     for (int j = 0; j < TM / 16; ++j) {
-      const int co = co0 + j * 16 + col;
-      const u32x4 av = *(const u32x4 *)(a.Wt + (long long)co * a.Kpad + k0);
-      acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, av), B, acc[j],
-                                                       0, 0, 0);
-    }
-  }
+      const int cb = co0 + j * 16 + grp * 4;
+      float v[4] = {acc[g][j][0], acc[g][j][1], acc[g][j][2], acc[g][j][3]};
+      if (a.mode == 0) {
+        if (a.skip) {
+          const uint2 s = *(const uint2 *)(a.skip + p[g] * a.Cout + cb);
+          v[0] += ws_bf2f((uint16_t)(s.x & 0xffffu));
+          v[1] += ws_bf2f((uint16_t)(s.x >> 16));
+          v[2] += ws_bf2f((uint16_t)(s.y & 0xffffu));
+          v[3] += ws_bf2f((uint16_t)(s.y >> 16));
+        }
+        if (a.relu) {
 #pragma unroll
-  for (int j = 0; j < TM / 16; ++j) {
-    const int cb = co0 + j * 16 + grp * 4;
-    float v[4] = {acc[j][0], acc[j][1], acc[j][2], acc[j][3]};
-    if (a.mode == 0) {
-      if (a.skip) {
-        const uint2 s = *(const uint2 *)(a.skip + p * a.Cout + cb);
-        v[0] += ws_bf2f((uint16_t)(s.x & 0xffffu));
-        v[1] += ws_bf2f((uint16_t)(s.x >> 16));
-        v[2] += ws_bf2f((uint16_t)(s.y & 0xffffu));
-        v[3] += ws_bf2f((uint16_t)(s.y >> 16));
+          for (int r = 0; r < 4; ++r) v[r] = fmaxf(v[r], 0.f);
+        }
+        uint2 o;
+        o.x = (uint32_t)ws_f2bf(v[0]) | ((uint32_t)ws_f2bf(v[1]) << 16);
+        o.y = (uint32_t)ws_f2bf(v[2]) | ((uint32_t)ws_f2bf(v[3]) << 16);
+        *(uint2 *)(a.Y + p[g] * a.Cout + cb) = o;
+      } else if (a.mode == 1) {
+        float4 *dst = (float4 *)(a.Yf + p[g] * a.Cout + cb);
+        float4 d = *dst;
+        d.x += v[0];
+        d.y += v[1];
+        d.z += v[2];
+        d.w += v[3];
+        *dst = d;
+      } else if (cb == 0) {
+        a.Yf[p[g]] = v[0];
       }
-      if (a.relu) {
-#pragma unroll
-        for (int r = 0; r < 4; ++r) v[r] = fmaxf(v[r], 0.f);
-      }
-      uint2 o;
-      o.x = (uint32_t)ws_f2bf(v[0]) | ((uint32_t)ws_f2bf(v[1]) << 16);
-      o.y = (uint32_t)ws_f2bf(v[2]) | ((uint32_t)ws_f2bf(v[3]) << 16);
-      *(uint2 *)(a.Y + p * a.Cout + cb) = o;
-    } else if (a.mode == 1) {
-      float4 *dst = (float4 *)(a.Yf + p * a.Cout + cb);
-      float4 d = *dst;
-      d.x += v[0];
-      d.y += v[1];
-      d.z += v[2];
-      d.w += v[3];
-      *dst = d;
-    } else if (cb == 0) {
-      a.Yf[p] = v[0];
     }
-  }
 }
 
 /* ------------------------------------------------------------------------ */
@@ -248,67 +272,95 @@ __global__ void k_ws_fill(float *p, long long n, float v) {
     p[i] = v;
 }
 
-/* P[n,h,w,c] = bf16(0.25 * sum of the 2x2 block of X) ; X is H x W, P H/2 x W/2 */
+/* Pooling and upsampling move 8 bf16 channels (16 bytes) or 4 fp32
+ * gradient channels per thread; C is a multiple of 8.  Vector index i covers
+ * channels 8 (or 4) i .. of the flat NHWC array. */
+
+/* P[n,h2,w2,c] = bf16(0.25 ((X00 + X01) + (X10 + X11))) over the 2x2 block */
 __global__ void k_ws_pool(const uint16_t *X, uint16_t *P, int H, int W, int C, long long n_out) {
-  const int H2 = H >> 1, W2 = W >> 1;
-  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n_out;
+  const int H2 = H >> 1, W2 = W >> 1, CV = C >> 3;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < (n_out >> 3);
        i += (long long)gridDim.x * blockDim.x) {
-    const int c = (int)(i % C);
-    const long long pix = i / C;
+    const int cv = (int)(i % CV);
+    const long long pix = i / CV;
     const int w2 = (int)(pix % W2);
     const long long r = pix / W2;
     const int h2 = (int)(r % H2);
     const int n = (int)(r / H2);
-    const long long b = ((long long)(n * H + 2 * h2) * W + 2 * w2) * C + c;
-    const float s = (ws_bf2f(X[b]) + ws_bf2f(X[b + C])) +
-                    (ws_bf2f(X[b + (long long)W * C]) + ws_bf2f(X[b + (long long)W * C + C]));
-    P[i] = ws_f2bf(0.25f * s);
+    const long long b = ((long long)(n * H + 2 * h2) * W + 2 * w2) * C + 8 * cv;
+    const u16x8 x00 = *(const u16x8 *)(X + b), x01 = *(const u16x8 *)(X + b + C);
+    const u16x8 x10 = *(const u16x8 *)(X + b + (long long)W * C);
+    const u16x8 x11 = *(const u16x8 *)(X + b + (long long)W * C + C);
+    u16x8 o;
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+      o[k] = ws_f2bf(0.25f * ((ws_bf2f(x00[k]) + ws_bf2f(x01[k])) +
+                              (ws_bf2f(x10[k]) + ws_bf2f(x11[k]))));
+    *(u16x8 *)(P + 8 * i) = o;
   }
 }
 
 /* GX[n,h,w,c] += 0.25 * GP[n,h/2,w/2,c] */
 __global__ void k_ws_pool_bwd(const float *GP, float *GX, int H, int W, int C, long long n_in) {
-  const int H2 = H >> 1, W2 = W >> 1;
-  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n_in;
+  const int H2 = H >> 1, W2 = W >> 1, CV = C >> 2;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < (n_in >> 2);
        i += (long long)gridDim.x * blockDim.x) {
-    const int c = (int)(i % C);
-    const long long pix = i / C;
+    const int cv = (int)(i % CV);
+    const long long pix = i / CV;
     const int w = (int)(pix % W);
     const long long r = pix / W;
     const int h = (int)(r % H);
     const int n = (int)(r / H);
-    GX[i] += 0.25f * GP[((long long)(n * H2 + (h >> 1)) * W2 + (w >> 1)) * C + c];
+    const float4 g = *(const float4 *)(GP + ((long long)(n * H2 + (h >> 1)) * W2 + (w >> 1)) * C +
+                                       4 * cv);
+    float4 *d = (float4 *)(GX + 4 * i);
+    float4 v = *d;
+    v.x += 0.25f * g.x;
+    v.y += 0.25f * g.y;
+    v.z += 0.25f * g.z;
+    v.w += 0.25f * g.w;
+    *d = v;
   }
 }
 
 /* U[n,h,w,c] = X[n,h/2,w/2,c] ; U is H x W */
 __global__ void k_ws_up(const uint16_t *X, uint16_t *U, int H, int W, int C, long long n_out) {
-  const int H2 = H >> 1, W2 = W >> 1;
-  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n_out;
+  const int H2 = H >> 1, W2 = W >> 1, CV = C >> 3;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < (n_out >> 3);
        i += (long long)gridDim.x * blockDim.x) {
-    const int c = (int)(i % C);
-    const long long pix = i / C;
+    const int cv = (int)(i % CV);
+    const long long pix = i / CV;
     const int w = (int)(pix % W);
     const long long r = pix / W;
     const int h = (int)(r % H);
     const int n = (int)(r / H);
-    U[i] = X[((long long)(n * H2 + (h >> 1)) * W2 + (w >> 1)) * C + c];
+    *(u16x8 *)(U + 8 * i) =
+        *(const u16x8 *)(X + ((long long)(n * H2 + (h >> 1)) * W2 + (w >> 1)) * C + 8 * cv);
   }
 }
 
 /* GX[n,h2,w2,c] += sum of GU over the 2x2 block ; GU is H x W */
 __global__ void k_ws_up_bwd(const float *GU, float *GX, int H, int W, int C, long long n_in) {
-  const int H2 = H >> 1, W2 = W >> 1;
-  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n_in;
+  const int H2 = H >> 1, W2 = W >> 1, CV = C >> 2;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < (n_in >> 2);
        i += (long long)gridDim.x * blockDim.x) {
-    const int c = (int)(i % C);
-    const long long pix = i / C;
+    const int cv = (int)(i % CV);
+    const long long pix = i / CV;
     const int w2 = (int)(pix % W2);
     const long long r = pix / W2;
     const int h2 = (int)(r % H2);
     const int n = (int)(r / H2);
-    const long long b = ((long long)(n * H + 2 * h2) * W + 2 * w2) * C + c;
-    GX[i] += (GU[b] + GU[b + C]) + (GU[b + (long long)W * C] + GU[b + (long long)W * C + C]);
+    const long long b = ((long long)(n * H + 2 * h2) * W + 2 * w2) * C + 4 * cv;
+    const float4 g00 = *(const float4 *)(GU + b), g01 = *(const float4 *)(GU + b + C);
+    const float4 g10 = *(const float4 *)(GU + b + (long long)W * C);
+    const float4 g11 = *(const float4 *)(GU + b + (long long)W * C + C);
+    float4 *d = (float4 *)(GX + 4 * i);
+    float4 v = *d;
+    v.x += (g00.x + g01.x) + (g10.x + g11.x);
+    v.y += (g00.y + g01.y) + (g10.y + g11.y);
+    v.z += (g00.z + g01.z) + (g10.z + g11.z);
+    v.w += (g00.w + g01.w) + (g10.w + g11.w);
+    *d = v;
   }
 }
 
@@ -440,10 +492,30 @@ __global__ __launch_bounds__(EW_BLOCK) void k_ws_rhs_sumsq(const double *rho, in
 }
 
 /* scal[0] = s = sqrt(mean f^2) (1 if f == 0), scal[1] = 1/s */
-__global__ void k_ws_rhs_scale(const double *part, int nb, long long count, double *scal) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    double t = 0.0;
-    for (int b = 0; b < nb; ++b) t += part[b];
+/* sum of part[stride b + v] over b < nb, one work-group of EW_BLOCK threads
+ * (thread t takes b = t, t + EW_BLOCK, ... in order, then a fixed tree):
+ * deterministic; valid in thread 0 */
+__device__ __forceinline__ double ws_group_sum(const double *part, int nb, int stride, int v,
+                                               double *red) {
+  double t = 0.0;
+  for (int b = threadIdx.x; b < nb; b += EW_BLOCK) t += part[(long long)stride * b + v];
+  red[threadIdx.x] = t;
+  __syncthreads();
+  for (int s = EW_BLOCK / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  const double r = red[0];
+  __syncthreads();
+  return r;
+}
+
+/* launched with one work-group of EW_BLOCK threads */
+__global__ __launch_bounds__(EW_BLOCK) void k_ws_rhs_scale(const double *part, int nb,
+                                                           long long count, double *scal) {
+  __shared__ double red[EW_BLOCK];
+  const double t = ws_group_sum(part, nb, 1, 0, red);
+  if (threadIdx.x == 0) {
     double s = sqrt(t / (double)count);
     if (!(s > 0.0)) s = 1.0;
     scal[0] = s;
@@ -685,12 +757,15 @@ __global__ __launch_bounds__(LS_BLOCK) void k_ls_init(const double *rho, const d
   }
 }
 
-/* out[v] = sum over blocks, in block order, of part[nv b + v] */
-__global__ void k_ls_reduce(const double *part, int nb, int nv, double *out) {
-  if (blockIdx.x != 0 || threadIdx.x >= (unsigned)nv) return;
-  double s = 0.0;
-  for (int b = 0; b < nb; ++b) s += part[nv * b + threadIdx.x];
-  out[threadIdx.x] = s;
+/* out[v] = sum over blocks of part[nv b + v], v < nv (one work-group of
+ * EW_BLOCK threads, fixed order) */
+__global__ __launch_bounds__(EW_BLOCK) void k_ls_reduce(const double *part, int nb, int nv,
+                                                        double *out) {
+  __shared__ double red[EW_BLOCK];
+  for (int v = 0; v < nv; ++v) {
+    const double t = ws_group_sum(part, nb, nv, v, red);
+    if (threadIdx.x == 0) out[v] = t;
+  }
 }
 
 /* zs = z + om4 (r - A z): one damped Jacobi sweep (z null: from z = 0) */
@@ -938,12 +1013,38 @@ static int ws_launch_conv(iskra_ws *w, const WsConv &c, int b, const uint16_t *X
     isk_set_error("warm start: conv shape %d->%d on %lld pixels not supported", cin, cout, npix);
     return ISKRA_E_ARG;
   }
-  const int tm = cout % 64 == 0 ? 64 : (cout % 32 == 0 ? 32 : 16);
-  dim3 grid((unsigned)(npix / 64), (unsigned)(cout / tm));
+  /* tile (TM output channels, NP pixel groups per wave): the first of the
+   * candidates, in order of fewer loads per MFMA, that gives at least
+   * WS_CONV_MIN_WG work-groups; else the one with the most */
+  static const int cand[6][2] = {{64, 4}, {32, 4}, {16, 4}, {64, 1}, {32, 1}, {16, 1}};
+  int tm = 16, np = 1;
+  long long best = -1;
+  for (int k = 0; k < 6; ++k) {
+    const int ct = cand[k][0], cn = cand[k][1];
+    if (cout % ct || npix % (64 * cn)) continue;
+    const long long wg = (npix / (64 * cn)) * (cout / ct);
+    if (wg >= WS_CONV_MIN_WG) {
+      tm = ct;
+      np = cn;
+      break;
+    }
+    if (wg > best) {
+      best = wg;
+      tm = ct;
+      np = cn;
+    }
+  }
+  dim3 grid((unsigned)(npix / (64 * np)), (unsigned)(cout / tm));
   hipStream_t st = w->ctx->stream;
-  if (tm == 64) hipLaunchKernelGGL(k_ws_conv<64>, grid, dim3(256), 0, st, a);
-  else if (tm == 32) hipLaunchKernelGGL(k_ws_conv<32>, grid, dim3(256), 0, st, a);
-  else hipLaunchKernelGGL(k_ws_conv<16>, grid, dim3(256), 0, st, a);
+  if (np == 4) {
+    if (tm == 64) hipLaunchKernelGGL((k_ws_conv<64, 4>), grid, dim3(256), 0, st, a);
+    else if (tm == 32) hipLaunchKernelGGL((k_ws_conv<32, 4>), grid, dim3(256), 0, st, a);
+    else hipLaunchKernelGGL((k_ws_conv<16, 4>), grid, dim3(256), 0, st, a);
+  } else {
+    if (tm == 64) hipLaunchKernelGGL((k_ws_conv<64, 1>), grid, dim3(256), 0, st, a);
+    else if (tm == 32) hipLaunchKernelGGL((k_ws_conv<32, 1>), grid, dim3(256), 0, st, a);
+    else hipLaunchKernelGGL((k_ws_conv<16, 1>), grid, dim3(256), 0, st, a);
+  }
   ISK_CHECK_LAUNCH();
   return ISKRA_OK;
 }
@@ -1018,7 +1119,7 @@ static int ws_forward(iskra_ws *w, int b) {
       const WsTensor &src = w->E[l - 1];
       const long long n = ws_npix(w, l, b) * src.C;
       const int Hs = ws_side(w, l - 1);
-      hipLaunchKernelGGL(k_ws_pool, dim3(ws_grid(n)), dim3(EW_BLOCK), 0, st,
+      hipLaunchKernelGGL(k_ws_pool, dim3(ws_grid(n >> 3)), dim3(EW_BLOCK), 0, st,
                          (const uint16_t *)src.a, w->P[l].a, Hs, Hs, src.C, n);
       ISK_CHECK_LAUNCH();
     }
@@ -1030,7 +1131,7 @@ static int ws_forward(iskra_ws *w, int b) {
     const WsTensor &src = ws_dec_out(w, l + 1);
     const long long n = ws_npix(w, l, b) * src.C;
     const int H = ws_side(w, l);
-    hipLaunchKernelGGL(k_ws_up, dim3(ws_grid(n)), dim3(EW_BLOCK), 0, st, (const uint16_t *)src.a,
+    hipLaunchKernelGGL(k_ws_up, dim3(ws_grid(n >> 3)), dim3(EW_BLOCK), 0, st, (const uint16_t *)src.a,
                        w->U[l].a, H, H, src.C, n);
     ISK_CHECK_LAUNCH();
     WRC(ws_fwd_conv(w, ws_dec(w, l, 0), b, w->U[l], w->E[l].a, w->Da[l].a, nullptr, 0));
@@ -1146,7 +1247,7 @@ static int ws_backward(iskra_ws *w, int b) {
     const WsTensor &dst = ws_dec_out(w, l + 1);
     const long long n = ws_npix(w, l + 1, b) * dst.C;
     const int H = ws_side(w, l);
-    hipLaunchKernelGGL(k_ws_up_bwd, dim3(ws_grid(n)), dim3(EW_BLOCK), 0, st,
+    hipLaunchKernelGGL(k_ws_up_bwd, dim3(ws_grid(n >> 2)), dim3(EW_BLOCK), 0, st,
                        (const float *)w->U[l].g, dst.g, H, H, dst.C, n);
     ISK_CHECK_LAUNCH();
   }
@@ -1161,7 +1262,7 @@ static int ws_backward(iskra_ws *w, int b) {
       const WsTensor &dst = w->E[l - 1];
       const long long n = ws_npix(w, l - 1, b) * dst.C;
       const int H = ws_side(w, l - 1);
-      hipLaunchKernelGGL(k_ws_pool_bwd, dim3(ws_grid(n)), dim3(EW_BLOCK), 0, st,
+      hipLaunchKernelGGL(k_ws_pool_bwd, dim3(ws_grid(n >> 2)), dim3(EW_BLOCK), 0, st,
                          (const float *)w->P[l].g, dst.g, H, H, dst.C, n);
       ISK_CHECK_LAUNCH();
     }
@@ -1237,7 +1338,7 @@ static int ws_rhs_scale(iskra_ws *w, double *fscale) {
   hipLaunchKernelGGL(k_ws_rhs_sumsq, dim3(nb), dim3(EW_BLOCK), 0, c->stream,
                      (const double *)c->rho, c->g.nx, *fscale, w->dpart);
   ISK_CHECK_LAUNCH();
-  hipLaunchKernelGGL(k_ws_rhs_scale, dim3(1), dim3(64), 0, c->stream, (const double *)w->dpart,
+  hipLaunchKernelGGL(k_ws_rhs_scale, dim3(1), dim3(EW_BLOCK), 0, c->stream, (const double *)w->dpart,
                      (int)nb, cnt, w->scal);
   ISK_CHECK_LAUNCH();
   return ISKRA_OK;
@@ -1636,7 +1737,7 @@ static int ls_precond(iskra_ws *w, int nx, double om4, int sweeps, const double 
   const unsigned nb = std::min(ws_grid(cnt), (unsigned)w->dpart_n);
   hipLaunchKernelGGL(k_ws_rhs_sumsq, dim3(nb), dim3(EW_BLOCK), 0, st, rin, nx, 1.0, w->dpart);
   ISK_CHECK_LAUNCH();
-  hipLaunchKernelGGL(k_ws_rhs_scale, dim3(1), dim3(64), 0, st, (const double *)w->dpart, (int)nb,
+  hipLaunchKernelGGL(k_ws_rhs_scale, dim3(1), dim3(EW_BLOCK), 0, st, (const double *)w->dpart, (int)nb,
                      cnt, w->scal);
   ISK_CHECK_LAUNCH();
   const long long S2 = (long long)w->S * w->S;
@@ -1719,7 +1820,7 @@ static int ls_capture_pair(iskra_ws *w) {
   hipLaunchKernelGGL(k_ws_rhs_sumsq, dim3(nb), dim3(EW_BLOCK), 0, c->stream,
                      (const double *)w->lr, nx, 1.0, w->dpart);
   ISK_CHECK_LAUNCH();
-  hipLaunchKernelGGL(k_ws_rhs_scale, dim3(1), dim3(64), 0, c->stream, (const double *)w->dpart,
+  hipLaunchKernelGGL(k_ws_rhs_scale, dim3(1), dim3(EW_BLOCK), 0, c->stream, (const double *)w->dpart,
                      (int)nb, cnt, w->scal);
   ISK_CHECK_LAUNCH();
   const long long S2 = (long long)w->S * w->S;
@@ -1746,7 +1847,7 @@ static int ls_iterate(iskra_ws *w, int warm, double om4, int sweeps, int maxit, 
                      warm == ISKRA_WARM_PREV ? (const double *)c->phi : (const double *)nullptr,
                      nx, fscale, w->lx, w->lr, w->lpart);
   ISK_CHECK_LAUNCH();
-  hipLaunchKernelGGL(k_ls_reduce, dim3(1), dim3(64), 0, c->stream, (const double *)w->lpart,
+  hipLaunchKernelGGL(k_ls_reduce, dim3(1), dim3(EW_BLOCK), 0, c->stream, (const double *)w->lpart,
                      nb, 2, w->ls + LS_FF);
   ISK_CHECK_LAUNCH();
   double s[2];
@@ -1764,7 +1865,7 @@ static int ls_iterate(iskra_ws *w, int warm, double om4, int sweeps, int maxit, 
       hipLaunchKernelGGL(k_ls_dot, dim3(nb), dim3(LS_BLOCK), 0, c->stream, z,
                          (const double *)w->lq[prev], nx, w->lpart);
       ISK_CHECK_LAUNCH();
-      hipLaunchKernelGGL(k_ls_reduce, dim3(1), dim3(64), 0, c->stream,
+      hipLaunchKernelGGL(k_ls_reduce, dim3(1), dim3(EW_BLOCK), 0, c->stream,
                          (const double *)w->lpart, nb, 1, w->ls + LS_ZQ);
       ISK_CHECK_LAUNCH();
     }
@@ -1776,14 +1877,14 @@ static int ls_iterate(iskra_ws *w, int warm, double om4, int sweeps, int maxit, 
                        (const double *)w->lp[cur], (const double *)w->lr, w->lq[cur], nx,
                        w->lpart);
     ISK_CHECK_LAUNCH();
-    hipLaunchKernelGGL(k_ls_reduce, dim3(1), dim3(64), 0, c->stream, (const double *)w->lpart,
+    hipLaunchKernelGGL(k_ls_reduce, dim3(1), dim3(EW_BLOCK), 0, c->stream, (const double *)w->lpart,
                        nb, 2, w->ls + LS_PQ);
     ISK_CHECK_LAUNCH();
     hipLaunchKernelGGL(k_ls_xr, dim3(nb), dim3(LS_BLOCK), 0, c->stream, w->lx, w->lr,
                        (const double *)w->lp[cur], (const double *)w->lq[cur], nx,
                        (const double *)w->ls, w->lpart);
     ISK_CHECK_LAUNCH();
-    hipLaunchKernelGGL(k_ls_reduce, dim3(1), dim3(64), 0, c->stream, (const double *)w->lpart,
+    hipLaunchKernelGGL(k_ls_reduce, dim3(1), dim3(EW_BLOCK), 0, c->stream, (const double *)w->lpart,
                        nb, 1, w->ls + LS_RR);
     ISK_CHECK_LAUNCH();
     if (caps && next_cap < ncaps && it == caps[next_cap]) {

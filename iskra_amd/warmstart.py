@@ -349,9 +349,36 @@ def forward_flops(levels, base, side):
     return f
 
 
+def inference_only(passes, side=512, levels=4, base=32, seed=0):
+    """`passes` forward passes of a He-initialised network on a random rho
+    (the kernels' own timing, e.g. under a profiler); returns the mean ms
+    per pass from the library's HIP events."""
+    ctx = Context(0)
+    ws = None
+    try:
+        ctx.grid(2, (side + 1, side + 1), (1.0, 1.0))
+        ws = WarmStart(ctx, levels=levels, base=base, side=side, max_batch=1)
+        ws.set_params(he_init(levels, base, seed))
+        ws.set_gain(1.0)
+        ctx.set_field(L.F_RHO, np.random.default_rng(seed).standard_normal((side + 1) ** 2))
+        ws.predict()
+        ctx.profile(True)
+        for _ in range(passes):
+            ws.predict()
+        st = ctx.stats()
+        ctx.profile(False)
+        return st["kernel_ms"]["warmstart"] / max(st["kernel_calls"]["warmstart"], 1)
+    finally:
+        if ws is not None:
+            ws.close()
+        ctx.close()
+
+
 def main(argv=None):
     import argparse
     ap = argparse.ArgumentParser(description="learned warm start: capture, train, evaluate")
+    ap.add_argument("--inference-only", type=int, default=0,
+                    help="only time this many forward passes of an untrained network")
     ap.add_argument("--side", type=int, default=512)
     ap.add_argument("--levels", type=int, default=4)
     ap.add_argument("--base", type=int, default=32)
@@ -369,6 +396,12 @@ def main(argv=None):
     ap.add_argument("--round-iters", default="1,2,4,8,16,32,64")
     ap.add_argument("--out", default=None)
     a = ap.parse_args(argv)
+    if a.inference_only:
+        ms = inference_only(a.inference_only, a.side, a.levels, a.base)
+        flops = forward_flops(a.levels, a.base, a.side)
+        print(json.dumps({"side": a.side, "levels": a.levels, "base": a.base,
+                          "forward_ms": ms, "tflops": flops / (ms * 1e-3) / 1e12}), flush=True)
+        return
     res = experiment(a.side, a.levels, a.base, a.runs, a.steps_per_run, a.train_steps, a.batch,
                      a.lr, log=lambda m: print(m, flush=True),
                      objective={"net": L.WS_ENERGY_NET, "cycle": L.WS_ENERGY,
