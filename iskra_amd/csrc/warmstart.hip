@@ -48,6 +48,9 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 #define WS_PART_BYTES (64ll << 20)
 #define EW_BLOCK 256
 #define WS_MAX_SWEEPS 16
+#ifndef WS_CONV_LDS
+#define WS_CONV_LDS 1 /* the LDS-staged convolution where its shape fits */
+#endif
 #ifndef WS_CONV_MIN_WG
 #define WS_CONV_MIN_WG 256 /* work-groups a conv tile should give (1 per CU) */
 #endif
@@ -167,6 +170,103 @@ __global__ __launch_bounds__(256) void k_ws_conv(ConvArgs a) {
         a.Yf[p[g]] = v[0];
       }
     }
+}
+
+/* The same convolution with the input staged in LDS: a work-group computes
+ * a 16 x 16 pixel tile x TM output channels (wave wv: rows 4 wv .. 4 wv + 3,
+ * one 16-pixel row per MFMA column block).  Per chunk of 32 input channels
+ * the 18 x 18 halo of the tile is staged once (pixel stride 40 channels, so
+ * the 16 pixels a quarter-wave reads fall in distinct banks), then the 9
+ * taps read their fragments from LDS.  Needs Cin % 32 == 0 and H, W
+ * multiples of 16; K is ordered (tap, ci) as in Wt. */
+#define WS_LDS_PIX 40
+template <int TM>
+__global__ __launch_bounds__(256) void k_ws_conv_lds(ConvArgs a) {
+  __shared__ __align__(16) uint16_t tile[18 * 18 * WS_LDS_PIX];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int col = lane & 15, grp = lane >> 4;
+  const int tw = a.W >> 4, th = a.H >> 4;
+  int t = blockIdx.x;
+  const int tc = t % tw;
+  t /= tw;
+  const int tr = t % th;
+  const int n = t / th;
+  const int h0 = tr * 16, w0 = tc * 16;
+  const int co0 = blockIdx.y * TM;
+  f32x4 acc[4][TM / 16];
+#pragma unroll
+  for (int g = 0; g < 4; ++g)
+#pragma unroll
+    for (int j = 0; j < TM / 16; ++j) acc[g][j] = (f32x4){0.f, 0.f, 0.f, 0.f};
+  for (int c0 = 0; c0 < a.Cin; c0 += 32) {
+    __syncthreads(); /* the previous chunk's fragments have been read */
+    for (int v = threadIdx.x; v < 18 * 18 * 4; v += 256) {
+      const int px = v >> 2, q = v & 3;
+      const int hy = px / 18, hx = px - hy * 18;
+      const int hh = h0 + hy - 1, ww = w0 + hx - 1;
+      u32x4 val = (u32x4){0u, 0u, 0u, 0u};
+      if (hh >= 0 && hh < a.H && ww >= 0 && ww < a.W)
+        val = *(const u32x4 *)(a.X + ((long long)(n * a.H + hh) * a.W + ww) * a.Cin + c0 + q * 8);
+      *(u32x4 *)(tile + px * WS_LDS_PIX + q * 8) = val;
+    }
+    __syncthreads();
+#pragma unroll 1
+    for (int tap = 0; tap < 9; ++tap) {
+      const int dy = tap / 3, dx = tap - (tap / 3) * 3;
+      const int k0 = tap * a.Cin + c0 + grp * 8;
+      u32x4 av[TM / 16];
+#pragma unroll
+      for (int j = 0; j < TM / 16; ++j)
+        av[j] = *(const u32x4 *)(a.Wt + (long long)(co0 + j * 16 + col) * a.Kpad + k0);
+#pragma unroll
+      for (int g = 0; g < 4; ++g) {
+        const int row = wv * 4 + g;
+        const u32x4 bv =
+            *(const u32x4 *)(tile + ((row + dy) * 18 + col + dx) * WS_LDS_PIX + grp * 8);
+#pragma unroll
+        for (int j = 0; j < TM / 16; ++j)
+          acc[g][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+              __builtin_bit_cast(bf16x8, av[j]), __builtin_bit_cast(bf16x8, bv), acc[g][j], 0, 0,
+              0);
+      }
+    }
+  }
+#pragma unroll
+  for (int g = 0; g < 4; ++g) {
+    const long long p = ((long long)(n * a.H + h0 + wv * 4 + g)) * a.W + w0 + col;
+#pragma unroll
+    for (int j = 0; j < TM / 16; ++j) {
+      const int cb = co0 + j * 16 + grp * 4;
+      float v[4] = {acc[g][j][0], acc[g][j][1], acc[g][j][2], acc[g][j][3]};
+      if (a.mode == 0) {
+        if (a.skip) {
+          const uint2 sk = *(const uint2 *)(a.skip + p * a.Cout + cb);
+          v[0] += ws_bf2f((uint16_t)(sk.x & 0xffffu));
+          v[1] += ws_bf2f((uint16_t)(sk.x >> 16));
+          v[2] += ws_bf2f((uint16_t)(sk.y & 0xffffu));
+          v[3] += ws_bf2f((uint16_t)(sk.y >> 16));
+        }
+        if (a.relu) {
+#pragma unroll
+          for (int r = 0; r < 4; ++r) v[r] = fmaxf(v[r], 0.f);
+        }
+        uint2 o;
+        o.x = (uint32_t)ws_f2bf(v[0]) | ((uint32_t)ws_f2bf(v[1]) << 16);
+        o.y = (uint32_t)ws_f2bf(v[2]) | ((uint32_t)ws_f2bf(v[3]) << 16);
+        *(uint2 *)(a.Y + p * a.Cout + cb) = o;
+      } else if (a.mode == 1) {
+        float4 *dst = (float4 *)(a.Yf + p * a.Cout + cb);
+        float4 d = *dst;
+        d.x += v[0];
+        d.y += v[1];
+        d.z += v[2];
+        d.w += v[3];
+        *dst = d;
+      } else if (cb == 0) {
+        a.Yf[p] = v[0];
+      }
+    }
+  }
 }
 
 /* ------------------------------------------------------------------------ */
@@ -1036,7 +1136,12 @@ static int ws_launch_conv(iskra_ws *w, const WsConv &c, int b, const uint16_t *X
   }
   dim3 grid((unsigned)(npix / (64 * np)), (unsigned)(cout / tm));
   hipStream_t st = w->ctx->stream;
-  if (np == 4) {
+  if (WS_CONV_LDS && np == 4 && cin % 32 == 0 && a.H % 16 == 0 && a.W % 16 == 0) {
+    /* the LDS-staged kernel: one 16 x 16 tile per work-group (npix / 256) */
+    if (tm == 64) hipLaunchKernelGGL((k_ws_conv_lds<64>), grid, dim3(256), 0, st, a);
+    else if (tm == 32) hipLaunchKernelGGL((k_ws_conv_lds<32>), grid, dim3(256), 0, st, a);
+    else hipLaunchKernelGGL((k_ws_conv_lds<16>), grid, dim3(256), 0, st, a);
+  } else if (np == 4) {
     if (tm == 64) hipLaunchKernelGGL((k_ws_conv<64, 4>), grid, dim3(256), 0, st, a);
     else if (tm == 32) hipLaunchKernelGGL((k_ws_conv<32, 4>), grid, dim3(256), 0, st, a);
     else hipLaunchKernelGGL((k_ws_conv<16, 4>), grid, dim3(256), 0, st, a);
