@@ -48,6 +48,9 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 #define WS_PART_BYTES (64ll << 20)
 #define EW_BLOCK 256
 #define WS_MAX_SWEEPS 16
+#ifndef WS_TAP_UNROLL
+#define WS_TAP_UNROLL 1
+#endif
 #ifndef WS_CONV_LDS
 #define WS_CONV_LDS 1 /* the LDS-staged convolution where its shape fits */
 #endif
@@ -210,7 +213,7 @@ __global__ __launch_bounds__(256) void k_ws_conv_lds(ConvArgs a) {
       *(u32x4 *)(tile + px * WS_LDS_PIX + q * 8) = val;
     }
     __syncthreads();
-#pragma unroll 1
+#pragma unroll WS_TAP_UNROLL
     for (int tap = 0; tap < 9; ++tap) {
       const int dy = tap / 3, dx = tap - (tap / 3) * 3;
       const int k0 = tap * a.Cin + c0 + grp * 8;
