@@ -601,25 +601,69 @@ struct CommitArgs {
   const long long *snap;   /* counts and next ids before the append */
 };
 
-/* work-groups of the commit kernel when the interaction ionises (ranking the
- * products is O(products^2), spread over them) */
+/* work-groups of the commit kernel when the interaction ionises (above
+ * MCC_SORT_MAX products the ranking is O(products^2), spread over them) */
 #define MCC_RANK_GROUPS 64
 #define MCC_RANK_TILE 1024
+/* threads per commit work-group; products ranked by one work-group's LDS
+ * bitonic sort when there are at most MCC_SORT_MAX of them */
+#define MCC_COMMIT_BLOCK 1024
+#define MCC_SORT_MAX 8192
+
+/* ids of product g at rank r */
+__device__ __forceinline__ void mcc_commit_ids(const CommitArgs &a, int g, long long rank) {
+  const long long enp = a.snap[0], enext = a.snap[1], inp = a.snap[2], inext = a.snap[3];
+  if (a.eid && enp + g < a.ecap) a.eid[enp + g] = (uint32_t)(enext + rank);
+  if (a.iid)
+    for (int r = 0; r < a.R; ++r) {
+      const long long pi = inp + (long long)g * a.R + r;
+      if (pi >= a.icap) break;
+      a.iid[pi] = (uint32_t)(inext + rank * a.R + r);
+    }
+}
 
 /* Ids of the products and the species headers.  Product g receives
  * next_id + rank(g), rank = the number of products whose parent id is below
  * its parent's (parent ids are unique, so ranks are 0..P-1), and its R ions
- * next_ion_id + rank R + r.  Work-group 0 then updates the headers; no
- * work-group reads them (they use the append kernel's snapshot). */
+ * next_ion_id + rank R + r.  Up to MCC_SORT_MAX products, work-group 0 sorts
+ * the keys (parent id, g) in LDS (bitonic) and the rank is the sorted
+ * position; above, every work-group counts smaller parent ids directly.
+ * Work-group 0 then updates the headers; no work-group reads them (they use
+ * the append kernel's snapshot). */
 __device__ __forceinline__ void mcc_commit_body(const CommitArgs &a, const int blk,
                                                 const int nblk_grid) {
-  __shared__ uint32_t tile[MCC_RANK_TILE];
+  __shared__ unsigned long long keys[MCC_SORT_MAX]; /* also the rank tile */
+  uint32_t *tile = (uint32_t *)keys;
   const long long tot = a.blk_off[a.nblk];
   if (tot == 0) return;
   const int P = (int)tot;
-  if (a.eid || (a.iid && a.R > 0)) {
-    const long long enp = a.snap[0], enext = a.snap[1], inp = a.snap[2], inext = a.snap[3];
-    for (int g0 = blk * (int)blockDim.x; g0 < P; g0 += nblk_grid * (int)blockDim.x) {
+  const int bd = (int)blockDim.x;
+  if ((a.eid || (a.iid && a.R > 0)) && P <= MCC_SORT_MAX) {
+    if (blk == 0) {
+      int n2 = 1;
+      while (n2 < P) n2 <<= 1;
+      for (int i = threadIdx.x; i < n2; i += bd)
+        keys[i] = i < P ? ((unsigned long long)a.pid_c[i] << 32) | (unsigned)i : ~0ull;
+      __syncthreads();
+      for (int k = 2; k <= n2; k <<= 1)
+        for (int j = k >> 1; j > 0; j >>= 1) {
+          for (int i = threadIdx.x; i < n2; i += bd) {
+            const int l = i ^ j;
+            if (l > i) {
+              const unsigned long long x = keys[i], y = keys[l];
+              if ((x > y) == ((i & k) == 0)) {
+                keys[i] = y;
+                keys[l] = x;
+              }
+            }
+          }
+          __syncthreads();
+        }
+      for (int r = threadIdx.x; r < P; r += bd)
+        mcc_commit_ids(a, (int)(keys[r] & 0xffffffffu), r);
+    }
+  } else if (a.eid || (a.iid && a.R > 0)) {
+    for (int g0 = blk * bd; g0 < P; g0 += nblk_grid * bd) {
       const int g = g0 + (int)threadIdx.x;
       const uint32_t mine = g < P ? a.pid_c[g] : 0u;
       int rank = 0;
@@ -630,15 +674,7 @@ __device__ __forceinline__ void mcc_commit_body(const CommitArgs &a, const int b
         __syncthreads();
         for (int q = 0; q < tn; ++q) rank += tile[q] < mine ? 1 : 0;
       }
-      if (g < P) {
-        if (a.eid && enp + g < a.ecap) a.eid[enp + g] = (uint32_t)(enext + rank);
-        if (a.iid)
-          for (int r = 0; r < a.R; ++r) {
-            const long long pi = inp + (long long)g * a.R + r;
-            if (pi >= a.icap) break;
-            a.iid[pi] = (uint32_t)(inext + (long long)rank * a.R + r);
-          }
-      }
+      if (g < P) mcc_commit_ids(a, g, rank);
     }
   }
   if (blk != 0 || threadIdx.x != 0) return;
@@ -660,7 +696,7 @@ __device__ __forceinline__ void mcc_commit_body(const CommitArgs &a, const int b
   }
 }
 
-__global__ __launch_bounds__(MCC_BLOCK) void k_mcc_commit(CommitArgs a) {
+__global__ __launch_bounds__(MCC_COMMIT_BLOCK) void k_mcc_commit(CommitArgs a) {
   mcc_commit_body(a, blockIdx.x, gridDim.x);
 }
 
@@ -902,7 +938,7 @@ int isk_mcc_run(iskra_ctx *c, int idx, double dt, int64_t *events) {
   hipLaunchKernelGGL(k_mcc_append, dim3(mcc_append_blocks(nblk)), dim3(MCC_BLOCK), 0,
                      c->stream, p);
   ISK_CHECK_LAUNCH();
-  hipLaunchKernelGGL(k_mcc_commit, dim3(mcc_commit_groups(m, nblk)), dim3(MCC_BLOCK), 0,
+  hipLaunchKernelGGL(k_mcc_commit, dim3(mcc_commit_groups(m, nblk)), dim3(MCC_COMMIT_BLOCK), 0,
                      c->stream, cm);
   ISK_CHECK_LAUNCH();
   if (p.ihdr) c->sp[m.ion].deposited = 0;
@@ -949,7 +985,7 @@ __global__ __launch_bounds__(MCC_BLOCK) void kb_mcc_append(const AppendArgs *A, 
   mcc_append_body(a, blockIdx.x - bstart[i]);
 }
 
-__global__ __launch_bounds__(MCC_BLOCK) void kb_mcc_commit(const CommitArgs *A,
+__global__ __launch_bounds__(MCC_COMMIT_BLOCK) void kb_mcc_commit(const CommitArgs *A,
                                                             const int *bstart, int n) {
   const int i = isk_bfind(bstart, n, blockIdx.x);
   const CommitArgs a = A[i];
@@ -985,7 +1021,7 @@ int isk_bbuild_mcc(iskra_ctx **cs, int n, int m, double dt, const int *cur_src,
                                MCC_BLOCK, 0)))
     return rc;
   return isk_blaunch_upload(&L[4], ISK_B_MCC_COMMIT, CM.data(), sizeof(CommitArgs), cblocks,
-                            MCC_BLOCK, 0);
+                            MCC_COMMIT_BLOCK, 0);
 }
 
 int isk_blaunch_mcc(const IskBLaunch &L, hipStream_t st) {
