@@ -13,7 +13,7 @@ device (Philox loader), so they are resident in HBM before the timed region.
 
 After the timed step, a reduced learned leg (513^2 nodes, the 4-level
 32-256-channel network, 1000 two-stream snapshots instead of 20,000 and
-1000 + 3 x 1000 training steps instead of 4000 + 3 x 1000; the full size is
+1000 steps and 4-8 rounds of iterate pairs, as at full size; the full size is
 `python -m iskra_amd.warmstart`) trains the network on the device as the
 coarse solver of the two-level cycle that preconditions a flexible CG
 (iskra_ws_solve), then compares, on held-out snapshots, Jacobi-PCG from

@@ -230,19 +230,23 @@ def solve_table(ctx, ws, snapshots, kinds=(L.POISSON_PCG, L.POISSON_MG), rtol=1e
 
 def experiment(side=512, levels=4, base=32, runs=8, steps_per_run=500, train_steps=1000,
                batch=8, lr=1e-3, held_steps=41, every=10, seed=0, log=print,
-               objective=L.WS_ENERGY_NET, omega=0.8, sweeps=2, rounds=3, round_snaps=200,
-               round_iters=(1, 2, 4, 8, 16, 32, 64), round_steps=1000, round_lr=3e-4,
-               round_keep=1000):
+               objective=L.WS_ENERGY_NET, omega=0.8, sweeps=2, rounds=4, round_snaps=100,
+               round_iters=(1, 2, 4, 8, 16, 32, 64, 128, 256), round_steps=1000, round_lr=3e-4,
+               round_keep=1000, max_rounds=8, val_snaps=3, val_iter=400):
     """Capture, train, evaluate; returns a dict of the measured figures.
 
     objective L.WS_ENERGY_NET or L.WS_ENERGY trains the network as the
     coarse solver of iskra_ws_solve's two-level cycle (`sweeps` Jacobi
-    sweeps of weight omega on either side); then `rounds` times: the pairs
-    of the solve's own iterates (`round_iters` iterations) on `round_snaps`
+    sweeps of weight omega on either side); then rounds: the pairs of the
+    solve's own iterates (`round_iters` iterations) on `round_snaps`
     training snapshots are added to the dataset and training continues
     (`round_steps` steps at `round_lr`) on the iterate pairs and the last
     `round_keep` snapshots, so the network learns the residuals it meets
-    inside the iteration without them being diluted by the snapshots.  The evaluation then includes that solve.
+    inside the iteration without them being diluted by the snapshots.  After
+    `rounds` rounds, each further round (up to `max_rounds`) is run only
+    while the learned solve of `val_snaps` training snapshots misses 1e-8
+    within `val_iter` iterations (a check on training data; the held-out
+    snapshots are never used).  The evaluation then includes that solve.
     L.WS_PHI_MSE trains a warm start only (no rounds).  The held-out
     snapshots come from a run the training never sees.  Everything runs on
     device 0 in one context, closed before returning."""
@@ -262,7 +266,7 @@ def experiment(side=512, levels=4, base=32, runs=8, steps_per_run=500, train_ste
         ws.set_params(he_init(levels, base, seed))
         ws.set_objective(objective, omega if energy else 0.0, sweeps if energy else 0)
         n = runs * steps_per_run
-        ws.reserve(held_steps + n + rounds * round_snaps * len(round_iters))
+        ws.reserve(held_steps + n + (max(rounds, max_rounds) if rounds else 0) * round_snaps * len(round_iters))
         held = []
         capture_two_stream(ws, runs=1, steps=held_steps, seed=seed * 100 + 99, record=held)
         pool = []
@@ -283,7 +287,23 @@ def experiment(side=512, levels=4, base=32, runs=8, steps_per_run=500, train_ste
         ws.set_gain(gain)
         pairs = 0
         rng = np.random.default_rng(seed + 17)
-        for rd in range(rounds):
+        val = [pool[j][0] for j in range(max(0, len(pool) - val_snaps), len(pool))]
+        tags = np.zeros((side + 1, side + 1), np.int8)
+        tags[0, :] = tags[-1, :] = tags[:, 0] = tags[:, -1] = 1
+
+        def validate():
+            ctx.poisson(L.POISSON_PCG, 1.0, tags.ravel(), rtol=1e-8, max_iter=100000)
+            ctx.tag_value(1, 0.0)
+            worst = 0.0
+            for rho in val:
+                ctx.set_field(L.F_RHO, rho)
+                _, rr = ws.solve(L.WARM_ZERO, omega, sweeps, max_iter=val_iter,
+                                 raise_unconverged=False)
+                worst = max(worst, rr)
+            return worst
+
+        done_rounds = 0
+        for rd in range(max_rounds if rounds else 0):
             pick = rng.choice(len(pool), min(round_snaps, len(pool)), replace=False)
             pairs += capture_iterate_pairs(ws, [pool[j][0] for j in pick], round_iters, omega,
                                            sweeps)
@@ -294,10 +314,18 @@ def experiment(side=512, levels=4, base=32, runs=8, steps_per_run=500, train_ste
                 c = min(max(1, round_steps // 5), round_steps - done)
                 ws.train(first, k, batch, c, round_lr, seed=seed + 7919 * (rd + 1) + done)
                 done += c
-            say(f"round {rd + 1}: {pairs} iterate pairs, {round_steps} more steps")
+            done_rounds = rd + 1
+            if done_rounds >= rounds:
+                worst = validate()
+                say(f"round {done_rounds}: {pairs} iterate pairs, {round_steps} more steps; "
+                    f"training-snapshot check: worst relres {worst:.2e} after {val_iter} iterations")
+                if worst <= 1e-8:
+                    break
+            else:
+                say(f"round {done_rounds}: {pairs} iterate pairs, {round_steps} more steps")
         ctx.sync()
         tr_s = time.perf_counter() - t
-        all_steps = train_steps + rounds * round_steps
+        all_steps = train_steps + done_rounds * round_steps
         e1 = ws.eval(0, held_steps)
         say(f"held-out loss {e0:.4e} -> {e1:.4e}")
         ctx.set_field(L.F_RHO, held[0][0])
@@ -324,7 +352,7 @@ def experiment(side=512, levels=4, base=32, runs=8, steps_per_run=500, train_ste
         flops = forward_flops(levels, base, side)
         out = dict(side=side, levels=levels, base=base, params=ws.nparam, snapshots=n,
                    capture_s=cap_s, train_steps=all_steps, batch=batch, train_s=tr_s,
-                   rounds=rounds, iterate_pairs=pairs,
+                   rounds=done_rounds, iterate_pairs=pairs,
                    train_ms_per_step=1e3 * tr_s / max(all_steps, 1),
                    objective=objname, heldout_loss_before=e0, heldout_loss_after=e1, gain=gain,
                    inference_ms=inf_ms, inference_tflops=flops / (inf_ms * 1e-3) / 1e12,
@@ -390,10 +418,11 @@ def main(argv=None):
     ap.add_argument("--objective", choices=("net", "cycle", "mse"), default="net")
     ap.add_argument("--omega", type=float, default=0.8)
     ap.add_argument("--sweeps", type=int, default=2)
-    ap.add_argument("--rounds", type=int, default=3)
-    ap.add_argument("--round-snaps", type=int, default=200)
+    ap.add_argument("--rounds", type=int, default=4)
+    ap.add_argument("--round-snaps", type=int, default=100)
     ap.add_argument("--round-steps", type=int, default=1000)
-    ap.add_argument("--round-iters", default="1,2,4,8,16,32,64")
+    ap.add_argument("--round-iters", default="1,2,4,8,16,32,64,128,256")
+    ap.add_argument("--max-rounds", type=int, default=8)
     ap.add_argument("--out", default=None)
     a = ap.parse_args(argv)
     if a.inference_only:
@@ -408,7 +437,8 @@ def main(argv=None):
                                 "mse": L.WS_PHI_MSE}[a.objective],
                      omega=a.omega, sweeps=a.sweeps, rounds=a.rounds, round_snaps=a.round_snaps,
                      round_steps=a.round_steps,
-                     round_iters=tuple(int(v) for v in a.round_iters.split(",")))
+                     round_iters=tuple(int(v) for v in a.round_iters.split(",")),
+                     max_rounds=a.max_rounds)
     s = json.dumps(res)
     print(s, flush=True)
     if a.out:
