@@ -33,7 +33,8 @@ enum iskra_status {
   ISKRA_E_PROB = 4,        /* MCC probability assertion (Chemistry/src/mcc.jl:244-246,273-279) */
   ISKRA_E_STATE = 5,       /* call out of order (no grid / solver / species) */
   ISKRA_E_CONVERGE = 6,    /* Poisson solve did not reach its tolerance */
-  ISKRA_E_UNSUPPORTED = 7  /* configuration this build does not implement */
+  ISKRA_E_UNSUPPORTED = 7, /* configuration this build does not implement */
+  ISKRA_E_IO = 8           /* file output failed (the openPMD writer, iskra_pmd.h) */
 };
 
 typedef struct iskra_ctx iskra_ctx;

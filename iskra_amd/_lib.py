@@ -14,7 +14,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(HERE, "libiskra_hip.so")
 
-OK, E_ARG, E_HIP, E_CAPACITY, E_PROB, E_STATE, E_CONVERGE, E_UNSUPPORTED = range(8)
+OK, E_ARG, E_HIP, E_CAPACITY, E_PROB, E_STATE, E_CONVERGE, E_UNSUPPORTED, E_IO = range(9)
 BND_NONE, BND_WRAP, BND_DISCARD = 0, 1, 2
 F_RHO, F_PHI, F_EX, F_EY, F_BG, F_DENSITY, F_TARGET = range(7)
 POISSON_DIRECT, POISSON_PCG, POISSON_MG, POISSON_MGPCG, POISSON_SPECTRAL = range(5)

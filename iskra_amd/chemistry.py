@@ -147,10 +147,23 @@ class MonteCarloCollisions:
         n = np.broadcast_to(tgt.n, grid.n)
         ctx.set_field(L.F_TARGET, np.ascontiguousarray(n.T).ravel(), index=self._id)
         self._ctx = ctx
+        self._acc = np.zeros((len(self.collisions), ctx.nn), dtype=np.int64)
 
     def counts(self, reset=False):
-        """nu per node and process (mcc.jl:283-288), accumulated on the device."""
-        return self._ctx.mcc_counts(self._id, len(self.collisions), reset)
+        """nu per process and node (mcc.jl:283-288), shape (processes, nodes),
+        summed over the steps since binding (or since the last reset)."""
+        tot = self._ctx.mcc_counts(self._id, len(self.collisions), reset) + self._acc
+        if reset:
+            self._acc[:] = 0
+        return tot
+
+    def _step_begin(self):
+        """Move the device counts into the running sum, so that the device
+        counts hold the next step's events only (the reference's nu)."""
+        self._acc += self._ctx.mcc_counts(self._id, len(self.collisions), True)
+
+    def _last_step(self):
+        return self._ctx.mcc_counts(self._id, len(self.collisions), False)
 
 
 def _accept(r):
@@ -226,11 +239,21 @@ class DirectSimulationMonteCarlo:
                                "DSMC.IonizationCollision")
         self._id = ctx.dsmc(c.source._sid, c.target._sid, c.rate.nodes[:, 0], c.rate.nodes[:, 1])
         self._ctx = ctx
+        self._acc = np.zeros(ctx.nn, dtype=np.int64)
 
     def counts(self, reset=False):
-        """nuDSMC per node (the cell's lower-left node), accumulated on the
-        device since the last reset."""
-        return self._ctx.dsmc_counts(self._id, reset)
+        """nuDSMC per node (the cell's lower-left node), summed over the steps
+        since binding (or since the last reset)."""
+        tot = self._ctx.dsmc_counts(self._id, reset) + self._acc
+        if reset:
+            self._acc[:] = 0
+        return tot
+
+    def _step_begin(self):
+        self._acc += self._ctx.dsmc_counts(self._id, True)
+
+    def _last_step(self):
+        return self._ctx.dsmc_counts(self._id, False)
 
 
 def dsmc(reaction_list):

@@ -25,6 +25,7 @@ import math
 import numpy as np
 
 from . import _lib as L
+from . import diagnostics
 from .finite_difference_method import PoissonSolver
 from .regular_grids import UniformGrid
 
@@ -466,6 +467,7 @@ def bind(config):
         if isinstance(inter, (chemistry.MonteCarloCollisions,
                               chemistry.DirectSimulationMonteCarlo)):
             inter._bind(ctx, g)
+    diagnostics.register_simulation(config)
     return ctx
 
 
@@ -585,7 +587,9 @@ def solve(config, dt=1e-5, timesteps=200, seed=None):
     of every kinetic species (gather, push, after_push boundary, deposit),
     rho = sum q n (+ q n of fluid species), phi from the Poisson solve, E,
     then after_loop(it, it*dt - dt, dt).  When after_loop is the default and
-    there are no per-step sources, the iterations run as one device call."""
+    there are no per-step sources, the iterations run as one device call (two
+    when collision counts or fluid records need the last step's start, see
+    iskra_amd.diagnostics)."""
     if seed is not None:
         config.seed = int(seed)
     ctx = bind(config)
@@ -594,13 +598,20 @@ def solve(config, dt=1e-5, timesteps=200, seed=None):
     dt = float(dt)
     fused = after_loop is _noop_after_loop and not config.sources
     if fused:
-        ctx.step(dt, int(timesteps))
+        n = int(timesteps)
+        if n > 1 and diagnostics.needs_step_begin(config):
+            ctx.step(dt, n - 1)
+            n = 1
+        diagnostics.step_begin(config, dt)
+        if n > 0:
+            ctx.step(dt, n)
         _mark_device_newer(config)
     else:
         for it in range(1, int(timesteps) + 1):
             for src in config.sources:
                 init(src, src.species, dt)
             _sync_to_device(config)
+            diagnostics.step_begin(config, dt)
             ctx.step(dt, 1)
             _mark_device_newer(config)
             after_loop(it, it * dt - dt, dt)
