@@ -434,6 +434,9 @@ typedef struct iskra_batch iskra_batch;
 int iskra_batch_create(iskra_ctx **ctxs, int n, iskra_batch **batch);
 int iskra_batch_destroy(iskra_batch *batch);
 int iskra_batch_set_graph(iskra_batch *batch, int enable);
+/* ISKRA_E_UNSUPPORTED for a one-member batch when a rocprofiler-sdk tool is
+ * loaded (ROCP_TOOL_LIBRARIES set): that case ended once in a segmentation
+ * fault inside the HIP runtime; profile the context's iskra_step instead. */
 int iskra_batch_step(iskra_batch *batch, double dt, int64_t nsteps);
 /* graph replays issued so far (each one 2 x sort_every steps) */
 int iskra_batch_graphs_launched(iskra_batch *batch, int64_t *count);

@@ -17,6 +17,7 @@
  * There is no counterpart in the reference, which runs one simulation per
  * process (ParticleInCell/src/ParticleInCell.jl:84-139).
  */
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
@@ -319,8 +320,18 @@ int iskra_batch_step(iskra_batch *b, double dt, int64_t nsteps) {
     isk_set_error("batch_step: dt must be positive");
     return ISKRA_E_ARG;
   }
-  ISK_HIP(hipSetDevice(b->device));
   const int n = (int)b->cs.size();
+  /* A one-member batch stepped under a rocprofiler-sdk tool (rocprofv3 sets
+   * ROCP_TOOL_LIBRARIES) ended once in a segmentation fault inside the HIP
+   * runtime, below this call, with no cause found in this library; it is
+   * refused there.  Profile the member's own iskra_step instead (same
+   * kernels); batches of several members are not affected. */
+  if (n == 1 && getenv("ROCP_TOOL_LIBRARIES")) {
+    isk_set_error("batch_step: a one-member batch is not stepped under a rocprofiler-sdk "
+                  "tool; profile the context's own iskra_step");
+    return ISKRA_E_UNSUPPORTED;
+  }
+  ISK_HIP(hipSetDevice(b->device));
   for (int i = 0; i < n; ++i) ISK_HIP(hipStreamSynchronize(b->cs[i]->stream));
   if (nsteps == 0) return ISKRA_OK;
   /* descriptors are rebuilt per call: buffers, tunings and scratch of the

@@ -61,3 +61,22 @@ def test_batch_members_equal_single_runs(gpu_ready):
     E.close_all(ctxs, b)
     E.close_all(ctxs2, b2)
     E.close_all(singles)
+
+
+def test_one_member_batch_is_refused_under_a_profiler_tool(gpu_ready, monkeypatch):
+    """iskra_batch_step refuses a one-member batch while a rocprofiler-sdk
+    tool is loaded (ROCP_TOOL_LIBRARIES set) before it queues any work; the
+    same batch steps normally without it."""
+    from iskra_amd import _lib as L
+    from iskra_amd import ensemble as E
+    from iskra_amd._lib import Batch
+    ctxs, dt = E.build(_insts()[:1], TUNINGS[:1], sort_every=8)
+    b = Batch(ctxs)
+    monkeypatch.setenv("ROCP_TOOL_LIBRARIES", "set-by-the-test")
+    with pytest.raises(L.IskraError) as e:
+        b.step(dt, 4)
+    assert e.value.code == L.E_UNSUPPORTED
+    assert ctxs[0].stats()["steps"] == 0
+    monkeypatch.delenv("ROCP_TOOL_LIBRARIES")
+    b.step(dt, 4)
+    assert ctxs[0].stats()["steps"] == 4
