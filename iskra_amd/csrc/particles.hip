@@ -1150,7 +1150,10 @@ __device__ __forceinline__ void push_body(const PushArgs &a, const int blk) {
       }
       if (a.do_push) {
         const unsigned k = (unsigned)b0 + u * BLK + threadIdx.x;
-        cb[k] = alive ? (i | (j << 16)) : -1;
+        /* the cell is stored only when it changed: slow particles (the ions)
+         * then leave whole sectors of the cell array unwritten */
+        const int32_t nc = alive ? (i | (j << 16)) : -1;
+        if (nc != c) cb[k] = nc;
         oxb[k] = hx;
         vxb[k] = vx;
         if (DIM == 2) {
