@@ -187,7 +187,9 @@ enum {
   ISKRA_F_EY = 3,       /* E_y at nodes (stored FP32) */
   ISKRA_F_BG = 4,       /* fixed background charge density (fluid species) */
   ISKRA_F_DENSITY = 5,  /* number density of species `index` (download only) */
-  ISKRA_F_TARGET = 6    /* MCC target gas density of interaction `index` */
+  ISKRA_F_TARGET = 6,   /* MCC target gas density of interaction `index` */
+  ISKRA_F_FLUID_N = 7,  /* density n of mobile fluid `index` (iskra_fluid_add) */
+  ISKRA_F_FLUID_DN = 8  /* its change dn in the last advance (download only) */
 };
 int iskra_field_download(iskra_ctx *ctx, int which, int index, double *out);
 int iskra_field_upload(iskra_ctx *ctx, int which, int index, const double *in);
@@ -352,6 +354,23 @@ int iskra_circuit_set(iskra_ctx *ctx, double R, double L, double C, double v_off
                       int negative);
 /* the circuit's current, capacitor charge, time and last external voltage */
 int iskra_circuit_state(iskra_ctx *ctx, double *i, double *q, double *t, double *vext);
+
+/* ---- fluid species --------------------------------------------------------- */
+/* A mobile FluidSpecies (pic/fluid.jl:1-11, problem/configuration.jl:104-108):
+ * charge q, mass m, mobility mu (the diffusion coefficient of
+ * calculate_advection_diffusion) and node densities n (nodes, x fastest).
+ * Its charge q n enters rho with the background (ISKRA_F_BG, which then
+ * names the fixed part of the background only).  iskra_step advances every
+ * fluid once per step before the deposit merge, with the E the step starts
+ * from: advance!(::FluidSpecies) (ParticleInCell.jl:74-82), v = (E dt)(q/m),
+ * dn = calculate_advection_diffusion(n, mu, v, dh, dt)
+ * (FiniteDifferenceMethod/src/advection_diffusion.jl:4-38), n += dn.  A
+ * fluid whose density never changes can stay in ISKRA_F_BG instead.  Not
+ * available in batches. */
+int iskra_fluid_add(iskra_ctx *ctx, double q, double m, double mu, const double *n, int *fid);
+/* The same update of every fluid once, outside the step (advance! of the
+ * fluids alone, with the current E). */
+int iskra_fluid_advance(iskra_ctx *ctx, double dt);
 
 /* ---- random numbers ------------------------------------------------------ */
 /* The seed of all device draws (Random.seed!, src/iskra.jl:22-26). */

@@ -16,7 +16,7 @@ LIBPATH = os.path.join(HERE, "libiskra_hip.so")
 
 OK, E_ARG, E_HIP, E_CAPACITY, E_PROB, E_STATE, E_CONVERGE, E_UNSUPPORTED, E_IO = range(9)
 BND_NONE, BND_WRAP, BND_DISCARD = 0, 1, 2
-F_RHO, F_PHI, F_EX, F_EY, F_BG, F_DENSITY, F_TARGET = range(7)
+F_RHO, F_PHI, F_EX, F_EY, F_BG, F_DENSITY, F_TARGET, F_FLUID_N, F_FLUID_DN = range(9)
 POISSON_DIRECT, POISSON_PCG, POISSON_MG, POISSON_MGPCG, POISSON_SPECTRAL = range(5)
 PUSHER_XY, PUSHER_RZ = 0, 1
 SURF_ABSORBING, SURF_REFLECTIVE, SURF_PERIODIC = 1, 2, 3
@@ -31,7 +31,7 @@ EXPORTS = [
     "iskra_grid_set", "iskra_grid_set_axial", "iskra_pusher_set", "iskra_dsmc_add",
     "iskra_dsmc_run", "iskra_dsmc_counts", "iskra_tracker_create", "iskra_tracker_surface",
     "iskra_tracker_table", "iskra_electrode_create", "iskra_electrode_state",
-    "iskra_circuit_set", "iskra_circuit_state", "iskra_tuning_set", "iskra_tuning_get", "iskra_species_add",
+    "iskra_circuit_set", "iskra_circuit_state", "iskra_fluid_add", "iskra_fluid_advance", "iskra_tuning_set", "iskra_tuning_get", "iskra_species_add",
     "iskra_species_set_boundary", "iskra_species_sort_every", "iskra_species_margin",
     "iskra_species_upload",
     "iskra_species_download",
@@ -108,6 +108,8 @@ def lib():
             "iskra_electrode_state": (I, [P, I, P, P, P, P]),
             "iskra_circuit_set": (I, [P, D, D, D, D, D, D, D, I, I]),
             "iskra_circuit_state": (I, [P, P, P, P, P]),
+            "iskra_fluid_add": (I, [P, D, D, D, P, C.POINTER(I)]),
+            "iskra_fluid_advance": (I, [P, D]),
             "iskra_tuning_set": (I, [P, C.POINTER(Tuning)]),
             "iskra_tuning_get": (I, [P, C.POINTER(Tuning)]),
             "iskra_species_add": (I, [P, D, D, D, I64, I, C.POINTER(I)]),
@@ -497,6 +499,17 @@ class Context:
         check(lib().iskra_circuit_set(self.h, float(R), float(L), float(C_), float(v_offset),
                                       float(v_amplitude), float(v_freq), float(v_phase),
                                       int(positive), int(negative)))
+
+    def fluid(self, q, m, mu, n):
+        """A mobile fluid species (iskra_fluid_add): node densities n, x fastest."""
+        v = np.ascontiguousarray(n, dtype=np.float64).ravel()
+        assert v.size == self.nn
+        out = C.c_int()
+        check(lib().iskra_fluid_add(self.h, float(q), float(m), float(mu), _p(v), C.byref(out)))
+        return out.value
+
+    def fluid_advance(self, dt):
+        check(lib().iskra_fluid_advance(self.h, float(dt)))
 
     def circuit_state(self):
         v = [C.c_double() for _ in range(4)]

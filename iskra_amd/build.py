@@ -19,7 +19,7 @@ LIB = os.path.join(HERE, "libiskra_hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 SOURCES = ["api.hip", "particles.hip", "poisson.hip", "mcc.hip", "dsmc.hip", "electrode.hip",
-           "warmstart.hip", "batch.hip"]
+           "fluid.hip", "warmstart.hip", "batch.hip"]
 
 FLAGS = [
     "--offload-arch=gfx950",

@@ -317,6 +317,7 @@ static void release_fields(iskra_ctx *c) {
   for (int m = 0; m < c->ndsmc; ++m) isk_dsmc_release(&c->dsmc[m]);
   c->ndsmc = 0;
   isk_electrodes_release(c);
+  isk_fluid_release(c);
   isk_poisson_free(c);
   c->ps.kind = -1;
   void *ptrs[] = {c->rho, c->phi, c->bg, c->vol, c->E, c->sort_H, c->sort_S, c->sort_tot,
@@ -908,7 +909,18 @@ int iskra_field_download(iskra_ctx *c, int which, int index, double *out) {
   switch (which) {
     case ISKRA_F_RHO: ISK_HIP(hipMemcpy(out, c->rho, nn * 8, hipMemcpyDeviceToHost)); break;
     case ISKRA_F_PHI: ISK_HIP(hipMemcpy(out, c->phi, nn * 8, hipMemcpyDeviceToHost)); break;
-    case ISKRA_F_BG: ISK_HIP(hipMemcpy(out, c->bg, nn * 8, hipMemcpyDeviceToHost)); break;
+    case ISKRA_F_BG:
+      ISK_HIP(hipMemcpy(out, c->nfluid ? c->bg_fixed : c->bg, nn * 8, hipMemcpyDeviceToHost));
+      break;
+    case ISKRA_F_FLUID_N:
+    case ISKRA_F_FLUID_DN:
+      if (index < 0 || index >= c->nfluid) {
+        isk_set_error("field_download: fluid %d of %d", index, c->nfluid);
+        return ISKRA_E_ARG;
+      }
+      ISK_HIP(hipMemcpy(out, which == ISKRA_F_FLUID_N ? c->fl[index].n : c->fl[index].dn,
+                        nn * 8, hipMemcpyDeviceToHost));
+      break;
     case ISKRA_F_EX:
     case ISKRA_F_EY: {
       if (which == ISKRA_F_EY && dim == 1) {
@@ -958,7 +970,20 @@ int iskra_field_upload(iskra_ctx *c, int which, int index, const double *in) {
   switch (which) {
     case ISKRA_F_RHO: ISK_HIP(hipMemcpy(c->rho, in, nn * 8, hipMemcpyHostToDevice)); break;
     case ISKRA_F_PHI: ISK_HIP(hipMemcpy(c->phi, in, nn * 8, hipMemcpyHostToDevice)); break;
-    case ISKRA_F_BG: ISK_HIP(hipMemcpy(c->bg, in, nn * 8, hipMemcpyHostToDevice)); break;
+    case ISKRA_F_BG:
+      ISK_HIP(hipMemcpy(c->nfluid ? c->bg_fixed : c->bg, in, nn * 8, hipMemcpyHostToDevice));
+      RC(isk_fluid_bg(c));
+      ISK_HIP(hipStreamSynchronize(c->stream));
+      break;
+    case ISKRA_F_FLUID_N:
+      if (index < 0 || index >= c->nfluid) {
+        isk_set_error("field_upload: fluid %d of %d", index, c->nfluid);
+        return ISKRA_E_ARG;
+      }
+      ISK_HIP(hipMemcpy(c->fl[index].n, in, nn * 8, hipMemcpyHostToDevice));
+      RC(isk_fluid_bg(c));
+      ISK_HIP(hipStreamSynchronize(c->stream));
+      break;
     case ISKRA_F_EX:
     case ISKRA_F_EY: {
       if (which == ISKRA_F_EY && dim == 1) {
@@ -1245,6 +1270,51 @@ __global__ void k_tag_waveform(double *tagval, int tag, double off, double amp, 
   tagval[tag] = off + amp * s;
 }
 
+/* ---- fluid species ---- */
+int iskra_fluid_add(iskra_ctx *c, double q, double m, double mu, const double *n, int *fid) {
+  CHECK_CTX(c);
+  CHECK_GRID(c);
+  if (!n || !fid || !(m > 0.0) || !isfinite(q) || !isfinite(mu) || !isfinite(m)) {
+    isk_set_error("fluid_add: need n, fid, finite q and mu, and m > 0");
+    return ISKRA_E_ARG;
+  }
+  if (c->nfluid >= ISK_MAX_FLUIDS) {
+    isk_set_error("fluid_add: at most %d fluid species", ISK_MAX_FLUIDS);
+    return ISKRA_E_UNSUPPORTED;
+  }
+  const int nn = isk_nodes(c);
+  ISK_HIP(hipStreamSynchronize(c->stream));
+  if (!c->nfluid) {
+    /* the background set so far becomes the fixed part */
+    if (isk_alloc((void **)&c->bg_fixed, (size_t)nn * 8)) return ISKRA_E_HIP;
+    ISK_HIP(hipMemcpy(c->bg_fixed, c->bg, (size_t)nn * 8, hipMemcpyDeviceToDevice));
+  }
+  IskFluid &F = c->fl[c->nfluid];
+  if (isk_alloc((void **)&F.n, (size_t)nn * 8) || isk_alloc((void **)&F.dn, (size_t)nn * 8))
+    return ISKRA_E_HIP;
+  ISK_HIP(hipMemcpy(F.n, n, (size_t)nn * 8, hipMemcpyHostToDevice));
+  ISK_HIP(hipMemset(F.dn, 0, (size_t)nn * 8));
+  F.q = q;
+  F.qm = q / m;
+  F.mu = mu;
+  *fid = c->nfluid++;
+  RC(isk_fluid_bg(c));
+  ISK_HIP(hipStreamSynchronize(c->stream));
+  return ISKRA_OK;
+}
+
+int iskra_fluid_advance(iskra_ctx *c, double dt) {
+  CHECK_CTX(c);
+  CHECK_GRID(c);
+  if (!(dt > 0)) {
+    isk_set_error("fluid_advance: dt must be positive");
+    return ISKRA_E_ARG;
+  }
+  RC(isk_fluid_step(c, dt));
+  ISK_HIP(hipStreamSynchronize(c->stream));
+  return ISKRA_OK;
+}
+
 int iskra_step(iskra_ctx *c, double dt, int64_t nsteps) {
   CHECK_CTX(c);
   CHECK_GRID(c);
@@ -1264,6 +1334,8 @@ int iskra_step(iskra_ctx *c, double dt, int64_t nsteps) {
       RC(isk_prof_end(c, ev, ISKRA_K_MCC, 0));
     }
     for (int m = 0; m < c->ndsmc; ++m) RC(isk_dsmc_run(c, m, dt, nullptr));
+    /* fluids before the deposit merge adds their charge, with the previous E */
+    RC(isk_fluid_step(c, dt));
     RC(isk_push_deposit_all(c, dt));
     if (c->d_elec) RC(isk_electrodes_after_push(c, dt));
     int iters = 0;

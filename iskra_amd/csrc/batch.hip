@@ -253,6 +253,10 @@ int iskra_batch_create(iskra_ctx **ctxs, int n, iskra_batch **out) {
                     i);
       return ISKRA_E_UNSUPPORTED;
     }
+    if (c->nfluid) {
+      isk_set_error("batch_create: instance %d has mobile fluid species", i);
+      return ISKRA_E_UNSUPPORTED;
+    }
     if (c->device != c0->device || c->nspecies != c0->nspecies || c->nmcc != c0->nmcc ||
         c->t.sort_every != c0->t.sort_every || c->step_count != c0->step_count) {
       isk_set_error("batch_create: instance %d differs from instance 0 in device, species or "

@@ -31,6 +31,7 @@
 #define ISK_MAX_SPECIES 8
 #define ISK_MAX_MCC 4
 #define ISK_MAX_DSMC 4
+#define ISK_MAX_FLUIDS 4
 /* floating electrodes (each one sigma degree of freedom) */
 #define ISK_MAX_ELECTRODES 8
 
@@ -222,6 +223,11 @@ struct IskPoisson {
   double *d_dg = nullptr;            /* row diagonals (Jacobi) */
 };
 
+struct IskFluid {
+  double *n = nullptr, *dn = nullptr;
+  double q = 0.0, qm = 0.0, mu = 0.0;
+};
+
 struct iskra_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -237,6 +243,11 @@ struct iskra_ctx {
   int ndsmc = 0;
   IskDsmc dsmc[ISK_MAX_DSMC];
   IskTracker trk;
+  /* mobile fluid species (advance!(::FluidSpecies)): density and the last
+   * step's change per node; bg then holds bg_fixed + sum q n */
+  int nfluid = 0;
+  IskFluid fl[ISK_MAX_FLUIDS];
+  double *bg_fixed = nullptr;
   int nelec = 0;
   IskElectrode elec[ISK_MAX_ELECTRODES];
   int nfloat = 0;              /* floating electrodes = sigma degrees of freedom */
@@ -374,6 +385,10 @@ int isk_dsmc_add(iskra_ctx *c, int src, int tgt, int ntab, const double *tab_g,
 int isk_dsmc_run(iskra_ctx *c, int idx, double dt, int64_t *events);
 int isk_dsmc_counts(iskra_ctx *c, int idx, int reset, int64_t *out);
 void isk_dsmc_release(IskDsmc *m);
+/* fluid species (fluid.hip) */
+int isk_fluid_step(iskra_ctx *c, double dt);
+int isk_fluid_bg(iskra_ctx *c);
+void isk_fluid_release(iskra_ctx *c);
 int isk_noalloc_init(int device);
 int isk_rng_export_noalloc(iskra_ctx *c, uint32_t stream, uint32_t step,
                            const uint32_t *ids, int64_t n, int nblocks, uint32_t *out);

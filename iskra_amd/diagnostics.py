@@ -26,9 +26,9 @@ file holds is what the reference's records hold after the step:
   - nuMCC-<species>-<k> and nuDSMC count the events of the last step (the
     loop restarts the device counts at every step's start; after a fused
     solve, of its last step);
-  - the fluid records hold dn = 0 (fluids are static on the device) and
-    v = E dt q/m with the E of the step's start, as the reference's
-    advance! computes them (ParticleInCell.jl:74-82).
+  - the fluid records hold n, the last step's dn (zero for a fluid the
+    update cannot change) and v = E dt q/m with the E of the step's start,
+    as the reference's advance! computes them (ParticleInCell.jl:74-82).
 Particles are written in the device's order; the id record pairs them with
 the reference's.  Records a script registers itself with `field`,
 `particle` or `probe` hold a copy of the array passed, as the reference's
@@ -373,8 +373,18 @@ def register_simulation(config):
         records["n" + part.name] = FieldRecord(
             "1/m^2", g, (lambda p: lambda: ctx.field(L.F_DENSITY, p._sid)[None, :])(part))
     for f in (s for s in config.species if isinstance(s, FluidSpecies)):
-        zero = np.zeros((1, ctx.nn))
-        records["d" + f.name] = FieldRecord("1/m^2", g, lambda z=zero: z)
+        # density(fluid) = n enters the density loop's records too (ParticleInCell.jl:118-124)
+        if f._ctx is not None:  # advanced on the device
+            records["n" + f.name] = FieldRecord(
+                "1/m^2", g, (lambda f: lambda: ctx.field(L.F_FLUID_N, f._fid)[None, :])(f))
+            records["d" + f.name] = FieldRecord(
+                "1/m^2", g, (lambda f: lambda: ctx.field(L.F_FLUID_DN, f._fid)[None, :])(f))
+        else:
+            nconst = np.ascontiguousarray(
+                np.broadcast_to(f._n, g.n), dtype=np.float64).T.ravel()[None, :]
+            records["n" + f.name] = FieldRecord("1/m^2", g, lambda c=nconst: c)
+            zero = np.zeros((1, ctx.nn))
+            records["d" + f.name] = FieldRecord("1/m^2", g, lambda z=zero: z)
         def vel(f=f):
             E = state.E_start if state.E_start is not None else np.zeros((3, ctx.nn))
             return E * (state.dt * (f.q / f.m))

@@ -126,23 +126,75 @@ class KineticSpecies:
 
 
 class FluidSpecies:
-    """FluidSpecies{D} (pic/fluid.jl:1-11): a background with density n (per
-    node), mobility mu, charge q, mass m, temperature T.  It enters rho as
-    q*n (no volume division, fluid.jl:11) and is the MCC target gas.  Its
-    density is held constant, which is what the reference's
-    advection-diffusion update gives for the fluids `bind` accepts
-    (advection_diffusion.jl:4-38); others are refused there."""
+    """FluidSpecies{D} (pic/fluid.jl:1-11): density n (per node), mobility
+    mu, charge q, mass m, temperature T.  It enters rho as q*n (no volume
+    division, fluid.jl:11) and may be the MCC target gas.  A fluid that the
+    reference's update cannot change (mu = 0, and in 2-D also q = 0:
+    advection_diffusion.jl:4-38) is a constant background on the device;
+    any other is advanced there every step (iskra_fluid_add), and n is then
+    a host mirror of the device density, downloaded when read and uploaded
+    before the next device call when changed, as the kinetic species'
+    arrays are."""
 
     def __init__(self, name, mu, q, m, n, T):
         self.name = name
         self.mu = float(mu)
         self.q = float(q)
         self.m = float(m)
-        self.n = np.asarray(n, dtype=np.float64)
+        self._n = np.array(n, dtype=np.float64)
         self.T = float(T)
+        self._ctx = None
+        self._fid = -1
+        self._grid_n = None
+        self._host_stale = False
+        self._dev_stale = False
 
     def __repr__(self):
         return self.name
+
+    def _pull(self):
+        if self._ctx is not None and self._host_stale:
+            self._n[...] = self._ctx.field(L.F_FLUID_N, self._fid).reshape(
+                self._grid_n[::-1]).T
+            self._host_stale = False
+
+    def _push(self):
+        if self._ctx is not None and self._dev_stale:
+            self._ctx.set_field(L.F_FLUID_N, np.ascontiguousarray(self._n.T).ravel(),
+                                index=self._fid)
+            self._dev_stale = False
+
+    @property
+    def n(self):
+        if self._ctx is not None:
+            self._pull()
+            self._dev_stale = True
+        return self._n
+
+    @n.setter
+    def n(self, val):
+        if self._ctx is not None:
+            self._pull()
+            self._dev_stale = True
+            self._n[...] = val
+        else:
+            self._n = np.array(val, dtype=np.float64)
+
+
+def create_fluid_species(name, mu, q, m, mx, my=None, T=300.0):
+    """create_fluid_species (problem/configuration.jl:104-108): zero density
+    on mx (x my) nodes."""
+    return FluidSpecies(name, mu, q, m, np.zeros((mx,) if my is None else (mx, my)), T)
+
+
+class DensitySource:
+    """DensitySource{D} (pic/sources.jl:3-6): init adds delta (per node) to
+    a fluid's density (sample!, sources.jl:36-38)."""
+
+    def __init__(self, delta, grid):
+        self.delta = np.asarray(delta, dtype=np.float64)
+        self.grid = grid
+        self.species = None
 
 
 class MaxwellianSource:
@@ -385,6 +437,12 @@ def _kinetic_species(config):
     return [s for s in config.species if isinstance(s, KineticSpecies)]
 
 
+def _fluid_can_change(f, g):
+    """The reference's update can change this fluid's density: it diffuses
+    (mu != 0) or, in 2-D, is advected by E (q != 0)."""
+    return f.mu != 0.0 or (g.dim == 2 and f.q != 0.0)
+
+
 def _fluid_is_static(f, g):
     """True when the reference's fluid update leaves n unchanged at every step
     (advance!(::FluidSpecies), ParticleInCell.jl:74-82, with
@@ -411,22 +469,30 @@ def _fluid_is_static(f, g):
 def bind(config):
     """Create the device context for `config` and move grid, solver,
     species, fluids and interactions onto it.  Called by `solve`; callable
-    directly to use the operators below outside the loop.  Fluid species are
-    held constant on the device, so a fluid whose density the reference would
-    update (a mobile fluid with non-uniform density, or a charged fluid on a
-    2-D grid) is refused with ISKRA_E_UNSUPPORTED before anything runs."""
+    directly to use the operators below outside the loop.  Fluids the
+    reference's update cannot change join the constant background; the
+    others are advanced on the device every step.  An MCC target gas whose
+    density the update would change is refused with ISKRA_E_UNSUPPORTED
+    (the collisions read a fixed target density)."""
     from ._lib import Context, IskraError
     from . import chemistry
 
     g = config.grid
     if not isinstance(g, UniformGrid):
         raise TypeError("config.grid must come from create_uniform_grid")
+    targets = {id(i.target) for i in config.interactions
+               if isinstance(i, chemistry.MonteCarloCollisions)}
+    mobile = []
     for f in _fluid_species(config):
-        if not _fluid_is_static(f, g):
-            raise IskraError(L.E_UNSUPPORTED,
-                             f"{f.name}: the fluid's density would change every step "
-                             "(advection_diffusion.jl:4-38); only fluids the reference "
-                             "leaves constant are supported")
+        f._ctx = None
+        if id(f) in targets:
+            if not _fluid_is_static(f, g):
+                raise IskraError(L.E_UNSUPPORTED,
+                                 f"{f.name}: an MCC target gas whose density would change "
+                                 "every step (advection_diffusion.jl:4-38); the collisions "
+                                 "read a fixed target density")
+        elif _fluid_can_change(f, g):
+            mobile.append(f)
     ctx = Context(config.device)
     _grid_to(ctx, g)
     if config.pusher is not None and getattr(config.pusher, "kind", "xy") == "rz":
@@ -454,8 +520,15 @@ def bind(config):
         s._push()
     bg = np.zeros(g.n)
     for f in _fluid_species(config):
-        bg += f.q * np.broadcast_to(f.n, g.n)
+        if f not in mobile:
+            bg += f.q * np.broadcast_to(f.n, g.n)
     ctx.set_field(L.F_BG, bg.T.ravel())
+    for f in mobile:
+        f._n = np.array(np.broadcast_to(f._n, g.n), dtype=np.float64)
+        f._grid_n = tuple(g.n)
+        f._fid = ctx.fluid(f.q, f.m, f.mu, np.ascontiguousarray(f._n.T).ravel())
+        f._ctx = ctx
+        f._host_stale = f._dev_stale = False
     if isinstance(config.solver, PoissonSolver):
         config.solver._ctx = ctx
         config.solver._configure(ctx)
@@ -484,12 +557,18 @@ def _grid_to(ctx, g):
 def _sync_to_device(config):
     for s in _kinetic_species(config):
         s._push()
+    for f in _fluid_species(config):
+        f._push()
 
 
 def _mark_device_newer(config):
     for s in _kinetic_species(config):
         s._host_stale = True
         s._dev_stale = False
+    for f in _fluid_species(config):
+        if f._ctx is not None:
+            f._host_stale = True
+            f._dev_stale = False
 
 
 # ---------------------------------------------------------------------------
@@ -502,6 +581,13 @@ def init(src, species, dt):
     count).  The draw stream is (species << 24) | call count: species below
     64 and up to 2^24 calls per species keep every call's stream distinct,
     and below the MCC streams (0x4D43xxxx)."""
+    if isinstance(src, DensitySource):
+        # sample!(::DensitySource, ::FluidSpecies) (sources.jl:36-38): n .+= delta
+        if not isinstance(species, FluidSpecies):
+            raise TypeError("init: a DensitySource feeds a FluidSpecies")
+        n = species.n
+        n += np.broadcast_to(src.delta, n.shape)
+        return
     if species._ctx is None:
         raise RuntimeError("init: the species is not bound to a simulation yet (call it from "
                            "enter_loop, or bind(config) first)")

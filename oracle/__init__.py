@@ -886,6 +886,36 @@ class MirSim:
             self.end_step(dt)
 
 
+def ref_advection_diffusion(n, D, v, dh, dt):
+    """calculate_advection_diffusion(n, D, v, dh, dt)
+    (FiniteDifferenceMethod/src/advection_diffusion.jl:4-38) in FP64, each
+    expression evaluated as the reference writes it (left to right, 2dx as
+    2*dx, dx^2 as dx*dx).  n: (nx,) or (nx, ny) indexed [i, j] like the
+    reference's arrays; v: (nx, 3) or (nx, ny, 3); returns dn of n's shape,
+    zero on the boundary nodes.  1-D: diffusion only (the advection lines
+    are commented out there, :13-15)."""
+    n = np.asarray(n, dtype=np.float64)
+    d = np.zeros_like(n)
+    if n.ndim == 1:
+        dx = dh[0]
+        if len(n) >= 3:
+            d[1:-1] = 0.0 + ((dt * D) * ((n[:-2] - 2.0 * n[1:-1]) + n[2:])) / (dx * dx)
+        return d
+    dx, dy = dh[0], dh[1]
+    if n.shape[0] < 3 or n.shape[1] < 3:
+        return d
+    vx, vy = v[..., 0], v[..., 1]
+    c = (slice(1, -1), slice(1, -1))
+    L, R = (slice(None, -2), slice(1, -1)), (slice(2, None), slice(1, -1))
+    Dn, Up = (slice(1, -1), slice(None, -2)), (slice(1, -1), slice(2, None))
+    lap = (((n[L] + n[Dn]) - 4.0 * n[c]) + n[R]) + n[Up]
+    t = 0.0 + ((dt * D) * lap) / (dx * dy)
+    t = t - ((dt * n[c]) * (((vx[R] - vx[L]) + vy[Up]) - vy[Dn])) / (2.0 * dx)
+    t = t - (dt * ((vx[c] * (n[R] - n[L])) + (vy[c] * (n[Up] - n[Dn])))) / dx
+    d[c] = t
+    return d
+
+
 class RefSim:
     """The reference's time loop in FP64 (ParticleInCell.jl:84-139) built from
     the ref_* restatements, with the reference's data model: absolute FP64
@@ -913,6 +943,7 @@ class RefSim:
         self.phi = np.zeros(nn)
         self.E = np.zeros(3 * nn)  # component-major, as ref_efield writes it
         self.species = []
+        self.fluids = []
         self.pinv = None
         self.tags = None
         self.tagval = np.zeros(17)
@@ -938,6 +969,26 @@ class RefSim:
             x=np.ascontiguousarray(x.T), v=np.ascontiguousarray(np.asarray(v, np.float64).T),
             wg=np.full(n, float(w0)), id=np.arange(1, n + 1, dtype=np.uint32)))
         return len(self.species) - 1
+
+    def add_fluid(self, q, m, mu, n):
+        """A FluidSpecies (pic/fluid.jl:1-11) with node densities n (nodes,
+        x fastest), advanced every step by advance!(::FluidSpecies)
+        (ParticleInCell.jl:74-82)."""
+        self.fluids.append(dict(q=q, m=m, mu=mu, n=np.array(n, dtype=np.float64).ravel(),
+                                dn=np.zeros(self.nx * self.ny)))
+        return len(self.fluids) - 1
+
+    def _advance_fluid(self, fl, dt):
+        """v = E dt (q/m) with the E the step starts from, n += dn"""
+        nn, sh = self.nx * self.ny, (self.ny, self.nx)
+        E = self.E.reshape(3, nn)
+        v = np.stack([(E[k].reshape(sh).T * dt) * (fl["q"] / fl["m"]) for k in range(3)], axis=-1)
+        n = fl["n"].reshape(sh).T
+        if self.dim == 1:
+            n, v = n[:, 0], v[:, 0, :]
+        dn = ref_advection_diffusion(n, fl["mu"], v, (self.dx, self.dy), dt)
+        fl["dn"] = np.ascontiguousarray(dn.reshape(self.nx, -1).T).ravel()
+        fl["n"] = fl["n"] + fl["dn"]
 
     def _advance_species(self, sp, dt):
         D, n = self.dim, sp["np"]
@@ -983,9 +1034,13 @@ class RefSim:
         for _ in range(nsteps):
             for sp in self.species:
                 self._advance_species(sp, dt)
+            for fl in self.fluids:
+                self._advance_fluid(fl, dt)
             rho = np.zeros(self.nx * self.ny)
             for s, sp in enumerate(self.species):
                 rho += sp["q"] * self.density(s)
+            for fl in self.fluids:
+                rho += fl["q"] * fl["n"]
             self.rho = rho + self.bg
             if self.pinv is not None:
                 self.phi = self.pinv @ (-self.rho / self.eps0)

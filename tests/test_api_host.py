@@ -146,11 +146,13 @@ def test_bench_refuses_more_than_one_gpu():
     assert "one GPU" in r.stderr and r.stdout == ""
 
 
-def test_fluids_the_reference_would_update_are_refused():
+def test_fluid_classification_and_moving_target_gas_refused():
     """advection_diffusion.jl:4-38 (called by advance!(::FluidSpecies),
     ParticleInCell.jl:74-82) leaves n unchanged for a uniform 1-D fluid, any
-    fluid with mu = 0 in 1-D, and a neutral uniform 2-D fluid; every other
-    fluid would change and is refused before any device work."""
+    fluid with mu = 0 in 1-D, and a neutral uniform 2-D fluid.  Fluids whose
+    parameters let the update change them (mu != 0, or q != 0 in 2-D) are
+    advanced on the device, except an MCC target gas, which must stay
+    constant and is refused before any device work when it would not."""
     g1 = rg.create_uniform_grid(np.linspace(0, 1, 17))
     g2 = rg.create_uniform_grid(np.linspace(0, 1, 9), np.linspace(0, 1, 9))
     g2q = rg.create_uniform_grid(np.linspace(0, 1, 9), np.linspace(0, 1, 2))
@@ -167,10 +169,17 @@ def test_fluids_the_reference_would_update_are_refused():
         assert pic._fluid_is_static(f, g), f.name
     for g, f in moving:
         assert not pic._fluid_is_static(f, g), f.name
-        cfg = pic.Config(grid=g, species=[f])
+        assert pic._fluid_can_change(f, g), f.name
+        e = pic.create_kinetic_species("e-", 10, -1.0, 1.0, 1.0, D=g.dim)
+        sig = ch.CrossSection(np.array([0.0, 10.0]), np.array([1e-20, 1e-20]))
+        coll = ch.mcc(ch.reactions([(sig, [e, f], [e, f])]))
+        cfg = pic.Config(grid=g, species=[e, f], interactions=[coll])
         with pytest.raises(L.IskraError) as ei:
             pic.bind(cfg)
         assert ei.value.code == L.E_UNSUPPORTED and cfg.ctx is None
+    assert not pic._fluid_can_change(pic.FluidSpecies("bg", 0.0, 1.0, 1.0, ramp, 300.0), g1)
+    assert not pic._fluid_can_change(pic.FluidSpecies("gas", 0.0, 0.0, 1.0, 0.0, 300.0), g2)
+    assert pic._fluid_can_change(pic.FluidSpecies("ions", 0.0, 1.0, 1.0, 0.0, 300.0), g2)
 
 
 def test_axial_grid_pusher_and_solver_host_side(orc):
