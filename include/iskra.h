@@ -269,6 +269,12 @@ int iskra_mcc_run(iskra_ctx *ctx, int mcc_id, double dt, int64_t *events);
 /* Per-node event counts nu[node][k] accumulated since the last reset
  * (mcc.jl:283-288), out: nodes x n_coll col-major. */
 int iskra_mcc_counts(iskra_ctx *ctx, int mcc_id, int reset, int64_t *out);
+/* The same counts for the last step of the latest iskra_step call only (the
+ * reference's nu, which perform! recomputes every step): iskra_step sets
+ * the earlier steps' counts apart before it runs its last step.  Events of
+ * iskra_mcc_run calls made since then are included; batch steps do not set
+ * counts apart. */
+int iskra_mcc_counts_last(iskra_ctx *ctx, int mcc_id, int64_t *out);
 
 /* ---- direct simulation Monte Carlo ------------------------------------- */
 /* dsmc(@reactions ...) with one DSMC.ElasticCollision
@@ -294,6 +300,9 @@ int iskra_dsmc_run(iskra_ctx *ctx, int dsmc_id, double dt, int64_t *events);
  * node (the reference's nuDSMC field, dsmc.jl:124-141): out has one entry
  * per node (nodes x 1, col-major). */
 int iskra_dsmc_counts(iskra_ctx *ctx, int dsmc_id, int reset, int64_t *out);
+/* The counts of the last step of the latest iskra_step call (see
+ * iskra_mcc_counts_last; dsmc.jl:124-141). */
+int iskra_dsmc_counts_last(iskra_ctx *ctx, int dsmc_id, int64_t *out);
 
 /* ---- surfaces, electrodes and the external circuit --------------------- */
 /* Surfaces of the cell-edge tracker (ParticleInCell/src/pic/surfaces/

@@ -40,7 +40,7 @@ EXPORTS = [
     "iskra_push", "iskra_deposit_species", "iskra_deposit", "iskra_field_download",
     "iskra_field_upload", "iskra_poisson_set", "iskra_poisson_set_tag_value",
     "iskra_poisson_set_tag_waveform", "iskra_poisson_solve", "iskra_efield", "iskra_mcc_add",
-    "iskra_mcc_run", "iskra_mcc_counts", "iskra_rng_set", "iskra_rng_export",
+    "iskra_mcc_run", "iskra_mcc_counts", "iskra_mcc_counts_last", "iskra_dsmc_counts_last", "iskra_rng_set", "iskra_rng_export",
     "iskra_rng_export_noalloc", "iskra_step",
     "iskra_stats_get", "iskra_profile", "iskra_bench_copy",
     "iskra_ws_create", "iskra_ws_destroy", "iskra_ws_param_count", "iskra_ws_set_params",
@@ -137,6 +137,8 @@ def lib():
             "iskra_mcc_add": (I, [P, I, I, P, P, P, P, P, I, D, D, D, C.POINTER(I)]),
             "iskra_mcc_run": (I, [P, I, D, P]),
             "iskra_mcc_counts": (I, [P, I, I, P]),
+            "iskra_mcc_counts_last": (I, [P, I, P]),
+            "iskra_dsmc_counts_last": (I, [P, I, P]),
             "iskra_rng_set": (I, [P, U64]),
             "iskra_rng_export": (I, [P, U32, U32, P, I64, I, P]),
             "iskra_rng_export_noalloc": (I, [P, U32, U32, P, I64, I, P]),
@@ -445,6 +447,12 @@ class Context:
         check(lib().iskra_mcc_counts(self.h, mid, int(reset), _p(out)))
         return out.reshape(ncoll, self.nn)
 
+    def mcc_counts_last(self, mid, ncoll):
+        """the counts of the last step of the latest step call"""
+        out = np.zeros(self.nn * ncoll, dtype=np.int64)
+        check(lib().iskra_mcc_counts_last(self.h, mid, _p(out)))
+        return out.reshape(ncoll, self.nn)
+
     # RNG
     def dsmc(self, src, tgt, tab_g, tab_sigma):
         """One DSMC.ElasticCollision interaction; returns its id."""
@@ -463,6 +471,11 @@ class Context:
     def dsmc_counts(self, did, reset=False):
         out = np.zeros(self.nn, dtype=np.int64)
         check(lib().iskra_dsmc_counts(self.h, int(did), int(reset), _p(out)))
+        return out
+
+    def dsmc_counts_last(self, did):
+        out = np.zeros(self.nn, dtype=np.int64)
+        check(lib().iskra_dsmc_counts_last(self.h, int(did), _p(out)))
         return out
 
     # surfaces, electrodes, circuit

@@ -147,23 +147,15 @@ class MonteCarloCollisions:
         n = np.broadcast_to(tgt.n, grid.n)
         ctx.set_field(L.F_TARGET, np.ascontiguousarray(n.T).ravel(), index=self._id)
         self._ctx = ctx
-        self._acc = np.zeros((len(self.collisions), ctx.nn), dtype=np.int64)
 
     def counts(self, reset=False):
         """nu per process and node (mcc.jl:283-288), shape (processes, nodes),
         summed over the steps since binding (or since the last reset)."""
-        tot = self._ctx.mcc_counts(self._id, len(self.collisions), reset) + self._acc
-        if reset:
-            self._acc[:] = 0
-        return tot
-
-    def _step_begin(self):
-        """Move the device counts into the running sum, so that the device
-        counts hold the next step's events only (the reference's nu)."""
-        self._acc += self._ctx.mcc_counts(self._id, len(self.collisions), True)
+        return self._ctx.mcc_counts(self._id, len(self.collisions), reset)
 
     def _last_step(self):
-        return self._ctx.mcc_counts(self._id, len(self.collisions), False)
+        """the reference's nu: the events of the last step"""
+        return self._ctx.mcc_counts_last(self._id, len(self.collisions))
 
 
 def _accept(r):
@@ -239,21 +231,15 @@ class DirectSimulationMonteCarlo:
                                "DSMC.IonizationCollision")
         self._id = ctx.dsmc(c.source._sid, c.target._sid, c.rate.nodes[:, 0], c.rate.nodes[:, 1])
         self._ctx = ctx
-        self._acc = np.zeros(ctx.nn, dtype=np.int64)
 
     def counts(self, reset=False):
         """nuDSMC per node (the cell's lower-left node), summed over the steps
         since binding (or since the last reset)."""
-        tot = self._ctx.dsmc_counts(self._id, reset) + self._acc
-        if reset:
-            self._acc[:] = 0
-        return tot
-
-    def _step_begin(self):
-        self._acc += self._ctx.dsmc_counts(self._id, True)
+        return self._ctx.dsmc_counts(self._id, reset)
 
     def _last_step(self):
-        return self._ctx.dsmc_counts(self._id, False)
+        """the reference's nuDSMC: the events of the last step"""
+        return self._ctx.dsmc_counts_last(self._id)
 
 
 def dsmc(reaction_list):

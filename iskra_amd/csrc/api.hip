@@ -305,7 +305,7 @@ static void release_fields(iskra_ctx *c) {
   c->nspecies = 0;
   for (int m = 0; m < c->nmcc; ++m) {
     IskMcc &q = c->mcc[m];
-    void *ptrs[] = {q.d_eps, q.d_sig, q.d_off, q.target_n, q.nu, q.events, q.errflag,
+    void *ptrs[] = {q.d_eps, q.d_sig, q.d_off, q.target_n, q.nu, q.nu_acc, q.events, q.errflag,
                     q.prod_flag, q.stage, q.blk_cnt, q.cand_i, q.cand_cnt,
                     q.stage_pid, q.pid_c, q.snap};
     for (void *p : ptrs) isk_free(p);
@@ -1181,7 +1181,9 @@ int iskra_mcc_run(iskra_ctx *c, int mcc_id, double dt, int64_t *events) {
   return iskra_sync(c);
 }
 
-int iskra_mcc_counts(iskra_ctx *c, int mcc_id, int reset, int64_t *out) {
+/* mode 0: running sum + live counts; 1: the same, then both cleared;
+ * 2: the live counts alone (since the last fold) */
+static int mcc_counts_mode(iskra_ctx *c, int mcc_id, int mode, int64_t *out) {
   CHECK_CTX(c);
   if (mcc_id < 0 || mcc_id >= c->nmcc) {
     isk_set_error("mcc id %d out of range", mcc_id);
@@ -1190,12 +1192,53 @@ int iskra_mcc_counts(iskra_ctx *c, int mcc_id, int reset, int64_t *out) {
   IskMcc &m = c->mcc[mcc_id];
   int nn = isk_nodes(c);
   ISK_HIP(hipStreamSynchronize(c->stream));
-  std::vector<unsigned long long> nu((size_t)nn * m.ncoll);
+  std::vector<unsigned long long> nu((size_t)nn * m.ncoll), acc(nu.size(), 0ull);
   ISK_HIP(hipMemcpy(nu.data(), m.nu, nu.size() * 8, hipMemcpyDeviceToHost));
+  if (mode != 2) ISK_HIP(hipMemcpy(acc.data(), m.nu_acc, acc.size() * 8, hipMemcpyDeviceToHost));
   if (out)
     for (int n = 0; n < nn; ++n)
-      for (int k = 0; k < m.ncoll; ++k) out[(size_t)k * nn + n] = (int64_t)nu[(size_t)n * m.ncoll + k];
-  if (reset) ISK_HIP(hipMemset(m.nu, 0, nu.size() * 8));
+      for (int k = 0; k < m.ncoll; ++k) {
+        const size_t q = (size_t)n * m.ncoll + k;
+        out[(size_t)k * nn + n] = (int64_t)(nu[q] + acc[q]);
+      }
+  if (mode == 1) {
+    ISK_HIP(hipMemset(m.nu, 0, nu.size() * 8));
+    ISK_HIP(hipMemset(m.nu_acc, 0, nu.size() * 8));
+  }
+  return ISKRA_OK;
+}
+
+int iskra_mcc_counts(iskra_ctx *c, int mcc_id, int reset, int64_t *out) {
+  return mcc_counts_mode(c, mcc_id, reset ? 1 : 0, out);
+}
+
+int iskra_mcc_counts_last(iskra_ctx *c, int mcc_id, int64_t *out) {
+  if (!out) return ISKRA_E_ARG;
+  return mcc_counts_mode(c, mcc_id, 2, out);
+}
+
+/* move every interaction's event counts into its running sum */
+__global__ void k_counts_fold(unsigned long long *acc, unsigned long long *nu, int n) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) {
+    acc[k] += nu[k];
+    nu[k] = 0ull;
+  }
+}
+
+static int isk_counts_fold(iskra_ctx *c) {
+  const int nn = isk_nodes(c);
+  for (int m = 0; m < c->nmcc; ++m) {
+    const int n = nn * c->mcc[m].ncoll;
+    hipLaunchKernelGGL(k_counts_fold, dim3((n + 255) / 256), dim3(256), 0, c->stream,
+                       c->mcc[m].nu_acc, c->mcc[m].nu, n);
+    ISK_CHECK_LAUNCH();
+  }
+  for (int m = 0; m < c->ndsmc; ++m) {
+    hipLaunchKernelGGL(k_counts_fold, dim3((nn + 255) / 256), dim3(256), 0, c->stream,
+                       c->dsmc[m].nu_acc, c->dsmc[m].nu, nn);
+    ISK_CHECK_LAUNCH();
+  }
   return ISKRA_OK;
 }
 
@@ -1233,7 +1276,16 @@ int iskra_dsmc_counts(iskra_ctx *c, int dsmc_id, int reset, int64_t *out) {
     isk_set_error("dsmc id %d out of range", dsmc_id);
     return ISKRA_E_ARG;
   }
-  return isk_dsmc_counts(c, dsmc_id, reset, out);
+  return isk_dsmc_counts(c, dsmc_id, reset ? 1 : 0, out);
+}
+
+int iskra_dsmc_counts_last(iskra_ctx *c, int dsmc_id, int64_t *out) {
+  CHECK_CTX(c);
+  if (dsmc_id < 0 || dsmc_id >= c->ndsmc || !out) {
+    isk_set_error("dsmc id %d out of range or NULL out", dsmc_id);
+    return ISKRA_E_ARG;
+  }
+  return isk_dsmc_counts(c, dsmc_id, 2, out);
 }
 
 /* ---- RNG ---- */
@@ -1328,6 +1380,8 @@ int iskra_step(iskra_ctx *c, double dt, int64_t nsteps) {
   }
   for (int64_t it = 0; it < nsteps; ++it) {
     hipEvent_t ev = nullptr;
+    /* the counts of the call's last step stay apart (iskra_*_counts_last) */
+    if (it == nsteps - 1 && (c->nmcc || c->ndsmc)) RC(isk_counts_fold(c));
     if (c->nmcc) {
       RC(isk_prof_begin(c, &ev));
       for (int m = 0; m < c->nmcc; ++m) RC(isk_mcc_run(c, m, dt, nullptr));

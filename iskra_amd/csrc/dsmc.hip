@@ -212,12 +212,14 @@ int isk_dsmc_add(iskra_ctx *c, int src, int tgt, int ntab, const double *tab_g,
   const int ncell = c->g.ncx * c->g.ncy, nn = isk_nodes(c);
   if (isk_alloc((void **)&m.d_g, ntab * 8) || isk_alloc((void **)&m.d_sig, ntab * 8) ||
       isk_alloc((void **)&m.rem, (size_t)ncell * 8) || isk_alloc((void **)&m.nu, (size_t)nn * 8) ||
+      isk_alloc((void **)&m.nu_acc, (size_t)nn * 8) ||
       isk_alloc((void **)&m.events, 8) || isk_alloc((void **)&m.rng, (size_t)ncell * 4 * 4))
     return ISKRA_E_HIP;
   ISK_HIP(hipMemcpy(m.d_g, tab_g, ntab * 8, hipMemcpyHostToDevice));
   ISK_HIP(hipMemcpy(m.d_sig, tab_sig, ntab * 8, hipMemcpyHostToDevice));
   ISK_HIP(hipMemset(m.rem, 0, (size_t)ncell * 8));
   ISK_HIP(hipMemset(m.nu, 0, (size_t)nn * 8));
+  ISK_HIP(hipMemset(m.nu_acc, 0, (size_t)nn * 8));
   ISK_HIP(hipMemset(m.events, 0, 8));
   m.step = 0;
   m.stream = 0x44530000u + (uint32_t)idx;
@@ -227,7 +229,7 @@ int isk_dsmc_add(iskra_ctx *c, int src, int tgt, int ntab, const double *tab_g,
 }
 
 void isk_dsmc_release(IskDsmc *m) {
-  void *p[] = {m->d_g, m->d_sig, m->rem, m->nu, m->events, m->rng};
+  void *p[] = {m->d_g, m->d_sig, m->rem, m->nu, m->nu_acc, m->events, m->rng};
   for (void *q : p) isk_free(q);
   *m = IskDsmc();
 }
@@ -316,14 +318,20 @@ int isk_dsmc_run(iskra_ctx *c, int idx, double dt, int64_t *events) {
   return ISKRA_OK;
 }
 
+/* reset 0: running sum + live counts; 1: the same, then both cleared; 2: the
+ * live counts alone (since the last fold) */
 int isk_dsmc_counts(iskra_ctx *c, int idx, int reset, int64_t *out) {
   IskDsmc &m = c->dsmc[idx];
   const int nn = isk_nodes(c);
   ISK_HIP(hipStreamSynchronize(c->stream));
-  std::vector<unsigned long long> nu(nn);
+  std::vector<unsigned long long> nu(nn), acc(nn);
   ISK_HIP(hipMemcpy(nu.data(), m.nu, (size_t)nn * 8, hipMemcpyDeviceToHost));
+  if (reset != 2) ISK_HIP(hipMemcpy(acc.data(), m.nu_acc, (size_t)nn * 8, hipMemcpyDeviceToHost));
   if (out)
-    for (int n = 0; n < nn; ++n) out[n] = (int64_t)nu[n];
-  if (reset) ISK_HIP(hipMemset(m.nu, 0, (size_t)nn * 8));
+    for (int n = 0; n < nn; ++n) out[n] = (int64_t)(nu[n] + acc[n]);
+  if (reset == 1) {
+    ISK_HIP(hipMemset(m.nu, 0, (size_t)nn * 8));
+    ISK_HIP(hipMemset(m.nu_acc, 0, (size_t)nn * 8));
+  }
   return ISKRA_OK;
 }

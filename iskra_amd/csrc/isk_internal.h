@@ -119,7 +119,8 @@ struct IskMcc {
   double m_eV;               /* source mass in eV units (mcc.jl:26) */
   double *target_n;          /* nodes */
   double max_n;
-  unsigned long long *nu;    /* nodes * ncoll event counts */
+  unsigned long long *nu;    /* nodes * ncoll event counts since the last fold */
+  unsigned long long *nu_acc;/* the counts folded out of nu (iskra_step) */
   unsigned long long *events;/* ncoll */
   int *errflag;
   /* ionisation product staging */
@@ -143,7 +144,8 @@ struct IskDsmc {
   double sgmax = 0.0;              /* maximum(rate) * argmax(rate) */
   double *d_g = nullptr, *d_sig = nullptr;
   double *rem = nullptr;           /* cells: collisions_remaining */
-  unsigned long long *nu = nullptr;     /* nodes: accepted events since reset */
+  unsigned long long *nu = nullptr;     /* nodes: accepted events since the last fold */
+  unsigned long long *nu_acc = nullptr; /* the counts folded out of nu (iskra_step) */
   unsigned long long *events = nullptr; /* accepted events of the last call */
   int *rng = nullptr;              /* 4 x cells: cell ranges of source, target */
   uint32_t stream = 0, step = 0;

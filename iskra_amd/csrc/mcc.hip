@@ -742,6 +742,7 @@ int isk_mcc_setup(iskra_ctx *c, int idx) {
       isk_alloc((void **)&m.d_off, (m.ncoll + 1) * 4) ||
       isk_alloc((void **)&m.target_n, (size_t)nn * 8) ||
       isk_alloc((void **)&m.nu, (size_t)nn * m.ncoll * 8) ||
+      isk_alloc((void **)&m.nu_acc, (size_t)nn * m.ncoll * 8) ||
       isk_alloc((void **)&m.events, ISK_MAX_COLL * 8) || isk_alloc((void **)&m.errflag, 64))
     return ISKRA_E_HIP;
   ISK_HIP(hipMemcpy(m.d_eps, m.tab_eps.data(), ntab * 8, hipMemcpyHostToDevice));
@@ -749,6 +750,7 @@ int isk_mcc_setup(iskra_ctx *c, int idx) {
   ISK_HIP(hipMemcpy(m.d_off, m.tab_off.data(), (m.ncoll + 1) * 4, hipMemcpyHostToDevice));
   ISK_HIP(hipMemset(m.target_n, 0, (size_t)nn * 8));
   ISK_HIP(hipMemset(m.nu, 0, (size_t)nn * m.ncoll * 8));
+  ISK_HIP(hipMemset(m.nu_acc, 0, (size_t)nn * m.ncoll * 8));
   ISK_HIP(hipMemset(m.events, 0, ISK_MAX_COLL * 8));
   ISK_HIP(hipMemset(m.errflag, 0, 64));
   m.max_n = 0.0;

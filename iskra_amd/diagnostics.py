@@ -24,8 +24,8 @@ file holds is what the reference's records hold after the step:
   - particles, densities, rho, phi, E, B and the probes are the state at the
     end of the step (nothing later in the reference's step changes them);
   - nuMCC-<species>-<k> and nuDSMC count the events of the last step (the
-    loop restarts the device counts at every step's start; after a fused
-    solve, of its last step);
+    device sets the earlier steps' counts apart before the last step of
+    every step call);
   - the fluid records hold n, the last step's dn (zero for a fluid the
     update cannot change) and v = E dt q/m with the E of the step's start,
     as the reference's advance! computes them (ParticleInCell.jl:74-82).
@@ -415,29 +415,21 @@ def register_simulation(config):
 
 
 def needs_step_begin(config):
-    """True when some record depends on where a step starts (collision
-    counts, fluid drift velocities)."""
-    from . import chemistry
+    """True when some record depends on where the last step starts (the
+    fluids' drift velocity uses the E of that moment)."""
     from .particle_in_cell import FluidSpecies
 
-    return any(isinstance(i, (chemistry.MonteCarloCollisions, chemistry.DirectSimulationMonteCarlo))
-               for i in config.interactions) or \
-        any(isinstance(s, FluidSpecies) for s in config.species)
+    return any(isinstance(s, FluidSpecies) for s in config.species)
 
 
 def step_begin(config, dt):
-    """Called by the loop before each device step that may be saved: restart
-    the per-step collision counts and keep the E the step starts from."""
-    from . import chemistry
+    """Called by the loop before each device step that may be saved: keep
+    the step's dt and, with fluid species, the E the step starts from."""
     from .particle_in_cell import FluidSpecies
 
     state = getattr(config, "_diag", None)
     if state is None:
         return
     state.dt = float(dt)
-    for inter in config.interactions:
-        if isinstance(inter, (chemistry.MonteCarloCollisions,
-                              chemistry.DirectSimulationMonteCarlo)):
-            inter._step_begin()
     if any(isinstance(s, FluidSpecies) for s in config.species):
         state.E_start = _efield(config.ctx, config.grid)

@@ -669,13 +669,13 @@ _noop_after_loop = after_loop
 def solve(config, dt=1e-5, timesteps=200, seed=None):
     """solve(config, dt, timesteps) (ParticleInCell.jl:84-139).
 
-    Per iteration: sources, MCC interactions (with the previous E), advance!
-    of every kinetic species (gather, push, after_push boundary, deposit),
-    rho = sum q n (+ q n of fluid species), phi from the Poisson solve, E,
-    then after_loop(it, it*dt - dt, dt).  When after_loop is the default and
-    there are no per-step sources, the iterations run as one device call (two
-    when collision counts or fluid records need the last step's start, see
-    iskra_amd.diagnostics)."""
+    Per iteration: sources, MCC and DSMC interactions (with the previous E),
+    advance! of every kinetic species (gather, push, after_push boundary,
+    deposit) and of every mobile fluid, rho = sum q n, phi from the Poisson
+    solve, E, then after_loop(it, it*dt - dt, dt).  When after_loop is the
+    default and there are no per-step sources, the iterations run as one
+    device call (two with fluid species, whose drift-velocity record needs
+    the E the last step starts from, see iskra_amd.diagnostics)."""
     if seed is not None:
         config.seed = int(seed)
     ctx = bind(config)

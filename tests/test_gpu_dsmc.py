@@ -87,10 +87,14 @@ def test_dsmc_in_time_loop_bitexact(gpu_ready, orc):
         k = sim.add_dsmc(0, 1, [1e2, 1e3, 1e4], [3e-19, 4e-19, 2e-19])
         dt = 2e-8
         total = 0
+        prev = sim.dsmcs[k]["nu"].copy()
         for it in range(12):
             ctx.step(dt, 1)
             sim.advance(dt)
             assert np.array_equal(ctx.field(L.F_RHO), sim.rho), it
+            # the step's own counts (the reference's nuDSMC, dsmc.jl:124-141)
+            assert np.array_equal(ctx.dsmc_counts_last(did), sim.dsmcs[k]["nu"] - prev), it
+            prev = sim.dsmcs[k]["nu"].copy()
             sim.set_phi(ctx.field(L.F_PHI))
             sim.end_step(dt)
             total += sim.dsmcs[k]["last"]
