@@ -96,6 +96,20 @@ How each oracle function is pinned (checks live in tests/):
                          its rows state; the segment-end rows agree with it.
   surfaces.CircuitRLC    advance_circuit! converges to the closed-form RL
                          step response and RC discharge as dt -> 0.
+  ref_advection_diffusion closed forms of the discrete operators: a sine
+                         mode's Laplacian in 1-D; in 2-D a uniform density in
+                         a linear drift (the reference divides both axes'
+                         terms by 2 dx: dx != dy tells it from dy), a linear
+                         density in a constant drift, x^2 + y^2 with the
+                         reference's dx dy divisor; a literal loop
+                         transcription bit for bit (tests/test_oracle_fluid.py).
+  RefSim fluids          advance! of a fluid from the previous step's E (the
+                         first step sees E = 0) against the restatement.
+  openpmd_layout         (oracle/openpmd_layout.py) files written by the
+                         library and read back with h5py compared object by
+                         object; three planted layout mistakes caught; the
+                         unit dimensions against the SI definitions of the
+                         units (tests/test_openpmd.py).
   orc_ws_run             (learned warm start, oracle/ws_oracle.c; no reference
                          counterpart) FP64 gradients against central finite
                          differences; convolution, pooling and upsampling
