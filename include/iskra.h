@@ -381,6 +381,21 @@ int iskra_fluid_add(iskra_ctx *ctx, double q, double m, double mu, const double 
  * fluids alone, with the current E). */
 int iskra_fluid_advance(iskra_ctx *ctx, double dt);
 
+/* ---- checkpoint / resume -------------------------------------------------- */
+/* No reference counterpart (the reference has no checkpoint, TODO:3).  The
+ * dynamic state of a context as one host blob: the live particles of every
+ * species with their next id, the step counter and seed, phi and E, the
+ * draw counters and event counts of the MCC and DSMC interactions and the
+ * DSMC remainders, fluid densities, electrode and circuit state and the
+ * electrode values.  Load it into a context set up like the saved one (same
+ * grid, species, interactions, fluids, electrodes, solver and tuning; a
+ * mismatch is ISKRA_E_STATE): the run then continues bit for bit as it would
+ * have without the interruption (particles re-sorted by cell; results do
+ * not depend on their order).  size gives the bytes save writes. */
+int iskra_checkpoint_size(iskra_ctx *ctx, int64_t *bytes);
+int iskra_checkpoint_save(iskra_ctx *ctx, void *buf, int64_t bytes);
+int iskra_checkpoint_load(iskra_ctx *ctx, const void *buf, int64_t bytes);
+
 /* ---- random numbers ------------------------------------------------------ */
 /* The seed of all device draws (Random.seed!, src/iskra.jl:22-26). */
 int iskra_rng_set(iskra_ctx *ctx, uint64_t seed);

@@ -31,7 +31,7 @@ EXPORTS = [
     "iskra_grid_set", "iskra_grid_set_axial", "iskra_pusher_set", "iskra_dsmc_add",
     "iskra_dsmc_run", "iskra_dsmc_counts", "iskra_tracker_create", "iskra_tracker_surface",
     "iskra_tracker_table", "iskra_electrode_create", "iskra_electrode_state",
-    "iskra_circuit_set", "iskra_circuit_state", "iskra_fluid_add", "iskra_fluid_advance", "iskra_tuning_set", "iskra_tuning_get", "iskra_species_add",
+    "iskra_circuit_set", "iskra_circuit_state", "iskra_fluid_add", "iskra_fluid_advance", "iskra_checkpoint_size", "iskra_checkpoint_save", "iskra_checkpoint_load", "iskra_tuning_set", "iskra_tuning_get", "iskra_species_add",
     "iskra_species_set_boundary", "iskra_species_sort_every", "iskra_species_margin",
     "iskra_species_upload",
     "iskra_species_download",
@@ -110,6 +110,9 @@ def lib():
             "iskra_circuit_state": (I, [P, P, P, P, P]),
             "iskra_fluid_add": (I, [P, D, D, D, P, C.POINTER(I)]),
             "iskra_fluid_advance": (I, [P, D]),
+            "iskra_checkpoint_size": (I, [P, C.POINTER(I64)]),
+            "iskra_checkpoint_save": (I, [P, P, I64]),
+            "iskra_checkpoint_load": (I, [P, P, I64]),
             "iskra_tuning_set": (I, [P, C.POINTER(Tuning)]),
             "iskra_tuning_get": (I, [P, C.POINTER(Tuning)]),
             "iskra_species_add": (I, [P, D, D, D, I64, I, C.POINTER(I)]),
@@ -523,6 +526,18 @@ class Context:
 
     def fluid_advance(self, dt):
         check(lib().iskra_fluid_advance(self.h, float(dt)))
+
+    def checkpoint(self):
+        """the context's dynamic state as bytes (uint8 array)"""
+        n = C.c_int64()
+        check(lib().iskra_checkpoint_size(self.h, C.byref(n)))
+        buf = np.zeros(max(n.value, 1), dtype=np.uint8)
+        check(lib().iskra_checkpoint_save(self.h, _p(buf), n.value))
+        return buf[:n.value]
+
+    def restore(self, blob):
+        b = np.ascontiguousarray(blob, dtype=np.uint8)
+        check(lib().iskra_checkpoint_load(self.h, _p(b), b.size))
 
     def circuit_state(self):
         v = [C.c_double() for _ in range(4)]

@@ -1493,3 +1493,288 @@ int iskra_bench_copy(iskra_ctx *c, int64_t bytes, int iters, int nontemporal, do
 }
 
 } /* extern "C" */
+
+/* ---- checkpoint / resume ---- */
+/* The dynamic state of a context as one host blob: the particles (live ones,
+ * in the current device order), the counters that draws, product ids and
+ * the sort schedule depend on, the fields the next step reads (phi, E), the
+ * collision counts and DSMC remainders, fluid densities, electrode and
+ * circuit state and the electrode values.  The configuration (grid, species,
+ * interactions, solver, tuning) is not in it: load expects a context set up
+ * the same way and checks a fingerprint of it.  Results after a load equal
+ * those of the uninterrupted run: particles are re-sorted by cell on load,
+ * and no result depends on particle order (integer deposit, id-keyed draws,
+ * products numbered by parent id). */
+namespace {
+struct CkHeader {
+  char magic[8];
+  int32_t version, dim;
+  int64_t nx, ny;
+  int32_t nspecies, nmcc, ndsmc, nfluid, has_elec, has_tags, track[ISK_MAX_SPECIES], pad;
+  int64_t step_count;
+  uint64_t seed;
+  double t_now;
+};
+const char CK_MAGIC[8] = {'I', 'S', 'K', 'R', 'A', 'C', 'K', '1'};
+
+void ck_put(std::vector<char> &b, const void *p, size_t n) {
+  const char *q = (const char *)p;
+  b.insert(b.end(), q, q + n);
+}
+
+int ck_put_dev(std::vector<char> &b, const void *dev, size_t n) {
+  size_t at = b.size();
+  b.resize(at + n);
+  if (n) ISK_HIP(hipMemcpy(b.data() + at, dev, n, hipMemcpyDeviceToHost));
+  return ISKRA_OK;
+}
+
+CkHeader ck_header(const iskra_ctx *c) {
+  CkHeader h;
+  memset(&h, 0, sizeof h);
+  memcpy(h.magic, CK_MAGIC, 8);
+  h.version = 1;
+  h.dim = c->g.dim;
+  h.nx = c->g.nx;
+  h.ny = c->g.ny;
+  h.nspecies = c->nspecies;
+  h.nmcc = c->nmcc;
+  h.ndsmc = c->ndsmc;
+  h.nfluid = c->nfluid;
+  h.has_elec = c->d_elec != nullptr;
+  h.has_tags = c->ps.d_tag_val != nullptr;
+  for (int s = 0; s < c->nspecies; ++s) h.track[s] = c->sp[s].track_ids;
+  h.step_count = c->step_count;
+  h.seed = c->seed;
+  h.t_now = c->t_now;
+  return h;
+}
+
+int ck_build(iskra_ctx *c, std::vector<char> &b) {
+  ISK_HIP(hipStreamSynchronize(c->stream));
+  const int nn = isk_nodes(c);
+  const size_t ncell = (size_t)c->g.ncx * c->g.ncy;
+  CkHeader h = ck_header(c);
+  ck_put(b, &h, sizeof h);
+  for (int s = 0; s < c->nspecies; ++s) {
+    IskSpecies *sp = &c->sp[s];
+    long long hdr[HDR_WORDS];
+    ISK_HIP(hipMemcpy(hdr, sp->hdr, sizeof(hdr), hipMemcpyDeviceToHost));
+    const int64_t n = hdr[HDR_NP];
+    std::vector<int32_t> cell(n);
+    std::vector<float> ox(n), oy(n, 0.0f), vx(n), vy(n), vz(n);
+    std::vector<uint32_t> id(n, 0u);
+    RC(iskra_species_get_state(c, s, n, nullptr, cell.data(), ox.data(), oy.data(), vx.data(),
+                               vy.data(), vz.data(), id.data()));
+    int64_t live = 0;
+    for (int64_t p = 0; p < n; ++p)
+      if (cell[p] >= 0) {
+        cell[live] = cell[p];
+        ox[live] = ox[p];
+        oy[live] = oy[p];
+        vx[live] = vx[p];
+        vy[live] = vy[p];
+        vz[live] = vz[p];
+        id[live] = id[p];
+        ++live;
+      }
+    const int64_t words[3] = {live, (int64_t)hdr[HDR_NEXT_ID], (int64_t)hdr[HDR_PUSHED]};
+    ck_put(b, words, sizeof words);
+    ck_put(b, cell.data(), live * 4);
+    ck_put(b, ox.data(), live * 4);
+    ck_put(b, oy.data(), live * 4);
+    ck_put(b, vx.data(), live * 4);
+    ck_put(b, vy.data(), live * 4);
+    ck_put(b, vz.data(), live * 4);
+    ck_put(b, id.data(), live * 4);
+  }
+  RC(ck_put_dev(b, c->phi, (size_t)nn * 8));
+  RC(ck_put_dev(b, c->E, (size_t)nn * 4 * c->g.dim));
+  for (int m = 0; m < c->nmcc; ++m) {
+    const IskMcc &q = c->mcc[m];
+    const int32_t w[2] = {q.ncoll, (int32_t)q.step};
+    ck_put(b, w, sizeof w);
+    RC(ck_put_dev(b, q.nu, (size_t)nn * q.ncoll * 8));
+    RC(ck_put_dev(b, q.nu_acc, (size_t)nn * q.ncoll * 8));
+  }
+  for (int m = 0; m < c->ndsmc; ++m) {
+    const IskDsmc &q = c->dsmc[m];
+    const uint32_t w[2] = {q.step, 0u};
+    ck_put(b, w, sizeof w);
+    RC(ck_put_dev(b, q.rem, ncell * 8));
+    RC(ck_put_dev(b, q.nu, (size_t)nn * 8));
+    RC(ck_put_dev(b, q.nu_acc, (size_t)nn * 8));
+  }
+  for (int f = 0; f < c->nfluid; ++f) {
+    RC(ck_put_dev(b, c->fl[f].n, (size_t)nn * 8));
+    RC(ck_put_dev(b, c->fl[f].dn, (size_t)nn * 8));
+  }
+  if (c->d_elec) RC(ck_put_dev(b, c->d_elec, EL_WORDS * 8));
+  if (c->ps.d_tag_val) RC(ck_put_dev(b, c->ps.d_tag_val, (ISK_MAX_TAGS + 1) * 8));
+  return ISKRA_OK;
+}
+
+struct CkReader {
+  const char *p, *end;
+  bool take(void *dst, size_t n) {
+    if ((size_t)(end - p) < n) return false;
+    memcpy(dst, p, n);
+    p += n;
+    return true;
+  }
+  const char *skip(size_t n) {
+    if ((size_t)(end - p) < n) return nullptr;
+    const char *q = p;
+    p += n;
+    return q;
+  }
+};
+}  // namespace
+
+int iskra_checkpoint_size(iskra_ctx *c, int64_t *bytes) {
+  CHECK_CTX(c);
+  CHECK_GRID(c);
+  if (!bytes) return ISKRA_E_ARG;
+  std::vector<char> b;
+  RC(ck_build(c, b));
+  *bytes = (int64_t)b.size();
+  return ISKRA_OK;
+}
+
+int iskra_checkpoint_save(iskra_ctx *c, void *buf, int64_t bytes) {
+  CHECK_CTX(c);
+  CHECK_GRID(c);
+  if (!buf) return ISKRA_E_ARG;
+  std::vector<char> b;
+  RC(ck_build(c, b));
+  if ((int64_t)b.size() > bytes) {
+    isk_set_error("checkpoint_save: %lld bytes needed, %lld given", (long long)b.size(),
+                  (long long)bytes);
+    return ISKRA_E_ARG;
+  }
+  memcpy(buf, b.data(), b.size());
+  return ISKRA_OK;
+}
+
+int iskra_checkpoint_load(iskra_ctx *c, const void *buf, int64_t bytes) {
+  CHECK_CTX(c);
+  CHECK_GRID(c);
+  if (!buf || bytes < (int64_t)sizeof(CkHeader)) return ISKRA_E_ARG;
+  CkReader r{(const char *)buf, (const char *)buf + bytes};
+  CkHeader h, mine = ck_header(c);
+  r.take(&h, sizeof h);
+  if (memcmp(h.magic, CK_MAGIC, 8) || h.version != 1) {
+    isk_set_error("checkpoint_load: not a checkpoint of this library");
+    return ISKRA_E_ARG;
+  }
+  if (h.dim != mine.dim || h.nx != mine.nx || h.ny != mine.ny || h.nspecies != mine.nspecies ||
+      h.nmcc != mine.nmcc || h.ndsmc != mine.ndsmc || h.nfluid != mine.nfluid ||
+      h.has_elec != mine.has_elec || h.has_tags != mine.has_tags ||
+      memcmp(h.track, mine.track, sizeof h.track)) {
+    isk_set_error("checkpoint_load: the context is not set up like the saved one (grid, "
+                  "species, interactions, fluids, electrodes or solver differ)");
+    return ISKRA_E_STATE;
+  }
+  const int nn = isk_nodes(c);
+  const size_t ncell = (size_t)c->g.ncx * c->g.ncy;
+  const char *bad = "checkpoint_load: truncated checkpoint";
+  for (int s = 0; s < c->nspecies; ++s) {
+    int64_t w[3];
+    if (!r.take(w, sizeof w) || w[0] < 0 || w[0] > c->sp[s].cap) {
+      isk_set_error("checkpoint_load: species %d: bad particle count", s);
+      return ISKRA_E_ARG;
+    }
+    const int64_t n = w[0];
+    const char *cell = r.skip(n * 4), *ox = r.skip(n * 4), *oy = r.skip(n * 4),
+               *vx = r.skip(n * 4), *vy = r.skip(n * 4), *vz = r.skip(n * 4),
+               *id = r.skip(n * 4);
+    if (!id) {
+      isk_set_error("%s", bad);
+      return ISKRA_E_ARG;
+    }
+    RC(iskra_species_set_state(c, s, n, (const int32_t *)cell, (const float *)ox,
+                               (const float *)oy, (const float *)vx, (const float *)vy,
+                               (const float *)vz,
+                               c->sp[s].track_ids ? (const uint32_t *)id : nullptr));
+    long long hdr[HDR_WORDS];
+    ISK_HIP(hipMemcpy(hdr, c->sp[s].hdr, sizeof(hdr), hipMemcpyDeviceToHost));
+    hdr[HDR_NEXT_ID] = w[1];
+    hdr[HDR_PUSHED] = w[2];
+    ISK_HIP(hipMemcpy(c->sp[s].hdr, hdr, sizeof(hdr), hipMemcpyHostToDevice));
+  }
+  const char *phi = r.skip((size_t)nn * 8), *E = r.skip((size_t)nn * 4 * c->g.dim);
+  if (!E) {
+    isk_set_error("%s", bad);
+    return ISKRA_E_ARG;
+  }
+  ISK_HIP(hipMemcpy(c->phi, phi, (size_t)nn * 8, hipMemcpyHostToDevice));
+  ISK_HIP(hipMemcpy(c->E, E, (size_t)nn * 4 * c->g.dim, hipMemcpyHostToDevice));
+  for (int m = 0; m < c->nmcc; ++m) {
+    IskMcc &q = c->mcc[m];
+    int32_t w[2];
+    if (!r.take(w, sizeof w) || w[0] != q.ncoll) {
+      isk_set_error("checkpoint_load: interaction %d has another process count", m);
+      return ISKRA_E_STATE;
+    }
+    const size_t sz = (size_t)nn * q.ncoll * 8;
+    const char *nu = r.skip(sz), *acc = r.skip(sz);
+    if (!acc) {
+      isk_set_error("%s", bad);
+      return ISKRA_E_ARG;
+    }
+    q.step = (uint32_t)w[1];
+    ISK_HIP(hipMemcpy(q.nu, nu, sz, hipMemcpyHostToDevice));
+    ISK_HIP(hipMemcpy(q.nu_acc, acc, sz, hipMemcpyHostToDevice));
+  }
+  for (int m = 0; m < c->ndsmc; ++m) {
+    IskDsmc &q = c->dsmc[m];
+    uint32_t w[2];
+    const char *rem = r.take(w, sizeof w) ? r.skip(ncell * 8) : nullptr;
+    const char *nu = rem ? r.skip((size_t)nn * 8) : nullptr;
+    const char *acc = nu ? r.skip((size_t)nn * 8) : nullptr;
+    if (!acc) {
+      isk_set_error("%s", bad);
+      return ISKRA_E_ARG;
+    }
+    q.step = w[0];
+    ISK_HIP(hipMemcpy(q.rem, rem, ncell * 8, hipMemcpyHostToDevice));
+    ISK_HIP(hipMemcpy(q.nu, nu, (size_t)nn * 8, hipMemcpyHostToDevice));
+    ISK_HIP(hipMemcpy(q.nu_acc, acc, (size_t)nn * 8, hipMemcpyHostToDevice));
+  }
+  for (int f = 0; f < c->nfluid; ++f) {
+    const char *n = r.skip((size_t)nn * 8), *dn = r.skip((size_t)nn * 8);
+    if (!dn) {
+      isk_set_error("%s", bad);
+      return ISKRA_E_ARG;
+    }
+    ISK_HIP(hipMemcpy(c->fl[f].n, n, (size_t)nn * 8, hipMemcpyHostToDevice));
+    ISK_HIP(hipMemcpy(c->fl[f].dn, dn, (size_t)nn * 8, hipMemcpyHostToDevice));
+  }
+  RC(isk_fluid_bg(c));
+  if (c->d_elec) {
+    const char *el = r.skip(EL_WORDS * 8);
+    if (!el) {
+      isk_set_error("%s", bad);
+      return ISKRA_E_ARG;
+    }
+    ISK_HIP(hipMemcpy(c->d_elec, el, EL_WORDS * 8, hipMemcpyHostToDevice));
+  }
+  if (c->ps.d_tag_val) {
+    const char *tv = r.skip((ISK_MAX_TAGS + 1) * 8);
+    if (!tv) {
+      isk_set_error("%s", bad);
+      return ISKRA_E_ARG;
+    }
+    ISK_HIP(hipMemcpy(c->ps.d_tag_val, tv, (ISK_MAX_TAGS + 1) * 8, hipMemcpyHostToDevice));
+    memcpy(c->ps.tag_val, tv, (ISK_MAX_TAGS + 1) * 8);
+  }
+  if (r.p != r.end) {
+    isk_set_error("checkpoint_load: %lld bytes left over", (long long)(r.end - r.p));
+    return ISKRA_E_ARG;
+  }
+  c->step_count = h.step_count;
+  c->seed = h.seed;
+  c->t_now = h.t_now;
+  ISK_HIP(hipStreamSynchronize(c->stream));
+  return ISKRA_OK;
+}

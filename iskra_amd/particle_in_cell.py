@@ -666,7 +666,30 @@ def grid_to_particle(grid, part, E=None):
 _noop_after_loop = after_loop
 
 
-def solve(config, dt=1e-5, timesteps=200, seed=None):
+def checkpoint(config, path):
+    """Save the bound simulation's state to path (.npz, no pickled objects):
+    the device state (iskra_checkpoint_save), the sources' call counters and
+    the step count.  No reference counterpart (its TODO:3)."""
+    _sync_to_device(config)
+    ctx = config.ctx
+    np.savez(path, blob=ctx.checkpoint(),
+             nsample=np.array([getattr(s, "_nsample", 0) for s in _kinetic_species(config)],
+                              dtype=np.int64),
+             step=np.array([ctx.stats()["steps"] + getattr(ctx, "_step0", 0)], dtype=np.int64))
+
+
+def _restore(config, path):
+    with np.load(path, allow_pickle=False) as z:
+        blob, ns, step = z["blob"], z["nsample"], int(z["step"][0])
+    config.ctx.restore(blob)
+    config.ctx._step0 = step
+    for s, n in zip(_kinetic_species(config), ns):
+        s._nsample = int(n)
+    _mark_device_newer(config)
+    return step
+
+
+def solve(config, dt=1e-5, timesteps=200, seed=None, resume=None):
     """solve(config, dt, timesteps) (ParticleInCell.jl:84-139).
 
     Per iteration: sources, MCC and DSMC interactions (with the previous E),
@@ -675,11 +698,20 @@ def solve(config, dt=1e-5, timesteps=200, seed=None):
     solve, E, then after_loop(it, it*dt - dt, dt).  When after_loop is the
     default and there are no per-step sources, the iterations run as one
     device call (two with fluid species, whose drift-velocity record needs
-    the E the last step starts from, see iskra_amd.diagnostics)."""
+    the E the last step starts from, see iskra_amd.diagnostics).
+
+    resume: a file written by checkpoint(); the configuration must be set
+    up as the saved one was.  Its state replaces enter_loop, and the
+    iterations continue the saved run's numbering; the results equal those
+    of the run that was never interrupted."""
     if seed is not None:
         config.seed = int(seed)
     ctx = bind(config)
-    enter_loop()
+    it0 = 0
+    if resume is not None:
+        it0 = _restore(config, resume)
+    else:
+        enter_loop()
     _sync_to_device(config)
     dt = float(dt)
     fused = after_loop is _noop_after_loop and not config.sources
@@ -693,7 +725,7 @@ def solve(config, dt=1e-5, timesteps=200, seed=None):
             ctx.step(dt, n)
         _mark_device_newer(config)
     else:
-        for it in range(1, int(timesteps) + 1):
+        for it in range(it0 + 1, it0 + int(timesteps) + 1):
             for src in config.sources:
                 init(src, src.species, dt)
             _sync_to_device(config)
