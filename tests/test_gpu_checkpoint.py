@@ -120,3 +120,43 @@ def test_checkpoint_refused_by_another_set_up(gpu_ready):
     assert e.value.code in (L.E_ARG, L.E_STATE)
     a.close()
     other.close()
+
+
+def _circuit_ctx(orc):
+    """problem/07_boundaries.jl's box (driven and floating electrodes, a
+    grounded one, reflecting walls) with an RLC circuit and an electron beam,
+    as tests/test_gpu_surfaces.py sets it up"""
+    from iskra_amd import _lib as L
+    from test_gpu_surfaces import ME, QE, _geometry_ctx
+    ctx, _, _, _, ids = _geometry_ctx(orc)
+    ctx.circuit(R=1.0, L=1e-6, C_=1e-6, v_offset=0.0, v_amplitude=1.0, v_freq=5e6,
+                positive=ids["driven"], negative=ids["grounded"])
+    s = ctx.species(-QE, ME, 5e4, 4000, track_ids=True)
+    ctx.boundary(s, [L.BND_WRAP, L.BND_WRAP])
+    ctx.sample_maxwellian(s, 4000, [0.5, 0.5], [3e6] * 3, dx=[0.25, 0.25], seed=3, stream=0)
+    return ctx, s, ids
+
+
+def test_electrodes_and_circuit_resumed(gpu_ready, orc):
+    """Surface charges, the circuit's current, charge and time, the driven
+    electrode's value and the BiCGStab potential carry over a load: 24 steps
+    against 12 + load + 12."""
+    from iskra_amd import _lib as L
+    dt = 1e-8
+    a, sa, ids = _circuit_ctx(orc)
+    a.step(dt, 24)
+    b, _, _ = _circuit_ctx(orc)
+    b.step(dt, 12)
+    blob = b.checkpoint()
+    b.close()
+    c, sc, _ = _circuit_ctx(orc)
+    c.restore(blob)
+    c.step(dt, 12)
+    _assert_same_particles(a, sa, c, sc, "electrons")
+    assert np.array_equal(a.field(L.F_PHI), c.field(L.F_PHI))
+    assert a.circuit_state() == c.circuit_state()
+    for k in ("driven", "floating"):
+        assert a.electrode_state(ids[k]) == c.electrode_state(ids[k]), k
+    assert a.circuit_state()["i"] != 0.0
+    a.close()
+    c.close()
