@@ -177,7 +177,8 @@ class ParticleRecord:
         a = np.asarray(val)
         dtype = {np.dtype(np.uint32): PMD_U32, np.dtype(np.int64): PMD_I64}.get(a.dtype, PMD_F64)
         a = np.ascontiguousarray(a, dtype=a.dtype if dtype != PMD_F64 else np.float64)
-        a = a.reshape(-1, int(np_)) if np_ else a.reshape(-1, 0)
+        if a.ndim == 1:
+            a = a[None, :]
         ncomp = a.shape[0] if self.withcomponents else 1
         _check(lib.iskra_pmd_particle(it._h, key.encode(), self.units.encode(), int(np_), ncomp,
                                       int(self.withcomponents), dtype, _ptr(a), int(np_),

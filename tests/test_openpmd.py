@@ -188,6 +188,20 @@ def test_file_layout_1d_and_axial(tmp_path):
     assert data["/data/3/fields/phi"].shape == (1, 17)
 
 
+def test_empty_species_records(tmp_path):
+    """a species with no particles: zero-length datasets, shape [0]"""
+    diag.records.clear()
+    part = _species("e-", 4, 2, -1.0, 1.0, 1.0, 5)
+    part.np = 0
+    _register_species(part)
+    path = diag.new_iteration(lambda it: diag.save_records(it, "e-/"), str(tmp_path), 2, 0.0, 1.0)
+    tree, data = read_back(path, tmp_path)
+    exp = OL.skeleton(2, 0.0, 1.0)
+    ds = OL.species_records(exp, 2, "e-", 0, part.m, part.q, part.w0, [[], []],
+                            [[], [], []], [])
+    compare(tree, data, exp, ds)
+
+
 def test_h5dump_reads_the_file(tmp_path):
     diag.records.clear()
     grid = rg.create_uniform_grid(np.linspace(0.0, 1.0, 5), np.linspace(0.0, 1.0, 4))
