@@ -20,7 +20,10 @@ coarse solver of the two-level cycle that preconditions a flexible CG
 zero, from the previous phi and from the network's prediction, multigrid
 likewise, and the network-preconditioned solve, all to the same FP64
 relative residual 1e-8: the "learned vs plain" part of the metric;
---no-warmstart skips it.
+--no-warmstart skips it.  Then the single-context 1-D configurations are
+timed (runs_1d: the two-stream instability, 64 cells and 20,000 electrons
+over 500 steps, and the RF discharge with MCC, 512 cells and 1 M electrons +
+1 M ions over 2000 steps); --no-runs-1d skips them.
 
 Prints one JSON line (see the keys below).  --gpus above 1 exits: this build
 runs on one GPU.
@@ -196,6 +199,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--no-warmstart", action="store_true")
+    ap.add_argument("--no-runs-1d", action="store_true",
+                    help="skip the 1-D two-stream and RF-discharge timings")
     ap.add_argument("--ws-runs", type=int, default=4)
     ap.add_argument("--ws-train-steps", type=int, default=1000)
     args = ap.parse_args()
@@ -375,6 +380,18 @@ def main():
                                           if pcg["fcg"]["converged"] else None),
             "pcg_iters_plain_over_warm_start": pcg["zero"]["iters"] / pcg["learned"]["iters"],
         }
+    if not args.no_runs_1d:
+        # the single-context 1-D configurations (not the headline workload):
+        # the periodic two-stream instability and the RF discharge with MCC,
+        # each timed over whole device calls after a warm-up
+        from iskra_amd import runs
+        from iskra_amd.ensemble import ccp_context
+        out["runs_1d"] = [
+            runs.run("two-stream 1-D, 64 cells, 20,000 electrons", runs.two_stream_context, 500,
+                     batch=False),
+            runs.run("RF discharge 1-D with MCC, 512 cells, 1M e- + 1M ions",
+                     lambda: ccp_context(dict(cells=512, particles=1_000_000, volts=200.0,
+                                              torr=1.0, seed=1)), 2000, batch=False)]
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cells)
     print(json.dumps(out), flush=True)
