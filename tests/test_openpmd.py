@@ -254,6 +254,35 @@ def test_writer_errors(tmp_path, capsys):
         diag.field("rho", "C/m^2", np.ones(4), grid)
 
 
+def test_c_abi_rejects_bad_arguments(tmp_path):
+    """NULL pointers, bad grids, component counts and dtypes are errors,
+    not crashes, at the C boundary (include/iskra_pmd.h)"""
+    import ctypes as C
+    lib = diag.pmd_lib()
+    h = C.c_void_p()
+    assert lib.iskra_pmd_open(None, 0, 0.0, 1.0, None, C.byref(h)) == L.E_ARG
+    assert lib.iskra_pmd_open(str(tmp_path).encode(), -1, 0.0, 1.0, None, C.byref(h)) == L.E_ARG
+    assert lib.iskra_pmd_open(str(tmp_path).encode(), 5, 0.0, 1.0, None, C.byref(h)) == L.OK
+    data = np.zeros(9)
+    ptr = data.ctypes.data_as(C.c_void_p)
+    g = diag.Grid(dim=3, axial=0)
+    g.nodes[0], g.nodes[1] = 3, 3
+    assert lib.iskra_pmd_field(h, b"rho", b"C", C.byref(g), 1, 0, ptr, None, 0.0) == L.E_ARG
+    g.dim = 2
+    assert lib.iskra_pmd_field(h, b"rho", b"C", None, 1, 0, ptr, None, 0.0) == L.E_ARG
+    assert lib.iskra_pmd_field(h, b"rho", b"C", C.byref(g), 2, 0, ptr, None, 0.0) == L.E_ARG
+    assert lib.iskra_pmd_field(h, b"/rho", b"C", C.byref(g), 1, 0, ptr, None, 0.0) == L.E_ARG
+    assert lib.iskra_pmd_field(h, b"rho", b"C", C.byref(g), 1, 0, ptr, None, 0.0) == L.OK
+    assert lib.iskra_pmd_particle(h, b"e-/x", b"m", 4, 1, 0, 7, ptr, 4, 0, 0.0) == L.E_ARG
+    assert lib.iskra_pmd_particle(h, b"e-/x", b"m", 4, 1, 0, 0, ptr, 3, 0, 0.0) == L.E_ARG
+    assert lib.iskra_pmd_particle(h, b"e-/x", b"m", 4, 1, 0, 0, None, 4, 0, 0.0) == L.E_ARG
+    assert lib.iskra_pmd_particle_constant(h, b"e-/m", b"kg", 1.0, -1, 0, 0.0) == L.E_ARG
+    assert lib.iskra_pmd_probe(h, b"", b"V", 1.0, 0.0) == L.E_ARG
+    assert lib.iskra_pmd_close(h) == L.OK
+    assert lib.iskra_pmd_close(None) == L.E_ARG
+    assert lib.iskra_pmd_field(None, b"rho", b"C", C.byref(g), 1, 0, ptr, None, 0.0) == L.E_STATE
+
+
 def test_pmd_library_exports_the_header():
     hdr = open(os.path.join(ROOT, "include", "iskra_pmd.h")).read()
     names = set(re.findall(r"\b(iskra_pmd_\w+)\s*\(", hdr))
